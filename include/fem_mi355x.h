@@ -1,0 +1,254 @@
+/*
+ * fem_mi355x.h - C ABI of the MI355X (gfx950) finite-element library.
+ *
+ * The library replaces the device work of jax-fem's assembly -> linear solve ->
+ * adjoint path (jax_fem/fe.py, jax_fem/problem.py, jax_fem/solver.py).  It is a
+ * set of stateless launchers over caller-owned device buffers: every pointer
+ * named here is device memory unless its comment says "host"; nothing is
+ * allocated inside except where a comment says so.  Every call is enqueued on
+ * the HIP stream passed as `stream` (a hipStream_t, NULL = default stream) and
+ * returns an int status (FEM_OK = 0, negative on error).  On error,
+ * fem_last_error() returns a message (thread-local, valid until the next call
+ * on the same thread).  Calls that read results back to the host (sizes,
+ * solver statistics) synchronise the stream; they say so below.
+ *
+ * Conventions (same as the reference):
+ *   dof index      = vec * node + component              (problem.py:89)
+ *   cell dof index = vec * a + component, a = local node   (problem.py:197, 211)
+ *   per-cell tangent K[c][a*vec+i][b*vec+j] = dR_{a,i} / du_{b,j}  (problem.py:262-266)
+ *   residual R(u) = sum_q P(grad u) : grad v JxW + mass + surface terms
+ *                                                        (problem.py:189-259, 296)
+ *
+ * Matrix storage.  The global tangent has a vec x vec block structure over the
+ * node graph.  The node-graph pattern is kept as block CSR (row_ptr/col_idx,
+ * columns sorted), which expands exactly to the scalar CSR pattern the
+ * reference builds with PETSc from its COO I/J (problem.py:86-107,
+ * solver.py:472-478).  Values live in a sliced-ELL layout over 64-row slices
+ * (SELL-64): block (row, k) of slice s = row/64 is stored at block position
+ * slice_ptr[s] + 64*k + row%64, and value v (0..vec*vec-1, row-major in the
+ * block) of it at  vals[slice_ptr[s]*vec*vec + (k*vec*vec + v)*64 + row%64].
+ * Slots past a row's length hold zero values and the row itself as column.
+ */
+#ifndef FEM_MI355X_H
+#define FEM_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  FEM_OK = 0,
+  FEM_ERR_BAD_ARG = -1,       /* invalid argument / unsupported combination   */
+  FEM_ERR_NOT_CONVERGED = -2, /* Krylov solve hit maxiter                     */
+  FEM_ERR_NAN = -3,           /* non-finite value met in a solve              */
+  FEM_ERR_HIP = -4,           /* HIP runtime error                            */
+  FEM_ERR_CAPACITY = -5       /* a fixed per-row capacity was exceeded        */
+};
+
+/* ---- enums ----------------------------------------------------------- */
+enum { FEM_F32 = 0, FEM_F64 = 1 };           /* arithmetic precision          */
+enum { FEM_HEX8 = 8, FEM_HEX27 = 27 };       /* = nodes per cell              */
+enum {                                       /* constitutive laws (tensor_map)*/
+  FEM_MAT_POISSON = 1,        /* P = c0 * grad u                      (vec 1) */
+  FEM_MAT_LINEAR_ELASTIC = 2, /* sigma = c0 tr(eps) I + 2 c1 eps      (vec 3) */
+  FEM_MAT_SIMP = 3,           /* as LINEAR_ELASTIC with lambda = c0 E(t),
+                                 mu = c1 E(t), E(t) = c2 + (c3 - c2) t^c4, t =
+                                 per-quad parameter                  (vec 3) */
+  FEM_MAT_NEO_HOOKEAN = 4     /* P = d psi / dF, psi = mu/2 (J^-2/3 I1 - 3) +
+                                 kappa/2 (J - 1)^2, mu = c0 s, kappa = c1 s,
+                                 s = per-quad parameter or 1         (vec 3) */
+};
+
+/* ---- plain descriptor structs (device pointers) ------------------------ */
+typedef struct {
+  int32_t nodes_per_cell;        /* N: 8 (HEX8) or 27 (HEX27)                 */
+  int32_t num_quads;             /* Q                                         */
+  int32_t vec;                   /* 1 or 3                                    */
+  int32_t num_face_quads;        /* FQ                                        */
+  int64_t num_cells;
+  int64_t num_nodes;
+  const double*  points;         /* (num_nodes, 3)                            */
+  const int32_t* cells;          /* (num_cells, N), meshio/VTK node order     */
+  const double*  shape_vals;     /* (Q, N)     basis.py:141-175               */
+  const double*  shape_grads_ref;/* (Q, N, 3)                                 */
+  const double*  quad_weights;   /* (Q)                                       */
+  const double*  face_shape_vals;/* (6, FQ, N) basis.py:178-250               */
+  const double*  face_shape_grads_ref; /* (6, FQ, N, 3)                      */
+  const double*  face_weights;   /* (6, FQ)                                   */
+  const double*  face_normals;   /* (6, 3)                                    */
+} fem_mesh_t;
+
+typedef struct {
+  int32_t kind;                  /* FEM_MAT_*                                 */
+  int32_t has_param;             /* 1: `param` holds one value per (cell, quad) */
+  double  c[6];                  /* constants, meaning per kind above         */
+  const double* param;           /* (num_cells, Q) or NULL                    */
+} fem_material_t;
+
+typedef struct {
+  int64_t num_rows;              /* block rows (= mesh nodes)                 */
+  int64_t nnzb;                  /* stored blocks                             */
+  int64_t num_slices;            /* ceil(num_rows / 64)                       */
+  int64_t sell_blocks;           /* slice_ptr[num_slices]                     */
+  int32_t vec;                   /* block size                                */
+  int32_t num_cells;             /* cells feeding the contributions          */
+  int32_t nodes_per_cell;
+  int32_t pad;
+  const int32_t* row_ptr;        /* (num_rows + 1)                            */
+  const int32_t* col_idx;        /* (nnzb), sorted within a row               */
+  const int32_t* diag_slot;      /* (num_rows) slot of the diagonal block, -1 */
+  const int32_t* contrib_ptr;    /* (nnzb + 1) into contrib_idx               */
+  const int32_t* contrib_idx;    /* (num_cells*N*N) cell-block id c*N*N+a*N+b */
+  const int32_t* slice_ptr;      /* (num_slices + 1) in blocks                */
+  const int32_t* sell_col;       /* (sell_blocks) column node per SELL block  */
+} fem_pattern_t;
+
+/* FP64 matrix-free tangent operator y = dR/du(u) x on free dofs, y = x on
+ * Dirichlet dofs; used for the FP64 residual of iterative refinement. */
+typedef struct {
+  const fem_mesh_t*     mesh;
+  const fem_material_t* mat;
+  const double*  u;              /* (num_nodes*vec) linearisation point       */
+  const int32_t* inc_ptr;        /* node -> (cell, local node) incidence      */
+  const int32_t* inc_idx;
+  int64_t        n_bc;
+  const int32_t* bc_dofs;        /* (n_bc) unique Dirichlet dofs              */
+  double*        cell_scratch;   /* (num_cells * N * vec) doubles             */
+} fem_operator_t;
+
+typedef struct {
+  double  rtol;                  /* stop when |r| <= max(rtol |b|, atol)      */
+  double  atol;
+  int32_t maxiter;
+  int32_t check_every;           /* iterations between host status reads      */
+  int32_t precision;             /* FEM_F64: plain FP64 PCG. FEM_F32: FP32 PCG,
+                                    x accumulated in FP64, FP64 true-residual
+                                    replacement through `op` (required)       */
+  int32_t precond;               /* 1: Jacobi, 0: none                        */
+  double  reliable_delta;        /* FP32 mode: replace the residual when the
+                                    recursive |r| fell by this factor         */
+  int32_t xcd_remap;             /* 1: XCD-contiguous block order in SpMV     */
+  int32_t pad;
+} fem_pcg_options_t;
+
+typedef struct {
+  int32_t iterations;
+  int32_t replacements;          /* FP64 residual replacements (FP32 mode)    */
+  int32_t status;                /* FEM_OK / FEM_ERR_NOT_CONVERGED / NAN      */
+  int32_t pad;
+  double  b_norm;                /* |b|                                       */
+  double  res_norm;              /* final true residual norm |b - A x|        */
+} fem_pcg_result_t;
+
+/* ---- library --------------------------------------------------------- */
+const char* fem_last_error(void);
+int fem_version(void);
+
+/* ---- pattern construction (problem.py:86-107, solver.py:472-478) ------- */
+/* Node -> (cell, local node) incidence lists, sorted ascending by c*N+a.
+ * inc_ptr (num_nodes+1), inc_idx (num_entities*N); ws >= fem_scan_ws_bytes(num_nodes)
+ * + 4*num_nodes bytes.  Also used for boundary faces (conn = face cells). */
+int fem_incidence(int64_t num_entities, int32_t N, const int32_t* conn,
+                  int64_t num_nodes, int32_t* inc_ptr, int32_t* inc_idx,
+                  void* ws, void* stream);
+/* Workspace bytes for fem_exclusive_scan over n items. */
+int64_t fem_scan_ws_bytes(int64_t n);
+/* out[0..n] = exclusive prefix sum of in[0..n-1]; out[n] = total. */
+int fem_exclusive_scan(int64_t n, const int32_t* in, int32_t* out, void* ws, void* stream);
+/* Per node row: sort the (column, cell-block) candidates of its incident cells,
+ * write the sorted cell-block ids to contrib_idx[inc_ptr[row]*N ...] and the
+ * number of distinct columns to row_len.  FEM_ERR_CAPACITY past 2048
+ * candidates per row. */
+int fem_pattern_rows(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32_t* inc_idx,
+                     int32_t* contrib_idx, int32_t* row_len, int32_t* err_flag, void* stream);
+/* Fill col_idx, contrib_ptr, diag_slot from row_ptr (scan of row_len). */
+int fem_pattern_fill(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32_t* contrib_idx,
+                     const int32_t* row_ptr, int32_t* col_idx, int32_t* contrib_ptr,
+                     int32_t* diag_slot, void* stream);
+/* 64*max(row_len) per slice, to be scanned into slice_ptr. */
+int fem_sell_slice_sizes(int64_t num_rows, const int32_t* row_ptr, int32_t* slice_blocks, void* stream);
+/* Fill sell_col from the block CSR pattern. */
+int fem_sell_fill(const fem_pattern_t* pat, int32_t* sell_col, void* stream);
+
+/* ---- element kernels (fe.py:112-218, problem.py:189-356) --------------- */
+/* shape_grads (C,Q,N,3), JxW (C,Q), quad points (C,Q,3); any output may be NULL. */
+int fem_geometry(const fem_mesh_t* mesh, double* shape_grads, double* JxW,
+                 double* quad_points, void* stream);
+/* Per-cell tangent blocks Ke[c][a][b][vec][vec] in `precision` (float/double). */
+int fem_cell_tangent(const fem_mesh_t* mesh, const fem_material_t* mat, int32_t precision,
+                     const double* u, void* Ke, void* stream);
+/* Per-cell internal-force residual re[c][a][vec] (FP64). */
+int fem_cell_residual(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u,
+                      double* re, void* stream);
+/* Per-cell tangent-vector product re[c][a][vec] = K_c(u) v_c (FP64). */
+int fem_cell_jvp(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u,
+                 const double* v, double* re, void* stream);
+/* Per-cell load re[c][a][i] = sum_q m[c][q][i] N_a(q) JxW(c,q) (mass_map term, FP64). */
+int fem_cell_load(const fem_mesh_t* mesh, const double* m, double* re, void* stream);
+/* Transpose of fem_cell_load: lq[c][q][i] = sum_a lam[cells[c][a]][i] N_a(q) JxW(c,q)
+ * (parameter gradient of a mass_map term). */
+int fem_cell_load_adjoint(const fem_mesh_t* mesh, const double* lam, double* lq, void* stream);
+/* Adjoint kernel (solver.py:1386-1416): g[c][q] = -JxW lam_q . dP/dtheta(grad u_q) grad N,
+ * i.e. -lambda^T dR/dtheta_{c,q} for the per-quad material parameter. */
+int fem_cell_adjoint(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u,
+                     const double* lam, double* g, void* stream);
+/* Boundary faces (fe.py:143-183, 199-218): faces (S,2) = (cell, local face).
+ * Outputs (any may be NULL): face shape grads (S,FQ,N,3), nanson scale (S,FQ),
+ * physical quad points (S,FQ,3). */
+int fem_face_geometry(const fem_mesh_t* mesh, int64_t S, const int32_t* faces,
+                      double* face_shape_grads, double* nanson, double* quad_points, void* stream);
+/* Face load rf[s][a][i] = sum_fq t[s][fq][i] Nf_a(fq) nanson(s,fq) (surface_map term). */
+int fem_face_load(const fem_mesh_t* mesh, int64_t S, const int32_t* faces, const double* t,
+                  double* rf, void* stream);
+/* Transpose of fem_face_load: lq[s][fq][i] = sum_a lam[node_a][i] Nf_a(fq) nanson(s,fq). */
+int fem_face_load_adjoint(const fem_mesh_t* mesh, int64_t S, const int32_t* faces, const double* lam,
+                          double* lq, void* stream);
+
+/* ---- scatter / global matrix (problem.py:426-437, solver.py:522-529) --- */
+/* out[n*vec+i] = base[n*vec+i] (0 if base is NULL; base may alias out)
+ *              + sum over incidences t of node n of ent[inc_idx[t]*vec+i],
+ * summed in incidence order (problem.py:426-437 scatter-add, deterministic). */
+int fem_gather_nodal(int64_t num_nodes, int32_t vec, const int32_t* inc_ptr, const int32_t* inc_idx,
+                     const double* ent, const double* base, double* out, void* stream);
+/* SELL values = sum of the cell blocks of Ke (duplicates summed in cell order). */
+int fem_gather_matrix(const fem_pattern_t* pat, int32_t precision, const void* Ke, void* vals, void* stream);
+/* Dirichlet row elimination (PETSc zeroRows, diag = 1): row d zeroed, A[d,d] = 1. */
+int fem_matrix_bc_rows(const fem_pattern_t* pat, int32_t precision, void* vals, int64_t n_bc,
+                       const int32_t* bc_dofs, void* stream);
+/* dinv[d] = 1/A[d,d] (0 where the diagonal is absent), in `precision`. */
+int fem_jacobi(const fem_pattern_t* pat, int32_t precision, const void* vals, void* dinv, void* stream);
+/* res[d] = u[d] - scale*g[d] on Dirichlet dofs (solver.py:290-304). */
+int fem_residual_bc(int64_t n_bc, const int32_t* bc_dofs, const double* bc_vals, const double* u,
+                    double* res, double scale, void* stream);
+/* y = A x in `precision` (x, y of that precision). */
+int fem_spmv(const fem_pattern_t* pat, int32_t precision, const void* vals, const void* x, void* y,
+             void* stream);
+/* Scalar CSR export: indptr (vec*rows+1), indices (nnz), data (nnz, FP64). */
+int fem_export_csr(const fem_pattern_t* pat, int32_t precision, const void* vals, int32_t* indptr,
+                   int32_t* indices, double* data, void* stream);
+/* y = dR/du(u) x (FP64, matrix-free, BC rows = identity). */
+int fem_operator_apply(const fem_operator_t* op, const double* x, double* y, void* stream);
+
+/* ---- Krylov solve (solver.py:63-92, 261-284) -------------------------- */
+/* Workspace bytes for fem_pcg_solve. */
+int64_t fem_pcg_ws_bytes(const fem_pattern_t* pat, int32_t precision);
+/* Jacobi-PCG on the row-eliminated tangent.  b, x are FP64 (x holds x0 on
+ * entry).  vals/dinv are in opt->precision.  `op` is required for FEM_F32
+ * and ignored for FEM_F64.  `host_status` is pinned host memory of at least
+ * 64 bytes.  Synchronises the stream.  Returns FEM_OK or the solve status. */
+int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv,
+                  const fem_operator_t* op, const double* b, double* x,
+                  const fem_pcg_options_t* opt, void* ws, void* host_status,
+                  fem_pcg_result_t* result /* host */, void* stream);
+
+/* ---- vector helpers -------------------------------------------------- */
+/* out = sum_i x_i y_i (FP64), deterministic; ws >= 8 * 1024 bytes. */
+int fem_dot(int64_t n, const double* x, const double* y, double* out, void* ws, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FEM_MI355X_H */

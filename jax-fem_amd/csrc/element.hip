@@ -1,0 +1,625 @@
+// Per-cell element kernels: geometry, tangent, residual, tangent-vector
+// product, adjoint parameter gradient, body/surface loads.
+//
+// Replaces the vmapped per-cell work of jax-fem:
+//   fe.py:112-141       get_shape_grads (Jacobian, inverse, physical grad N, JxW)
+//   fe.py:143-183       get_face_shape_grads (Nanson scale)
+//   fe.py:185-218       physical (surface) quadrature points
+//   problem.py:189-259  laplace / mass / surface kernels
+//   problem.py:261-356  value_and_jacfwd per cell (here: closed-form tangents)
+//   solver.py:1386-1416 vjp of the residual w.r.t. parameters (adjoint)
+//
+// Layout: a cell is processed by TPC threads (one wave for HEX8, four for
+// HEX27); the nodal gather (node ids, coordinates, solution) is staged in LDS,
+// then per-quad Jacobians, physical shape gradients and material states are
+// built in LDS and consumed by (a,b) block pairs or (a,i) rows.  Geometry is
+// computed from coordinates relative to the cell's first node (differences in
+// FP64, then cast), so FP32 instances keep a relative accuracy of ~eps32.
+#include "materials.hpp"
+
+#include <algorithm>
+
+namespace fem {
+
+template <int N_, int Q_>
+struct ElemCfg {
+  static constexpr int N = N_;
+  static constexpr int Q = Q_;
+  static constexpr int TPC = (N_ * N_ <= 64) ? 64 : 256;  // threads per cell
+  static constexpr int CPW = 256 / TPC;                   // cells per workgroup
+};
+
+// LDS state of one cell.
+template <int N, int Q, typename T>
+struct CellLds {
+  double xd[N][3];
+  T invJ[Q][9];
+  T jxw[Q];
+  T g[Q][N][3];
+};
+
+// Stage node coordinates, Jacobians and physical gradients of cell c.
+// Must be called by all threads of the workgroup (contains barriers).
+template <int N, int Q, int TPC, typename T>
+__device__ inline void stage_geometry(const fem_mesh_t& mesh, int64_t c, bool active, int t,
+                                      const T (&dN)[Q][N][3], const double (&w)[Q], CellLds<N, Q, T>& L,
+                                      int32_t (&node)[N]) {
+  if (active && t < N) {
+    const int32_t n = mesh.cells[c * N + t];
+    node[t] = n;
+    L.xd[t][0] = mesh.points[(int64_t)n * 3 + 0];
+    L.xd[t][1] = mesh.points[(int64_t)n * 3 + 1];
+    L.xd[t][2] = mesh.points[(int64_t)n * 3 + 2];
+  }
+  __syncthreads();
+  if (active) {
+    for (int q = t; q < Q; q += TPC) {
+      T J[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) J[k] = T(0);
+      for (int a = 0; a < N; ++a) {
+        const T x0 = T(L.xd[a][0] - L.xd[0][0]);
+        const T x1 = T(L.xd[a][1] - L.xd[0][1]);
+        const T x2 = T(L.xd[a][2] - L.xd[0][2]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const T d = dN[q][a][j];
+          J[0 * 3 + j] += x0 * d;
+          J[1 * 3 + j] += x1 * d;
+          J[2 * 3 + j] += x2 * d;
+        }
+      }
+      const T det = det3(J);
+      inv3(J, det, L.invJ[q]);
+      L.jxw[q] = det * T(w[q]);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (int it = t; it < Q * N; it += TPC) {
+      const int q = it / N, a = it % N;
+      const T* iJ = L.invJ[q];
+      const T d0 = dN[q][a][0], d1 = dN[q][a][1], d2 = dN[q][a][2];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) L.g[q][a][j] = d0 * iJ[0 * 3 + j] + d1 * iJ[1 * 3 + j] + d2 * iJ[2 * 3 + j];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int Q, typename T>
+__device__ inline void load_tables(const fem_mesh_t& mesh, T (&dN)[Q][N][3], double (&w)[Q]) {
+  T* flat = &dN[0][0][0];
+  for (int i = threadIdx.x; i < Q * N * 3; i += blockDim.x) flat[i] = T(mesh.shape_grads_ref[i]);
+  for (int i = threadIdx.x; i < Q; i += blockDim.x) w[i] = mesh.quad_weights[i];
+}
+
+// grad f at quad q from nodal values f[N][V] (LDS)
+template <int N, int Q, int V, typename T>
+__device__ inline void quad_grad(const CellLds<N, Q, T>& L, const T (&f)[N][V], int q, T (&G)[V][3]) {
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G[i][j] = T(0);
+  for (int a = 0; a < N; ++a) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const T fa = f[a][i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G[i][j] += fa * L.g[q][a][j];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ tangent
+template <class Cfg, class MAT, typename T>
+__global__ __launch_bounds__(256) void k_cell_tangent(fem_mesh_t mesh, fem_material_t mat,
+                                                      const double* __restrict__ u, T* __restrict__ Ke) {
+  constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW, V = MAT::V;
+  using St = typename MAT::template St<T>;
+  __shared__ T s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ CellLds<N, Q, T> s_cell[CPW];
+  __shared__ T s_u[CPW][N][V];
+  __shared__ St s_st[CPW][Q];
+  __shared__ int32_t s_node[CPW][N];
+  const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  const int64_t c = (int64_t)blockIdx.x * CPW + w;
+  const bool active = c < mesh.num_cells;
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  CellLds<N, Q, T>& L = s_cell[w];
+  if (MAT::kNeedsU && active && t < N) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) s_u[w][t][i] = T(u[(int64_t)s_node[w][t] * V + i]);
+  }
+  __syncthreads();
+  if (active) {
+    for (int q = t; q < Q; q += TPC) {
+      T G[V][3];
+      if (MAT::kNeedsU) {
+        quad_grad<N, Q, V, T>(L, s_u[w], q, G);
+      } else {
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) G[i][j] = T(0);
+      }
+      const double prm = mat.has_param ? mat.param[c * Q + q] : 1.0;
+      s_st[w][q] = MAT::template prep<T>(mat, prm, G);
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  for (int pr = t; pr < N * N; pr += TPC) {
+    const int a = pr / N, b = pr % N;
+    T K[V][V];
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int k = 0; k < V; ++k) K[i][k] = T(0);
+    for (int q = 0; q < Q; ++q) MAT::tangent(s_st[w][q], L.g[q][a], L.g[q][b], L.jxw[q], K);
+    T* out = Ke + ((size_t)c * N * N + pr) * (V * V);
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int k = 0; k < V; ++k) out[i * V + k] = K[i][k];
+  }
+}
+
+// -------------------------------------------- residual / jvp (FP64 vectors)
+// MODE 0: re = sum_q JxW P(grad u) grad N          (internal force)
+// MODE 1: re = sum_q JxW dP/dF(grad u)[grad v] grad N   (tangent-vector product)
+template <class Cfg, class MAT, typename T, int MODE>
+__global__ __launch_bounds__(256) void k_cell_vector(fem_mesh_t mesh, fem_material_t mat,
+                                                     const double* __restrict__ u, const double* __restrict__ v,
+                                                     double* __restrict__ re) {
+  constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW, V = MAT::V;
+  __shared__ T s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ CellLds<N, Q, T> s_cell[CPW];
+  __shared__ T s_u[CPW][N][V];
+  __shared__ T s_v[CPW][N][V];
+  __shared__ T s_P[CPW][Q][V][3];
+  __shared__ int32_t s_node[CPW][N];
+  const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  const int64_t c = (int64_t)blockIdx.x * CPW + w;
+  const bool active = c < mesh.num_cells;
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  CellLds<N, Q, T>& L = s_cell[w];
+  if (active && t < N) {
+    const int64_t n = s_node[w][t];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      s_u[w][t][i] = T(u[n * V + i]);
+      if (MODE == 1) s_v[w][t][i] = T(v[n * V + i]);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (int q = t; q < Q; q += TPC) {
+      T G[V][3], P[V][3];
+      quad_grad<N, Q, V, T>(L, s_u[w], q, G);
+      const double prm = mat.has_param ? mat.param[c * Q + q] : 1.0;
+      const auto st = MAT::template prep<T>(mat, prm, G);
+      if (MODE == 0) {
+        MAT::stress(st, G, P);
+      } else {
+        T dG[V][3];
+        quad_grad<N, Q, V, T>(L, s_v[w], q, dG);
+        MAT::jvp(st, dG, P);
+      }
+      const T jw = L.jxw[q];
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) s_P[w][q][i][j] = P[i][j] * jw;
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  for (int it = t; it < N * V; it += TPC) {
+    const int a = it / V, i = it % V;
+    T r = 0;
+    for (int q = 0; q < Q; ++q)
+      r += s_P[w][q][i][0] * L.g[q][a][0] + s_P[w][q][i][1] * L.g[q][a][1] + s_P[w][q][i][2] * L.g[q][a][2];
+    re[((size_t)c * N + a) * V + i] = double(r);
+  }
+}
+
+// ------------------------------------------------------------------ adjoint
+// g[c][q] = - JxW (dP/dtheta)(grad u) : grad lam   at quad q of cell c.
+template <class Cfg, class MAT>
+__global__ __launch_bounds__(256) void k_cell_adjoint(fem_mesh_t mesh, fem_material_t mat,
+                                                      const double* __restrict__ u, const double* __restrict__ lam,
+                                                      double* __restrict__ gout) {
+  using T = double;
+  constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW, V = MAT::V;
+  __shared__ T s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ CellLds<N, Q, T> s_cell[CPW];
+  __shared__ T s_u[CPW][N][V];
+  __shared__ T s_l[CPW][N][V];
+  __shared__ int32_t s_node[CPW][N];
+  const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  const int64_t c = (int64_t)blockIdx.x * CPW + w;
+  const bool active = c < mesh.num_cells;
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  CellLds<N, Q, T>& L = s_cell[w];
+  if (active && t < N) {
+    const int64_t n = s_node[w][t];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      s_u[w][t][i] = u[n * V + i];
+      s_l[w][t][i] = lam[n * V + i];
+    }
+  }
+  __syncthreads();
+  if (!active) return;
+  for (int q = t; q < Q; q += TPC) {
+    T G[V][3], GL[V][3], dP[V][3];
+    quad_grad<N, Q, V, T>(L, s_u[w], q, G);
+    quad_grad<N, Q, V, T>(L, s_l[w], q, GL);
+    const double prm = mat.has_param ? mat.param[c * Q + q] : 1.0;
+    const auto st = MAT::template prep<T>(mat, prm, G);
+    MAT::dparam(mat, prm, st, G, dP);
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) s += dP[i][j] * GL[i][j];
+    gout[c * Q + q] = -s * L.jxw[q];
+  }
+}
+
+// ------------------------------------------------ geometry and body loads
+template <class Cfg>
+__global__ __launch_bounds__(256) void k_geometry(fem_mesh_t mesh, double* __restrict__ shape_grads,
+                                                  double* __restrict__ JxW, double* __restrict__ qpts) {
+  using T = double;
+  constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW;
+  __shared__ T s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ CellLds<N, Q, T> s_cell[CPW];
+  __shared__ int32_t s_node[CPW][N];
+  const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  const int64_t c = (int64_t)blockIdx.x * CPW + w;
+  const bool active = c < mesh.num_cells;
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  if (!active) return;
+  CellLds<N, Q, T>& L = s_cell[w];
+  if (shape_grads)
+    for (int it = t; it < Q * N * 3; it += TPC) shape_grads[(size_t)c * Q * N * 3 + it] = (&L.g[0][0][0])[it];
+  if (JxW)
+    for (int q = t; q < Q; q += TPC) JxW[c * Q + q] = L.jxw[q];
+  if (qpts)
+    for (int it = t; it < Q * 3; it += TPC) {
+      const int q = it / 3, d = it % 3;
+      double x = 0;
+      for (int a = 0; a < N; ++a) x += mesh.shape_vals[q * N + a] * L.xd[a][d];
+      qpts[(size_t)c * Q * 3 + it] = x;
+    }
+}
+
+// re[c][a][i] = sum_q m[c][q][i] N_a(q) JxW(c, q); the vec count is runtime (1..3).
+template <class Cfg>
+__global__ __launch_bounds__(256) void k_cell_load(fem_mesh_t mesh, int V, const double* __restrict__ m,
+                                                   double* __restrict__ re) {
+  using T = double;
+  constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW;
+  __shared__ T s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ CellLds<N, Q, T> s_cell[CPW];
+  __shared__ int32_t s_node[CPW][N];
+  const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  const int64_t c = (int64_t)blockIdx.x * CPW + w;
+  const bool active = c < mesh.num_cells;
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  if (!active) return;
+  CellLds<N, Q, T>& L = s_cell[w];
+  for (int it = t; it < N * V; it += TPC) {
+    const int a = it / V, i = it % V;
+    double r = 0;
+    for (int q = 0; q < Q; ++q) r += m[((size_t)c * Q + q) * V + i] * mesh.shape_vals[q * N + a] * L.jxw[q];
+    re[((size_t)c * N + a) * V + i] = r;
+  }
+}
+
+// Transpose of k_cell_load: lq[c][q][i] = sum_a lam[node_a][i] N_a(q) JxW(c, q).
+template <class Cfg>
+__global__ __launch_bounds__(256) void k_cell_load_adjoint(fem_mesh_t mesh, int V, const double* __restrict__ lam,
+                                                           double* __restrict__ lq) {
+  using T = double;
+  constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW;
+  __shared__ T s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ CellLds<N, Q, T> s_cell[CPW];
+  __shared__ int32_t s_node[CPW][N];
+  const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  const int64_t c = (int64_t)blockIdx.x * CPW + w;
+  const bool active = c < mesh.num_cells;
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  if (!active) return;
+  CellLds<N, Q, T>& L = s_cell[w];
+  for (int it = t; it < Q * V; it += TPC) {
+    const int q = it / V, i = it % V;
+    double r = 0;
+    for (int a = 0; a < N; ++a) r += lam[(int64_t)s_node[w][a] * V + i] * mesh.shape_vals[q * N + a];
+    lq[((size_t)c * Q + q) * V + i] = r * L.jxw[q];
+  }
+}
+
+// --------------------------------------------------------------- faces
+// One thread per selected face (boundary sets are small).
+template <int N>
+__global__ void k_face(fem_mesh_t mesh, int64_t S, const int32_t* __restrict__ faces, double* __restrict__ fgrads,
+                       double* __restrict__ nanson, double* __restrict__ fqpts, const double* __restrict__ tvals,
+                       double* __restrict__ rf, const double* __restrict__ lam, double* __restrict__ lamq) {
+  const int FQ = mesh.num_face_quads;
+  const int V = mesh.vec;
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < S; s += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = faces[2 * s];
+    const int f = faces[2 * s + 1];
+    int32_t node[N];
+    for (int a = 0; a < N; ++a) node[a] = mesh.cells[c * N + a];
+    const double* x0 = mesh.points + (int64_t)node[0] * 3;
+    if (rf)
+      for (int k = 0; k < N * V; ++k) rf[s * N * V + k] = 0.0;
+    for (int fq = 0; fq < FQ; ++fq) {
+      const double* dN = mesh.face_shape_grads_ref + ((size_t)f * FQ + fq) * N * 3;
+      const double* Nv = mesh.face_shape_vals + ((size_t)f * FQ + fq) * N;
+      double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int a = 0; a < N; ++a) {
+        const double* xa = mesh.points + (int64_t)node[a] * 3;
+        for (int i = 0; i < 3; ++i) {
+          const double xi = xa[i] - x0[i];
+          for (int j = 0; j < 3; ++j) J[i * 3 + j] += xi * dN[a * 3 + j];
+        }
+      }
+      const double det = det3(J);
+      double iJ[9];
+      inv3(J, det, iJ);
+      const double* nrm = mesh.face_normals + f * 3;
+      double nv[3];
+      for (int j = 0; j < 3; ++j) nv[j] = nrm[0] * iJ[0 * 3 + j] + nrm[1] * iJ[1 * 3 + j] + nrm[2] * iJ[2 * 3 + j];
+      const double ns = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]) * det * mesh.face_weights[f * FQ + fq];
+      if (nanson) nanson[s * FQ + fq] = ns;
+      if (fgrads)
+        for (int a = 0; a < N; ++a)
+          for (int j = 0; j < 3; ++j)
+            fgrads[((s * FQ + fq) * N + a) * 3 + j] =
+                dN[a * 3 + 0] * iJ[0 * 3 + j] + dN[a * 3 + 1] * iJ[1 * 3 + j] + dN[a * 3 + 2] * iJ[2 * 3 + j];
+      if (fqpts)
+        for (int d = 0; d < 3; ++d) {
+          double x = 0;
+          for (int a = 0; a < N; ++a) x += Nv[a] * mesh.points[(int64_t)node[a] * 3 + d];
+          fqpts[(s * FQ + fq) * 3 + d] = x;
+        }
+      if (rf)
+        for (int a = 0; a < N; ++a)
+          for (int i = 0; i < V; ++i) rf[(s * N + a) * V + i] += tvals[(s * FQ + fq) * V + i] * Nv[a] * ns;
+      if (lamq)
+        for (int i = 0; i < V; ++i) {
+          double acc = 0;
+          for (int a = 0; a < N; ++a) acc += lam[(int64_t)node[a] * V + i] * Nv[a];
+          lamq[(s * FQ + fq) * V + i] = acc * ns;
+        }
+    }
+  }
+}
+
+// ----------------------------------------------------------- dispatching
+template <class Cfg>
+static int grid_cells(const fem_mesh_t& m) {
+  return (int)ceil_div(m.num_cells, Cfg::CPW);
+}
+
+// Calls f(Cfg{}) for the supported (nodes per cell, quads) pairs.
+template <class F>
+static int with_cfg(const fem_mesh_t& m, F&& f) {
+  const int N = m.nodes_per_cell, Q = m.num_quads;
+  if (N == 8 && Q == 8) return f(ElemCfg<8, 8>{});
+  if (N == 8 && Q == 27) return f(ElemCfg<8, 27>{});
+  if (N == 27 && Q == 27) return f(ElemCfg<27, 27>{});
+  set_error("unsupported element/quadrature combination: %d nodes, %d quadrature points "
+            "(supported: HEX8 with 8 or 27 points, HEX27 with 27 points)", N, Q);
+  return FEM_ERR_BAD_ARG;
+}
+
+template <class F>
+static int with_mat(const fem_material_t& mat, int vec, F&& f) {
+  switch (mat.kind) {
+    case FEM_MAT_POISSON:
+      if (vec != 1) break;
+      return f(MatPoisson{});
+    case FEM_MAT_LINEAR_ELASTIC:
+      if (vec != 3) break;
+      return f(MatElastic<false>{});
+    case FEM_MAT_SIMP:
+      if (vec != 3 || !mat.has_param) break;
+      return f(MatElastic<true>{});
+    case FEM_MAT_NEO_HOOKEAN:
+      if (vec != 3) break;
+      return f(MatNeoHookean{});
+    default:
+      break;
+  }
+  set_error("unsupported material kind %d for vec=%d (param %d)", mat.kind, vec, mat.has_param);
+  return FEM_ERR_BAD_ARG;
+}
+
+static int check_mesh(const fem_mesh_t* m) {
+  FEM_CHECK_ARG(m && m->points && m->cells && m->shape_grads_ref && m->quad_weights && m->shape_vals,
+                "mesh descriptor has NULL fields");
+  FEM_CHECK_ARG(m->num_cells >= 0 && m->num_nodes > 0, "mesh sizes invalid");
+  FEM_CHECK_ARG(m->vec == 1 || m->vec == 3, "vec must be 1 or 3 (got %d)", m->vec);
+  return FEM_OK;
+}
+
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" {
+
+int fem_geometry(const fem_mesh_t* mesh, double* shape_grads, double* JxW, double* quad_points, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    hipLaunchKernelGGL(k_geometry<Cfg>, dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0, as_stream(stream), *mesh,
+                       shape_grads, JxW, quad_points);
+    FEM_LAUNCHED();
+    return FEM_OK;
+  });
+}
+
+int fem_cell_tangent(const fem_mesh_t* mesh, const fem_material_t* mat, int32_t precision, const double* u,
+                     void* Ke, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(mat && Ke && u, "fem_cell_tangent: NULL argument");
+  FEM_CHECK_ARG(precision == FEM_F32 || precision == FEM_F64, "fem_cell_tangent: bad precision");
+  FEM_CHECK_ARG(!mat->has_param || mat->param, "fem_cell_tangent: material param missing");
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    return with_mat(*mat, mesh->vec, [&](auto m) -> int {
+      using MAT = decltype(m);
+      if (precision == FEM_F32)
+        hipLaunchKernelGGL((k_cell_tangent<Cfg, MAT, float>), dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0,
+                           as_stream(stream), *mesh, *mat, u, (float*)Ke);
+      else
+        hipLaunchKernelGGL((k_cell_tangent<Cfg, MAT, double>), dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0,
+                           as_stream(stream), *mesh, *mat, u, (double*)Ke);
+      FEM_LAUNCHED();
+      return FEM_OK;
+    });
+  });
+}
+
+static int cell_vector(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u, const double* v,
+                       double* re, int mode, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(mat && u && re && (mode == 0 || v), "cell vector kernel: NULL argument");
+  FEM_CHECK_ARG(!mat->has_param || mat->param, "cell vector kernel: material param missing");
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    return with_mat(*mat, mesh->vec, [&](auto m) -> int {
+      using MAT = decltype(m);
+      if (mode == 0)
+        hipLaunchKernelGGL((k_cell_vector<Cfg, MAT, double, 0>), dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0,
+                           as_stream(stream), *mesh, *mat, u, v, re);
+      else
+        hipLaunchKernelGGL((k_cell_vector<Cfg, MAT, double, 1>), dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0,
+                           as_stream(stream), *mesh, *mat, u, v, re);
+      FEM_LAUNCHED();
+      return FEM_OK;
+    });
+  });
+}
+
+int fem_cell_residual(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u, double* re,
+                      void* stream) {
+  return cell_vector(mesh, mat, u, nullptr, re, 0, stream);
+}
+
+int fem_cell_jvp(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u, const double* v, double* re,
+                 void* stream) {
+  return cell_vector(mesh, mat, u, v, re, 1, stream);
+}
+
+int fem_cell_adjoint(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u, const double* lam,
+                     double* g, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(mat && u && lam && g, "fem_cell_adjoint: NULL argument");
+  FEM_CHECK_ARG(mat->has_param && mat->param, "fem_cell_adjoint: the material has no per-quad parameter");
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    return with_mat(*mat, mesh->vec, [&](auto m) -> int {
+      using MAT = decltype(m);
+      hipLaunchKernelGGL((k_cell_adjoint<Cfg, MAT>), dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0, as_stream(stream),
+                         *mesh, *mat, u, lam, g);
+      FEM_LAUNCHED();
+      return FEM_OK;
+    });
+  });
+}
+
+int fem_cell_load(const fem_mesh_t* mesh, const double* m, double* re, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(m && re, "fem_cell_load: NULL argument");
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    hipLaunchKernelGGL(k_cell_load<Cfg>, dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0, as_stream(stream), *mesh,
+                       mesh->vec, m, re);
+    FEM_LAUNCHED();
+    return FEM_OK;
+  });
+}
+
+static int face_common(const fem_mesh_t* mesh, int64_t S, const int32_t* faces, double* fgrads, double* nanson,
+                       double* fqpts, const double* t, double* rf, const double* lam, double* lamq, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(mesh->face_shape_vals && mesh->face_shape_grads_ref && mesh->face_weights && mesh->face_normals &&
+                    mesh->num_face_quads > 0,
+                "face tables missing from mesh descriptor");
+  FEM_CHECK_ARG(S >= 0 && (S == 0 || faces), "face kernel: bad face list");
+  if (S == 0) return FEM_OK;
+  const int grid = (int)std::min<int64_t>(ceil_div(S, 128), 4096);
+  if (mesh->nodes_per_cell == 8)
+    hipLaunchKernelGGL(k_face<8>, dim3(grid), dim3(128), 0, as_stream(stream), *mesh, S, faces, fgrads, nanson, fqpts,
+                       t, rf, lam, lamq);
+  else if (mesh->nodes_per_cell == 27)
+    hipLaunchKernelGGL(k_face<27>, dim3(grid), dim3(128), 0, as_stream(stream), *mesh, S, faces, fgrads, nanson,
+                       fqpts, t, rf, lam, lamq);
+  else
+    FEM_CHECK_ARG(false, "face kernel: unsupported nodes per cell %d", mesh->nodes_per_cell);
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_face_geometry(const fem_mesh_t* mesh, int64_t S, const int32_t* faces, double* face_shape_grads,
+                      double* nanson, double* quad_points, void* stream) {
+  return face_common(mesh, S, faces, face_shape_grads, nanson, quad_points, nullptr, nullptr, nullptr, nullptr, stream);
+}
+
+int fem_face_load(const fem_mesh_t* mesh, int64_t S, const int32_t* faces, const double* t, double* rf,
+                  void* stream) {
+  FEM_CHECK_ARG(t && rf, "fem_face_load: NULL argument");
+  return face_common(mesh, S, faces, nullptr, nullptr, nullptr, t, rf, nullptr, nullptr, stream);
+}
+
+int fem_face_load_adjoint(const fem_mesh_t* mesh, int64_t S, const int32_t* faces, const double* lam, double* lq,
+                          void* stream) {
+  FEM_CHECK_ARG(lam && lq, "fem_face_load_adjoint: NULL argument");
+  return face_common(mesh, S, faces, nullptr, nullptr, nullptr, nullptr, nullptr, lam, lq, stream);
+}
+
+int fem_cell_load_adjoint(const fem_mesh_t* mesh, const double* lam, double* lq, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(lam && lq, "fem_cell_load_adjoint: NULL argument");
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    hipLaunchKernelGGL(k_cell_load_adjoint<Cfg>, dim3(grid_cells<Cfg>(*mesh)), dim3(256), 0, as_stream(stream),
+                       *mesh, mesh->vec, lam, lq);
+    FEM_LAUNCHED();
+    return FEM_OK;
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,473 @@
+"""Device state of a problem: mesh descriptors, matrix pattern, buffers, calls.
+
+Everything numerical here is a call into libfem_mi355x (csrc/); torch only
+allocates device memory, supplies the stream and evaluates the user's own
+mass/surface callbacks at quadrature points.
+"""
+import ctypes as C
+import logging
+
+import numpy as onp
+import torch
+from torch.func import vmap
+
+from . import _lib
+from ._lib import check, ptr, stream_ptr
+from .materials import identify
+
+logger = logging.getLogger('jax_fem_amd')
+
+
+def _i32(x, device):
+    return torch.as_tensor(onp.ascontiguousarray(onp.asarray(x, dtype=onp.int32)), device=device)
+
+
+def _f64(x, device):
+    return torch.as_tensor(onp.ascontiguousarray(onp.asarray(x, dtype=onp.float64)), device=device)
+
+
+class DeviceMesh:
+    """Device copies of one FiniteElement's mesh and reference tables."""
+
+    def __init__(self, fe):
+        L = _lib.lib()
+        self.L = L
+        self.device = torch.device('cuda', torch.cuda.current_device())
+        d = self.device
+        self.points = _f64(fe.points, d)
+        self.cells = _i32(fe.cells, d)
+        self.shape_vals = _f64(fe.shape_vals, d)
+        self.shape_grads_ref = _f64(fe.shape_grads_ref, d)
+        self.quad_weights = _f64(fe.quad_weights, d)
+        self.face_shape_vals = _f64(fe.face_shape_vals, d)
+        self.face_shape_grads_ref = _f64(fe.face_shape_grads_ref, d)
+        self.face_weights = _f64(fe.face_quad_weights, d)
+        self.face_normals = _f64(fe.face_normals, d)
+        self.C, self.N = fe.cells.shape
+        self.Q = fe.num_quads
+        self.FQ = fe.num_face_quads
+        self.V = fe.vec
+        self.Nn = fe.num_total_nodes
+        m = _lib.FemMesh()
+        m.nodes_per_cell, m.num_quads, m.vec, m.num_face_quads = self.N, self.Q, self.V, self.FQ
+        m.num_cells, m.num_nodes = self.C, self.Nn
+        for name in ('points', 'cells', 'shape_vals', 'shape_grads_ref', 'quad_weights', 'face_shape_vals',
+                     'face_shape_grads_ref', 'face_weights', 'face_normals'):
+            setattr(m, name, getattr(self, name).data_ptr())
+        self.desc = m
+        self.ref = C.byref(m)
+
+    def empty(self, *shape, dtype=torch.float64):
+        return torch.empty(shape, dtype=dtype, device=self.device)
+
+    # -------------------------------------------------------- geometry
+    def geometry(self, want_grads=False, want_jxw=False, want_qpts=False):
+        g = self.empty(self.C, self.Q, self.N, 3) if want_grads else None
+        j = self.empty(self.C, self.Q) if want_jxw else None
+        x = self.empty(self.C, self.Q, 3) if want_qpts else None
+        check(self.L.fem_geometry(self.ref, ptr(g), ptr(j), ptr(x), stream_ptr()), 'fem_geometry')
+        return g, j, x
+
+    def faces_tensor(self, boundary_inds):
+        return _i32(onp.asarray(boundary_inds).reshape(-1, 2), self.device)
+
+    def face_geometry(self, boundary_inds, want_grads=False, want_nanson=False, want_qpts=False):
+        f = boundary_inds if torch.is_tensor(boundary_inds) else self.faces_tensor(boundary_inds)
+        S = f.shape[0]
+        g = self.empty(S, self.FQ, self.N, 3) if want_grads else None
+        ns = self.empty(S, self.FQ) if want_nanson else None
+        x = self.empty(S, self.FQ, 3) if want_qpts else None
+        check(self.L.fem_face_geometry(self.ref, S, ptr(f), ptr(g), ptr(ns), ptr(x), stream_ptr()),
+              'fem_face_geometry')
+        return g, ns, x
+
+    def incidence(self, conn):
+        """Node -> (entity, local node) incidence of an (E, N) int32 connectivity."""
+        E, N = conn.shape
+        inc_ptr = self.empty(self.Nn + 1, dtype=torch.int32)
+        inc_idx = self.empty(max(E * N, 1), dtype=torch.int32)
+        wsb = 4 * self.Nn + 512 + int(self.L.fem_scan_ws_bytes(self.Nn))
+        ws = self.empty(wsb, dtype=torch.uint8)
+        check(self.L.fem_incidence(E, N, ptr(conn), self.Nn, ptr(inc_ptr), ptr(inc_idx), ptr(ws), stream_ptr()),
+              'fem_incidence')
+        return inc_ptr, inc_idx
+
+    def scan(self, x):
+        n = x.numel()
+        out = self.empty(n + 1, dtype=torch.int32)
+        ws = self.empty(int(self.L.fem_scan_ws_bytes(n)), dtype=torch.uint8)
+        check(self.L.fem_exclusive_scan(n, ptr(x), ptr(out), ptr(ws), stream_ptr()), 'fem_exclusive_scan')
+        return out
+
+
+def _eval_quad_map(fn, V, u, x, ivs):
+    """Evaluate a user point map fn(u (V,), x (3,), *iv) at P points -> (P, V) float64.
+
+    Tries torch.func.vmap, then a transposed batch (point[d] -> row d), then a
+    point loop; the batched results are checked against point evaluation.
+    """
+    P = x.shape[0]
+    dev = x.device
+
+    def fix(r):
+        r = torch.as_tensor(r, dtype=torch.float64, device=dev)
+        if r.shape == (V,):
+            r = r.expand(P, V)
+        return r.reshape(P, V)
+
+    sample = onp.unique(onp.linspace(0, P - 1, num=min(P, 7)).astype(int))
+
+    def point(i):
+        return torch.as_tensor(fn(u[i], x[i], *[iv[i] for iv in ivs]), dtype=torch.float64,
+                               device=dev).reshape(V)
+
+    ref = torch.stack([point(int(i)) for i in sample]) if P else None
+
+    def ok(r):
+        return P == 0 or torch.allclose(r[sample], ref, rtol=1e-12, atol=1e-300)
+
+    try:
+        r = fix(vmap(fn)(u, x, *ivs))
+        if ok(r):
+            return r
+    except Exception:
+        pass
+    try:
+        tr = lambda t: t.transpose(0, -1) if t.dim() > 1 else t
+        r = fn(tr(u), tr(x), *[tr(iv) for iv in ivs])
+        r = torch.as_tensor(r, dtype=torch.float64, device=dev)
+        r = r.expand(P, V) if r.shape == (V,) else r.transpose(0, -1).reshape(P, V)
+        if ok(r):
+            return r
+    except Exception:
+        pass
+    return torch.stack([point(i) for i in range(P)]) if P else torch.zeros((0, V), dtype=torch.float64, device=dev)
+
+
+def _flatten_iv(iv, lead, device):
+    """(lead..., rest) internal variable -> (prod(lead), rest) float64 device tensor."""
+    t = torch.as_tensor(iv)
+    if t.device != device or t.dtype != torch.float64:
+        t = t.to(device=device, dtype=torch.float64)
+    return t.reshape(int(onp.prod(lead)), *t.shape[len(lead):])
+
+
+class DeviceProblem:
+    """Pattern, boundary data, loads and solver buffers of one Problem."""
+
+    def __init__(self, problem):
+        fe = problem.fes[0]
+        self.problem = problem
+        self.fe = fe
+        self.dm = fe.device_mesh()
+        dm = self.dm
+        self.L = dm.L
+        self.V = fe.vec
+        self.n = dm.Nn * self.V
+        self.inc_ptr, self.inc_idx = dm.incidence(dm.cells)
+        self._build_pattern()
+        self.surfaces = []
+        for bi in problem.boundary_inds_list:
+            faces = dm.faces_tensor(bi)
+            conn = dm.cells[faces[:, 0].long()].contiguous() if faces.shape[0] else \
+                torch.zeros((0, dm.N), dtype=torch.int32, device=dm.device)
+            fptr, fidx = dm.incidence(conn)
+            _, _, xq = dm.face_geometry(faces, want_qpts=True)
+            self.surfaces.append(dict(faces=faces, inc_ptr=fptr, inc_idx=fidx, xq=xq))
+        self._bc_sources = None
+        self._mat_cache = {}
+        self.buffers = {}
+        self.material = None
+        self.param = None
+        self.load = None
+        self.host_status = torch.empty(64, dtype=torch.uint8).pin_memory()
+
+    # ------------------------------------------------------------ pattern
+    def _build_pattern(self):
+        dm, L = self.dm, self.L
+        N, Nn = dm.N, dm.Nn
+        contrib_idx = dm.empty(max(dm.C * N * N, 1), dtype=torch.int32)
+        row_len = dm.empty(Nn, dtype=torch.int32)
+        err = dm.empty(1, dtype=torch.int32)
+        check(L.fem_pattern_rows(dm.ref, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(contrib_idx), ptr(row_len),
+                                 ptr(err), stream_ptr()), 'fem_pattern_rows')
+        if int(err.item()) > 0:
+            raise _lib.FemError(f"a node row has {int(err.item())} candidate blocks (capacity 2048)")
+        row_ptr = dm.scan(row_len)
+        nnzb = int(row_ptr[-1].item())
+        col_idx = dm.empty(nnzb, dtype=torch.int32)
+        contrib_ptr = dm.empty(nnzb + 1, dtype=torch.int32)
+        diag_slot = dm.empty(Nn, dtype=torch.int32)
+        check(L.fem_pattern_fill(dm.ref, ptr(self.inc_ptr), ptr(contrib_idx), ptr(row_ptr), ptr(col_idx),
+                                 ptr(contrib_ptr), ptr(diag_slot), stream_ptr()), 'fem_pattern_fill')
+        ns = (Nn + 63) // 64
+        slice_blocks = dm.empty(ns, dtype=torch.int32)
+        check(L.fem_sell_slice_sizes(Nn, ptr(row_ptr), ptr(slice_blocks), stream_ptr()), 'fem_sell_slice_sizes')
+        slice_ptr = dm.scan(slice_blocks)
+        sell_blocks = int(slice_ptr[-1].item())
+        sell_col = dm.empty(sell_blocks, dtype=torch.int32)
+        p = _lib.FemPattern()
+        p.num_rows, p.nnzb, p.num_slices, p.sell_blocks = Nn, nnzb, ns, sell_blocks
+        p.vec, p.num_cells, p.nodes_per_cell = self.V, dm.C, N
+        self.pat_tensors = dict(row_ptr=row_ptr, col_idx=col_idx, diag_slot=diag_slot, contrib_ptr=contrib_ptr,
+                                contrib_idx=contrib_idx, slice_ptr=slice_ptr, sell_col=sell_col)
+        for k, t in self.pat_tensors.items():
+            setattr(p, k, t.data_ptr())
+        self.pat = p
+        self.pref = C.byref(p)
+        check(L.fem_sell_fill(self.pref, ptr(sell_col), stream_ptr()), 'fem_sell_fill')
+        self.nnzb = nnzb
+        self.sell_blocks = sell_blocks
+        logger.debug("pattern: %d block rows, %d blocks (%d padded), vec %d", Nn, nnzb, sell_blocks, self.V)
+
+    # --------------------------------------------------------- preparation
+    def refresh_bc(self):
+        fe = self.fe
+        sources = tuple((id(a), id(b), id(c)) for a, b, c in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list))
+        if sources == self._bc_sources:
+            return
+        dofs, vals = [], []
+        for node_inds, vec_inds, v in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list):
+            node_inds = onp.asarray(node_inds).reshape(-1)
+            dofs.append(node_inds * self.V + onp.asarray(vec_inds).reshape(-1))
+            vals.append(onp.broadcast_to(onp.asarray(v, dtype=onp.float64).reshape(-1), node_inds.shape))
+        if dofs:
+            d = onp.concatenate(dofs)
+            v = onp.concatenate(vals)
+            # the last Dirichlet set wins on a shared dof (solver.py:297-300 applies them in order)
+            rd = d[::-1]
+            u, first = onp.unique(rd, return_index=True)
+            d, v = u, v[::-1][first]
+        else:
+            d, v = onp.zeros((0,), onp.int64), onp.zeros((0,))
+        self.bc_dofs = _i32(d, self.dm.device)
+        self.bc_vals = _f64(v, self.dm.device)
+        self.bc_dofs_long = self.bc_dofs.long()
+        self.n_bc = int(d.shape[0])
+        self._bc_sources = sources
+
+    def prepare(self):
+        """Material, per-quad parameter, constant loads and BC data from the problem's current state."""
+        pb, dm = self.problem, self.dm
+        self.refresh_bc()
+        ivs = list(pb.internal_vars) if pb.internal_vars is not None else []
+        C_, Q = dm.C, dm.Q
+        iv_shapes = tuple(tuple(torch.as_tensor(iv).shape[2:]) for iv in ivs)
+        for iv in ivs:
+            shp = tuple(torch.as_tensor(iv).shape)
+            if shp[:2] != (C_, Q):
+                raise ValueError(f"internal variable of shape {shp} must be (num_cells, num_quads, ...) = ({C_}, {Q}, ...)")
+        if not hasattr(pb, 'get_tensor_map'):
+            raise NotImplementedError("problems without get_tensor_map are outside the device path")
+        key = iv_shapes
+        if key not in self._mat_cache:
+            self._mat_cache[key] = identify(pb.get_tensor_map(), self.V, iv_shapes)
+        mat = self._mat_cache[key]
+        self.material = mat
+        if mat.has_param:
+            k = getattr(mat, 'param_index', 0)
+            if k >= len(ivs):
+                raise ValueError(f"{mat!r} reads internal variable {k}, the problem has {len(ivs)}")
+            self.param_index = k
+            self.param = _flatten_iv(ivs[k], (C_, Q), dm.device).reshape(C_, Q).contiguous()
+        else:
+            self.param_index = None
+            self.param = None
+        self.mat_desc = mat.to_struct(self.param.data_ptr() if self.param is not None else None)
+        self.mref = C.byref(self.mat_desc)
+        self.load = self._constant_loads(ivs)
+
+    def _constant_loads(self, ivs):
+        pb, dm = self.problem, self.dm
+        V, dev = self.V, dm.device
+        total = None
+        if hasattr(pb, 'get_mass_map'):
+            if 'xq' not in self.buffers:
+                self.buffers['xq'] = dm.geometry(want_qpts=True)[2].reshape(-1, 3)
+            xq = self.buffers['xq']
+            P = xq.shape[0]
+            civ = [_flatten_iv(iv, (dm.C, dm.Q), dev) for iv in ivs]
+            m = self._eval_u_independent(pb.get_mass_map(), xq, civ, 'mass_map')
+            cell = self.cell_buffer()
+            check(self.L.fem_cell_load(dm.ref, ptr(m), ptr(cell), stream_ptr()), 'fem_cell_load')
+            total = dm.empty(self.n)
+            check(self.L.fem_gather_nodal(dm.Nn, V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell), None,
+                                          ptr(total), stream_ptr()), 'fem_gather_nodal')
+            del P
+        if self.surfaces:
+            if not hasattr(pb, 'get_surface_maps'):
+                raise NotImplementedError("surface integrals without get_surface_maps are outside the device path")
+            smaps = pb.get_surface_maps()
+            for i, srf in enumerate(self.surfaces):
+                S = srf['faces'].shape[0]
+                if S == 0:
+                    continue
+                siv = pb.internal_vars_surfaces[i] if i < len(pb.internal_vars_surfaces) else ()
+                civ = [_flatten_iv(iv, (S, dm.FQ), dev) for iv in siv]
+                t = self._eval_u_independent(smaps[i], srf['xq'].reshape(-1, 3), civ, 'surface_map')
+                rf = dm.empty(S * dm.N * V)
+                check(self.L.fem_face_load(dm.ref, S, ptr(srf['faces']), ptr(t), ptr(rf), stream_ptr()),
+                      'fem_face_load')
+                if total is None:
+                    total = torch.zeros(self.n, dtype=torch.float64, device=dev)
+                check(self.L.fem_gather_nodal(dm.Nn, V, ptr(srf['inc_ptr']), ptr(srf['inc_idx']), ptr(rf),
+                                              ptr(total), ptr(total), stream_ptr()), 'fem_gather_nodal')
+        return total
+
+    def _eval_u_independent(self, fn, xq, civ, what):
+        P = xq.shape[0]
+        u0 = torch.zeros((P, self.V), dtype=torch.float64, device=xq.device)
+        vals = _eval_quad_map(fn, self.V, u0, xq, civ)
+        if P:
+            idx = torch.as_tensor(onp.unique(onp.linspace(0, P - 1, num=min(P, 5)).astype(int)), device=xq.device)
+            u1 = torch.full((len(idx), self.V), 0.37, dtype=torch.float64, device=xq.device)
+            alt = _eval_quad_map(fn, self.V, u1, xq[idx], [c[idx] for c in civ])
+            if not torch.allclose(alt, vals[idx], rtol=1e-12, atol=0):
+                raise NotImplementedError(f"a {what} that depends on u is outside the device path")
+        return vals.contiguous()
+
+    # ------------------------------------------------------------ buffers
+    def cell_buffer(self):
+        if 'cell' not in self.buffers:
+            self.buffers['cell'] = self.dm.empty(self.dm.C * self.dm.N * self.V)
+        return self.buffers['cell']
+
+    def matrix_buffers(self, precision):
+        key = ('mat', precision)
+        if key not in self.buffers:
+            dt = torch.float32 if precision == _lib.FEM_F32 else torch.float64
+            dm = self.dm
+            Ke = dm.empty(max(dm.C * dm.N * dm.N * self.V * self.V, 1), dtype=dt)
+            vals = dm.empty(max(self.sell_blocks * self.V * self.V, 1), dtype=dt)
+            dinv = dm.empty(self.n, dtype=dt)
+            self.buffers[key] = (Ke, vals, dinv)
+        return self.buffers[key]
+
+    # -------------------------------------------------------------- compute
+    def residual_unconstrained(self, u):
+        """R(u) without Dirichlet rows, FP64 (problem.py:439-445)."""
+        dm = self.dm
+        cell = self.cell_buffer()
+        check(self.L.fem_cell_residual(dm.ref, self.mref, ptr(u), ptr(cell), stream_ptr()), 'fem_cell_residual')
+        res = dm.empty(self.n)
+        check(self.L.fem_gather_nodal(dm.Nn, self.V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell),
+                                      ptr(self.load), ptr(res), stream_ptr()), 'fem_gather_nodal')
+        return res
+
+    def residual(self, u, scale=1.0):
+        """R(u) with Dirichlet rows u - g (solver.py:290-304)."""
+        res = self.residual_unconstrained(u)
+        check(self.L.fem_residual_bc(self.n_bc, ptr(self.bc_dofs), ptr(self.bc_vals), ptr(u), ptr(res),
+                                     float(scale), stream_ptr()), 'fem_residual_bc')
+        return res
+
+    def tangent(self, u, precision):
+        """Assemble the row-eliminated tangent in `precision` (problem.py:447-460, solver.py:522-529)."""
+        dm = self.dm
+        Ke, vals, dinv = self.matrix_buffers(precision)
+        s = stream_ptr()
+        check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
+        check(self.L.fem_gather_matrix(self.pref, precision, ptr(Ke), ptr(vals), s), 'fem_gather_matrix')
+        check(self.L.fem_matrix_bc_rows(self.pref, precision, ptr(vals), self.n_bc, ptr(self.bc_dofs), s),
+              'fem_matrix_bc_rows')
+        check(self.L.fem_jacobi(self.pref, precision, ptr(vals), ptr(dinv), s), 'fem_jacobi')
+        self.current = (precision, Ke, vals, dinv)
+        return self.current
+
+    def operator(self, u):
+        op = _lib.FemOperator()
+        op.mesh = C.pointer(self.dm.desc)
+        op.mat = C.pointer(self.mat_desc)
+        op.u = u.data_ptr()
+        op.inc_ptr, op.inc_idx = self.inc_ptr.data_ptr(), self.inc_idx.data_ptr()
+        op.n_bc = self.n_bc
+        op.bc_dofs = self.bc_dofs.data_ptr() if self.n_bc else None
+        op.cell_scratch = self.cell_buffer().data_ptr()
+        self._op_keep = (op, u)
+        return op
+
+    def apply_operator(self, u, x):
+        op = self.operator(u)
+        y = self.dm.empty(self.n)
+        check(self.L.fem_operator_apply(C.byref(op), ptr(x), ptr(y), stream_ptr()), 'fem_operator_apply')
+        return y
+
+    def solve(self, u, b, x0, cfg):
+        """Jacobi-PCG on the current tangent; returns (x, result struct)."""
+        precision, Ke, vals, dinv = self.current
+        o = _lib.FemPcgOptions()
+        o.rtol, o.atol, o.maxiter = cfg['tol'], cfg['atol'], cfg['maxiter']
+        o.check_every, o.precision, o.precond = cfg['check_every'], precision, 1 if cfg['precond'] else 0
+        o.reliable_delta, o.xcd_remap = cfg['reliable_delta'], 1 if cfg['xcd_remap'] else 0
+        key = ('pcg_ws', precision)
+        if key not in self.buffers:
+            nb = int(self.L.fem_pcg_ws_bytes(self.pref, precision))
+            self.buffers[key] = self.dm.empty(nb, dtype=torch.uint8)
+        ws = self.buffers[key]
+        op = self.operator(u) if precision == _lib.FEM_F32 else None
+        x = x0.clone()
+        res = _lib.FemPcgResult()
+        check(self.L.fem_pcg_solve(self.pref, ptr(vals), ptr(dinv), C.byref(op) if op is not None else None,
+                                   ptr(b), ptr(x), C.byref(o), ptr(ws), C.c_void_p(self.host_status.data_ptr()),
+                                   C.byref(res), stream_ptr()), 'fem_pcg_solve')
+        return x, res
+
+    def dot(self, x, y):
+        if 'dot_ws' not in self.buffers:
+            self.buffers['dot_ws'] = self.dm.empty(8192, dtype=torch.uint8)
+            self.buffers['dot_out'] = self.dm.empty(1)
+        out = self.buffers['dot_out']
+        check(self.L.fem_dot(x.numel(), ptr(x), ptr(y), ptr(out), ptr(self.buffers['dot_ws']), stream_ptr()),
+              'fem_dot')
+        return float(out.item())
+
+    def norm(self, x):
+        return self.dot(x, x) ** 0.5
+
+    def param_grad(self, u, lam):
+        """-lambda^T dR/dtheta per (cell, quad) for the material parameter (solver.py:1386-1416)."""
+        dm = self.dm
+        g = dm.empty(dm.C, dm.Q)
+        check(self.L.fem_cell_adjoint(dm.ref, self.mref, ptr(u), ptr(lam), ptr(g), stream_ptr()),
+              'fem_cell_adjoint')
+        return g
+
+    def load_adjoint(self, lam):
+        dm = self.dm
+        lq = dm.empty(dm.C, dm.Q, self.V)
+        check(self.L.fem_cell_load_adjoint(dm.ref, ptr(lam), ptr(lq), stream_ptr()), 'fem_cell_load_adjoint')
+        return lq
+
+    def face_load_adjoint(self, i, lam):
+        dm = self.dm
+        srf = self.surfaces[i]
+        S = srf['faces'].shape[0]
+        lq = dm.empty(S, dm.FQ, self.V)
+        check(self.L.fem_face_load_adjoint(dm.ref, S, ptr(srf['faces']), ptr(lam), ptr(lq), stream_ptr()),
+              'fem_face_load_adjoint')
+        return lq
+
+    # ------------------------------------------------------------- export
+    def export_csr(self, precision=None):
+        """Scalar CSR (indptr, indices, data) of the current matrix on the host."""
+        if precision is None:
+            prec, _, vals, _ = self.current
+        else:
+            prec, (_, vals, _) = precision, self.matrix_buffers(precision)
+        nnz = self.nnzb * self.V * self.V
+        indptr = self.dm.empty(self.n + 1, dtype=torch.int32)
+        indices = self.dm.empty(max(nnz, 1), dtype=torch.int32)
+        data = self.dm.empty(max(nnz, 1))
+        check(self.L.fem_export_csr(self.pref, prec, ptr(vals), ptr(indptr), ptr(indices), ptr(data), stream_ptr()),
+              'fem_export_csr')
+        return indptr.cpu().numpy(), indices[:nnz].cpu().numpy(), data[:nnz].cpu().numpy()
+
+    def cell_tangents(self, u, precision=_lib.FEM_F64):
+        """Per-cell tangents (C, N*V, N*V) in the reference's jac layout (problem.py:262-266, 453)."""
+        dm = self.dm
+        dt = torch.float32 if precision == _lib.FEM_F32 else torch.float64
+        Ke = dm.empty(dm.C * dm.N * dm.N * self.V * self.V, dtype=dt)
+        check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), stream_ptr()),
+              'fem_cell_tangent')
+        N, V = dm.N, self.V
+        return Ke.reshape(dm.C, N, N, V, V).permute(0, 1, 3, 2, 4).reshape(dm.C, N * V, N * V)

@@ -1,0 +1,141 @@
+"""ctypes binding of libfem_mi355x.so (declarations in include/fem_mi355x.h).
+
+The library is the only compute path: if it is missing or no GPU is visible,
+``lib()`` raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import torch
+
+LIB_NAME = 'libfem_mi355x.so'
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+FEM_OK, FEM_ERR_BAD_ARG, FEM_ERR_NOT_CONVERGED, FEM_ERR_NAN, FEM_ERR_HIP, FEM_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
+FEM_F32, FEM_F64 = 0, 1
+FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN = 1, 2, 3, 4
+
+_vp = C.c_void_p
+_i32, _i64, _f64 = C.c_int32, C.c_int64, C.c_double
+
+
+class FemMesh(C.Structure):
+    _fields_ = [('nodes_per_cell', _i32), ('num_quads', _i32), ('vec', _i32), ('num_face_quads', _i32),
+                ('num_cells', _i64), ('num_nodes', _i64),
+                ('points', _vp), ('cells', _vp), ('shape_vals', _vp), ('shape_grads_ref', _vp),
+                ('quad_weights', _vp), ('face_shape_vals', _vp), ('face_shape_grads_ref', _vp),
+                ('face_weights', _vp), ('face_normals', _vp)]
+
+
+class FemMaterial(C.Structure):
+    _fields_ = [('kind', _i32), ('has_param', _i32), ('c', _f64 * 6), ('param', _vp)]
+
+
+class FemPattern(C.Structure):
+    _fields_ = [('num_rows', _i64), ('nnzb', _i64), ('num_slices', _i64), ('sell_blocks', _i64),
+                ('vec', _i32), ('num_cells', _i32), ('nodes_per_cell', _i32), ('pad', _i32),
+                ('row_ptr', _vp), ('col_idx', _vp), ('diag_slot', _vp), ('contrib_ptr', _vp),
+                ('contrib_idx', _vp), ('slice_ptr', _vp), ('sell_col', _vp)]
+
+
+class FemOperator(C.Structure):
+    _fields_ = [('mesh', C.POINTER(FemMesh)), ('mat', C.POINTER(FemMaterial)), ('u', _vp),
+                ('inc_ptr', _vp), ('inc_idx', _vp), ('n_bc', _i64), ('bc_dofs', _vp),
+                ('cell_scratch', _vp)]
+
+
+class FemPcgOptions(C.Structure):
+    _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32),
+                ('precision', _i32), ('precond', _i32), ('reliable_delta', _f64),
+                ('xcd_remap', _i32), ('pad', _i32)]
+
+
+class FemPcgResult(C.Structure):
+    _fields_ = [('iterations', _i32), ('replacements', _i32), ('status', _i32), ('pad', _i32),
+                ('b_norm', _f64), ('res_norm', _f64)]
+
+
+_P = C.POINTER
+_SIGNATURES = {
+    'fem_last_error': (C.c_char_p, []),
+    'fem_version': (C.c_int, []),
+    'fem_incidence': (C.c_int, [_i64, _i32, _vp, _i64, _vp, _vp, _vp, _vp]),
+    'fem_scan_ws_bytes': (_i64, [_i64]),
+    'fem_exclusive_scan': (C.c_int, [_i64, _vp, _vp, _vp, _vp]),
+    'fem_pattern_rows': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_pattern_fill': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_sell_slice_sizes': (C.c_int, [_i64, _vp, _vp, _vp]),
+    'fem_sell_fill': (C.c_int, [_P(FemPattern), _vp, _vp]),
+    'fem_geometry': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp]),
+    'fem_cell_tangent': (C.c_int, [_P(FemMesh), _P(FemMaterial), _i32, _vp, _vp, _vp]),
+    'fem_cell_residual': (C.c_int, [_P(FemMesh), _P(FemMaterial), _vp, _vp, _vp]),
+    'fem_cell_jvp': (C.c_int, [_P(FemMesh), _P(FemMaterial), _vp, _vp, _vp, _vp]),
+    'fem_cell_load': (C.c_int, [_P(FemMesh), _vp, _vp, _vp]),
+    'fem_cell_load_adjoint': (C.c_int, [_P(FemMesh), _vp, _vp, _vp]),
+    'fem_cell_adjoint': (C.c_int, [_P(FemMesh), _P(FemMaterial), _vp, _vp, _vp, _vp]),
+    'fem_face_geometry': (C.c_int, [_P(FemMesh), _i64, _vp, _vp, _vp, _vp, _vp]),
+    'fem_face_load': (C.c_int, [_P(FemMesh), _i64, _vp, _vp, _vp, _vp]),
+    'fem_face_load_adjoint': (C.c_int, [_P(FemMesh), _i64, _vp, _vp, _vp, _vp]),
+    'fem_gather_nodal': (C.c_int, [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_gather_matrix': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp]),
+    'fem_matrix_bc_rows': (C.c_int, [_P(FemPattern), _i32, _vp, _i64, _vp, _vp]),
+    'fem_jacobi': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp]),
+    'fem_residual_bc': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _f64, _vp]),
+    'fem_spmv': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp, _vp]),
+    'fem_export_csr': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp, _vp, _vp]),
+    'fem_operator_apply': (C.c_int, [_P(FemOperator), _vp, _vp, _vp]),
+    'fem_pcg_ws_bytes': (_i64, [_P(FemPattern), _i32]),
+    'fem_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemOperator), _vp, _vp, _P(FemPcgOptions),
+                                _vp, _vp, _P(FemPcgResult), _vp]),
+    'fem_dot': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_LIB = None
+
+
+class FemError(RuntimeError):
+    pass
+
+
+def load(path=LIB_PATH):
+    """Load the shared library and attach signatures (no GPU needed)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(path):
+            raise FemError(f"{LIB_NAME} is not built ({path}); run __graft_entry__.build() "
+                           "or make -C jax-fem_amd/csrc")
+        torch.cuda  # the HIP runtime of torch is the one the library binds to
+        lib = C.CDLL(path)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def lib():
+    """The library, for compute calls: requires a visible GPU."""
+    if not torch.cuda.is_available():
+        raise FemError("no GPU visible: the device path needs an MI355X (there is no CPU fallback)")
+    return load()
+
+
+def check(rc, what=''):
+    if rc != FEM_OK:
+        msg = load().fem_last_error().decode(errors='replace')
+        raise FemError(f"{what} failed with status {rc}: {msg}")
+    return rc
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_ptr():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)

@@ -1,0 +1,333 @@
+"""Constitutive laws that run on the device, and recognition of user tensor maps.
+
+In jax-fem a ``Problem`` subclass returns an arbitrary JAX function from
+``get_tensor_map`` (problem.py:283-287) and the tangent comes from forward-mode
+AD of it (problem.py:262-266).  The device path has a fixed set of laws with
+closed-form tangents (csrc/materials.hpp).  ``get_tensor_map`` may return
+
+* one of the descriptors below (``Poisson``, ``LinearElastic``, ``SIMPElastic``,
+  ``NeoHookean``), or
+* a plain torch function ``f(u_grad, *internal_vars) -> P`` written like the
+  reference's JAX one; ``identify`` evaluates it on sample gradients (float64,
+  CPU) and matches it to one of the laws, recovering the constants.  A map
+  that matches none raises ``NotImplementedError``.
+
+Every descriptor is also callable with the reference signature, so
+post-processing code that calls the tensor map (e.g. a traction integral) keeps
+working on torch tensors.
+"""
+import math
+
+import torch
+
+from . import _lib
+
+
+class Material:
+    """Base descriptor: kind, constants (c0..c5), whether a per-quad parameter is read."""
+    kind = None
+    vec = None
+    has_param = False
+
+    def constants(self):
+        raise NotImplementedError
+
+    def to_struct(self, param_ptr=None):
+        m = _lib.FemMaterial()
+        m.kind = self.kind
+        m.has_param = 1 if self.has_param else 0
+        c = list(self.constants()) + [0.0] * 6
+        for i in range(6):
+            m.c[i] = float(c[i])
+        m.param = param_ptr
+        return m
+
+
+class Poisson(Material):
+    """P = k grad u (tests/benchmarks/linear_poisson/test_linear_poisson.py:15-17).
+
+    With ``param=True`` the conductivity is ``k * theta`` for the per-quad
+    internal variable theta.
+    """
+    kind = _lib.FEM_MAT_POISSON
+    vec = 1
+
+    def __init__(self, k=1.0, param=False):
+        self.k = float(k)
+        self.has_param = bool(param)
+
+    def constants(self):
+        return (self.k,)
+
+    def __call__(self, u_grad, *iv):
+        s = self.k * (_scalar(iv[0]) if self.has_param else 1.0)
+        return s * u_grad
+
+    def __repr__(self):
+        return f"Poisson(k={self.k}, param={self.has_param})"
+
+
+def lame(E, nu):
+    mu = E / (2. * (1. + nu))
+    lmbda = E * nu / ((1. + nu) * (1. - 2. * nu))
+    return lmbda, mu
+
+
+class LinearElastic(Material):
+    """sigma = lambda tr(eps) I + 2 mu eps (test_linear_elasticity_cube.py:16-25)."""
+    kind = _lib.FEM_MAT_LINEAR_ELASTIC
+    vec = 3
+
+    def __init__(self, E=None, nu=None, lmbda=None, mu=None):
+        if lmbda is None:
+            lmbda, mu = lame(float(E), float(nu))
+        self.lmbda, self.mu = float(lmbda), float(mu)
+
+    def constants(self):
+        return (self.lmbda, self.mu)
+
+    def __call__(self, u_grad, *iv):
+        eps = 0.5 * (u_grad + u_grad.transpose(-1, -2))
+        tr = eps.diagonal(dim1=-2, dim2=-1).sum(-1)
+        eye = torch.eye(3, dtype=u_grad.dtype, device=u_grad.device)
+        return self.lmbda * tr[..., None, None] * eye + 2. * self.mu * eps
+
+    def __repr__(self):
+        return f"LinearElastic(lmbda={self.lmbda}, mu={self.mu})"
+
+
+class SIMPElastic(Material):
+    """Isotropic elasticity with E(theta) = Emin + (E0 - Emin) theta^penal.
+
+    lambda = lmbda1 * E(theta), mu = mu1 * E(theta), where (lmbda1, mu1) are the
+    Lame constants for E = 1.  theta is the per-quad internal variable
+    (topology-optimisation notebook, cell 9).  ``SIMPElastic(E0=1, Emin=0,
+    penal=1)`` is elasticity with a per-quad Young's modulus field.
+    """
+    kind = _lib.FEM_MAT_SIMP
+    vec = 3
+    has_param = True
+
+    def __init__(self, E0=1.0, Emin=1e-3, nu=0.3, penal=3.0, lmbda1=None, mu1=None):
+        if lmbda1 is None:
+            lmbda1, mu1 = lame(1.0, float(nu))
+        self.lmbda1, self.mu1 = float(lmbda1), float(mu1)
+        self.E0, self.Emin, self.penal = float(E0), float(Emin), float(penal)
+
+    def constants(self):
+        return (self.lmbda1, self.mu1, self.Emin, self.E0, self.penal)
+
+    def modulus(self, theta):
+        return self.Emin + (self.E0 - self.Emin) * theta ** self.penal
+
+    def __call__(self, u_grad, *iv):
+        E = self.modulus(_scalar(iv[0]))
+        eps = 0.5 * (u_grad + u_grad.transpose(-1, -2))
+        tr = eps.diagonal(dim1=-2, dim2=-1).sum(-1)
+        eye = torch.eye(3, dtype=u_grad.dtype, device=u_grad.device)
+        return E * (self.lmbda1 * tr[..., None, None] * eye + 2. * self.mu1 * eps)
+
+    def __repr__(self):
+        return (f"SIMPElastic(E0={self.E0}, Emin={self.Emin}, penal={self.penal}, "
+                f"lmbda1={self.lmbda1}, mu1={self.mu1})")
+
+
+class NeoHookean(Material):
+    """psi = mu/2 (J^-2/3 I1 - 3) + kappa/2 (J - 1)^2, P = d psi/dF
+    (test_hyper_elasticity.py:16-34).  With ``param=True`` mu and kappa scale
+    with the per-quad internal variable (compute_gradients notebook, cell 11).
+    """
+    kind = _lib.FEM_MAT_NEO_HOOKEAN
+    vec = 3
+
+    def __init__(self, E=None, nu=None, mu=None, kappa=None, param=False):
+        if mu is None:
+            mu = E / (2. * (1. + nu))
+            kappa = E / (3. * (1. - 2. * nu))
+        self.mu, self.kappa = float(mu), float(kappa)
+        self.has_param = bool(param)
+
+    def constants(self):
+        return (self.mu, self.kappa)
+
+    def __call__(self, u_grad, *iv):
+        s = _scalar(iv[0]) if self.has_param else 1.0
+        F = u_grad + torch.eye(3, dtype=u_grad.dtype, device=u_grad.device)
+        J = torch.linalg.det(F)
+        H = torch.linalg.inv(F).transpose(-1, -2)
+        I1 = (F * F).sum((-1, -2))
+        coef = torch.as_tensor(s * self.mu * J ** (-2. / 3.), dtype=F.dtype)
+        c1 = coef * I1 / 3.
+        c2 = torch.as_tensor(s * self.kappa * J * (J - 1.), dtype=F.dtype)
+        return coef[..., None, None] * F + (c2 - c1)[..., None, None] * H
+
+    def __repr__(self):
+        return f"NeoHookean(mu={self.mu}, kappa={self.kappa}, param={self.has_param})"
+
+
+def _scalar(x):
+    x = torch.as_tensor(x)
+    return x.reshape(x.shape[:-1]) if x.dim() and x.shape[-1] == 1 else x
+
+
+# ----------------------------------------------------------------- probing
+_RTOL = 1e-9
+
+
+def _call(fn, G, ivs):
+    out = fn(G, *ivs)
+    return torch.as_tensor(out, dtype=torch.float64).reshape(G.shape)
+
+
+def _close(a, b, scale):
+    return float((a - b).abs().max()) <= _RTOL * max(scale, 1e-300)
+
+
+def _iv_sample(shape, value):
+    return torch.full(tuple(shape), float(value), dtype=torch.float64)
+
+
+def identify(tensor_map, vec, iv_shapes=()):
+    """Return a ``Material`` equivalent to ``tensor_map`` or raise NotImplementedError.
+
+    ``iv_shapes`` are the per-quad shapes of the internal variables passed after
+    ``u_grad`` (problem.py:208).  At most one internal variable with a single
+    value per quad point can drive the device laws.
+    """
+    if isinstance(tensor_map, Material):
+        if tensor_map.vec != vec:
+            raise ValueError(f"{tensor_map!r} is for vec={tensor_map.vec}, problem has vec={vec}")
+        return tensor_map
+    gen = torch.Generator().manual_seed(1234)
+
+    def rnd(scale=1.0):
+        return (torch.rand((vec, 3), generator=gen, dtype=torch.float64) - 0.5) * 2 * scale
+
+    n_iv = len(iv_shapes)
+    base = [_iv_sample(s, 0.7) for s in iv_shapes]
+    # Does the map depend on the internal variables at all?
+    G = rnd(1e-2)
+    P0 = _call(tensor_map, G, base)
+    depends = []
+    for k in range(n_iv):
+        alt = list(base)
+        alt[k] = _iv_sample(iv_shapes[k], 0.35)
+        depends.append(not _close(_call(tensor_map, G, alt), P0, float(P0.abs().max())))
+    driving = [k for k in range(n_iv) if depends[k]]
+    if len(driving) > 1:
+        raise NotImplementedError("tensor maps driven by more than one internal variable are not on the device path")
+    pk = driving[0] if driving else None
+    if pk is not None and math.prod(iv_shapes[pk]) != 1:
+        raise NotImplementedError("the internal variable of the tensor map must hold one value per quadrature point")
+
+    def at(theta):
+        ivs = list(base)
+        if pk is not None:
+            ivs[pk] = _iv_sample(iv_shapes[pk], theta)
+        return lambda Gm: _call(tensor_map, Gm, ivs)
+
+    f1 = at(1.0)
+    lin = _linear_constants(f1, vec, rnd)
+    if vec == 1:
+        if lin is None:
+            raise NotImplementedError("vec=1 tensor map is not linear isotropic (P = k grad u)")
+        k1 = lin
+        if pk is None:
+            return Poisson(k1)
+        for th in (0.3, 1.7):
+            kt = _linear_constants(at(th), 1, rnd)
+            if kt is None or abs(kt - k1 * th) > _RTOL * abs(k1) * max(1, th):
+                raise NotImplementedError("vec=1 tensor map must be k * theta * grad u in its internal variable")
+        return Poisson(k1, param=True)
+    if vec != 3:
+        raise NotImplementedError(f"vec={vec} is not on the device path")
+    if lin is not None:
+        lm1, mu1 = lin
+        if pk is None:
+            return LinearElastic(lmbda=lm1, mu=mu1)
+        # E(theta) = Emin + (E0 - Emin) theta^p, normalised so that E(1) = E0 = 1.
+        f0 = at(0.0)
+        if all(float(f0(rnd()).abs().max()) == 0.0 for _ in range(2)):
+            Emin = 0.0
+        else:
+            m0 = _linear_constants(f0, 3, rnd)
+            if m0 is None:
+                raise NotImplementedError("elastic tensor map is not linear at every parameter value")
+            Emin = m0[1] / mu1
+        mh = _linear_constants(at(0.5), 3, rnd)
+        if mh is None:
+            raise NotImplementedError("elastic tensor map is not linear at every parameter value")
+        ratio = (mh[1] / mu1 - Emin) / (1. - Emin)
+        if not (0 < ratio < 1):
+            raise NotImplementedError("parameter dependence of the elastic map is not of SIMP form")
+        penal = math.log(ratio) / math.log(0.5)
+        mat = SIMPElastic(E0=1.0, Emin=Emin, penal=penal, lmbda1=lm1, mu1=mu1)
+        for th in (0.23, 0.81, 1.37):
+            c = _linear_constants(at(th), 3, rnd)
+            E = mat.modulus(th)
+            if c is None or abs(c[0] - lm1 * E) > 1e-8 * abs(lm1 * E) or abs(c[1] - mu1 * E) > 1e-8 * abs(mu1 * E):
+                raise NotImplementedError("parameter dependence of the elastic map is not of SIMP form")
+        return mat
+    nh = _neo_hookean_constants(f1, rnd)
+    if nh is None:
+        raise NotImplementedError(
+            "tensor map not recognised: the device path supports Poisson, linear elastic, SIMP elastic and "
+            "Neo-Hookean laws (return a jax_fem_amd.materials descriptor to state one explicitly)")
+    mu, kappa = nh
+    if pk is None:
+        return NeoHookean(mu=mu, kappa=kappa)
+    for th in (0.3, 1.6):
+        c = _neo_hookean_constants(at(th), rnd)
+        if c is None or abs(c[0] - mu * th) > 1e-8 * abs(mu) * max(1, th) or \
+                abs(c[1] - kappa * th) > 1e-8 * abs(kappa) * max(1, th):
+            raise NotImplementedError("Neo-Hookean map must scale linearly with its internal variable")
+    return NeoHookean(mu=mu, kappa=kappa, param=True)
+
+
+def _linear_constants(f, vec, rnd):
+    """(k) for vec 1 or (lambda, mu) for vec 3 if f is isotropic linear, else None."""
+    G1, G2 = rnd(), rnd()
+    P1, P2 = f(G1), f(G2)
+    scale = float(max(P1.abs().max(), P2.abs().max()))
+    if scale == 0:
+        return None
+    if not _close(f(G1 + 2.0 * G2), P1 + 2.0 * P2, scale) or not _close(f(torch.zeros_like(G1)), 0 * P1, scale):
+        return None
+    if vec == 1:
+        k = float(f(torch.tensor([[1., 0., 0.]], dtype=torch.float64))[0, 0])
+        for G in (G1, G2):
+            if not _close(f(G), k * G, scale):
+                return None
+        return k
+    E01 = torch.zeros(3, 3, dtype=torch.float64)
+    E01[0, 1] = 1.
+    mu = float(f(E01)[0, 1])
+    E00 = torch.zeros(3, 3, dtype=torch.float64)
+    E00[0, 0] = 1.
+    lm = float(f(E00)[0, 0]) - 2. * mu
+    ref = LinearElastic(lmbda=lm, mu=mu)
+    for G in (G1, G2):
+        if not _close(f(G), ref(G), scale):
+            return None
+    return lm, mu
+
+
+def _neo_hookean_constants(f, rnd):
+    """Least-squares (mu, kappa) with P = mu A(F) + kappa B(F); None if it does not fit."""
+    rows, rhs = [], []
+    samples = [rnd(0.2) for _ in range(4)]
+    for G in samples:
+        F = G + torch.eye(3, dtype=torch.float64)
+        J = torch.linalg.det(F)
+        H = torch.linalg.inv(F).T
+        I1 = (F * F).sum()
+        A = J ** (-2. / 3.) * (F - I1 / 3. * H)
+        B = J * (J - 1.) * H
+        rows.append(torch.stack([A.reshape(-1), B.reshape(-1)], dim=1))
+        rhs.append(f(G).reshape(-1))
+    M = torch.cat(rows)
+    y = torch.cat(rhs)
+    sol = torch.linalg.lstsq(M, y[:, None]).solution[:, 0]
+    if not _close(M @ sol, y, float(y.abs().max())):
+        return None
+    return float(sol[0]), float(sol[1])

@@ -1,0 +1,346 @@
+"""GPU parity: every device stage against the FP64 oracle on the same inputs.
+
+Tolerances: FP64 paths 1e-12 relative (geometry, tangents, residuals, matrix
+values) and solver tolerance for solutions; FP32 tangents 2e-6 relative to the
+largest entry; bit-exact for the sparsity pattern.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from conftest import golden
+from oracle import basis as OB
+from oracle import fem as OF
+from problems import (LinearPoisson, LinearElasticityCube, LinearElasticityCylinder, HyperElasticity,
+                      SIMPElasticity, LoadedElastic, box, isclose)
+
+from jax_fem_amd import _lib
+from jax_fem_amd.generate_mesh import Mesh, box_mesh_hex27
+from jax_fem_amd.problem import Problem
+from jax_fem_amd.solver import solver, ad_wrapper
+from jax_fem_amd import materials as M
+
+pytestmark = pytest.mark.gpu
+
+
+def _rng(seed=0):
+    return np.random.default_rng(seed)
+
+
+def distorted_box(n=(3, 4, 5), seed=0, amp=0.15):
+    mesh = box(*n)
+    pts = mesh.points.copy()
+    h = 1.0 / max(n)
+    pts += amp * h * _rng(seed).uniform(-1, 1, pts.shape)
+    return Mesh(pts, mesh.cells)
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+class ElasticLE(Problem):
+    def get_tensor_map(self):
+        return M.LinearElastic(E=70e3, nu=0.3)
+
+
+class NH(Problem):
+    def get_tensor_map(self):
+        return M.NeoHookean(E=10., nu=0.3)
+
+
+def oracle_problem(p, mat, theta=None):
+    fe = p.fes[0]
+    return OF.Problem(fe.points, fe.cells, fe.vec, ele_type=fe.ele_type, quadrature_order=fe.quadrature_order,
+                      mat=mat, theta=theta)
+
+
+# ------------------------------------------------------------------ geometry
+@pytest.mark.parametrize('ele', ['HEX8', 'HEX27'])
+def test_geometry_vs_oracle(gpu, ele):
+    if ele == 'HEX8':
+        mesh, order = distorted_box(), None
+    else:
+        m = box_mesh_hex27(2, 3, 2, 1., 1.5, 1.)
+        pts = m.points + 0.03 * _rng(1).uniform(-1, 1, m.points.shape)
+        mesh, order = Mesh(pts, m.cells_dict['hexahedron27']), 4
+    p = ElasticLE(mesh, vec=3, dim=3, ele_type=ele, quadrature_order=order)
+    g, jxw = p.fes[0].get_shape_grads()
+    sv, sg, w = OB.shape_vals_and_grads(ele, order)
+    G, J = OF.shape_grads(mesh.points, mesh.cells, sg, w)
+    assert rel(g.cpu().numpy(), G) < 1e-12
+    assert rel(jxw.cpu().numpy(), J) < 1e-12
+    x = p.fes[0].get_physical_quad_points().cpu().numpy()
+    assert rel(x, OF.quad_points(mesh.points, mesh.cells, sv)) < 1e-13
+
+
+def test_face_geometry_cylinder_area(gpu):
+    d = golden('linear_elasticity_cylinder')
+    mesh = Mesh(d['points'], d['cells'])
+    top = lambda p: isclose(p[2], 10.)
+    p = LinearElasticityCylinder(mesh, vec=3, dim=3, dirichlet_bc_info=None, location_fns=[top])
+    bi = p.boundary_inds_list[0]
+    g, ns = p.fes[0].get_face_shape_grads(bi)
+    area = float(ns.sum())
+    np.testing.assert_almost_equal(area, float(d['surface_area']), decimal=4)  # test_linear_elasticity_cylinder.py:263
+    fv, fg, fw, fn, fi = OB.face_shape_vals_and_grads('HEX8')
+    G, NS = OF.face_shape_grads(mesh.points, mesh.cells, bi, fg, fw, fn)
+    assert rel(ns.cpu().numpy(), NS) < 1e-12 and rel(g.cpu().numpy(), G) < 1e-12
+
+
+# ------------------------------------------------------------------- pattern
+@pytest.mark.parametrize('case', ['box_v1', 'box_v3', 'cylinder_v3', 'hex27_v3'])
+def test_pattern_bit_exact(gpu, case):
+    if case == 'hex27_v3':
+        m = box_mesh_hex27(2, 2, 3, 1., 1., 1.)
+        mesh, ele, vec, order = Mesh(m.points, m.cells_dict['hexahedron27']), 'HEX27', 3, 4
+    elif case == 'cylinder_v3':
+        d = golden('linear_elasticity_cylinder')
+        mesh, ele, vec, order = Mesh(d['points'], d['cells']), 'HEX8', 3, None
+    else:
+        mesh, ele, vec, order = box(3, 4, 5), 'HEX8', (1 if case == 'box_v1' else 3), None
+    cls = LinearPoisson if vec == 1 else ElasticLE
+    p = cls(mesh, vec=vec, dim=3, ele_type=ele, quadrature_order=order)
+    dev = p.device()
+    dev.prepare()
+    u = torch.zeros(dev.n, dtype=torch.float64, device='cuda')
+    dev.tangent(u, _lib.FEM_F64)
+    indptr, indices, data = dev.export_csr()
+    n = dev.n
+    A = sp.coo_matrix((np.ones(len(p.I)), (p.I, p.J)), shape=(n, n)).tocsr()
+    A.sum_duplicates()
+    A.sort_indices()
+    assert np.array_equal(indptr, A.indptr) and np.array_equal(indices, A.indices)
+
+
+def test_pattern_nnz_closed_form(gpu):
+    # HEX8: prod(3 n_d + 1) V^2 ; HEX27: prod(8 n_d + 1) V^2
+    p = ElasticLE(box(4, 3, 5), vec=3, dim=3)
+    assert p.device().nnzb * 9 == 13 * 10 * 16 * 9
+    m = box_mesh_hex27(2, 3, 2, 1., 1., 1.)
+    p = ElasticLE(Mesh(m.points, m.cells_dict['hexahedron27']), vec=3, dim=3, ele_type='HEX27', quadrature_order=4)
+    assert p.device().nnzb * 9 == 17 * 25 * 17 * 9
+
+
+# ------------------------------------------------------------- cell tangents
+MATS = {
+    'poisson': (1, lambda: M.Poisson(2.5), lambda: OF.Material('poisson', k=2.5)),
+    'elastic': (3, lambda: M.LinearElastic(E=70e3, nu=0.3), lambda: OF.Material('elastic', **dict(zip(('lmbda', 'mu'), OF.lame(70e3, 0.3))))),
+    'simp': (3, lambda: M.SIMPElastic(E0=1., Emin=1e-3, nu=0.3, penal=3.),
+             lambda: OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., 0.3))), Emin=1e-3, E0=1., p=3.)),
+    'neo_hookean': (3, lambda: M.NeoHookean(E=10., nu=0.3),
+                    lambda: OF.Material('neo_hookean', mu=10. / 2.6, kappa=10. / (3 * 0.4))),
+}
+
+
+def make_material_problem(name, ele='HEX8', seed=3):
+    vec, dmat, omat = MATS[name]
+    if ele == 'HEX8':
+        mesh, order = distorted_box((3, 3, 4), seed=seed), None
+    else:
+        m = box_mesh_hex27(2, 2, 2, 1., 1., 1.)
+        mesh, order = Mesh(m.points + 0.02 * _rng(seed).uniform(-1, 1, m.points.shape), m.cells_dict['hexahedron27']), 4
+
+    class P(Problem):
+        def get_tensor_map(self):
+            return dmat()
+
+        def set_params(self, theta):
+            self.internal_vars = [theta]
+
+    p = P(mesh, vec=vec, dim=3, ele_type=ele, quadrature_order=order,
+          dirichlet_bc_info=[[lambda x: isclose(x[0], 0.)] * vec, list(range(vec)), [lambda x: 0.] * vec])
+    theta = None
+    if name == 'simp':
+        theta = _rng(seed).uniform(0.3, 1.0, (p.num_cells, p.fes[0].num_quads))
+        p.set_params(torch.as_tensor(theta))
+    u = 0.05 * _rng(seed + 1).standard_normal(p.fes[0].num_total_dofs)
+    return p, oracle_problem(p, omat(), theta), u
+
+
+@pytest.mark.parametrize('name', list(MATS))
+@pytest.mark.parametrize('ele', ['HEX8', 'HEX27'])
+def test_cell_tangent_and_residual_vs_oracle(gpu, name, ele):
+    p, o, u = make_material_problem(name, ele)
+    dev = p.device()
+    dev.prepare()
+    ut = torch.as_tensor(u, device='cuda')
+    K64 = dev.cell_tangents(ut, _lib.FEM_F64).cpu().numpy()
+    K32 = dev.cell_tangents(ut, _lib.FEM_F32).cpu().numpy()
+    U = u.reshape(-1, p.fes[0].vec)
+    Ko = OF.cell_tangent(o.mat, U, o.cells, o.G, o.JxW, o.theta)
+    assert rel(K64, Ko) < 1e-12
+    assert rel(K32, Ko) < 2e-6
+    r = p.compute_residual([ut.reshape(-1, p.fes[0].vec)])[0].cpu().numpy()
+    ro = OF.scatter(OF.cell_residual(o.mat, U, o.cells, o.G, o.JxW, o.theta), o.cells, len(o.points))
+    assert rel(r, ro) < 1e-12
+    # matrix values (with Dirichlet rows) against scipy COO->CSR + zeroRows of the oracle
+    dev.tangent(ut, _lib.FEM_F64)
+    indptr, indices, data = dev.export_csr()
+    o.bc_dofs = dev.bc_dofs.cpu().numpy().astype(np.int64)
+    A = o.tangent(u)
+    assert np.array_equal(indptr, A.indptr) and np.array_equal(indices, A.indices)
+    assert rel(data, A.data) < 1e-12
+    # matrix-free FP64 operator == assembled FP64 matrix
+    x = torch.as_tensor(_rng(9).standard_normal(dev.n), device='cuda')
+    y = dev.apply_operator(ut, x).cpu().numpy()
+    assert rel(y, A @ x.cpu().numpy()) < 1e-12
+
+
+def test_assembly_deterministic(gpu):
+    p, o, u = make_material_problem('neo_hookean')
+    dev = p.device()
+    dev.prepare()
+    ut = torch.as_tensor(u, device='cuda')
+    outs = []
+    for _ in range(2):
+        dev.tangent(ut, _lib.FEM_F32)
+        outs.append(dev.export_csr()[2].copy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+# --------------------------------------------------------- end-to-end fixtures
+def _fixture_poisson(precision):
+    d = golden('linear_poisson')
+    mesh = Mesh(d['points'], d['cells'])
+    bc = [[lambda p: isclose(p[0], 0.), lambda p: isclose(p[0], 1.)], [0, 0], [lambda p: 0., lambda p: 1.]]
+    p = LinearPoisson(mesh, vec=1, dim=3, dirichlet_bc_info=bc)
+    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    return sol, d['sol'], 5
+
+
+def _fixture_cube(precision):
+    d = golden('linear_elasticity_cube')
+    mesh = Mesh(d['points'], d['cells'])
+    left = lambda p: isclose(p[0], 0.)
+    bc = [[left] * 3, [0, 1, 2], [lambda p: 1.] * 3]
+    p = LinearElasticityCube(mesh, vec=3, dim=3, dirichlet_bc_info=bc, location_fns=[lambda p: isclose(p[0], 1.)])
+    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    return sol, d['sol'], 5
+
+
+def _fixture_cylinder(precision):
+    d = golden('linear_elasticity_cylinder')
+    mesh = Mesh(d['points'], d['cells'])
+    bottom = lambda p: isclose(p[2], 0.)
+    bc = [[bottom] * 3, [0, 1, 2], [lambda p: 0.] * 3]
+    p = LinearElasticityCylinder(mesh, vec=3, dim=3, dirichlet_bc_info=bc, location_fns=[lambda p: isclose(p[2], 10.)])
+    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    return sol, d['sol'], 3
+
+
+def _fixture_hyper(precision):
+    d = golden('hyperelasticity')
+    mesh = Mesh(d['points'], d['cells'])
+    top, bottom = (lambda p: isclose(p[2], 10.)), (lambda p: isclose(p[2], 0.))
+    bc = [[bottom] * 3 + [top] * 3, [0, 1, 2, 0, 1, 2], [lambda p: 0.] * 5 + [lambda p: 1.]]
+    p = HyperElasticity(mesh, vec=3, dim=3, dirichlet_bc_info=bc)
+    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    # traction on the top face (test_hyper_elasticity.py:36-70, 121-127)
+    bi = p.fes[0].get_boundary_conditions_inds([top])[0]
+    g, ns = p.fes[0].get_face_shape_grads(bi)
+    cells = torch.as_tensor(p.fes[0].cells, device='cuda')
+    ug = torch.einsum('sav,sqaj->sqvj', sol[cells[torch.as_tensor(bi[:, 0], device='cuda')]], g)
+    P = torch.func.vmap(p.get_tensor_map())(ug.reshape(-1, 3, 3)).reshape(ug.shape)
+    trac = (P @ torch.tensor([0., 0., 1.], dtype=torch.float64, device='cuda'))
+    tz = float((trac * ns[..., None]).sum((0, 1))[2])
+    np.testing.assert_almost_equal(tz, float(d['traction']), decimal=5)
+    return sol, d['sol'], 5
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+@pytest.mark.parametrize('case', ['poisson', 'cube', 'cylinder', 'hyper'])
+def test_fenicsx_fixtures(gpu, case, precision):
+    fn = {'poisson': _fixture_poisson, 'cube': _fixture_cube, 'cylinder': _fixture_cylinder,
+          'hyper': _fixture_hyper}[case]
+    sol, ref, decimal = fn(precision)
+    got = sol.cpu().numpy().reshape(ref.shape).astype(np.float32)  # save_sol writes float32 (utils.py:45)
+    np.testing.assert_array_almost_equal(ref, got, decimal=decimal)
+
+
+# -------------------------------------------------------------------- solvers
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_solution_vs_oracle_direct(gpu, precision):
+    mesh = box(12, 6, 6, 2., 1., 1.)
+    left = lambda p: isclose(p[0], 0.)
+    p = LoadedElastic(mesh, vec=3, dim=3, dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.] * 3],
+                      location_fns=[lambda p: isclose(p[0], 2.)])
+    sol = solver(p, {'jax_solver': {'precision': precision, 'tol': 1e-12, 'atol': 1e-14}})[0].cpu().numpy()
+    o = oracle_problem(p, OF.Material('elastic', **dict(zip(('lmbda', 'mu'), OF.lame(70e3, 0.3)))))
+    o.bc_dofs, o.bc_vals = p.device().bc_dofs.cpu().numpy().astype(np.int64), p.device().bc_vals.cpu().numpy()
+    bi = p.boundary_inds_list[0]
+    o.add_surface(bi, np.broadcast_to(np.array([0., 0., 1.]), (len(bi), o.fw.shape[1], 3)))
+    ref = o.newton(linear='direct').reshape(-1, 3)
+    err = np.linalg.norm(sol - ref) / np.linalg.norm(ref)
+    assert err < 1e-8, err
+
+
+def test_pcg_bit_reproducible(gpu):
+    outs = [_fixture_cube('mixed')[0].cpu().numpy() for _ in range(2)]
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_pcg_not_converged_raises(gpu):
+    d = golden('linear_elasticity_cube')
+    mesh = Mesh(d['points'], d['cells'])
+    left = lambda p: isclose(p[0], 0.)
+    p = LinearElasticityCube(mesh, vec=3, dim=3, dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 1.] * 3],
+                             location_fns=[lambda p: isclose(p[0], 1.)])
+    with pytest.raises(AssertionError, match="failed to converge"):
+        solver(p, {'jax_solver': {'maxiter': 1}})
+
+
+# -------------------------------------------------------------------- adjoint
+def simp_setup(n=(8, 4, 4), seed=0):
+    mesh = box(*n, 2., 1., 1.)
+    left = lambda p: isclose(p[0], 0.)
+    load = lambda p: isclose(p[0], 2.) & (p[2] <= 0.25 + 1e-5)
+    p = SIMPElasticity(mesh, vec=3, dim=3, dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.] * 3],
+                       location_fns=[load])
+    theta = torch.as_tensor(0.5 + _rng(seed).uniform(-0.05, 0.05, p.num_cells), device='cuda')
+    return p, theta
+
+
+def compliance(p, sol):
+    # J = F . u with the (constant) load vector: load is -R_surface, so F.u = -load.u
+    dev = p.device()
+    return -torch.dot(dev.load, sol.reshape(-1))
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_adjoint_simp_vs_oracle_and_fd(gpu, precision):
+    p, theta = simp_setup()
+    opts = {'jax_solver': {'precision': precision, 'tol': 1e-12, 'atol': 1e-14}}
+    fwd = ad_wrapper(p, opts, opts)
+    th = theta.clone().requires_grad_(True)
+    J = compliance(p, fwd(th)[0])
+    J.backward()
+    g = th.grad.cpu().numpy()
+    # oracle: same problem, adjoint by sparse direct solves
+    o = oracle_problem(p, OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., 0.3))), Emin=1e-3, E0=1., p=3.),
+                       theta=np.repeat(theta.cpu().numpy()[:, None], 8, axis=1))
+    dev = p.device()
+    o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
+    o.load = dev.load.cpu().numpy().reshape(-1, 3)
+    u = o.newton(linear='direct')
+    gq, lam = o.adjoint_theta(u, -o.load.reshape(-1))
+    go = gq.sum(1)
+    assert rel(g, go) < 1e-7
+    # compliance is self-adjoint: lambda = u on free dofs
+    free = np.setdiff1d(np.arange(o.n), o.bc_dofs)
+    assert rel(lam[free], u[free]) < 1e-9
+    # closed form dJ/dtheta_e = -3 theta^2 (E0 - Emin) u_e^T K1_e u_e (< 0)
+    assert np.all(g < 0)
+    # central differences on seeded cells
+    h = 1e-6
+    cells = _rng(1).choice(p.num_cells, 4, replace=False)
+    for c in cells:
+        tp, tm = theta.clone(), theta.clone()
+        tp[c] += h
+        tm[c] -= h
+        Jp = float(compliance(p, fwd(tp)[0]))
+        Jm = float(compliance(p, fwd(tm)[0]))
+        fd = (Jp - Jm) / (2 * h)
+        assert abs(fd - g[c]) <= 1e-4 * abs(g).max(), (c, fd, g[c])
