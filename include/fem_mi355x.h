@@ -51,6 +51,12 @@ enum {
 /* ---- enums ----------------------------------------------------------- */
 enum { FEM_F32 = 0, FEM_F64 = 1 };           /* arithmetic precision          */
 enum { FEM_HEX8 = 8, FEM_HEX27 = 27 };       /* = nodes per cell              */
+enum {                                       /* storage form of the matrix    */
+  FEM_FORM_STANDARD = 0,      /* block (i, j) holds A_ij                      */
+  FEM_FORM_DIFFERENCE = 1     /* diagonal slot holds R_i = sum_j A_ij and the
+                                 product is y_i = sum_{j!=i} A_ij (x_j - x_i)
+                                 + R_i x_i (exact on translations)            */
+};
 enum {                                       /* constitutive laws (tensor_map)*/
   FEM_MAT_POISSON = 1,        /* P = c0 * grad u                      (vec 1) */
   FEM_MAT_LINEAR_ELASTIC = 2, /* sigma = c0 tr(eps) I + 2 c1 eps      (vec 3) */
@@ -131,7 +137,8 @@ typedef struct {
   double  reliable_delta;        /* FP32 mode: replace the residual when the
                                     recursive |r| fell by this factor         */
   int32_t xcd_remap;             /* 1: XCD-contiguous block order in SpMV     */
-  int32_t pad;
+  int32_t timing;                /* 1: time the two iteration kernels with HIP
+                                    events (adds a record per launch)         */
 } fem_pcg_options_t;
 
 typedef struct {
@@ -141,6 +148,11 @@ typedef struct {
   int32_t pad;
   double  b_norm;                /* |b|                                       */
   double  res_norm;              /* final true residual norm |b - A x|        */
+  double  spmv_ms;               /* timing=1: mean duration of the fused SpMV
+                                    kernel (p update, A p, <p, A p>)          */
+  double  update_ms;             /* timing=1: mean duration of the update kernel */
+  int32_t timed_launches;        /* launches averaged into spmv_ms            */
+  int32_t pad2;
 } fem_pcg_result_t;
 
 /* ---- library --------------------------------------------------------- */
@@ -213,22 +225,27 @@ int fem_face_load_adjoint(const fem_mesh_t* mesh, int64_t S, const int32_t* face
  * summed in incidence order (problem.py:426-437 scatter-add, deterministic). */
 int fem_gather_nodal(int64_t num_nodes, int32_t vec, const int32_t* inc_ptr, const int32_t* inc_idx,
                      const double* ent, const double* base, double* out, void* stream);
-/* SELL values = sum of the cell blocks of Ke (duplicates summed in cell order). */
-int fem_gather_matrix(const fem_pattern_t* pat, int32_t precision, const void* Ke, void* vals, void* stream);
-/* Dirichlet row elimination (PETSc zeroRows, diag = 1): row d zeroed, A[d,d] = 1. */
+/* SELL values (in `precision`) = sum of the cell blocks of Ke (in `ke_precision`, equal
+ * to `precision` or FP64), duplicates summed in cell order then rounded once
+ * (PETSc setValuesCOO, solver.py:522-526), stored in `form`; dinv (may be
+ * NULL) receives 1/A[d,d] in `precision`. */
+int fem_gather_matrix(const fem_pattern_t* pat, int32_t ke_precision, const void* Ke, int32_t precision,
+                      int32_t form, void* vals, void* dinv, void* stream);
+/* Dirichlet row elimination (PETSc zeroRows, diag = 1, solver.py:527-528): row d zeroed,
+ * A[d,d] = 1, dinv[d] = 1 (dinv may be NULL).  Valid for both forms. */
 int fem_matrix_bc_rows(const fem_pattern_t* pat, int32_t precision, void* vals, int64_t n_bc,
-                       const int32_t* bc_dofs, void* stream);
+                       const int32_t* bc_dofs, void* dinv, void* stream);
 /* dinv[d] = 1/A[d,d] (0 where the diagonal is absent), in `precision`. */
 int fem_jacobi(const fem_pattern_t* pat, int32_t precision, const void* vals, void* dinv, void* stream);
 /* res[d] = u[d] - scale*g[d] on Dirichlet dofs (solver.py:290-304). */
 int fem_residual_bc(int64_t n_bc, const int32_t* bc_dofs, const double* bc_vals, const double* u,
                     double* res, double scale, void* stream);
-/* y = A x in `precision` (x, y of that precision). */
-int fem_spmv(const fem_pattern_t* pat, int32_t precision, const void* vals, const void* x, void* y,
-             void* stream);
+/* y = A x in `precision` (x, y of that precision), matrix stored in `form`. */
+int fem_spmv(const fem_pattern_t* pat, int32_t precision, int32_t form, const void* vals, const void* x,
+             void* y, void* stream);
 /* Scalar CSR export: indptr (vec*rows+1), indices (nnz), data (nnz, FP64). */
-int fem_export_csr(const fem_pattern_t* pat, int32_t precision, const void* vals, int32_t* indptr,
-                   int32_t* indices, double* data, void* stream);
+int fem_export_csr(const fem_pattern_t* pat, int32_t precision, int32_t form, const void* vals,
+                   int32_t* indptr, int32_t* indices, double* data, void* stream);
 /* y = dR/du(u) x (FP64, matrix-free, BC rows = identity). */
 int fem_operator_apply(const fem_operator_t* op, const double* x, double* y, void* stream);
 
@@ -236,16 +253,17 @@ int fem_operator_apply(const fem_operator_t* op, const double* x, double* y, voi
 /* Workspace bytes for fem_pcg_solve. */
 int64_t fem_pcg_ws_bytes(const fem_pattern_t* pat, int32_t precision);
 /* Jacobi-PCG on the row-eliminated tangent.  b, x are FP64 (x holds x0 on
- * entry).  vals/dinv are in opt->precision.  `op` is required for FEM_F32
+ * entry).  vals/dinv are in opt->precision; an FP64 matrix is in standard
+ * form, an FP32 matrix in difference form.  `op` is required for FEM_F32
  * and ignored for FEM_F64.  `host_status` is pinned host memory of at least
- * 64 bytes.  Synchronises the stream.  Returns FEM_OK or the solve status. */
+ * 128 bytes.  Synchronises the stream.  Returns FEM_OK or the solve status. */
 int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv,
                   const fem_operator_t* op, const double* b, double* x,
                   const fem_pcg_options_t* opt, void* ws, void* host_status,
                   fem_pcg_result_t* result /* host */, void* stream);
 
 /* ---- vector helpers -------------------------------------------------- */
-/* out = sum_i x_i y_i (FP64), deterministic; ws >= 8 * 1024 bytes. */
+/* out = sum_i x_i y_i (FP64), deterministic; ws >= 8 * 512 bytes. */
 int fem_dot(int64_t n, const double* x, const double* y, double* out, void* ws, void* stream);
 
 #ifdef __cplusplus

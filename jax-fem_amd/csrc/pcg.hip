@@ -7,88 +7,152 @@
 // exact boundary increment (solver.py:402-409), so the residual and search
 // direction vanish on those rows and CG runs on the SPD free block.
 //
-// Two kernels per iteration, both ending in a deterministic grid reduction
-// whose last block computes the CG scalars on the device:
-//   A: p = z + beta p_old (ping-pong buffers), q = A p, <p, q> -> alpha
-//   B: x += alpha p, r -= alpha q, z = D^-1 r, <r, z>, <r, r> -> beta, done
-// Precision FEM_F64: everything FP64.  FEM_F32: matrix, r, z, p, q in FP32,
-// x accumulated in FP64, and the recursive residual is replaced by the FP64
-// true residual b - A x (A applied matrix-free in FP64) whenever it has
-// dropped by `reliable_delta` and at convergence ("reliable updates").
+// Two kernels per iteration k:
+//   A_k: reduce the <r,z>, <r,r> partials of B_{k-1} (or of a restart),
+//        convergence test, beta, p = z + beta p_old (ping-pong buffers),
+//        q = A p, partials of <p, q>
+//   B_k: reduce the <p,q> partials of A_k, alpha = rz / pq,
+//        x += alpha p, r -= alpha q, z = D^-1 r, partials of <r,z>, <r,r>
+// Every block reduces the previous kernel's per-block partials itself, in the
+// same fixed order, so all blocks agree bit for bit and no grid-wide fence or
+// atomic is needed; scalars written by block 0 are read by the next kernel
+// (parity-indexed slots where a kernel reads the previous iteration's value).
+//
+// Precision FEM_F64: everything FP64, standard matrix form.
+// FEM_F32: FP32 matrix in difference form, FP32 r, z, p, q; x accumulated in
+// FP64; the recursive residual is replaced by the FP64 true residual b - A x
+// (A applied matrix-free in FP64) each time it has dropped by reliable_delta
+// and at convergence; beta keeps the value of the recursion across a
+// replacement.
 #include "fem_common.hpp"
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 int fem_operator_apply_impl(const fem_operator_t* op, const double* x, double* y, hipStream_t s);
 
 namespace fem {
 
 struct PcgScalars {
-  double alpha, beta, rz, pq, rr, rr_true;
-  int32_t done, iter, nan, pad;
+  double rzs[2];         // rz used for alpha at iteration k, slot k & 1
+  double rr;             // <r,r> at the start of the last iteration A ran
+  double rr_true;        // |b - A x|^2 of the last true-residual evaluation
+  double pad0[4];
+  int32_t done, iter, nan, restart_iter;
+  int32_t pad1[4];
 };
+static_assert(sizeof(PcgScalars) <= 128, "host status buffer");
 
-template <int V, typename T>
-__global__ __launch_bounds__(256) void k_pcg_spmv(fem_pattern_t pat, const T* __restrict__ vals,
-                                                  const T* __restrict__ z, const T* __restrict__ p_old,
-                                                  T* __restrict__ p_new, T* __restrict__ q, PcgScalars* sc,
-                                                  double* __restrict__ partials, unsigned* __restrict__ counter,
-                                                  int remap) {
+// Sum of `n` partials starting at p, identical in every block (blockDim == 256).
+__device__ inline double block_reduce_partials(const double* __restrict__ p, int n, double* smem) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) acc += p[i];
+  return block_sum_256(acc, smem);
+}
+
+// y_i = sum_k V_k x_{c_k}        (standard form)
+// y_i = sum_k V_k (c_k == i ? x_i : x_{c_k} - x_i)   (difference form: the
+//       diagonal slot holds the row-sum block R_i = sum_j A_ij, so the product
+//       is exact on translations whatever the rounding of the stored values)
+// with x = z + beta p_old gathered on the fly; xr returns x at the row.
+template <int V, bool DIFF, typename T>
+__device__ inline void sell_row(const fem_pattern_t& pat, const T* __restrict__ vals, int64_t row,
+                                const T* __restrict__ z, const T* __restrict__ p_old, T beta, T (&acc)[V],
+                                T (&xr)[V]) {
   constexpr int VV = V * V;
+  const int64_t s = row >> 6;
+  const int lane = (int)(row & 63);
+  const int base = pat.slice_ptr[s];
+  const int L = (pat.slice_ptr[s + 1] - base) >> 6;
+  const int32_t* cp = pat.sell_col + base + lane;
+  const T* vp = vals + (size_t)base * VV + lane;
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    xr[j] = z[row * V + j] + beta * p_old[row * V + j];
+    acc[j] = T(0);
+  }
+#pragma unroll 2
+  for (int k = 0; k < L; ++k) {
+    const int64_t c = cp[(size_t)k * kSlice];
+    T xc[V];
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      const T v = z[c * V + j] + beta * p_old[c * V + j];
+      xc[j] = DIFF ? (c == row ? xr[j] : v - xr[j]) : v;
+    }
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int j = 0; j < V; ++j) acc[i] += vp[((size_t)k * VV + i * V + j) * kSlice] * xc[j];
+  }
+}
+
+template <int V, bool DIFF, typename T>
+__global__ __launch_bounds__(256) void k_pcg_a(fem_pattern_t pat, const T* __restrict__ vals,
+                                               const T* __restrict__ z, const T* __restrict__ p_old,
+                                               T* __restrict__ p_new, T* __restrict__ q, PcgScalars* sc,
+                                               const double* __restrict__ partB, int nB,
+                                               const double* __restrict__ partR, int nR,
+                                               double* __restrict__ partA, int k, double tol2, int maxiter,
+                                               int remap) {
   __shared__ double smem[8];
   if (sc->done) return;
-  const T beta = T(sc->beta);
+  const bool restarted = sc->restart_iter == k;
+  const double* src = restarted ? partR : partB;
+  const int nsrc = restarted ? nR : nB;
+  const double rz = block_reduce_partials(src, nsrc, smem);
+  const double rr = block_reduce_partials(src + nsrc, nsrc, smem + 4);
+  const bool bad = !isfinite(rr) || !isfinite(rz);
+  if (bad || rr <= tol2 || k >= maxiter) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = 1;
+      sc->iter = k;
+      sc->rr = rr;
+      if (bad) sc->nan = 1;
+    }
+    return;
+  }
+  // beta: 0 at the start; after a residual replacement the recursion's value
+  // (from B_{k-1}'s partials); otherwise rz_k / rz_{k-1}
+  double beta_d = 0.0;
+  if (k > 0) {
+    const double rz_rec = restarted ? block_reduce_partials(partB, nB, smem) : rz;
+    beta_d = rz_rec / sc->rzs[(k - 1) & 1];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rzs[k & 1] = rz;
+    sc->rr = rr;
+  }
+  const T beta = T(beta_d);
   const int nb = gridDim.x;
   const int blk = remap ? xcd_logical_block(blockIdx.x, nb) : blockIdx.x;
   const int64_t row = (int64_t)blk * 256 + threadIdx.x;
   T dot = T(0);
   if (row < pat.num_rows) {
-    const int64_t s = row >> 6;
-    const int lane = (int)(row & 63);
-    const int base = pat.slice_ptr[s];
-    const int L = (pat.slice_ptr[s + 1] - base) >> 6;
-    const int32_t* cp = pat.sell_col + base + lane;
-    const T* vp = vals + (size_t)base * VV + lane;
-    T acc[V];
-#pragma unroll
-    for (int i = 0; i < V; ++i) acc[i] = T(0);
-#pragma unroll 2
-    for (int k = 0; k < L; ++k) {
-      const int64_t c = cp[(size_t)k * kSlice];
-      T xc[V];
-#pragma unroll
-      for (int j = 0; j < V; ++j) xc[j] = z[c * V + j] + beta * p_old[c * V + j];
-#pragma unroll
-      for (int i = 0; i < V; ++i)
-#pragma unroll
-        for (int j = 0; j < V; ++j) acc[i] += vp[((size_t)k * VV + i * V + j) * kSlice] * xc[j];
-    }
+    T acc[V], pr[V];
+    sell_row<V, DIFF, T>(pat, vals, row, z, p_old, beta, acc, pr);
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      const T pn = z[row * V + i] + beta * p_old[row * V + i];
-      p_new[row * V + i] = pn;
+      p_new[row * V + i] = pr[i];
       q[row * V + i] = acc[i];
-      dot += pn * acc[i];
+      dot += pr[i] * acc[i];
     }
   }
-  double d = block_sum_256(double(dot), smem);
-  double mine[1] = {d}, tot[1];
-  if (last_block_sums<1>(mine, partials, counter, tot, smem) && threadIdx.x == 0) {
-    sc->pq = tot[0];
-    sc->alpha = sc->rz / tot[0];
-  }
+  const double d = block_sum_256(double(dot), smem);
+  if (threadIdx.x == 0) partA[blockIdx.x] = d;
 }
 
 template <typename T, typename TX>
-__global__ __launch_bounds__(256) void k_pcg_update(int64_t n, TX* __restrict__ x, T* __restrict__ r,
-                                                    T* __restrict__ z, const T* __restrict__ p,
-                                                    const T* __restrict__ q, const T* __restrict__ dinv,
-                                                    PcgScalars* sc, double* __restrict__ partials,
-                                                    unsigned* __restrict__ counter, double tol2, int maxiter) {
+__global__ __launch_bounds__(256) void k_pcg_b(int64_t n, TX* __restrict__ x, T* __restrict__ r,
+                                               T* __restrict__ z, const T* __restrict__ p,
+                                               const T* __restrict__ q, const T* __restrict__ dinv,
+                                               PcgScalars* sc, const double* __restrict__ partA, int nA,
+                                               double* __restrict__ partB, int k) {
   __shared__ double smem[8];
   if (sc->done) return;
-  const double alpha_d = sc->alpha;
+  const double pq = block_reduce_partials(partA, nA, smem);
+  const double alpha_d = sc->rzs[k & 1] / pq;
   const T alpha = T(alpha_d);
   T rz = T(0), rr = T(0);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -100,42 +164,41 @@ __global__ __launch_bounds__(256) void k_pcg_update(int64_t n, TX* __restrict__ 
     rz += ri * zi;
     rr += ri * ri;
   }
-  double mine[2] = {block_sum_256(double(rz), smem), block_sum_256(double(rr), smem + 4)};
-  double tot[2];
-  if (last_block_sums<2>(mine, partials, counter, tot, smem) && threadIdx.x == 0) {
-    const double rz_new = tot[0], rr_new = tot[1];
-    sc->beta = rz_new / sc->rz;
-    sc->rz = rz_new;
-    sc->rr = rr_new;
-    sc->iter += 1;
-    const bool bad = !isfinite(rr_new) || !isfinite(rz_new);
-    if (bad) sc->nan = 1;
-    if (bad || rr_new <= tol2 || sc->iter >= maxiter) sc->done = 1;
+  const double s1 = block_sum_256(double(rz), smem);
+  const double s2 = block_sum_256(double(rr), smem + 4);
+  if (threadIdx.x == 0) {
+    partB[blockIdx.x] = s1;
+    partB[gridDim.x + blockIdx.x] = s2;
+    if (blockIdx.x == 0) sc->iter = k + 1;
   }
 }
 
-// r = (T)(b - y) and <b - y, b - y> in FP64 -> rr_true.
+// r = (T)(b - y) (y = NULL: r = b); per-block partials of |b - y|^2.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_resid(int64_t n, const double* __restrict__ b, const double* __restrict__ y,
-                                                   T* __restrict__ r, PcgScalars* sc, double* __restrict__ partials,
-                                                   unsigned* __restrict__ counter) {
+                                                   T* __restrict__ r, double* __restrict__ part) {
   __shared__ double smem[8];
   double rr = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double ri = b[i] - y[i];
+    const double ri = b[i] - (y ? y[i] : 0.0);
     r[i] = T(ri);
     rr += ri * ri;
   }
-  double mine[1] = {block_sum_256(rr, smem)}, tot[1];
-  if (last_block_sums<1>(mine, partials, counter, tot, smem) && threadIdx.x == 0) sc->rr_true = tot[0];
+  const double s = block_sum_256(rr, smem);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// z = D^-1 r, rz, rr; init (beta = 0) or residual replacement (beta = rz_new / rz_old).
+__global__ __launch_bounds__(256) void k_pcg_finish_rr(const double* __restrict__ part, int n, PcgScalars* sc) {
+  __shared__ double smem[8];
+  const double s = block_reduce_partials(part, n, smem);
+  if (threadIdx.x == 0) sc->rr_true = s;
+}
+
+// z = D^-1 r and the partials of <r,z>, <r,r> for iteration k (start or residual replacement).
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_restart(int64_t n, const T* __restrict__ r, T* __restrict__ z,
                                                      const T* __restrict__ dinv, PcgScalars* sc,
-                                                     double* __restrict__ partials, unsigned* __restrict__ counter,
-                                                     int init) {
+                                                     double* __restrict__ partR, int k) {
   __shared__ double smem[8];
   T rz = T(0), rr = T(0);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -144,16 +207,15 @@ __global__ __launch_bounds__(256) void k_pcg_restart(int64_t n, const T* __restr
     rz += r[i] * zi;
     rr += r[i] * r[i];
   }
-  double mine[2] = {block_sum_256(double(rz), smem), block_sum_256(double(rr), smem + 4)};
-  double tot[2];
-  if (last_block_sums<2>(mine, partials, counter, tot, smem) && threadIdx.x == 0) {
-    sc->beta = init ? 0.0 : tot[0] / sc->rz;
-    sc->rz = tot[0];
-    sc->rr = tot[1];
-    sc->done = 0;
-    if (init) {
-      sc->iter = 0;
-      sc->nan = 0;
+  const double s1 = block_sum_256(double(rz), smem);
+  const double s2 = block_sum_256(double(rr), smem + 4);
+  if (threadIdx.x == 0) {
+    partR[blockIdx.x] = s1;
+    partR[gridDim.x + blockIdx.x] = s2;
+    if (blockIdx.x == 0) {
+      sc->restart_iter = k;
+      sc->done = 0;
+      sc->iter = k;
     }
   }
 }
@@ -164,12 +226,7 @@ __global__ void k_fill(int64_t n, T* __restrict__ v, T val) {
     v[i] = val;
 }
 
-template <typename TS, typename TD>
-__global__ void k_cast(int64_t n, const TS* __restrict__ a, TD* __restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = TD(a[i]);
-}
-
+// y = A x for FP64 x, y (standard form values of precision T).
 template <int V, typename T>
 __global__ __launch_bounds__(256) void k_spmv_x(fem_pattern_t pat, const T* __restrict__ vals,
                                                 const double* __restrict__ x, double* __restrict__ y) {
@@ -196,7 +253,20 @@ __global__ __launch_bounds__(256) void k_spmv_x(fem_pattern_t pat, const T* __re
   for (int i = 0; i < V; ++i) y[row * V + i] = acc[i];
 }
 
-constexpr int kUpdGridCap = 2048;
+constexpr int kVecGrid = 1024;  // blocks of the vector kernels (partials per reduction)
+
+struct EventPool {
+  std::vector<hipEvent_t> e;
+  bool ok = true;
+  explicit EventPool(int n) : e(n, nullptr) {
+    for (auto& x : e)
+      if (hipEventCreate(&x) != hipSuccess) ok = false;
+  }
+  ~EventPool() {
+    for (auto& x : e)
+      if (x) (void)hipEventDestroy(x);
+  }
+};
 
 struct Ws {
   void* r;
@@ -206,8 +276,10 @@ struct Ws {
   void* q;
   void* ones;
   double* y;
-  double* partials;
-  unsigned* counters;
+  double* partA;
+  double* partB;
+  double* partR;
+  double* partT;
   PcgScalars* sc;
 };
 
@@ -218,7 +290,6 @@ static int64_t grid_a(const fem_pattern_t* pat) { return ceil_div(pat->num_rows,
 static int64_t ws_layout(const fem_pattern_t* pat, int precision, void* base, Ws* w) {
   const int64_t n = pat->num_rows * pat->vec;
   const int64_t es = precision == FEM_F32 ? 4 : 8;
-  const int64_t nparts = grid_a(pat) + 2 * kUpdGridCap + 64;  // kernel A, then kernel B's two sums
   int64_t off = 0;
   auto take = [&](int64_t bytes) {
     char* p = base ? reinterpret_cast<char*>(base) + off : nullptr;
@@ -234,8 +305,10 @@ static int64_t ws_layout(const fem_pattern_t* pat, int precision, void* base, Ws
   o->q = take(n * es);
   o->ones = take(n * es);
   o->y = reinterpret_cast<double*>(take(n * 8));
-  o->partials = reinterpret_cast<double*>(take(nparts * 8));
-  o->counters = reinterpret_cast<unsigned*>(take(4 * sizeof(unsigned)));
+  o->partA = reinterpret_cast<double*>(take(grid_a(pat) * 8));
+  o->partB = reinterpret_cast<double*>(take(2 * kVecGrid * 8));
+  o->partR = reinterpret_cast<double*>(take(2 * kVecGrid * 8));
+  o->partT = reinterpret_cast<double*>(take(kVecGrid * 8));
   o->sc = reinterpret_cast<PcgScalars*>(take(sizeof(PcgScalars)));
   return off;
 }
@@ -244,9 +317,11 @@ template <int V, typename T>
 static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, const fem_operator_t* op,
                    const double* b, double* x, const fem_pcg_options_t* opt, void* ws_base,
                    PcgScalars* host_sc, fem_pcg_result_t* res, hipStream_t s) {
+  constexpr bool kMixed = std::is_same<T, float>::value;
+  constexpr bool kDiff = kMixed;  // the FP32 matrix is stored in difference form
   const int64_t n = pat->num_rows * V;
   Ws w;
-  ws_layout(pat, std::is_same<T, float>::value ? FEM_F32 : FEM_F64, ws_base, &w);
+  ws_layout(pat, kMixed ? FEM_F32 : FEM_F64, ws_base, &w);
   T* r = (T*)w.r;
   T* z = (T*)w.z;
   T* pbuf[2] = {(T*)w.p0, (T*)w.p1};
@@ -257,51 +332,46 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
     FEM_LAUNCHED();
     dinv = (T*)w.ones;
   }
-  const bool mixed = std::is_same<T, float>::value;
   const int gA = (int)grid_a(pat);
-  const int gB = (int)std::min<int64_t>(ceil_div(n, 256), kUpdGridCap);
-  unsigned* cnt = w.counters;
-  FEM_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned), s));
+  const int gV = (int)std::min<int64_t>(ceil_div(n, 256), kVecGrid);
   FEM_HIP(hipMemsetAsync(w.sc, 0, sizeof(PcgScalars), s));
   hipLaunchKernelGGL(k_fill<T>, dim3(256), dim3(256), 0, s, n, pbuf[0], T(0));
   FEM_LAUNCHED();
 
-  // |b|^2 (reuse the residual kernel with y = 0)
-  hipLaunchKernelGGL(k_fill<double>, dim3(256), dim3(256), 0, s, n, w.y, 0.0);
-  FEM_LAUNCHED();
-  hipLaunchKernelGGL(k_pcg_resid<T>, dim3(gB), dim3(256), 0, s, n, b, (const double*)w.y, r, w.sc, w.partials, cnt + 2);
-  FEM_LAUNCHED();
-  FEM_HIP(hipMemcpyAsync(host_sc, w.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-  FEM_HIP(hipStreamSynchronize(s));
-  const double bb = host_sc->rr_true;
-  const double tol2 = std::max(opt->rtol * opt->rtol * bb, opt->atol * opt->atol);
-  res->b_norm = std::sqrt(bb);
-
+  auto read_status = [&]() -> int {
+    FEM_HIP(hipMemcpyAsync(host_sc, w.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    FEM_HIP(hipStreamSynchronize(s));
+    return FEM_OK;
+  };
+  // r = b - y (y = NULL: r = b), |r|^2 -> rr_true
+  auto residual_from = [&](const double* y) -> int {
+    hipLaunchKernelGGL(k_pcg_resid<T>, dim3(gV), dim3(256), 0, s, n, b, y, r, w.partT);
+    hipLaunchKernelGGL(k_pcg_finish_rr, dim3(1), dim3(256), 0, s, (const double*)w.partT, gV, w.sc);
+    FEM_LAUNCHED();
+    return read_status();
+  };
   auto true_residual = [&]() -> int {
-    if (mixed) {
-      int rc = fem_operator_apply_impl(op, x, w.y, s);
-      if (rc) return rc;
+    if (kMixed) {
+      if (int rc = fem_operator_apply_impl(op, x, w.y, s)) return rc;
     } else {
       hipLaunchKernelGGL((k_spmv_x<V, T>), dim3(gA), dim3(256), 0, s, *pat, vals, (const double*)x, w.y);
       FEM_LAUNCHED();
     }
-    hipLaunchKernelGGL(k_pcg_resid<T>, dim3(gB), dim3(256), 0, s, n, b, (const double*)w.y, r, w.sc, w.partials,
-                       cnt + 2);
-    FEM_LAUNCHED();
-    return FEM_OK;
+    return residual_from(w.y);
   };
 
-  if (int rc = true_residual()) return rc;
-  hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gB), dim3(256), 0, s, n, (const T*)r, z, dinv, w.sc, w.partials,
-                     cnt + 3, 1);
-  FEM_LAUNCHED();
-  FEM_HIP(hipMemcpyAsync(host_sc, w.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-  FEM_HIP(hipStreamSynchronize(s));
-  double rr_true = host_sc->rr_true;
-  double rr_ref = rr_true;
+  if (int rc = residual_from(nullptr)) return rc;  // |b|^2
+  const double bb = host_sc->rr_true;
+  const double tol2 = std::max(opt->rtol * opt->rtol * bb, opt->atol * opt->atol);
+  res->b_norm = std::sqrt(bb);
   res->iterations = 0;
   res->replacements = 0;
   res->status = FEM_OK;
+  res->timed_launches = 0;
+  res->spmv_ms = res->update_ms = 0.0;
+
+  if (int rc = true_residual()) return rc;
+  double rr_true = host_sc->rr_true;
   if (!std::isfinite(rr_true)) {
     res->status = FEM_ERR_NAN;
     res->res_norm = rr_true;
@@ -311,64 +381,87 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
     res->res_norm = std::sqrt(rr_true);
     return FEM_OK;
   }
+  hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gV), dim3(256), 0, s, n, (const T*)r, z, dinv, w.sc, w.partR, 0);
+  FEM_LAUNCHED();
+  double rr_ref = rr_true;
   const int check = std::max(1, opt->check_every);
-  const double delta2 = mixed ? opt->reliable_delta * opt->reliable_delta : 0.0;
+  const double delta2 = kMixed ? opt->reliable_delta * opt->reliable_delta : 0.0;
+  EventPool ev(opt->timing ? 3 * check : 0);
+  if (!ev.ok) {
+    set_error("fem_pcg_solve: hipEventCreate failed");
+    return FEM_ERR_HIP;
+  }
+  double spmv_ms = 0.0, upd_ms = 0.0;
+  int timed = 0;
   int iter = 0;
   for (;;) {
+    const int iter0 = iter;
     for (int m = 0; m < check; ++m) {
-      const int cur = (iter + m) & 1;
-      hipLaunchKernelGGL((k_pcg_spmv<V, T>), dim3(gA), dim3(256), 0, s, *pat, vals, (const T*)z,
-                         (const T*)pbuf[cur], pbuf[cur ^ 1], q, w.sc, w.partials, cnt + 0, opt->xcd_remap);
-      hipLaunchKernelGGL((k_pcg_update<T, double>), dim3(gB), dim3(256), 0, s, n, x, r, z, (const T*)pbuf[cur ^ 1],
-                         (const T*)q, dinv, w.sc, w.partials + gA, cnt + 1, tol2, opt->maxiter);
+      const int k = iter0 + m;
+      const int cur = k & 1;
+      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m], s));
+      hipLaunchKernelGGL((k_pcg_a<V, kDiff, T>), dim3(gA), dim3(256), 0, s, *pat, vals, (const T*)z,
+                         (const T*)pbuf[cur], pbuf[cur ^ 1], q, w.sc, (const double*)w.partB, gV,
+                         (const double*)w.partR, gV, w.partA, k, tol2, opt->maxiter, opt->xcd_remap);
+      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m + 1], s));
+      hipLaunchKernelGGL((k_pcg_b<T, double>), dim3(gV), dim3(256), 0, s, n, x, r, z, (const T*)pbuf[cur ^ 1],
+                         (const T*)q, dinv, w.sc, (const double*)w.partA, gA, w.partB, k);
+      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m + 2], s));
     }
     FEM_LAUNCHED();
-    FEM_HIP(hipMemcpyAsync(host_sc, w.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    FEM_HIP(hipStreamSynchronize(s));
+    if (int rc = read_status()) return rc;
     iter = host_sc->iter;
+    if (opt->timing) {
+      // only the launches that ran an iteration (the rest returned on `done`)
+      for (int m = 0; m < std::min(check, iter - iter0); ++m) {
+        float ta = 0.f, tb = 0.f;
+        FEM_HIP(hipEventElapsedTime(&ta, ev.e[3 * m], ev.e[3 * m + 1]));
+        FEM_HIP(hipEventElapsedTime(&tb, ev.e[3 * m + 1], ev.e[3 * m + 2]));
+        spmv_ms += ta;
+        upd_ms += tb;
+        ++timed;
+      }
+    }
     if (host_sc->nan) {
       res->status = FEM_ERR_NAN;
       break;
     }
     const bool conv_rec = host_sc->done && host_sc->rr <= tol2;
-    const bool trigger = mixed && host_sc->rr < delta2 * rr_ref;
+    const bool trigger = kMixed && !host_sc->done && host_sc->rr < delta2 * rr_ref;
     if (conv_rec || trigger) {
       if (int rc = true_residual()) return rc;
-      FEM_HIP(hipMemcpyAsync(host_sc, w.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-      FEM_HIP(hipStreamSynchronize(s));
       rr_true = host_sc->rr_true;
       if (rr_true <= tol2) break;
       if (!std::isfinite(rr_true)) {
         res->status = FEM_ERR_NAN;
         break;
       }
-      // residual replacement: keep the search direction, restart the recurrences from b - A x
-      hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gB), dim3(256), 0, s, n, (const T*)r, z, dinv, w.sc, w.partials,
-                         cnt + 3, 0);
+      if (iter >= opt->maxiter) {
+        res->status = FEM_ERR_NOT_CONVERGED;
+        break;
+      }
+      // residual replacement at iteration `iter`: keep the search direction
+      hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gV), dim3(256), 0, s, n, (const T*)r, z, dinv, w.sc, w.partR,
+                         iter);
       FEM_LAUNCHED();
       rr_ref = rr_true;
       res->replacements += 1;
+      continue;
     }
-    if (iter >= opt->maxiter) {
+    if (host_sc->done || iter >= opt->maxiter) {  // maxiter reached
       res->status = FEM_ERR_NOT_CONVERGED;
       break;
     }
   }
-  // final true residual
-  if (res->status == FEM_OK && !(rr_true <= tol2)) {
+  if (!(rr_true <= tol2) || res->status != FEM_OK) {
     if (int rc = true_residual()) return rc;
-    FEM_HIP(hipMemcpyAsync(host_sc, w.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    FEM_HIP(hipStreamSynchronize(s));
-    rr_true = host_sc->rr_true;
-  }
-  if (res->status != FEM_OK && res->status != FEM_ERR_NAN) {
-    if (int rc = true_residual()) return rc;
-    FEM_HIP(hipMemcpyAsync(host_sc, w.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    FEM_HIP(hipStreamSynchronize(s));
     rr_true = host_sc->rr_true;
   }
   res->iterations = iter;
   res->res_norm = std::sqrt(rr_true);
+  res->timed_launches = timed;
+  res->spmv_ms = timed ? spmv_ms / timed : 0.0;
+  res->update_ms = timed ? upd_ms / timed : 0.0;
   return FEM_OK;
 }
 

@@ -31,10 +31,13 @@ __global__ void k_gather_nodal(int64_t num_nodes, int V, const int32_t* __restri
 
 // ------------------------------------------------------- matrix values
 // One thread per row: for every SELL slot k, sum the cell blocks that land in
-// block (row, col_k) and store them (zeros in padding slots).
-template <int V, typename T>
-__global__ __launch_bounds__(256) void k_gather_matrix(fem_pattern_t pat, const T* __restrict__ Ke,
-                                                       T* __restrict__ vals) {
+// block (row, col_k) and store them (zeros in padding slots).  DIFF stores
+// the row-sum block R_row = sum_k A(row, k) (accumulated in TS) in the
+// diagonal slot instead of the diagonal block (difference form of the SpMV).
+// dinv (may be NULL) receives 1 / A(row*V+c, row*V+c).
+template <int V, bool DIFF, typename TS, typename T>
+__global__ __launch_bounds__(256) void k_gather_matrix(fem_pattern_t pat, const TS* __restrict__ Ke,
+                                                       T* __restrict__ vals, T* __restrict__ dinv) {
   constexpr int VV = V * V;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= pat.num_slices * kSlice) return;
@@ -45,27 +48,53 @@ __global__ __launch_bounds__(256) void k_gather_matrix(fem_pattern_t pat, const 
   const bool real = t < pat.num_rows;
   const int rb = real ? pat.row_ptr[t] : 0;
   const int len = real ? pat.row_ptr[t + 1] - rb : 0;
+  const int ds = real ? pat.diag_slot[t] : -1;
   T* vp = vals + (size_t)base * VV + lane;
-  for (int k = 0; k < L; ++k) {
-    T acc[VV];
+  TS rsum[VV];
+  TS diag[V];
 #pragma unroll
-    for (int v = 0; v < VV; ++v) acc[v] = T(0);
+  for (int v = 0; v < VV; ++v) rsum[v] = TS(0);
+#pragma unroll
+  for (int c = 0; c < V; ++c) diag[c] = TS(0);
+  for (int k = 0; k < L; ++k) {
+    TS acc[VV];
+#pragma unroll
+    for (int v = 0; v < VV; ++v) acc[v] = TS(0);
     if (k < len) {
       const int c0 = pat.contrib_ptr[rb + k], c1 = pat.contrib_ptr[rb + k + 1];
       for (int ci = c0; ci < c1; ++ci) {
-        const T* src = Ke + (size_t)pat.contrib_idx[ci] * VV;
+        const TS* src = Ke + (size_t)pat.contrib_idx[ci] * VV;
 #pragma unroll
         for (int v = 0; v < VV; ++v) acc[v] += src[v];
       }
     }
 #pragma unroll
-    for (int v = 0; v < VV; ++v) vp[((size_t)k * VV + v) * kSlice] = acc[v];
+    for (int v = 0; v < VV; ++v) rsum[v] += acc[v];
+    if (k == ds) {
+#pragma unroll
+      for (int c = 0; c < V; ++c) diag[c] = acc[c * V + c];
+    }
+    if (!(DIFF && k == ds)) {
+#pragma unroll
+      for (int v = 0; v < VV; ++v) vp[((size_t)k * VV + v) * kSlice] = T(acc[v]);
+    }
+  }
+  if (DIFF && ds >= 0) {
+#pragma unroll
+    for (int v = 0; v < VV; ++v) vp[((size_t)ds * VV + v) * kSlice] = T(rsum[v]);
+  }
+  if (dinv && real) {
+#pragma unroll
+    for (int c = 0; c < V; ++c) dinv[t * V + c] = diag[c] != TS(0) ? T(TS(1) / diag[c]) : T(0);
   }
 }
 
 // Zero row d of the matrix and put 1 on its diagonal (PETSc zeroRows, diag = 1).
+// The row of an identity row sums to e_d, so the same values serve both
+// forms; dinv[d] = 1.
 template <int V, typename T>
-__global__ void k_bc_rows(fem_pattern_t pat, T* __restrict__ vals, int64_t n_bc, const int32_t* __restrict__ dofs) {
+__global__ void k_bc_rows(fem_pattern_t pat, T* __restrict__ vals, int64_t n_bc, const int32_t* __restrict__ dofs,
+                          T* __restrict__ dinv) {
   constexpr int VV = V * V;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_bc; i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t d = dofs[i];
@@ -81,6 +110,7 @@ __global__ void k_bc_rows(fem_pattern_t pat, T* __restrict__ vals, int64_t n_bc,
       for (int cj = 0; cj < V; ++cj) vp[((size_t)k * VV + ci * V + cj) * kSlice] = T(0);
     const int ds = pat.diag_slot[row];
     if (ds >= 0) vp[((size_t)ds * VV + ci * V + ci) * kSlice] = T(1);
+    if (dinv) dinv[d] = T(1);
   }
 }
 
@@ -111,7 +141,7 @@ __global__ void k_residual_bc(int64_t n_bc, const int32_t* __restrict__ dofs, co
 }
 
 // ------------------------------------------------------------ SpMV
-template <int V, typename T>
+template <int V, bool DIFF, typename T>
 __global__ __launch_bounds__(256) void k_spmv(fem_pattern_t pat, const T* __restrict__ vals, const T* __restrict__ x,
                                               T* __restrict__ y) {
   constexpr int VV = V * V;
@@ -126,11 +156,14 @@ __global__ __launch_bounds__(256) void k_spmv(fem_pattern_t pat, const T* __rest
   T acc[V];
 #pragma unroll
   for (int i = 0; i < V; ++i) acc[i] = T(0);
+  T xr[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) xr[j] = x[row * V + j];
   for (int k = 0; k < L; ++k) {
     const int64_t c = cp[(size_t)k * kSlice];
     T xc[V];
 #pragma unroll
-    for (int j = 0; j < V; ++j) xc[j] = x[c * V + j];
+    for (int j = 0; j < V; ++j) xc[j] = DIFF ? (c == row ? xr[j] : x[c * V + j] - xr[j]) : x[c * V + j];
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -141,15 +174,18 @@ __global__ __launch_bounds__(256) void k_spmv(fem_pattern_t pat, const T* __rest
 }
 
 // Scalar CSR of the block matrix: row r = V*node + ci holds the blocks of node
-// row `node` in column order, each contributing V consecutive columns.
+// row `node` in column order, each contributing V consecutive columns.  In
+// difference form the diagonal block is recovered as R - sum of the others
+// (in FP64).
 template <int V, typename T>
-__global__ void k_export_csr(fem_pattern_t pat, const T* __restrict__ vals, int32_t* __restrict__ indptr,
+__global__ void k_export_csr(fem_pattern_t pat, const T* __restrict__ vals, int form, int32_t* __restrict__ indptr,
                              int32_t* __restrict__ indices, double* __restrict__ data) {
   constexpr int VV = V * V;
   for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < pat.num_rows;
        row += (int64_t)gridDim.x * blockDim.x) {
     const int rb = pat.row_ptr[row];
     const int len = pat.row_ptr[row + 1] - rb;
+    const int ds = pat.diag_slot[row];
     const int64_t s = row >> 6;
     const int lane = (int)(row & 63);
     const T* vp = vals + (size_t)pat.slice_ptr[s] * VV + lane;
@@ -160,7 +196,13 @@ __global__ void k_export_csr(fem_pattern_t pat, const T* __restrict__ vals, int3
         for (int cj = 0; cj < V; ++cj) {
           const int64_t pos = start + (int64_t)k * V + cj;
           if (indices) indices[pos] = pat.col_idx[rb + k] * V + cj;
-          if (data && vals) data[pos] = double(vp[((size_t)k * VV + ci * V + cj) * kSlice]);
+          if (data && vals) {
+            double v = double(vp[((size_t)k * VV + ci * V + cj) * kSlice]);
+            if (form == FEM_FORM_DIFFERENCE && k == ds)
+              for (int kk = 0; kk < len; ++kk)
+                if (kk != ds) v -= double(vp[((size_t)kk * VV + ci * V + cj) * kSlice]);
+            data[pos] = v;
+          }
         }
     }
     if (indptr && row == pat.num_rows - 1) indptr[pat.num_rows * V] = (int32_t)((int64_t)VV * pat.row_ptr[pat.num_rows]);
@@ -177,15 +219,21 @@ __global__ void k_copy_bc(int64_t n_bc, const int32_t* __restrict__ dofs, const 
 }
 
 __global__ __launch_bounds__(256) void k_dot(int64_t n, const double* __restrict__ x, const double* __restrict__ y,
-                                             double* __restrict__ partials, unsigned* __restrict__ counter,
-                                             double* __restrict__ out) {
+                                             double* __restrict__ partials) {
   __shared__ double smem[8];
   double s = 0;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     s += x[i] * y[i];
   s = block_sum_256(s, smem);
-  double mine[1] = {s}, tot[1];
-  if (last_block_sums<1>(mine, partials, counter, tot, smem) && threadIdx.x == 0) *out = tot[0];
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_dot_finish(const double* __restrict__ partials, int n, double* __restrict__ out) {
+  __shared__ double smem[8];
+  double s = 0;
+  for (int i = threadIdx.x; i < n; i += 256) s += partials[i];
+  s = block_sum_256(s, smem);
+  if (threadIdx.x == 0) *out = s;
 }
 
 static int grid_for(int64_t n, int threads, int cap = 65536) {
@@ -243,22 +291,42 @@ int fem_gather_nodal(int64_t num_nodes, int32_t vec, const int32_t* inc_ptr, con
   return FEM_OK;
 }
 
-int fem_gather_matrix(const fem_pattern_t* pat, int32_t precision, const void* Ke, void* vals, void* stream) {
+int fem_gather_matrix(const fem_pattern_t* pat, int32_t ke_precision, const void* Ke, int32_t precision, int32_t form,
+                      void* vals, void* dinv, void* stream) {
   if (int rc = check_pat(pat)) return rc;
-  FEM_CHECK_ARG(Ke && vals && pat->contrib_ptr && pat->contrib_idx, "fem_gather_matrix: NULL argument");
+  FEM_CHECK_ARG(Ke && vals && pat->contrib_ptr && pat->contrib_idx && pat->diag_slot,
+                "fem_gather_matrix: NULL argument");
+  FEM_CHECK_ARG(ke_precision == precision || (ke_precision == FEM_F64 && precision == FEM_F32),
+                "fem_gather_matrix: Ke precision must equal the matrix precision or be FP64");
+  FEM_CHECK_ARG(form == FEM_FORM_STANDARD || form == FEM_FORM_DIFFERENCE, "fem_gather_matrix: bad form");
+  const int64_t threads = pat->num_slices * kSlice;
+  const dim3 grid((unsigned)ceil_div(threads, 256));
   return with_vt(pat->vec, precision, [&](auto vc, auto tv) -> int {
     constexpr int V = decltype(vc)::value;
     using T = decltype(tv);
-    const int64_t threads = pat->num_slices * kSlice;
-    hipLaunchKernelGGL((k_gather_matrix<V, T>), dim3((unsigned)ceil_div(threads, 256)), dim3(256), 0,
-                       as_stream(stream), *pat, (const T*)Ke, (T*)vals);
+    hipStream_t s = as_stream(stream);
+    if (ke_precision == FEM_F64) {
+      if (form == FEM_FORM_DIFFERENCE)
+        hipLaunchKernelGGL((k_gather_matrix<V, true, double, T>), grid, dim3(256), 0, s, *pat, (const double*)Ke,
+                           (T*)vals, (T*)dinv);
+      else
+        hipLaunchKernelGGL((k_gather_matrix<V, false, double, T>), grid, dim3(256), 0, s, *pat, (const double*)Ke,
+                           (T*)vals, (T*)dinv);
+    } else {
+      if (form == FEM_FORM_DIFFERENCE)
+        hipLaunchKernelGGL((k_gather_matrix<V, true, T, T>), grid, dim3(256), 0, s, *pat, (const T*)Ke, (T*)vals,
+                           (T*)dinv);
+      else
+        hipLaunchKernelGGL((k_gather_matrix<V, false, T, T>), grid, dim3(256), 0, s, *pat, (const T*)Ke, (T*)vals,
+                           (T*)dinv);
+    }
     FEM_LAUNCHED();
     return FEM_OK;
   });
 }
 
 int fem_matrix_bc_rows(const fem_pattern_t* pat, int32_t precision, void* vals, int64_t n_bc, const int32_t* bc_dofs,
-                       void* stream) {
+                       void* dinv, void* stream) {
   if (int rc = check_pat(pat)) return rc;
   FEM_CHECK_ARG(vals && (n_bc == 0 || bc_dofs) && pat->diag_slot, "fem_matrix_bc_rows: NULL argument");
   if (n_bc == 0) return FEM_OK;
@@ -266,7 +334,7 @@ int fem_matrix_bc_rows(const fem_pattern_t* pat, int32_t precision, void* vals, 
     constexpr int V = decltype(vc)::value;
     using T = decltype(tv);
     hipLaunchKernelGGL((k_bc_rows<V, T>), dim3(grid_for(n_bc, 256)), dim3(256), 0, as_stream(stream), *pat, (T*)vals,
-                       n_bc, bc_dofs);
+                       n_bc, bc_dofs, (T*)dinv);
     FEM_LAUNCHED();
     return FEM_OK;
   });
@@ -295,29 +363,36 @@ int fem_residual_bc(int64_t n_bc, const int32_t* bc_dofs, const double* bc_vals,
   return FEM_OK;
 }
 
-int fem_spmv(const fem_pattern_t* pat, int32_t precision, const void* vals, const void* x, void* y, void* stream) {
+int fem_spmv(const fem_pattern_t* pat, int32_t precision, int32_t form, const void* vals, const void* x, void* y,
+             void* stream) {
   if (int rc = check_pat(pat)) return rc;
   FEM_CHECK_ARG(vals && x && y && pat->sell_col, "fem_spmv: NULL argument");
+  FEM_CHECK_ARG(form == FEM_FORM_STANDARD || form == FEM_FORM_DIFFERENCE, "fem_spmv: bad form");
   return with_vt(pat->vec, precision, [&](auto vc, auto tv) -> int {
     constexpr int V = decltype(vc)::value;
     using T = decltype(tv);
-    hipLaunchKernelGGL((k_spmv<V, T>), dim3((unsigned)ceil_div(pat->num_rows, 256)), dim3(256), 0, as_stream(stream),
-                       *pat, (const T*)vals, (const T*)x, (T*)y);
+    const dim3 grid((unsigned)ceil_div(pat->num_rows, 256));
+    if (form == FEM_FORM_DIFFERENCE)
+      hipLaunchKernelGGL((k_spmv<V, true, T>), grid, dim3(256), 0, as_stream(stream), *pat, (const T*)vals,
+                         (const T*)x, (T*)y);
+    else
+      hipLaunchKernelGGL((k_spmv<V, false, T>), grid, dim3(256), 0, as_stream(stream), *pat, (const T*)vals,
+                         (const T*)x, (T*)y);
     FEM_LAUNCHED();
     return FEM_OK;
   });
 }
 
-int fem_export_csr(const fem_pattern_t* pat, int32_t precision, const void* vals, int32_t* indptr, int32_t* indices,
-                   double* data, void* stream) {
+int fem_export_csr(const fem_pattern_t* pat, int32_t precision, int32_t form, const void* vals, int32_t* indptr,
+                   int32_t* indices, double* data, void* stream) {
   if (int rc = check_pat(pat)) return rc;
-  FEM_CHECK_ARG(pat->col_idx, "fem_export_csr: pattern without col_idx");
+  FEM_CHECK_ARG(pat->col_idx && pat->diag_slot, "fem_export_csr: pattern without col_idx");
   FEM_CHECK_ARG((int64_t)pat->vec * pat->vec * pat->nnzb < (int64_t)INT32_MAX, "fem_export_csr: nnz exceeds int32");
   return with_vt(pat->vec, precision, [&](auto vc, auto tv) -> int {
     constexpr int V = decltype(vc)::value;
     using T = decltype(tv);
     hipLaunchKernelGGL((k_export_csr<V, T>), dim3(grid_for(pat->num_rows, 256)), dim3(256), 0, as_stream(stream),
-                       *pat, (const T*)vals, indptr, indices, data);
+                       *pat, (const T*)vals, form, indptr, indices, data);
     FEM_LAUNCHED();
     return FEM_OK;
   });
@@ -331,10 +406,9 @@ int fem_dot(int64_t n, const double* x, const double* y, double* out, void* ws, 
   FEM_CHECK_ARG(n >= 0 && x && y && out && ws, "fem_dot: bad arguments");
   const int grid = grid_for(n, 256, 512);
   double* partials = reinterpret_cast<double*>(ws);
-  unsigned* counter = reinterpret_cast<unsigned*>(partials + 512);
   hipStream_t s = as_stream(stream);
-  FEM_HIP(hipMemsetAsync(counter, 0, sizeof(unsigned), s));
-  hipLaunchKernelGGL(k_dot, dim3(grid), dim3(256), 0, s, n, x, y, partials, counter, out);
+  hipLaunchKernelGGL(k_dot, dim3(grid), dim3(256), 0, s, n, x, y, partials);
+  hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(256), 0, s, (const double*)partials, grid, out);
   FEM_LAUNCHED();
   return FEM_OK;
 }

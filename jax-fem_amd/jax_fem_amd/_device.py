@@ -180,7 +180,7 @@ class DeviceProblem:
         self.material = None
         self.param = None
         self.load = None
-        self.host_status = torch.empty(64, dtype=torch.uint8).pin_memory()
+        self.host_status = torch.empty(128, dtype=torch.uint8).pin_memory()
 
     # ------------------------------------------------------------ pattern
     def _build_pattern(self):
@@ -361,17 +361,26 @@ class DeviceProblem:
                                      float(scale), stream_ptr()), 'fem_residual_bc')
         return res
 
-    def tangent(self, u, precision):
-        """Assemble the row-eliminated tangent in `precision` (problem.py:447-460, solver.py:522-529)."""
+    def tangent(self, u, precision, ke_precision=None):
+        """Assemble the row-eliminated tangent in `precision` (problem.py:447-460, solver.py:522-529).
+
+        ke_precision: arithmetic of the per-cell tangents (default: `precision`); FP64 cell
+        tangents are summed in FP64 and rounded once into an FP32 matrix.
+        """
         dm = self.dm
-        Ke, vals, dinv = self.matrix_buffers(precision)
+        ke_precision = precision if ke_precision is None else ke_precision
+        _, vals, dinv = self.matrix_buffers(precision)
+        Ke = self.matrix_buffers(ke_precision)[0]
         s = stream_ptr()
-        check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
-        check(self.L.fem_gather_matrix(self.pref, precision, ptr(Ke), ptr(vals), s), 'fem_gather_matrix')
-        check(self.L.fem_matrix_bc_rows(self.pref, precision, ptr(vals), self.n_bc, ptr(self.bc_dofs), s),
+        # FP32 matrices are stored in difference form (exact on translations), FP64 in standard form
+        form = _lib.FEM_FORM_DIFFERENCE if precision == _lib.FEM_F32 else _lib.FEM_FORM_STANDARD
+        check(self.L.fem_cell_tangent(dm.ref, self.mref, ke_precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
+        check(self.L.fem_gather_matrix(self.pref, ke_precision, ptr(Ke), precision, form, ptr(vals), ptr(dinv), s),
+              'fem_gather_matrix')
+        check(self.L.fem_matrix_bc_rows(self.pref, precision, ptr(vals), self.n_bc, ptr(self.bc_dofs), ptr(dinv), s),
               'fem_matrix_bc_rows')
-        check(self.L.fem_jacobi(self.pref, precision, ptr(vals), ptr(dinv), s), 'fem_jacobi')
         self.current = (precision, Ke, vals, dinv)
+        self.form = form
         return self.current
 
     def operator(self, u):
@@ -399,6 +408,7 @@ class DeviceProblem:
         o.rtol, o.atol, o.maxiter = cfg['tol'], cfg['atol'], cfg['maxiter']
         o.check_every, o.precision, o.precond = cfg['check_every'], precision, 1 if cfg['precond'] else 0
         o.reliable_delta, o.xcd_remap = cfg['reliable_delta'], 1 if cfg['xcd_remap'] else 0
+        o.timing = 1 if cfg.get('timing', False) else 0
         key = ('pcg_ws', precision)
         if key not in self.buffers:
             nb = int(self.L.fem_pcg_ws_bytes(self.pref, precision))
@@ -411,6 +421,13 @@ class DeviceProblem:
                                    ptr(b), ptr(x), C.byref(o), ptr(ws), C.c_void_p(self.host_status.data_ptr()),
                                    C.byref(res), stream_ptr()), 'fem_pcg_solve')
         return x, res
+
+    def spmv(self, x):
+        """y = A x with the current matrix, in its precision and storage form."""
+        prec, _, vals, _ = self.current
+        y = torch.empty_like(x)
+        check(self.L.fem_spmv(self.pref, prec, self.form, ptr(vals), ptr(x), ptr(y), stream_ptr()), 'fem_spmv')
+        return y
 
     def dot(self, x, y):
         if 'dot_ws' not in self.buffers:
@@ -448,17 +465,15 @@ class DeviceProblem:
         return lq
 
     # ------------------------------------------------------------- export
-    def export_csr(self, precision=None):
+    def export_csr(self):
         """Scalar CSR (indptr, indices, data) of the current matrix on the host."""
-        if precision is None:
-            prec, _, vals, _ = self.current
-        else:
-            prec, (_, vals, _) = precision, self.matrix_buffers(precision)
+        prec, _, vals, _ = self.current
         nnz = self.nnzb * self.V * self.V
         indptr = self.dm.empty(self.n + 1, dtype=torch.int32)
         indices = self.dm.empty(max(nnz, 1), dtype=torch.int32)
         data = self.dm.empty(max(nnz, 1))
-        check(self.L.fem_export_csr(self.pref, prec, ptr(vals), ptr(indptr), ptr(indices), ptr(data), stream_ptr()),
+        check(self.L.fem_export_csr(self.pref, prec, self.form, ptr(vals), ptr(indptr), ptr(indices), ptr(data),
+                                    stream_ptr()),
               'fem_export_csr')
         return indptr.cpu().numpy(), indices[:nnz].cpu().numpy(), data[:nnz].cpu().numpy()
 

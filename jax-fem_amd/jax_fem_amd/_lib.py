@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 FEM_OK, FEM_ERR_BAD_ARG, FEM_ERR_NOT_CONVERGED, FEM_ERR_NAN, FEM_ERR_HIP, FEM_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
 FEM_F32, FEM_F64 = 0, 1
+FEM_FORM_STANDARD, FEM_FORM_DIFFERENCE = 0, 1
 FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN = 1, 2, 3, 4
 
 _vp = C.c_void_p
@@ -47,12 +48,13 @@ class FemOperator(C.Structure):
 class FemPcgOptions(C.Structure):
     _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32),
                 ('precision', _i32), ('precond', _i32), ('reliable_delta', _f64),
-                ('xcd_remap', _i32), ('pad', _i32)]
+                ('xcd_remap', _i32), ('timing', _i32)]
 
 
 class FemPcgResult(C.Structure):
     _fields_ = [('iterations', _i32), ('replacements', _i32), ('status', _i32), ('pad', _i32),
-                ('b_norm', _f64), ('res_norm', _f64)]
+                ('b_norm', _f64), ('res_norm', _f64), ('spmv_ms', _f64), ('update_ms', _f64),
+                ('timed_launches', _i32), ('pad2', _i32)]
 
 
 _P = C.POINTER
@@ -77,12 +79,12 @@ _SIGNATURES = {
     'fem_face_load': (C.c_int, [_P(FemMesh), _i64, _vp, _vp, _vp, _vp]),
     'fem_face_load_adjoint': (C.c_int, [_P(FemMesh), _i64, _vp, _vp, _vp, _vp]),
     'fem_gather_nodal': (C.c_int, [_i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp]),
-    'fem_gather_matrix': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp]),
-    'fem_matrix_bc_rows': (C.c_int, [_P(FemPattern), _i32, _vp, _i64, _vp, _vp]),
+    'fem_gather_matrix': (C.c_int, [_P(FemPattern), _i32, _vp, _i32, _i32, _vp, _vp, _vp]),
+    'fem_matrix_bc_rows': (C.c_int, [_P(FemPattern), _i32, _vp, _i64, _vp, _vp, _vp]),
     'fem_jacobi': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp]),
     'fem_residual_bc': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _f64, _vp]),
-    'fem_spmv': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp, _vp]),
-    'fem_export_csr': (C.c_int, [_P(FemPattern), _i32, _vp, _vp, _vp, _vp, _vp]),
+    'fem_spmv': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp]),
+    'fem_export_csr': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     'fem_operator_apply': (C.c_int, [_P(FemOperator), _vp, _vp, _vp]),
     'fem_pcg_ws_bytes': (_i64, [_P(FemPattern), _i32]),
     'fem_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemOperator), _vp, _vp, _P(FemPcgOptions),

@@ -29,7 +29,8 @@ _NEWTON_OPTION_KEYS = frozenset({'tol', 'rel_tol', 'line_search_flag', 'initial_
 # precision 'fp64' (plain FP64 PCG) or 'mixed' (FP32 matrix and PCG, FP64
 # refinement residual and iterate).
 JAX_SOLVER_DEFAULTS = dict(precond=True, precision='fp64', tol=1e-10, atol=1e-10, maxiter=10000,
-                           check_every=32, reliable_delta=0.1, xcd_remap=True)
+                           check_every=16, reliable_delta=0.01, xcd_remap=True, assembly=None, timing=False)
+_ASSEMBLY = {None: None, 'fp32': _lib.FEM_F32, 'fp64': _lib.FEM_F64}
 _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32}
 
 
@@ -70,6 +71,9 @@ def linear_config(linear_options):
     if cfg['precision'] not in _PRECISIONS:
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}, got {cfg['precision']!r}")
     cfg['precision_code'] = _PRECISIONS[cfg['precision']]
+    if cfg['assembly'] not in _ASSEMBLY:
+        raise ValueError(f"assembly must be 'fp32' or 'fp64', got {cfg['assembly']!r}")
+    cfg['assembly_code'] = _ASSEMBLY[cfg['assembly']]
     return cfg
 
 
@@ -141,7 +145,7 @@ def solver(problem, solver_options={}):
     while rel > rel_tol and res_val > tol:
         n_iters += 1
         t0 = time.perf_counter()
-        dev.tangent(dofs, lin['precision_code'])
+        dev.tangent(dofs, lin['precision_code'], lin['assembly_code'])
         torch.cuda.synchronize()
         timing['global_matrix'] += time.perf_counter() - t0
         b = -res
@@ -149,7 +153,8 @@ def solver(problem, solver_options={}):
         t0 = time.perf_counter()
         inc, info = linear_solve(dev, dofs, b, x0, lin)
         timing['linear'] += time.perf_counter() - t0
-        stats.append((info.iterations, info.replacements, info.res_norm))
+        stats.append(dict(iterations=info.iterations, replacements=info.replacements, res_norm=info.res_norm,
+                          spmv_ms=info.spmv_ms, update_ms=info.update_ms))
         dofs = _line_search(dev, dofs, inc) if cfg.get('line_search_flag', False) else dofs + inc
         t0 = time.perf_counter()
         res = dev.residual(dofs)
@@ -202,14 +207,14 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
     dev.prepare()
     u = problem._flat(sol_list)
     lin = linear_config(adjoint_solver_options)
-    dev.tangent(u, lin['precision_code'])
+    dev.tangent(u, lin['precision_code'], lin['assembly_code'])
     v = torch.as_tensor(v_list[0] if isinstance(v_list, (list, tuple)) else v_list).to(
         device=u.device, dtype=torch.float64).reshape(-1).clone()
     if dev.n_bc:
         v[dev.bc_dofs_long] = 0.
     lam, info = linear_solve(dev, u, v, torch.zeros_like(v), lin)
     problem._adjoint_stats = dict(iterations=info.iterations, replacements=info.replacements,
-                                  res_norm=info.res_norm)
+                                  res_norm=info.res_norm, spmv_ms=info.spmv_ms, update_ms=info.update_ms)
     outs, cots = [], []
     if dev.material.has_param:
         k = dev.param_index

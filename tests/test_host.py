@@ -35,7 +35,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.FemMaterial) == 8 + 6 * 8 + 8
     assert ctypes.sizeof(_lib.FemPattern) == 4 * 8 + 4 * 4 + 7 * 8
     assert ctypes.sizeof(_lib.FemPcgOptions) == 8 + 8 + 4 * 4 + 8 + 8
-    assert ctypes.sizeof(_lib.FemPcgResult) == 16 + 16
+    assert ctypes.sizeof(_lib.FemPcgResult) == 16 + 16 + 16 + 8
 
 
 def test_compute_entry_fails_loudly_without_gpu():
