@@ -60,35 +60,62 @@ __device__ inline T block_sum_256(T v, T* smem /* >= 4 */) {
   return t;
 }
 
-// "Last block" completion: every block publishes its partial, the last block
-// to arrive sums all partials in index order (deterministic for a fixed grid)
-// and runs `fin(total...)`.  counter must start at 0; the last block resets it.
-template <int K, typename T>
-__device__ inline bool last_block_sums(const T (&mine)[K], T* partials, unsigned* counter,
-                                       T (&total)[K], T* smem /* >= 4*K */) {
-  __shared__ bool s_last;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = mine[k];
-    __threadfence();
-    unsigned t = atomicAdd(counter, 1u);
-    s_last = (t == gridDim.x - 1);
-  }
+// Block-wide sum for blockDim.x == NT (a multiple of 64, <= 1024); result
+// valid in all threads.  smem holds >= NT/64 values.
+template <int NT, typename T>
+__device__ inline T block_sum(T v, T* smem) {
+  constexpr int NW = NT / 64;
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) smem[w] = v;
   __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
+  T t = T(0);
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    T acc = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-      acc += __hip_atomic_load(&partials[(size_t)k * gridDim.x + i], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-    }
-    total[k] = block_sum_256(acc, smem + 4 * k);
-  }
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
+  for (int i = 0; i < NW; ++i) t += smem[i];
+  __syncthreads();
+  return t;
 }
+
+// Slice-parallel SELL-64 row product.  A slice (64 rows) is processed by W
+// waves; wave w accumulates the slots k = w, w + W, ... of its 64 rows, so
+// each slice has W independent streams of coalesced value loads in flight.
+// `xat(c)` returns the V entries of the input vector at block column c.
+// DIFF: difference form (diagonal slot holds the row-sum block; off-diagonal
+// blocks multiply x_c - x_row).  Returns this wave's partial sums.
+template <int V, bool DIFF, int W, int U, typename T, typename XAt>
+__device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __restrict__ vals, int64_t s, int w,
+                                          int lane, const XAt& xat, const T (&xr)[V], T (&acc)[V]) {
+  constexpr int VV = V * V;
+  const int64_t row = s * kSlice + lane;
+  const int base = pat.slice_ptr[s];
+  const int L = (pat.slice_ptr[s + 1] - base) >> 6;
+  const int32_t* cp = pat.sell_col + base + lane;
+  const T* vp = vals + (size_t)base * VV + lane;
+#pragma unroll
+  for (int i = 0; i < V; ++i) acc[i] = T(0);
+#pragma unroll U
+  for (int k = w; k < L; k += W) {
+    const int64_t c = cp[(size_t)k * kSlice];
+    T xc[V];
+    xat(c, xc);
+    if (DIFF) {
+#pragma unroll
+      for (int j = 0; j < V; ++j) xc[j] = c == row ? xr[j] : xc[j] - xr[j];
+    }
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int j = 0; j < V; ++j) acc[i] += vp[((size_t)k * VV + i * V + j) * kSlice] * xc[j];
+  }
+}
+
+// Tuned slice split: FP32 4 waves x unroll 2, FP64 8 waves x unroll 1.
+template <typename T>
+struct SliceCfg {
+  static constexpr int W = sizeof(T) == 4 ? 4 : 8;
+  static constexpr int U = sizeof(T) == 4 ? 2 : 1;
+  static constexpr int NT = 64 * W;
+};
 
 // XCD-contiguous block order: hardware dispatches block b to XCD b % 8; this
 // maps the blocks of one XCD onto a contiguous range of logical block ids so

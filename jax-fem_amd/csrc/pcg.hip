@@ -7,16 +7,15 @@
 // exact boundary increment (solver.py:402-409), so the residual and search
 // direction vanish on those rows and CG runs on the SPD free block.
 //
-// Two kernels per iteration k:
-//   A_k: reduce the <r,z>, <r,r> partials of B_{k-1} (or of a restart),
-//        convergence test, beta, p = z + beta p_old (ping-pong buffers),
-//        q = A p, partials of <p, q>
-//   B_k: reduce the <p,q> partials of A_k, alpha = rz / pq,
-//        x += alpha p, r -= alpha q, z = D^-1 r, partials of <r,z>, <r,r>
-// Every block reduces the previous kernel's per-block partials itself, in the
-// same fixed order, so all blocks agree bit for bit and no grid-wide fence or
-// atomic is needed; scalars written by block 0 are read by the next kernel
-// (parity-indexed slots where a kernel reads the previous iteration's value).
+// Four launches per iteration k:
+//   A_k : p = z + beta p_old (ping-pong buffers), q = A p, per-block <p, q>
+//         (one 64-row slice per block, W waves splitting its slots)
+//   RA_k: one block: pq = sum of A's partials, alpha = rz / pq
+//   B_k : x += alpha p, r -= alpha q, z = D^-1 r, per-block <r,z>, <r,r>
+//   RB_k: one block: rz, rr; convergence / NaN / maxiter test; beta = rz / rz_old
+// The reductions sum per-block partials in a fixed order (bit-reproducible);
+// no grid-wide fence or atomic is used.  Scalars live in device memory and
+// are written by one kernel and read by the next.
 //
 // Precision FEM_F64: everything FP64, standard matrix form.
 // FEM_F32: FP32 matrix in difference form, FP32 r, z, p, q; x accumulated in
@@ -35,12 +34,13 @@ int fem_operator_apply_impl(const fem_operator_t* op, const double* x, double* y
 namespace fem {
 
 struct PcgScalars {
-  double rzs[2];         // rz used for alpha at iteration k, slot k & 1
-  double rr;             // <r,r> at the start of the last iteration A ran
+  double rz;             // <r, z> of the current iteration
+  double alpha, beta;
+  double rr;             // <r, r> of the current iteration
   double rr_true;        // |b - A x|^2 of the last true-residual evaluation
-  double pad0[4];
-  int32_t done, iter, nan, restart_iter;
-  int32_t pad1[4];
+  double pad0[3];
+  int32_t done, iter, nan, pad1;
+  int32_t pad2[4];
 };
 static_assert(sizeof(PcgScalars) <= 128, "host status buffer");
 
@@ -51,108 +51,96 @@ __device__ inline double block_reduce_partials(const double* __restrict__ p, int
   return block_sum_256(acc, smem);
 }
 
-// y_i = sum_k V_k x_{c_k}        (standard form)
-// y_i = sum_k V_k (c_k == i ? x_i : x_{c_k} - x_i)   (difference form: the
-//       diagonal slot holds the row-sum block R_i = sum_j A_ij, so the product
-//       is exact on translations whatever the rounding of the stored values)
-// with x = z + beta p_old gathered on the fly; xr returns x at the row.
-template <int V, bool DIFF, typename T>
-__device__ inline void sell_row(const fem_pattern_t& pat, const T* __restrict__ vals, int64_t row,
-                                const T* __restrict__ z, const T* __restrict__ p_old, T beta, T (&acc)[V],
-                                T (&xr)[V]) {
-  constexpr int VV = V * V;
-  const int64_t s = row >> 6;
-  const int lane = (int)(row & 63);
-  const int base = pat.slice_ptr[s];
-  const int L = (pat.slice_ptr[s + 1] - base) >> 6;
-  const int32_t* cp = pat.sell_col + base + lane;
-  const T* vp = vals + (size_t)base * VV + lane;
-#pragma unroll
-  for (int j = 0; j < V; ++j) {
-    xr[j] = z[row * V + j] + beta * p_old[row * V + j];
-    acc[j] = T(0);
-  }
-#pragma unroll 2
-  for (int k = 0; k < L; ++k) {
-    const int64_t c = cp[(size_t)k * kSlice];
-    T xc[V];
-#pragma unroll
-    for (int j = 0; j < V; ++j) {
-      const T v = z[c * V + j] + beta * p_old[c * V + j];
-      xc[j] = DIFF ? (c == row ? xr[j] : v - xr[j]) : v;
-    }
-#pragma unroll
-    for (int i = 0; i < V; ++i)
-#pragma unroll
-      for (int j = 0; j < V; ++j) acc[i] += vp[((size_t)k * VV + i * V + j) * kSlice] * xc[j];
-  }
-}
 
+
+// Kernel A: one 64-row slice per block; the W waves split the slots
+// (sell_slice_partial) and wave 0 combines them through LDS, forms
+// p = z + beta p_old at the row, stores p and q = A p and the block's <p, q>.
+// The input of the product, x_c = z_c + beta p_old_c, is formed on the fly.
 template <int V, bool DIFF, typename T>
-__global__ __launch_bounds__(256) void k_pcg_a(fem_pattern_t pat, const T* __restrict__ vals,
-                                               const T* __restrict__ z, const T* __restrict__ p_old,
-                                               T* __restrict__ p_new, T* __restrict__ q, PcgScalars* sc,
-                                               const double* __restrict__ partB, int nB,
-                                               const double* __restrict__ partR, int nR,
-                                               double* __restrict__ partA, int k, double tol2, int maxiter,
-                                               int remap) {
-  __shared__ double smem[8];
+__global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, const T* __restrict__ vals,
+                                                           const T* __restrict__ z, const T* __restrict__ p_old,
+                                                           T* __restrict__ p_new, T* __restrict__ q,
+                                                           const PcgScalars* sc, double* __restrict__ partA,
+                                                           int remap) {
+  constexpr int W = SliceCfg<T>::W, U = SliceCfg<T>::U, NT = SliceCfg<T>::NT;
+  __shared__ double smem[NT / 64];
+  __shared__ T red[W][kSlice][V];
   if (sc->done) return;
-  const bool restarted = sc->restart_iter == k;
-  const double* src = restarted ? partR : partB;
-  const int nsrc = restarted ? nR : nB;
-  const double rz = block_reduce_partials(src, nsrc, smem);
-  const double rr = block_reduce_partials(src + nsrc, nsrc, smem + 4);
-  const bool bad = !isfinite(rr) || !isfinite(rz);
-  if (bad || rr <= tol2 || k >= maxiter) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      sc->done = 1;
-      sc->iter = k;
-      sc->rr = rr;
-      if (bad) sc->nan = 1;
-    }
-    return;
+  const T beta = T(sc->beta);
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t s = remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int64_t row = s * kSlice + lane;
+  const bool real = row < pat.num_rows;
+  auto xat = [&](int64_t c, T (&xc)[V]) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) xc[j] = z[c * V + j] + beta * p_old[c * V + j];
+  };
+  T xr[V], acc[V];
+  xat(real ? row : 0, xr);
+  sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, xat, xr, acc);
+  if (w > 0) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
   }
-  // beta: 0 at the start; after a residual replacement the recursion's value
-  // (from B_{k-1}'s partials); otherwise rz_k / rz_{k-1}
-  double beta_d = 0.0;
-  if (k > 0) {
-    const double rz_rec = restarted ? block_reduce_partials(partB, nB, smem) : rz;
-    beta_d = rz_rec / sc->rzs[(k - 1) & 1];
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sc->rzs[k & 1] = rz;
-    sc->rr = rr;
-  }
-  const T beta = T(beta_d);
-  const int nb = gridDim.x;
-  const int blk = remap ? xcd_logical_block(blockIdx.x, nb) : blockIdx.x;
-  const int64_t row = (int64_t)blk * 256 + threadIdx.x;
+  __syncthreads();
   T dot = T(0);
-  if (row < pat.num_rows) {
-    T acc[V], pr[V];
-    sell_row<V, DIFF, T>(pat, vals, row, z, p_old, beta, acc, pr);
+  if (w == 0 && real) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      p_new[row * V + i] = pr[i];
-      q[row * V + i] = acc[i];
-      dot += pr[i] * acc[i];
+      T t = acc[i];
+#pragma unroll
+      for (int o = 1; o < W; ++o) t += red[o][lane][i];
+      p_new[row * V + i] = xr[i];
+      q[row * V + i] = t;
+      dot += xr[i] * t;
     }
   }
-  const double d = block_sum_256(double(dot), smem);
+  const double d = block_sum<NT>(double(dot), smem);
   if (threadIdx.x == 0) partA[blockIdx.x] = d;
+}
+
+// RA: alpha = rz / sum(partA)
+__global__ __launch_bounds__(1024) void k_pcg_ra(const double* __restrict__ partA, int nA, PcgScalars* sc) {
+  __shared__ double smem[16];
+  if (sc->done) return;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nA; i += 1024) a += partA[i];
+  const double pq = block_sum<1024>(a, smem);
+  if (threadIdx.x == 0) sc->alpha = sc->rz / pq;
+}
+
+// RB: rz, rr of iteration k+1; stop test; beta.
+__global__ __launch_bounds__(1024) void k_pcg_rb(const double* __restrict__ partB, int nB, PcgScalars* sc, int k,
+                                                 double tol2, int maxiter) {
+  __shared__ double smem[16];
+  if (sc->done) return;
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = threadIdx.x; i < nB; i += 1024) {
+    a0 += partB[i];
+    a1 += partB[nB + i];
+  }
+  const double rz = block_sum<1024>(a0, smem);
+  const double rr = block_sum<1024>(a1, smem);
+  if (threadIdx.x == 0) {
+    const bool bad = !isfinite(rr) || !isfinite(rz);
+    sc->beta = rz / sc->rz;
+    sc->rz = rz;
+    sc->rr = rr;
+    sc->iter = k + 1;
+    if (bad) sc->nan = 1;
+    if (bad || rr <= tol2 || k + 1 >= maxiter) sc->done = 1;
+  }
 }
 
 template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_pcg_b(int64_t n, TX* __restrict__ x, T* __restrict__ r,
                                                T* __restrict__ z, const T* __restrict__ p,
                                                const T* __restrict__ q, const T* __restrict__ dinv,
-                                               PcgScalars* sc, const double* __restrict__ partA, int nA,
-                                               double* __restrict__ partB, int k) {
+                                               const PcgScalars* sc, double* __restrict__ partB) {
   __shared__ double smem[8];
   if (sc->done) return;
-  const double pq = block_reduce_partials(partA, nA, smem);
-  const double alpha_d = sc->rzs[k & 1] / pq;
+  const double alpha_d = sc->alpha;
   const T alpha = T(alpha_d);
   T rz = T(0), rr = T(0);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -169,7 +157,6 @@ __global__ __launch_bounds__(256) void k_pcg_b(int64_t n, TX* __restrict__ x, T*
   if (threadIdx.x == 0) {
     partB[blockIdx.x] = s1;
     partB[gridDim.x + blockIdx.x] = s2;
-    if (blockIdx.x == 0) sc->iter = k + 1;
   }
 }
 
@@ -194,11 +181,10 @@ __global__ __launch_bounds__(256) void k_pcg_finish_rr(const double* __restrict_
   if (threadIdx.x == 0) sc->rr_true = s;
 }
 
-// z = D^-1 r and the partials of <r,z>, <r,r> for iteration k (start or residual replacement).
+// z = D^-1 r and the partials of <r,z>, <r,r> (start or residual replacement).
 template <typename T>
 __global__ __launch_bounds__(256) void k_pcg_restart(int64_t n, const T* __restrict__ r, T* __restrict__ z,
-                                                     const T* __restrict__ dinv, PcgScalars* sc,
-                                                     double* __restrict__ partR, int k) {
+                                                     const T* __restrict__ dinv, double* __restrict__ partR) {
   __shared__ double smem[8];
   T rz = T(0), rr = T(0);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -212,10 +198,29 @@ __global__ __launch_bounds__(256) void k_pcg_restart(int64_t n, const T* __restr
   if (threadIdx.x == 0) {
     partR[blockIdx.x] = s1;
     partR[gridDim.x + blockIdx.x] = s2;
-    if (blockIdx.x == 0) {
-      sc->restart_iter = k;
-      sc->done = 0;
-      sc->iter = k;
+  }
+}
+
+// Start (init = 1: beta = 0, counters reset) or residual replacement at
+// iteration k (beta keeps the recursion's value; only rz and rr change).
+__global__ __launch_bounds__(1024) void k_pcg_rr(const double* __restrict__ partR, int n, PcgScalars* sc, int init,
+                                                 int k) {
+  __shared__ double smem[16];
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    a0 += partR[i];
+    a1 += partR[n + i];
+  }
+  const double rz = block_sum<1024>(a0, smem);
+  const double rr = block_sum<1024>(a1, smem);
+  if (threadIdx.x == 0) {
+    sc->rz = rz;
+    sc->rr = rr;
+    sc->done = 0;
+    sc->iter = k;
+    if (init) {
+      sc->beta = 0.0;
+      sc->nan = 0;
     }
   }
 }
@@ -285,7 +290,7 @@ struct Ws {
 
 static int64_t align256(int64_t b) { return (b + 255) / 256 * 256; }
 
-static int64_t grid_a(const fem_pattern_t* pat) { return ceil_div(pat->num_rows, 256); }
+static int64_t grid_a(const fem_pattern_t* pat) { return pat->num_slices; }
 
 static int64_t ws_layout(const fem_pattern_t* pat, int precision, void* base, Ws* w) {
   const int64_t n = pat->num_rows * pat->vec;
@@ -354,7 +359,8 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
     if (kMixed) {
       if (int rc = fem_operator_apply_impl(op, x, w.y, s)) return rc;
     } else {
-      hipLaunchKernelGGL((k_spmv_x<V, T>), dim3(gA), dim3(256), 0, s, *pat, vals, (const double*)x, w.y);
+      hipLaunchKernelGGL((k_spmv_x<V, T>), dim3((unsigned)ceil_div(pat->num_rows, 256)), dim3(256), 0, s, *pat, vals,
+                         (const double*)x, w.y);
       FEM_LAUNCHED();
     }
     return residual_from(w.y);
@@ -381,7 +387,8 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
     res->res_norm = std::sqrt(rr_true);
     return FEM_OK;
   }
-  hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gV), dim3(256), 0, s, n, (const T*)r, z, dinv, w.sc, w.partR, 0);
+  hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gV), dim3(256), 0, s, n, (const T*)r, z, dinv, w.partR);
+  hipLaunchKernelGGL(k_pcg_rr, dim3(1), dim3(1024), 0, s, (const double*)w.partR, gV, w.sc, 1, 0);
   FEM_LAUNCHED();
   double rr_ref = rr_true;
   const int check = std::max(1, opt->check_every);
@@ -400,12 +407,14 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
       const int k = iter0 + m;
       const int cur = k & 1;
       if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m], s));
-      hipLaunchKernelGGL((k_pcg_a<V, kDiff, T>), dim3(gA), dim3(256), 0, s, *pat, vals, (const T*)z,
-                         (const T*)pbuf[cur], pbuf[cur ^ 1], q, w.sc, (const double*)w.partB, gV,
-                         (const double*)w.partR, gV, w.partA, k, tol2, opt->maxiter, opt->xcd_remap);
+      hipLaunchKernelGGL((k_pcg_a<V, kDiff, T>), dim3(gA), dim3(SliceCfg<T>::NT), 0, s, *pat, vals, (const T*)z,
+                         (const T*)pbuf[cur], pbuf[cur ^ 1], q, (const PcgScalars*)w.sc, w.partA, opt->xcd_remap);
       if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m + 1], s));
+      hipLaunchKernelGGL(k_pcg_ra, dim3(1), dim3(1024), 0, s, (const double*)w.partA, gA, w.sc);
       hipLaunchKernelGGL((k_pcg_b<T, double>), dim3(gV), dim3(256), 0, s, n, x, r, z, (const T*)pbuf[cur ^ 1],
-                         (const T*)q, dinv, w.sc, (const double*)w.partA, gA, w.partB, k);
+                         (const T*)q, dinv, (const PcgScalars*)w.sc, w.partB);
+      hipLaunchKernelGGL(k_pcg_rb, dim3(1), dim3(1024), 0, s, (const double*)w.partB, gV, w.sc, k, tol2,
+                         opt->maxiter);
       if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m + 2], s));
     }
     FEM_LAUNCHED();
@@ -441,8 +450,8 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
         break;
       }
       // residual replacement at iteration `iter`: keep the search direction
-      hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gV), dim3(256), 0, s, n, (const T*)r, z, dinv, w.sc, w.partR,
-                         iter);
+      hipLaunchKernelGGL(k_pcg_restart<T>, dim3(gV), dim3(256), 0, s, n, (const T*)r, z, dinv, w.partR);
+      hipLaunchKernelGGL(k_pcg_rr, dim3(1), dim3(1024), 0, s, (const double*)w.partR, gV, w.sc, 0, iter);
       FEM_LAUNCHED();
       rr_ref = rr_true;
       res->replacements += 1;

@@ -141,36 +141,37 @@ __global__ void k_residual_bc(int64_t n_bc, const int32_t* __restrict__ dofs, co
 }
 
 // ------------------------------------------------------------ SpMV
+// One block per slice; the W waves split the slots and wave 0 combines them.
 template <int V, bool DIFF, typename T>
-__global__ __launch_bounds__(256) void k_spmv(fem_pattern_t pat, const T* __restrict__ vals, const T* __restrict__ x,
-                                              T* __restrict__ y) {
-  constexpr int VV = V * V;
-  const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (row >= pat.num_rows) return;
-  const int64_t s = row >> 6;
-  const int lane = (int)(row & 63);
-  const int base = pat.slice_ptr[s];
-  const int L = (pat.slice_ptr[s + 1] - base) >> 6;
-  const int32_t* cp = pat.sell_col + base + lane;
-  const T* vp = vals + (size_t)base * VV + lane;
-  T acc[V];
+__global__ __launch_bounds__(SliceCfg<T>::NT) void k_spmv(fem_pattern_t pat, const T* __restrict__ vals,
+                                                          const T* __restrict__ x, T* __restrict__ y) {
+  constexpr int W = SliceCfg<T>::W, U = SliceCfg<T>::U;
+  __shared__ T red[W][kSlice][V];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t s = blockIdx.x;
+  const int64_t row = s * kSlice + lane;
+  const bool real = row < pat.num_rows;
+  auto xat = [&](int64_t c, T (&xc)[V]) {
 #pragma unroll
-  for (int i = 0; i < V; ++i) acc[i] = T(0);
-  T xr[V];
+    for (int j = 0; j < V; ++j) xc[j] = x[c * V + j];
+  };
+  T xr[V], acc[V];
+  xat(real ? row : 0, xr);
+  sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, xat, xr, acc);
+  if (w > 0) {
 #pragma unroll
-  for (int j = 0; j < V; ++j) xr[j] = x[row * V + j];
-  for (int k = 0; k < L; ++k) {
-    const int64_t c = cp[(size_t)k * kSlice];
-    T xc[V];
-#pragma unroll
-    for (int j = 0; j < V; ++j) xc[j] = DIFF ? (c == row ? xr[j] : x[c * V + j] - xr[j]) : x[c * V + j];
-#pragma unroll
-    for (int i = 0; i < V; ++i)
-#pragma unroll
-      for (int j = 0; j < V; ++j) acc[i] += vp[((size_t)k * VV + i * V + j) * kSlice] * xc[j];
+    for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
   }
+  __syncthreads();
+  if (w == 0 && real) {
 #pragma unroll
-  for (int i = 0; i < V; ++i) y[row * V + i] = acc[i];
+    for (int i = 0; i < V; ++i) {
+      T t = acc[i];
+#pragma unroll
+      for (int o = 1; o < W; ++o) t += red[o][lane][i];
+      y[row * V + i] = t;
+    }
+  }
 }
 
 // Scalar CSR of the block matrix: row r = V*node + ci holds the blocks of node
@@ -371,13 +372,13 @@ int fem_spmv(const fem_pattern_t* pat, int32_t precision, int32_t form, const vo
   return with_vt(pat->vec, precision, [&](auto vc, auto tv) -> int {
     constexpr int V = decltype(vc)::value;
     using T = decltype(tv);
-    const dim3 grid((unsigned)ceil_div(pat->num_rows, 256));
+    const dim3 grid((unsigned)pat->num_slices), block(SliceCfg<T>::NT);
     if (form == FEM_FORM_DIFFERENCE)
-      hipLaunchKernelGGL((k_spmv<V, true, T>), grid, dim3(256), 0, as_stream(stream), *pat, (const T*)vals,
-                         (const T*)x, (T*)y);
+      hipLaunchKernelGGL((k_spmv<V, true, T>), grid, block, 0, as_stream(stream), *pat, (const T*)vals, (const T*)x,
+                         (T*)y);
     else
-      hipLaunchKernelGGL((k_spmv<V, false, T>), grid, dim3(256), 0, as_stream(stream), *pat, (const T*)vals,
-                         (const T*)x, (T*)y);
+      hipLaunchKernelGGL((k_spmv<V, false, T>), grid, block, 0, as_stream(stream), *pat, (const T*)vals, (const T*)x,
+                         (T*)y);
     FEM_LAUNCHED();
     return FEM_OK;
   });
