@@ -122,6 +122,8 @@ def run(args):
             f"update {lin['update_ms']*1e3:.1f} us")
 
     if args.only is not None:
+        if args.calib_copy_mib:
+            copy_calibration(args.calib_copy_mib)
         print(json.dumps({name: {k: v for k, v in res[name].items()} for name in paths}), flush=True)
         return
     # adjoint gradient of the compliance C = f.u w.r.t. theta
@@ -182,6 +184,19 @@ def run(args):
     print(json.dumps(out), flush=True)
 
 
+def copy_calibration(mib):
+    """Plain device copy of a known size (3 launches): the reference against which
+    hardware byte counters are scaled when profiling."""
+    import torch
+    n = mib * 2**20 // 4
+    src = torch.rand(n, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copy calibration: {n * 4} bytes read + {n * 4} bytes written per launch")
+
+
 def cpu_baseline(dev, gpu_iterations):
     """Oracle FP64 CPU restatement on a bounded sample, scaled to the workload."""
     import numpy as np
@@ -220,6 +235,8 @@ def main():
     ap.add_argument('--no-cpu-baseline', dest='cpu_baseline', action='store_false')
     ap.add_argument('--only', choices=['fp64', 'mixed'], default=None,
                     help='time one path only (no adjoint, no summary line); for profiling')
+    ap.add_argument('--calib-copy-mib', type=int, default=0,
+                    help='with --only: also run a plain copy of this many MiB (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (--gpus {args.gpus} requested); "
