@@ -79,12 +79,15 @@ __device__ inline T block_sum(T v, T* smem) {
 // Slice-parallel SELL-64 row product.  A slice (64 rows) is processed by W
 // waves; wave w accumulates the slots k = w, w + W, ... of its 64 rows, so
 // each slice has W independent streams of coalesced value loads in flight.
-// `xat(c)` returns the V entries of the input vector at block column c.
+// `xat(c, xc)` loads the V entries of the input vector at block column c.
 // DIFF: difference form (diagonal slot holds the row-sum block; off-diagonal
-// blocks multiply x_c - x_row).  Returns this wave's partial sums.
+// blocks multiply x_c - x_row).  The arithmetic is FP64 whatever the storage
+// type T: the difference of two FP32 inputs is exact in FP64 and the products
+// are accumulated without FP32 rounding (the kernel is HBM-bound, so the FP64
+// FMAs cost no time).  Returns this wave's partial sums.
 template <int V, bool DIFF, int W, int U, typename T, typename XAt>
 __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __restrict__ vals, int64_t s, int w,
-                                          int lane, const XAt& xat, const T (&xr)[V], T (&acc)[V]) {
+                                          int lane, const XAt& xat, const double (&xr)[V], double (&acc)[V]) {
   constexpr int VV = V * V;
   const int64_t row = s * kSlice + lane;
   const int base = pat.slice_ptr[s];
@@ -92,20 +95,19 @@ __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __r
   const int32_t* cp = pat.sell_col + base + lane;
   const T* vp = vals + (size_t)base * VV + lane;
 #pragma unroll
-  for (int i = 0; i < V; ++i) acc[i] = T(0);
+  for (int i = 0; i < V; ++i) acc[i] = 0.0;
 #pragma unroll U
   for (int k = w; k < L; k += W) {
     const int64_t c = cp[(size_t)k * kSlice];
-    T xc[V];
-    xat(c, xc);
-    if (DIFF) {
+    T xt[V];
+    xat(c, xt);
+    double xc[V];
 #pragma unroll
-      for (int j = 0; j < V; ++j) xc[j] = c == row ? xr[j] : xc[j] - xr[j];
-    }
+    for (int j = 0; j < V; ++j) xc[j] = (DIFF && c != row) ? double(xt[j]) - xr[j] : double(xt[j]);
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
-      for (int j = 0; j < V; ++j) acc[i] += vp[((size_t)k * VV + i * V + j) * kSlice] * xc[j];
+      for (int j = 0; j < V; ++j) acc[i] = fma(double(vp[((size_t)k * VV + i * V + j) * kSlice]), xc[j], acc[i]);
   }
 }
 

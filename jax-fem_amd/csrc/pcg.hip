@@ -18,8 +18,9 @@
 // are written by one kernel and read by the next.
 //
 // Precision FEM_F64: everything FP64, standard matrix form.
-// FEM_F32: FP32 matrix in difference form, FP32 r, z, p, q; x accumulated in
-// FP64; the recursive residual is replaced by the FP64 true residual b - A x
+// FEM_F32: FP32 matrix in difference form, FP32 r, z, p, q (FP32 storage,
+// FP64 arithmetic inside the product); x accumulated in FP64; the recursive
+// residual is replaced by the FP64 true residual b - A x
 // (A applied matrix-free in FP64) each time it has dropped by reliable_delta
 // and at convergence; beta keeps the value of the recursion across a
 // replacement.
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, co
                                                            int remap) {
   constexpr int W = SliceCfg<T>::W, U = SliceCfg<T>::U, NT = SliceCfg<T>::NT;
   __shared__ double smem[NT / 64];
-  __shared__ T red[W][kSlice][V];
+  __shared__ double red[W][kSlice][V];
   if (sc->done) return;
   const T beta = T(sc->beta);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -76,27 +77,31 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, co
 #pragma unroll
     for (int j = 0; j < V; ++j) xc[j] = z[c * V + j] + beta * p_old[c * V + j];
   };
-  T xr[V], acc[V];
-  xat(real ? row : 0, xr);
+  T pr[V];
+  xat(real ? row : 0, pr);
+  double xr[V], acc[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) xr[j] = double(pr[j]);
   sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, xat, xr, acc);
   if (w > 0) {
 #pragma unroll
     for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
   }
   __syncthreads();
-  T dot = T(0);
+  double dot = 0.0;
   if (w == 0 && real) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      T t = acc[i];
+      double t = acc[i];
 #pragma unroll
       for (int o = 1; o < W; ++o) t += red[o][lane][i];
-      p_new[row * V + i] = xr[i];
-      q[row * V + i] = t;
-      dot += xr[i] * t;
+      const T qi = T(t);
+      p_new[row * V + i] = pr[i];
+      q[row * V + i] = qi;
+      dot += xr[i] * double(qi);
     }
   }
-  const double d = block_sum<NT>(double(dot), smem);
+  const double d = block_sum<NT>(dot, smem);
   if (threadIdx.x == 0) partA[blockIdx.x] = d;
 }
 

@@ -146,7 +146,7 @@ template <int V, bool DIFF, typename T>
 __global__ __launch_bounds__(SliceCfg<T>::NT) void k_spmv(fem_pattern_t pat, const T* __restrict__ vals,
                                                           const T* __restrict__ x, T* __restrict__ y) {
   constexpr int W = SliceCfg<T>::W, U = SliceCfg<T>::U;
-  __shared__ T red[W][kSlice][V];
+  __shared__ double red[W][kSlice][V];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t s = blockIdx.x;
   const int64_t row = s * kSlice + lane;
@@ -155,8 +155,11 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_spmv(fem_pattern_t pat, con
 #pragma unroll
     for (int j = 0; j < V; ++j) xc[j] = x[c * V + j];
   };
-  T xr[V], acc[V];
-  xat(real ? row : 0, xr);
+  T xt[V];
+  xat(real ? row : 0, xt);
+  double xr[V], acc[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) xr[j] = double(xt[j]);
   sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, xat, xr, acc);
   if (w > 0) {
 #pragma unroll
@@ -166,10 +169,10 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_spmv(fem_pattern_t pat, con
   if (w == 0 && real) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      T t = acc[i];
+      double t = acc[i];
 #pragma unroll
       for (int o = 1; o < W; ++o) t += red[o][lane][i];
-      y[row * V + i] = t;
+      y[row * V + i] = T(t);
     }
   }
 }
