@@ -13,6 +13,23 @@ Jacobi-PCG to |r| <= 1e-10 |b|, update, new residual); for this linear problem
 (FP32 matrix + FP32 PCG, FP64 iterate and FP64 residual replacement);
 the plain FP64 path is timed beside it, and its tight solve (1e-12) is the
 reference for the relative error.
+
+``--workload`` selects one of the other configurations of the same path
+instead (one JSON line each, same keys; not the default line):
+  poisson      3D Poisson, HEX8 32^3 on the unit cube, f ~ N(0,1) per node
+               (seed 0) as the source, u = 0 on x = 0; FP64 and mixed.
+  neo_hookean  Neo-Hookean cube, HEX8 64^3, E = 10, nu = 0.3, bottom fixed,
+               top u_z = 0.1 k/5 in 5 load steps (u_x, u_y free), Newton to
+               |R| <= 1e-8 from the previous step's solution.
+  simp         SIMP compliance gradient, HEX8 160x80x40 on 2x1x0.5, theta =
+               0.5 + U(-0.05, 0.05) per cell (seed 0), E = 1e-3 + theta^3
+               (1 - 1e-3), clamped x = 0, traction (0, 0, -1) on the 2x2
+               faces at x = 2, y-centred, z at the bottom; adjoint vs central
+               differences on 16 cells (seed 1).
+  hex27        HEX27 linear elasticity, 48^3 cells on the unit cube, 27-point
+               Gauss rule, E = exp(N(0, 0.25^2)) per cell (seed 0), nu = 0.3,
+               clamped z = 0, body force (0, 0, -1); compliance gradient
+               w.r.t. the E field.
 """
 import argparse
 import json
@@ -60,10 +77,11 @@ def build_problem(nx, ny, nz):
 
 
 def sell_bytes(dev, elem_bytes, vec=3):
-    """Algorithmic bytes of one fused SpMV launch: SELL values + column indices once,
-    z and p_old read once per row (gathers counted once), p_new and q written."""
+    """Algorithmic bytes of one fused SpMV launch: the matrix blocks (values + one column
+    index each, padding not counted) once, z and p_old read once per row (gathers counted
+    once), p_new and q written."""
     n = dev.n
-    return dev.sell_blocks * (vec * vec * elem_bytes + 4) + n * elem_bytes * 4
+    return dev.nnzb * (vec * vec * elem_bytes + 4) + n * elem_bytes * 4
 
 
 def run(args):
@@ -226,9 +244,328 @@ def cpu_baseline(dev, gpu_iterations):
                       f"scaled by nnz x{scale_nnz:.0f} and by the {gpu_iterations} iterations of the FP64 GPU solve"}
 
 
+# ---------------------------------------------------------------- other workloads
+def _emit(workload, value_s, dtype, extra):
+    out = {"metric": METRIC, "value": value_s, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+           "config": {"workload": workload, "global_batch": 1, "parallelism": "single GPU"}}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def _rel(a, b):
+    import torch
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+def _timed_solve(p, opts, **kw):
+    import torch
+    from jax_fem_amd.solver import solver
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    sol = solver(p, dict(opts, **kw))[0]
+    torch.cuda.synchronize()
+    return time.perf_counter() - t, sol.clone(), p._solve_stats
+
+
+def _paths(args):
+    return {'fp64': {'jax_solver': {'precision': 'fp64'}},
+            'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}}}
+
+
+def workload_poisson(args):
+    """3D Poisson -lap u = f, HEX8 32^3, f nodal N(0,1) interpolated at the quadrature points."""
+    import numpy as np
+    import torch
+    from jax_fem_amd.generate_mesh import Mesh, box_mesh
+    from jax_fem_amd.problem import Problem
+
+    class PoissonSource(Problem):
+        def get_tensor_map(self):
+            return lambda u_grad, f: u_grad
+
+        def get_mass_map(self):
+            return lambda u, x, f: -f.reshape(1)
+
+    torch.cuda.set_device(0)
+    n = 32
+    m = box_mesh(n, n, n, 1., 1., 1.)
+    mesh = Mesh(m.points, m.cells_dict['hexahedron'])
+    left = lambda x: np.isclose(x[0], 0., atol=1e-5)
+    p = PoissonSource(mesh, vec=1, dim=3, dirichlet_bc_info=[[left], [0], [lambda x: 0.]])
+    f_node = np.random.default_rng(0).standard_normal(len(m.points))
+    fe = p.fes[0]
+    f_q = torch.as_tensor(np.asarray(fe.shape_vals) @ f_node[np.asarray(fe.cells)].T).T.contiguous()
+    p.internal_vars = [f_q]
+    ref = _timed_solve(p, {'jax_solver': {'precision': 'fp64', 'tol': 1e-12, 'atol': 0.}})[1]
+    res = {}
+    for name, o in _paths(args).items():
+        for _ in range(args.warmup):
+            _timed_solve(p, o)
+        runs = [_timed_solve(p, o) for _ in range(args.steps)]
+        res[name] = dict(s_per_newton_step=statistics.median(r[0] for r in runs), rel_err=_rel(runs[-1][1], ref),
+                         pcg_iterations=runs[-1][2]['linear'][0]['iterations'], sol=runs[-1][1])
+    res['mixed']['rel_err_vs_fp64'] = _rel(res['mixed']['sol'], res['fp64']['sol'])
+    for name in res:
+        del res[name]['sol']
+        log(f"poisson {name}: {res[name]}")
+    extra = {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'], "detail": res}
+    if args.cpu_baseline:
+        from oracle import fem as OF
+        o = OF.Problem(m.points, m.cells_dict['hexahedron'], 1, mat=OF.Material('poisson', k=1.),
+                       bc_info=[[left], [0], [lambda x: 0.]])
+        o.add_mass(-np.einsum('qa,ca->cq', o.sv, f_node[o.cells])[..., None])
+        t = time.perf_counter()
+        u = o.newton(linear='pcg')
+        dt = time.perf_counter() - t
+        extra["cpu_baseline"] = {"value": dt, "unit": "s per Newton step", "cores": 1, "kind": "port",
+                                 "sample": f"full workload: oracle FP64 assembly + Jacobi-PCG ({o.newton_its} its)",
+                                 "rel_err_vs_gpu_fp64": float(np.linalg.norm(u - ref.cpu().numpy().reshape(-1)) /
+                                                              np.linalg.norm(u))}
+    _emit(f"HEX8 Poisson 32^3 unit cube, {p.fes[0].num_total_nodes} dofs, Jacobi-PCG rtol 1e-10",
+          res['mixed']['s_per_newton_step'], "f32+f64", extra)
+
+
+def workload_neo_hookean(args):
+    """Neo-Hookean cube 64^3, top face pulled to u_z = 0.1 in 5 load steps."""
+    import numpy as np
+    import torch
+    from jax_fem_amd.generate_mesh import Mesh, box_mesh
+    from jax_fem_amd.problem import Problem
+
+    E, nu = 10., 0.3
+    mu, kappa = E / (2. * (1. + nu)), E / (3. * (1. - 2. * nu))
+
+    class NeoHookeanCube(Problem):
+        # tensor map of the reference's hyperelasticity example (P = d psi / dF)
+        def get_tensor_map(self):
+            def psi(F):
+                J = torch.linalg.det(F)
+                I1 = torch.trace(F.T @ F)
+                return mu / 2. * (J ** (-2. / 3.) * I1 - 3.) + kappa / 2. * (J - 1.) ** 2
+            P = torch.func.grad(psi)
+            return lambda u_grad: P(u_grad + torch.eye(3, dtype=u_grad.dtype, device=u_grad.device))
+
+    torch.cuda.set_device(0)
+    n = args.nh_cells
+    m = box_mesh(n, n, n, 1., 1., 1.)
+    mesh = Mesh(m.points, m.cells_dict['hexahedron'])
+    bottom = lambda x: np.isclose(x[2], 0., atol=1e-5)
+    top = lambda x: np.isclose(x[2], 1., atol=1e-5)
+
+    def bc(k):
+        disp = 0.1 * k / 5
+        return [[bottom] * 3 + [top], [0, 1, 2, 2], [lambda x: 0.] * 3 + [lambda x: disp]]
+
+    p = NeoHookeanCube(mesh, vec=3, dim=3, dirichlet_bc_info=bc(1))
+    res, sols = {}, {}
+    for name, o in _paths(args).items():
+        o = dict(o, tol=1e-8, rel_tol=1e-14)
+        for _ in range(args.warmup):     # untimed: one Newton solve of the first load step
+            p.fes[0].update_Dirichlet_boundary_conditions(bc(1))
+            _timed_solve(p, o)
+        tot_t, tot_newton, tot_pcg = 0., 0, 0
+        sol = None
+        for k in range(1, 6):
+            p.fes[0].update_Dirichlet_boundary_conditions(bc(k))
+            kw = {} if sol is None else {'initial_guess': [sol]}
+            dt, sol, st = _timed_solve(p, o, **kw)
+            tot_t += dt
+            tot_newton += st['newton_iterations']
+            tot_pcg += sum(li['iterations'] for li in st['linear'])
+            log(f"neo_hookean {name} step {k}: {st['newton_iterations']} Newton its, |R| = {st['res_norm']:.2e}, "
+                f"{dt:.3f} s")
+        sols[name] = sol
+        res[name] = dict(s_total=tot_t, newton_iterations=tot_newton, s_per_newton_step=tot_t / tot_newton,
+                         pcg_iterations=tot_pcg, final_residual=st['res_norm'])
+    res['mixed']['rel_err_vs_fp64'] = _rel(sols['mixed'], sols['fp64'])
+    log(f"neo_hookean: {res}")
+    _emit(f"HEX8 Neo-Hookean cube {n}^3, {p.fes[0].num_total_nodes * 3} dofs, 5 load steps, Newton |R| <= 1e-8",
+          res['mixed']['s_per_newton_step'], "f32+f64",
+          {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'], "detail": res})
+
+
+def _compliance_var(p, f, u):
+    """Variational compliance 2 f.u - u.K u with u.K u summed from positive cell energies: its error is
+    second order in the solve error, so central differences of it resolve tiny parameter steps."""
+    import torch
+    dev = p.device()
+    dev.prepare()
+    re = dev.cell_internal_force(u)                        # (C, N, V): K_e u_e
+    ue = u.reshape(-1, dev.V)[dev.dm.cells.long()]
+    return float(2. * torch.dot(f, u) - (re * ue).sum())
+
+
+def workload_simp(args):
+    """SIMP compliance gradient on the 160x80x40 cantilever; adjoint vs central differences."""
+    import numpy as np
+    import torch
+    from jax_fem_amd.generate_mesh import Mesh, box_mesh
+    from jax_fem_amd.problem import Problem
+    from jax_fem_amd.solver import ad_wrapper
+
+    nu, Emin, E0, penal = 0.3, 1e-3, 1.0, 3.0
+
+    class SIMP(Problem):
+        def get_tensor_map(self):
+            def stress(u_grad, theta):
+                E = Emin + (E0 - Emin) * theta[0] ** penal
+                mu_, lm = E / (2. * (1. + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+                eps = 0.5 * (u_grad + u_grad.T)
+                return lm * torch.trace(eps) * torch.eye(3, dtype=u_grad.dtype) + 2 * mu_ * eps
+            return stress
+
+        def get_surface_maps(self):
+            return [lambda u, x: torch.tensor([0., 0., 1.], dtype=torch.float64)]
+
+        def set_params(self, params):
+            self.internal_vars = [params.reshape(-1, 1)[:, None, :].expand(-1, self.fes[0].num_quads, 1)]
+
+    torch.cuda.set_device(0)
+    Nx, Ny, Nz = args.simp_cells
+    Lx, Ly, Lz = 2., 1., 0.5
+    m = box_mesh(Nx, Ny, Nz, Lx, Ly, Lz)
+    mesh = Mesh(m.points, m.cells_dict['hexahedron'])
+    hy, hz = Ly / Ny, Lz / Nz
+    left = lambda x: np.isclose(x[0], 0., atol=1e-5)
+    patch = lambda x: np.isclose(x[0], Lx, atol=1e-5) & (np.abs(x[1] - Ly / 2) <= hy + 1e-6) & (x[2] <= 2 * hz + 1e-6)
+    p = SIMP(mesh, vec=3, dim=3, dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda x: 0.] * 3], location_fns=[patch])
+    assert len(p.boundary_inds_list[0]) == 4, len(p.boundary_inds_list[0])
+    C = p.num_cells
+    theta0 = torch.as_tensor(0.5 + np.random.default_rng(0).uniform(-0.05, 0.05, C), device='cuda')
+    res, grads = {}, {}
+    for name, o in _paths(args).items():
+        fwd = ad_wrapper(p, o, o)
+        fw, bw = [], []
+        for it in range(args.warmup + args.steps):
+            th = theta0.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            u = fwd(th)[0]
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            Cval = torch.dot(-p.device().load, u.reshape(-1))
+            Cval.backward()
+            torch.cuda.synchronize()
+            if it >= args.warmup:
+                fw.append(t1 - t)
+                bw.append(time.perf_counter() - t1)
+        grads[name] = th.grad.detach().clone()
+        res[name] = dict(s_per_newton_step=statistics.median(fw), s_per_adjoint_gradient=statistics.median(bw),
+                         compliance=float(Cval), adjoint_pcg_iterations=p._adjoint_stats['iterations'])
+        log(f"simp {name}: {res[name]}")
+    res['mixed']['grad_rel_diff_vs_fp64'] = _rel(grads['mixed'], grads['fp64'])
+    # central differences of the variational compliance on 16 seeded cells (FP64 solves to rtol 1e-11)
+    cells = np.random.default_rng(1).choice(C, 16, replace=False)
+    h = args.fd_h
+    fopt = {'jax_solver': {'precision': 'fp64', 'tol': 1e-11, 'atol': 0.}}
+    f = -p.device().load
+    fd = []
+    for e in cells:
+        vals = []
+        for sgn in (1., -1.):
+            th = theta0.clone()
+            th[e] += sgn * h
+            p.set_params(th)
+            u = _timed_solve(p, fopt)[1].reshape(-1)
+            vals.append(_compliance_var(p, f, u))
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    fd = np.array(fd)
+    adj = grads['fp64'][torch.as_tensor(cells, device='cuda')].cpu().numpy()
+    fd_err = float(np.max(np.abs(adj - fd) / np.abs(fd)))
+    log(f"simp FD (h={h}) vs adjoint on 16 cells: max rel diff {fd_err:.2e}")
+    _emit(f"HEX8 SIMP cantilever {Nx}x{Ny}x{Nz} on {Lx}x{Ly}x{Lz}, {p.fes[0].num_total_nodes * 3} dofs, "
+          f"compliance gradient", res['mixed']['s_per_adjoint_gradient'], "f32+f64",
+          {"fp64_s_per_adjoint_gradient": res['fp64']['s_per_adjoint_gradient'],
+           "fd_check": {"cells": cells.tolist(), "h": h, "max_rel_diff": fd_err, "fd": fd.tolist(),
+                        "adjoint": adj.tolist()}, "detail": res})
+
+
+def workload_hex27(args):
+    """HEX27 elasticity 48^3 with a log-normal per-cell Young's modulus; gradient of compliance w.r.t. E."""
+    import numpy as np
+    import torch
+    from jax_fem_amd.generate_mesh import Mesh, box_mesh_hex27
+    from jax_fem_amd.problem import Problem
+    from jax_fem_amd.solver import ad_wrapper
+
+    nu = 0.3
+
+    class Hex27Elastic(Problem):
+        def get_tensor_map(self):
+            def stress(u_grad, E):
+                mu_, lm = E / (2. * (1. + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+                eps = 0.5 * (u_grad + u_grad.T)
+                return lm * torch.trace(eps) * torch.eye(3, dtype=u_grad.dtype) + 2 * mu_ * eps
+            return stress
+
+        def get_mass_map(self):
+            return lambda u, x, E: torch.tensor([0., 0., 1.], dtype=torch.float64)
+
+        def set_params(self, params):
+            self.internal_vars = [params.reshape(-1, 1).expand(-1, self.fes[0].num_quads)]
+
+    torch.cuda.set_device(0)
+    n = args.hex27_cells
+    m = box_mesh_hex27(n, n, n, 1., 1., 1.)
+    mesh = Mesh(m.points, m.cells_dict['hexahedron27'])
+    bottom = lambda x: np.isclose(x[2], 0., atol=1e-5)
+    t0 = time.perf_counter()
+    p = Hex27Elastic(mesh, vec=3, dim=3, ele_type='HEX27', quadrature_order=4,
+                     dirichlet_bc_info=[[bottom] * 3, [0, 1, 2], [lambda x: 0.] * 3])
+    C = p.num_cells
+    E = torch.as_tensor(np.exp(np.random.default_rng(0).normal(0., 0.25, C)), device='cuda')
+    p.set_params(E)
+    dev = p.device()
+    dev.prepare()
+    log(f"hex27 setup {time.perf_counter() - t0:.1f}s: {C} cells, {dev.n} dofs, {dev.nnzb * 9} nnz, "
+        f"{dev.sell_blocks} SELL blocks ({dev.sell_blocks / dev.nnzb - 1:.1%} padding)")
+    res, grads, sols = {}, {}, {}
+    for name, o in _paths(args).items():
+        o = {'jax_solver': dict(o['jax_solver'], timing=True)}
+        fwd = ad_wrapper(p, o, o)
+        fw, bw = [], []
+        for it in range(args.warmup + args.steps):
+            e = E.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            u = fwd(e)[0]
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            Cval = torch.dot(-dev.load, u.reshape(-1))
+            Cval.backward()
+            torch.cuda.synchronize()
+            if it >= args.warmup:
+                fw.append(t1 - t)
+                bw.append(time.perf_counter() - t1)
+        grads[name] = e.grad.detach().clone()
+        sols[name] = u.detach().clone()
+        lin = p._solve_stats['linear'][0]
+        res[name] = dict(s_per_newton_step=statistics.median(fw), s_per_adjoint_gradient=statistics.median(bw),
+                         pcg_iterations=lin['iterations'], spmv_ms=lin['spmv_ms'], update_ms=lin['update_ms'],
+                         timing=p._solve_stats['timing'])
+        log(f"hex27 {name}: {res[name]}")
+    res['mixed']['rel_err_vs_fp64'] = _rel(sols['mixed'], sols['fp64'])
+    res['mixed']['grad_rel_diff_vs_fp64'] = _rel(grads['mixed'], grads['fp64'])
+    # compliance self-adjointness: dC/dE_e = -u_e^T K_e(1) u_e = -(energy of cell e) / E_e * 2 ... checked by sign
+    assert bool((grads['fp64'] < 0).all()), "compliance gradient w.r.t. E must be negative"
+    spmv_b = {nm: sell_bytes(dev, es) for nm, es in (('mixed', 4), ('fp64', 8))}
+    _emit(f"HEX27 elasticity {n}^3 unit cube, 27-pt Gauss, {dev.n} dofs, log-normal E field",
+          res['mixed']['s_per_newton_step'], "f32+f64",
+          {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'],
+           "spmv_GBs": {nm: spmv_b[nm] / (res[nm]['spmv_ms'] * 1e-3) / 1e9 for nm in spmv_b if res[nm]['spmv_ms']},
+           "detail": res})
+
+
+WORKLOADS = {'poisson': workload_poisson, 'neo_hookean': workload_neo_hookean, 'simp': workload_simp,
+             'hex27': workload_hex27}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--workload', choices=['cantilever', 'poisson', 'neo_hookean', 'simp', 'hex27'],
+                    default='cantilever')
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--cells', type=int, nargs=3, default=[128, 64, 64])
@@ -236,6 +573,10 @@ def main():
     ap.add_argument('--no-cpu-baseline', dest='cpu_baseline', action='store_false')
     ap.add_argument('--only', choices=['fp64', 'mixed'], default=None,
                     help='time one path only (no adjoint, no summary line); for profiling')
+    ap.add_argument('--nh-cells', type=int, default=64)
+    ap.add_argument('--simp-cells', type=int, nargs=3, default=[160, 80, 40])
+    ap.add_argument('--hex27-cells', type=int, default=48)
+    ap.add_argument('--fd-h', type=float, default=1e-6)
     ap.add_argument('--calib-copy-mib', type=int, default=0,
                     help='with --only: also run a plain copy of this many MiB (counter calibration)')
     args = ap.parse_args()
@@ -243,7 +584,10 @@ def main():
         print(f"bench.py measures one GPU: this build runs on a single MI355X (--gpus {args.gpus} requested); "
               "multi-GPU sharding is described in DESIGN.md but not implemented", file=sys.stderr)
         sys.exit(2)
-    run(args)
+    if args.workload == 'cantilever':
+        run(args)
+    else:
+        WORKLOADS[args.workload](args)
 
 
 if __name__ == '__main__':

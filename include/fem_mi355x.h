@@ -110,6 +110,9 @@ typedef struct {
   const int32_t* contrib_idx;    /* (num_cells*N*N) cell-block id c*N*N+a*N+b */
   const int32_t* slice_ptr;      /* (num_slices + 1) in blocks                */
   const int32_t* sell_col;       /* (sell_blocks) column node per SELL block  */
+  const int32_t* sell_row;       /* (num_slices*64) block row held by each slice
+                                    lane, -1 for padding lanes               */
+  const int32_t* sell_pos;       /* (num_rows) lane s*64+l holding each row   */
 } fem_pattern_t;
 
 /* FP64 matrix-free tangent operator y = dR/du(u) x on free dofs, y = x on
@@ -180,9 +183,14 @@ int fem_pattern_rows(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32
 int fem_pattern_fill(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32_t* contrib_idx,
                      const int32_t* row_ptr, int32_t* col_idx, int32_t* contrib_ptr,
                      int32_t* diag_slot, void* stream);
-/* 64*max(row_len) per slice, to be scanned into slice_ptr. */
-int fem_sell_slice_sizes(int64_t num_rows, const int32_t* row_ptr, int32_t* slice_blocks, void* stream);
-/* Fill sell_col from the block CSR pattern. */
+/* Row order of the SELL slices: within each window of 2048 rows, rows sorted
+ * stably by decreasing length (SELL-C-sigma), so a slice holds rows of equal
+ * length and carries no padding.  sell_row (num_slices*64), sell_pos (num_rows). */
+int fem_sell_order(int64_t num_rows, const int32_t* row_ptr, int32_t* sell_row, int32_t* sell_pos, void* stream);
+/* 64*max(row_len) per slice (rows taken in sell_row order), to be scanned into slice_ptr. */
+int fem_sell_slice_sizes(int64_t num_rows, const int32_t* row_ptr, const int32_t* sell_row, int32_t* slice_blocks,
+                         void* stream);
+/* Fill sell_col from the block CSR pattern (needs row_ptr, col_idx, slice_ptr, sell_row). */
 int fem_sell_fill(const fem_pattern_t* pat, int32_t* sell_col, void* stream);
 
 /* ---- element kernels (fe.py:112-218, problem.py:189-356) --------------- */

@@ -81,15 +81,16 @@ __device__ inline T block_sum(T v, T* smem) {
 // each slice has W independent streams of coalesced value loads in flight.
 // `xat(c, xc)` loads the V entries of the input vector at block column c.
 // DIFF: difference form (diagonal slot holds the row-sum block; off-diagonal
-// blocks multiply x_c - x_row).  The arithmetic is FP64 whatever the storage
+// blocks multiply x_c - x_row).  `row` is the block row of this lane
+// (sell_row; -1 on padding lanes, whose slots hold zeros).  The arithmetic is FP64 whatever the storage
 // type T: the difference of two FP32 inputs is exact in FP64 and the products
 // are accumulated without FP32 rounding (the kernel is HBM-bound, so the FP64
 // FMAs cost no time).  Returns this wave's partial sums.
 template <int V, bool DIFF, int W, int U, typename T, typename XAt>
 __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __restrict__ vals, int64_t s, int w,
-                                          int lane, const XAt& xat, const double (&xr)[V], double (&acc)[V]) {
+                                          int lane, int64_t row, const XAt& xat, const double (&xr)[V],
+                                          double (&acc)[V]) {
   constexpr int VV = V * V;
-  const int64_t row = s * kSlice + lane;
   const int base = pat.slice_ptr[s];
   const int L = (pat.slice_ptr[s + 1] - base) >> 6;
   const int32_t* cp = pat.sell_col + base + lane;
@@ -111,11 +112,12 @@ __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __r
   }
 }
 
-// Tuned slice split: FP32 4 waves x unroll 2, FP64 8 waves x unroll 1.
+// Slice split chosen by a sweep of (W, U) on the 1.6M-dof HEX8 elasticity
+// matrix: FP32 4 waves x unroll 2, FP64 2 waves x unroll 2.
 template <typename T>
 struct SliceCfg {
-  static constexpr int W = sizeof(T) == 4 ? 4 : 8;
-  static constexpr int U = sizeof(T) == 4 ? 2 : 1;
+  static constexpr int W = sizeof(T) == 4 ? 4 : 2;
+  static constexpr int U = 2;
   static constexpr int NT = 64 * W;
 };
 

@@ -221,31 +221,66 @@ __global__ void k_pattern_fill(int64_t num_nodes, int32_t N, const int32_t* __re
   }
 }
 
-__global__ void k_slice_sizes(int64_t num_rows, const int32_t* __restrict__ row_ptr,
+// One block per window of kSortWindow rows: stable rank of each row by
+// decreasing length (padding rows, length -1, go last).  Each thread ranks
+// kSortWindow / kSortThreads rows against the whole window held in LDS.
+constexpr int kSortWindow = 2048;
+constexpr int kSortThreads = 1024;
+__global__ __launch_bounds__(kSortThreads) void k_sell_order(int64_t num_rows, int64_t num_lanes,
+                                                             const int32_t* __restrict__ row_ptr,
+                                                             int32_t* __restrict__ sell_row,
+                                                             int32_t* __restrict__ sell_pos) {
+  __shared__ int len[kSortWindow];
+  const int64_t base = blockIdx.x * (int64_t)kSortWindow;
+  for (int i = threadIdx.x; i < kSortWindow; i += kSortThreads) {
+    const int64_t r = base + i;
+    len[i] = r < num_rows ? row_ptr[r + 1] - row_ptr[r] : -1;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < kSortWindow; i += kSortThreads) {
+    const int my = len[i];
+    int rank = 0;
+    for (int j = 0; j < kSortWindow; ++j) {
+      const int l = len[j];
+      rank += (l > my) || (l == my && j < i);
+    }
+    const int64_t r = base + i;
+    const int64_t pos = base + rank;
+    // real rows rank first in their window, so their positions are < num_rows <= num_lanes
+    if (pos < num_lanes) sell_row[pos] = my >= 0 ? (int32_t)r : -1;
+    if (my >= 0) sell_pos[r] = (int32_t)pos;
+  }
+}
+
+__global__ void k_slice_sizes(const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ sell_row,
                               int32_t* __restrict__ slice_blocks, int64_t num_slices) {
   for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < num_slices;
        s += (int64_t)gridDim.x * blockDim.x) {
     int L = 0;
-    const int64_t r1 = min((s + 1) * kSlice, num_rows);
-    for (int64_t r = s * kSlice; r < r1; ++r) L = max(L, row_ptr[r + 1] - row_ptr[r]);
+    for (int l = 0; l < kSlice; ++l) {
+      const int r = sell_row[s * kSlice + l];
+      if (r >= 0) L = max(L, row_ptr[r + 1] - row_ptr[r]);
+    }
     slice_blocks[s] = L * kSlice;
   }
 }
 
-__global__ void k_sell_fill(int64_t num_rows, int64_t num_slices, const int32_t* __restrict__ row_ptr,
+// Padding slots hold col = row (col 0 on padding lanes) and zero values.
+__global__ void k_sell_fill(int64_t num_slices, const int32_t* __restrict__ row_ptr,
                             const int32_t* __restrict__ col_idx, const int32_t* __restrict__ slice_ptr,
-                            int32_t* __restrict__ sell_col) {
+                            const int32_t* __restrict__ sell_row, int32_t* __restrict__ sell_col) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < num_slices * kSlice;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t s = t / kSlice;
     const int lane = (int)(t % kSlice);
     const int base = slice_ptr[s];
     const int L = (slice_ptr[s + 1] - base) / kSlice;
-    const bool real = t < num_rows;
-    const int rb = real ? row_ptr[t] : 0;
-    const int len = real ? row_ptr[t + 1] - rb : 0;
+    const int row = sell_row[t];
+    const bool real = row >= 0;
+    const int rb = real ? row_ptr[row] : 0;
+    const int len = real ? row_ptr[row + 1] - rb : 0;
     for (int k = 0; k < L; ++k)
-      sell_col[base + (int64_t)k * kSlice + lane] = k < len ? col_idx[rb + k] : (real ? (int32_t)t : 0);
+      sell_col[base + (int64_t)k * kSlice + lane] = k < len ? col_idx[rb + k] : (real ? row : 0);
   }
 }
 
@@ -340,22 +375,32 @@ int fem_pattern_fill(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32
   return FEM_OK;
 }
 
-int fem_sell_slice_sizes(int64_t num_rows, const int32_t* row_ptr, int32_t* slice_blocks, void* stream) {
-  FEM_CHECK_ARG(num_rows > 0 && row_ptr && slice_blocks, "fem_sell_slice_sizes: bad arguments");
+int fem_sell_order(int64_t num_rows, const int32_t* row_ptr, int32_t* sell_row, int32_t* sell_pos, void* stream) {
+  FEM_CHECK_ARG(num_rows > 0 && row_ptr && sell_row && sell_pos, "fem_sell_order: bad arguments");
+  const int64_t lanes = ceil_div(num_rows, kSlice) * kSlice;
+  hipLaunchKernelGGL(k_sell_order, dim3((unsigned)ceil_div(num_rows, kSortWindow)), dim3(kSortThreads), 0,
+                     as_stream(stream), num_rows, lanes, row_ptr, sell_row, sell_pos);
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_sell_slice_sizes(int64_t num_rows, const int32_t* row_ptr, const int32_t* sell_row, int32_t* slice_blocks,
+                         void* stream) {
+  FEM_CHECK_ARG(num_rows > 0 && row_ptr && sell_row && slice_blocks, "fem_sell_slice_sizes: bad arguments");
   const int64_t ns = ceil_div(num_rows, kSlice);
-  hipLaunchKernelGGL(k_slice_sizes, dim3(grid_for(ns, 256)), dim3(256), 0, as_stream(stream), num_rows, row_ptr,
+  hipLaunchKernelGGL(k_slice_sizes, dim3(grid_for(ns, 256)), dim3(256), 0, as_stream(stream), row_ptr, sell_row,
                      slice_blocks, ns);
   FEM_LAUNCHED();
   return FEM_OK;
 }
 
 int fem_sell_fill(const fem_pattern_t* pat, int32_t* sell_col, void* stream) {
-  FEM_CHECK_ARG(pat && pat->row_ptr && pat->col_idx && pat->slice_ptr && sell_col,
+  FEM_CHECK_ARG(pat && pat->row_ptr && pat->col_idx && pat->slice_ptr && pat->sell_row && sell_col,
                 "fem_sell_fill: bad arguments");
   FEM_CHECK_ARG(pat->num_slices == ceil_div(pat->num_rows, kSlice), "fem_sell_fill: num_slices mismatch");
   hipLaunchKernelGGL(k_sell_fill, dim3(grid_for(pat->num_slices * kSlice, 256)), dim3(256), 0,
-                     as_stream(stream), pat->num_rows, pat->num_slices, pat->row_ptr, pat->col_idx,
-                     pat->slice_ptr, sell_col);
+                     as_stream(stream), pat->num_slices, pat->row_ptr, pat->col_idx, pat->slice_ptr, pat->sell_row,
+                     sell_col);
   FEM_LAUNCHED();
   return FEM_OK;
 }

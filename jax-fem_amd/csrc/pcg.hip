@@ -71,8 +71,8 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, co
   const T beta = T(sc->beta);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t s = remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-  const int64_t row = s * kSlice + lane;
-  const bool real = row < pat.num_rows;
+  const int64_t row = pat.sell_row[s * kSlice + lane];
+  const bool real = row >= 0;
   auto xat = [&](int64_t c, T (&xc)[V]) {
 #pragma unroll
     for (int j = 0; j < V; ++j) xc[j] = z[c * V + j] + beta * p_old[c * V + j];
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, co
   double xr[V], acc[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) xr[j] = double(pr[j]);
-  sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, xat, xr, acc);
+  sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, row, xat, xr, acc);
   if (w > 0) {
 #pragma unroll
     for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
@@ -105,28 +105,36 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, co
   if (threadIdx.x == 0) partA[blockIdx.x] = d;
 }
 
-// RA: alpha = rz / sum(partA)
+// RA: alpha = rz / sum(partA).  Eight independent loads per thread are in
+// flight at once (the sum is latency-bound); the order of the sum is fixed.
 __global__ __launch_bounds__(1024) void k_pcg_ra(const double* __restrict__ partA, int nA, PcgScalars* sc) {
   __shared__ double smem[16];
+  double a[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) a[u] = 0.0;
+  for (int base = 0; base < nA; base += 8 * 1024) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = base + u * 1024 + (int)threadIdx.x;
+      if (i < nA) a[u] += partA[i];
+    }
+  }
+  const double t = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  const double pq = block_sum<1024>(t, smem);
   if (sc->done) return;
-  double a = 0.0;
-  for (int i = threadIdx.x; i < nA; i += 1024) a += partA[i];
-  const double pq = block_sum<1024>(a, smem);
   if (threadIdx.x == 0) sc->alpha = sc->rz / pq;
 }
 
-// RB: rz, rr of iteration k+1; stop test; beta.
+// RB: rz, rr of iteration k+1; stop test; beta.  nB <= 1024.
 __global__ __launch_bounds__(1024) void k_pcg_rb(const double* __restrict__ partB, int nB, PcgScalars* sc, int k,
                                                  double tol2, int maxiter) {
   __shared__ double smem[16];
-  if (sc->done) return;
-  double a0 = 0.0, a1 = 0.0;
-  for (int i = threadIdx.x; i < nB; i += 1024) {
-    a0 += partB[i];
-    a1 += partB[nB + i];
-  }
+  const int i = threadIdx.x;
+  const double a0 = i < nB ? partB[i] : 0.0;
+  const double a1 = i < nB ? partB[nB + i] : 0.0;
   const double rz = block_sum<1024>(a0, smem);
   const double rr = block_sum<1024>(a1, smem);
+  if (sc->done) return;
   if (threadIdx.x == 0) {
     const bool bad = !isfinite(rr) || !isfinite(rz);
     sc->beta = rz / sc->rz;
@@ -243,8 +251,9 @@ __global__ __launch_bounds__(256) void k_spmv_x(fem_pattern_t pat, const T* __re
   constexpr int VV = V * V;
   const int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (row >= pat.num_rows) return;
-  const int64_t s = row >> 6;
-  const int lane = (int)(row & 63);
+  const int64_t pos = pat.sell_pos[row];
+  const int64_t s = pos >> 6;
+  const int lane = (int)(pos & 63);
   const int base = pat.slice_ptr[s];
   const int L = (pat.slice_ptr[s + 1] - base) >> 6;
   const int32_t* cp = pat.sell_col + base + lane;
@@ -493,7 +502,8 @@ int64_t fem_pcg_ws_bytes(const fem_pattern_t* pat, int32_t precision) {
 int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv, const fem_operator_t* op,
                   const double* b, double* x, const fem_pcg_options_t* opt, void* ws, void* host_status,
                   fem_pcg_result_t* result, void* stream) {
-  FEM_CHECK_ARG(pat && pat->slice_ptr && pat->sell_col && pat->num_rows > 0, "fem_pcg_solve: bad pattern");
+  FEM_CHECK_ARG(pat && pat->slice_ptr && pat->sell_col && pat->sell_row && pat->sell_pos && pat->num_rows > 0,
+                "fem_pcg_solve: bad pattern");
   FEM_CHECK_ARG(vals && b && x && opt && ws && host_status && result, "fem_pcg_solve: NULL argument");
   FEM_CHECK_ARG(opt->precision == FEM_F32 || opt->precision == FEM_F64, "fem_pcg_solve: bad precision");
   FEM_CHECK_ARG(opt->precision == FEM_F64 || op, "fem_pcg_solve: FP32 mode needs the FP64 operator");

@@ -30,7 +30,7 @@ __global__ void k_gather_nodal(int64_t num_nodes, int V, const int32_t* __restri
 }
 
 // ------------------------------------------------------- matrix values
-// One thread per row: for every SELL slot k, sum the cell blocks that land in
+// One thread per slice lane: for every SELL slot k, sum the cell blocks that land in
 // block (row, col_k) and store them (zeros in padding slots).  DIFF stores
 // the row-sum block R_row = sum_k A(row, k) (accumulated in TS) in the
 // diagonal slot instead of the diagonal block (difference form of the SpMV).
@@ -45,10 +45,11 @@ __global__ __launch_bounds__(256) void k_gather_matrix(fem_pattern_t pat, const 
   const int lane = (int)(t & 63);
   const int base = pat.slice_ptr[s];
   const int L = (pat.slice_ptr[s + 1] - base) >> 6;
-  const bool real = t < pat.num_rows;
-  const int rb = real ? pat.row_ptr[t] : 0;
-  const int len = real ? pat.row_ptr[t + 1] - rb : 0;
-  const int ds = real ? pat.diag_slot[t] : -1;
+  const int row = pat.sell_row[t];
+  const bool real = row >= 0;
+  const int rb = real ? pat.row_ptr[row] : 0;
+  const int len = real ? pat.row_ptr[row + 1] - rb : 0;
+  const int ds = real ? pat.diag_slot[row] : -1;
   T* vp = vals + (size_t)base * VV + lane;
   TS rsum[VV];
   TS diag[V];
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(256) void k_gather_matrix(fem_pattern_t pat, const 
   }
   if (dinv && real) {
 #pragma unroll
-    for (int c = 0; c < V; ++c) dinv[t * V + c] = diag[c] != TS(0) ? T(TS(1) / diag[c]) : T(0);
+    for (int c = 0; c < V; ++c) dinv[(int64_t)row * V + c] = diag[c] != TS(0) ? T(TS(1) / diag[c]) : T(0);
   }
 }
 
@@ -100,8 +101,9 @@ __global__ void k_bc_rows(fem_pattern_t pat, T* __restrict__ vals, int64_t n_bc,
     const int64_t d = dofs[i];
     const int64_t row = d / V;
     const int ci = (int)(d % V);
-    const int64_t s = row >> 6;
-    const int lane = (int)(row & 63);
+    const int64_t pos = pat.sell_pos[row];
+    const int64_t s = pos >> 6;
+    const int lane = (int)(pos & 63);
     const int base = pat.slice_ptr[s];
     const int L = (pat.slice_ptr[s + 1] - base) >> 6;
     T* vp = vals + (size_t)base * VV + lane;
@@ -119,8 +121,9 @@ __global__ void k_jacobi(fem_pattern_t pat, const T* __restrict__ vals, T* __res
   constexpr int VV = V * V;
   for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < pat.num_rows;
        row += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = row >> 6;
-    const int lane = (int)(row & 63);
+    const int64_t pos = pat.sell_pos[row];
+    const int64_t s = pos >> 6;
+    const int lane = (int)(pos & 63);
     const int base = pat.slice_ptr[s];
     const T* vp = vals + (size_t)base * VV + lane;
     const int ds = pat.diag_slot[row];
@@ -149,8 +152,8 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_spmv(fem_pattern_t pat, con
   __shared__ double red[W][kSlice][V];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t s = blockIdx.x;
-  const int64_t row = s * kSlice + lane;
-  const bool real = row < pat.num_rows;
+  const int64_t row = pat.sell_row[s * kSlice + lane];
+  const bool real = row >= 0;
   auto xat = [&](int64_t c, T (&xc)[V]) {
 #pragma unroll
     for (int j = 0; j < V; ++j) xc[j] = x[c * V + j];
@@ -160,7 +163,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_spmv(fem_pattern_t pat, con
   double xr[V], acc[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) xr[j] = double(xt[j]);
-  sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, xat, xr, acc);
+  sell_slice_partial<V, DIFF, W, U, T>(pat, vals, s, w, lane, row, xat, xr, acc);
   if (w > 0) {
 #pragma unroll
     for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
@@ -190,8 +193,9 @@ __global__ void k_export_csr(fem_pattern_t pat, const T* __restrict__ vals, int 
     const int rb = pat.row_ptr[row];
     const int len = pat.row_ptr[row + 1] - rb;
     const int ds = pat.diag_slot[row];
-    const int64_t s = row >> 6;
-    const int lane = (int)(row & 63);
+    const int64_t pos = pat.sell_pos[row];
+    const int64_t s = pos >> 6;
+    const int lane = (int)(pos & 63);
     const T* vp = vals + (size_t)pat.slice_ptr[s] * VV + lane;
     for (int ci = 0; ci < V; ++ci) {
       const int64_t start = (int64_t)VV * rb + (int64_t)ci * V * len;
@@ -257,7 +261,8 @@ static int with_vt(int vec, int precision, F&& f) {
 }
 
 static int check_pat(const fem_pattern_t* p) {
-  FEM_CHECK_ARG(p && p->row_ptr && p->slice_ptr && p->num_rows > 0, "pattern descriptor invalid");
+  FEM_CHECK_ARG(p && p->row_ptr && p->slice_ptr && p->sell_row && p->sell_pos && p->num_rows > 0,
+                "pattern descriptor invalid");
   FEM_CHECK_ARG(p->num_slices == ceil_div(p->num_rows, kSlice), "pattern num_slices mismatch");
   return FEM_OK;
 }

@@ -201,8 +201,12 @@ class DeviceProblem:
         check(L.fem_pattern_fill(dm.ref, ptr(self.inc_ptr), ptr(contrib_idx), ptr(row_ptr), ptr(col_idx),
                                  ptr(contrib_ptr), ptr(diag_slot), stream_ptr()), 'fem_pattern_fill')
         ns = (Nn + 63) // 64
+        sell_row = dm.empty(ns * 64, dtype=torch.int32)
+        sell_pos = dm.empty(Nn, dtype=torch.int32)
+        check(L.fem_sell_order(Nn, ptr(row_ptr), ptr(sell_row), ptr(sell_pos), stream_ptr()), 'fem_sell_order')
         slice_blocks = dm.empty(ns, dtype=torch.int32)
-        check(L.fem_sell_slice_sizes(Nn, ptr(row_ptr), ptr(slice_blocks), stream_ptr()), 'fem_sell_slice_sizes')
+        check(L.fem_sell_slice_sizes(Nn, ptr(row_ptr), ptr(sell_row), ptr(slice_blocks), stream_ptr()),
+              'fem_sell_slice_sizes')
         slice_ptr = dm.scan(slice_blocks)
         sell_blocks = int(slice_ptr[-1].item())
         sell_col = dm.empty(sell_blocks, dtype=torch.int32)
@@ -210,7 +214,8 @@ class DeviceProblem:
         p.num_rows, p.nnzb, p.num_slices, p.sell_blocks = Nn, nnzb, ns, sell_blocks
         p.vec, p.num_cells, p.nodes_per_cell = self.V, dm.C, N
         self.pat_tensors = dict(row_ptr=row_ptr, col_idx=col_idx, diag_slot=diag_slot, contrib_ptr=contrib_ptr,
-                                contrib_idx=contrib_idx, slice_ptr=slice_ptr, sell_col=sell_col)
+                                contrib_idx=contrib_idx, slice_ptr=slice_ptr, sell_col=sell_col, sell_row=sell_row,
+                                sell_pos=sell_pos)
         for k, t in self.pat_tensors.items():
             setattr(p, k, t.data_ptr())
         self.pat = p
@@ -353,6 +358,13 @@ class DeviceProblem:
         check(self.L.fem_gather_nodal(dm.Nn, self.V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell),
                                       ptr(self.load), ptr(res), stream_ptr()), 'fem_gather_nodal')
         return res
+
+    def cell_internal_force(self, u):
+        """Per-cell internal force K_e(u) u_e as (C, N, V), FP64 (the laplace-kernel value of problem.py:189-214)."""
+        dm = self.dm
+        cell = self.cell_buffer()
+        check(self.L.fem_cell_residual(dm.ref, self.mref, ptr(u), ptr(cell), stream_ptr()), 'fem_cell_residual')
+        return cell[:dm.C * dm.N * self.V].reshape(dm.C, dm.N, self.V).clone()
 
     def residual(self, u, scale=1.0):
         """R(u) with Dirichlet rows u - g (solver.py:290-304)."""

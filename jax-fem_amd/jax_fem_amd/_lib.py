@@ -36,7 +36,8 @@ class FemPattern(C.Structure):
     _fields_ = [('num_rows', _i64), ('nnzb', _i64), ('num_slices', _i64), ('sell_blocks', _i64),
                 ('vec', _i32), ('num_cells', _i32), ('nodes_per_cell', _i32), ('pad', _i32),
                 ('row_ptr', _vp), ('col_idx', _vp), ('diag_slot', _vp), ('contrib_ptr', _vp),
-                ('contrib_idx', _vp), ('slice_ptr', _vp), ('sell_col', _vp)]
+                ('contrib_idx', _vp), ('slice_ptr', _vp), ('sell_col', _vp), ('sell_row', _vp),
+                ('sell_pos', _vp)]
 
 
 class FemOperator(C.Structure):
@@ -66,7 +67,8 @@ _SIGNATURES = {
     'fem_exclusive_scan': (C.c_int, [_i64, _vp, _vp, _vp, _vp]),
     'fem_pattern_rows': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp, _vp, _vp]),
     'fem_pattern_fill': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    'fem_sell_slice_sizes': (C.c_int, [_i64, _vp, _vp, _vp]),
+    'fem_sell_order': (C.c_int, [_i64, _vp, _vp, _vp, _vp]),
+    'fem_sell_slice_sizes': (C.c_int, [_i64, _vp, _vp, _vp, _vp]),
     'fem_sell_fill': (C.c_int, [_P(FemPattern), _vp, _vp]),
     'fem_geometry': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp]),
     'fem_cell_tangent': (C.c_int, [_P(FemMesh), _P(FemMaterial), _i32, _vp, _vp, _vp]),

@@ -115,6 +115,32 @@ def test_pattern_bit_exact(gpu, case):
     assert np.array_equal(indptr, A.indptr) and np.array_equal(indices, A.indices)
 
 
+def test_sell_order_is_a_permutation_without_padding(gpu):
+    # SELL-C-sigma: sell_row / sell_pos are inverse permutations of the block rows and the
+    # length-sorted slices of a HEX27 lattice (rows of 27, 45, 75, 125 blocks) carry little padding
+    m = box_mesh_hex27(6, 5, 7, 1., 1., 1.)
+    p = ElasticLE(Mesh(m.points, m.cells_dict['hexahedron27']), vec=3, dim=3, ele_type='HEX27', quadrature_order=4)
+    dev = p.device()
+    t = dev.pat_tensors
+    rows = t['sell_row'].cpu().numpy()
+    pos = t['sell_pos'].cpu().numpy()
+    Nn = dev.dm.Nn
+    real = rows >= 0
+    assert real.sum() == Nn and np.array_equal(np.sort(rows[real]), np.arange(Nn))
+    assert np.array_equal(rows[pos], np.arange(Nn))
+    # each row stays inside its window of 2048 rows; the order is a stable sort by decreasing length
+    assert np.all(pos // 2048 == np.arange(Nn) // 2048)
+    lens = np.diff(t['row_ptr'].cpu().numpy())
+    order = np.concatenate([w + np.argsort(-lens[w:w + 2048], kind='stable') for w in range(0, Nn, 2048)])
+    assert np.array_equal(rows[real], order)
+    L = np.zeros(len(rows), np.int64)
+    L[:Nn] = lens[order]
+    expected = int((L.reshape(-1, 64).max(1) * 64).sum())
+    unsorted = np.zeros(len(rows), np.int64)
+    unsorted[:Nn] = lens
+    assert dev.sell_blocks == expected < int((unsorted.reshape(-1, 64).max(1) * 64).sum())
+
+
 def test_pattern_nnz_closed_form(gpu):
     # HEX8: prod(3 n_d + 1) V^2 ; HEX27: prod(8 n_d + 1) V^2
     p = ElasticLE(box(4, 3, 5), vec=3, dim=3)

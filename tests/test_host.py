@@ -33,7 +33,7 @@ def test_struct_sizes_match_header_layout():
     # plain C layout: ints first, then 64-bit fields / pointers
     assert ctypes.sizeof(_lib.FemMesh) == 4 * 4 + 2 * 8 + 9 * 8
     assert ctypes.sizeof(_lib.FemMaterial) == 8 + 6 * 8 + 8
-    assert ctypes.sizeof(_lib.FemPattern) == 4 * 8 + 4 * 4 + 7 * 8
+    assert ctypes.sizeof(_lib.FemPattern) == 4 * 8 + 4 * 4 + 9 * 8
     assert ctypes.sizeof(_lib.FemPcgOptions) == 8 + 8 + 4 * 4 + 8 + 8
     assert ctypes.sizeof(_lib.FemPcgResult) == 16 + 16 + 16 + 8
 
