@@ -49,7 +49,11 @@ enum {
 };
 
 /* ---- enums ----------------------------------------------------------- */
-enum { FEM_F32 = 0, FEM_F64 = 1 };           /* arithmetic precision          */
+enum { FEM_F32 = 0, FEM_F64 = 1,            /* arithmetic precision          */
+       FEM_BF16X3 = 2 };                     /* fem_cell_tangent only: HEX27 elastic
+                                                tangents from split-BF16 MFMA
+                                                products (hi*hi + hi*lo + lo*hi,
+                                                FP32 accumulation), Ke in FP32 */
 enum { FEM_HEX8 = 8, FEM_HEX27 = 27 };       /* = nodes per cell              */
 enum {                                       /* storage form of the matrix    */
   FEM_FORM_STANDARD = 0,      /* block (i, j) holds A_ij                      */
@@ -183,10 +187,12 @@ int fem_pattern_rows(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32
 int fem_pattern_fill(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32_t* contrib_idx,
                      const int32_t* row_ptr, int32_t* col_idx, int32_t* contrib_ptr,
                      int32_t* diag_slot, void* stream);
-/* Row order of the SELL slices: within each window of 2048 rows, rows sorted
- * stably by decreasing length (SELL-C-sigma), so a slice holds rows of equal
- * length and carries no padding.  sell_row (num_slices*64), sell_pos (num_rows). */
-int fem_sell_order(int64_t num_rows, const int32_t* row_ptr, int32_t* sell_row, int32_t* sell_pos, void* stream);
+/* Row order of the SELL slices.  sort = 0: natural order.  sort = 1: within
+ * each window of 2048 rows, rows sorted stably by decreasing length
+ * (SELL-C-sigma), so a slice holds rows of equal length and carries little
+ * padding.  sell_row (num_slices*64), sell_pos (num_rows). */
+int fem_sell_order(int64_t num_rows, int32_t sort, const int32_t* row_ptr, int32_t* sell_row, int32_t* sell_pos,
+                   void* stream);
 /* 64*max(row_len) per slice (rows taken in sell_row order), to be scanned into slice_ptr. */
 int fem_sell_slice_sizes(int64_t num_rows, const int32_t* row_ptr, const int32_t* sell_row, int32_t* slice_blocks,
                          void* stream);
@@ -197,7 +203,9 @@ int fem_sell_fill(const fem_pattern_t* pat, int32_t* sell_col, void* stream);
 /* shape_grads (C,Q,N,3), JxW (C,Q), quad points (C,Q,3); any output may be NULL. */
 int fem_geometry(const fem_mesh_t* mesh, double* shape_grads, double* JxW,
                  double* quad_points, void* stream);
-/* Per-cell tangent blocks Ke[c][a][b][vec][vec] in `precision` (float/double). */
+/* Per-cell tangent blocks Ke[c][a][b][vec][vec] in `precision` (float/double;
+ * FEM_BF16X3 writes float).  HEX27 cells with 27 quadrature points and a
+ * LINEAR_ELASTIC or SIMP law run on MFMA (FP64 MFMA for FEM_F64). */
 int fem_cell_tangent(const fem_mesh_t* mesh, const fem_material_t* mat, int32_t precision,
                      const double* u, void* Ke, void* stream);
 /* Per-cell internal-force residual re[c][a][vec] (FP64). */

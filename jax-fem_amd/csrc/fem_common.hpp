@@ -85,7 +85,9 @@ __device__ inline T block_sum(T v, T* smem) {
 // (sell_row; -1 on padding lanes, whose slots hold zeros).  The arithmetic is FP64 whatever the storage
 // type T: the difference of two FP32 inputs is exact in FP64 and the products
 // are accumulated without FP32 rounding (the kernel is HBM-bound, so the FP64
-// FMAs cost no time).  Returns this wave's partial sums.
+// FMAs cost no time).  The matrix stream (values, column ids) is read once per
+// product with nontemporal loads, which keeps the gathered vector entries in
+// the caches.  Returns this wave's partial sums.
 template <int V, bool DIFF, int W, int U, typename T, typename XAt>
 __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __restrict__ vals, int64_t s, int w,
                                           int lane, int64_t row, const XAt& xat, const double (&xr)[V],
@@ -99,7 +101,7 @@ __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __r
   for (int i = 0; i < V; ++i) acc[i] = 0.0;
 #pragma unroll U
   for (int k = w; k < L; k += W) {
-    const int64_t c = cp[(size_t)k * kSlice];
+    const int64_t c = __builtin_nontemporal_load(cp + (size_t)k * kSlice);
     T xt[V];
     xat(c, xt);
     double xc[V];
@@ -108,7 +110,8 @@ __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __r
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
-      for (int j = 0; j < V; ++j) acc[i] = fma(double(vp[((size_t)k * VV + i * V + j) * kSlice]), xc[j], acc[i]);
+      for (int j = 0; j < V; ++j)
+        acc[i] = fma(double(__builtin_nontemporal_load(vp + ((size_t)k * VV + i * V + j) * kSlice)), xc[j], acc[i]);
   }
 }
 

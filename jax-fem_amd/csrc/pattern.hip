@@ -226,7 +226,7 @@ __global__ void k_pattern_fill(int64_t num_nodes, int32_t N, const int32_t* __re
 // kSortWindow / kSortThreads rows against the whole window held in LDS.
 constexpr int kSortWindow = 2048;
 constexpr int kSortThreads = 1024;
-__global__ __launch_bounds__(kSortThreads) void k_sell_order(int64_t num_rows, int64_t num_lanes,
+__global__ __launch_bounds__(kSortThreads) void k_sell_order(int64_t num_rows, int64_t num_lanes, int sort,
                                                              const int32_t* __restrict__ row_ptr,
                                                              int32_t* __restrict__ sell_row,
                                                              int32_t* __restrict__ sell_pos) {
@@ -239,10 +239,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sell_order(int64_t num_rows, i
   __syncthreads();
   for (int i = threadIdx.x; i < kSortWindow; i += kSortThreads) {
     const int my = len[i];
-    int rank = 0;
-    for (int j = 0; j < kSortWindow; ++j) {
-      const int l = len[j];
-      rank += (l > my) || (l == my && j < i);
+    int rank = i;
+    if (sort) {
+      rank = 0;
+      for (int j = 0; j < kSortWindow; ++j) {
+        const int l = len[j];
+        rank += (l > my) || (l == my && j < i);
+      }
     }
     const int64_t r = base + i;
     const int64_t pos = base + rank;
@@ -375,11 +378,12 @@ int fem_pattern_fill(const fem_mesh_t* mesh, const int32_t* inc_ptr, const int32
   return FEM_OK;
 }
 
-int fem_sell_order(int64_t num_rows, const int32_t* row_ptr, int32_t* sell_row, int32_t* sell_pos, void* stream) {
+int fem_sell_order(int64_t num_rows, int32_t sort, const int32_t* row_ptr, int32_t* sell_row, int32_t* sell_pos,
+                   void* stream) {
   FEM_CHECK_ARG(num_rows > 0 && row_ptr && sell_row && sell_pos, "fem_sell_order: bad arguments");
   const int64_t lanes = ceil_div(num_rows, kSlice) * kSlice;
   hipLaunchKernelGGL(k_sell_order, dim3((unsigned)ceil_div(num_rows, kSortWindow)), dim3(kSortThreads), 0,
-                     as_stream(stream), num_rows, lanes, row_ptr, sell_row, sell_pos);
+                     as_stream(stream), num_rows, lanes, sort ? 1 : 0, row_ptr, sell_row, sell_pos);
   FEM_LAUNCHED();
   return FEM_OK;
 }

@@ -203,12 +203,19 @@ class DeviceProblem:
         ns = (Nn + 63) // 64
         sell_row = dm.empty(ns * 64, dtype=torch.int32)
         sell_pos = dm.empty(Nn, dtype=torch.int32)
-        check(L.fem_sell_order(Nn, ptr(row_ptr), ptr(sell_row), ptr(sell_pos), stream_ptr()), 'fem_sell_order')
         slice_blocks = dm.empty(ns, dtype=torch.int32)
-        check(L.fem_sell_slice_sizes(Nn, ptr(row_ptr), ptr(sell_row), ptr(slice_blocks), stream_ptr()),
-              'fem_sell_slice_sizes')
-        slice_ptr = dm.scan(slice_blocks)
-        sell_blocks = int(slice_ptr[-1].item())
+        # natural row order unless its slices carry more than 5% padding (rows of mixed
+        # lengths, e.g. HEX27 lattices); then rows are length-sorted within 2048-row windows
+        for sort in (0, 1):
+            check(L.fem_sell_order(Nn, sort, ptr(row_ptr), ptr(sell_row), ptr(sell_pos), stream_ptr()),
+                  'fem_sell_order')
+            check(L.fem_sell_slice_sizes(Nn, ptr(row_ptr), ptr(sell_row), ptr(slice_blocks), stream_ptr()),
+                  'fem_sell_slice_sizes')
+            slice_ptr = dm.scan(slice_blocks)
+            sell_blocks = int(slice_ptr[-1].item())
+            if sell_blocks <= 1.05 * nnzb:
+                break
+        self.sell_sorted = bool(sort)
         sell_col = dm.empty(sell_blocks, dtype=torch.int32)
         p = _lib.FemPattern()
         p.num_rows, p.nnzb, p.num_slices, p.sell_blocks = Nn, nnzb, ns, sell_blocks
@@ -377,17 +384,19 @@ class DeviceProblem:
         """Assemble the row-eliminated tangent in `precision` (problem.py:447-460, solver.py:522-529).
 
         ke_precision: arithmetic of the per-cell tangents (default: `precision`); FP64 cell
-        tangents are summed in FP64 and rounded once into an FP32 matrix.
+        tangents are summed in FP64 and rounded once into an FP32 matrix.  FEM_BF16X3 (HEX27
+        elastic laws) forms them from split-BF16 MFMA products and stores them in FP32.
         """
         dm = self.dm
         ke_precision = precision if ke_precision is None else ke_precision
         _, vals, dinv = self.matrix_buffers(precision)
-        Ke = self.matrix_buffers(ke_precision)[0]
+        ke_storage = _lib.FEM_F32 if ke_precision == _lib.FEM_BF16X3 else ke_precision
+        Ke = self.matrix_buffers(ke_storage)[0]
         s = stream_ptr()
         # FP32 matrices are stored in difference form (exact on translations), FP64 in standard form
         form = _lib.FEM_FORM_DIFFERENCE if precision == _lib.FEM_F32 else _lib.FEM_FORM_STANDARD
         check(self.L.fem_cell_tangent(dm.ref, self.mref, ke_precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
-        check(self.L.fem_gather_matrix(self.pref, ke_precision, ptr(Ke), precision, form, ptr(vals), ptr(dinv), s),
+        check(self.L.fem_gather_matrix(self.pref, ke_storage, ptr(Ke), precision, form, ptr(vals), ptr(dinv), s),
               'fem_gather_matrix')
         check(self.L.fem_matrix_bc_rows(self.pref, precision, ptr(vals), self.n_bc, ptr(self.bc_dofs), ptr(dinv), s),
               'fem_matrix_bc_rows')
@@ -492,7 +501,7 @@ class DeviceProblem:
     def cell_tangents(self, u, precision=_lib.FEM_F64):
         """Per-cell tangents (C, N*V, N*V) in the reference's jac layout (problem.py:262-266, 453)."""
         dm = self.dm
-        dt = torch.float32 if precision == _lib.FEM_F32 else torch.float64
+        dt = torch.float32 if precision in (_lib.FEM_F32, _lib.FEM_BF16X3) else torch.float64
         Ke = dm.empty(dm.C * dm.N * dm.N * self.V * self.V, dtype=dt)
         check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), stream_ptr()),
               'fem_cell_tangent')

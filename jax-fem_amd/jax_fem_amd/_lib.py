@@ -12,7 +12,7 @@ LIB_NAME = 'libfem_mi355x.so'
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 FEM_OK, FEM_ERR_BAD_ARG, FEM_ERR_NOT_CONVERGED, FEM_ERR_NAN, FEM_ERR_HIP, FEM_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
-FEM_F32, FEM_F64 = 0, 1
+FEM_F32, FEM_F64, FEM_BF16X3 = 0, 1, 2
 FEM_FORM_STANDARD, FEM_FORM_DIFFERENCE = 0, 1
 FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN = 1, 2, 3, 4
 
@@ -67,7 +67,7 @@ _SIGNATURES = {
     'fem_exclusive_scan': (C.c_int, [_i64, _vp, _vp, _vp, _vp]),
     'fem_pattern_rows': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp, _vp, _vp]),
     'fem_pattern_fill': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    'fem_sell_order': (C.c_int, [_i64, _vp, _vp, _vp, _vp]),
+    'fem_sell_order': (C.c_int, [_i64, _i32, _vp, _vp, _vp, _vp]),
     'fem_sell_slice_sizes': (C.c_int, [_i64, _vp, _vp, _vp, _vp]),
     'fem_sell_fill': (C.c_int, [_P(FemPattern), _vp, _vp]),
     'fem_geometry': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp]),

@@ -30,7 +30,7 @@ _NEWTON_OPTION_KEYS = frozenset({'tol', 'rel_tol', 'line_search_flag', 'initial_
 # refinement residual and iterate).
 JAX_SOLVER_DEFAULTS = dict(precond=True, precision='fp64', tol=1e-10, atol=1e-10, maxiter=10000,
                            check_every=16, reliable_delta=0.01, xcd_remap=True, assembly=None, timing=False)
-_ASSEMBLY = {None: None, 'fp32': _lib.FEM_F32, 'fp64': _lib.FEM_F64}
+_ASSEMBLY = {None: None, 'fp32': _lib.FEM_F32, 'fp64': _lib.FEM_F64, 'bf16x3': _lib.FEM_BF16X3}
 _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32}
 
 
@@ -72,7 +72,7 @@ def linear_config(linear_options):
         raise ValueError(f"precision must be one of {sorted(_PRECISIONS)}, got {cfg['precision']!r}")
     cfg['precision_code'] = _PRECISIONS[cfg['precision']]
     if cfg['assembly'] not in _ASSEMBLY:
-        raise ValueError(f"assembly must be 'fp32' or 'fp64', got {cfg['assembly']!r}")
+        raise ValueError(f"assembly must be 'fp32', 'fp64' or 'bf16x3', got {cfg['assembly']!r}")
     cfg['assembly_code'] = _ASSEMBLY[cfg['assembly']]
     return cfg
 

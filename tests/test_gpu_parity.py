@@ -121,6 +121,7 @@ def test_sell_order_is_a_permutation_without_padding(gpu):
     m = box_mesh_hex27(6, 5, 7, 1., 1., 1.)
     p = ElasticLE(Mesh(m.points, m.cells_dict['hexahedron27']), vec=3, dim=3, ele_type='HEX27', quadrature_order=4)
     dev = p.device()
+    assert dev.sell_sorted
     t = dev.pat_tensors
     rows = t['sell_row'].cpu().numpy()
     pos = t['sell_pos'].cpu().numpy()
@@ -139,6 +140,15 @@ def test_sell_order_is_a_permutation_without_padding(gpu):
     unsorted = np.zeros(len(rows), np.int64)
     unsorted[:Nn] = lens
     assert dev.sell_blocks == expected < int((unsorted.reshape(-1, 64).max(1) * 64).sum())
+
+
+def test_sell_natural_order_for_hex8(gpu):
+    # HEX8 lattices pad < 5% in natural order, which keeps consecutive rows in a slice
+    dev = ElasticLE(box(6, 5, 7), vec=3, dim=3).device()
+    assert not dev.sell_sorted
+    rows = dev.pat_tensors['sell_row'].cpu().numpy()
+    Nn = dev.dm.Nn
+    assert np.array_equal(rows[:Nn], np.arange(Nn)) and np.all(rows[Nn:] == -1)
 
 
 def test_pattern_nnz_closed_form(gpu):
@@ -214,6 +224,38 @@ def test_cell_tangent_and_residual_vs_oracle(gpu, name, ele):
     x = torch.as_tensor(_rng(9).standard_normal(dev.n), device='cuda')
     y = dev.apply_operator(ut, x).cpu().numpy()
     assert rel(y, A @ x.cpu().numpy()) < 1e-12
+
+
+@pytest.mark.parametrize('name', ['elastic', 'simp'])
+def test_hex27_bf16x3_mfma_tangent(gpu, name):
+    """Split-BF16 MFMA tangents (hi*hi + hi*lo + lo*hi, FP32 accumulation): about 16
+    significant bits per product, checked against the FP64 oracle; symmetric and with
+    rigid translations in the null space to the same accuracy."""
+    p, o, u = make_material_problem(name, 'HEX27')
+    dev = p.device()
+    dev.prepare()
+    ut = torch.as_tensor(u, device='cuda')
+    Kb = dev.cell_tangents(ut, _lib.FEM_BF16X3).double().cpu().numpy()
+    Ko = OF.cell_tangent(o.mat, u.reshape(-1, 3), o.cells, o.G, o.JxW, o.theta)
+    err = rel(Kb, Ko)
+    assert 1e-9 < err < 1e-4, err          # not FP64-exact, but far better than one BF16 product (4e-3)
+    assert rel(Kb, np.swapaxes(Kb, 1, 2)) < 1e-4
+    t = np.tile([0.3, -1.1, 0.7], 27)
+    assert np.abs(Kb @ t).max() < 1e-4 * np.abs(Kb).max()
+
+
+def test_hex27_bf16x3_mixed_solve(gpu):
+    """Mixed PCG on a BF16x3-assembled HEX27 matrix, refined against the FP64 residual."""
+    from problems import LoadedElastic
+    m = box_mesh_hex27(4, 3, 3, 2., 1., 1.)
+    mesh = Mesh(m.points, m.cells_dict['hexahedron27'])
+    left = lambda x: isclose(x[0], 0.)
+    p = LoadedElastic(mesh, vec=3, dim=3, ele_type='HEX27', quadrature_order=4,
+                      dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda x: 0.] * 3],
+                      location_fns=[lambda x: isclose(x[0], 2.)])
+    ref = solver(p, {'jax_solver': {'precision': 'fp64', 'tol': 1e-12, 'atol': 0.}})[0].cpu().numpy()
+    sol = solver(p, {'jax_solver': {'precision': 'mixed', 'assembly': 'bf16x3'}})[0].cpu().numpy()
+    assert np.linalg.norm(sol - ref) / np.linalg.norm(ref) < 1e-8
 
 
 def test_difference_form_spmv(gpu):
