@@ -522,7 +522,8 @@ def workload_hex27(args):
         f"{dev.sell_blocks} SELL blocks ({dev.sell_blocks / dev.nnzb - 1:.1%} padding)")
     res, grads, sols = {}, {}, {}
     for name, o in _paths(args).items():
-        o = {'jax_solver': dict(o['jax_solver'], timing=True)}
+        # rtol 1e-11: at 1e-10 either solution is already ~1e-8 from the exact one on this mesh
+        o = {'jax_solver': dict(o['jax_solver'], timing=True, tol=1e-11, atol=0.)}
         fwd = ad_wrapper(p, o, o)
         fw, bw = [], []
         for it in range(args.warmup + args.steps):
@@ -550,7 +551,7 @@ def workload_hex27(args):
     # compliance self-adjointness: dC/dE_e = -u_e^T K_e(1) u_e = -(energy of cell e) / E_e * 2 ... checked by sign
     assert bool((grads['fp64'] < 0).all()), "compliance gradient w.r.t. E must be negative"
     spmv_b = {nm: sell_bytes(dev, es) for nm, es in (('mixed', 4), ('fp64', 8))}
-    _emit(f"HEX27 elasticity {n}^3 unit cube, 27-pt Gauss, {dev.n} dofs, log-normal E field",
+    _emit(f"HEX27 elasticity {n}^3 unit cube, 27-pt Gauss, {dev.n} dofs, log-normal E field, Jacobi-PCG rtol 1e-11",
           res['mixed']['s_per_newton_step'], "f32+f64",
           {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'],
            "spmv_GBs": {nm: spmv_b[nm] / (res[nm]['spmv_ms'] * 1e-3) / 1e9 for nm in spmv_b if res[nm]['spmv_ms']},
@@ -569,7 +570,7 @@ def main():
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--cells', type=int, nargs=3, default=[128, 64, 64])
-    ap.add_argument('--mixed-assembly', choices=['fp32', 'fp64'], default='fp64')
+    ap.add_argument('--mixed-assembly', choices=['fp32', 'fp64', 'bf16x3'], default='fp64')
     ap.add_argument('--no-cpu-baseline', dest='cpu_baseline', action='store_false')
     ap.add_argument('--only', choices=['fp64', 'mixed'], default=None,
                     help='time one path only (no adjoint, no summary line); for profiling')

@@ -406,7 +406,8 @@ int fem_cell_tangent(const fem_mesh_t* mesh, const fem_material_t* mat, int32_t 
   FEM_CHECK_ARG(precision != FEM_BF16X3 || hex27_tangent_mfma_applies(*mesh, *mat),
                 "fem_cell_tangent: FEM_BF16X3 needs HEX27 cells, 27 quadrature points and an elastic law");
   if (mesh->num_cells == 0) return FEM_OK;
-  if (precision == FEM_BF16X3) return hex27_tangent_mfma(*mesh, *mat, precision, Ke, as_stream(stream));
+  if (precision != FEM_F32 && hex27_tangent_mfma_applies(*mesh, *mat))
+    return hex27_tangent_mfma(*mesh, *mat, precision, Ke, as_stream(stream));
   return with_cfg(*mesh, [&](auto cfg) -> int {
     using Cfg = decltype(cfg);
     return with_mat(*mat, mesh->vec, [&](auto m) -> int {

@@ -10,7 +10,9 @@
 //                  + delta_ik c1 (Z[(a,0),(b,0)] + Z[(a,1),(b,1)] + Z[(a,2),(b,2)]).
 // Z is formed on MFMA in 16 x 16 tiles (81 padded to 96, 36 tiles, 9 per
 // wave), staged in LDS and expanded into Ke[c][a][b][i][k] with coalesced
-// stores.  Two arithmetic modes:
+// stores.  Operand layouts: A[i][k] in lane i + 16 k (f64; for bf16 the four
+// k of one lane are k = 4 (lane / 16) + e), B[k][j] in lane j + 16 k; the two
+// instructions place the 16 x 16 result differently (see the stores).  Two arithmetic modes:
 //   FP64 : v_mfma_f64_16x16x4f64, exact FP64 products and sums.
 //   BF16x3: each operand split x = hi + lo (hi = bf16(x), lo = bf16(x - hi));
 //          hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x16bf16_1k with FP32
@@ -83,9 +85,10 @@ __global__ __launch_bounds__(256) void k_tangent_hex27_mfma(fem_mesh_t mesh, fem
           const double b = (q < kQ && n < kM) ? g[q][n / 3][n % 3] : 0.0;
           acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
         }
+        // f64 16x16x4 result: D[i][j] in lane j + 16 (i % 4), register i / 4
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = tm * 16 + 4 * (lane >> 4) + r, col = tn * 16 + (lane & 15);
+          const int row = tm * 16 + (lane >> 4) + 4 * r, col = tn * 16 + (lane & 15);
           if (row < kM && col < kM) Z[row][col] = TZ(acc[r]);
         }
       } else {
@@ -108,6 +111,7 @@ __global__ __launch_bounds__(256) void k_tangent_hex27_mfma(fem_mesh_t mesh, fem
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, acc, 0, 0, 0);
           acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, acc, 0, 0, 0);
         }
+        // bf16 16x16x16 result: D[i][j] in lane j + 16 (i / 4), register i % 4
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = tm * 16 + 4 * (lane >> 4) + r, col = tn * 16 + (lane & 15);

@@ -150,6 +150,8 @@ def _simp_fd_check(p, theta, ncells, h, tol, precision='fp64'):
     g = th.grad.cpu().numpy()
     assert np.all(g < 0)
     fopt = {'jax_solver': {'precision': 'fp64', 'tol': 1e-11, 'atol': 0.}}
+    # noise floor of a central difference of an FP64 compliance: ~1e-15 |C| / h
+    floor = 1e-15 * abs(float(J.detach())) / h
     for e in np.random.default_rng(1).choice(p.num_cells, ncells, replace=False):
         vals = []
         for s in (1., -1.):
@@ -158,7 +160,7 @@ def _simp_fd_check(p, theta, ncells, h, tol, precision='fp64'):
             p.set_params(t)
             vals.append(_variational_compliance(p, solver(p, fopt)[0]))
         fd = (vals[0] - vals[1]) / (2 * h)
-        assert abs(fd - g[e]) <= tol * abs(g[e]), (e, fd, g[e])
+        assert abs(fd - g[e]) <= tol * abs(g[e]) + floor, (e, fd, g[e], floor)
     return g
 
 
@@ -222,11 +224,12 @@ def test_hex27_efield_adjoint_vs_oracle(gpu, precision):
 
 
 def test_hex27_full_size_mixed_matches_fp64(gpu):
-    # 48^3 HEX27 cells (2.7M dofs): mixed vs FP64 solution and gradient, sign and energy identity
+    # 48^3 HEX27 cells (2.7M dofs): mixed vs FP64 solution and gradient, sign and energy identity.
+    # Both solve to rtol 1e-11: at 1e-10 each is ~1e-8 away from the exact solution here.
     p, E = _hex27(48)
     res = {}
     for precision in ('fp64', 'mixed'):
-        opts = {'jax_solver': {'precision': precision}}
+        opts = {'jax_solver': {'precision': precision, 'tol': 1e-11, 'atol': 0.}}
         fwd = ad_wrapper(p, opts, opts)
         e = E.clone().requires_grad_(True)
         u = fwd(e)[0]
