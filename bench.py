@@ -76,6 +76,20 @@ def build_problem(nx, ny, nz):
     return p
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_pcg_a_hbm_traffic_cantilever_128x64x64.md')
+
+
+def measured_traffic(cells):
+    """HBM bytes per launch of k_pcg_a measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
+    a streaming kernel of known size) on the 128x64x64 workload; None for other sizes."""
+    if list(cells) != [128, 64, 64] or not os.path.exists(TRAFFIC_PROFILE):
+        return {}
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith('traffic_bytes_per_launch:'):
+            return {k: int(v) for k, v in (kv.split('=') for kv in line.split(':', 1)[1].split())}
+    return {}
+
+
 def sell_bytes(dev, elem_bytes, vec=3):
     """Algorithmic bytes of one fused SpMV launch: the matrix blocks (values + one column
     index each, padding not counted) once, z and p_old read once per row (gathers counted
@@ -169,6 +183,7 @@ def run(args):
         del res[name]['grad']
 
     mixed = res['mixed']
+    traffic = measured_traffic(args.cells)
     spmv_bytes = sell_bytes(dev, 4)
     achieved = spmv_bytes / (mixed['spmv_ms'] * 1e-3) / 1e9 if mixed['spmv_ms'] else None
     fp64_bytes = sell_bytes(dev, 8)
@@ -189,12 +204,18 @@ def run(args):
         "adjoint_grad_rel_diff_mixed_vs_fp64": gerr,
         "pcg_iterations": {"mixed": mixed['pcg_iterations'], "fp64": res['fp64']['pcg_iterations'],
                            "mixed_replacements": mixed['replacements']},
-        "roofline": {"bound": "hbm", "kernel": "k_pcg_spmv<3,float> (SELL-64 BSR3 FP32)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_pcg_a<3,true,float> (p update + SELL-64 3x3-block SpMV, FP32 storage, FP64 "
+                               "arithmetic, block partials of <p,Ap>)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS if achieved else None,
+                     "traffic": traffic.get('k_pcg_a_f32'),
+                     "traffic_source": "profiles/k_pcg_a_hbm_traffic_cantilever_128x64x64.md" if traffic else None,
                      "algorithmic_bytes": spmv_bytes, "avg_launch_ms": mixed['spmv_ms']},
-        "fp64_spmv": {"achieved_GBs": fp64_achieved, "avg_launch_ms": res['fp64']['spmv_ms'],
-                      "algorithmic_bytes": fp64_bytes},
+        "fp64_spmv": {"kernel": "k_pcg_a<3,false,double>", "achieved_GBs": fp64_achieved,
+                      "frac": fp64_achieved / PEAK_HBM_GBS if fp64_achieved else None,
+                      "avg_launch_ms": res['fp64']['spmv_ms'], "algorithmic_bytes": fp64_bytes,
+                      "traffic": traffic.get('k_pcg_a_f64')},
         "detail": res,
     }
     if args.cpu_baseline:
