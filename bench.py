@@ -401,9 +401,18 @@ def workload_neo_hookean(args):
                          pcg_iterations=tot_pcg, final_residual=st['res_norm'])
     res['mixed']['rel_err_vs_fp64'] = _rel(sols['mixed'], sols['fp64'])
     log(f"neo_hookean: {res}")
+    extra = {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'], "detail": res}
+    if args.cpu_baseline:
+        from oracle import fem as OF
+        ns = 16
+        ms = box_mesh(ns, ns, ns, 1., 1., 1.)
+        o = OF.Problem(ms.points, ms.cells_dict['hexahedron'], 3, mat=OF.Material('neo_hookean', mu=mu, kappa=kappa))
+        u = 0.01 * np.random.default_rng(0).standard_normal(o.n)
+        extra["cpu_baseline"] = _cpu_port_sample(o, p.num_cells, p.device().nnzb * 9,
+                                                 res['fp64']['pcg_iterations'] / res['fp64']['newton_iterations'],
+                                                 "s per Newton step", u)
     _emit(f"HEX8 Neo-Hookean cube {n}^3, {p.fes[0].num_total_nodes * 3} dofs, 5 load steps, Newton |R| <= 1e-8",
-          res['mixed']['s_per_newton_step'], "f32+f64",
-          {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'], "detail": res})
+          res['mixed']['s_per_newton_step'], "f32+f64", extra)
 
 
 def _compliance_var(p, f, u):
@@ -495,11 +504,22 @@ def workload_simp(args):
     adj = grads['fp64'][torch.as_tensor(cells, device='cuda')].cpu().numpy()
     fd_err = float(np.max(np.abs(adj - fd) / np.abs(fd)))
     log(f"simp FD (h={h}) vs adjoint on 16 cells: max rel diff {fd_err:.2e}")
+    cpu = None
+    if args.cpu_baseline:
+        from oracle import fem as OF
+        ms = box_mesh(Nx // 4, Ny // 4, Nz // 4, Lx, Ly, Lz)
+        cs = ms.cells_dict['hexahedron']
+        th = 0.5 + np.random.default_rng(0).uniform(-0.05, 0.05, len(cs))
+        o = OF.Problem(ms.points, cs, 3, theta=np.repeat(th[:, None], 8, 1),
+                       mat=OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., nu))), Emin=Emin, E0=E0,
+                                       p=penal))
+        cpu = _cpu_port_sample(o, C, p.device().nnzb * 9, res['fp64']['adjoint_pcg_iterations'],
+                               "s per adjoint gradient")
     _emit(f"HEX8 SIMP cantilever {Nx}x{Ny}x{Nz} on {Lx}x{Ly}x{Lz}, {p.fes[0].num_total_nodes * 3} dofs, "
           f"compliance gradient", res['mixed']['s_per_adjoint_gradient'], "f32+f64",
-          {"fp64_s_per_adjoint_gradient": res['fp64']['s_per_adjoint_gradient'],
-           "fd_check": {"cells": cells.tolist(), "h": h, "max_rel_diff": fd_err, "fd": fd.tolist(),
-                        "adjoint": adj.tolist()}, "detail": res})
+          dict({"fp64_s_per_adjoint_gradient": res['fp64']['s_per_adjoint_gradient'],
+                "fd_check": {"cells": cells.tolist(), "h": h, "max_rel_diff": fd_err, "fd": fd.tolist(),
+                             "adjoint": adj.tolist()}, "detail": res}, **({"cpu_baseline": cpu} if cpu else {})))
 
 
 def workload_hex27(args):
@@ -572,11 +592,43 @@ def workload_hex27(args):
     # compliance self-adjointness: dC/dE_e = -u_e^T K_e(1) u_e = -(energy of cell e) / E_e * 2 ... checked by sign
     assert bool((grads['fp64'] < 0).all()), "compliance gradient w.r.t. E must be negative"
     spmv_b = {nm: sell_bytes(dev, es) for nm, es in (('mixed', 4), ('fp64', 8))}
+    cpu = None
+    if args.cpu_baseline:
+        from oracle import fem as OF
+        ns = 6
+        ms = box_mesh_hex27(ns, ns, ns, 1., 1., 1.)
+        cs = ms.cells_dict['hexahedron27']
+        Es = np.exp(np.random.default_rng(0).normal(0., 0.25, len(cs)))
+        o = OF.Problem(ms.points, cs, 3, ele_type='HEX27', quadrature_order=4, theta=np.repeat(Es[:, None], 27, 1),
+                       mat=OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., nu))), Emin=0., E0=1., p=1.))
+        cpu = _cpu_port_sample(o, C, dev.nnzb * 9, res['fp64']['pcg_iterations'], "s per Newton step")
     _emit(f"HEX27 elasticity {n}^3 unit cube, 27-pt Gauss, {dev.n} dofs, log-normal E field, Jacobi-PCG rtol 1e-11",
           res['mixed']['s_per_newton_step'], "f32+f64",
-          {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'],
-           "spmv_GBs": {nm: spmv_b[nm] / (res[nm]['spmv_ms'] * 1e-3) / 1e9 for nm in spmv_b if res[nm]['spmv_ms']},
-           "detail": res})
+          dict({"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'],
+                "spmv_GBs": {nm: spmv_b[nm] / (res[nm]['spmv_ms'] * 1e-3) / 1e9 for nm in spmv_b
+                             if res[nm]['spmv_ms']},
+                "detail": res}, **({"cpu_baseline": cpu} if cpu else {})))
+
+
+def _cpu_port_sample(o, gpu_cells, gpu_nnz, gpu_iterations, what, u=None):
+    """The oracle's FP64 tangent assembly and 50 Jacobi-PCG iterations on a small sample of the same
+    problem family, on one core, scaled by the cell and nonzero ratios and the GPU's FP64 iteration count."""
+    import numpy as np
+    u = np.zeros(o.n) if u is None else u
+    t = time.perf_counter()
+    A = o.tangent(u)
+    t_asm = time.perf_counter() - t
+    b = np.random.default_rng(0).standard_normal(o.n)
+    its = 50
+    t = time.perf_counter()
+    OF_pcg = __import__('oracle.fem', fromlist=['pcg_jacobi']).pcg_jacobi
+    OF_pcg(A, b, np.zeros(o.n), A.diagonal(), tol=0., atol=0., maxiter=its)
+    t_it = (time.perf_counter() - t) / its
+    sc, sn = gpu_cells / len(o.cells), gpu_nnz / A.nnz
+    return {"value": t_asm * sc + t_it * sn * gpu_iterations, "unit": what, "cores": 1, "kind": "port",
+            "sample": f"oracle (numpy/scipy FP64) on {len(o.cells)} cells: tangent assembly {t_asm:.2f}s scaled by "
+                      f"cells x{sc:.0f}, {its} Jacobi-PCG iterations ({t_it * 1e3:.1f} ms each) scaled by nnz "
+                      f"x{sn:.0f} and by {gpu_iterations} FP64 GPU iterations"}
 
 
 WORKLOADS = {'poisson': workload_poisson, 'neo_hookean': workload_neo_hookean, 'simp': workload_simp,

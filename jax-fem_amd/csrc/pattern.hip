@@ -324,7 +324,8 @@ int fem_exclusive_scan(int64_t n, const int32_t* in, int32_t* out, void* ws, voi
 
 int fem_incidence(int64_t num_entities, int32_t N, const int32_t* conn, int64_t num_nodes,
                   int32_t* inc_ptr, int32_t* inc_idx, void* ws, void* stream) {
-  FEM_CHECK_ARG(num_entities >= 0 && N > 0 && num_nodes > 0 && conn && inc_ptr && inc_idx && ws,
+  FEM_CHECK_ARG(num_entities >= 0 && N > 0 && num_nodes > 0 && (conn || num_entities == 0) && inc_ptr && inc_idx &&
+                    ws,
                 "fem_incidence: bad arguments");
   FEM_CHECK_ARG(num_entities * N < (int64_t)INT32_MAX, "fem_incidence: too many entries for int32 indices");
   hipStream_t s = as_stream(stream);
