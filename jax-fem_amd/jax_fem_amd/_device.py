@@ -481,6 +481,8 @@ class DeviceProblem:
         srf = self.surfaces[i]
         S = srf['faces'].shape[0]
         lq = dm.empty(S, dm.FQ, self.V)
+        if S == 0:
+            return lq
         check(self.L.fem_face_load_adjoint(dm.ref, S, ptr(srf['faces']), ptr(lam), ptr(lq), stream_ptr()),
               'fem_face_load_adjoint')
         return lq
