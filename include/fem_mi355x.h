@@ -24,10 +24,13 @@
  * columns sorted), which expands exactly to the scalar CSR pattern the
  * reference builds with PETSc from its COO I/J (problem.py:86-107,
  * solver.py:472-478).  Values live in a sliced-ELL layout over 64-row slices
- * (SELL-64): block (row, k) of slice s = row/64 is stored at block position
- * slice_ptr[s] + 64*k + row%64, and value v (0..vec*vec-1, row-major in the
- * block) of it at  vals[slice_ptr[s]*vec*vec + (k*vec*vec + v)*64 + row%64].
- * Slots past a row's length hold zero values and the row itself as column.
+ * (SELL-64): row `row` occupies lane l = sell_pos[row] % 64 of slice
+ * s = sell_pos[row] / 64 (sell_row is the inverse map; rows are in natural
+ * order unless length-sorted within 2048-row windows, see fem_sell_order).
+ * Block k of the row is stored at block position slice_ptr[s] + 64*k + l, and
+ * value v (0..vec*vec-1, row-major in the block) of it at
+ * vals[slice_ptr[s]*vec*vec + (k*vec*vec + v)*64 + l].  Slots past a row's
+ * length hold zero values and the row itself as column.
  */
 #ifndef FEM_MI355X_H
 #define FEM_MI355X_H
@@ -251,15 +254,21 @@ int fem_gather_matrix(const fem_pattern_t* pat, int32_t ke_precision, const void
  * A[d,d] = 1, dinv[d] = 1 (dinv may be NULL).  Valid for both forms. */
 int fem_matrix_bc_rows(const fem_pattern_t* pat, int32_t precision, void* vals, int64_t n_bc,
                        const int32_t* bc_dofs, void* dinv, void* stream);
-/* dinv[d] = 1/A[d,d] (0 where the diagonal is absent), in `precision`. */
+/* Jacobi M = x / diag(A) (solver.py:75-77): dinv[d] = 1/A[d,d] (0 where the
+ * diagonal is absent), in `precision`.  Standard-form matrices only (the
+ * diagonal slot of a difference-form matrix holds row sums; fem_gather_matrix
+ * returns the true D^-1 for those). */
 int fem_jacobi(const fem_pattern_t* pat, int32_t precision, const void* vals, void* dinv, void* stream);
 /* res[d] = u[d] - scale*g[d] on Dirichlet dofs (solver.py:290-304). */
 int fem_residual_bc(int64_t n_bc, const int32_t* bc_dofs, const double* bc_vals, const double* u,
                     double* res, double scale, void* stream);
-/* y = A x in `precision` (x, y of that precision), matrix stored in `form`. */
+/* y = A x in `precision` (x, y of that precision), matrix stored in `form`
+ * (the BCOO product inside jax_solve, solver.py:67-84).  FP64 arithmetic
+ * for both storage precisions. */
 int fem_spmv(const fem_pattern_t* pat, int32_t precision, int32_t form, const void* vals, const void* x,
              void* y, void* stream);
-/* Scalar CSR export: indptr (vec*rows+1), indices (nnz), data (nnz, FP64). */
+/* Scalar CSR export, the PETSc AIJ / getValuesCSR layout of get_A
+ * (solver.py:540-553): indptr (vec*rows+1), indices (nnz), data (nnz, FP64). */
 int fem_export_csr(const fem_pattern_t* pat, int32_t precision, int32_t form, const void* vals,
                    int32_t* indptr, int32_t* indices, double* data, void* stream);
 /* y = dR/du(u) x (FP64, matrix-free, BC rows = identity). */
@@ -279,7 +288,8 @@ int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv,
                   fem_pcg_result_t* result /* host */, void* stream);
 
 /* ---- vector helpers -------------------------------------------------- */
-/* out = sum_i x_i y_i (FP64), deterministic; ws >= 8 * 512 bytes. */
+/* out = sum_i x_i y_i (FP64), deterministic; ws >= 8 * 512 bytes (the
+ * residual norms of the Newton loop, solver.py:1319-1333). */
 int fem_dot(int64_t n, const double* x, const double* y, double* out, void* ws, void* stream);
 
 #ifdef __cplusplus
