@@ -114,10 +114,12 @@ def run(args):
     log(f"setup {time.perf_counter() - t0:.1f}s: {dev.dm.C} cells, {dev.n} dofs, {dev.nnzb * 9} nnz "
         f"({dev.sell_blocks} SELL blocks)")
 
+    # timed steps run without per-launch events; one extra step with events gives the kernel averages
     opts = {
-        'fp64': {'jax_solver': {'precision': 'fp64', 'timing': True}},
-        'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly, 'timing': True}},
+        'fp64': {'jax_solver': {'precision': 'fp64'}},
+        'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}},
     }
+    event_opts = {k: {'jax_solver': dict(v['jax_solver'], timing=True)} for k, v in opts.items()}
     # reference: FP64 path solved to rtol 1e-11 (about the attainable FP64 residual here)
     ref_opts = {'jax_solver': {'precision': 'fp64', 'tol': 1e-11, 'atol': 0.}}
     ref = None
@@ -144,14 +146,17 @@ def run(args):
             stats.append(st)
         err = float(torch.linalg.vector_norm(sol - ref) / torch.linalg.vector_norm(ref)) if ref is not None \
             else float('nan')
-        lin = stats[-1]['linear'][0]
+        solver(p, event_opts[name])
+        lin = p._solve_stats['linear'][0]
         res[name] = dict(s_per_newton_step=statistics.median(times), times=times, rel_err=err,
                          newton_iterations=stats[-1]['newton_iterations'], pcg_iterations=lin['iterations'],
                          replacements=lin['replacements'], spmv_ms=lin['spmv_ms'], update_ms=lin['update_ms'],
                          timing=stats[-1]['timing'])
+        tm = stats[-1]['timing']
         log(f"{name}: {res[name]['s_per_newton_step']:.4f} s/step, rel err {err:.2e}, "
             f"{lin['iterations']} its, {lin['replacements']} repl, spmv {lin['spmv_ms']*1e3:.1f} us, "
-            f"update {lin['update_ms']*1e3:.1f} us")
+            f"update {lin['update_ms']*1e3:.1f} us; residuals {tm['local_assembly']*1e3:.1f} ms, "
+            f"tangent {tm['global_matrix']*1e3:.1f} ms, linear {tm['linear']*1e3:.1f} ms")
 
     if args.only is not None:
         if args.calib_copy_mib:
@@ -564,7 +569,7 @@ def workload_hex27(args):
     res, grads, sols = {}, {}, {}
     for name, o in _paths(args).items():
         # rtol 1e-11: at 1e-10 either solution is already ~1e-8 from the exact one on this mesh
-        o = {'jax_solver': dict(o['jax_solver'], timing=True, tol=1e-11, atol=0.)}
+        o = {'jax_solver': dict(o['jax_solver'], tol=1e-11, atol=0.)}
         fwd = ad_wrapper(p, o, o)
         fw, bw = [], []
         for it in range(args.warmup + args.steps):
@@ -582,6 +587,8 @@ def workload_hex27(args):
                 bw.append(time.perf_counter() - t1)
         grads[name] = e.grad.detach().clone()
         sols[name] = u.detach().clone()
+        from jax_fem_amd.solver import solver
+        solver(p, {'jax_solver': dict(o['jax_solver'], timing=True)})     # kernel averages (HIP events)
         lin = p._solve_stats['linear'][0]
         res[name] = dict(s_per_newton_step=statistics.median(fw), s_per_adjoint_gradient=statistics.median(bw),
                          pcg_iterations=lin['iterations'], spmv_ms=lin['spmv_ms'], update_ms=lin['update_ms'],
