@@ -70,9 +70,14 @@ enum {                                       /* constitutive laws (tensor_map)*/
   FEM_MAT_SIMP = 3,           /* as LINEAR_ELASTIC with lambda = c0 E(t),
                                  mu = c1 E(t), E(t) = c2 + (c3 - c2) t^c4, t =
                                  per-quad parameter                  (vec 3) */
-  FEM_MAT_NEO_HOOKEAN = 4     /* P = d psi / dF, psi = mu/2 (J^-2/3 I1 - 3) +
+  FEM_MAT_NEO_HOOKEAN = 4,    /* P = d psi / dF, psi = mu/2 (J^-2/3 I1 - 3) +
                                  kappa/2 (J - 1)^2, mu = c0 s, kappa = c1 s,
                                  s = per-quad parameter or 1         (vec 3) */
+  FEM_MAT_J2_PLASTIC = 5      /* small-strain J2 perfect plasticity, radial
+                                 return from the per-quad state (sigma_old,
+                                 eps_old): lambda = c0, mu = c1, yield stress
+                                 c2 (tests/benchmarks/plasticity/
+                                 test_plasticity.py:32-64)          (vec 3) */
 };
 
 /* ---- plain descriptor structs (device pointers) ------------------------ */
@@ -99,6 +104,8 @@ typedef struct {
   int32_t has_param;             /* 1: `param` holds one value per (cell, quad) */
   double  c[6];                  /* constants, meaning per kind above         */
   const double* param;           /* (num_cells, Q) or NULL                    */
+  const double* state;           /* J2_PLASTIC: (num_cells, Q, 18) = sigma_old
+                                    (3x3 row-major) then eps_old; else NULL  */
 } fem_material_t;
 
 typedef struct {

@@ -57,8 +57,7 @@ __global__ __launch_bounds__(256) void k_cell_tangent(fem_mesh_t mesh, fem_mater
 #pragma unroll
           for (int j = 0; j < 3; ++j) G[i][j] = T(0);
       }
-      const double prm = mat.has_param ? mat.param[c * Q + q] : 1.0;
-      s_st[w][q] = MAT::template prep<T>(mat, prm, G);
+      s_st[w][q] = MAT::template prep<T>(mat, c * Q + q, G);
     }
   }
   __syncthreads();
@@ -114,8 +113,7 @@ __global__ __launch_bounds__(256) void k_cell_vector(fem_mesh_t mesh, fem_materi
     for (int q = t; q < Q; q += TPC) {
       T G[V][3], P[V][3];
       quad_grad<N, Q, V, T>(L, s_u[w], q, G);
-      const double prm = mat.has_param ? mat.param[c * Q + q] : 1.0;
-      const auto st = MAT::template prep<T>(mat, prm, G);
+      const auto st = MAT::template prep<T>(mat, c * Q + q, G);
       if (MODE == 0) {
         MAT::stress(st, G, P);
       } else {
@@ -176,9 +174,8 @@ __global__ __launch_bounds__(256) void k_cell_adjoint(fem_mesh_t mesh, fem_mater
     T G[V][3], GL[V][3], dP[V][3];
     quad_grad<N, Q, V, T>(L, s_u[w], q, G);
     quad_grad<N, Q, V, T>(L, s_l[w], q, GL);
-    const double prm = mat.has_param ? mat.param[c * Q + q] : 1.0;
-    const auto st = MAT::template prep<T>(mat, prm, G);
-    MAT::dparam(mat, prm, st, G, dP);
+    const auto st = MAT::template prep<T>(mat, c * Q + q, G);
+    MAT::dparam(mat, c * Q + q, st, G, dP);
     T s = 0;
 #pragma unroll
     for (int i = 0; i < V; ++i)
@@ -363,10 +360,14 @@ static int with_mat(const fem_material_t& mat, int vec, F&& f) {
     case FEM_MAT_NEO_HOOKEAN:
       if (vec != 3) break;
       return f(MatNeoHookean{});
+    case FEM_MAT_J2_PLASTIC:
+      if (vec != 3 || !mat.state) break;
+      return f(MatJ2{});
     default:
       break;
   }
-  set_error("unsupported material kind %d for vec=%d (param %d)", mat.kind, vec, mat.has_param);
+  set_error("unsupported material kind %d for vec=%d (param %d, state %s)", mat.kind, vec, mat.has_param,
+            mat.state ? "set" : "NULL");
   return FEM_ERR_BAD_ARG;
 }
 

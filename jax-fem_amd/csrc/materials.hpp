@@ -7,10 +7,13 @@
 //            problem.py:262-266 written out in closed form)
 //   jvp      dP = dP/dF : dG                             (matrix-free tangent product)
 //   dparam   dP/dtheta for the per-quad parameter        (adjoint, solver.py:1386-1416)
+// `cq` = cell * Q + quad indexes the per-quad parameter and state.
 #pragma once
 #include "fem_common.hpp"
 
 namespace fem {
+
+__device__ inline double param_at(const fem_material_t& m, int64_t cq) { return m.has_param ? m.param[cq] : 1.0; }
 
 // P = k grad u  (Poisson / heat, tests/benchmarks/linear_poisson/test_linear_poisson.py:15-17)
 struct MatPoisson {
@@ -21,9 +24,9 @@ struct MatPoisson {
     T k;
   };
   template <typename T>
-  __device__ static St<T> prep(const fem_material_t& m, double param, const T (&)[1][3]) {
+  __device__ static St<T> prep(const fem_material_t& m, int64_t cq, const T (&)[1][3]) {
     St<T> s;
-    s.k = T(m.c[0]) * (m.has_param ? T(param) : T(1));
+    s.k = T(m.c[0]) * T(param_at(m, cq));
     return s;
   }
   template <typename T>
@@ -40,7 +43,7 @@ struct MatPoisson {
     stress(s, dG, dP);
   }
   template <typename T>
-  __device__ static void dparam(const fem_material_t& m, double, const St<T>&, const T (&G)[1][3],
+  __device__ static void dparam(const fem_material_t& m, int64_t, const St<T>&, const T (&G)[1][3],
                                 T (&dP)[1][3]) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) dP[0][j] = m.has_param ? T(m.c[0]) * G[0][j] : T(0);
@@ -66,9 +69,9 @@ struct MatElastic {
     return m.c[4] == 0.0 ? 0.0 : (m.c[3] - m.c[2]) * m.c[4] * pow(t, m.c[4] - 1.0);
   }
   template <typename T>
-  __device__ static St<T> prep(const fem_material_t& m, double param, const T (&)[3][3]) {
+  __device__ static St<T> prep(const fem_material_t& m, int64_t cq, const T (&)[3][3]) {
     St<T> s;
-    const double E = kSimp ? modulus(m, param) : 1.0;
+    const double E = kSimp ? modulus(m, m.param[cq]) : 1.0;
     s.lam = T(m.c[0] * E);
     s.mu = T(m.c[1] * E);
     return s;
@@ -99,9 +102,9 @@ struct MatElastic {
     lin(s.lam, s.mu, dG, dP);
   }
   template <typename T>
-  __device__ static void dparam(const fem_material_t& m, double param, const St<T>&, const T (&G)[3][3],
+  __device__ static void dparam(const fem_material_t& m, int64_t cq, const St<T>&, const T (&G)[3][3],
                                 T (&dP)[3][3]) {
-    const double dE = kSimp ? dmodulus(m, param) : 0.0;
+    const double dE = kSimp ? dmodulus(m, m.param[cq]) : 0.0;
     lin(T(m.c[0] * dE), T(m.c[1] * dE), G, dP);
   }
 };
@@ -119,7 +122,7 @@ struct MatNeoHookean {
     T J, I1, a, ma, k;  // a = J^-2/3, ma = mu * a, k = kappa
   };
   template <typename T>
-  __device__ static St<T> prep(const fem_material_t& m, double param, const T (&G)[3][3]) {
+  __device__ static St<T> prep(const fem_material_t& m, int64_t cq, const T (&G)[3][3]) {
     St<T> s;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -136,7 +139,7 @@ struct MatNeoHookean {
 #pragma unroll
     for (int i = 0; i < 9; ++i) I1 += s.F[i] * s.F[i];
     s.I1 = I1;
-    const T sc = m.has_param ? T(param) : T(1);
+    const T sc = T(param_at(m, cq));
     s.a = pow(s.J, T(-2.0 / 3.0));
     s.ma = T(m.c[0]) * sc * s.a;
     s.k = T(m.c[1]) * sc;
@@ -200,7 +203,7 @@ struct MatNeoHookean {
       }
   }
   template <typename T>
-  __device__ static void dparam(const fem_material_t& m, double param, const St<T>& s, const T (&G)[3][3],
+  __device__ static void dparam(const fem_material_t& m, int64_t, const St<T>& s, const T (&G)[3][3],
                                 T (&dP)[3][3]) {
     if (!m.has_param) {
 #pragma unroll
@@ -216,6 +219,116 @@ struct MatNeoHookean {
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) dP[i][j] = ma * s.F[i * 3 + j] + (c2 - c1) * s.H[i * 3 + j];
+  }
+};
+
+// Small-strain J2 (von Mises) perfect plasticity with radial return from the
+// state of the last converged load step (tests/benchmarks/plasticity/
+// test_plasticity.py:32-64):
+//   eps = sym(G), sig_tr = lambda tr(eps - eps_old) I + 2 mu (eps - eps_old) + sig_old,
+//   s = dev(sig_tr), q = sqrt(3/2 s:s), f = q - sig_y,
+//   sig = sig_tr - max(f, 0) s / q.
+// Consistent tangent (the derivative of that map, which the reference obtains
+// by forward-mode AD): with beta = max(f, 0) / q and gam = 3 mu sig_y / q^3 on
+// the plastic branch (beta = gam = 0 on the elastic one),
+//   dsig = lam_t tr(deps) I + 2 mu_t deps - gam (s : deps) s,
+//   lam_t = lambda + 2 mu beta / 3, mu_t = mu (1 - beta).
+// c0 = lambda, c1 = mu, c2 = sig_y; state[cq] = (sig_old, eps_old), 3x3 row-major each.
+struct MatJ2 {
+  static constexpr int V = 3;
+  static constexpr bool kNeedsU = true;
+  template <typename T>
+  struct St {
+    T sig[9], s[9];
+    T lam_t, mu_t, gam;
+  };
+  template <typename T>
+  __device__ static St<T> prep(const fem_material_t& m, int64_t cq, const T (&G)[3][3]) {
+    St<T> st;
+    const double* so = m.state + cq * 18;
+    const double lam = m.c[0], mu = m.c[1], sy = m.c[2];
+    double de[9], tr = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) de[i * 3 + j] = 0.5 * (double(G[i][j]) + double(G[j][i])) - so[9 + i * 3 + j];
+    tr = de[0] + de[4] + de[8];
+    double sig[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sig[k] = 2.0 * mu * de[k] + so[k];
+    sig[0] += lam * tr;
+    sig[4] += lam * tr;
+    sig[8] += lam * tr;
+    const double p = (sig[0] + sig[4] + sig[8]) / 3.0;
+    double sd[9], ss = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      sd[k] = sig[k] - ((k == 0 || k == 4 || k == 8) ? p : 0.0);
+      ss += sd[k] * sd[k];
+    }
+    const double q = ss > 0.0 ? sqrt(1.5 * ss) : 0.0;
+    const double f = q - sy;
+    double beta = 0.0, gam = 0.0;
+    if (f > 0.0) {
+      beta = f / q;
+      gam = 3.0 * mu * sy / (q * q * q);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      st.sig[k] = T(sig[k] - beta * sd[k]);
+      st.s[k] = T(sd[k]);
+    }
+    st.lam_t = T(lam + 2.0 * mu * beta / 3.0);
+    st.mu_t = T(mu * (1.0 - beta));
+    st.gam = T(gam);
+    return st;
+  }
+  template <typename T>
+  __device__ static void stress(const St<T>& st, const T (&)[3][3], T (&P)[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) P[i][j] = st.sig[i * 3 + j];
+  }
+  template <typename T>
+  __device__ static void tangent(const St<T>& st, const T* ga, const T* gb, T w, T (&K)[3][3]) {
+    T sa[3], sb[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      sa[i] = st.s[i * 3 + 0] * ga[0] + st.s[i * 3 + 1] * ga[1] + st.s[i * 3 + 2] * ga[2];
+      sb[i] = st.s[i * 3 + 0] * gb[0] + st.s[i * 3 + 1] * gb[1] + st.s[i * 3 + 2] * gb[2];
+    }
+    const T l = w * st.lam_t, mt = w * st.mu_t, g = w * st.gam;
+    const T gg = mt * (ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        K[i][k] += l * ga[i] * gb[k] + mt * ga[k] * gb[i] + (i == k ? gg : T(0)) - g * sa[i] * sb[k];
+  }
+  template <typename T>
+  __device__ static void jvp(const St<T>& st, const T (&dG)[3][3], T (&dP)[3][3]) {
+    T de[9], sde = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        de[i * 3 + j] = T(0.5) * (dG[i][j] + dG[j][i]);
+        sde += st.s[i * 3 + j] * de[i * 3 + j];
+      }
+    const T tr = de[0] + de[4] + de[8];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        dP[i][j] = T(2) * st.mu_t * de[i * 3 + j] - st.gam * sde * st.s[i * 3 + j] + (i == j ? st.lam_t * tr : T(0));
+  }
+  template <typename T>
+  __device__ static void dparam(const fem_material_t&, int64_t, const St<T>&, const T (&)[3][3], T (&dP)[3][3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dP[i][j] = T(0);
   }
 };
 

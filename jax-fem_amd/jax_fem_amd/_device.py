@@ -285,7 +285,14 @@ class DeviceProblem:
         else:
             self.param_index = None
             self.param = None
-        self.mat_desc = mat.to_struct(self.param.data_ptr() if self.param is not None else None)
+        if mat.has_state:
+            sig = _flatten_iv(ivs[mat.sig_index], (C_, Q, 3, 3), dm.device).reshape(C_, Q, 9)
+            eps = _flatten_iv(ivs[mat.eps_index], (C_, Q, 3, 3), dm.device).reshape(C_, Q, 9)
+            self.state = torch.cat([sig, eps], dim=2).contiguous()
+        else:
+            self.state = None
+        self.mat_desc = mat.to_struct(self.param.data_ptr() if self.param is not None else None,
+                                      self.state.data_ptr() if self.state is not None else None)
         self.mref = C.byref(self.mat_desc)
         self.load = self._constant_loads(ivs)
 

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 FEM_OK, FEM_ERR_BAD_ARG, FEM_ERR_NOT_CONVERGED, FEM_ERR_NAN, FEM_ERR_HIP, FEM_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
 FEM_F32, FEM_F64, FEM_BF16X3 = 0, 1, 2
 FEM_FORM_STANDARD, FEM_FORM_DIFFERENCE = 0, 1
-FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN = 1, 2, 3, 4
+FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN, FEM_MAT_J2_PLASTIC = 1, 2, 3, 4, 5
 
 _vp = C.c_void_p
 _i32, _i64, _f64 = C.c_int32, C.c_int64, C.c_double
@@ -29,7 +29,7 @@ class FemMesh(C.Structure):
 
 
 class FemMaterial(C.Structure):
-    _fields_ = [('kind', _i32), ('has_param', _i32), ('c', _f64 * 6), ('param', _vp)]
+    _fields_ = [('kind', _i32), ('has_param', _i32), ('c', _f64 * 6), ('param', _vp), ('state', _vp)]
 
 
 class FemPattern(C.Structure):

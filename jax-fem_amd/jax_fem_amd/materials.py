@@ -6,7 +6,7 @@ AD of it (problem.py:262-266).  The device path has a fixed set of laws with
 closed-form tangents (csrc/materials.hpp).  ``get_tensor_map`` may return
 
 * one of the descriptors below (``Poisson``, ``LinearElastic``, ``SIMPElastic``,
-  ``NeoHookean``), or
+  ``NeoHookean``, ``J2Plastic``), or
 * a plain torch function ``f(u_grad, *internal_vars) -> P`` written like the
   reference's JAX one; ``identify`` evaluates it on sample gradients (float64,
   CPU) and matches it to one of the laws, recovering the constants.  A map
@@ -32,7 +32,9 @@ class Material:
     def constants(self):
         raise NotImplementedError
 
-    def to_struct(self, param_ptr=None):
+    has_state = False
+
+    def to_struct(self, param_ptr=None, state_ptr=None):
         m = _lib.FemMaterial()
         m.kind = self.kind
         m.has_param = 1 if self.has_param else 0
@@ -40,6 +42,7 @@ class Material:
         for i in range(6):
             m.c[i] = float(c[i])
         m.param = param_ptr
+        m.state = state_ptr
         return m
 
 
@@ -165,6 +168,43 @@ class NeoHookean(Material):
         return f"NeoHookean(mu={self.mu}, kappa={self.kappa}, param={self.has_param})"
 
 
+class J2Plastic(Material):
+    """Small-strain J2 perfect plasticity with radial return (test_plasticity.py:32-64).
+
+    The tensor map reads two internal variables, the stress and the strain of
+    the last converged load step (positions ``sig_index``, ``eps_index``):
+    sig_tr = lambda tr(eps - eps_old) I + 2 mu (eps - eps_old) + sig_old and the
+    deviatoric part is returned to the von Mises surface of radius ``sig0``.
+    """
+    kind = _lib.FEM_MAT_J2_PLASTIC
+    vec = 3
+    has_state = True
+
+    def __init__(self, E=None, nu=None, sig0=250., lmbda=None, mu=None, sig_index=0, eps_index=1):
+        if lmbda is None:
+            lmbda, mu = lame(float(E), float(nu))
+        self.lmbda, self.mu, self.sig0 = float(lmbda), float(mu), float(sig0)
+        self.sig_index, self.eps_index = int(sig_index), int(eps_index)
+
+    def constants(self):
+        return (self.lmbda, self.mu, self.sig0)
+
+    def __call__(self, u_grad, *iv):
+        sig_old, eps_old = torch.as_tensor(iv[self.sig_index]), torch.as_tensor(iv[self.eps_index])
+        eye = torch.eye(3, dtype=u_grad.dtype, device=u_grad.device)
+        de = 0.5 * (u_grad + u_grad.transpose(-1, -2)) - eps_old
+        tr = de.diagonal(dim1=-2, dim2=-1).sum(-1)[..., None, None]
+        sig = self.lmbda * tr * eye + 2. * self.mu * de + sig_old
+        s = sig - sig.diagonal(dim1=-2, dim2=-1).sum(-1)[..., None, None] / 3. * eye
+        q = torch.sqrt(torch.clamp(1.5 * (s * s).sum((-1, -2)), min=0.))[..., None, None]
+        f = torch.clamp(q - self.sig0, min=0.)
+        return sig - torch.where(q > 0, f / torch.where(q > 0, q, torch.ones_like(q)), torch.zeros_like(q)) * s
+
+    def __repr__(self):
+        return (f"J2Plastic(lmbda={self.lmbda}, mu={self.mu}, sig0={self.sig0}, "
+                f"sig_index={self.sig_index}, eps_index={self.eps_index})")
+
+
 def _scalar(x):
     x = torch.as_tensor(x)
     return x.reshape(x.shape[:-1]) if x.dim() and x.shape[-1] == 1 else x
@@ -214,6 +254,8 @@ def identify(tensor_map, vec, iv_shapes=()):
         alt[k] = _iv_sample(iv_shapes[k], 0.35)
         depends.append(not _close(_call(tensor_map, G, alt), P0, float(P0.abs().max())))
     driving = [k for k in range(n_iv) if depends[k]]
+    if vec == 3 and len(driving) == 2 and all(tuple(iv_shapes[k]) == (3, 3) for k in driving):
+        return _identify_j2(tensor_map, iv_shapes, driving, gen)
     if len(driving) > 1:
         raise NotImplementedError("tensor maps driven by more than one internal variable are not on the device path")
     pk = driving[0] if driving else None
@@ -282,6 +324,64 @@ def identify(tensor_map, vec, iv_shapes=()):
                 abs(c[1] - kappa * th) > 1e-8 * abs(kappa) * max(1, th):
             raise NotImplementedError("Neo-Hookean map must scale linearly with its internal variable")
     return NeoHookean(mu=mu, kappa=kappa, param=True)
+
+
+def _identify_j2(tensor_map, iv_shapes, driving, gen):
+    """J2 plasticity driven by (sig_old, eps_old): find which internal variable is which,
+    the elastic constants and the yield stress, then verify the whole map on random
+    elastic and plastic states against ``J2Plastic``."""
+    n_iv = len(iv_shapes)
+    zero = [torch.zeros(tuple(s), dtype=torch.float64) for s in iv_shapes]
+    sym = lambda A: 0.5 * (A + A.T)
+
+    def rnd(scale):
+        return (torch.rand((3, 3), generator=gen, dtype=torch.float64) - 0.5) * 2 * scale
+
+    def call(G, ivs):
+        return _call(tensor_map, G, ivs)
+
+    # at G = 0 with a small stress in one slot and zero elsewhere, the output is that stress
+    S = sym(rnd(1e-6))
+    a, b = driving
+    ivs = list(zero)
+    ivs[a] = S
+    sig_index, eps_index = (a, b) if _close(call(torch.zeros(3, 3, dtype=torch.float64), ivs), S, 1e-6) else (b, a)
+    # elastic constants from tiny strains (far inside any yield surface)
+    f0 = lambda G: call(G, zero)
+    E01 = torch.zeros(3, 3, dtype=torch.float64)
+    E01[0, 1] = 1e-9
+    mu = float(f0(E01)[0, 1]) / 1e-9
+    E00 = torch.zeros(3, 3, dtype=torch.float64)
+    E00[0, 0] = 1e-9
+    lmbda = float(f0(E00)[0, 0]) / 1e-9 - 2. * mu
+    if not (mu > 0):
+        raise NotImplementedError("tensor map with two 3x3 internal variables is not recognised as J2 plasticity")
+    # yield stress: a pure shear past the yield point returns the von Mises stress sig0;
+    # the shear is grown by decades so the first yielding one is only ~10x past yield
+    sig0 = None
+    for t in (10.0 ** k for k in range(-9, 4)):
+        Gs = torch.zeros(3, 3, dtype=torch.float64)
+        Gs[0, 1] = Gs[1, 0] = t
+        P = f0(Gs)
+        s = P - torch.trace(P) / 3. * torch.eye(3, dtype=torch.float64)
+        q = float(torch.sqrt(1.5 * (s * s).sum()))
+        if q < 0.99 * math.sqrt(3.) * 2. * mu * t:
+            sig0 = q
+            break
+    if sig0 is None:
+        raise NotImplementedError("tensor map with two 3x3 internal variables does not yield (no J2 law found)")
+    mat = J2Plastic(lmbda=lmbda, mu=mu, sig0=sig0, sig_index=sig_index, eps_index=eps_index)
+    # verify on elastic and plastic states
+    ystrain = sig0 / (2. * mu)
+    for scale in (0.1, 0.5, 2.0, 10.0):
+        G = rnd(scale * ystrain)
+        ivs = list(zero)
+        ivs[sig_index] = sym(rnd(0.5 * sig0))
+        ivs[eps_index] = sym(rnd(scale * ystrain))
+        got, ref = call(G, ivs), mat(G, *ivs)
+        if not _close(got, ref, float(ref.abs().max())):
+            raise NotImplementedError("tensor map with two 3x3 internal variables is not recognised as J2 plasticity")
+    return mat
 
 
 def _linear_constants(f, vec, rnd):

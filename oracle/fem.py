@@ -90,7 +90,8 @@ class Material:
     """Stress P(G) and its derivative A_iJkL = dP_iJ/dG_kL, batched over (...,V,3).
 
     kind: 'poisson' (k), 'elastic' (lmbda, mu), 'simp' (lmbda1, mu1, Emin, E0, p; per-quad
-    theta), 'neo_hookean' (mu, kappa; optional per-quad scale).
+    theta), 'neo_hookean' (mu, kappa; optional per-quad scale), 'j2' (lmbda, mu, sig0; per-quad
+    state (..., 18) = sigma_old then eps_old, 3x3 row-major, passed as theta).
     """
 
     def __init__(self, kind, **c):
@@ -107,7 +108,27 @@ class Material:
         c = self.c
         return (c['E0'] - c['Emin']) * c['p'] * theta ** (c['p'] - 1)
 
+    def _j2(self, G, state):
+        """Radial return of test_plasticity.py:47-64: stress, deviator, beta = f+/q, gam = 3 mu sig0/q^3."""
+        c = self.c
+        st = np.asarray(state)
+        sig_old = st[..., :9].reshape(st.shape[:-1] + (3, 3))
+        eps_old = st[..., 9:].reshape(st.shape[:-1] + (3, 3))
+        de = 0.5 * (G + np.swapaxes(G, -1, -2)) - eps_old
+        tr = np.trace(de, axis1=-2, axis2=-1)[..., None, None]
+        sig = c['lmbda'] * tr * np.eye(3) + 2. * c['mu'] * de + sig_old
+        sd = sig - np.trace(sig, axis1=-2, axis2=-1)[..., None, None] / 3. * np.eye(3)
+        q = np.sqrt(1.5 * np.sum(sd * sd, axis=(-2, -1)))
+        f = q - c['sig0']
+        plastic = f > 0
+        qs = np.where(plastic, q, 1.)
+        beta = np.where(plastic, f / qs, 0.)
+        gam = np.where(plastic, 3. * c['mu'] * c['sig0'] / qs ** 3, 0.)
+        return sig - beta[..., None, None] * sd, sd, beta, gam
+
     def stress(self, G, theta=None):
+        if self.kind == 'j2':
+            return self._j2(G, theta)[0]
         if self.kind == 'poisson':
             k = self.c['k'] * (1. if theta is None else np.asarray(theta)[..., None, None])
             return k * G
@@ -128,6 +149,14 @@ class Material:
     def tangent(self, G, theta=None):
         """A (...,V,3,V,3)."""
         d = np.eye(3)
+        if self.kind == 'j2':
+            _, sd, beta, gam = self._j2(G, theta)
+            c = self.c
+            lt = (c['lmbda'] + 2. * c['mu'] * beta / 3.)[..., None, None, None, None]
+            mt = (c['mu'] * (1. - beta))[..., None, None, None, None]
+            return lt * np.einsum('ij,kl->ijkl', d, d) + mt * (np.einsum('ik,jl->ijkl', d, d) +
+                                                             np.einsum('il,jk->ijkl', d, d)) - \
+                gam[..., None, None, None, None] * np.einsum('...ij,...kl->...ijkl', sd, sd)
         if self.kind == 'poisson':
             k = self.c['k'] * (1. if theta is None else np.asarray(theta)[..., None, None, None, None])
             return k * np.broadcast_to(d[None, :, None, :], G.shape[:-2] + (1, 3, 1, 3))

@@ -112,3 +112,63 @@ class LoadedElastic(Problem):
 
     def get_surface_maps(self):
         return [lambda u, x: torch.tensor([0., 0., 1.], dtype=torch.float64)]
+
+
+class Plasticity(Problem):
+    """J2 plasticity of tests/benchmarks/plasticity/test_plasticity.py:16-87, with torch in
+    place of jax.numpy: the internal variables are the stress and strain of the last load step."""
+    def custom_init(self):
+        self.fe = self.fes[0]
+        C, Q = self.fe.num_cells, self.fe.num_quads
+        self.epsilons_old = torch.zeros((C, Q, self.fe.vec, self.dim), dtype=torch.float64)
+        self.sigmas_old = torch.zeros_like(self.epsilons_old)
+        self.internal_vars = [self.sigmas_old, self.epsilons_old]
+
+    def get_tensor_map(self):
+        _, stress_return_map = self.get_maps()
+        return stress_return_map
+
+    def get_maps(self):
+        def safe_sqrt(x):
+            return torch.where(x > 0., torch.sqrt(torch.clamp(x, min=0.)), 0. * x)
+
+        def safe_divide(x, y):
+            return torch.where(y == 0., 0. * x, x / torch.where(y == 0., torch.ones_like(y), y))
+
+        def strain(u_grad):
+            return 0.5 * (u_grad + u_grad.T)
+
+        def stress(epsilon):
+            E, nu = 70.e3, 0.3
+            mu = E / (2. * (1. + nu))
+            lmbda = E * nu / ((1 + nu) * (1 - 2 * nu))
+            return lmbda * torch.trace(epsilon) * torch.eye(self.dim, dtype=epsilon.dtype,
+                                                            device=epsilon.device) + 2 * mu * epsilon
+
+        def stress_return_map(u_grad, sigma_old, epsilon_old):
+            sig0 = 250.
+            epsilon_inc = strain(u_grad) - epsilon_old
+            sigma_trial = stress(epsilon_inc) + sigma_old
+            s_dev = sigma_trial - 1. / self.dim * torch.trace(sigma_trial) * torch.eye(
+                self.dim, dtype=u_grad.dtype, device=u_grad.device)
+            s_norm = safe_sqrt(3. / 2. * torch.sum(s_dev * s_dev))
+            f_yield = s_norm - sig0
+            f_yield_plus = torch.where(f_yield > 0., f_yield, 0. * f_yield)
+            return sigma_trial - safe_divide(f_yield_plus * s_dev, s_norm)
+        return strain, stress_return_map
+
+    def update_stress_strain(self, sol):
+        u_grads = self.fe.sol_to_grad(sol)
+        strain, stress_rm = self.get_maps()
+        vs = torch.func.vmap(torch.func.vmap(stress_rm))
+        ve = torch.func.vmap(torch.func.vmap(strain))
+        sig_old = torch.as_tensor(self.sigmas_old, device=u_grads.device)
+        eps_old = torch.as_tensor(self.epsilons_old, device=u_grads.device)
+        self.sigmas_old = vs(u_grads, sig_old, eps_old)
+        self.epsilons_old = ve(u_grads)
+        self.internal_vars = [self.sigmas_old, self.epsilons_old]
+
+    def compute_avg_stress(self):
+        JxW = self.fe.JxW
+        sig = (self.sigmas_old.reshape(-1, 3, 3) * JxW.reshape(-1)[:, None, None]).sum(0)
+        return sig / JxW.sum()

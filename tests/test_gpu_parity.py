@@ -13,7 +13,7 @@ from conftest import golden
 from oracle import basis as OB
 from oracle import fem as OF
 from problems import (LinearPoisson, LinearElasticityCube, LinearElasticityCylinder, HyperElasticity,
-                      SIMPElasticity, LoadedElastic, box, isclose)
+                      Plasticity, SIMPElasticity, LoadedElastic, box, isclose)
 
 from jax_fem_amd import _lib
 from jax_fem_amd.generate_mesh import Mesh, box_mesh_hex27
@@ -351,6 +351,66 @@ def _fixture_hyper(precision):
     tz = float((trac * ns[..., None]).sum((0, 1))[2])
     np.testing.assert_almost_equal(tz, float(d['traction']), decimal=5)
     return sol, d['sol'], 5
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_fenicsx_plasticity_fixture(gpu, precision):
+    """J2 plasticity, 5 load steps with the state carried by internal variables
+    (test_plasticity.py:94-161): solution at decimal 5, average sigma_zz at decimal 3."""
+    d = golden('plasticity')
+    mesh = Mesh(d['points'], d['cells'])
+    top, bottom = (lambda p: isclose(p[2], 10.)), (lambda p: isclose(p[2], 0.))
+
+    def bc(disp):
+        return [[bottom] * 3 + [top] * 3, [0, 1, 2, 0, 1, 2], [lambda p: 0.] * 5 + [lambda p: disp]]
+
+    problem = Plasticity(mesh, vec=3, dim=3, dirichlet_bc_info=bc(float(d['disps'][0])))
+    avg = []
+    for disp in d['disps']:
+        problem.fe.update_Dirichlet_boundary_conditions(bc(float(disp)))
+        sol = solver(problem, {'jax_solver': {'precision': precision}})[0]
+        problem.update_stress_strain(sol)
+        avg.append(float(problem.compute_avg_stress()[2, 2]))
+    assert isinstance(problem.device().material, M.J2Plastic)
+    got = sol.cpu().numpy().astype(np.float32)
+    np.testing.assert_array_almost_equal(d['sol'], got, decimal=5)
+    np.testing.assert_array_almost_equal(d['avg_stresses'], np.array(avg), decimal=3)
+
+
+def test_j2_tangent_and_residual_vs_oracle(gpu):
+    """J2 tangents and residuals on a distorted mesh with a random stress/strain history that
+    puts most quadrature points on the plastic branch."""
+    mesh = distorted_box((3, 3, 4), seed=4)
+    p = Plasticity(mesh, vec=3, dim=3, dirichlet_bc_info=[[lambda x: isclose(x[0], 0.)] * 3, [0, 1, 2],
+                                                         [lambda x: 0.] * 3])
+    C, Q = p.num_cells, p.fes[0].num_quads
+    rng = _rng(5)
+    sym = lambda a: 0.5 * (a + np.swapaxes(a, -1, -2))
+    sig = sym(150. * rng.standard_normal((C, Q, 3, 3)))
+    eps = sym(2e-3 * rng.standard_normal((C, Q, 3, 3)))
+    p.internal_vars = [torch.as_tensor(sig), torch.as_tensor(eps)]
+    u = 2e-3 * rng.standard_normal(p.fes[0].num_total_dofs)
+    dev = p.device()
+    dev.prepare()
+    ut = torch.as_tensor(u, device='cuda')
+    lm, mu = M.lame(70e3, 0.3)
+    om = OF.Material('j2', lmbda=lm, mu=mu, sig0=250.)
+    state = np.concatenate([sig.reshape(C, Q, 9), eps.reshape(C, Q, 9)], 2)
+    o = oracle_problem(p, om, state)
+    U = u.reshape(-1, 3)
+    Gu = np.einsum('cav,cqaj->cqvj', U[o.cells], o.G)
+    assert (om._j2(Gu, state)[2] > 0).mean() > 0.5
+    K64 = dev.cell_tangents(ut, _lib.FEM_F64).cpu().numpy()
+    Ko = OF.cell_tangent(om, U, o.cells, o.G, o.JxW, state)
+    assert rel(K64, Ko) < 1e-12
+    r = p.compute_residual([ut.reshape(-1, 3)])[0].cpu().numpy()
+    ro = OF.scatter(OF.cell_residual(om, U, o.cells, o.G, o.JxW, state), o.cells, len(o.points))
+    assert rel(r, ro) < 1e-12
+    x = torch.as_tensor(_rng(9).standard_normal(dev.n), device='cuda')
+    dev.tangent(ut, _lib.FEM_F64)
+    o.bc_dofs = dev.bc_dofs.cpu().numpy().astype(np.int64)
+    A = o.tangent(u)
+    assert rel(dev.apply_operator(ut, x).cpu().numpy(), A @ x.cpu().numpy()) < 1e-12
 
 
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])

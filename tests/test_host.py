@@ -10,7 +10,7 @@ import torch
 from conftest import ROOT
 from oracle import basis as OB
 from oracle import fem as OF
-from problems import LinearElasticityCube, LinearPoisson, HyperElasticity, SIMPElasticity, box, isclose
+from problems import LinearElasticityCube, LinearPoisson, HyperElasticity, Plasticity, SIMPElasticity, box, isclose
 
 from jax_fem_amd import _lib, basis, materials as M
 from jax_fem_amd.generate_mesh import box_mesh, box_mesh_hex27, Mesh
@@ -32,7 +32,7 @@ def test_library_exports_every_header_symbol():
 def test_struct_sizes_match_header_layout():
     # plain C layout: ints first, then 64-bit fields / pointers
     assert ctypes.sizeof(_lib.FemMesh) == 4 * 4 + 2 * 8 + 9 * 8
-    assert ctypes.sizeof(_lib.FemMaterial) == 8 + 6 * 8 + 8
+    assert ctypes.sizeof(_lib.FemMaterial) == 8 + 6 * 8 + 8 + 8
     assert ctypes.sizeof(_lib.FemPattern) == 4 * 8 + 4 * 4 + 9 * 8
     assert ctypes.sizeof(_lib.FemPcgOptions) == 8 + 8 + 4 * 4 + 8 + 8
     assert ctypes.sizeof(_lib.FemPcgResult) == 16 + 16 + 16 + 8
@@ -175,3 +175,29 @@ def test_material_descriptors_agree_with_oracle():
                                OF.Material('elastic', lmbda=lm, mu=mu).stress(G.numpy()), rtol=1e-13)
     np.testing.assert_allclose(M.NeoHookean(E=10., nu=0.3)(G).numpy(),
                                OF.Material('neo_hookean', mu=10 / 2.6, kappa=10 / 1.2).stress(G.numpy()), rtol=1e-12)
+
+
+def test_identify_reference_j2_return_map():
+    # the reference's plasticity tensor map (sigma_old, eps_old as internal variables)
+    p = Plasticity.__new__(Plasticity)
+    p.dim = 3
+    mat = M.identify(p.get_tensor_map(), 3, ((3, 3), (3, 3)))
+    assert isinstance(mat, M.J2Plastic)
+    lm, mu = M.lame(70e3, 0.3)
+    assert abs(mat.lmbda - lm) < 1e-6 * lm and abs(mat.mu - mu) < 1e-6 * mu and abs(mat.sig0 - 250.) < 1e-9
+    assert (mat.sig_index, mat.eps_index) == (0, 1)
+    # swapped argument order is found too
+    f = p.get_tensor_map()
+    mat2 = M.identify(lambda G, e, s: f(G, s, e), 3, ((3, 3), (3, 3)))
+    assert (mat2.sig_index, mat2.eps_index) == (1, 0)
+    # the descriptor reproduces the map on plastic states (and the oracle's return map)
+    rng = np.random.default_rng(3)
+    sym = lambda a: 0.5 * (a + a.T)
+    o = OF.Material('j2', lmbda=lm, mu=mu, sig0=250.)
+    for _ in range(5):
+        G = torch.as_tensor(0.02 * rng.standard_normal((3, 3)))
+        so = torch.as_tensor(sym(200 * rng.standard_normal((3, 3))))
+        eo = torch.as_tensor(sym(1e-3 * rng.standard_normal((3, 3))))
+        np.testing.assert_allclose(mat(G, so, eo).numpy(), f(G, so, eo).numpy(), rtol=1e-12, atol=1e-9)
+        st = np.concatenate([so.numpy().reshape(9), eo.numpy().reshape(9)])
+        np.testing.assert_allclose(o.stress(G.numpy(), st), f(G, so, eo).numpy(), rtol=1e-12, atol=1e-9)

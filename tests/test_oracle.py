@@ -326,3 +326,54 @@ def test_adjoint_matches_central_differences_and_self_adjointness():
         fd = (compliance(tp)[0] - compliance(tm)[0]) / (2 * h)
         np.testing.assert_allclose(g[e], fd, rtol=1e-5)
         assert g[e] < 0
+
+
+def test_j2_tangent_is_derivative_of_return_map():
+    E, nu = 70e3, 0.3
+    lm, mu = OF.lame(E, nu)
+    m = OF.Material('j2', lmbda=lm, mu=mu, sig0=250.)
+    rng = np.random.default_rng(0)
+    sym = lambda a: 0.5 * (a + np.swapaxes(a, -1, -2))
+    G = 0.01 * rng.standard_normal((6, 3, 3))
+    st = np.concatenate([sym(100 * rng.standard_normal((6, 3, 3))).reshape(6, 9),
+                         sym(1e-3 * rng.standard_normal((6, 3, 3))).reshape(6, 9)], 1)
+    st[0] = 0.
+    G[0] *= 1e-4                                   # an elastic point
+    beta = m._j2(G, st)[2]
+    assert beta[0] == 0 and np.all(beta[1:] > 0)   # and plastic ones
+    A = m.tangent(G, st)
+    h = 1e-7
+    for k in range(3):
+        for l in range(3):
+            E_ = np.zeros((3, 3))
+            E_[k, l] = h
+            fd = (m.stress(G + E_, st) - m.stress(G - E_, st)) / (2 * h)
+            np.testing.assert_allclose(A[..., k, l], fd, rtol=1e-5, atol=1e-6 * np.abs(A).max())
+    # returned stresses lie on the yield surface
+    s = m.stress(G, st)
+    sd = s - np.trace(s, axis1=-2, axis2=-1)[..., None, None] / 3 * np.eye(3)
+    np.testing.assert_allclose(np.sqrt(1.5 * (sd * sd).sum((-2, -1)))[1:], 250., rtol=1e-12)
+
+
+def test_fixture_plasticity_solution_and_average_stress():
+    d = golden('plasticity')
+    lm, mu = OF.lame(70e3, 0.3)
+    pts, cells = d['points'], d['cells']
+    top, bottom = _close(2, 10.), _close(2, 0.)
+    probe = OF.Problem(pts, cells, 3)
+    C, Q = probe.JxW.shape
+    state = np.zeros((C, Q, 18))
+    u = None
+    avg = []
+    # load history of test_plasticity.py:105-145: top face pulled to each disp in turn
+    for disp in d['disps']:
+        bc = [[bottom] * 3 + [top] * 3, [0, 1, 2, 0, 1, 2], [lambda x: 0.] * 5 + [lambda x: float(disp)]]
+        p = OF.Problem(pts, cells, 3, mat=OF.Material('j2', lmbda=lm, mu=mu, sig0=250.), theta=state, bc_info=bc)
+        u = p.newton(linear='direct')
+        # update_stress_strain (test_plasticity.py:72-77)
+        Gu = np.einsum('cav,cqaj->cqvj', u.reshape(-1, 3)[cells], p.G)
+        sig = p.mat.stress(Gu, state)
+        state = np.concatenate([sig.reshape(C, Q, 9), (0.5 * (Gu + np.swapaxes(Gu, -1, -2))).reshape(C, Q, 9)], 2)
+        avg.append((sig[..., 2, 2] * p.JxW).sum() / p.JxW.sum())
+    np.testing.assert_allclose(u.reshape(-1, 3), d['sol'], atol=1e-5)
+    np.testing.assert_allclose(avg, d['avg_stresses'], atol=1e-3)
