@@ -243,3 +243,112 @@ def test_hex27_full_size_mixed_matches_fp64(gpu):
     ue = u.reshape(-1, 3)[p.device().dm.cells.long()]
     energy2 = (p.device().cell_internal_force(u) * ue).sum((1, 2))
     assert _rel((-energy2 / E).cpu(), g.cpu()) < 1e-9
+
+
+# ------------------------------------------- gradients through the load maps
+class PoissonLoadParams(Problem):
+    """Source and flux depend on parameters: mass map -theta (1 + x) with a per-cell theta,
+    surface map -s with a per-face-quad s (internal_vars_surfaces)."""
+    def get_tensor_map(self):
+        return lambda u_grad, th: u_grad
+
+    def get_mass_map(self):
+        return lambda u, x, th: -(th * (1. + x[0])).reshape(1)
+
+    def get_surface_maps(self):
+        return [lambda u, x, s: -s.reshape(1)]
+
+    def set_params(self, params):
+        theta, s = params
+        Q = self.fes[0].num_quads
+        S, FQ = len(self.boundary_inds_list[0]), self.fes[0].face_shape_vals.shape[1]
+        self.internal_vars = [theta.reshape(-1, 1).expand(-1, Q)]
+        self.internal_vars_surfaces = [[s.reshape(1, 1).expand(S, FQ)]]
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_gradient_through_mass_and_surface_maps(gpu, precision):
+    left = lambda x: isclose(x[0], 0.)
+    p = PoissonLoadParams(box(6, 4, 4), vec=1, dim=3, dirichlet_bc_info=[[left], [0], [lambda x: 0.]],
+                          location_fns=[lambda x: isclose(x[0], 1.)])
+    opts = {'jax_solver': {'precision': precision, 'tol': 1e-12, 'atol': 0.}}
+    fwd = ad_wrapper(p, opts, opts)
+    rng = np.random.default_rng(2)
+    theta = torch.as_tensor(1. + 0.3 * rng.standard_normal(p.num_cells), device='cuda').requires_grad_(True)
+    s = torch.tensor(0.7, dtype=torch.float64, device='cuda', requires_grad=True)
+    w = torch.as_tensor(rng.standard_normal(p.fes[0].num_total_nodes), device='cuda')
+    J = torch.dot(w, fwd((theta, s))[0].reshape(-1))
+    J.backward()
+    g_th, g_s = theta.grad.cpu().numpy(), float(s.grad)
+
+    def J_of(th, sv):
+        with torch.no_grad():
+            return float(torch.dot(w, fwd((th, sv))[0].reshape(-1)))
+    # the problem is linear in (theta, s): central differences are exact up to the solve tolerance
+    h = 1e-3
+    th0 = theta.detach()
+    for c in rng.choice(p.num_cells, 4, replace=False):
+        tp, tm = th0.clone(), th0.clone()
+        tp[c] += h
+        tm[c] -= h
+        fd = (J_of(tp, s.detach()) - J_of(tm, s.detach())) / (2 * h)
+        assert abs(fd - g_th[c]) <= 1e-6 * np.abs(g_th).max(), (c, fd, g_th[c])
+    fd_s = (J_of(th0, s.detach() + h) - J_of(th0, s.detach() - h)) / (2 * h)
+    assert abs(fd_s - g_s) <= 1e-7 * abs(g_s)
+
+
+# -------------------------------------- gradients of the other parameterised laws
+class ScaledNeoHookean(Problem):
+    """Neo-Hookean with mu and kappa scaled by a per-cell field (compute_gradients notebook, cell 11)."""
+    def get_tensor_map(self):
+        E, nu = 10., 0.3
+        mu, kappa = E / (2. * (1. + nu)), E / (3. * (1. - 2. * nu))
+
+        def psi(F, rho):
+            J = torch.linalg.det(F)
+            return rho * (mu / 2. * (J ** (-2. / 3.) * torch.trace(F.T @ F) - 3.) + kappa / 2. * (J - 1.) ** 2)
+        P = torch.func.grad(psi)
+        return lambda u_grad, rho: P(u_grad + torch.eye(3, dtype=u_grad.dtype, device=u_grad.device), rho[0])
+
+    def set_params(self, rho):
+        self.internal_vars = [rho.reshape(-1, 1)[:, None, :].expand(-1, self.fes[0].num_quads, 1)]
+
+
+class ScaledPoisson(Problem):
+    def get_tensor_map(self):
+        return lambda u_grad, k: k * u_grad
+
+    def get_mass_map(self):
+        return lambda u, x, k: torch.ones(1, dtype=torch.float64)
+
+    def set_params(self, k):
+        self.internal_vars = [k.reshape(-1, 1).expand(-1, self.fes[0].num_quads)]
+
+
+@pytest.mark.parametrize('law', ['neo_hookean', 'poisson'])
+def test_parameter_gradient_vs_central_differences(gpu, law):
+    rng = np.random.default_rng(4)
+    if law == 'neo_hookean':
+        bottom, top = (lambda x: isclose(x[2], 0.)), (lambda x: isclose(x[2], 1.))
+        p = ScaledNeoHookean(box(3, 3, 4), vec=3, dim=3,
+                             dirichlet_bc_info=[[bottom] * 3 + [top], [0, 1, 2, 2], [lambda x: 0.] * 3 + [lambda x: 0.08]])
+    else:
+        p = ScaledPoisson(box(5, 4, 4), vec=1, dim=3, dirichlet_bc_info=[[lambda x: isclose(x[0], 0.)], [0],
+                                                                          [lambda x: 0.]])
+    opts = {'jax_solver': {'precision': 'fp64', 'tol': 1e-13, 'atol': 0.}, 'tol': 1e-12, 'rel_tol': 1e-14}
+    fwd = ad_wrapper(p, opts, opts)
+    rho = torch.as_tensor(1. + 0.2 * rng.uniform(-1, 1, p.num_cells), device='cuda')
+    w = torch.as_tensor(rng.standard_normal(p.fes[0].num_total_dofs), device='cuda')
+    r = rho.clone().requires_grad_(True)
+    torch.dot(w, fwd(r)[0].reshape(-1)).backward()
+    g = r.grad.cpu().numpy()
+    h = 1e-6
+    for c in rng.choice(p.num_cells, 4, replace=False):
+        vals = []
+        for sg in (1., -1.):
+            t = rho.clone()
+            t[c] += sg * h
+            with torch.no_grad():
+                vals.append(float(torch.dot(w, fwd(t)[0].reshape(-1))))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - g[c]) <= 1e-5 * np.abs(g).max(), (law, c, fd, g[c])
