@@ -193,6 +193,8 @@ class DeviceProblem:
                                  ptr(err), stream_ptr()), 'fem_pattern_rows')
         if int(err.item()) > 0:
             raise _lib.FemError(f"a node row has {int(err.item())} candidate blocks (capacity 2048)")
+        if int(row_len.sum(dtype=torch.int64).item()) >= 2 ** 31:
+            raise _lib.FemError("the block pattern has more than 2^31 blocks (int32 row pointers)")
         row_ptr = dm.scan(row_len)
         nnzb = int(row_ptr[-1].item())
         col_idx = dm.empty(nnzb, dtype=torch.int32)
@@ -211,6 +213,8 @@ class DeviceProblem:
                   'fem_sell_order')
             check(L.fem_sell_slice_sizes(Nn, ptr(row_ptr), ptr(sell_row), ptr(slice_blocks), stream_ptr()),
                   'fem_sell_slice_sizes')
+            if int(slice_blocks.sum(dtype=torch.int64).item()) >= 2 ** 31:
+                raise _lib.FemError("the SELL-64 layout needs more than 2^31 blocks (int32 slice pointers)")
             slice_ptr = dm.scan(slice_blocks)
             sell_blocks = int(slice_ptr[-1].item())
             if sell_blocks <= 1.05 * nnzb:
