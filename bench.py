@@ -91,11 +91,11 @@ def measured_traffic(cells):
 
 
 def sell_bytes(dev, elem_bytes, vec=3):
-    """Algorithmic bytes of one fused SpMV launch: the matrix blocks (values + one column
-    index each, padding not counted) once, z and p_old read once per row (gathers counted
-    once), p_new and q written."""
+    """Algorithmic bytes of one SpMV launch of the PCG iteration: the matrix blocks (values +
+    one column index each, padding not counted) once, p read once (gathers counted once),
+    q written."""
     n = dev.n
-    return dev.nnzb * (vec * vec * elem_bytes + 4) + n * elem_bytes * 4
+    return dev.nnzb * (vec * vec * elem_bytes + 4) + n * elem_bytes * 2
 
 
 def run(args):
@@ -210,7 +210,7 @@ def run(args):
         "pcg_iterations": {"mixed": mixed['pcg_iterations'], "fp64": res['fp64']['pcg_iterations'],
                            "mixed_replacements": mixed['replacements']},
         "roofline": {"bound": "hbm",
-                     "kernel": "k_pcg_a<3,true,float> (p update + SELL-64 3x3-block SpMV, FP32 storage, FP64 "
+                     "kernel": "k_pcg_a<3,true,float> (SELL-64 3x3-block SpMV q = Ap, FP32 storage, FP64 "
                                "arithmetic, block partials of <p,Ap>)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,

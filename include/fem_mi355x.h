@@ -154,7 +154,7 @@ typedef struct {
   double  reliable_delta;        /* FP32 mode: replace the residual when the
                                     recursive |r| fell by this factor         */
   int32_t xcd_remap;             /* 1: XCD-contiguous block order in SpMV     */
-  int32_t timing;                /* 1: time the two iteration kernels with HIP
+  int32_t timing;                /* 1: time the iteration phases with HIP
                                     events (adds a record per launch)         */
 } fem_pcg_options_t;
 
@@ -165,9 +165,10 @@ typedef struct {
   int32_t pad;
   double  b_norm;                /* |b|                                       */
   double  res_norm;              /* final true residual norm |b - A x|        */
-  double  spmv_ms;               /* timing=1: mean duration of the fused SpMV
-                                    kernel (p update, A p, <p, A p>)          */
-  double  update_ms;             /* timing=1: mean duration of the update kernel */
+  double  spmv_ms;               /* timing=1: mean duration of the SpMV kernel
+                                    (A p and the partials of <p, A p>)       */
+  double  update_ms;             /* timing=1: mean duration of the rest of an
+                                    iteration (p update, reductions, x/r/z)  */
   int32_t timed_launches;        /* launches averaged into spmv_ms            */
   int32_t pad2;
 } fem_pcg_result_t;

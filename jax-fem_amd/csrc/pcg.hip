@@ -7,12 +7,15 @@
 // exact boundary increment (solver.py:402-409), so the residual and search
 // direction vanish on those rows and CG runs on the SPD free block.
 //
-// Four launches per iteration k:
-//   A_k : p = z + beta p_old (ping-pong buffers), q = A p, per-block <p, q>
-//         (one 64-row slice per block, W waves splitting its slots)
+// Five launches per iteration k:
+//   P_k : p = z + beta p (row-local, in place)
+//   A_k : q = A p, per-block <p, q> (one 64-row slice per block, W waves
+//         splitting its slots; the product gathers p only)
 //   RA_k: one block: pq = sum of A's partials, alpha = rz / pq
 //   B_k : x += alpha p, r -= alpha q, z = D^-1 r, per-block <r,z>, <r,r>
 //   RB_k: one block: rz, rr; convergence / NaN / maxiter test; beta = rz / rz_old
+// Forming p inside the product (gathering z and p_old per column) cost ~20% of
+// the product's bandwidth; the separate P pass streams 3 vectors once.
 // The reductions sum per-block partials in a fixed order (bit-reproducible);
 // no grid-wide fence or atomic is used.  Scalars live in device memory and
 // are written by one kernel and read by the next.
@@ -54,28 +57,35 @@ __device__ inline double block_reduce_partials(const double* __restrict__ p, int
 
 
 
+// Kernel P: p = z + beta p in place (beta = 0 at a start).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pcg_p(int64_t n, const T* __restrict__ z, T* __restrict__ p,
+                                               const PcgScalars* sc) {
+  if (sc->done) return;
+  const T beta = T(sc->beta);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+
 // Kernel A: one 64-row slice per block; the W waves split the slots
-// (sell_slice_partial) and wave 0 combines them through LDS, forms
-// p = z + beta p_old at the row, stores p and q = A p and the block's <p, q>.
-// The input of the product, x_c = z_c + beta p_old_c, is formed on the fly.
+// (sell_slice_partial) and wave 0 combines them through LDS, stores
+// q = A p and the block's <p, q>.
 template <int V, bool DIFF, typename T>
 __global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, const T* __restrict__ vals,
-                                                           const T* __restrict__ z, const T* __restrict__ p_old,
-                                                           T* __restrict__ p_new, T* __restrict__ q,
+                                                           const T* __restrict__ p, T* __restrict__ q,
                                                            const PcgScalars* sc, double* __restrict__ partA,
                                                            int remap) {
   constexpr int W = SliceCfg<T>::W, U = SliceCfg<T>::U, NT = SliceCfg<T>::NT;
   __shared__ double smem[NT / 64];
   __shared__ double red[W][kSlice][V];
   if (sc->done) return;
-  const T beta = T(sc->beta);
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t s = remap ? xcd_logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const int64_t row = pat.sell_row[s * kSlice + lane];
   const bool real = row >= 0;
   auto xat = [&](int64_t c, T (&xc)[V]) {
 #pragma unroll
-    for (int j = 0; j < V; ++j) xc[j] = z[c * V + j] + beta * p_old[c * V + j];
+    for (int j = 0; j < V; ++j) xc[j] = p[c * V + j];
   };
   T pr[V];
   xat(real ? row : 0, pr);
@@ -96,7 +106,6 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_pcg_a(fem_pattern_t pat, co
 #pragma unroll
       for (int o = 1; o < W; ++o) t += red[o][lane][i];
       const T qi = T(t);
-      p_new[row * V + i] = pr[i];
       q[row * V + i] = qi;
       dot += xr[i] * double(qi);
     }
@@ -290,8 +299,7 @@ struct EventPool {
 struct Ws {
   void* r;
   void* z;
-  void* p0;
-  void* p1;
+  void* p;
   void* q;
   void* ones;
   double* y;
@@ -319,8 +327,7 @@ static int64_t ws_layout(const fem_pattern_t* pat, int precision, void* base, Ws
   Ws* o = w ? w : &tmp;
   o->r = take(n * es);
   o->z = take(n * es);
-  o->p0 = take(n * es);
-  o->p1 = take(n * es);
+  o->p = take(n * es);
   o->q = take(n * es);
   o->ones = take(n * es);
   o->y = reinterpret_cast<double*>(take(n * 8));
@@ -343,7 +350,7 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
   ws_layout(pat, kMixed ? FEM_F32 : FEM_F64, ws_base, &w);
   T* r = (T*)w.r;
   T* z = (T*)w.z;
-  T* pbuf[2] = {(T*)w.p0, (T*)w.p1};
+  T* p = (T*)w.p;
   T* q = (T*)w.q;
   const T* dinv = dinv_in;
   if (!opt->precond || !dinv) {
@@ -354,7 +361,7 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
   const int gA = (int)grid_a(pat);
   const int gV = (int)std::min<int64_t>(ceil_div(n, 256), kVecGrid);
   FEM_HIP(hipMemsetAsync(w.sc, 0, sizeof(PcgScalars), s));
-  hipLaunchKernelGGL(k_fill<T>, dim3(256), dim3(256), 0, s, n, pbuf[0], T(0));
+  hipLaunchKernelGGL(k_fill<T>, dim3(256), dim3(256), 0, s, n, p, T(0));
   FEM_LAUNCHED();
 
   auto read_status = [&]() -> int {
@@ -407,7 +414,7 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
   double rr_ref = rr_true;
   const int check = std::max(1, opt->check_every);
   const double delta2 = kMixed ? opt->reliable_delta * opt->reliable_delta : 0.0;
-  EventPool ev(opt->timing ? 3 * check : 0);
+  EventPool ev(opt->timing ? 4 * check : 0);
   if (!ev.ok) {
     set_error("fem_pcg_solve: hipEventCreate failed");
     return FEM_ERR_HIP;
@@ -419,17 +426,18 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
     const int iter0 = iter;
     for (int m = 0; m < check; ++m) {
       const int k = iter0 + m;
-      const int cur = k & 1;
-      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m], s));
-      hipLaunchKernelGGL((k_pcg_a<V, kDiff, T>), dim3(gA), dim3(SliceCfg<T>::NT), 0, s, *pat, vals, (const T*)z,
-                         (const T*)pbuf[cur], pbuf[cur ^ 1], q, (const PcgScalars*)w.sc, w.partA, opt->xcd_remap);
-      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m + 1], s));
+      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[4 * m], s));
+      hipLaunchKernelGGL(k_pcg_p<T>, dim3(gV), dim3(256), 0, s, n, (const T*)z, p, (const PcgScalars*)w.sc);
+      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[4 * m + 1], s));
+      hipLaunchKernelGGL((k_pcg_a<V, kDiff, T>), dim3(gA), dim3(SliceCfg<T>::NT), 0, s, *pat, vals, (const T*)p, q,
+                         (const PcgScalars*)w.sc, w.partA, opt->xcd_remap);
+      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[4 * m + 2], s));
       hipLaunchKernelGGL(k_pcg_ra, dim3(1), dim3(1024), 0, s, (const double*)w.partA, gA, w.sc);
-      hipLaunchKernelGGL((k_pcg_b<T, double>), dim3(gV), dim3(256), 0, s, n, x, r, z, (const T*)pbuf[cur ^ 1],
-                         (const T*)q, dinv, (const PcgScalars*)w.sc, w.partB);
+      hipLaunchKernelGGL((k_pcg_b<T, double>), dim3(gV), dim3(256), 0, s, n, x, r, z, (const T*)p, (const T*)q, dinv,
+                         (const PcgScalars*)w.sc, w.partB);
       hipLaunchKernelGGL(k_pcg_rb, dim3(1), dim3(1024), 0, s, (const double*)w.partB, gV, w.sc, k, tol2,
                          opt->maxiter);
-      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[3 * m + 2], s));
+      if (opt->timing) FEM_HIP(hipEventRecord(ev.e[4 * m + 3], s));
     }
     FEM_LAUNCHED();
     if (int rc = read_status()) return rc;
@@ -437,11 +445,12 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
     if (opt->timing) {
       // only the launches that ran an iteration (the rest returned on `done`)
       for (int m = 0; m < std::min(check, iter - iter0); ++m) {
-        float ta = 0.f, tb = 0.f;
-        FEM_HIP(hipEventElapsedTime(&ta, ev.e[3 * m], ev.e[3 * m + 1]));
-        FEM_HIP(hipEventElapsedTime(&tb, ev.e[3 * m + 1], ev.e[3 * m + 2]));
-        spmv_ms += ta;
-        upd_ms += tb;
+        float t0 = 0.f, t1 = 0.f, t2 = 0.f;
+        FEM_HIP(hipEventElapsedTime(&t0, ev.e[4 * m], ev.e[4 * m + 1]));
+        FEM_HIP(hipEventElapsedTime(&t1, ev.e[4 * m + 1], ev.e[4 * m + 2]));
+        FEM_HIP(hipEventElapsedTime(&t2, ev.e[4 * m + 2], ev.e[4 * m + 3]));
+        spmv_ms += t1;
+        upd_ms += t0 + t2;
         ++timed;
       }
     }
