@@ -391,6 +391,7 @@ def workload_neo_hookean(args):
             p.fes[0].update_Dirichlet_boundary_conditions(bc(1))
             _timed_solve(p, o)
         tot_t, tot_newton, tot_pcg = 0., 0, 0
+        phases = {'local_assembly': 0., 'global_matrix': 0., 'linear': 0.}
         sol = None
         for k in range(1, 6):
             p.fes[0].update_Dirichlet_boundary_conditions(bc(k))
@@ -399,11 +400,14 @@ def workload_neo_hookean(args):
             tot_t += dt
             tot_newton += st['newton_iterations']
             tot_pcg += sum(li['iterations'] for li in st['linear'])
+            for key in phases:
+                phases[key] += st['timing'][key]
             log(f"neo_hookean {name} step {k}: {st['newton_iterations']} Newton its, |R| = {st['res_norm']:.2e}, "
                 f"{dt:.3f} s")
         sols[name] = sol
         res[name] = dict(s_total=tot_t, newton_iterations=tot_newton, s_per_newton_step=tot_t / tot_newton,
-                         pcg_iterations=tot_pcg, final_residual=st['res_norm'])
+                         pcg_iterations=tot_pcg, final_residual=st['res_norm'],
+                         phases_s={k: round(v, 4) for k, v in phases.items()})
     res['mixed']['rel_err_vs_fp64'] = _rel(sols['mixed'], sols['fp64'])
     log(f"neo_hookean: {res}")
     extra = {"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'], "detail": res}
