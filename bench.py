@@ -115,11 +115,8 @@ def run(args):
         f"({dev.sell_blocks} SELL blocks)")
 
     # timed steps run without per-launch events; one extra step with events gives the kernel averages
-    opts = {
-        'fp64': {'jax_solver': {'precision': 'fp64'}},
-        'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}},
-    }
-    event_opts = {k: {'jax_solver': dict(v['jax_solver'], timing=True)} for k, v in opts.items()}
+    opts = _paths(args)
+    event_opts = {k: _evented(v) for k, v in opts.items()}
     # reference: FP64 path solved to rtol 1e-11 (about the attainable FP64 residual here)
     ref_opts = {'jax_solver': {'precision': 'fp64', 'tol': 1e-11, 'atol': 0.}}
     ref = None
@@ -153,6 +150,10 @@ def run(args):
                          replacements=lin['replacements'], spmv_ms=lin['spmv_ms'], update_ms=lin['update_ms'],
                          timing=stats[-1]['timing'])
         tm = stats[-1]['timing']
+        if getattr(dev, 'amg_timing', None):
+            res[name]['amg_timing'] = dict(dev.amg_timing)
+            log(f"{name}: AMG setup {dev.amg_timing['setup_s']*1e3:.1f} ms, solve {dev.amg_timing['solve_s']*1e3:.1f} ms,"
+                f" levels {dev._amg.describe()}")
         log(f"{name}: {res[name]['s_per_newton_step']:.4f} s/step, rel err {err:.2e}, "
             f"{lin['iterations']} its, {lin['replacements']} repl, spmv {lin['spmv_ms']*1e3:.1f} us, "
             f"update {lin['update_ms']*1e3:.1f} us; residuals {tm['local_assembly']*1e3:.1f} ms, "
@@ -199,7 +200,7 @@ def run(args):
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"HEX8 linear-elasticity cantilever 2x1x1, {args.cells[0]}x{args.cells[1]}x"
-                               f"{args.cells[2]} cells, {dev.n} dofs, Jacobi-PCG rtol 1e-10",
+                               f"{args.cells[2]} cells, {dev.n} dofs, {_solver_name(args)} rtol 1e-10",
                    "global_batch": 1, "seq_len": dev.n, "parallelism": "single GPU"},
         "fp64_s_per_newton_step": res['fp64']['s_per_newton_step'],
         "speedup_mixed_vs_fp64": res['fp64']['s_per_newton_step'] / mixed['s_per_newton_step'],
@@ -295,8 +296,25 @@ def _timed_solve(p, opts, **kw):
 
 
 def _paths(args):
+    if args.solver == 'amg':
+        return {'fp64': {'amgx_solver': {'precision': 'fp64'}}, 'mixed': {'amgx_solver': {'precision': 'mixed'}}}
     return {'fp64': {'jax_solver': {'precision': 'fp64'}},
             'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}}}
+
+
+def _solver_name(args):
+    return 'SA-AMG-CG' if args.solver == 'amg' else 'Jacobi-PCG'
+
+
+def _opt(o, **kw):
+    """Solver options `o` with keys of its linear-solver dict replaced."""
+    k = next(iter(o))
+    return {k: dict(o[k], **kw)}
+
+
+def _evented(o):
+    """Options of the extra solve that times the Jacobi-PCG kernels with HIP events."""
+    return _opt(o, timing=True) if 'jax_solver' in o else o
 
 
 def workload_poisson(args):
@@ -348,7 +366,7 @@ def workload_poisson(args):
                                  "sample": f"full workload: oracle FP64 assembly + Jacobi-PCG ({o.newton_its} its)",
                                  "rel_err_vs_gpu_fp64": float(np.linalg.norm(u - ref.cpu().numpy().reshape(-1)) /
                                                               np.linalg.norm(u))}
-    _emit(f"HEX8 Poisson 32^3 unit cube, {p.fes[0].num_total_nodes} dofs, Jacobi-PCG rtol 1e-10",
+    _emit(f"HEX8 Poisson 32^3 unit cube, {p.fes[0].num_total_nodes} dofs, {_solver_name(args)} rtol 1e-10",
           res['mixed']['s_per_newton_step'], "f32+f64", extra)
 
 
@@ -420,7 +438,8 @@ def workload_neo_hookean(args):
         extra["cpu_baseline"] = _cpu_port_sample(o, p.num_cells, p.device().nnzb * 9,
                                                  res['fp64']['pcg_iterations'] / res['fp64']['newton_iterations'],
                                                  "s per Newton step", u)
-    _emit(f"HEX8 Neo-Hookean cube {n}^3, {p.fes[0].num_total_nodes * 3} dofs, 5 load steps, Newton |R| <= 1e-8",
+    _emit(f"HEX8 Neo-Hookean cube {n}^3, {p.fes[0].num_total_nodes * 3} dofs, 5 load steps, Newton |R| <= 1e-8, "
+          f"{_solver_name(args)}",
           res['mixed']['s_per_newton_step'], "f32+f64", extra)
 
 
@@ -525,7 +544,7 @@ def workload_simp(args):
         cpu = _cpu_port_sample(o, C, p.device().nnzb * 9, res['fp64']['adjoint_pcg_iterations'],
                                "s per adjoint gradient")
     _emit(f"HEX8 SIMP cantilever {Nx}x{Ny}x{Nz} on {Lx}x{Ly}x{Lz}, {p.fes[0].num_total_nodes * 3} dofs, "
-          f"compliance gradient", res['mixed']['s_per_adjoint_gradient'], "f32+f64",
+          f"compliance gradient, {_solver_name(args)}", res['mixed']['s_per_adjoint_gradient'], "f32+f64",
           dict({"fp64_s_per_adjoint_gradient": res['fp64']['s_per_adjoint_gradient'],
                 "fd_check": {"cells": cells.tolist(), "h": h, "max_rel_diff": fd_err, "fd": fd.tolist(),
                              "adjoint": adj.tolist()}, "detail": res}, **({"cpu_baseline": cpu} if cpu else {})))
@@ -573,7 +592,7 @@ def workload_hex27(args):
     res, grads, sols = {}, {}, {}
     for name, o in _paths(args).items():
         # rtol 1e-11: at 1e-10 either solution is already ~1e-8 from the exact one on this mesh
-        o = {'jax_solver': dict(o['jax_solver'], tol=1e-11, atol=0.)}
+        o = _opt(o, tol=1e-11, atol=0.)
         fwd = ad_wrapper(p, o, o)
         fw, bw = [], []
         for it in range(args.warmup + args.steps):
@@ -592,7 +611,7 @@ def workload_hex27(args):
         grads[name] = e.grad.detach().clone()
         sols[name] = u.detach().clone()
         from jax_fem_amd.solver import solver
-        solver(p, {'jax_solver': dict(o['jax_solver'], timing=True)})     # kernel averages (HIP events)
+        solver(p, _evented(o))     # kernel averages (HIP events, Jacobi-PCG)
         lin = p._solve_stats['linear'][0]
         res[name] = dict(s_per_newton_step=statistics.median(fw), s_per_adjoint_gradient=statistics.median(bw),
                          pcg_iterations=lin['iterations'], spmv_ms=lin['spmv_ms'], update_ms=lin['update_ms'],
@@ -613,7 +632,7 @@ def workload_hex27(args):
         o = OF.Problem(ms.points, cs, 3, ele_type='HEX27', quadrature_order=4, theta=np.repeat(Es[:, None], 27, 1),
                        mat=OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., nu))), Emin=0., E0=1., p=1.))
         cpu = _cpu_port_sample(o, C, dev.nnzb * 9, res['fp64']['pcg_iterations'], "s per Newton step")
-    _emit(f"HEX27 elasticity {n}^3 unit cube, 27-pt Gauss, {dev.n} dofs, log-normal E field, Jacobi-PCG rtol 1e-11",
+    _emit(f"HEX27 elasticity {n}^3 unit cube, 27-pt Gauss, {dev.n} dofs, log-normal E field, {_solver_name(args)} rtol 1e-11",
           res['mixed']['s_per_newton_step'], "f32+f64",
           dict({"fp64_s_per_newton_step": res['fp64']['s_per_newton_step'],
                 "spmv_GBs": {nm: spmv_b[nm] / (res[nm]['spmv_ms'] * 1e-3) / 1e9 for nm in spmv_b
@@ -655,6 +674,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--cells', type=int, nargs=3, default=[128, 64, 64])
     ap.add_argument('--mixed-assembly', choices=['fp32', 'fp64', 'bf16x3'], default='fp64')
+    ap.add_argument('--solver', choices=['pcg', 'amg'], default='pcg',
+                    help="pcg: Jacobi-PCG ('jax_solver'); amg: smoothed-aggregation AMG-CG ('amgx_solver')")
     ap.add_argument('--no-cpu-baseline', dest='cpu_baseline', action='store_false')
     ap.add_argument('--only', choices=['fp64', 'mixed'], default=None,
                     help='time one path only (no adjoint, no summary line); for profiling')

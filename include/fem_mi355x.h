@@ -295,6 +295,140 @@ int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv,
                   const fem_pcg_options_t* opt, void* ws, void* host_status,
                   fem_pcg_result_t* result /* host */, void* stream);
 
+/* ---- smoothed-aggregation AMG (the amgx_solver option, solver.py:141-259,
+ *      261-284) ------------------------------------------------------------
+ * The reference hands the assembled PETSc matrix to AMGX.  Here the hierarchy
+ * is built on the GPU from the assembled matrix: MIS-2 aggregation of the
+ * block graph, tentative prolongators from the near null space (rigid-body
+ * modes for vec 3, constants for vec 1), one damped-Jacobi smoothing step of
+ * the prolongator, Galerkin products by a deterministic block SpGEMM, and a
+ * dense inverse of the coarsest operator.  The preconditioner is a V-cycle
+ * with Chebyshev smoothing inside FP64 CG.  Setup calls run in FP64; the
+ * host reads sizes between them (the library allocates nothing), and every
+ * result is bit-reproducible run to run (no floating-point atomics). */
+typedef struct {
+  int64_t        rows;           /* block rows                                */
+  int64_t        cols;           /* block columns                             */
+  int64_t        nnzb;           /* stored blocks                             */
+  int32_t        br, bc;         /* block shape (1, 3 or 6)                   */
+  const int32_t* row_ptr;        /* (rows+1)                                  */
+  const int32_t* col_idx;        /* (nnzb), ascending within a row            */
+  const void*    vals;           /* (nnzb*br*bc), row-major blocks            */
+} fem_bcsr_t;
+
+typedef struct {
+  const fem_pattern_t* pat;      /* level 0: SELL pattern of the matrix       */
+  const void*    sell_vals;      /* level 0: standard-form SELL values        */
+  fem_bcsr_t     A;              /* levels >= 1: Galerkin operator            */
+  fem_bcsr_t     P;              /* prolongator from the next level           */
+  fem_bcsr_t     R;              /* restriction P^T                           */
+  const double*  dinv;           /* (rows*bs) inverse diagonal, FP64          */
+  double         lmax;           /* largest eigenvalue estimate of D^-1 A     */
+} fem_amg_level_t;
+
+typedef struct {
+  int32_t        num_levels;     /* smoothed levels (>= 1)                    */
+  int32_t        precision;      /* value storage of A, P, R and the level-0
+                                    SELL copy: FEM_F32 or FEM_F64; vectors and
+                                    arithmetic are FP64                       */
+  int32_t        degree;         /* Chebyshev degree of pre/post smoothing    */
+  int32_t        pad;
+  const fem_amg_level_t* levels; /* host array [num_levels]                   */
+  int64_t        coarse_n;       /* unknowns of the coarsest level            */
+  const double*  coarse_inv;     /* (coarse_n x coarse_n) dense inverse       */
+} fem_amg_t;
+
+typedef struct {
+  double  rtol;                  /* stop when |r| <= max(rtol |b|, atol)      */
+  double  atol;
+  int32_t maxiter;
+  int32_t check_every;           /* iterations between host status reads      */
+} fem_amg_options_t;
+
+/* Workspace bytes of fem_amg_aggregate on an n-node graph. */
+int64_t fem_amg_aggregate_ws_bytes(int64_t n);
+/* MIS-2 aggregation of the graph (row_ptr, col_idx; self loops included):
+ * agg[i] = aggregate of node i, *n_agg (host) = number of aggregates.
+ * host_status: pinned host memory >= 128 bytes.  Synchronises the stream. */
+int fem_amg_aggregate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, int32_t* agg, int64_t* n_agg,
+                      void* ws, void* host_status, void* stream);
+/* Members of each aggregate: agg_ptr (n_agg+1), memb (n) ascending within an
+ * aggregate.  ws >= 4*n_agg rounded up to 256, + fem_scan_ws_bytes(n_agg). */
+int fem_amg_members(int64_t n, const int32_t* agg, int64_t n_agg, int32_t* agg_ptr, int32_t* memb, void* ws,
+                    void* stream);
+/* Tentative prolongator by modified Gram-Schmidt per aggregate: P0 (n x bs x
+ * ns, the one block of node i's row, column agg[i]) and the coarse near null
+ * space Bc (n_agg*ns x ns).  Level 0 (B == NULL): rigid-body modes (vec 3,
+ * ns 6, rotations about the aggregate's first node) or constants (vec 1, ns
+ * 1) from `points` (n x 3); dofs with bc_mask[d] != 0 get zero rows.  Coarser
+ * levels: the rows of B (n*bs x ns). */
+int fem_amg_tentative(int64_t n, int32_t bs, int32_t ns, int64_t n_agg, const int32_t* agg_ptr,
+                      const int32_t* memb, const double* points, const uint8_t* bc_mask, const double* B,
+                      double* P0, double* Bc, void* stream);
+/* Dirichlet mask: mask[d] = 1 for d in bc_dofs, else 0 (n dofs). */
+int fem_amg_bc_mask(int64_t n, int64_t n_bc, const int32_t* bc_dofs, uint8_t* mask, void* stream);
+/* Block CSR copy (FP64) of a standard-form FP64 SELL matrix; the pattern's
+ * row_ptr/col_idx are the CSR structure. */
+int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, double* bcsr_vals, void* stream);
+/* dinv = 1 / diag(A) (FP64 A); on levels >= 1 (fix_zero = 1) a zero
+ * diagonal entry (an empty coarse row) is first set to 1 in A. */
+int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream);
+/* Largest-eigenvalue estimate of D^-1 A by `steps` power steps from a fixed
+ * start vector.  A is the standard-form FP64 SELL matrix (pat, sell_vals) if
+ * pat != NULL, else the FP64 block CSR matrix A.  ws >=
+ * fem_amg_lmax_ws_bytes(unknowns).  Synchronises the stream. */
+int64_t fem_amg_lmax_ws_bytes(int64_t n);
+int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const double* sell_vals, const double* dinv,
+                 int32_t steps, double* lmax, void* ws, void* stream);
+/* C = X Y for block CSR X (br x k) and Y (k x bc), FP64.  The structure is
+ * computed once and kept as a plan, so a new matrix costs one numeric pass:
+ *   symbolic  row_counts[r] = blocks of row r of C; row_pairs[r] (may be
+ *             NULL) = block products of row r; *max_pairs (host) = their
+ *             maximum (an input of the plan pass)
+ *   plan      given C->row_ptr (scan of row_counts) and pair_ptr (scan of
+ *             row_pairs): writes C->col_idx (ascending), plan_ptr (C->nnzb+1)
+ *             and plan (2 int32 per product: X entry, Y entry), each output
+ *             block's products in a fixed (sorted) order
+ *   numeric   C->vals from X->vals and Y->vals along the plan (one thread per
+ *             output block, products summed in plan order: bit-reproducible)
+ * Rows of more than 16384 products, or left-operand rows of more than 2048
+ * blocks, return FEM_ERR_CAPACITY.  ws >= 256 bytes of device memory.
+ * Symbolic and plan synchronise. */
+int fem_bcsr_spgemm_symbolic(const fem_bcsr_t* X, const fem_bcsr_t* Y, int32_t* row_counts, int32_t* row_pairs,
+                             int64_t* max_pairs, void* ws, void* host_status, void* stream);
+int fem_bcsr_spgemm_plan(const fem_bcsr_t* X, const fem_bcsr_t* Y, int64_t max_pairs, fem_bcsr_t* C,
+                         const int32_t* pair_ptr, int32_t* plan_ptr, int32_t* plan, void* ws, void* host_status,
+                         void* stream);
+int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t* C, const int32_t* plan_ptr,
+                            const int32_t* plan, void* stream);
+/* Smoothed prolongator in place: W holds A P0 on the pattern of P; becomes
+ * P = P0 - omega D^-1 A P0 (P0 as from fem_amg_tentative, columns agg). */
+int fem_amg_smooth_prolongator(fem_bcsr_t* W, const double* P0, const int32_t* agg, const double* dinv,
+                               double omega, void* stream);
+/* T = X^T (block CSR; T->rows = X->cols, blocks bc x br).  Structure:
+ * T->row_ptr, T->col_idx (ascending) and src (nnzb: the X entry of each T
+ * entry); ws >= 4*X->cols rounded up to 256, + fem_scan_ws_bytes(X->cols).
+ * Values (FP64): T->vals from X->vals along src. */
+int fem_bcsr_transpose_structure(const fem_bcsr_t* X, fem_bcsr_t* T, int32_t* src, void* ws, void* stream);
+int fem_bcsr_transpose_values(const fem_bcsr_t* X, fem_bcsr_t* T, const int32_t* src, void* stream);
+/* Dense inverse of a FP64 block CSR matrix (n = rows*br <= 4096) by
+ * Gauss-Jordan without pivoting (SPD); ws >= 8*n*n bytes. */
+int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* stream);
+/* out[i] = (float)in[i]. */
+int fem_amg_to_f32(int64_t n, const double* in, float* out, void* stream);
+/* z = M r, one V-cycle; ws >= fem_amg_ws_bytes(amg). */
+int64_t fem_amg_ws_bytes(const fem_amg_t* amg);
+int fem_amg_vcycle(const fem_amg_t* amg, const double* r, double* z, void* ws, void* stream);
+/* AMG-preconditioned CG on the row-eliminated tangent (the outer operator is
+ * the FP64 standard-form SELL matrix `vals`; b, x FP64, x holds x0).  Uses
+ * ws >= fem_amg_solve_ws_bytes(pat, amg).  Synchronises.  Returns FEM_OK
+ * when the solve ran; its outcome (FEM_OK / FEM_ERR_NOT_CONVERGED /
+ * FEM_ERR_NAN) is result->status. */
+int64_t fem_amg_solve_ws_bytes(const fem_pattern_t* pat, const fem_amg_t* amg);
+int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_amg_t* amg, const double* b,
+                      double* x, const fem_amg_options_t* opt, void* ws, void* host_status,
+                      fem_pcg_result_t* result /* host */, void* stream);
+
 /* ---- vector helpers -------------------------------------------------- */
 /* out = sum_i x_i y_i (FP64), deterministic; ws >= 8 * 512 bytes (the
  * residual norms of the Newton loop, solver.py:1319-1333). */

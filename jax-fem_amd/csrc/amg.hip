@@ -1,0 +1,1786 @@
+// Smoothed-aggregation algebraic multigrid on the device.
+//
+// The reference's `amgx_solver` option (solver.py:141-259, chosen in
+// linear_solver, solver.py:261-284) hands the assembled PETSc matrix to AMGX.
+// Here the hierarchy is built on the GPU from the assembled SELL-64 matrix and
+// used as the preconditioner of FP64 CG:
+//   setup  MIS-2 aggregation by max-propagation rounds; tentative prolongator
+//          by modified Gram-Schmidt per aggregate (one wave per aggregate);
+//          P = (I - omega D^-1 A) P0; Galerkin R (A P) by a block SpGEMM that
+//          sorts each output row's block products in LDS (bitonic) and sums
+//          every output block in sorted order; transpose by counting sort;
+//          Gauss-Jordan inverse of the coarsest operator
+//   cycle  V-cycle with Chebyshev smoothing (SELL-64 kernels on level 0, block
+//          CSR kernels below), dense GEMV on the coarsest level
+//   solve  CG whose scalars stay on the device; the host reads a status block
+//          every `check_every` iterations
+// No floating-point atomics: every result is bit-reproducible run to run.
+#include "fem_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace fem {
+namespace {
+
+constexpr int kXCap = 1024;     // max blocks in one row of the left SpGEMM operand (sorted-pair kernel)
+constexpr int kDotBlocks = 1024;
+constexpr double kRankTol = 1e-10;
+
+// ------------------------------------------------------------- priorities
+__host__ __device__ inline uint32_t amg_hash(uint32_t x) {
+  x *= 0x9E3779B1u;
+  x ^= x >> 16;
+  x *= 0x85EBCA6Bu;
+  x ^= x >> 13;
+  x *= 0xC2B2AE35u;
+  x ^= x >> 16;
+  return x;
+}
+// unique 61-bit priority of node i (i < 2^31)
+__device__ inline uint64_t amg_key(int64_t i) {
+  return ((uint64_t)(amg_hash((uint32_t)i) >> 2) << 31) | (uint64_t)i;
+}
+// (rank, key): rank 2 = root, 1 = undecided, 0 = decided not root
+__device__ inline uint64_t mis_tuple(int8_t st, int64_t j) {
+  const uint64_t rank = st == 1 ? 2u : (st == 0 ? 1u : 0u);
+  return (rank << 61) | amg_key(j);
+}
+
+inline unsigned grid_for(int64_t n, int nt) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, nt), 1 << 20));
+}
+
+// --------------------------------------------------------------- MIS-2
+__global__ void k_mis_init(int64_t n, int8_t* __restrict__ state) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    state[i] = 0;
+}
+
+// t1[i] = max over the neighbourhood of i of the (rank, key) tuples.
+__global__ void k_mis_prop1(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            const int8_t* __restrict__ state, uint64_t* __restrict__ t1, int32_t* __restrict__ flag) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *flag = 0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t m = 0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int64_t j = ci[e];
+      m = max(m, mis_tuple(state[j], j));
+    }
+    t1[i] = m;
+  }
+}
+
+// t2 = max of t1 over the neighbourhood (= the distance-2 maximum): an
+// undecided node that is its own maximum becomes a root, one that sees a root
+// within distance 2 is decided; *flag = 1 while undecided nodes remain.
+__global__ void k_mis_prop2(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            const uint64_t* __restrict__ t1, int8_t* __restrict__ state, int32_t* __restrict__ flag) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (state[i] != 0) continue;
+    uint64_t m = 0;
+    for (int e = rp[i]; e < rp[i + 1]; ++e) m = max(m, t1[ci[e]]);
+    if (m == mis_tuple(0, i))
+      state[i] = 1;
+    else if ((m >> 61) == 2)
+      state[i] = 2;
+    else
+      atomicOr(flag, 1);
+  }
+}
+
+__global__ void k_root_flags(int64_t n, const int8_t* __restrict__ state, int32_t* __restrict__ f) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    f[i] = state[i] == 1 ? 1 : 0;
+}
+
+// roots and their neighbours (roots are >= 3 apart: at most one root neighbour)
+__global__ void k_agg_pass1(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            const int8_t* __restrict__ state, const int32_t* __restrict__ root_id,
+                            int32_t* __restrict__ agg1) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t a = -1;
+    if (state[i] == 1) {
+      a = root_id[i];
+    } else {
+      for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = ci[e];
+        if (state[j] == 1) {
+          a = root_id[j];
+          break;
+        }
+      }
+    }
+    agg1[i] = a;
+  }
+}
+
+// distance-2 nodes join the assigned neighbour of largest priority
+__global__ void k_agg_pass2(int64_t n, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            const int32_t* __restrict__ agg1, int32_t* __restrict__ agg, int32_t* __restrict__ err) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int32_t a = agg1[i];
+    if (a < 0) {
+      uint64_t best = 0;
+      for (int e = rp[i]; e < rp[i + 1]; ++e) {
+        const int j = ci[e];
+        if (agg1[j] >= 0) {
+          const uint64_t k = amg_key(j) + 1;
+          if (k > best) {
+            best = k;
+            a = agg1[j];
+          }
+        }
+      }
+      if (a < 0) atomicOr(err, 1);
+    }
+    agg[i] = a;
+  }
+}
+
+// -------------------------------------------------------------- members
+__global__ void k_count_keys(int64_t n, const int32_t* __restrict__ key, int32_t* __restrict__ cnt) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[key[i]], 1);
+}
+
+__global__ void k_fill_members(int64_t n, const int32_t* __restrict__ agg, const int32_t* __restrict__ ptr,
+                               int32_t* __restrict__ cursor, int32_t* __restrict__ memb) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = agg[i];
+    memb[ptr[a] + atomicAdd(&cursor[a], 1)] = (int32_t)i;
+  }
+}
+
+// insertion sort of each segment of v (segments are short); `aux` (may be
+// NULL) is permuted alongside
+__global__ void k_sort_segments(int64_t m, const int32_t* __restrict__ ptr, int32_t* __restrict__ v,
+                                int32_t* __restrict__ aux) {
+  for (int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; s < m; s += (int64_t)gridDim.x * blockDim.x) {
+    const int b = ptr[s], e = ptr[s + 1];
+    for (int i = b + 1; i < e; ++i) {
+      const int32_t key = v[i];
+      const int32_t ax = aux ? aux[i] : 0;
+      int j = i - 1;
+      while (j >= b && v[j] > key) {
+        v[j + 1] = v[j];
+        if (aux) aux[j + 1] = aux[j];
+        --j;
+      }
+      v[j + 1] = key;
+      if (aux) aux[j + 1] = ax;
+    }
+  }
+}
+
+// ------------------------------------------------------------ tentative
+// One wave per aggregate.  The aggregate's rows of the near null space are
+// written into P0 and orthonormalised there (each lane owns rows r = lane,
+// lane + 64, ...; the column dot products are wave sums).
+template <int BS, int NS>
+__global__ __launch_bounds__(256) void k_tentative(int64_t n_agg, const int32_t* __restrict__ agg_ptr,
+                                                   const int32_t* __restrict__ memb, const double* __restrict__ points,
+                                                   const uint8_t* __restrict__ bc_mask, const double* __restrict__ B,
+                                                   double* __restrict__ P0, double* __restrict__ Bc) {
+  const int lane = threadIdx.x & 63;
+  const int64_t a = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  if (a >= n_agg) return;  // wave-uniform
+  const int b = agg_ptr[a];
+  const int rows = (agg_ptr[a + 1] - b) * BS;
+  const int first = memb[b];
+  for (int r = lane; r < rows; r += 64) {
+    const int node = memb[b + r / BS], comp = r % BS;
+    double v[NS];
+    if (B) {
+#pragma unroll
+      for (int j = 0; j < NS; ++j) v[j] = B[((int64_t)node * BS + comp) * NS + j];
+    } else if (NS == 1) {
+      v[0] = 1.0;
+    } else {
+      const double x = points[(int64_t)node * 3 + 0] - points[(int64_t)first * 3 + 0];
+      const double y = points[(int64_t)node * 3 + 1] - points[(int64_t)first * 3 + 1];
+      const double z = points[(int64_t)node * 3 + 2] - points[(int64_t)first * 3 + 2];
+#pragma unroll
+      for (int j = 0; j < NS; ++j) v[j] = 0.0;
+      v[comp] = 1.0;
+      // rotations: (-y, x, 0), (0, -z, y), (z, 0, -x)
+      if (comp == 0) {
+        v[3] = -y;
+        v[5] = z;
+      } else if (comp == 1) {
+        v[3] = x;
+        v[4] = -z;
+      } else {
+        v[4] = y;
+        v[5] = -x;
+      }
+    }
+    if (!B && bc_mask && bc_mask[(int64_t)node * BS + comp])
+#pragma unroll
+      for (int j = 0; j < NS; ++j) v[j] = 0.0;
+    double* dst = P0 + ((int64_t)node * BS + comp) * NS;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) dst[j] = v[j];
+  }
+  double R[NS][NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k)
+#pragma unroll
+    for (int j = 0; j < NS; ++j) R[k][j] = 0.0;
+  auto at = [&](int r, int j) -> double& {
+    return P0[((int64_t)memb[b + r / BS] * BS + r % BS) * NS + j];
+  };
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    double s = 0.0;
+    for (int r = lane; r < rows; r += 64) s += at(r, j) * at(r, j);
+    const double n0 = sqrt(wave_sum(s));
+#pragma unroll
+    for (int k = 0; k < j; ++k) {
+      double d = 0.0;
+      for (int r = lane; r < rows; r += 64) d += at(r, k) * at(r, j);
+      d = wave_sum(d);
+      R[k][j] = d;
+      for (int r = lane; r < rows; r += 64) at(r, j) -= d * at(r, k);
+    }
+    s = 0.0;
+    for (int r = lane; r < rows; r += 64) s += at(r, j) * at(r, j);
+    const double nv = sqrt(wave_sum(s));
+    const bool keep = nv > kRankTol * n0 && nv > 0.0;
+    const double scale = keep ? 1.0 / nv : 0.0;
+    R[j][j] = keep ? nv : 0.0;
+    for (int r = lane; r < rows; r += 64) at(r, j) *= scale;
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NS; ++k)
+#pragma unroll
+      for (int j = 0; j < NS; ++j) Bc[(a * NS + k) * NS + j] = R[k][j];
+  }
+}
+
+__global__ void k_bc_mask(int64_t n, uint8_t* __restrict__ mask) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    mask[i] = 0;
+}
+__global__ void k_bc_mask_set(int64_t n_bc, const int32_t* __restrict__ dofs, uint8_t* __restrict__ mask) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_bc; i += (int64_t)gridDim.x * blockDim.x)
+    mask[dofs[i]] = 1;
+}
+
+// ------------------------------------------------------ matrix helpers
+template <int V>
+__global__ void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv, double* __restrict__ out) {
+  constexpr int VV = V * V;
+  for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < pat.num_rows;
+       row += (int64_t)gridDim.x * blockDim.x) {
+    const int rb = pat.row_ptr[row], len = pat.row_ptr[row + 1] - rb;
+    const int64_t pos = pat.sell_pos[row];
+    const double* vp = sv + (size_t)pat.slice_ptr[pos >> 6] * VV + (pos & 63);
+    for (int k = 0; k < len; ++k)
+#pragma unroll
+      for (int v = 0; v < VV; ++v) out[(size_t)(rb + k) * VV + v] = vp[((size_t)k * VV + v) * kSlice];
+  }
+}
+
+// A row without a diagonal block (not produced by the Galerkin products)
+// gets dinv = 0, i.e. no smoothing on it.
+template <int BS>
+__global__ void k_bcsr_dinv(int64_t rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                            double* __restrict__ v, int fix, double* __restrict__ dinv) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+    int k = -1;
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+      if (ci[e] == i) {
+        k = e;
+        break;
+      }
+    if (k < 0) {
+#pragma unroll
+      for (int c = 0; c < BS; ++c) dinv[i * BS + c] = 0.0;
+      continue;
+    }
+#pragma unroll
+    for (int c = 0; c < BS; ++c) {
+      double& d = v[((size_t)k * BS + c) * BS + c];
+      if (fix && d == 0.0) d = 1.0;
+      dinv[i * BS + c] = d != 0.0 ? 1.0 / d : 0.0;
+    }
+  }
+}
+
+template <int BS, int NS>
+__global__ void k_smooth_p(int64_t rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                           double* __restrict__ v, const double* __restrict__ P0, const int32_t* __restrict__ agg,
+                           const double* __restrict__ dinv, double omega) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = agg[i];
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+      const bool own = ci[k] == a;
+#pragma unroll
+      for (int r = 0; r < BS; ++r) {
+        const double s = -omega * dinv[i * BS + r];
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+          double& w = v[((size_t)k * BS + r) * NS + j];
+          double p = s * w;
+          if (own) p += P0[(i * BS + r) * NS + j];
+          w = p;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ transpose
+__global__ void k_count_cols(int64_t nnzb, const int32_t* __restrict__ ci, int32_t* __restrict__ cnt) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnzb; e += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[ci[e]], 1);
+}
+
+__global__ void k_t_fill(int64_t rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                         const int32_t* __restrict__ trp, int32_t* __restrict__ cursor, int32_t* __restrict__ tci,
+                         int32_t* __restrict__ src) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x)
+    for (int e = rp[i]; e < rp[i + 1]; ++e) {
+      const int c = ci[e];
+      const int q = trp[c] + atomicAdd(&cursor[c], 1);
+      tci[q] = (int32_t)i;
+      src[q] = e;
+    }
+}
+
+template <int BR, int BC>
+__global__ void k_t_values(int64_t nnzb, const int32_t* __restrict__ src, const double* __restrict__ xv,
+                           double* __restrict__ tv) {
+  for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnzb; q += (int64_t)gridDim.x * blockDim.x) {
+    const double* s = xv + (size_t)src[q] * BR * BC;
+    double* d = tv + (size_t)q * BR * BC;
+#pragma unroll
+    for (int i = 0; i < BR; ++i)
+#pragma unroll
+      for (int j = 0; j < BC; ++j) d[j * BR + i] = s[i * BC + j];
+  }
+}
+
+// --------------------------------------------------------------- SpGEMM
+struct BView {
+  int64_t rows;
+  const int32_t* rp;
+  const int32_t* ci;
+};
+
+__global__ void k_spgemm_pairs(int64_t rows, const int32_t* __restrict__ xrp, const int32_t* __restrict__ xci,
+                               const int32_t* __restrict__ yrp, int32_t* __restrict__ mx,
+                               int32_t* __restrict__ row_pairs) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+    int p = 0;
+    for (int e = xrp[r]; e < xrp[r + 1]; ++e) p += yrp[xci[e] + 1] - yrp[xci[e]];
+    if (row_pairs) row_pairs[r] = p;
+    atomicMax(&mx[0], p);
+    atomicMax(&mx[1], xrp[r + 1] - xrp[r]);
+  }
+}
+
+// In-place inclusive scan of a[0..m) in LDS by NT threads (chunked);
+// `part` holds NT ints.  Contains barriers: all threads must call it.
+template <int NT>
+__device__ inline void lds_inclusive_scan(int32_t* a, int m, int32_t* part) {
+  const int t = threadIdx.x;
+  const int chunk = (m + NT - 1) / NT;
+  const int b = min(m, t * chunk), e = min(m, b + chunk);
+  int32_t s = 0;
+  for (int i = b; i < e; ++i) {
+    s += a[i];
+    a[i] = s;
+  }
+  part[t] = s;
+  __syncthreads();
+  if (t == 0) {
+    int32_t acc = 0;
+    for (int i = 0; i < NT; ++i) {
+      const int32_t v = part[i];
+      part[i] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  const int32_t off = part[t];
+  for (int i = b; i < e; ++i) a[i] += off;
+  __syncthreads();
+}
+
+// largest e in [0, xl) with pref[e] <= p (pref nondecreasing, pref[xl] > p)
+__device__ inline int pair_row(const int32_t* pref, int xl, int p) {
+  int lo = 0, hi = xl;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pref[mid] <= p)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One workgroup per output row r of C = X Y.  The row's block products are
+// enumerated as pairs p (X entry e, Y entry f), keyed (column of f, p) and
+// bitonic-sorted in LDS; equal columns form one output block, summed in p
+// order.  FILL = false: count the output blocks; FILL = true: write them.
+// Plan of row r (FILL): output block u of the row is column cci[cb + u] and
+// sums the products plan[2 q], plan[2 q + 1] (X entry, Y entry) for q in
+// [plan_ptr[cb + u], plan_ptr[cb + u + 1]), in sorted order.
+struct PlanOut {
+  const int32_t* crp;       // C row pointers (FILL)
+  int32_t* cci;             // C column ids
+  const int32_t* pair_ptr;  // (rows+1) scan of the products per row
+  int32_t* plan_ptr;        // (C nnzb + 1)
+  int32_t* plan;            // (2 * total products)
+};
+
+template <int NT, int CAP, bool FILL>
+__global__ __launch_bounds__(NT) void k_spgemm_row(BView X, BView Y, int32_t* __restrict__ counts, PlanOut o,
+                                                   int32_t* __restrict__ err) {
+  __shared__ uint64_t keys[CAP];
+  __shared__ int32_t pos[CAP + 1];
+  __shared__ int32_t pref[kXCap + 1];
+  __shared__ int32_t part[NT];
+  const int t = threadIdx.x;
+  const int64_t r = blockIdx.x;
+  const int xb = X.rp[r], xl = X.rp[r + 1] - xb;
+  if (xl > kXCap) {  // uniform over the workgroup
+    if (t == 0) atomicOr(err, 1);
+    return;
+  }
+  for (int e = t; e < xl; e += NT) {
+    const int c = X.ci[xb + e];
+    pref[e + 1] = Y.rp[c + 1] - Y.rp[c];
+  }
+  if (t == 0) pref[0] = 0;
+  __syncthreads();
+  lds_inclusive_scan<NT>(pref + 1, xl, part);
+  const int np = pref[xl];
+  if (np > CAP) {
+    if (t == 0) atomicOr(err, 2);
+    return;
+  }
+  int P2 = 1;
+  while (P2 < np) P2 <<= 1;
+  for (int p = t; p < P2; p += NT) {
+    uint64_t k = ~0ull;
+    if (p < np) {
+      const int e = pair_row(pref, xl, p);
+      const int f = Y.rp[X.ci[xb + e]] + (p - pref[e]);
+      k = ((uint64_t)(uint32_t)Y.ci[f] << 32) | (uint32_t)p;
+    }
+    keys[p] = k;
+  }
+  __syncthreads();
+  for (int k = 2; k <= P2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P2; i += NT) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = keys[i], b = keys[ixj];
+          const bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int p = t; p < np; p += NT) pos[p] = (p == 0 || (keys[p] >> 32) != (keys[p - 1] >> 32)) ? 1 : 0;
+  __syncthreads();
+  lds_inclusive_scan<NT>(pos, np, part);
+  const int U = np > 0 ? pos[np - 1] : 0;
+  if (!FILL) {
+    if (t == 0) counts[r] = U;
+    return;
+  }
+  const int64_t cb = o.crp[r];
+  if (o.crp[r + 1] - cb != U) {
+    if (t == 0) atomicOr(err, 4);
+    return;
+  }
+  __shared__ int32_t seg[CAP + 1];  // output block u = sorted products [seg[u], seg[u+1])
+  for (int p = t; p < np; p += NT)
+    if (p == 0 || (keys[p] >> 32) != (keys[p - 1] >> 32)) {
+      const int u = pos[p] - 1;
+      seg[u] = p;
+      o.cci[cb + u] = (int32_t)(keys[p] >> 32);
+    }
+  __syncthreads();
+  const int64_t pb = o.pair_ptr[r];
+  for (int u = t; u < U; u += NT) o.plan_ptr[cb + u] = (int32_t)(pb + seg[u]);
+  if (t == 0) o.plan_ptr[cb + U] = o.pair_ptr[r + 1];
+  for (int q = t; q < np; q += NT) {
+    const int pp = (int)(keys[q] & 0xFFFFFFFFu);
+    const int e = pair_row(pref, xl, pp);
+    o.plan[2 * (pb + q)] = xb + e;
+    o.plan[2 * (pb + q) + 1] = Y.rp[X.ci[xb + e]] + (pp - pref[e]);
+  }
+}
+
+// Rows with many products (up to kBigCap): the output columns are first
+// collected in an LDS hash table and ranked (sorted), so a product's sort key
+// fits 32 bits (rank << 18 | p); otherwise as k_spgemm_row.
+constexpr int kBigCap = 16384;   // products per row
+constexpr int kBigHash = 4096;   // hash slots (output columns per row <= kBigHash / 2)
+constexpr int kBigXCap = 2048;   // blocks in one row of X
+
+__device__ inline int hash_slot(int col) { return (int)(((uint32_t)col * 2654435761u) >> 20); }  // 12 bits
+
+template <int NT, bool FILL>
+__global__ __launch_bounds__(NT) void k_spgemm_row_big(BView X, BView Y, int32_t* __restrict__ counts, PlanOut o,
+                                                       int32_t* __restrict__ err) {
+  static_assert(kBigHash == 4096, "hash_slot yields 12 bits");
+  __shared__ uint32_t keys[kBigCap];
+  __shared__ int32_t tab[kBigHash];
+  __shared__ int32_t rank[kBigHash];
+  __shared__ int32_t ucol[kBigHash / 2];
+  __shared__ int32_t seg[kBigHash / 2 + 1];
+  __shared__ int32_t pref[kBigXCap + 1];
+  __shared__ int32_t part[NT];
+  __shared__ int32_t flag;
+  const int t = threadIdx.x;
+  const int64_t r = blockIdx.x;
+  const int xb = X.rp[r], xl = X.rp[r + 1] - xb;
+  if (xl > kBigXCap) {
+    if (t == 0) atomicOr(err, 1);
+    return;
+  }
+  for (int e = t; e < xl; e += NT) {
+    const int c = X.ci[xb + e];
+    pref[e + 1] = Y.rp[c + 1] - Y.rp[c];
+  }
+  for (int i = t; i < kBigHash; i += NT) tab[i] = -1;
+  if (t == 0) {
+    pref[0] = 0;
+    flag = 0;
+  }
+  __syncthreads();
+  lds_inclusive_scan<NT>(pref + 1, xl, part);
+  const int np = pref[xl];
+  if (np > kBigCap) {
+    if (t == 0) atomicOr(err, 2);
+    return;
+  }
+  auto pair_col = [&](int p) -> int {
+    const int e = pair_row(pref, xl, p);
+    return Y.ci[Y.rp[X.ci[xb + e]] + (p - pref[e])];
+  };
+  // 1. distinct output columns into the hash table (bounded probing)
+  for (int p = t; p < np; p += NT) {
+    const int c = pair_col(p);
+    int h = hash_slot(c);
+    int probes = 0;
+    while (true) {
+      const int prev = atomicCAS(&tab[h], -1, c);
+      if (prev == -1 || prev == c) break;
+      h = (h + 1) & (kBigHash - 1);
+      if (++probes >= kBigHash) {
+        flag = 1;
+        break;
+      }
+    }
+  }
+  __syncthreads();
+  // 2. compact and sort the distinct columns
+  for (int i = t; i < kBigHash; i += NT) rank[i] = tab[i] >= 0 ? 1 : 0;
+  __syncthreads();
+  lds_inclusive_scan<NT>(rank, kBigHash, part);
+  const int U = rank[kBigHash - 1];
+  if (U > kBigHash / 2 || flag) {
+    if (t == 0) atomicOr(err, 8);
+    return;
+  }
+  int U2 = 1;
+  while (U2 < U) U2 <<= 1;
+  for (int i = t; i < kBigHash; i += NT)
+    if (tab[i] >= 0) ucol[rank[i] - 1] = tab[i];
+  __syncthreads();
+  for (int i = U + t; i < U2; i += NT) ucol[i] = INT32_MAX;
+  __syncthreads();
+  for (int k = 2; k <= U2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < U2; i += NT) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const int a = ucol[i], b = ucol[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            ucol[i] = b;
+            ucol[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  if (!FILL) {
+    if (t == 0) counts[r] = U;
+    return;
+  }
+  const int64_t cb = o.crp[r];
+  if (o.crp[r + 1] - cb != U) {
+    if (t == 0) atomicOr(err, 4);
+    return;
+  }
+  // 3. rank of every occupied slot
+  for (int u = t; u < U; u += NT) {
+    const int c = ucol[u];
+    int h = hash_slot(c);
+    while (tab[h] != c) h = (h + 1) & (kBigHash - 1);  // present: the probe ends
+    rank[h] = u;
+    o.cci[cb + u] = c;
+  }
+  __syncthreads();
+  // 4. sort the products by (rank, p)
+  int P2 = 1;
+  while (P2 < np) P2 <<= 1;
+  for (int p = t; p < P2; p += NT) {
+    uint32_t k = 0xFFFFFFFFu;
+    if (p < np) {
+      const int c = pair_col(p);
+      int h = hash_slot(c);
+      while (tab[h] != c) h = (h + 1) & (kBigHash - 1);
+      k = ((uint32_t)rank[h] << 18) | (uint32_t)p;
+    }
+    keys[p] = k;
+  }
+  __syncthreads();
+  for (int k = 2; k <= P2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P2; i += NT) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint32_t a = keys[i], b = keys[ixj];
+          if ((a > b) == ((i & k) == 0)) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int p = t; p < np; p += NT)
+    if (p == 0 || (keys[p] >> 18) != (keys[p - 1] >> 18)) seg[keys[p] >> 18] = p;
+  __syncthreads();
+  const int64_t pb = o.pair_ptr[r];
+  for (int u = t; u < U; u += NT) o.plan_ptr[cb + u] = (int32_t)(pb + seg[u]);
+  if (t == 0) o.plan_ptr[cb + U] = o.pair_ptr[r + 1];
+  for (int q = t; q < np; q += NT) {
+    const int pp = (int)(keys[q] & 0x3FFFFu);
+    const int e = pair_row(pref, xl, pp);
+    o.plan[2 * (pb + q)] = xb + e;
+    o.plan[2 * (pb + q) + 1] = Y.rp[X.ci[xb + e]] + (pp - pref[e]);
+  }
+}
+
+// C = X Y by its plan: one thread per output block, products in plan order.
+template <int XR, int XC, int YC>
+__global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_t* __restrict__ plan_ptr,
+                                                        const int32_t* __restrict__ plan, const double* __restrict__ xv,
+                                                        const double* __restrict__ yv, double* __restrict__ cv) {
+  for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
+    double acc[XR][YC];
+#pragma unroll
+    for (int i = 0; i < XR; ++i)
+#pragma unroll
+      for (int j = 0; j < YC; ++j) acc[i][j] = 0.0;
+    for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
+      const double* x = xv + (size_t)plan[2 * (int64_t)q] * XR * XC;
+      const double* y = yv + (size_t)plan[2 * (int64_t)q + 1] * XC * YC;
+      double xa[XR][XC], ya[XC][YC];
+#pragma unroll
+      for (int i = 0; i < XR; ++i)
+#pragma unroll
+        for (int m = 0; m < XC; ++m) xa[i][m] = x[i * XC + m];
+#pragma unroll
+      for (int m = 0; m < XC; ++m)
+#pragma unroll
+        for (int j = 0; j < YC; ++j) ya[m][j] = y[m * YC + j];
+#pragma unroll
+      for (int i = 0; i < XR; ++i)
+#pragma unroll
+        for (int j = 0; j < YC; ++j)
+#pragma unroll
+          for (int m = 0; m < XC; ++m) acc[i][j] = fma(xa[i][m], ya[m][j], acc[i][j]);
+    }
+    double* c = cv + (size_t)b * XR * YC;
+#pragma unroll
+    for (int i = 0; i < XR; ++i)
+#pragma unroll
+      for (int j = 0; j < YC; ++j) c[i * YC + j] = acc[i][j];
+  }
+}
+
+// Sort each row's (column, source) pairs by column: one workgroup per row,
+// bitonic in LDS (rows of at most kSortCap entries; longer rows, which the
+// hierarchy does not produce, are sorted by one thread).
+constexpr int kSortCap = 4096;
+__global__ __launch_bounds__(256) void k_sort_rows(int64_t rows, const int32_t* __restrict__ ptr,
+                                                   int32_t* __restrict__ col, int32_t* __restrict__ src) {
+  __shared__ uint64_t k[kSortCap];
+  const int64_t r = blockIdx.x;
+  const int t = threadIdx.x;
+  const int b = ptr[r], n = ptr[r + 1] - b;
+  if (n <= 1) return;
+  if (n > kSortCap) {
+    if (t == 0)
+      for (int i = b + 1; i < b + n; ++i) {
+        const int32_t c = col[i], sv = src[i];
+        int j = i - 1;
+        while (j >= b && col[j] > c) {
+          col[j + 1] = col[j];
+          src[j + 1] = src[j];
+          --j;
+        }
+        col[j + 1] = c;
+        src[j + 1] = sv;
+      }
+    return;
+  }
+  int P2 = 1;
+  while (P2 < n) P2 <<= 1;
+  for (int i = t; i < P2; i += 256)
+    k[i] = i < n ? (((uint64_t)(uint32_t)col[b + i] << 32) | (uint32_t)src[b + i]) : ~0ull;
+  __syncthreads();
+  for (int kk = 2; kk <= P2; kk <<= 1)
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P2; i += 256) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const uint64_t a = k[i], c = k[ixj];
+          if ((a > c) == ((i & kk) == 0)) {
+            k[i] = c;
+            k[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  for (int i = t; i < n; i += 256) {
+    col[b + i] = (int32_t)(k[i] >> 32);
+    src[b + i] = (int32_t)(k[i] & 0xFFFFFFFFu);
+  }
+}
+
+// ------------------------------------------------------------ dense
+__global__ void k_dense_zero(int64_t nn, double* __restrict__ M) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nn; i += (int64_t)gridDim.x * blockDim.x)
+    M[i] = 0.0;
+}
+
+__global__ void k_dense_scatter(int64_t rows, int br, int bc, int64_t n, const int32_t* __restrict__ rp,
+                                const int32_t* __restrict__ ci, const double* __restrict__ v, double* __restrict__ M) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x)
+    for (int k = rp[i]; k < rp[i + 1]; ++k)
+      for (int a = 0; a < br; ++a)
+        for (int b = 0; b < bc; ++b) M[(i * br + a) * n + (int64_t)ci[k] * bc + b] = v[((size_t)k * br + a) * bc + b];
+}
+
+// one Gauss-Jordan pivot step src -> dst (in-place inversion formula)
+__global__ void k_gj_step(int64_t n, int64_t k, const double* __restrict__ src, double* __restrict__ dst) {
+  const double d = 1.0 / src[k * n + k];
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / n, j = e % n;
+    double v;
+    if (i == k && j == k)
+      v = d;
+    else if (i == k)
+      v = src[k * n + j] * d;
+    else if (j == k)
+      v = -src[i * n + k] * d;
+    else
+      v = src[e] - src[i * n + k] * src[k * n + j] * d;
+    dst[e] = v;
+  }
+}
+
+__global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+__global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+// y = M x, one wave per row
+__global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __restrict__ M,
+                                                    const double* __restrict__ x, double* __restrict__ y) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s = fma(M[i * n + j], x[j], s);
+  s = wave_sum(s);
+  if (lane == 0) y[i] = s;
+}
+
+// ---------------------------------------------------------- operator apply
+enum { AP_STORE = 0, AP_ADD = 1, AP_RESID = 2, AP_CHEB = 3, AP_DSTORE = 4, AP_PQ = 5 };
+
+struct ApplyArgs {
+  const double* x;     // gathered input
+  double* y;           // output (CHEB: the new iterate)
+  const double* rhs;   // RESID, CHEB
+  const double* dinv;  // CHEB, DSTORE
+  double* d;           // CHEB: search direction (in/out)
+  double c1, c2;       // CHEB: d = c1 d + c2 D^-1 (rhs - A x), y = x + d
+  double* part;        // PQ: per-block partials of <x, A x>
+};
+
+template <int MODE>
+__device__ inline void apply_epilogue(const ApplyArgs& a, int64_t idx, double ax) {
+  if (MODE == AP_STORE || MODE == AP_PQ) {
+    a.y[idx] = ax;
+  } else if (MODE == AP_ADD) {
+    a.y[idx] += ax;
+  } else if (MODE == AP_RESID) {
+    a.y[idx] = a.rhs[idx] - ax;
+  } else if (MODE == AP_DSTORE) {
+    a.y[idx] = a.dinv[idx] * ax;
+  } else {
+    const double r = a.dinv[idx] * (a.rhs[idx] - ax);
+    const double dn = a.c1 != 0.0 ? fma(a.c1, a.d[idx], a.c2 * r) : a.c2 * r;
+    a.d[idx] = dn;
+    a.y[idx] = a.x[idx] + dn;
+  }
+}
+
+// Block CSR operator, G lanes per block row.
+template <int BR, int BC, int G, int MODE, typename T>
+__global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t* __restrict__ rp,
+                                                    const int32_t* __restrict__ ci, const T* __restrict__ v,
+                                                    ApplyArgs a) {
+  const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t row = gid / G;
+  const int sub = (int)(gid % G);
+  double acc[BR];
+#pragma unroll
+  for (int i = 0; i < BR; ++i) acc[i] = 0.0;
+  if (row < rows) {
+    const int b = rp[row];
+    const int nf = (rp[row + 1] - b) * BC;
+    for (int f = sub; f < nf; f += G) {
+      const int k = b + f / BC, j = f % BC;
+      const double xv = a.x[(int64_t)ci[k] * BC + j];
+      const T* vb = v + (size_t)k * BR * BC + j;
+#pragma unroll
+      for (int i = 0; i < BR; ++i) acc[i] = fma(double(vb[i * BC]), xv, acc[i]);
+    }
+  }
+#pragma unroll
+  for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int i = 0; i < BR; ++i) acc[i] += __shfl_xor(acc[i], off, 64);
+  if (row < rows && sub == 0) {
+#pragma unroll
+    for (int i = 0; i < BR; ++i) apply_epilogue<MODE>(a, row * BR + i, acc[i]);
+  }
+}
+
+// Standard-form SELL-64 operator (level 0), FP64 vectors, one block per slice.
+template <int V, int MODE, typename T>
+__global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pat, const T* __restrict__ vals,
+                                                                ApplyArgs a, const int32_t* __restrict__ done) {
+  constexpr int W = SliceCfg<T>::W, U = SliceCfg<T>::U;
+  __shared__ double red[W][kSlice][V];
+  if (done && *done) return;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t s = blockIdx.x;
+  const int64_t row = pat.sell_row[s * kSlice + lane];
+  const bool real = row >= 0;
+  auto xat = [&](int64_t c, double (&xc)[V]) {
+#pragma unroll
+    for (int j = 0; j < V; ++j) xc[j] = a.x[c * V + j];
+  };
+  double xr[V], acc[V];
+#pragma unroll
+  for (int j = 0; j < V; ++j) xr[j] = 0.0;
+  sell_slice_partial<V, false, W, U, T, decltype(xat), double>(pat, vals, s, w, lane, row, xat, xr, acc);
+  if (w > 0) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
+  }
+  __syncthreads();
+  if (w != 0) return;
+  double pq = 0.0;
+  if (real) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      double t = acc[i];
+#pragma unroll
+      for (int o = 1; o < W; ++o) t += red[o][lane][i];
+      apply_epilogue<MODE>(a, row * V + i, t);
+      if (MODE == AP_PQ) pq = fma(a.x[row * V + i], t, pq);
+    }
+  }
+  if (MODE == AP_PQ) {
+    pq = wave_sum(pq);
+    if (lane == 0) a.part[s] = pq;
+  }
+}
+
+// first Chebyshev step from a zero iterate: d = c2 D^-1 rhs, x = d
+__global__ void k_cheb_first(int64_t n, const double* __restrict__ rhs, const double* __restrict__ dinv, double c2,
+                             double* __restrict__ d, double* __restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = c2 * dinv[i] * rhs[i];
+    d[i] = v;
+    x[i] = v;
+  }
+}
+
+// power-iteration start vector: hash-derived values in [0.5, 1.5)
+__global__ void k_start_vec(int64_t n, double* __restrict__ x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.5 + (double)(amg_hash((uint32_t)i) >> 8) / 16777216.0;
+}
+
+// x = y / sqrt(*nrm2)
+__global__ void k_scale_by_norm(int64_t n, const double* __restrict__ y, const double* __restrict__ nrm2,
+                                double* __restrict__ x) {
+  const double inv = 1.0 / sqrt(*nrm2);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = y[i] * inv;
+}
+
+// ---------------------------------------------------------------- CG
+struct AmgScalars {
+  double rz, alpha, beta, rr, stop2, bb;
+  int32_t done, iter, nan, first, maxiter, pad;
+};
+
+__global__ __launch_bounds__(256) void k_dot_part(int64_t n, const double* __restrict__ x,
+                                                  const double* __restrict__ y, double* __restrict__ part,
+                                                  const int32_t* __restrict__ done) {
+  __shared__ double red[4];
+  if (done && *done) return;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    s = fma(x[i], y[i], s);
+  s = block_sum_256(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__device__ inline double sum_parts(const double* part, int np, double* red) {
+  double s = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+  return block_sum_256(s, red);
+}
+
+__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ part, int np, double* __restrict__ out) {
+  __shared__ double red[4];
+  const double s = sum_parts(part, np, red);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// start: sc->rr holds <r0, r0>, sc->bb holds <b, b>
+__global__ void k_cg_start(AmgScalars* sc, double rtol, double atol, int maxiter) {
+  const double stop = fmax(rtol * sqrt(sc->bb), atol);
+  sc->stop2 = stop * stop;
+  sc->iter = 0;
+  sc->first = 1;
+  sc->maxiter = maxiter;
+  sc->nan = !isfinite(sc->rr);
+  sc->done = sc->nan || sc->rr <= sc->stop2 || maxiter <= 0;
+}
+
+__global__ __launch_bounds__(256) void k_cg_beta(const double* __restrict__ part, int np, AmgScalars* sc) {
+  __shared__ double red[4];
+  if (sc->done) return;
+  const double rz = sum_parts(part, np, red);
+  if (threadIdx.x == 0) {
+    if (!isfinite(rz)) {
+      sc->nan = 1;
+      sc->done = 1;
+      return;
+    }
+    sc->beta = sc->first ? 0.0 : rz / sc->rz;
+    sc->rz = rz;
+    sc->first = 0;
+  }
+}
+
+__global__ void k_cg_p(int64_t n, const double* __restrict__ z, double* __restrict__ p, const AmgScalars* sc) {
+  if (sc->done) return;
+  const double beta = sc->beta;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = beta == 0.0 ? z[i] : fma(beta, p[i], z[i]);
+}
+
+__global__ __launch_bounds__(256) void k_cg_alpha(const double* __restrict__ part, int np, AmgScalars* sc) {
+  __shared__ double red[4];
+  if (sc->done) return;
+  const double pq = sum_parts(part, np, red);
+  if (threadIdx.x == 0) {
+    if (!isfinite(pq) || pq == 0.0) {
+      sc->nan = !isfinite(pq);
+      sc->done = 1;
+      sc->alpha = 0.0;
+      return;
+    }
+    sc->alpha = sc->rz / pq;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cg_xr(int64_t n, const double* __restrict__ p, const double* __restrict__ q,
+                                               double* __restrict__ x, double* __restrict__ r,
+                                               double* __restrict__ part, const AmgScalars* sc) {
+  __shared__ double red[4];
+  if (sc->done) return;
+  const double alpha = sc->alpha;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    x[i] = fma(alpha, p[i], x[i]);
+    const double ri = fma(-alpha, q[i], r[i]);
+    r[i] = ri;
+    s = fma(ri, ri, s);
+  }
+  s = block_sum_256(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_cg_rr(const double* __restrict__ part, int np, AmgScalars* sc) {
+  __shared__ double red[4];
+  if (sc->done) return;
+  const double rr = sum_parts(part, np, red);
+  if (threadIdx.x == 0) {
+    sc->rr = rr;
+    sc->iter += 1;
+    if (!isfinite(rr)) {
+      sc->nan = 1;
+      sc->done = 1;
+    } else if (rr <= sc->stop2 || sc->iter >= sc->maxiter) {
+      sc->done = 1;
+    }
+  }
+}
+
+// ------------------------------------------------------------ dispatch
+template <int MODE, typename T>
+int bcsr_apply(const fem_bcsr_t& M, const ApplyArgs& a, hipStream_t s) {
+  if (M.rows == 0) return FEM_OK;
+  // lanes per block row: about 3 (block, column) items per lane
+  const double per_row = M.rows ? double(M.nnzb) * M.bc / double(M.rows) : 0.0;
+  const int G = per_row > 96.0 ? 64 : (per_row > 24.0 ? 16 : 8);
+  const unsigned grid = (unsigned)ceil_div(M.rows * G, 256);
+  const T* v = static_cast<const T*>(M.vals);
+#define FEM_AMG_APPLY(BR, BC)                                                                                  \
+  if (M.br == BR && M.bc == BC) {                                                                              \
+    if (G == 64)                                                                                               \
+      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 64, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
+                         M.col_idx, v, a);                                                                     \
+    else if (G == 16)                                                                                          \
+      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 16, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
+                         M.col_idx, v, a);                                                                     \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 8, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr,  \
+                         M.col_idx, v, a);                                                                     \
+    FEM_LAUNCHED();                                                                                            \
+    return FEM_OK;                                                                                             \
+  }
+  FEM_AMG_APPLY(1, 1)
+  FEM_AMG_APPLY(3, 3)
+  FEM_AMG_APPLY(6, 6)
+  FEM_AMG_APPLY(3, 6)
+  FEM_AMG_APPLY(6, 3)
+#undef FEM_AMG_APPLY
+  set_error("block CSR operator: unsupported block shape %d x %d", M.br, M.bc);
+  return FEM_ERR_BAD_ARG;
+}
+
+template <int MODE, typename T>
+int sell_apply(const fem_pattern_t& pat, const void* vals, const ApplyArgs& a, const int32_t* done, hipStream_t s) {
+  if (pat.num_slices == 0) return FEM_OK;
+  const unsigned grid = (unsigned)pat.num_slices;
+  const T* v = static_cast<const T*>(vals);
+  if (pat.vec == 3)
+    hipLaunchKernelGGL((k_sell_apply<3, MODE, T>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
+  else if (pat.vec == 1)
+    hipLaunchKernelGGL((k_sell_apply<1, MODE, T>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
+  else
+    FEM_CHECK_ARG(false, "AMG: vec must be 1 or 3");
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int64_t level_n(const fem_amg_t& amg, int l) {
+  if (l == 0) return amg.levels[0].pat->num_rows * amg.levels[0].pat->vec;
+  return amg.levels[l].A.rows * amg.levels[l].A.br;
+}
+
+size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Chebyshev on [lo, hi] = [lmax / 30, 1.1 lmax]
+struct Cheb {
+  double theta, delta, sigma;
+  explicit Cheb(double lmax) {
+    const double hi = 1.1 * lmax, lo = lmax / 30.0;
+    theta = 0.5 * (hi + lo);
+    delta = 0.5 * (hi - lo);
+    sigma = theta / delta;
+  }
+};
+
+template <typename T>
+int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgs& a, hipStream_t s) {
+  const fem_amg_level_t& L = amg.levels[l];
+  if (l == 0) {
+    if (mode == AP_CHEB) return sell_apply<AP_CHEB, T>(*L.pat, L.sell_vals, a, nullptr, s);
+    return sell_apply<AP_RESID, T>(*L.pat, L.sell_vals, a, nullptr, s);
+  }
+  if (mode == AP_CHEB) return bcsr_apply<AP_CHEB, T>(L.A, a, s);
+  return bcsr_apply<AP_RESID, T>(L.A, a, s);
+}
+
+template <typename T>
+int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipStream_t s) {
+  const int NL = amg.num_levels;
+  std::vector<double*> rhs(NL + 1), xa(NL), xb(NL), dd(NL), res(NL), cur(NL + 1);
+  size_t off = 0;
+  auto take = [&](int64_t n) {
+    double* p = reinterpret_cast<double*>(ws + off);
+    off += align256((size_t)std::max<int64_t>(n, 1) * 8);
+    return p;
+  };
+  for (int l = 0; l < NL; ++l) {
+    const int64_t n = level_n(amg, l);
+    rhs[l] = l == 0 ? const_cast<double*>(r) : take(n);
+    xa[l] = take(n);
+    xb[l] = take(n);
+    dd[l] = take(n);
+    res[l] = take(n);
+  }
+  rhs[NL] = take(amg.coarse_n);
+  cur[NL] = take(amg.coarse_n);
+  const int deg = amg.degree;
+  // down
+  for (int l = 0; l < NL; ++l) {
+    const fem_amg_level_t& L = amg.levels[l];
+    const int64_t n = level_n(amg, l);
+    const Cheb cb(L.lmax);
+    double* x = xa[l];
+    double* other = xb[l];
+    hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(n, 256)), dim3(256), 0, s, n, rhs[l], L.dinv, 1.0 / cb.theta,
+                       dd[l], x);
+    FEM_LAUNCHED();
+    double rho = 1.0 / cb.sigma;
+    for (int k = 1; k < deg; ++k) {
+      const double rn = 1.0 / (2.0 * cb.sigma - rho);
+      ApplyArgs a{x, other, rhs[l], L.dinv, dd[l], rn * rho, 2.0 * rn / cb.delta, nullptr};
+      if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
+      rho = rn;
+      std::swap(x, other);
+    }
+    ApplyArgs ar{x, res[l], rhs[l], nullptr, nullptr, 0.0, 0.0, nullptr};
+    if (int rc = level_apply<T>(amg, l, AP_RESID, ar, s)) return rc;
+    ApplyArgs arr{res[l], rhs[l + 1], nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
+    if (int rc = bcsr_apply<AP_STORE, T>(L.R, arr, s)) return rc;
+    cur[l] = x;
+    xb[l] = other;
+  }
+  // coarsest
+  hipLaunchKernelGGL(k_dense_gemv, dim3((unsigned)ceil_div(amg.coarse_n * 64, 256)), dim3(256), 0, s, amg.coarse_n,
+                     amg.coarse_inv, rhs[NL], cur[NL]);
+  FEM_LAUNCHED();
+  // up
+  for (int l = NL - 1; l >= 0; --l) {
+    const fem_amg_level_t& L = amg.levels[l];
+    const Cheb cb(L.lmax);
+    double* x = cur[l];
+    double* other = xb[l];
+    ApplyArgs ap{cur[l + 1], x, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
+    if (int rc = bcsr_apply<AP_ADD, T>(L.P, ap, s)) return rc;
+    double rho = 1.0 / cb.sigma;
+    for (int k = 0; k < deg; ++k) {
+      double c1 = 0.0, c2 = 1.0 / cb.theta;
+      if (k > 0) {
+        const double rn = 1.0 / (2.0 * cb.sigma - rho);
+        c1 = rn * rho;
+        c2 = 2.0 * rn / cb.delta;
+        rho = rn;
+      }
+      double* out = (l == 0 && k == deg - 1) ? z : other;
+      ApplyArgs a{x, out, rhs[l], L.dinv, dd[l], c1, c2, nullptr};
+      if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
+      other = x;
+      x = out;
+    }
+    cur[l] = x;
+  }
+  return FEM_OK;
+}
+
+int check_amg(const fem_amg_t* amg) {
+  FEM_CHECK_ARG(amg && amg->levels && amg->num_levels >= 1 && amg->coarse_inv && amg->coarse_n > 0,
+                "AMG hierarchy descriptor incomplete");
+  FEM_CHECK_ARG(amg->precision == FEM_F32 || amg->precision == FEM_F64, "AMG precision must be FEM_F32 or FEM_F64");
+  FEM_CHECK_ARG(amg->degree >= 1 && amg->degree <= 8, "AMG Chebyshev degree must be in [1, 8]");
+  const fem_amg_level_t& L0 = amg->levels[0];
+  FEM_CHECK_ARG(L0.pat && L0.sell_vals && L0.dinv, "AMG level 0 needs the SELL pattern, values and dinv");
+  for (int l = 0; l < amg->num_levels; ++l) {
+    const fem_amg_level_t& L = amg->levels[l];
+    FEM_CHECK_ARG(L.dinv && L.P.vals && L.R.vals && L.lmax > 0.0, "AMG level %d incomplete", l);
+    if (l > 0) FEM_CHECK_ARG(L.A.vals && L.A.br == L.A.bc, "AMG level %d operator missing", l);
+    const int64_t next = l + 1 < amg->num_levels ? amg->levels[l + 1].A.rows * amg->levels[l + 1].A.br
+                                                 : amg->coarse_n;
+    FEM_CHECK_ARG(L.P.rows * L.P.br == level_n(*amg, l) && L.P.cols * L.P.bc == next &&
+                      L.R.rows * L.R.br == next && L.R.cols * L.R.bc == level_n(*amg, l),
+                  "AMG level %d: prolongator shapes do not match the levels", l);
+  }
+  return FEM_OK;
+}
+
+int64_t vcycle_ws(const fem_amg_t& amg) {
+  size_t b = 0;
+  for (int l = 0; l < amg.num_levels; ++l) b += 5 * align256((size_t)std::max<int64_t>(level_n(amg, l), 1) * 8);
+  b += 2 * align256((size_t)std::max<int64_t>(amg.coarse_n, 1) * 8);
+  return (int64_t)b;
+}
+
+BView view(const fem_bcsr_t& M) { return BView{M.rows, M.row_ptr, M.col_idx}; }
+
+template <bool FILL>
+int spgemm_dispatch(int64_t max_pairs, const fem_bcsr_t& X, const fem_bcsr_t& Y, int32_t* counts, const PlanOut& o,
+                    int32_t* err, hipStream_t s) {
+  const BView xv = view(X), yv = view(Y);
+  const dim3 grid((unsigned)X.rows);
+  if (max_pairs <= 256)
+    hipLaunchKernelGGL((k_spgemm_row<64, 256, FILL>), grid, dim3(64), 0, s, xv, yv, counts, o, err);
+  else if (max_pairs <= 1024)
+    hipLaunchKernelGGL((k_spgemm_row<256, 1024, FILL>), grid, dim3(256), 0, s, xv, yv, counts, o, err);
+  else if (max_pairs <= kBigCap)
+    hipLaunchKernelGGL((k_spgemm_row_big<256, FILL>), grid, dim3(256), 0, s, xv, yv, counts, o, err);
+  else {
+    set_error("block SpGEMM: a row has %lld block products (capacity %d)", (long long)max_pairs, kBigCap);
+    return FEM_ERR_CAPACITY;
+  }
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+// read `bytes` (<= 128) from the device through the pinned status buffer
+int read_dev(const void* dev, size_t bytes, void* host_status, hipStream_t s, void* out) {
+  FEM_HIP(hipMemcpyAsync(host_status, dev, bytes, hipMemcpyDeviceToHost, s));
+  FEM_HIP(hipStreamSynchronize(s));
+  memcpy(out, host_status, bytes);
+  return FEM_OK;
+}
+int read_int(const int32_t* dev, void* host_status, hipStream_t s, int32_t* out) {
+  return read_dev(dev, sizeof(int32_t), host_status, s, out);
+}
+
+}  // namespace
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" {
+
+int64_t fem_amg_aggregate_ws_bytes(int64_t n) {
+  const int64_t m = std::max<int64_t>(n, 1);
+  return (int64_t)(align256(m) + align256(m * 8) + 2 * align256(m * 4) + align256((m + 1) * 4) +
+                   align256((size_t)fem_scan_ws_bytes(m)) + 256);
+}
+
+int fem_amg_aggregate(int64_t n, const int32_t* row_ptr, const int32_t* col_idx, int32_t* agg, int64_t* n_agg,
+                      void* ws, void* host_status, void* stream) {
+  FEM_CHECK_ARG(n >= 0 && agg && n_agg && ws && host_status && (n == 0 || (row_ptr && col_idx)),
+                "fem_amg_aggregate: bad arguments");
+  FEM_CHECK_ARG(n < (int64_t(1) << 31), "fem_amg_aggregate: more than 2^31 nodes");
+  hipStream_t s = as_stream(stream);
+  *n_agg = 0;
+  if (n == 0) return FEM_OK;
+  char* w = static_cast<char*>(ws);
+  int8_t* state = reinterpret_cast<int8_t*>(w);
+  w += align256(n);
+  uint64_t* t1 = reinterpret_cast<uint64_t*>(w);
+  w += align256(n * 8);
+  int32_t* flags = reinterpret_cast<int32_t*>(w);
+  w += align256(n * 4);
+  int32_t* root_id = reinterpret_cast<int32_t*>(w);
+  w += align256((n + 1) * 4);
+  int32_t* flag = reinterpret_cast<int32_t*>(w);
+  w += 256;
+  int32_t* tmp = reinterpret_cast<int32_t*>(w);
+  w += align256(n * 4);
+  void* scan_ws = w;
+  const unsigned g = grid_for(n, 256);
+  hipLaunchKernelGGL(k_mis_init, dim3(g), dim3(256), 0, s, n, state);
+  FEM_LAUNCHED();
+  for (int round = 0;; ++round) {
+    FEM_CHECK_ARG(round < 10000, "fem_amg_aggregate: MIS-2 rounds did not terminate");
+    hipLaunchKernelGGL(k_mis_prop1, dim3(g), dim3(256), 0, s, n, row_ptr, col_idx, state, t1, flag);
+    FEM_LAUNCHED();
+    hipLaunchKernelGGL(k_mis_prop2, dim3(g), dim3(256), 0, s, n, row_ptr, col_idx, t1, state, flag);
+    FEM_LAUNCHED();
+    int32_t more = 0;
+    if (int rc = read_int(flag, host_status, s, &more)) return rc;
+    if (!more) break;
+  }
+  hipLaunchKernelGGL(k_root_flags, dim3(g), dim3(256), 0, s, n, state, flags);
+  FEM_LAUNCHED();
+  if (int rc = fem_exclusive_scan(n, flags, root_id, scan_ws, stream)) return rc;
+  hipLaunchKernelGGL(k_agg_pass1, dim3(g), dim3(256), 0, s, n, row_ptr, col_idx, state, root_id, tmp);
+  FEM_LAUNCHED();
+  FEM_HIP(hipMemsetAsync(flag, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_agg_pass2, dim3(g), dim3(256), 0, s, n, row_ptr, col_idx, tmp, agg, flag);
+  FEM_LAUNCHED();
+  int32_t bad = 0, total = 0;
+  if (int rc = read_int(flag, host_status, s, &bad)) return rc;
+  if (int rc = read_int(root_id + n, host_status, s, &total)) return rc;
+  if (bad) {
+    set_error("fem_amg_aggregate: a node was left without an aggregate");
+    return FEM_ERR_BAD_ARG;
+  }
+  *n_agg = total;
+  return FEM_OK;
+}
+
+int fem_amg_members(int64_t n, const int32_t* agg, int64_t n_agg, int32_t* agg_ptr, int32_t* memb, void* ws,
+                    void* stream) {
+  FEM_CHECK_ARG(n >= 0 && n_agg >= 0 && agg_ptr && ws && (n == 0 || (agg && memb)), "fem_amg_members: bad arguments");
+  hipStream_t s = as_stream(stream);
+  char* w = static_cast<char*>(ws);
+  int32_t* cnt = reinterpret_cast<int32_t*>(w);
+  w += align256(std::max<int64_t>(n_agg, 1) * 4);
+  void* scan_ws = w;
+  FEM_HIP(hipMemsetAsync(cnt, 0, std::max<int64_t>(n_agg, 1) * 4, s));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_count_keys, dim3(grid_for(n, 256)), dim3(256), 0, s, n, agg, cnt);
+    FEM_LAUNCHED();
+  }
+  if (int rc = fem_exclusive_scan(n_agg, cnt, agg_ptr, scan_ws, stream)) return rc;
+  if (n == 0) return FEM_OK;
+  FEM_HIP(hipMemsetAsync(cnt, 0, std::max<int64_t>(n_agg, 1) * 4, s));
+  hipLaunchKernelGGL(k_fill_members, dim3(grid_for(n, 256)), dim3(256), 0, s, n, agg, agg_ptr, cnt, memb);
+  FEM_LAUNCHED();
+  hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(n_agg, 256)), dim3(256), 0, s, n_agg, agg_ptr, memb,
+                     (int32_t*)nullptr);
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_amg_tentative(int64_t n, int32_t bs, int32_t ns, int64_t n_agg, const int32_t* agg_ptr, const int32_t* memb,
+                      const double* points, const uint8_t* bc_mask, const double* B, double* P0, double* Bc,
+                      void* stream) {
+  FEM_CHECK_ARG(n >= 0 && n_agg >= 0 && agg_ptr && P0 && Bc && (n == 0 || memb), "fem_amg_tentative: bad arguments");
+  FEM_CHECK_ARG(B || points || ns == 1, "fem_amg_tentative: level 0 with 6 modes needs the node coordinates");
+  hipStream_t s = as_stream(stream);
+  if (n_agg == 0) return FEM_OK;
+  const unsigned grid = (unsigned)ceil_div(n_agg * 64, 256);
+#define FEM_AMG_TENT(BS, NS)                                                                                  \
+  if (bs == BS && ns == NS) {                                                                                 \
+    hipLaunchKernelGGL((k_tentative<BS, NS>), dim3(grid), dim3(256), 0, s, n_agg, agg_ptr, memb, points, bc_mask, \
+                       B, P0, Bc);                                                                            \
+    FEM_LAUNCHED();                                                                                           \
+    return FEM_OK;                                                                                            \
+  }
+  if (!B) {
+    FEM_CHECK_ARG((bs == 3 && ns == 6) || (bs == 1 && ns == 1), "fem_amg_tentative: level 0 needs vec 3 (6 modes) "
+                  "or vec 1 (1 mode)");
+  }
+  FEM_AMG_TENT(3, 6)
+  FEM_AMG_TENT(6, 6)
+  FEM_AMG_TENT(1, 1)
+#undef FEM_AMG_TENT
+  set_error("fem_amg_tentative: unsupported block size %d with %d modes", bs, ns);
+  return FEM_ERR_BAD_ARG;
+}
+
+int fem_amg_bc_mask(int64_t n, int64_t n_bc, const int32_t* bc_dofs, uint8_t* mask, void* stream) {
+  FEM_CHECK_ARG(n >= 0 && n_bc >= 0 && mask && (n_bc == 0 || bc_dofs), "fem_amg_bc_mask: bad arguments");
+  hipStream_t s = as_stream(stream);
+  if (n == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_bc_mask, dim3(grid_for(n, 256)), dim3(256), 0, s, n, mask);
+  FEM_LAUNCHED();
+  if (n_bc > 0) {
+    hipLaunchKernelGGL(k_bc_mask_set, dim3(grid_for(n_bc, 256)), dim3(256), 0, s, n_bc, bc_dofs, mask);
+    FEM_LAUNCHED();
+  }
+  return FEM_OK;
+}
+
+int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, double* bcsr_vals, void* stream) {
+  FEM_CHECK_ARG(pat && sell_vals && bcsr_vals, "fem_amg_sell_to_bcsr: NULL argument");
+  hipStream_t s = as_stream(stream);
+  if (pat->num_rows == 0) return FEM_OK;
+  const unsigned g = grid_for(pat->num_rows, 256);
+  if (pat->vec == 3)
+    hipLaunchKernelGGL(k_sell_to_bcsr<3>, dim3(g), dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
+  else if (pat->vec == 1)
+    hipLaunchKernelGGL(k_sell_to_bcsr<1>, dim3(g), dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
+  else
+    FEM_CHECK_ARG(false, "fem_amg_sell_to_bcsr: vec must be 1 or 3");
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream) {
+  FEM_CHECK_ARG(A && dinv && A->br == A->bc && (A->rows == 0 || (A->row_ptr && A->col_idx && A->vals)),
+                "fem_amg_dinv: bad arguments");
+  hipStream_t s = as_stream(stream);
+  if (A->rows == 0) return FEM_OK;
+  const unsigned g = grid_for(A->rows, 256);
+  double* v = const_cast<double*>(static_cast<const double*>(A->vals));
+  switch (A->br) {
+    case 1:
+      hipLaunchKernelGGL(k_bcsr_dinv<1>, dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v, fix_zero,
+                         dinv);
+      break;
+    case 3:
+      hipLaunchKernelGGL(k_bcsr_dinv<3>, dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v, fix_zero,
+                         dinv);
+      break;
+    case 6:
+      hipLaunchKernelGGL(k_bcsr_dinv<6>, dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v, fix_zero,
+                         dinv);
+      break;
+    default:
+      FEM_CHECK_ARG(false, "fem_amg_dinv: unsupported block size %d", A->br);
+  }
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int64_t fem_amg_lmax_ws_bytes(int64_t n) {
+  n = std::max<int64_t>(n, 1);
+  return (int64_t)(2 * align256(n * 8) + align256(kDotBlocks * 8) + 256);
+}
+
+int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const double* sell_vals, const double* dinv,
+                 int32_t steps, double* lmax, void* ws, void* stream) {
+  FEM_CHECK_ARG(dinv && lmax && ws && steps >= 1, "fem_amg_lmax: bad arguments");
+  FEM_CHECK_ARG(pat ? (sell_vals != nullptr && pat->num_rows > 0) : (A && A->br == A->bc && A->rows > 0),
+                "fem_amg_lmax: give a SELL matrix (pat, sell_vals) or a square block CSR matrix");
+  hipStream_t s = as_stream(stream);
+  const int64_t n = pat ? pat->num_rows * pat->vec : A->rows * A->br;
+  char* w = static_cast<char*>(ws);
+  double* x = reinterpret_cast<double*>(w);
+  w += align256(n * 8);
+  double* y = reinterpret_cast<double*>(w);
+  w += align256(n * 8);
+  double* part = reinterpret_cast<double*>(w);
+  w += align256(kDotBlocks * 8);
+  double* nrm = reinterpret_cast<double*>(w);
+  const unsigned g = grid_for(n, 256);
+  hipLaunchKernelGGL(k_start_vec, dim3(g), dim3(256), 0, s, n, x);
+  FEM_LAUNCHED();
+  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, x, x, part, (const int32_t*)nullptr);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, nrm);
+  hipLaunchKernelGGL(k_scale_by_norm, dim3(g), dim3(256), 0, s, n, x, nrm, x);
+  FEM_LAUNCHED();
+  for (int k = 0; k < steps; ++k) {
+    ApplyArgs a{x, y, nullptr, dinv, nullptr, 0.0, 0.0, nullptr};
+    if (int rc = pat ? sell_apply<AP_DSTORE, double>(*pat, sell_vals, a, nullptr, s)
+                     : bcsr_apply<AP_DSTORE, double>(*A, a, s))
+      return rc;
+    hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, y, y, part, (const int32_t*)nullptr);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, nrm);
+    hipLaunchKernelGGL(k_scale_by_norm, dim3(g), dim3(256), 0, s, n, y, nrm, x);
+    FEM_LAUNCHED();
+  }
+  double v = 0.0;  // |y|^2 of the last step
+  FEM_HIP(hipMemcpyAsync(&v, nrm, sizeof(double), hipMemcpyDeviceToHost, s));
+  FEM_HIP(hipStreamSynchronize(s));
+  *lmax = sqrt(v);
+  return FEM_OK;
+}
+
+int fem_bcsr_spgemm_symbolic(const fem_bcsr_t* X, const fem_bcsr_t* Y, int32_t* row_counts, int32_t* row_pairs,
+                             int64_t* max_pairs, void* ws, void* host_status, void* stream) {
+  FEM_CHECK_ARG(X && Y && row_counts && max_pairs && ws && host_status, "fem_bcsr_spgemm_symbolic: NULL argument");
+  FEM_CHECK_ARG(X->bc == Y->br && X->cols == Y->rows, "fem_bcsr_spgemm_symbolic: inner dimensions differ");
+  hipStream_t s = as_stream(stream);
+  *max_pairs = 0;
+  if (X->rows == 0) return FEM_OK;
+  int32_t* mx = static_cast<int32_t*>(ws);  // [0] max products, [1] max X row, [2] error flags
+  FEM_HIP(hipMemsetAsync(mx, 0, 3 * sizeof(int32_t), s));
+  hipLaunchKernelGGL(k_spgemm_pairs, dim3(grid_for(X->rows, 256)), dim3(256), 0, s, X->rows, X->row_ptr,
+                     X->col_idx, Y->row_ptr, mx, row_pairs);
+  FEM_LAUNCHED();
+  int32_t h[3];
+  if (int rc = read_dev(mx, sizeof(h), host_status, s, h)) return rc;
+  const int xcap = h[0] <= 1024 ? kXCap : kBigXCap;
+  if (h[1] > xcap) {
+    set_error("block SpGEMM: a row of the left operand has %d blocks (capacity %d)", h[1], xcap);
+    return FEM_ERR_CAPACITY;
+  }
+  if (int rc = spgemm_dispatch<false>(h[0], *X, *Y, row_counts, PlanOut{}, mx + 2, s)) return rc;
+  if (int rc = read_dev(mx, sizeof(h), host_status, s, h)) return rc;
+  if (h[2]) {
+    set_error("block SpGEMM: capacity exceeded in the symbolic pass (flags %d)", h[2]);
+    return FEM_ERR_CAPACITY;
+  }
+  *max_pairs = h[0];
+  return FEM_OK;
+}
+
+int fem_bcsr_spgemm_plan(const fem_bcsr_t* X, const fem_bcsr_t* Y, int64_t max_pairs, fem_bcsr_t* C,
+                         const int32_t* pair_ptr, int32_t* plan_ptr, int32_t* plan, void* ws, void* host_status,
+                         void* stream) {
+  FEM_CHECK_ARG(X && Y && C && pair_ptr && plan_ptr && ws && host_status && C->row_ptr,
+                "fem_bcsr_spgemm_plan: NULL argument");
+  FEM_CHECK_ARG(X->bc == Y->br && X->cols == Y->rows && C->rows == X->rows && C->cols == Y->cols &&
+                    C->br == X->br && C->bc == Y->bc,
+                "fem_bcsr_spgemm_plan: shapes do not match");
+  hipStream_t s = as_stream(stream);
+  if (X->rows == 0) return FEM_OK;
+  FEM_CHECK_ARG(C->nnzb == 0 || (C->col_idx && plan), "fem_bcsr_spgemm_plan: output arrays missing");
+  int32_t* err = static_cast<int32_t*>(ws);
+  FEM_HIP(hipMemsetAsync(err, 0, sizeof(int32_t), s));
+  const PlanOut o{C->row_ptr, const_cast<int32_t*>(C->col_idx), pair_ptr, plan_ptr, plan};
+  if (int rc = spgemm_dispatch<true>(max_pairs, *X, *Y, nullptr, o, err, s)) return rc;
+  int32_t e = 0;
+  if (int rc = read_int(err, host_status, s, &e)) return rc;
+  if (e) {
+    set_error("block SpGEMM: plan pass disagrees with the symbolic pass (flags %d)", e);
+    return FEM_ERR_CAPACITY;
+  }
+  return FEM_OK;
+}
+
+int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t* C, const int32_t* plan_ptr,
+                            const int32_t* plan, void* stream) {
+  FEM_CHECK_ARG(X && Y && C && plan_ptr, "fem_bcsr_spgemm_numeric: NULL argument");
+  FEM_CHECK_ARG(X->bc == Y->br && C->br == X->br && C->bc == Y->bc, "fem_bcsr_spgemm_numeric: shapes do not match");
+  hipStream_t s = as_stream(stream);
+  if (C->nnzb == 0) return FEM_OK;
+  FEM_CHECK_ARG(plan && X->vals && Y->vals && C->vals, "fem_bcsr_spgemm_numeric: values missing");
+  const double* xv = static_cast<const double*>(X->vals);
+  const double* yv = static_cast<const double*>(Y->vals);
+  double* cv = static_cast<double*>(const_cast<void*>(C->vals));
+  const unsigned g = grid_for(C->nnzb, 256);
+#define FEM_AMG_NUM(XR, XC, YC)                                                                               \
+  if (X->br == XR && X->bc == XC && Y->bc == YC) {                                                            \
+    hipLaunchKernelGGL((k_spgemm_numeric<XR, XC, YC>), dim3(g), dim3(256), 0, s, C->nnzb, plan_ptr, plan, xv, yv, \
+                       cv);                                                                                   \
+    FEM_LAUNCHED();                                                                                           \
+    return FEM_OK;                                                                                            \
+  }
+  FEM_AMG_NUM(1, 1, 1)
+  FEM_AMG_NUM(3, 3, 6)
+  FEM_AMG_NUM(6, 3, 6)
+  FEM_AMG_NUM(6, 6, 6)
+#undef FEM_AMG_NUM
+  set_error("block SpGEMM: unsupported block shapes %d x %d times %d x %d", X->br, X->bc, Y->br, Y->bc);
+  return FEM_ERR_BAD_ARG;
+}
+
+int fem_amg_smooth_prolongator(fem_bcsr_t* W, const double* P0, const int32_t* agg, const double* dinv,
+                               double omega, void* stream) {
+  FEM_CHECK_ARG(W && P0 && agg && dinv, "fem_amg_smooth_prolongator: NULL argument");
+  hipStream_t s = as_stream(stream);
+  if (W->rows == 0) return FEM_OK;
+  const unsigned g = grid_for(W->rows, 256);
+  double* v = static_cast<double*>(const_cast<void*>(W->vals));
+#define FEM_AMG_SMOOTH(BS, NS)                                                                               \
+  if (W->br == BS && W->bc == NS) {                                                                          \
+    hipLaunchKernelGGL((k_smooth_p<BS, NS>), dim3(g), dim3(256), 0, s, W->rows, W->row_ptr, W->col_idx, v, P0, \
+                       agg, dinv, omega);                                                                    \
+    FEM_LAUNCHED();                                                                                          \
+    return FEM_OK;                                                                                           \
+  }
+  FEM_AMG_SMOOTH(3, 6)
+  FEM_AMG_SMOOTH(6, 6)
+  FEM_AMG_SMOOTH(1, 1)
+#undef FEM_AMG_SMOOTH
+  set_error("fem_amg_smooth_prolongator: unsupported block shape %d x %d", W->br, W->bc);
+  return FEM_ERR_BAD_ARG;
+}
+
+int fem_bcsr_transpose_structure(const fem_bcsr_t* X, fem_bcsr_t* T, int32_t* src, void* ws, void* stream) {
+  FEM_CHECK_ARG(X && T && ws && T->row_ptr, "fem_bcsr_transpose_structure: NULL argument");
+  FEM_CHECK_ARG(T->rows == X->cols && T->cols == X->rows && T->nnzb == X->nnzb && T->br == X->bc && T->bc == X->br,
+                "fem_bcsr_transpose_structure: output descriptor does not match X^T");
+  hipStream_t s = as_stream(stream);
+  char* w = static_cast<char*>(ws);
+  int32_t* cnt = reinterpret_cast<int32_t*>(w);
+  w += align256(std::max<int64_t>(X->cols, 1) * 4);
+  void* scan_ws = w;
+  int32_t* trp = const_cast<int32_t*>(T->row_ptr);
+  int32_t* tci = const_cast<int32_t*>(T->col_idx);
+  FEM_HIP(hipMemsetAsync(cnt, 0, std::max<int64_t>(X->cols, 1) * 4, s));
+  if (X->nnzb > 0) {
+    hipLaunchKernelGGL(k_count_cols, dim3(grid_for(X->nnzb, 256)), dim3(256), 0, s, X->nnzb, X->col_idx, cnt);
+    FEM_LAUNCHED();
+  }
+  if (int rc = fem_exclusive_scan(X->cols, cnt, trp, scan_ws, stream)) return rc;
+  if (X->nnzb == 0) return FEM_OK;
+  FEM_CHECK_ARG(tci && src, "fem_bcsr_transpose_structure: output arrays missing");
+  FEM_HIP(hipMemsetAsync(cnt, 0, std::max<int64_t>(X->cols, 1) * 4, s));
+  hipLaunchKernelGGL(k_t_fill, dim3(grid_for(X->rows, 256)), dim3(256), 0, s, X->rows, X->row_ptr, X->col_idx, trp,
+                     cnt, tci, src);
+  FEM_LAUNCHED();
+  if (T->rows > 0) {
+    hipLaunchKernelGGL(k_sort_rows, dim3((unsigned)T->rows), dim3(256), 0, s, T->rows, trp, tci, src);
+    FEM_LAUNCHED();
+  }
+  return FEM_OK;
+}
+
+int fem_bcsr_transpose_values(const fem_bcsr_t* X, fem_bcsr_t* T, const int32_t* src, void* stream) {
+  FEM_CHECK_ARG(X && T && (X->nnzb == 0 || (src && X->vals && T->vals)), "fem_bcsr_transpose_values: NULL argument");
+  FEM_CHECK_ARG(T->nnzb == X->nnzb && T->br == X->bc && T->bc == X->br, "fem_bcsr_transpose_values: shapes differ");
+  hipStream_t s = as_stream(stream);
+  if (X->nnzb == 0) return FEM_OK;
+  const double* xv = static_cast<const double*>(X->vals);
+  double* tv = static_cast<double*>(const_cast<void*>(T->vals));
+  const unsigned g = grid_for(X->nnzb, 256);
+#define FEM_AMG_T(BR, BC)                                                                                   \
+  if (X->br == BR && X->bc == BC) {                                                                         \
+    hipLaunchKernelGGL((k_t_values<BR, BC>), dim3(g), dim3(256), 0, s, X->nnzb, src, xv, tv);               \
+    FEM_LAUNCHED();                                                                                         \
+    return FEM_OK;                                                                                          \
+  }
+  FEM_AMG_T(3, 6)
+  FEM_AMG_T(6, 6)
+  FEM_AMG_T(1, 1)
+  FEM_AMG_T(6, 3)
+  FEM_AMG_T(3, 3)
+#undef FEM_AMG_T
+  set_error("fem_bcsr_transpose_values: unsupported block shape %d x %d", X->br, X->bc);
+  return FEM_ERR_BAD_ARG;
+}
+
+int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* stream) {
+  FEM_CHECK_ARG(A && inv && ws && A->rows > 0, "fem_amg_coarse_inverse: bad arguments");
+  const int64_t n = A->rows * A->br;
+  FEM_CHECK_ARG(n <= 4096 && A->cols * A->bc == n, "fem_amg_coarse_inverse: %lld unknowns (at most 4096, square)",
+                (long long)n);
+  hipStream_t s = as_stream(stream);
+  double* a = inv;
+  double* b = static_cast<double*>(ws);
+  const unsigned g = grid_for(n * n, 256);
+  hipLaunchKernelGGL(k_dense_zero, dim3(g), dim3(256), 0, s, n * n, a);
+  hipLaunchKernelGGL(k_dense_scatter, dim3(grid_for(A->rows, 256)), dim3(256), 0, s, A->rows, A->br, A->bc, n,
+                     A->row_ptr, A->col_idx, static_cast<const double*>(A->vals), a);
+  FEM_LAUNCHED();
+  for (int64_t k = 0; k < n; ++k) {
+    hipLaunchKernelGGL(k_gj_step, dim3(g), dim3(256), 0, s, n, k, a, b);
+    std::swap(a, b);
+  }
+  FEM_LAUNCHED();
+  if (a != inv) {
+    hipLaunchKernelGGL(k_copy, dim3(g), dim3(256), 0, s, n * n, a, inv);
+    FEM_LAUNCHED();
+  }
+  return FEM_OK;
+}
+
+int fem_amg_to_f32(int64_t n, const double* in, float* out, void* stream) {
+  FEM_CHECK_ARG(n >= 0 && (n == 0 || (in && out)), "fem_amg_to_f32: bad arguments");
+  if (n == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_to_f32, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), n, in, out);
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int64_t fem_amg_ws_bytes(const fem_amg_t* amg) {
+  if (!amg || !amg->levels || amg->num_levels < 1 || !amg->levels[0].pat) return 0;
+  return vcycle_ws(*amg);
+}
+
+int fem_amg_vcycle(const fem_amg_t* amg, const double* r, double* z, void* ws, void* stream) {
+  if (int rc = check_amg(amg)) return rc;
+  FEM_CHECK_ARG(r && z && ws && r != z, "fem_amg_vcycle: bad arguments");
+  hipStream_t s = as_stream(stream);
+  char* w = static_cast<char*>(ws);
+  return amg->precision == FEM_F32 ? vcycle_impl<float>(*amg, r, z, w, s) : vcycle_impl<double>(*amg, r, z, w, s);
+}
+
+int64_t fem_amg_solve_ws_bytes(const fem_pattern_t* pat, const fem_amg_t* amg) {
+  if (!pat) return 0;
+  const int64_t n = std::max<int64_t>(pat->num_rows * pat->vec, 1);
+  const int64_t np = std::max<int64_t>(pat->num_slices, kDotBlocks);
+  return fem_amg_ws_bytes(amg) + (int64_t)(4 * align256(n * 8) + align256(np * 8) + align256(sizeof(AmgScalars)));
+}
+
+int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_amg_t* amg, const double* b, double* x,
+                      const fem_amg_options_t* opt, void* ws, void* host_status, fem_pcg_result_t* result,
+                      void* stream) {
+  if (int rc = check_amg(amg)) return rc;
+  FEM_CHECK_ARG(pat && vals && b && x && opt && ws && host_status && result, "fem_amg_pcg_solve: NULL argument");
+  FEM_CHECK_ARG(amg->levels[0].pat->num_rows == pat->num_rows && amg->levels[0].pat->vec == pat->vec,
+                "fem_amg_pcg_solve: the hierarchy was built for another pattern");
+  FEM_CHECK_ARG(opt->maxiter >= 0 && opt->check_every >= 1, "fem_amg_pcg_solve: bad options");
+  hipStream_t s = as_stream(stream);
+  const int64_t n = pat->num_rows * pat->vec;
+  char* w = static_cast<char*>(ws);
+  char* cyc = w;
+  w += vcycle_ws(*amg);
+  double* r = reinterpret_cast<double*>(w);
+  w += align256(std::max<int64_t>(n, 1) * 8);
+  double* z = reinterpret_cast<double*>(w);
+  w += align256(std::max<int64_t>(n, 1) * 8);
+  double* p = reinterpret_cast<double*>(w);
+  w += align256(std::max<int64_t>(n, 1) * 8);
+  double* q = reinterpret_cast<double*>(w);
+  w += align256(std::max<int64_t>(n, 1) * 8);
+  double* part = reinterpret_cast<double*>(w);
+  w += align256(std::max<int64_t>(pat->num_slices, kDotBlocks) * 8);
+  AmgScalars* sc = reinterpret_cast<AmgScalars*>(w);
+  const int32_t* done = &sc->done;
+  const unsigned g = grid_for(n, 256);
+  *result = fem_pcg_result_t{};
+  // |b|^2, r = b - A x, |r|^2
+  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, b, b, part, (const int32_t*)nullptr);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, &sc->bb);
+  FEM_LAUNCHED();
+  ApplyArgs ar{x, r, b, nullptr, nullptr, 0.0, 0.0, nullptr};
+  if (int rc = sell_apply<AP_RESID, double>(*pat, vals, ar, nullptr, s)) return rc;
+  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, r, r, part, (const int32_t*)nullptr);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, &sc->rr);
+  hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1), 0, s, sc, opt->rtol, opt->atol, opt->maxiter);
+  FEM_LAUNCHED();
+  AmgScalars h{};
+  if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &h)) return rc;
+  int it = 0;
+  while (!h.done) {
+    if (int rc = (amg->precision == FEM_F32 ? vcycle_impl<float>(*amg, r, z, cyc, s)
+                                            : vcycle_impl<double>(*amg, r, z, cyc, s)))
+      return rc;
+    hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, r, z, part, done);
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, part, kDotBlocks, sc);
+    hipLaunchKernelGGL(k_cg_p, dim3(g), dim3(256), 0, s, n, z, p, sc);
+    FEM_LAUNCHED();
+    ApplyArgs apq{p, q, nullptr, nullptr, nullptr, 0.0, 0.0, part};
+    if (int rc = sell_apply<AP_PQ, double>(*pat, vals, apq, done, s)) return rc;
+    hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, part, (int)pat->num_slices, sc);
+    hipLaunchKernelGGL(k_cg_xr, dim3(kDotBlocks), dim3(256), 0, s, n, p, q, x, r, part, sc);
+    hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, s, part, kDotBlocks, sc);
+    FEM_LAUNCHED();
+    ++it;
+    if (it % opt->check_every == 0 || it >= opt->maxiter) {
+      if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &h)) return rc;
+      if (it >= opt->maxiter && !h.done) break;
+    }
+  }
+  // true residual
+  ApplyArgs af{x, q, b, nullptr, nullptr, 0.0, 0.0, nullptr};
+  if (int rc = sell_apply<AP_RESID, double>(*pat, vals, af, nullptr, s)) return rc;
+  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, q, q, part, (const int32_t*)nullptr);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, &sc->rz);
+  FEM_LAUNCHED();
+  AmgScalars fin{};
+  if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &fin)) return rc;
+  result->iterations = fin.iter;
+  result->replacements = 0;
+  result->b_norm = sqrt(fin.bb);
+  result->res_norm = sqrt(fin.rz);
+  result->status = fin.nan ? FEM_ERR_NAN : (fin.rr > fin.stop2 ? FEM_ERR_NOT_CONVERGED : FEM_OK);
+  return FEM_OK;
+}
+
+}  // extern "C"

@@ -88,7 +88,8 @@ __device__ inline T block_sum(T v, T* smem) {
 // FMAs cost no time).  The matrix stream (values, column ids) is read once per
 // product with nontemporal loads, which keeps the gathered vector entries in
 // the caches.  Returns this wave's partial sums.
-template <int V, bool DIFF, int W, int U, typename T, typename XAt>
+// TX is the type `xat` loads (the storage type T unless stated).
+template <int V, bool DIFF, int W, int U, typename T, typename XAt, typename TX = T>
 __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __restrict__ vals, int64_t s, int w,
                                           int lane, int64_t row, const XAt& xat, const double (&xr)[V],
                                           double (&acc)[V]) {
@@ -102,7 +103,7 @@ __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __r
 #pragma unroll U
   for (int k = w; k < L; k += W) {
     const int64_t c = __builtin_nontemporal_load(cp + (size_t)k * kSlice);
-    T xt[V];
+    TX xt[V];
     xat(c, xt);
     double xc[V];
 #pragma unroll

@@ -5,6 +5,7 @@ allocates device memory, supplies the stream and evaluates the user's own
 mass/surface callbacks at quadrature points.
 """
 import ctypes as C
+import time
 import logging
 
 import numpy as onp
@@ -181,6 +182,7 @@ class DeviceProblem:
         self.param = None
         self.load = None
         self.host_status = torch.empty(128, dtype=torch.uint8).pin_memory()
+        self._amg = None
 
     # ------------------------------------------------------------ pattern
     def _build_pattern(self):
@@ -434,7 +436,9 @@ class DeviceProblem:
         return y
 
     def solve(self, u, b, x0, cfg):
-        """Jacobi-PCG on the current tangent; returns (x, result struct)."""
+        """Jacobi-PCG (or AMG-CG for cfg['method'] == 'amg') on the current tangent; returns (x, result struct)."""
+        if cfg.get('method') == 'amg':
+            return self.amg_solve(b, x0, cfg)
         precision, Ke, vals, dinv = self.current
         o = _lib.FemPcgOptions()
         o.rtol, o.atol, o.maxiter = cfg['tol'], cfg['atol'], cfg['maxiter']
@@ -453,6 +457,26 @@ class DeviceProblem:
                                    ptr(b), ptr(x), C.byref(o), ptr(ws), C.c_void_p(self.host_status.data_ptr()),
                                    C.byref(res), stream_ptr()), 'fem_pcg_solve')
         return x, res
+
+    def amg_setup(self, cfg):
+        """Smoothed-aggregation hierarchy of the current FP64 matrix."""
+        precision, _, vals, _ = self.current
+        if precision != _lib.FEM_F64:
+            raise _lib.FemError("the AMG hierarchy is built from the FP64 matrix")
+        from .amg import DeviceAMG
+        if self._amg is None:
+            self._amg = DeviceAMG(self)
+        return self._amg.setup(vals, cfg['amg_precision'], cfg['degree'])
+
+    def amg_solve(self, b, x0, cfg):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        amg = self.amg_setup(cfg)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        out = amg.solve(self.current[2], b, x0, cfg)
+        self.amg_timing = dict(setup_s=t1 - t0, solve_s=time.perf_counter() - t1)
+        return out
 
     def spmv(self, x):
         """y = A x with the current matrix, in its precision and storage form."""

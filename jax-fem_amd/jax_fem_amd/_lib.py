@@ -58,6 +58,25 @@ class FemPcgResult(C.Structure):
                 ('timed_launches', _i32), ('pad2', _i32)]
 
 
+class FemBcsr(C.Structure):
+    _fields_ = [('rows', _i64), ('cols', _i64), ('nnzb', _i64), ('br', _i32), ('bc', _i32),
+                ('row_ptr', _vp), ('col_idx', _vp), ('vals', _vp)]
+
+
+class FemAmgLevel(C.Structure):
+    _fields_ = [('pat', C.POINTER(FemPattern)), ('sell_vals', _vp), ('A', FemBcsr), ('P', FemBcsr),
+                ('R', FemBcsr), ('dinv', _vp), ('lmax', _f64)]
+
+
+class FemAmg(C.Structure):
+    _fields_ = [('num_levels', _i32), ('precision', _i32), ('degree', _i32), ('pad', _i32),
+                ('levels', C.POINTER(FemAmgLevel)), ('coarse_n', _i64), ('coarse_inv', _vp)]
+
+
+class FemAmgOptions(C.Structure):
+    _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32)]
+
+
 _P = C.POINTER
 _SIGNATURES = {
     'fem_last_error': (C.c_char_p, []),
@@ -92,6 +111,29 @@ _SIGNATURES = {
     'fem_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemOperator), _vp, _vp, _P(FemPcgOptions),
                                 _vp, _vp, _P(FemPcgResult), _vp]),
     'fem_dot': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
+    'fem_amg_aggregate_ws_bytes': (_i64, [_i64]),
+    'fem_amg_aggregate': (C.c_int, [_i64, _vp, _vp, _vp, _P(_i64), _vp, _vp, _vp]),
+    'fem_amg_members': (C.c_int, [_i64, _vp, _i64, _vp, _vp, _vp, _vp]),
+    'fem_amg_tentative': (C.c_int, [_i64, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_amg_bc_mask': (C.c_int, [_i64, _i64, _vp, _vp, _vp]),
+    'fem_amg_sell_to_bcsr': (C.c_int, [_P(FemPattern), _vp, _vp, _vp]),
+    'fem_amg_dinv': (C.c_int, [_P(FemBcsr), _i32, _vp, _vp]),
+    'fem_amg_lmax_ws_bytes': (_i64, [_i64]),
+    'fem_amg_lmax': (C.c_int, [_P(FemBcsr), _P(FemPattern), _vp, _vp, _i32, _P(_f64), _vp, _vp]),
+    'fem_bcsr_spgemm_symbolic': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp, _P(_i64), _vp, _vp, _vp]),
+    'fem_bcsr_spgemm_plan': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _i64, _P(FemBcsr), _vp, _vp, _vp, _vp, _vp,
+                                       _vp]),
+    'fem_bcsr_spgemm_numeric': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp]),
+    'fem_amg_smooth_prolongator': (C.c_int, [_P(FemBcsr), _vp, _vp, _vp, _f64, _vp]),
+    'fem_bcsr_transpose_structure': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp]),
+    'fem_bcsr_transpose_values': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp]),
+    'fem_amg_coarse_inverse': (C.c_int, [_P(FemBcsr), _vp, _vp, _vp]),
+    'fem_amg_to_f32': (C.c_int, [_i64, _vp, _vp, _vp]),
+    'fem_amg_ws_bytes': (_i64, [_P(FemAmg)]),
+    'fem_amg_vcycle': (C.c_int, [_P(FemAmg), _vp, _vp, _vp, _vp]),
+    'fem_amg_solve_ws_bytes': (_i64, [_P(FemPattern), _P(FemAmg)]),
+    'fem_amg_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _P(FemAmg), _vp, _vp, _P(FemAmgOptions), _vp, _vp,
+                                    _P(FemPcgResult), _vp]),
 }
 
 EXPORTED = tuple(_SIGNATURES)

@@ -1,0 +1,308 @@
+"""Device smoothed-aggregation AMG: hierarchy setup and AMG-preconditioned CG.
+
+Device counterpart of jax-fem's `amgx_solver` option (solver.py:141-259,
+selected in linear_solver, solver.py:261-284).  The reference hands the
+assembled PETSc matrix to AMGX; here every stage runs in libfem_mi355x on the
+GPU and this module only sequences the calls, sizing the buffers between them
+(the library allocates nothing):
+
+  level 0   the assembled FP64 SELL matrix (row-eliminated, standard form) and
+            its block CSR copy; the FE block pattern is the aggregation graph
+  per level dinv -> MIS-2 aggregates -> tentative P0 (rigid-body modes or
+            constants, Dirichlet rows zero) -> lambda_max(D^-1 A) -> A P0 ->
+            P = P0 - omega D^-1 A P0 -> R = P^T -> Ac = R (A P)
+  coarsest  dense Gauss-Jordan inverse (<= 512 unknowns)
+
+The structure (aggregates, tentative prolongators, patterns, product plans)
+is built once per pattern and Dirichlet set; each new matrix then costs the
+numeric passes only.
+
+`precision='mixed'` stores the cycle's matrices (level-0 SELL copy, A, P, R)
+in FP32; the outer CG keeps the FP64 operator, so the solution accuracy is
+that of the FP64 solve.
+"""
+import ctypes as C
+import logging
+
+import torch
+
+from . import _lib
+from ._lib import check, ptr, stream_ptr
+
+logger = logging.getLogger('jax_fem_amd')
+
+COARSE_MAX = 512    # unknowns of the coarsest (dense) level
+MAX_LEVELS = 10
+POWER_STEPS = 15
+DEGREE = 2          # Chebyshev degree of pre- and post-smoothing
+
+
+class Bcsr:
+    """Block CSR matrix held in device tensors."""
+
+    def __init__(self, rows, cols, br, bc, row_ptr, col_idx, vals):
+        self.rows, self.cols, self.br, self.bc = int(rows), int(cols), int(br), int(bc)
+        self.row_ptr, self.col_idx, self.vals = row_ptr, col_idx, vals
+        self.nnzb = int(col_idx.numel()) if col_idx is not None else 0
+
+    def struct(self, vals=None):
+        v = self.vals if vals is None else vals
+        s = _lib.FemBcsr()
+        s.rows, s.cols, s.nnzb, s.br, s.bc = self.rows, self.cols, self.nnzb, self.br, self.bc
+        s.row_ptr = self.row_ptr.data_ptr()
+        s.col_idx = self.col_idx.data_ptr() if self.nnzb else None
+        s.vals = v.data_ptr() if self.nnzb else None
+        return s
+
+    def host(self):
+        """(row_ptr, col_idx, vals (nnzb, br, bc)) as numpy arrays (tests)."""
+        return (self.row_ptr.cpu().numpy(), self.col_idx.cpu().numpy(),
+                self.vals.reshape(-1, self.br, self.bc).cpu().numpy())
+
+
+class DeviceAMG:
+    """Hierarchy of one DeviceProblem's current FP64 matrix."""
+
+    def __init__(self, dev):
+        self.dev = dev
+        self.L = dev.L
+        self._mask = None
+        self._f32 = {}
+        self.levels = []
+        self.amg = None
+
+    # -------------------------------------------------------------- helpers
+    def _empty(self, n, dtype=torch.float64):
+        return torch.empty(max(int(n), 1), dtype=dtype, device=self.dev.dm.device)
+
+    def _scan(self, counts):
+        return self.dev.dm.scan(counts)
+
+    def _aggregate(self, n, row_ptr, col_idx):
+        ws = self._empty(self.L.fem_amg_aggregate_ws_bytes(n), torch.uint8)
+        agg = self._empty(n, torch.int32)
+        n_agg = C.c_int64(0)
+        check(self.L.fem_amg_aggregate(n, ptr(row_ptr), ptr(col_idx), ptr(agg), C.byref(n_agg), ptr(ws),
+                                       C.c_void_p(self.dev.host_status.data_ptr()), stream_ptr()),
+              'fem_amg_aggregate')
+        return agg[:n], int(n_agg.value)
+
+    def _members(self, n, agg, n_agg):
+        agg_ptr = self._empty(n_agg + 1, torch.int32)
+        memb = self._empty(n, torch.int32)
+        nb = ((4 * max(n_agg, 1) + 255) // 256) * 256 + int(self.L.fem_scan_ws_bytes(n_agg))
+        ws = self._empty(nb, torch.uint8)
+        check(self.L.fem_amg_members(n, ptr(agg), n_agg, ptr(agg_ptr), ptr(memb), ptr(ws), stream_ptr()),
+              'fem_amg_members')
+        return agg_ptr, memb
+
+    def _spgemm_plan(self, X, Y):
+        """Structure of C = X Y and its product plan (once per pattern)."""
+        xs, ys = X.struct(), Y.struct()
+        counts = self._empty(X.rows, torch.int32)
+        pairs = self._empty(X.rows, torch.int32)
+        mp = C.c_int64(0)
+        ws = self._empty(256, torch.uint8)
+        hs = C.c_void_p(self.dev.host_status.data_ptr())
+        check(self.L.fem_bcsr_spgemm_symbolic(C.byref(xs), C.byref(ys), ptr(counts), ptr(pairs), C.byref(mp),
+                                              ptr(ws), hs, stream_ptr()), 'fem_bcsr_spgemm_symbolic')
+        rp = self._scan(counts[:X.rows])
+        pair_ptr = self._scan(pairs[:X.rows])
+        nnzb = int(rp[-1].item())
+        total = int(pair_ptr[-1].item())
+        out = Bcsr(X.rows, Y.cols, X.br, Y.bc, rp, self._empty(nnzb, torch.int32)[:nnzb],
+                   self._empty(nnzb * X.br * Y.bc)[:nnzb * X.br * Y.bc])
+        plan_ptr = self._empty(nnzb + 1, torch.int32)
+        plan = self._empty(2 * total, torch.int32)
+        cs = out.struct()
+        check(self.L.fem_bcsr_spgemm_plan(C.byref(xs), C.byref(ys), mp.value, C.byref(cs), ptr(pair_ptr),
+                                          ptr(plan_ptr), ptr(plan), ptr(ws), hs, stream_ptr()),
+              'fem_bcsr_spgemm_plan')
+        return out, (plan_ptr, plan)
+
+    def _spgemm_numeric(self, X, Y, out, plan):
+        xs, ys, cs = X.struct(), Y.struct(), out.struct()
+        check(self.L.fem_bcsr_spgemm_numeric(C.byref(xs), C.byref(ys), C.byref(cs), ptr(plan[0]), ptr(plan[1]),
+                                             stream_ptr()), 'fem_bcsr_spgemm_numeric')
+
+    def _transpose_structure(self, X):
+        T = Bcsr(X.cols, X.rows, X.bc, X.br, self._empty(X.cols + 1, torch.int32),
+                 self._empty(X.nnzb, torch.int32)[:X.nnzb], self._empty(X.nnzb * X.br * X.bc))
+        src = self._empty(X.nnzb, torch.int32)
+        nb = ((4 * max(X.cols, 1) + 255) // 256) * 256 + int(self.L.fem_scan_ws_bytes(X.cols))
+        ws = self._empty(nb, torch.uint8)
+        xs, ts = X.struct(), T.struct()
+        check(self.L.fem_bcsr_transpose_structure(C.byref(xs), C.byref(ts), ptr(src), ptr(ws), stream_ptr()),
+              'fem_bcsr_transpose_structure')
+        return T, src
+
+    def _transpose_values(self, X, T, src):
+        xs, ts = X.struct(), T.struct()
+        check(self.L.fem_bcsr_transpose_values(C.byref(xs), C.byref(ts), ptr(src), stream_ptr()),
+              'fem_bcsr_transpose_values')
+
+    def _dinv(self, A, fix):
+        d = self._empty(A.rows * A.br)
+        s = A.struct()
+        check(self.L.fem_amg_dinv(C.byref(s), 1 if fix else 0, ptr(d), stream_ptr()), 'fem_amg_dinv')
+        return d
+
+    def _lmax(self, A, dinv, sell_vals=None):
+        """Power estimate of lambda_max(D^-1 A); level 0 runs on the SELL matrix."""
+        s = A.struct()
+        ws = self._empty(self.L.fem_amg_lmax_ws_bytes(A.rows * A.br), torch.uint8)
+        lam = C.c_double(0.0)
+        pat = self.dev.pref if sell_vals is not None else None
+        check(self.L.fem_amg_lmax(C.byref(s), pat, ptr(sell_vals), ptr(dinv), POWER_STEPS, C.byref(lam), ptr(ws),
+                                  stream_ptr()), 'fem_amg_lmax')
+        return float(lam.value)
+
+    def _to_f32(self, t, key):
+        out = self._f32.get(key)
+        if out is None or out.numel() != t.numel():
+            out = torch.empty(t.numel(), dtype=torch.float32, device=t.device)
+            self._f32[key] = out
+        check(self.L.fem_amg_to_f32(t.numel(), ptr(t), ptr(out), stream_ptr()), 'fem_amg_to_f32')
+        return out
+
+    # --------------------------------------------------------------- setup
+    def _structure(self):
+        """Value-independent part of the hierarchy, kept while the pattern and
+        the Dirichlet set are unchanged: aggregates, tentative prolongators,
+        the patterns of P, R, A P and R (A P), their product plans and the
+        transpose map.  Level 0's graph is the FE block pattern."""
+        dev, L = self.dev, self.L
+        src = getattr(self, '_struct_src', None)
+        if src is not None and src[0] is dev.pat_tensors['row_ptr'] and src[1] is dev.bc_dofs:
+            return self._struct
+        V = dev.V
+        ns = 6 if V == 3 else 1
+        self._mask = self._empty(dev.n, torch.uint8)
+        check(L.fem_amg_bc_mask(dev.n, dev.n_bc, ptr(dev.bc_dofs) if dev.n_bc else None, ptr(self._mask),
+                                stream_ptr()), 'fem_amg_bc_mask')
+        pt = dev.pat_tensors
+        A = Bcsr(dev.dm.Nn, dev.dm.Nn, V, V, pt['row_ptr'], pt['col_idx'], self._empty(dev.nnzb * V * V))
+        B, bs = None, V
+        levels = []
+        while A.rows * A.br > COARSE_MAX and len(levels) < MAX_LEVELS - 1:
+            agg, n_agg = self._aggregate(A.rows, A.row_ptr, A.col_idx)
+            if n_agg >= A.rows:
+                break
+            agg_ptr, memb = self._members(A.rows, agg, n_agg)
+            P0 = self._empty(A.rows * bs * ns)
+            Bc = self._empty(n_agg * ns * ns)
+            check(L.fem_amg_tentative(A.rows, bs, ns, n_agg, ptr(agg_ptr), ptr(memb),
+                                      ptr(dev.dm.points) if B is None else None,
+                                      ptr(self._mask) if B is None else None, ptr(B), ptr(P0), ptr(Bc),
+                                      stream_ptr()), 'fem_amg_tentative')
+            P0m = Bcsr(A.rows, n_agg, bs, ns, torch.arange(A.rows + 1, dtype=torch.int32, device=P0.device),
+                       agg, P0[:A.rows * bs * ns])
+            P, plan_w0 = self._spgemm_plan(A, P0m)          # A P0 on the pattern of P
+            R, src = self._transpose_structure(P)
+            W, plan_w = self._spgemm_plan(A, P)
+            Ac, plan_c = self._spgemm_plan(R, W)
+            levels.append(dict(A=A, agg=agg, n_agg=n_agg, P0=P0, P0m=P0m, Bc=Bc, P=P, R=R, R_src=src, W=W,
+                               plans=(plan_w0, plan_w, plan_c)))
+            A, B, bs = Ac, Bc, ns
+        if not levels:
+            raise _lib.FemError("the matrix is too small for a multigrid hierarchy; use 'jax_solver'")
+        n_c = A.rows * A.br
+        if n_c > 4096:
+            raise _lib.FemError(f"AMG coarsening stalled at {n_c} unknowns (the dense coarse solve takes <= 4096)")
+        self._struct = (levels, A)
+        self._struct_src = (dev.pat_tensors['row_ptr'], dev.bc_dofs)
+        return self._struct
+
+    def setup(self, vals64, precision=_lib.FEM_F64, degree=DEGREE):
+        """Hierarchy of the current FP64 matrix (standard-form SELL values `vals64`)."""
+        dev, L = self.dev, self.L
+        levels, coarse = self._structure()
+        A0 = levels[0]['A']
+        check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), stream_ptr()), 'fem_amg_sell_to_bcsr')
+        for l, lv in enumerate(levels):
+            A = lv['A']
+            lv['dinv'] = self._dinv(A, fix=l > 0)
+            lv['lam'] = self._lmax(A, lv['dinv'], vals64 if l == 0 else None)
+            omega = 4.0 / (3.0 * lv['lam'])
+            plan_w0, plan_w, plan_c = lv['plans']
+            P = lv['P']
+            self._spgemm_numeric(A, lv['P0m'], P, plan_w0)
+            ps = P.struct()
+            check(L.fem_amg_smooth_prolongator(C.byref(ps), ptr(lv['P0']), ptr(lv['agg']), ptr(lv['dinv']), omega,
+                                               stream_ptr()), 'fem_amg_smooth_prolongator')
+            self._transpose_values(P, lv['R'], lv['R_src'])
+            self._spgemm_numeric(A, P, lv['W'], plan_w)
+            nxt = levels[l + 1]['A'] if l + 1 < len(levels) else coarse
+            self._spgemm_numeric(lv['R'], lv['W'], nxt, plan_c)
+        self._dinv(coarse, fix=True)
+        n_c = coarse.rows * coarse.br
+        if getattr(self, '_coarse_n', None) != n_c:
+            self.coarse_inv = self._empty(n_c * n_c)
+            self._gj_ws = self._empty(n_c * n_c)
+            self._coarse_n = n_c
+        cs = coarse.struct()
+        check(L.fem_amg_coarse_inverse(C.byref(cs), ptr(self.coarse_inv), ptr(self._gj_ws), stream_ptr()),
+              'fem_amg_coarse_inverse')
+        self.levels, self.coarse = levels, coarse
+        self._descriptor(vals64, precision, degree)
+        return self
+
+    def _descriptor(self, vals64, precision, degree):
+        dev = self.dev
+        f32 = precision == _lib.FEM_F32
+        keep = []
+        arr = (_lib.FemAmgLevel * len(self.levels))()
+        for l, lv in enumerate(self.levels):
+            e = arr[l]
+            if l == 0:
+                e.pat = C.pointer(dev.pat)
+                sv = self._to_f32(vals64[:dev.sell_blocks * dev.V * dev.V], 'sell') if f32 else vals64
+                keep.append(sv)
+                e.sell_vals = sv.data_ptr()
+            for name in ('A', 'P', 'R'):
+                M = lv[name]
+                v = self._to_f32(M.vals, (l, name)) if (f32 and not (name == 'A' and l == 0)) else M.vals
+                keep.append(v)
+                setattr(e, name, M.struct(v))
+            e.dinv = lv['dinv'].data_ptr()
+            e.lmax = lv['lam']
+        amg = _lib.FemAmg()
+        amg.num_levels = len(self.levels)
+        amg.precision = _lib.FEM_F32 if f32 else _lib.FEM_F64
+        amg.degree = degree
+        amg.levels = arr
+        amg.coarse_n = self.coarse.rows * self.coarse.br
+        amg.coarse_inv = self.coarse_inv.data_ptr()
+        self._keep = (keep, arr)
+        self.amg = amg
+        self.precision = amg.precision
+
+    def describe(self):
+        return [dict(rows=lv['A'].rows * lv['A'].br, nnz=lv['A'].nnzb * lv['A'].br * lv['A'].bc,
+                     aggregates=lv['n_agg'], lmax=lv['lam'], p_nnz=lv['P'].nnzb * lv['P'].br * lv['P'].bc)
+                for lv in self.levels] + [dict(rows=self.coarse.rows * self.coarse.br)]
+
+    # --------------------------------------------------------------- apply
+    def vcycle(self, r):
+        """z = M r (one V-cycle)."""
+        z = torch.empty_like(r)
+        ws = self._empty(self.L.fem_amg_ws_bytes(C.byref(self.amg)), torch.uint8)
+        check(self.L.fem_amg_vcycle(C.byref(self.amg), ptr(r), ptr(z), ptr(ws), stream_ptr()), 'fem_amg_vcycle')
+        return z
+
+    def solve(self, vals64, b, x0, cfg):
+        """AMG-preconditioned CG on the FP64 operator; returns (x, result struct)."""
+        dev = self.dev
+        o = _lib.FemAmgOptions()
+        o.rtol, o.atol, o.maxiter, o.check_every = cfg['tol'], cfg['atol'], cfg['maxiter'], cfg['check_every']
+        nb = int(self.L.fem_amg_solve_ws_bytes(dev.pref, C.byref(self.amg)))
+        key = ('amg_ws', nb)
+        if dev.buffers.get('amg_ws_key') != key:
+            dev.buffers['amg_ws'] = self._empty(nb, torch.uint8)
+            dev.buffers['amg_ws_key'] = key
+        x = x0.clone()
+        res = _lib.FemPcgResult()
+        check(self.L.fem_amg_pcg_solve(dev.pref, ptr(vals64), C.byref(self.amg), ptr(b), ptr(x), C.byref(o),
+                                       ptr(dev.buffers['amg_ws']), C.c_void_p(dev.host_status.data_ptr()),
+                                       C.byref(res), stream_ptr()), 'fem_amg_pcg_solve')
+        return x, res

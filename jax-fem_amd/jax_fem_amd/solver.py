@@ -33,6 +33,14 @@ JAX_SOLVER_DEFAULTS = dict(precond=True, precision='fp64', tol=1e-10, atol=1e-10
 _ASSEMBLY = {None: None, 'fp32': _lib.FEM_F32, 'fp64': _lib.FEM_F64, 'bf16x3': _lib.FEM_BF16X3}
 _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32}
 
+# Keys of {'amgx_solver': {...}} (solver.py:251-259 reads only `cfg_path`, an
+# AMGX JSON file).  The device path builds a smoothed-aggregation hierarchy and
+# runs AMG-preconditioned FP64 CG; `precision` 'mixed' stores the hierarchy in
+# FP32.  From an AMGX config file only solver.tolerance and solver.max_iters are
+# taken (as atol and maxiter, AMGX's ABSOLUTE convergence).
+AMG_SOLVER_DEFAULTS = dict(cfg_path=None, precision='fp64', tol=1e-10, atol=1e-10, maxiter=1000, check_every=1,
+                           degree=2)
+
 
 def _resolve_solver_options(solver_options):
     """(method, cfg); legacy flat dicts become Newton (solver.py:1088-1106)."""
@@ -60,9 +68,12 @@ def linear_config(linear_options):
         user = {}
     elif 'jax_solver' in linear_options:
         user = dict(linear_options['jax_solver'] or {})
+    elif 'amgx_solver' in linear_options:
+        return _amg_config(dict(linear_options['amgx_solver'] or {}))
     else:
         raise NotImplementedError(
-            f"linear solver {sorted(present)} is not available on the device path; use 'jax_solver'")
+            f"linear solver {sorted(present)} is not available on the device path; "
+            "use 'jax_solver' or 'amgx_solver'")
     unknown = set(user) - set(JAX_SOLVER_DEFAULTS)
     if unknown:
         raise ValueError(f"unknown jax_solver options {sorted(unknown)}")
@@ -74,6 +85,31 @@ def linear_config(linear_options):
     if cfg['assembly'] not in _ASSEMBLY:
         raise ValueError(f"assembly must be 'fp32', 'fp64' or 'bf16x3', got {cfg['assembly']!r}")
     cfg['assembly_code'] = _ASSEMBLY[cfg['assembly']]
+    cfg['method'] = 'pcg'
+    return cfg
+
+
+def _amg_config(user):
+    unknown = set(user) - set(AMG_SOLVER_DEFAULTS)
+    if unknown:
+        raise ValueError(f"unknown amgx_solver options {sorted(unknown)}")
+    cfg = dict(AMG_SOLVER_DEFAULTS)
+    cfg.update(user)
+    if cfg['cfg_path'] is not None:
+        import json
+        with open(cfg['cfg_path']) as f:
+            amgx = json.load(f).get('solver', {})
+        if 'tolerance' in amgx:
+            cfg['atol'], cfg['tol'] = float(amgx['tolerance']), 0.0
+        if 'max_iters' in amgx:
+            cfg['maxiter'] = int(amgx['max_iters'])
+    if cfg['precision'] not in ('fp64', 'mixed'):
+        raise ValueError(f"amgx_solver precision must be 'fp64' or 'mixed', got {cfg['precision']!r}")
+    cfg['method'] = 'amg'
+    cfg['amg_precision'] = _lib.FEM_F32 if cfg['precision'] == 'mixed' else _lib.FEM_F64
+    cfg['precision_code'] = _lib.FEM_F64      # the outer CG operator is the FP64 matrix
+    cfg['assembly_code'] = None
+    cfg['precond'] = True
     return cfg
 
 
@@ -81,10 +117,11 @@ def linear_solve(dev, u, b, x0, cfg):
     """Solve the current tangent system; mirrors jax_solve's post-check (solver.py:86-90)."""
     x, info = dev.solve(u, b, x0, cfg)
     err = info.res_norm
-    logger.debug("PCG (%s): %d iterations, %d FP64 residual replacements, |b - A x| = %.3g",
-                 cfg['precision'], info.iterations, info.replacements, err)
+    logger.debug("%s (%s): %d iterations, %d FP64 residual replacements, |b - A x| = %.3g",
+                 'AMG-CG' if cfg.get('method') == 'amg' else 'PCG', cfg['precision'], info.iterations,
+                 info.replacements, err)
     if info.status == _lib.FEM_ERR_NOT_CONVERGED:
-        logger.warning("PCG reached maxiter=%d, |b - A x| = %.3g", cfg['maxiter'], err)
+        logger.warning("the linear solve reached maxiter=%d, |b - A x| = %.3g", cfg['maxiter'], err)
     assert info.status != _lib.FEM_ERR_NAN and math.isfinite(err) and err < 0.1, \
         f"JAX linear solver failed to converge with err = {err}"
     return x, info

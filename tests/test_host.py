@@ -36,6 +36,27 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.FemPattern) == 4 * 8 + 4 * 4 + 9 * 8
     assert ctypes.sizeof(_lib.FemPcgOptions) == 8 + 8 + 4 * 4 + 8 + 8
     assert ctypes.sizeof(_lib.FemPcgResult) == 16 + 16 + 16 + 8
+    assert ctypes.sizeof(_lib.FemBcsr) == 3 * 8 + 2 * 4 + 3 * 8
+    assert ctypes.sizeof(_lib.FemAmgLevel) == 8 + 8 + 3 * ctypes.sizeof(_lib.FemBcsr) + 8 + 8
+    assert ctypes.sizeof(_lib.FemAmg) == 4 * 4 + 8 + 8 + 8
+    assert ctypes.sizeof(_lib.FemAmgOptions) == 8 + 8 + 4 + 4
+
+
+def test_amgx_solver_options(tmp_path):
+    cfg = linear_config({'amgx_solver': {}})
+    assert cfg['method'] == 'amg' and cfg['precision_code'] == _lib.FEM_F64
+    assert cfg['amg_precision'] == _lib.FEM_F64
+    assert linear_config({'amgx_solver': {'precision': 'mixed'}})['amg_precision'] == _lib.FEM_F32
+    with pytest.raises(ValueError):
+        linear_config({'amgx_solver': {'smoother': 'JACOBI'}})
+    with pytest.raises(ValueError):
+        linear_config({'amgx_solver': {'precision': 'fp16'}})
+    # an AMGX config file (solver.py:158-187 layout): tolerance and max_iters are taken
+    path = tmp_path / 'amgx.json'
+    path.write_text('{"config_version": 2, "solver": {"solver": "PCG", "tolerance": 1e-9, "max_iters": 77}}')
+    cfg = linear_config({'amgx_solver': {'cfg_path': str(path)}})
+    assert cfg['atol'] == 1e-9 and cfg['tol'] == 0.0 and cfg['maxiter'] == 77
+    assert linear_config({})['method'] == 'pcg'
 
 
 def test_compute_entry_fails_loudly_without_gpu():
