@@ -8,11 +8,17 @@ same problem also yields the adjoint gradient dC/dtheta of the compliance
 C = f.u.  Data are synthetic (generated mesh); inputs are resident on the GPU.
 
 A step is one Newton step of ``solver`` (residual, tangent assembly, BCs,
-Jacobi-PCG to |r| <= 1e-10 |b|, update, new residual); for this linear problem
-``solver`` performs exactly one.  ``value`` is the mixed-precision time
-(FP32 matrix + FP32 PCG, FP64 iterate and FP64 residual replacement);
-the plain FP64 path is timed beside it, and its tight solve (1e-12) is the
-reference for the relative error.
+linear solve to |r| <= 1e-10 |b|, update, new residual); for this linear
+problem ``solver`` performs exactly one.  Four linear-solve paths are timed:
+  amg_mixed  smoothed-aggregation AMG-CG ('amgx_solver', precision 'mixed'):
+             FP64 outer CG on the FP64 matrix, V-cycle matrices stored in FP32
+             (FP64 arithmetic); hierarchy setup inside the step.  ``value``.
+  amg_fp64   the same with an FP64 hierarchy.
+  mixed      Jacobi-PCG ('jax_solver', 'mixed'): FP32 matrix and PCG, FP64
+             iterate and FP64 residual replacement.
+  fp64       Jacobi-PCG in FP64: the plain FP64 path.
+An FP64 Jacobi-PCG solve to rtol 1e-11 is the reference for the relative
+errors.
 
 ``--workload`` selects one of the other configurations of the same path
 instead (one JSON line each, same keys; not the default line):
@@ -77,17 +83,25 @@ def build_problem(nx, ny, nz):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_pcg_a_hbm_traffic_cantilever_128x64x64.md')
+AMG_TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_sell_apply_cheb_hbm_traffic_cantilever_128x64x64.md')
 
 
-def measured_traffic(cells):
-    """HBM bytes per launch of k_pcg_a measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by
-    a streaming kernel of known size) on the 128x64x64 workload; None for other sizes."""
-    if list(cells) != [128, 64, 64] or not os.path.exists(TRAFFIC_PROFILE):
+def measured_traffic(cells, path=TRAFFIC_PROFILE):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a streaming
+    kernel of known size) on the 128x64x64 workload, from the committed summary; {} otherwise."""
+    if list(cells) != [128, 64, 64] or not os.path.exists(path):
         return {}
-    for line in open(TRAFFIC_PROFILE):
+    for line in open(path):
         if line.startswith('traffic_bytes_per_launch:'):
             return {k: int(v) for k, v in (kv.split('=') for kv in line.split(':', 1)[1].split())}
     return {}
+
+
+def cheb_bytes(dev, elem_bytes, vec=3):
+    """Algorithmic bytes of one level-0 Chebyshev step of the V-cycle (k_sell_apply, CHEB): the
+    matrix blocks (values + one column index each, padding not counted) once, and the FP64
+    vectors x (gathered, counted once), rhs, D^-1, d (read and written) and the new x."""
+    return dev.nnzb * (vec * vec * elem_bytes + 4) + dev.n * 8 * 6
 
 
 def sell_bytes(dev, elem_bytes, vec=3):
@@ -114,8 +128,15 @@ def run(args):
     log(f"setup {time.perf_counter() - t0:.1f}s: {dev.dm.C} cells, {dev.n} dofs, {dev.nnzb * 9} nnz "
         f"({dev.sell_blocks} SELL blocks)")
 
-    # timed steps run without per-launch events; one extra step with events gives the kernel averages
-    opts = _paths(args)
+    # timed steps run without per-launch events; one extra step with events gives the kernel averages.
+    # Paths: smoothed-aggregation AMG-CG (amgx_solver) and Jacobi-PCG (jax_solver), each with the
+    # FP32-storage ("mixed") and the FP64 variant; the AMG mixed path is the headline value.
+    opts = {
+        'amg_mixed': {'amgx_solver': {'precision': 'mixed'}},
+        'amg_fp64': {'amgx_solver': {'precision': 'fp64'}},
+        'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}},
+        'fp64': {'jax_solver': {'precision': 'fp64'}},
+    }
     event_opts = {k: _evented(v) for k, v in opts.items()}
     # reference: FP64 path solved to rtol 1e-11 (about the attainable FP64 residual here)
     ref_opts = {'jax_solver': {'precision': 'fp64', 'tol': 1e-11, 'atol': 0.}}
@@ -132,7 +153,7 @@ def run(args):
         return time.perf_counter() - t, sol, p._solve_stats
 
     res = {}
-    paths = ('fp64', 'mixed') if args.only is None else (args.only,)
+    paths = tuple(opts) if args.only is None else (args.only,)
     for name in paths:
         for _ in range(args.warmup):
             newton_step(name)
@@ -146,17 +167,17 @@ def run(args):
         solver(p, event_opts[name])
         lin = p._solve_stats['linear'][0]
         res[name] = dict(s_per_newton_step=statistics.median(times), times=times, rel_err=err,
-                         newton_iterations=stats[-1]['newton_iterations'], pcg_iterations=lin['iterations'],
+                         newton_iterations=stats[-1]['newton_iterations'], iterations=lin['iterations'],
                          replacements=lin['replacements'], spmv_ms=lin['spmv_ms'], update_ms=lin['update_ms'],
                          timing=stats[-1]['timing'])
         tm = stats[-1]['timing']
-        if getattr(dev, 'amg_timing', None):
+        if name.startswith('amg') and getattr(dev, 'amg_timing', None):
             res[name]['amg_timing'] = dict(dev.amg_timing)
-            log(f"{name}: AMG setup {dev.amg_timing['setup_s']*1e3:.1f} ms, solve {dev.amg_timing['solve_s']*1e3:.1f} ms,"
-                f" levels {dev._amg.describe()}")
+            log(f"{name}: AMG setup {dev.amg_timing['setup_s']*1e3:.1f} ms, solve {dev.amg_timing['solve_s']*1e3:.1f} "
+                f"ms, levels {dev._amg.describe()}")
         log(f"{name}: {res[name]['s_per_newton_step']:.4f} s/step, rel err {err:.2e}, "
-            f"{lin['iterations']} its, {lin['replacements']} repl, spmv {lin['spmv_ms']*1e3:.1f} us, "
-            f"update {lin['update_ms']*1e3:.1f} us; residuals {tm['local_assembly']*1e3:.1f} ms, "
+            f"{lin['iterations']} its, {lin['replacements']} repl, kernel {lin['spmv_ms']*1e3:.1f} us, "
+            f"rest {lin['update_ms']*1e3:.1f} us; residuals {tm['local_assembly']*1e3:.1f} ms, "
             f"tangent {tm['global_matrix']*1e3:.1f} ms, linear {tm['linear']*1e3:.1f} ms")
 
     if args.only is not None:
@@ -166,7 +187,8 @@ def run(args):
         return
     # adjoint gradient of the compliance C = f.u w.r.t. theta
     f = -dev.load
-    for name in ('fp64', 'mixed'):
+    grads = {}
+    for name in opts:
         fwd = ad_wrapper(p, opts[name], opts[name])
         tms = []
         for it in range(args.warmup + args.steps):
@@ -178,46 +200,53 @@ def run(args):
             torch.cuda.synchronize()
             if it >= args.warmup:
                 tms.append(time.perf_counter() - t)
-        g = th.grad
         res[name]['s_per_adjoint_gradient'] = statistics.median(tms)
-        res[name]['adjoint_pcg_iterations'] = p._adjoint_stats['iterations']
-        res[name]['grad'] = g.detach().clone()
+        res[name]['adjoint_iterations'] = p._adjoint_stats['iterations']
+        grads[name] = th.grad.detach().clone()
         log(f"{name}: adjoint {res[name]['s_per_adjoint_gradient']:.4f} s, its {p._adjoint_stats['iterations']}")
-    gerr = float(torch.linalg.vector_norm(res['mixed']['grad'] - res['fp64']['grad']) /
-                 torch.linalg.vector_norm(res['fp64']['grad']))
-    for name in ('fp64', 'mixed'):
-        del res[name]['grad']
+    gref = grads['fp64']
+    gerr = {k: float(torch.linalg.vector_norm(g - gref) / torch.linalg.vector_norm(gref)) for k, g in grads.items()}
 
-    mixed = res['mixed']
+    head = res['amg_mixed']
+    amg_traffic = measured_traffic(args.cells, AMG_TRAFFIC_PROFILE)
     traffic = measured_traffic(args.cells)
+    cb = cheb_bytes(dev, 4)
+    achieved = cb / (head['spmv_ms'] * 1e-3) / 1e9 if head['spmv_ms'] else None
     spmv_bytes = sell_bytes(dev, 4)
-    achieved = spmv_bytes / (mixed['spmv_ms'] * 1e-3) / 1e9 if mixed['spmv_ms'] else None
+    pcg_achieved = spmv_bytes / (res['mixed']['spmv_ms'] * 1e-3) / 1e9 if res['mixed']['spmv_ms'] else None
     fp64_bytes = sell_bytes(dev, 8)
     fp64_achieved = fp64_bytes / (res['fp64']['spmv_ms'] * 1e-3) / 1e9 if res['fp64']['spmv_ms'] else None
     out = {
-        "metric": METRIC, "value": mixed['s_per_newton_step'], "unit": "s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": mixed['s_per_newton_step'] * 1e3,
+        "metric": METRIC, "value": head['s_per_newton_step'], "unit": "s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": head['s_per_newton_step'] * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"HEX8 linear-elasticity cantilever 2x1x1, {args.cells[0]}x{args.cells[1]}x"
-                               f"{args.cells[2]} cells, {dev.n} dofs, {_solver_name(args)} rtol 1e-10",
+                               f"{args.cells[2]} cells, {dev.n} dofs, mixed precision (FP32 storage, FP64 "
+                               f"arithmetic and outer CG), SA-AMG-CG rtol 1e-10",
                    "global_batch": 1, "seq_len": dev.n, "parallelism": "single GPU"},
         "fp64_s_per_newton_step": res['fp64']['s_per_newton_step'],
-        "speedup_mixed_vs_fp64": res['fp64']['s_per_newton_step'] / mixed['s_per_newton_step'],
-        "rel_err_mixed_vs_fp64_ref": mixed['rel_err'], "rel_err_fp64_vs_fp64_ref": res['fp64']['rel_err'],
-        "s_per_adjoint_gradient": {"mixed": mixed['s_per_adjoint_gradient'],
+        "speedup_mixed_vs_fp64": res['fp64']['s_per_newton_step'] / head['s_per_newton_step'],
+        "speedup_note": "value path (AMG-CG, FP32 hierarchy) vs the plain FP64 path (Jacobi-PCG, FP64)",
+        "paths_s_per_newton_step": {k: v['s_per_newton_step'] for k, v in res.items()},
+        "paths_s_per_adjoint_gradient": {k: v['s_per_adjoint_gradient'] for k, v in res.items()},
+        "paths_iterations": {k: v['iterations'] for k, v in res.items()},
+        "rel_err_vs_fp64_ref": {k: v['rel_err'] for k, v in res.items()},
+        "adjoint_grad_rel_diff_vs_fp64": gerr,
+        "s_per_adjoint_gradient": {"mixed": head['s_per_adjoint_gradient'],
                                    "fp64": res['fp64']['s_per_adjoint_gradient']},
-        "adjoint_grad_rel_diff_mixed_vs_fp64": gerr,
-        "pcg_iterations": {"mixed": mixed['pcg_iterations'], "fp64": res['fp64']['pcg_iterations'],
-                           "mixed_replacements": mixed['replacements']},
         "roofline": {"bound": "hbm",
-                     "kernel": "k_pcg_a<3,true,float> (SELL-64 3x3-block SpMV q = Ap, FP32 storage, FP64 "
-                               "arithmetic, block partials of <p,Ap>)",
+                     "kernel": "k_sell_apply<3,CHEB,float> (level-0 Chebyshev step of the V-cycle: SELL-64 3x3-block "
+                               "product on FP32 storage with FP64 arithmetic, fused D^-1 residual and update)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
-                     "traffic": traffic.get('k_pcg_a_f32'),
-                     "traffic_source": "profiles/k_pcg_a_hbm_traffic_cantilever_128x64x64.md" if traffic else None,
-                     "algorithmic_bytes": spmv_bytes, "avg_launch_ms": mixed['spmv_ms']},
+                     "traffic": amg_traffic.get('k_sell_apply_cheb_f32'),
+                     "traffic_source": os.path.relpath(AMG_TRAFFIC_PROFILE, ROOT) if amg_traffic else None,
+                     "algorithmic_bytes": cb, "avg_launch_ms": head['spmv_ms']},
+        "pcg_spmv": {"kernel": "k_pcg_a<3,true,float> (Jacobi-PCG SpMV q = Ap, FP32 storage)",
+                     "achieved_GBs": pcg_achieved, "frac": pcg_achieved / PEAK_HBM_GBS if pcg_achieved else None,
+                     "avg_launch_ms": res['mixed']['spmv_ms'], "algorithmic_bytes": spmv_bytes,
+                     "traffic": traffic.get('k_pcg_a_f32')},
         "fp64_spmv": {"kernel": "k_pcg_a<3,false,double>", "achieved_GBs": fp64_achieved,
                       "frac": fp64_achieved / PEAK_HBM_GBS if fp64_achieved else None,
                       "avg_launch_ms": res['fp64']['spmv_ms'], "algorithmic_bytes": fp64_bytes,
@@ -225,7 +254,7 @@ def run(args):
         "detail": res,
     }
     if args.cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(dev, res['fp64']['pcg_iterations'])
+        out["cpu_baseline"] = cpu_baseline(dev, res['fp64']['iterations'])
     print(json.dumps(out), flush=True)
 
 
@@ -313,8 +342,8 @@ def _opt(o, **kw):
 
 
 def _evented(o):
-    """Options of the extra solve that times the Jacobi-PCG kernels with HIP events."""
-    return _opt(o, timing=True) if 'jax_solver' in o else o
+    """Options of the extra solve that times the dominant kernels with HIP events."""
+    return _opt(o, timing=True)
 
 
 def workload_poisson(args):
@@ -677,7 +706,7 @@ def main():
     ap.add_argument('--solver', choices=['pcg', 'amg'], default='pcg',
                     help="pcg: Jacobi-PCG ('jax_solver'); amg: smoothed-aggregation AMG-CG ('amgx_solver')")
     ap.add_argument('--no-cpu-baseline', dest='cpu_baseline', action='store_false')
-    ap.add_argument('--only', choices=['fp64', 'mixed'], default=None,
+    ap.add_argument('--only', choices=['fp64', 'mixed', 'amg_fp64', 'amg_mixed'], default=None,
                     help='time one path only (no adjoint, no summary line); for profiling')
     ap.add_argument('--nh-cells', type=int, default=64)
     ap.add_argument('--simp-cells', type=int, nargs=3, default=[160, 80, 40])

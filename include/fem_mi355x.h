@@ -343,6 +343,11 @@ typedef struct {
   double  atol;
   int32_t maxiter;
   int32_t check_every;           /* iterations between host status reads      */
+  int32_t timing;                /* 1: at every status read, time the last
+                                    V-cycle and its first level-0 Chebyshev
+                                    SELL kernel with HIP events: result
+                                    spmv_ms / update_ms = their means         */
+  int32_t pad;
 } fem_amg_options_t;
 
 /* Workspace bytes of fem_amg_aggregate on an n-node graph. */

@@ -853,7 +853,8 @@ __device__ inline void apply_epilogue(const ApplyArgs& a, int64_t idx, double ax
   }
 }
 
-// Block CSR operator, G lanes per block row.
+// Block CSR operator, G lanes per block row: the lanes stride over the row's
+// (block, column) pairs and accumulate every block row entry.
 template <int BR, int BC, int G, int MODE, typename T>
 __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t* __restrict__ rp,
                                                     const int32_t* __restrict__ ci, const T* __restrict__ v,
@@ -893,7 +894,9 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   __shared__ double red[W][kSlice][V];
   if (done && *done) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t s = blockIdx.x;
+  // XCD-contiguous slice order: neighbouring slices (which gather the same
+  // vector entries) run on one XCD and share its L2
+  const int64_t s = xcd_logical_block(blockIdx.x, gridDim.x);
   const int64_t row = pat.sell_row[s * kSlice + lane];
   const bool real = row >= 0;
   auto xat = [&](int64_t c, double (&xc)[V]) {
@@ -903,6 +906,20 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   double xr[V], acc[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) xr[j] = 0.0;
+  // the epilogue's own-row operands are loaded before the matrix stream, so
+  // their latency overlaps it (wave 0 finishes the rows)
+  constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
+  double e_rhs[V], e_dinv[V], e_d[V], e_x[V];
+  if (w == 0 && real) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const int64_t idx = row * V + i;
+      e_rhs[i] = kRhs ? a.rhs[idx] : 0.0;
+      e_dinv[i] = (MODE == AP_CHEB || MODE == AP_DSTORE) ? a.dinv[idx] : 0.0;
+      e_d[i] = (MODE == AP_CHEB && a.c1 != 0.0) ? a.d[idx] : 0.0;
+      e_x[i] = (MODE == AP_CHEB || MODE == AP_PQ) ? a.x[idx] : 0.0;
+    }
+  }
   sell_slice_partial<V, false, W, U, T, decltype(xat), double>(pat, vals, s, w, lane, row, xat, xr, acc);
   if (w > 0) {
 #pragma unroll
@@ -917,8 +934,20 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       double t = acc[i];
 #pragma unroll
       for (int o = 1; o < W; ++o) t += red[o][lane][i];
-      apply_epilogue<MODE>(a, row * V + i, t);
-      if (MODE == AP_PQ) pq = fma(a.x[row * V + i], t, pq);
+      const int64_t idx = row * V + i;
+      if (MODE == AP_CHEB) {
+        const double r = e_dinv[i] * (e_rhs[i] - t);
+        const double dn = a.c1 != 0.0 ? fma(a.c1, e_d[i], a.c2 * r) : a.c2 * r;
+        a.d[idx] = dn;
+        a.y[idx] = e_x[i] + dn;
+      } else if (MODE == AP_RESID) {
+        a.y[idx] = e_rhs[i] - t;
+      } else if (MODE == AP_DSTORE) {
+        a.y[idx] = e_dinv[i] * t;
+      } else {
+        apply_epilogue<MODE>(a, idx, t);
+      }
+      if (MODE == AP_PQ) pq = fma(e_x[i], t, pq);
     }
   }
   if (MODE == AP_PQ) {
@@ -1067,9 +1096,9 @@ __global__ __launch_bounds__(256) void k_cg_rr(const double* __restrict__ part, 
 template <int MODE, typename T>
 int bcsr_apply(const fem_bcsr_t& M, const ApplyArgs& a, hipStream_t s) {
   if (M.rows == 0) return FEM_OK;
-  // lanes per block row: about 3 (block, column) items per lane
+  // lanes per block row from the mean number of (block, column) pairs in a row
   const double per_row = M.rows ? double(M.nnzb) * M.bc / double(M.rows) : 0.0;
-  const int G = per_row > 96.0 ? 64 : (per_row > 24.0 ? 16 : 8);
+  const int G = per_row > 32.0 ? 64 : (per_row > 6.0 ? 16 : 8);
   const unsigned grid = (unsigned)ceil_div(M.rows * G, 256);
   const T* v = static_cast<const T*>(M.vals);
 #define FEM_AMG_APPLY(BR, BC)                                                                                  \
@@ -1140,8 +1169,12 @@ int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgs& a, hipSt
   return bcsr_apply<AP_RESID, T>(L.A, a, s);
 }
 
+// ev (may be NULL): events recorded around the whole cycle (ev[2], ev[3]) and
+// around its first level-0 post-smoothing Chebyshev kernel (ev[0], ev[1]).
 template <typename T>
-int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipStream_t s) {
+int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipStream_t s,
+                hipEvent_t* ev = nullptr) {
+  if (ev) FEM_HIP(hipEventRecord(ev[2], s));
   const int NL = amg.num_levels;
   std::vector<double*> rhs(NL + 1), xa(NL), xb(NL), dd(NL), res(NL), cur(NL + 1);
   size_t off = 0;
@@ -1209,12 +1242,16 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
       }
       double* out = (l == 0 && k == deg - 1) ? z : other;
       ApplyArgs a{x, out, rhs[l], L.dinv, dd[l], c1, c2, nullptr};
+      const bool timed = ev && l == 0 && k == 0;
+      if (timed) FEM_HIP(hipEventRecord(ev[0], s));
       if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
+      if (timed) FEM_HIP(hipEventRecord(ev[1], s));
       other = x;
       x = out;
     }
     cur[l] = x;
   }
+  if (ev) FEM_HIP(hipEventRecord(ev[3], s));
   return FEM_OK;
 }
 
@@ -1746,10 +1783,23 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_am
   FEM_LAUNCHED();
   AmgScalars h{};
   if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &h)) return rc;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 4; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  if (opt->timing)
+    for (int i = 0; i < 4; ++i) FEM_HIP(hipEventCreate(&ev[i]));
+  double cheb_ms = 0.0, cycle_ms = 0.0;
+  int samples = 0;
   int it = 0;
   while (!h.done) {
-    if (int rc = (amg->precision == FEM_F32 ? vcycle_impl<float>(*amg, r, z, cyc, s)
-                                            : vcycle_impl<double>(*amg, r, z, cyc, s)))
+    hipEvent_t* e = opt->timing ? ev : nullptr;
+    if (int rc = (amg->precision == FEM_F32 ? vcycle_impl<float>(*amg, r, z, cyc, s, e)
+                                            : vcycle_impl<double>(*amg, r, z, cyc, s, e)))
       return rc;
     hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, r, z, part, done);
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, part, kDotBlocks, sc);
@@ -1764,8 +1814,21 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_am
     ++it;
     if (it % opt->check_every == 0 || it >= opt->maxiter) {
       if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &h)) return rc;
+      if (opt->timing) {
+        float a = 0.f, b = 0.f;
+        FEM_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        FEM_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
+        cheb_ms += a;
+        cycle_ms += b;
+        ++samples;
+      }
       if (it >= opt->maxiter && !h.done) break;
     }
+  }
+  if (samples) {
+    result->spmv_ms = cheb_ms / samples;
+    result->update_ms = cycle_ms / samples;
+    result->timed_launches = samples;
   }
   // true residual
   ApplyArgs af{x, q, b, nullptr, nullptr, 0.0, 0.0, nullptr};

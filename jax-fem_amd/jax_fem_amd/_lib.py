@@ -74,7 +74,8 @@ class FemAmg(C.Structure):
 
 
 class FemAmgOptions(C.Structure):
-    _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32)]
+    _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32), ('timing', _i32),
+                ('pad', _i32)]
 
 
 _P = C.POINTER

@@ -295,6 +295,7 @@ class DeviceAMG:
         dev = self.dev
         o = _lib.FemAmgOptions()
         o.rtol, o.atol, o.maxiter, o.check_every = cfg['tol'], cfg['atol'], cfg['maxiter'], cfg['check_every']
+        o.timing = 1 if cfg.get('timing', False) else 0
         nb = int(self.L.fem_amg_solve_ws_bytes(dev.pref, C.byref(self.amg)))
         key = ('amg_ws', nb)
         if dev.buffers.get('amg_ws_key') != key:

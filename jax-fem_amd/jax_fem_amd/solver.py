@@ -39,7 +39,7 @@ _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32
 # FP32.  From an AMGX config file only solver.tolerance and solver.max_iters are
 # taken (as atol and maxiter, AMGX's ABSOLUTE convergence).
 AMG_SOLVER_DEFAULTS = dict(cfg_path=None, precision='fp64', tol=1e-10, atol=1e-10, maxiter=1000, check_every=1,
-                           degree=2)
+                           degree=2, timing=False)
 
 
 def _resolve_solver_options(solver_options):
