@@ -396,8 +396,8 @@ int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const double* se
  *             block's products in a fixed (sorted) order
  *   numeric   C->vals from X->vals and Y->vals along the plan (one thread per
  *             output block, products summed in plan order: bit-reproducible)
- * Rows of more than 16384 products, or left-operand rows of more than 2048
- * blocks, return FEM_ERR_CAPACITY.  ws >= 256 bytes of device memory.
+ * Rows of 2^18 products or more, of more than 2048 output blocks, or
+ * left-operand rows of more than 2048 blocks return FEM_ERR_CAPACITY.  ws >= 256 bytes of device memory.
  * Symbolic and plan synchronise. */
 int fem_bcsr_spgemm_symbolic(const fem_bcsr_t* X, const fem_bcsr_t* Y, int32_t* row_counts, int32_t* row_pairs,
                              int64_t* max_pairs, void* ws, void* host_status, void* stream);
@@ -406,10 +406,12 @@ int fem_bcsr_spgemm_plan(const fem_bcsr_t* X, const fem_bcsr_t* Y, int64_t max_p
                          void* stream);
 int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t* C, const int32_t* plan_ptr,
                             const int32_t* plan, void* stream);
-/* Smoothed prolongator in place: W holds A P0 on the pattern of P; becomes
- * P = P0 - omega D^-1 A P0 (P0 as from fem_amg_tentative, columns agg). */
-int fem_amg_smooth_prolongator(fem_bcsr_t* W, const double* P0, const int32_t* agg, const double* dinv,
-                               double omega, void* stream);
+/* Smoothed prolongator P = P0 - omega D^-1 A P0 (FP64): P0 is the tentative
+ * prolongator as block CSR (one block per row, columns agg), P has the
+ * structure and (plan_ptr, plan) the plan of the product A P0. */
+int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P, const int32_t* plan_ptr,
+                               const int32_t* plan, const int32_t* agg, const double* dinv, double omega,
+                               void* stream);
 /* T = X^T (block CSR; T->rows = X->cols, blocks bc x br).  Structure:
  * T->row_ptr, T->col_idx (ascending) and src (nnzb: the X entry of each T
  * entry); ws >= 4*X->cols rounded up to 256, + fem_scan_ws_bytes(X->cols).

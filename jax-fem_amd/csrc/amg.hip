@@ -311,29 +311,6 @@ __global__ void k_bcsr_dinv(int64_t rows, const int32_t* __restrict__ rp, const 
   }
 }
 
-template <int BS, int NS>
-__global__ void k_smooth_p(int64_t rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                           double* __restrict__ v, const double* __restrict__ P0, const int32_t* __restrict__ agg,
-                           const double* __restrict__ dinv, double omega) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
-    const int a = agg[i];
-    for (int k = rp[i]; k < rp[i + 1]; ++k) {
-      const bool own = ci[k] == a;
-#pragma unroll
-      for (int r = 0; r < BS; ++r) {
-        const double s = -omega * dinv[i * BS + r];
-#pragma unroll
-        for (int j = 0; j < NS; ++j) {
-          double& w = v[((size_t)k * BS + r) * NS + j];
-          double p = s * w;
-          if (own) p += P0[(i * BS + r) * NS + j];
-          w = p;
-        }
-      }
-    }
-  }
-}
-
 // ------------------------------------------------------------ transpose
 __global__ void k_count_cols(int64_t nnzb, const int32_t* __restrict__ ci, int32_t* __restrict__ cnt) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < nnzb; e += (int64_t)gridDim.x * blockDim.x)
@@ -525,12 +502,16 @@ __global__ __launch_bounds__(NT) void k_spgemm_row(BView X, BView Y, int32_t* __
   }
 }
 
-// Rows with many products (up to kBigCap): the output columns are first
-// collected in an LDS hash table and ranked (sorted), so a product's sort key
-// fits 32 bits (rank << 18 | p); otherwise as k_spgemm_row.
-constexpr int kBigCap = 16384;   // products per row
-constexpr int kBigHash = 4096;   // hash slots (output columns per row <= kBigHash / 2)
+// Rows with many products: the output columns are first collected in an LDS
+// hash table and ranked (sorted), so a product's sort key fits 32 bits
+// (rank << 18 | p).  The products are then placed chunk by chunk (kBigCap at
+// a time, each chunk bitonic-sorted): an output block's products are in
+// (chunk, p) order, a fixed order.  Limits: 2^18 products and kBigHash / 2
+// output blocks per row, kBigXCap blocks per row of X.
+constexpr int kBigCap = 16384;   // products sorted at a time
+constexpr int kBigHash = 4096;   // hash slots
 constexpr int kBigXCap = 2048;   // blocks in one row of X
+constexpr int kBigPairs = 1 << 18;
 
 __device__ inline int hash_slot(int col) { return (int)(((uint32_t)col * 2654435761u) >> 20); }  // 12 bits
 
@@ -541,8 +522,9 @@ __global__ __launch_bounds__(NT) void k_spgemm_row_big(BView X, BView Y, int32_t
   __shared__ uint32_t keys[kBigCap];
   __shared__ int32_t tab[kBigHash];
   __shared__ int32_t rank[kBigHash];
-  __shared__ int32_t ucol[kBigHash / 2];
+  __shared__ int32_t ucol[kBigHash / 2];  // distinct columns; reused as running offsets
   __shared__ int32_t seg[kBigHash / 2 + 1];
+  __shared__ int32_t cs[kBigHash / 2];    // segment start of a rank within the current chunk
   __shared__ int32_t pref[kBigXCap + 1];
   __shared__ int32_t part[NT];
   __shared__ int32_t flag;
@@ -565,13 +547,18 @@ __global__ __launch_bounds__(NT) void k_spgemm_row_big(BView X, BView Y, int32_t
   __syncthreads();
   lds_inclusive_scan<NT>(pref + 1, xl, part);
   const int np = pref[xl];
-  if (np > kBigCap) {
+  if (np >= kBigPairs) {
     if (t == 0) atomicOr(err, 2);
     return;
   }
   auto pair_col = [&](int p) -> int {
     const int e = pair_row(pref, xl, p);
     return Y.ci[Y.rp[X.ci[xb + e]] + (p - pref[e])];
+  };
+  auto find = [&](int c) -> int {  // slot of a column present in the table
+    int h = hash_slot(c);
+    while (tab[h] != c) h = (h + 1) & (kBigHash - 1);
+    return h;
   };
   // 1. distinct output columns into the hash table (bounded probing)
   for (int p = t; p < np; p += NT) {
@@ -628,54 +615,73 @@ __global__ __launch_bounds__(NT) void k_spgemm_row_big(BView X, BView Y, int32_t
     if (t == 0) atomicOr(err, 4);
     return;
   }
-  // 3. rank of every occupied slot
+  // 3. rank of every occupied slot; products per rank
   for (int u = t; u < U; u += NT) {
     const int c = ucol[u];
-    int h = hash_slot(c);
-    while (tab[h] != c) h = (h + 1) & (kBigHash - 1);  // present: the probe ends
-    rank[h] = u;
+    rank[find(c)] = u;
     o.cci[cb + u] = c;
+    seg[u] = 0;
   }
   __syncthreads();
-  // 4. sort the products by (rank, p)
-  int P2 = 1;
-  while (P2 < np) P2 <<= 1;
-  for (int p = t; p < P2; p += NT) {
-    uint32_t k = 0xFFFFFFFFu;
-    if (p < np) {
-      const int c = pair_col(p);
-      int h = hash_slot(c);
-      while (tab[h] != c) h = (h + 1) & (kBigHash - 1);
-      k = ((uint32_t)rank[h] << 18) | (uint32_t)p;
+  for (int p = t; p < np; p += NT) atomicAdd(&seg[rank[find(pair_col(p))]], 1);
+  __syncthreads();
+  lds_inclusive_scan<NT>(seg, U, part);  // seg[u] = products of ranks <= u
+  const int64_t pb = o.pair_ptr[r];
+  for (int u = t; u < U; u += NT) {
+    const int base = u == 0 ? 0 : seg[u - 1];
+    o.plan_ptr[cb + u] = (int32_t)(pb + base);
+    cs[u] = base;  // first free position of rank u
+  }
+  if (t == 0) o.plan_ptr[cb + U] = o.pair_ptr[r + 1];
+  __syncthreads();
+  for (int u = t; u < U; u += NT) seg[u] = cs[u];  // seg[u]: next position of rank u
+  __syncthreads();
+  // 4. place the products chunk by chunk
+  for (int c0 = 0; c0 < np; c0 += kBigCap) {
+    const int m = min(kBigCap, np - c0);
+    int P2 = 1;
+    while (P2 < m) P2 <<= 1;
+    for (int q = t; q < P2; q += NT) {
+      uint32_t k = 0xFFFFFFFFu;
+      if (q < m) {
+        const int p = c0 + q;
+        k = ((uint32_t)rank[find(pair_col(p))] << 18) | (uint32_t)p;
+      }
+      keys[q] = k;
     }
-    keys[p] = k;
-  }
-  __syncthreads();
-  for (int k = 2; k <= P2; k <<= 1)
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = t; i < P2; i += NT) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const uint32_t a = keys[i], b = keys[ixj];
-          if ((a > b) == ((i & k) == 0)) {
-            keys[i] = b;
-            keys[ixj] = a;
+    __syncthreads();
+    for (int k = 2; k <= P2; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = t; i < P2; i += NT) {
+          const int ixj = i ^ j;
+          if (ixj > i) {
+            const uint32_t a = keys[i], b = keys[ixj];
+            if ((a > b) == ((i & k) == 0)) {
+              keys[i] = b;
+              keys[ixj] = a;
+            }
           }
         }
+        __syncthreads();
       }
-      __syncthreads();
+    for (int q = t; q < m; q += NT)
+      if (q == 0 || (keys[q] >> 18) != (keys[q - 1] >> 18)) cs[keys[q] >> 18] = q;
+    __syncthreads();
+    for (int q = t; q < m; q += NT) {
+      const int u = (int)(keys[q] >> 18);
+      const int pp = (int)(keys[q] & 0x3FFFFu);
+      const int e = pair_row(pref, xl, pp);
+      const int64_t pos = pb + seg[u] + (q - cs[u]);
+      o.plan[2 * pos] = xb + e;
+      o.plan[2 * pos + 1] = Y.rp[X.ci[xb + e]] + (pp - pref[e]);
     }
-  for (int p = t; p < np; p += NT)
-    if (p == 0 || (keys[p] >> 18) != (keys[p - 1] >> 18)) seg[keys[p] >> 18] = p;
-  __syncthreads();
-  const int64_t pb = o.pair_ptr[r];
-  for (int u = t; u < U; u += NT) o.plan_ptr[cb + u] = (int32_t)(pb + seg[u]);
-  if (t == 0) o.plan_ptr[cb + U] = o.pair_ptr[r + 1];
-  for (int q = t; q < np; q += NT) {
-    const int pp = (int)(keys[q] & 0x3FFFFu);
-    const int e = pair_row(pref, xl, pp);
-    o.plan[2 * (pb + q)] = xb + e;
-    o.plan[2 * (pb + q) + 1] = Y.rp[X.ci[xb + e]] + (pp - pref[e]);
+    __syncthreads();
+    for (int q = t; q < m; q += NT)
+      if (q == m - 1 || (keys[q] >> 18) != (keys[q + 1] >> 18)) {
+        const int u = (int)(keys[q] >> 18);
+        seg[u] += q - cs[u] + 1;
+      }
+    __syncthreads();
   }
 }
 
@@ -714,6 +720,58 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_
     for (int i = 0; i < XR; ++i)
 #pragma unroll
       for (int j = 0; j < YC; ++j) c[i * YC + j] = acc[i][j];
+  }
+}
+
+// P = P0 - omega D^-1 (A P0) along the plan of A P0 (Y = P0 as block CSR, one
+// block per row at column agg): one thread per block of P.
+template <int BS, int NS>
+__global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows, const int32_t* __restrict__ crp,
+                                                        const int32_t* __restrict__ cci,
+                                                        const int32_t* __restrict__ plan_ptr,
+                                                        const int32_t* __restrict__ plan, const double* __restrict__ av,
+                                                        const double* __restrict__ p0, const int32_t* __restrict__ agg,
+                                                        const double* __restrict__ dinv, double omega,
+                                                        double* __restrict__ pv) {
+  for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = rows;  // row i: crp[i] <= b < crp[i+1]
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (crp[mid] <= b)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int64_t i = lo;
+    double acc[BS][NS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+      for (int j = 0; j < NS; ++j) acc[r][j] = 0.0;
+    for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
+      const double* x = av + (size_t)plan[2 * (int64_t)q] * BS * BS;
+      const double* y = p0 + (size_t)plan[2 * (int64_t)q + 1] * BS * NS;
+#pragma unroll
+      for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int m = 0; m < BS; ++m) {
+          const double xm = x[r * BS + m];
+#pragma unroll
+          for (int j = 0; j < NS; ++j) acc[r][j] = fma(xm, y[m * NS + j], acc[r][j]);
+        }
+    }
+    const bool own = cci[b] == agg[i];
+    double* out = pv + (size_t)b * BS * NS;
+#pragma unroll
+    for (int r = 0; r < BS; ++r) {
+      const double sc = -omega * dinv[i * BS + r];
+#pragma unroll
+      for (int j = 0; j < NS; ++j) {
+        double v = sc * acc[r][j];
+        if (own) v += p0[(i * BS + r) * NS + j];
+        out[r * NS + j] = v;
+      }
+    }
   }
 }
 
@@ -1293,10 +1351,10 @@ int spgemm_dispatch(int64_t max_pairs, const fem_bcsr_t& X, const fem_bcsr_t& Y,
     hipLaunchKernelGGL((k_spgemm_row<64, 256, FILL>), grid, dim3(64), 0, s, xv, yv, counts, o, err);
   else if (max_pairs <= 1024)
     hipLaunchKernelGGL((k_spgemm_row<256, 1024, FILL>), grid, dim3(256), 0, s, xv, yv, counts, o, err);
-  else if (max_pairs <= kBigCap)
+  else if (max_pairs < kBigPairs)
     hipLaunchKernelGGL((k_spgemm_row_big<256, FILL>), grid, dim3(256), 0, s, xv, yv, counts, o, err);
   else {
-    set_error("block SpGEMM: a row has %lld block products (capacity %d)", (long long)max_pairs, kBigCap);
+    set_error("block SpGEMM: a row has %lld block products (capacity %d)", (long long)max_pairs, kBigPairs - 1);
     return FEM_ERR_CAPACITY;
   }
   FEM_LAUNCHED();
@@ -1615,25 +1673,32 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
   return FEM_ERR_BAD_ARG;
 }
 
-int fem_amg_smooth_prolongator(fem_bcsr_t* W, const double* P0, const int32_t* agg, const double* dinv,
-                               double omega, void* stream) {
-  FEM_CHECK_ARG(W && P0 && agg && dinv, "fem_amg_smooth_prolongator: NULL argument");
+int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P, const int32_t* plan_ptr,
+                               const int32_t* plan, const int32_t* agg, const double* dinv, double omega,
+                               void* stream) {
+  FEM_CHECK_ARG(A && P0 && P && plan_ptr && agg && dinv, "fem_amg_smooth_prolongator: NULL argument");
+  FEM_CHECK_ARG(A->br == A->bc && P0->br == A->bc && P->rows == A->rows && P->br == P0->br && P->bc == P0->bc &&
+                    P0->nnzb == P0->rows,
+                "fem_amg_smooth_prolongator: shapes do not match");
   hipStream_t s = as_stream(stream);
-  if (W->rows == 0) return FEM_OK;
-  const unsigned g = grid_for(W->rows, 256);
-  double* v = static_cast<double*>(const_cast<void*>(W->vals));
-#define FEM_AMG_SMOOTH(BS, NS)                                                                               \
-  if (W->br == BS && W->bc == NS) {                                                                          \
-    hipLaunchKernelGGL((k_smooth_p<BS, NS>), dim3(g), dim3(256), 0, s, W->rows, W->row_ptr, W->col_idx, v, P0, \
-                       agg, dinv, omega);                                                                    \
-    FEM_LAUNCHED();                                                                                          \
-    return FEM_OK;                                                                                           \
+  if (P->nnzb == 0) return FEM_OK;
+  FEM_CHECK_ARG(plan && A->vals && P0->vals && P->vals && P->col_idx, "fem_amg_smooth_prolongator: arrays missing");
+  const unsigned g = grid_for(P->nnzb, 256);
+  const double* av = static_cast<const double*>(A->vals);
+  const double* p0 = static_cast<const double*>(P0->vals);
+  double* pv = static_cast<double*>(const_cast<void*>(P->vals));
+#define FEM_AMG_SMOOTH(BS, NS)                                                                                  \
+  if (P->br == BS && P->bc == NS) {                                                                             \
+    hipLaunchKernelGGL((k_smooth_numeric<BS, NS>), dim3(g), dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,         \
+                       P->col_idx, plan_ptr, plan, av, p0, agg, dinv, omega, pv);                               \
+    FEM_LAUNCHED();                                                                                             \
+    return FEM_OK;                                                                                              \
   }
   FEM_AMG_SMOOTH(3, 6)
   FEM_AMG_SMOOTH(6, 6)
   FEM_AMG_SMOOTH(1, 1)
 #undef FEM_AMG_SMOOTH
-  set_error("fem_amg_smooth_prolongator: unsupported block shape %d x %d", W->br, W->bc);
+  set_error("fem_amg_smooth_prolongator: unsupported block shape %d x %d", P->br, P->bc);
   return FEM_ERR_BAD_ARG;
 }
 

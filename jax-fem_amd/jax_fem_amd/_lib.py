@@ -125,7 +125,8 @@ _SIGNATURES = {
     'fem_bcsr_spgemm_plan': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _i64, _P(FemBcsr), _vp, _vp, _vp, _vp, _vp,
                                        _vp]),
     'fem_bcsr_spgemm_numeric': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp]),
-    'fem_amg_smooth_prolongator': (C.c_int, [_P(FemBcsr), _vp, _vp, _vp, _f64, _vp]),
+    'fem_amg_smooth_prolongator': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp, _vp, _f64,
+                                             _vp]),
     'fem_bcsr_transpose_structure': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp]),
     'fem_bcsr_transpose_values': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp]),
     'fem_amg_coarse_inverse': (C.c_int, [_P(FemBcsr), _vp, _vp, _vp]),

@@ -226,9 +226,9 @@ class DeviceAMG:
             omega = 4.0 / (3.0 * lv['lam'])
             plan_w0, plan_w, plan_c = lv['plans']
             P = lv['P']
-            self._spgemm_numeric(A, lv['P0m'], P, plan_w0)
-            ps = P.struct()
-            check(L.fem_amg_smooth_prolongator(C.byref(ps), ptr(lv['P0']), ptr(lv['agg']), ptr(lv['dinv']), omega,
+            as_, p0s, ps = A.struct(), lv['P0m'].struct(), P.struct()
+            check(L.fem_amg_smooth_prolongator(C.byref(as_), C.byref(p0s), C.byref(ps), ptr(plan_w0[0]),
+                                               ptr(plan_w0[1]), ptr(lv['agg']), ptr(lv['dinv']), omega,
                                                stream_ptr()), 'fem_amg_smooth_prolongator')
             self._transpose_values(P, lv['R'], lv['R_src'])
             self._spgemm_numeric(A, P, lv['W'], plan_w)

@@ -159,3 +159,21 @@ def test_amg_adjoint_matches_jacobi_pcg(gpu):
         c.backward()
         grads.append(th.grad.clone())
     assert torch.linalg.norm(grads[1] - grads[0]) <= 1e-8 * torch.linalg.norm(grads[0])
+
+
+def test_amg_hex27_matches_jacobi_pcg(gpu):
+    # 16^3 HEX27 cells: rows of R (A P) have more than 16384 block products, so the
+    # product plans are built in several sorted chunks
+    from jax_fem_amd.generate_mesh import Mesh, box_mesh_hex27
+    m = box_mesh_hex27(16, 16, 16, 1., 1., 1.)
+    clamp = [[lambda x: isclose(x[2], 0.)] * 3, [0, 1, 2], [lambda x: 0.] * 3]
+    p = LoadedElastic(Mesh(m.points, m.cells_dict['hexahedron27']), vec=3, dim=3, ele_type='HEX27',
+                      quadrature_order=4, dirichlet_bc_info=clamp, location_fns=[lambda x: isclose(x[2], 1.)])
+    ref = solver(p, {'jax_solver': {'precision': 'fp64', 'tol': 1e-12, 'atol': 0.}})[0].clone()
+    its_j = p._solve_stats['linear'][0]['iterations']
+    x = solver(p, {'amgx_solver': {'precision': 'mixed', 'tol': 1e-12, 'atol': 0.}})[0]
+    its = p._solve_stats['linear'][0]['iterations']
+    assert torch.linalg.norm(x - ref) <= 1e-9 * torch.linalg.norm(ref)
+    assert its < its_j / 2, (its, its_j)
+    y = solver(p, {'amgx_solver': {'precision': 'mixed', 'tol': 1e-12, 'atol': 0.}})[0]
+    assert torch.equal(x, y)
