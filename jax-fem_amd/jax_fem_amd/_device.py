@@ -183,6 +183,7 @@ class DeviceProblem:
         self.load = None
         self.host_status = torch.empty(128, dtype=torch.uint8).pin_memory()
         self._amg = None
+        self._amg_uses = 0
 
     # ------------------------------------------------------------ pattern
     def _build_pattern(self):
@@ -458,6 +459,10 @@ class DeviceProblem:
                                    C.byref(res), stream_ptr()), 'fem_pcg_solve')
         return x, res
 
+    def drop_amg_hierarchy(self):
+        """Make the next AMG solve rebuild its numeric hierarchy (setup_reuse restarts)."""
+        self._amg_uses = 1 << 30
+
     def amg_setup(self, cfg):
         """Smoothed-aggregation hierarchy of the current FP64 matrix."""
         precision, _, vals, _ = self.current
@@ -469,9 +474,20 @@ class DeviceProblem:
         return self._amg.setup(vals, cfg['amg_precision'], cfg['degree'])
 
     def amg_solve(self, b, x0, cfg):
+        """AMG-CG on the current FP64 matrix.  With cfg['setup_reuse'] = k > 0 the
+        hierarchy of an earlier matrix serves as the preconditioner of up to k
+        further solves (the outer CG always uses the current matrix)."""
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        amg = self.amg_setup(cfg)
+        amg = self._amg
+        reuse = (cfg.get('setup_reuse', 0) > 0 and amg is not None and amg.amg is not None
+                 and amg.precision == cfg['amg_precision'] and self._amg_uses < cfg['setup_reuse']
+                 and amg.structure_current())
+        if reuse:
+            self._amg_uses += 1
+        else:
+            amg = self.amg_setup(cfg)
+            self._amg_uses = 0
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         out = amg.solve(self.current[2], b, x0, cfg)

@@ -166,15 +166,21 @@ class DeviceAMG:
         return out
 
     # --------------------------------------------------------------- setup
+    def structure_current(self):
+        """True when the cached structure matches the problem's pattern and Dirichlet set."""
+        dev = self.dev
+        src = getattr(self, '_struct_src', None)
+        return src is not None and src[0] is dev.pat_tensors['row_ptr'] and (
+            src[1] is dev.bc_dofs or (src[1].shape == dev.bc_dofs.shape and torch.equal(src[1], dev.bc_dofs)))
+
     def _structure(self):
         """Value-independent part of the hierarchy, kept while the pattern and
         the Dirichlet set are unchanged: aggregates, tentative prolongators,
         the patterns of P, R, A P and R (A P), their product plans and the
         transpose map.  Level 0's graph is the FE block pattern."""
         dev, L = self.dev, self.L
-        src = getattr(self, '_struct_src', None)
-        if src is not None and src[0] is dev.pat_tensors['row_ptr'] and src[1] is dev.bc_dofs:
-            return self._struct
+        if self.structure_current():
+            return self._struct     # same pattern and Dirichlet set (values may differ)
         V = dev.V
         ns = 6 if V == 3 else 1
         self._mask = self._empty(dev.n, torch.uint8)

@@ -36,10 +36,11 @@ _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32
 # Keys of {'amgx_solver': {...}} (solver.py:251-259 reads only `cfg_path`, an
 # AMGX JSON file).  The device path builds a smoothed-aggregation hierarchy and
 # runs AMG-preconditioned FP64 CG; `precision` 'mixed' stores the hierarchy in
-# FP32.  From an AMGX config file only solver.tolerance and solver.max_iters are
-# taken (as atol and maxiter, AMGX's ABSOLUTE convergence).
+# FP32; `setup_reuse` = k lets one hierarchy precondition k further solves
+# (Newton steps).  From an AMGX config file only solver.tolerance and
+# solver.max_iters are taken (as atol and maxiter, AMGX's ABSOLUTE convergence).
 AMG_SOLVER_DEFAULTS = dict(cfg_path=None, precision='fp64', tol=1e-10, atol=1e-10, maxiter=1000, check_every=1,
-                           degree=2, timing=False)
+                           degree=2, timing=False, setup_reuse=0)
 
 
 def _resolve_solver_options(solver_options):
