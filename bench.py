@@ -36,6 +36,8 @@ instead (one JSON line each, same keys; not the default line):
                Gauss rule, E = exp(N(0, 0.25^2)) per cell (seed 0), nu = 0.3,
                clamped z = 0, body force (0, 0, -1); compliance gradient
                w.r.t. the E field.
+  explicit     the residual-only path: central-difference explicit steps
+               (one compute_residual each) on the default mesh, lumped mass.
 """
 import argparse
 import json
@@ -693,14 +695,72 @@ def _cpu_port_sample(o, gpu_cells, gpu_nnz, gpu_iterations, what, u=None):
                       f"x{sn:.0f} and by {gpu_iterations} FP64 GPU iterations"}
 
 
+def workload_explicit(args):
+    """The residual-only path (compute_residual, problem.py:462-475) under central-difference
+    explicit steps, as in applications/explicit_dynamics/example.py:97-112, on the 128x64x64 HEX8
+    cantilever mesh (2x1x1, E = 70e3, nu = 0.3, rho = 1, clamped x = 0, row-sum lumped mass),
+    released from u_x = 1e-3 sin(pi x / 4).  A step is one compute_residual call and the update."""
+    import numpy as np
+    import torch
+    from jax_fem_amd import materials
+    from jax_fem_amd.generate_mesh import Mesh, box_mesh
+    from jax_fem_amd.problem import Problem
+
+    class Elastic(Problem):
+        def get_tensor_map(self):
+            return materials.LinearElastic(E=70e3, nu=0.3)
+
+    nx, ny, nz = args.cells
+    m = box_mesh(nx, ny, nz, 2., 1., 1.)
+    left = lambda p: np.isclose(p[0], 0., atol=1e-5)
+    p = Elastic(Mesh(m.points, m.cells_dict['hexahedron']), vec=3, dim=3,
+                dirichlet_bc_info=[[left] * 3, [0, 1, 2], [lambda p: 0.] * 3])
+    fe = p.fes[0]
+    dev = p.device()
+    dev.prepare()
+    sv = torch.as_tensor(fe.shape_vals, device='cuda')
+    cells = torch.as_tensor(fe.cells, device='cuda').long()
+    em = torch.einsum('qi,eq->ei', sv, torch.as_tensor(fe.JxW, device='cuda'))
+    nm = torch.zeros(fe.num_total_nodes, dtype=torch.float64, device='cuda').index_add_(0, cells.reshape(-1),
+                                                                                          em.reshape(-1))
+    mass = nm[:, None].repeat(1, 3)
+    fixed = torch.zeros((fe.num_total_nodes, 3), dtype=torch.bool, device='cuda')
+    fixed.view(-1)[dev.bc_dofs_long] = True
+    lm, mu = 70e3 * 0.3 / (1.3 * 0.4), 70e3 / 2.6
+    c = (lm + 2 * mu) ** 0.5
+    dt = 0.5 * (1.0 / nz) / c
+    x = torch.as_tensor(np.asarray(fe.points)[:, 0], device='cuda')
+    u = torch.zeros((fe.num_total_nodes, 3), dtype=torch.float64, device='cuda')
+    u[:, 0] = 1e-3 * torch.sin(np.pi * x / 4.)
+
+    def acceleration(v):
+        return torch.where(fixed, 0.0, -p.compute_residual([v])[0] / mass)
+
+    up = torch.where(fixed, 0.0, u + 0.5 * dt ** 2 * acceleration(u))
+    n_steps = max(args.steps, 1) * 20
+    for it in range(args.warmup * 20 + n_steps):
+        if it == args.warmup * 20:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+        un = torch.where(fixed, 0.0, 2.0 * u - up + dt ** 2 * acceleration(u))
+        up, u = u, un
+    torch.cuda.synchronize()
+    per_step = (time.perf_counter() - t0) / n_steps
+    energy = float((0.5 * mass * ((u - up) / dt) ** 2).sum())
+    log(f"explicit: {per_step * 1e3:.3f} ms per step over {n_steps} steps, kinetic energy {energy:.3e}")
+    _emit(f"HEX8 elasticity {nx}x{ny}x{nz} cells, {dev.n} dofs, explicit central differences "
+          f"(compute_residual per step, FP64)", per_step, "f64",
+          {"steps_timed": n_steps, "dt": dt, "kinetic_energy_end": energy})
+
+
 WORKLOADS = {'poisson': workload_poisson, 'neo_hookean': workload_neo_hookean, 'simp': workload_simp,
-             'hex27': workload_hex27}
+             'hex27': workload_hex27, 'explicit': workload_explicit}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--workload', choices=['cantilever', 'poisson', 'neo_hookean', 'simp', 'hex27'],
+    ap.add_argument('--workload', choices=['cantilever', 'poisson', 'neo_hookean', 'simp', 'hex27', 'explicit'],
                     default='cantilever')
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)

@@ -377,3 +377,25 @@ def test_fixture_plasticity_solution_and_average_stress():
         avg.append((sig[..., 2, 2] * p.JxW).sum() / p.JxW.sum())
     np.testing.assert_allclose(u.reshape(-1, 3), d['sol'], atol=1e-5)
     np.testing.assert_allclose(avg, d['avg_stresses'], atol=1e-3)
+
+
+def test_explicit_waveguide_matches_the_wave_solution():
+    # applications/explicit_dynamics/example.py: the residual-only path under central differences
+    import explicit_waveguide as W
+    pts, cells = OF.box_mesh(W.NX, W.NY, W.NZ, W.LX, W.LY, W.LZ)
+    lm, mu = OF.lame(W.E, W.NU)
+    o = OF.Problem(pts, cells, 3, mat=OF.Material('elastic', lmbda=lm, mu=mu),
+                   bc_info=[W.locations(np.isclose), [0, 1, 2], [lambda p: 0.] * 3])
+    c, k, steps, dt = W.time_grid(lm, mu)
+    em = W.RHO * np.einsum('qi,eq->ei', o.sv, o.JxW)
+    nm = np.zeros(len(pts))
+    np.add.at(nm, cells.reshape(-1), em.reshape(-1))
+    mass = np.repeat(nm[:, None], 3, 1)
+    fixed = np.zeros((len(pts), 3), bool)
+    fixed.reshape(-1)[o.bc_dofs] = True
+    u0 = np.zeros((len(pts), 3))
+    u0[:, 0] = W.AMPLITUDE * np.sin(k * pts[:, 0])
+    acc = lambda u: np.where(fixed, 0.0, -o.residual(u.reshape(-1), bc=False).reshape(-1, 3) / mass)
+    hist = np.array(W.run(acc, u0, fixed, dt, steps, np))
+    speed_err, l2 = W.wave_checks(hist[:, :, 0], nm, pts[:, 0], c, k, dt)
+    assert steps == 640 and speed_err < 1e-4 and l2 < 1e-3, (steps, speed_err, l2)
