@@ -139,6 +139,107 @@ __global__ __launch_bounds__(256) void k_cell_vector(fem_mesh_t mesh, fem_materi
   }
 }
 
+// HEX8 cells with 8 quadrature points: one thread per (cell, quadrature point),
+// 8 consecutive lanes per cell, everything in registers.  Each lane forms its
+// point's Jacobian, physical gradients and P (or dP), and the per-node sums
+// over the 8 points are lane shuffles (a fixed order).  Same results as
+// k_cell_vector up to the summation order.
+template <class MAT, int MODE>
+__global__ __launch_bounds__(256) void k_cell_vector_q8(fem_mesh_t mesh, fem_material_t mat,
+                                                        const double* __restrict__ u, const double* __restrict__ v,
+                                                        double* __restrict__ re) {
+  constexpr int N = 8, Q = 8, V = MAT::V;
+  __shared__ double s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  const int q = threadIdx.x & 7;
+  const int64_t c = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);
+  const bool active = c < mesh.num_cells;
+  const int64_t cc = active ? c : 0;  // idle lanes work on cell 0 and discard it (uniform shuffles)
+  int32_t node[N];
+#pragma unroll
+  for (int a = 0; a < N; ++a) node[a] = mesh.cells[cc * N + a];
+  const double* p0 = mesh.points + (int64_t)node[0] * 3;
+  const double x00 = p0[0], x01 = p0[1], x02 = p0[2];
+  double J[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = 0.0;
+#pragma unroll
+  for (int a = 1; a < N; ++a) {
+    const double* pa = mesh.points + (int64_t)node[a] * 3;
+    const double d0 = pa[0] - x00, d1 = pa[1] - x01, d2 = pa[2] - x02;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double dn = s_dN[q][a][j];
+      J[0 * 3 + j] += d0 * dn;
+      J[1 * 3 + j] += d1 * dn;
+      J[2 * 3 + j] += d2 * dn;
+    }
+  }
+  const double det = det3(J);
+  double iJ[9];
+  inv3(J, det, iJ);
+  const double jxw = det * s_w[q];
+  double g[N][3];
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+    const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g[a][j] = e0 * iJ[0 * 3 + j] + e1 * iJ[1 * 3 + j] + e2 * iJ[2 * 3 + j];
+  }
+  double G[V][3], P[V][3];
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G[i][j] = 0.0;
+#pragma unroll
+  for (int a = 0; a < N; ++a)
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const double ua = u[(int64_t)node[a] * V + i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) G[i][j] += ua * g[a][j];
+    }
+  const auto st = MAT::template prep<double>(mat, cc * Q + q, G);
+  if (MODE == 0) {
+    MAT::stress(st, G, P);
+  } else {
+    double dG[V][3];
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dG[i][j] = 0.0;
+#pragma unroll
+    for (int a = 0; a < N; ++a)
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        const double va = v[(int64_t)node[a] * V + i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dG[i][j] += va * g[a][j];
+      }
+    MAT::jvp(st, dG, P);
+  }
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P[i][j] *= jxw;
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+    double r[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) r[i] = P[i][0] * g[a][0] + P[i][1] * g[a][1] + P[i][2] * g[a][2];
+#pragma unroll
+    for (int off = 1; off < 8; off <<= 1)
+#pragma unroll
+      for (int i = 0; i < V; ++i) r[i] += __shfl_xor(r[i], off, 64);
+    if (active && q == a) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) re[((size_t)c * N + a) * V + i] = r[i];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ adjoint
 // g[c][q] = - JxW (dP/dtheta)(grad u) : grad lam   at quad q of cell c.
 template <class Cfg, class MAT>
@@ -431,6 +532,17 @@ static int cell_vector(const fem_mesh_t* mesh, const fem_material_t* mat, const 
   FEM_CHECK_ARG(mat && u && re && (mode == 0 || v), "cell vector kernel: NULL argument");
   FEM_CHECK_ARG(!mat->has_param || mat->param, "cell vector kernel: material param missing");
   if (mesh->num_cells == 0) return FEM_OK;
+  if (mesh->nodes_per_cell == 8 && mesh->num_quads == 8)
+    return with_mat(*mat, mesh->vec, [&](auto m) -> int {
+      using MAT = decltype(m);
+      const dim3 grid((unsigned)ceil_div(mesh->num_cells, 32));
+      if (mode == 0)
+        hipLaunchKernelGGL((k_cell_vector_q8<MAT, 0>), grid, dim3(256), 0, as_stream(stream), *mesh, *mat, u, v, re);
+      else
+        hipLaunchKernelGGL((k_cell_vector_q8<MAT, 1>), grid, dim3(256), 0, as_stream(stream), *mesh, *mat, u, v, re);
+      FEM_LAUNCHED();
+      return FEM_OK;
+    });
   return with_cfg(*mesh, [&](auto cfg) -> int {
     using Cfg = decltype(cfg);
     return with_mat(*mat, mesh->vec, [&](auto m) -> int {
