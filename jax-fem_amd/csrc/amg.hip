@@ -685,6 +685,26 @@ __global__ __launch_bounds__(NT) void k_spgemm_row_big(BView X, BView Y, int32_t
   }
 }
 
+// Load an R x C FP64 block; blocks whose size is a multiple of 16 bytes (all
+// but 3x3 and 1x1, given 256-byte aligned bases) use 16-byte loads.
+template <int R, int C>
+__device__ inline void load_block(const double* __restrict__ p, double (&a)[R][C]) {
+  if constexpr ((R * C) % 2 == 0 && (R * C * 8) % 16 == 0 && R * C > 9) {
+    const double2* p2 = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int k = 0; k < R * C / 2; ++k) {
+      const double2 v = p2[k];
+      a[(2 * k) / C][(2 * k) % C] = v.x;
+      a[(2 * k + 1) / C][(2 * k + 1) % C] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+#pragma unroll
+      for (int j = 0; j < C; ++j) a[i][j] = p[i * C + j];
+  }
+}
+
 // C = X Y by its plan: one thread per output block, products in plan order.
 template <int XR, int XC, int YC>
 __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_t* __restrict__ plan_ptr,
@@ -696,18 +716,17 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_
     for (int i = 0; i < XR; ++i)
 #pragma unroll
       for (int j = 0; j < YC; ++j) acc[i][j] = 0.0;
+    // several products' loads in flight per thread (the 6x3 x 3x6 case is
+    // register-bound and runs best without)
+    constexpr int kUnroll = (XR == 6 && XC == 3) ? 1 : 4;
+#pragma unroll kUnroll
     for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
-      const double* x = xv + (size_t)plan[2 * (int64_t)q] * XR * XC;
-      const double* y = yv + (size_t)plan[2 * (int64_t)q + 1] * XC * YC;
+      const int2 ef = reinterpret_cast<const int2*>(plan)[q];
+      const double* x = xv + (size_t)ef.x * XR * XC;
+      const double* y = yv + (size_t)ef.y * XC * YC;
       double xa[XR][XC], ya[XC][YC];
-#pragma unroll
-      for (int i = 0; i < XR; ++i)
-#pragma unroll
-        for (int m = 0; m < XC; ++m) xa[i][m] = x[i * XC + m];
-#pragma unroll
-      for (int m = 0; m < XC; ++m)
-#pragma unroll
-        for (int j = 0; j < YC; ++j) ya[m][j] = y[m * YC + j];
+      load_block<XR, XC>(x, xa);
+      load_block<XC, YC>(y, ya);
 #pragma unroll
       for (int i = 0; i < XR; ++i)
 #pragma unroll
@@ -748,17 +767,18 @@ __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows
     for (int r = 0; r < BS; ++r)
 #pragma unroll
       for (int j = 0; j < NS; ++j) acc[r][j] = 0.0;
+#pragma unroll 4
     for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
-      const double* x = av + (size_t)plan[2 * (int64_t)q] * BS * BS;
-      const double* y = p0 + (size_t)plan[2 * (int64_t)q + 1] * BS * NS;
+      const int2 ef = reinterpret_cast<const int2*>(plan)[q];
+      double xa[BS][BS], ya[BS][NS];
+      load_block<BS, BS>(av + (size_t)ef.x * BS * BS, xa);
+      load_block<BS, NS>(p0 + (size_t)ef.y * BS * NS, ya);
 #pragma unroll
       for (int r = 0; r < BS; ++r)
 #pragma unroll
-        for (int m = 0; m < BS; ++m) {
-          const double xm = x[r * BS + m];
+        for (int m = 0; m < BS; ++m)
 #pragma unroll
-          for (int j = 0; j < NS; ++j) acc[r][j] = fma(xm, y[m * NS + j], acc[r][j]);
-        }
+          for (int j = 0; j < NS; ++j) acc[r][j] = fma(xa[r][m], ya[m][j], acc[r][j]);
     }
     const bool own = cci[b] == agg[i];
     double* out = pv + (size_t)b * BS * NS;

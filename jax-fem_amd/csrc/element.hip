@@ -286,6 +286,73 @@ __global__ __launch_bounds__(256) void k_cell_adjoint(fem_mesh_t mesh, fem_mater
   }
 }
 
+// HEX8 with 8 quadrature points: one thread per (cell, quadrature point),
+// registers only (the per-point gradient needs no reduction).
+template <class MAT>
+__global__ __launch_bounds__(256) void k_cell_adjoint_q8(fem_mesh_t mesh, fem_material_t mat,
+                                                         const double* __restrict__ u, const double* __restrict__ lam,
+                                                         double* __restrict__ gout) {
+  constexpr int N = 8, Q = 8, V = MAT::V;
+  __shared__ double s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  const int q = threadIdx.x & 7;
+  const int64_t c = (int64_t)blockIdx.x * 32 + (threadIdx.x >> 3);
+  if (c >= mesh.num_cells) return;  // no cross-lane work below
+  int32_t node[N];
+#pragma unroll
+  for (int a = 0; a < N; ++a) node[a] = mesh.cells[c * N + a];
+  const double* p0 = mesh.points + (int64_t)node[0] * 3;
+  double J[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = 0.0;
+#pragma unroll
+  for (int a = 1; a < N; ++a) {
+    const double* pa = mesh.points + (int64_t)node[a] * 3;
+    const double d0 = pa[0] - p0[0], d1 = pa[1] - p0[1], d2 = pa[2] - p0[2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double dn = s_dN[q][a][j];
+      J[0 * 3 + j] += d0 * dn;
+      J[1 * 3 + j] += d1 * dn;
+      J[2 * 3 + j] += d2 * dn;
+    }
+  }
+  const double det = det3(J);
+  double iJ[9];
+  inv3(J, det, iJ);
+  double G[V][3], GL[V][3], dP[V][3];
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G[i][j] = GL[i][j] = 0.0;
+#pragma unroll
+  for (int a = 0; a < N; ++a) {
+    double g[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      g[j] = s_dN[q][a][0] * iJ[0 * 3 + j] + s_dN[q][a][1] * iJ[1 * 3 + j] + s_dN[q][a][2] * iJ[2 * 3 + j];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const double ua = u[(int64_t)node[a] * V + i], la = lam[(int64_t)node[a] * V + i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        G[i][j] += ua * g[j];
+        GL[i][j] += la * g[j];
+      }
+    }
+  }
+  const auto st = MAT::template prep<double>(mat, c * Q + q, G);
+  MAT::dparam(mat, c * Q + q, st, G, dP);
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) s += dP[i][j] * GL[i][j];
+  gout[c * Q + q] = -s * det * s_w[q];
+}
+
 // ------------------------------------------------ geometry and body loads
 template <class Cfg>
 __global__ __launch_bounds__(256) void k_geometry(fem_mesh_t mesh, double* __restrict__ shape_grads,
@@ -575,6 +642,14 @@ int fem_cell_adjoint(const fem_mesh_t* mesh, const fem_material_t* mat, const do
   FEM_CHECK_ARG(mat && u && lam && g, "fem_cell_adjoint: NULL argument");
   FEM_CHECK_ARG(mat->has_param && mat->param, "fem_cell_adjoint: the material has no per-quad parameter");
   if (mesh->num_cells == 0) return FEM_OK;
+  if (mesh->nodes_per_cell == 8 && mesh->num_quads == 8)
+    return with_mat(*mat, mesh->vec, [&](auto m) -> int {
+      using MAT = decltype(m);
+      hipLaunchKernelGGL((k_cell_adjoint_q8<MAT>), dim3((unsigned)ceil_div(mesh->num_cells, 32)), dim3(256), 0,
+                         as_stream(stream), *mesh, *mat, u, lam, g);
+      FEM_LAUNCHED();
+      return FEM_OK;
+    });
   return with_cfg(*mesh, [&](auto cfg) -> int {
     using Cfg = decltype(cfg);
     return with_mat(*mat, mesh->vec, [&](auto m) -> int {
