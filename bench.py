@@ -330,7 +330,7 @@ def _paths(args):
     if args.solver == 'amg':
         r = args.setup_reuse
         return {'fp64': {'amgx_solver': {'precision': 'fp64', 'setup_reuse': r}},
-                'mixed': {'amgx_solver': {'precision': 'mixed', 'setup_reuse': r}}}
+                'mixed': {'amgx_solver': {'precision': 'mixed', 'setup_reuse': r, 'outer': args.amg_outer}}}
     return {'fp64': {'jax_solver': {'precision': 'fp64'}},
             'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}}}
 
@@ -766,6 +766,9 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--cells', type=int, nargs=3, default=[128, 64, 64])
     ap.add_argument('--mixed-assembly', choices=['fp32', 'fp64', 'bf16x3'], default='fp64')
+    ap.add_argument('--amg-outer', choices=['fp64', 'fp32'], default='fp64',
+                    help="with --solver amg: operator of the mixed path's outer CG (fp32: difference form + "
+                         "FP64 residual replacement)")
     ap.add_argument('--setup-reuse', type=int, default=0,
                     help="with --solver amg: solves that reuse one AMG hierarchy (Newton sequences)")
     ap.add_argument('--solver', choices=['pcg', 'amg'], default='pcg',

@@ -332,7 +332,8 @@ typedef struct {
                                     SELL copy: FEM_F32 or FEM_F64; vectors and
                                     arithmetic are FP64                       */
   int32_t        degree;         /* Chebyshev degree of pre/post smoothing    */
-  int32_t        pad;
+  int32_t        level0_form;    /* form of the level-0 SELL values: FEM_FORM_
+                                    STANDARD, or FEM_FORM_DIFFERENCE (FP32)  */
   const fem_amg_level_t* levels; /* host array [num_levels]                   */
   int64_t        coarse_n;       /* unknowns of the coarsest level            */
   const double*  coarse_inv;     /* (coarse_n x coarse_n) dense inverse       */
@@ -348,6 +349,8 @@ typedef struct {
                                     SELL kernel with HIP events: result
                                     spmv_ms / update_ms = their means         */
   int32_t pad;
+  double  reliable_delta;        /* FP32 outer operator: replace the residual
+                                    when |r| fell by this factor             */
 } fem_amg_options_t;
 
 /* Workspace bytes of fem_amg_aggregate on an n-node graph. */
@@ -372,6 +375,9 @@ int fem_amg_tentative(int64_t n, int32_t bs, int32_t ns, int64_t n_agg, const in
                       double* P0, double* Bc, void* stream);
 /* Dirichlet mask: mask[d] = 1 for d in bc_dofs, else 0 (n dofs). */
 int fem_amg_bc_mask(int64_t n, int64_t n_bc, const int32_t* bc_dofs, uint8_t* mask, void* stream);
+/* FP32 difference-form copy (off-diagonal blocks rounded, diagonal slot = the
+ * FP64 row sum rounded once) of a standard-form FP64 SELL matrix. */
+int fem_amg_sell_to_f32_diff(const fem_pattern_t* pat, const double* sell_vals, float* out, void* stream);
 /* Block CSR copy (FP64) of a standard-form FP64 SELL matrix; the pattern's
  * row_ptr/col_idx are the CSR structure. */
 int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, double* bcsr_vals, void* stream);
@@ -426,14 +432,17 @@ int fem_amg_to_f32(int64_t n, const double* in, float* out, void* stream);
 /* z = M r, one V-cycle; ws >= fem_amg_ws_bytes(amg). */
 int64_t fem_amg_ws_bytes(const fem_amg_t* amg);
 int fem_amg_vcycle(const fem_amg_t* amg, const double* r, double* z, void* ws, void* stream);
-/* AMG-preconditioned CG on the row-eliminated tangent (the outer operator is
- * the FP64 standard-form SELL matrix `vals`; b, x FP64, x holds x0).  Uses
+/* AMG-preconditioned CG on the row-eliminated tangent (b, x FP64, x holds
+ * x0).  The outer operator is the FP64 standard-form SELL matrix `vals`, or,
+ * if vals32 != NULL, the FP32 difference-form copy vals32 with FP64 residual
+ * replacement through `vals` (opt->reliable_delta; convergence is decided on
+ * the FP64 true residual; result->replacements counts them).  Uses
  * ws >= fem_amg_solve_ws_bytes(pat, amg).  Synchronises.  Returns FEM_OK
  * when the solve ran; its outcome (FEM_OK / FEM_ERR_NOT_CONVERGED /
  * FEM_ERR_NAN) is result->status. */
 int64_t fem_amg_solve_ws_bytes(const fem_pattern_t* pat, const fem_amg_t* amg);
-int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_amg_t* amg, const double* b,
-                      double* x, const fem_amg_options_t* opt, void* ws, void* host_status,
+int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const float* vals32, const fem_amg_t* amg,
+                      const double* b, double* x, const fem_amg_options_t* opt, void* ws, void* host_status,
                       fem_pcg_result_t* result /* host */, void* stream);
 
 /* ---- vector helpers -------------------------------------------------- */

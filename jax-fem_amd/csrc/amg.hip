@@ -287,6 +287,36 @@ __global__ void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv,
 
 // A row without a diagonal block (not produced by the Galerkin products)
 // gets dinv = 0, i.e. no smoothing on it.
+// FP32 difference-form copy of a standard-form FP64 SELL matrix: off-diagonal
+// blocks rounded, the diagonal slot holds the FP64 row sum rounded once.
+template <int V>
+__global__ void k_sell_to_diff32(fem_pattern_t pat, const double* __restrict__ sv, float* __restrict__ out) {
+  constexpr int VV = V * V;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= pat.num_slices * kSlice) return;
+  const int64_t s = t >> 6;
+  const int lane = (int)(t & 63);
+  const int base = pat.slice_ptr[s];
+  const int L = (pat.slice_ptr[s + 1] - base) >> 6;
+  const int64_t row = pat.sell_row[t];
+  const int ds = row >= 0 ? pat.diag_slot[row] : -1;
+  const double* vp = sv + (size_t)base * VV + lane;
+  float* op = out + (size_t)base * VV + lane;
+  double rs[VV];
+#pragma unroll
+  for (int v = 0; v < VV; ++v) rs[v] = 0.0;
+  for (int k = 0; k < L; ++k)
+#pragma unroll
+    for (int v = 0; v < VV; ++v) {
+      const double x = vp[((size_t)k * VV + v) * kSlice];
+      rs[v] += x;
+      if (k != ds) op[((size_t)k * VV + v) * kSlice] = (float)x;
+    }
+  if (ds >= 0)
+#pragma unroll
+    for (int v = 0; v < VV; ++v) op[((size_t)ds * VV + v) * kSlice] = (float)rs[v];
+}
+
 template <int BS>
 __global__ void k_bcsr_dinv(int64_t rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
                             double* __restrict__ v, int fix, double* __restrict__ dinv) {
@@ -965,7 +995,9 @@ __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t*
 }
 
 // Standard-form SELL-64 operator (level 0), FP64 vectors, one block per slice.
-template <int V, int MODE, typename T>
+// DIFF: the matrix is in difference form (the diagonal slot holds the row
+// sum); the own-row entries of the input then enter every product.
+template <int V, int MODE, typename T, bool DIFF = false>
 __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pat, const T* __restrict__ vals,
                                                                 ApplyArgs a, const int32_t* __restrict__ done) {
   constexpr int W = SliceCfg<T>::W, U = SliceCfg<T>::U;
@@ -983,7 +1015,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   };
   double xr[V], acc[V];
 #pragma unroll
-  for (int j = 0; j < V; ++j) xr[j] = 0.0;
+  for (int j = 0; j < V; ++j) xr[j] = (DIFF && real) ? a.x[row * V + j] : 0.0;
   // the epilogue's own-row operands are loaded before the matrix stream, so
   // their latency overlaps it (wave 0 finishes the rows)
   constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
@@ -998,7 +1030,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       e_x[i] = (MODE == AP_CHEB || MODE == AP_PQ) ? a.x[idx] : 0.0;
     }
   }
-  sell_slice_partial<V, false, W, U, T, decltype(xat), double>(pat, vals, s, w, lane, row, xat, xr, acc);
+  sell_slice_partial<V, DIFF, W, U, T, decltype(xat), double>(pat, vals, s, w, lane, row, xat, xr, acc);
   if (w > 0) {
 #pragma unroll
     for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
@@ -1154,6 +1186,12 @@ __global__ __launch_bounds__(256) void k_cg_xr(int64_t n, const double* __restri
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// after an FP64 residual replacement (sc->rr = |b - A x|^2): the stop test
+__global__ void k_cg_replaced(AmgScalars* sc) {
+  sc->nan = sc->nan || !isfinite(sc->rr);
+  sc->done = sc->nan || sc->rr <= sc->stop2 || sc->iter >= sc->maxiter;
+}
+
 __global__ __launch_bounds__(256) void k_cg_rr(const double* __restrict__ part, int np, AmgScalars* sc) {
   __shared__ double red[4];
   if (sc->done) return;
@@ -1203,15 +1241,15 @@ int bcsr_apply(const fem_bcsr_t& M, const ApplyArgs& a, hipStream_t s) {
   return FEM_ERR_BAD_ARG;
 }
 
-template <int MODE, typename T>
+template <int MODE, typename T, bool DIFF = false>
 int sell_apply(const fem_pattern_t& pat, const void* vals, const ApplyArgs& a, const int32_t* done, hipStream_t s) {
   if (pat.num_slices == 0) return FEM_OK;
   const unsigned grid = (unsigned)pat.num_slices;
   const T* v = static_cast<const T*>(vals);
   if (pat.vec == 3)
-    hipLaunchKernelGGL((k_sell_apply<3, MODE, T>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
+    hipLaunchKernelGGL((k_sell_apply<3, MODE, T, DIFF>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
   else if (pat.vec == 1)
-    hipLaunchKernelGGL((k_sell_apply<1, MODE, T>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
+    hipLaunchKernelGGL((k_sell_apply<1, MODE, T, DIFF>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
   else
     FEM_CHECK_ARG(false, "AMG: vec must be 1 or 3");
   FEM_LAUNCHED();
@@ -1240,6 +1278,10 @@ template <typename T>
 int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgs& a, hipStream_t s) {
   const fem_amg_level_t& L = amg.levels[l];
   if (l == 0) {
+    if (amg.level0_form == FEM_FORM_DIFFERENCE) {
+      if (mode == AP_CHEB) return sell_apply<AP_CHEB, T, true>(*L.pat, L.sell_vals, a, nullptr, s);
+      return sell_apply<AP_RESID, T, true>(*L.pat, L.sell_vals, a, nullptr, s);
+    }
     if (mode == AP_CHEB) return sell_apply<AP_CHEB, T>(*L.pat, L.sell_vals, a, nullptr, s);
     return sell_apply<AP_RESID, T>(*L.pat, L.sell_vals, a, nullptr, s);
   }
@@ -1338,6 +1380,9 @@ int check_amg(const fem_amg_t* amg) {
                 "AMG hierarchy descriptor incomplete");
   FEM_CHECK_ARG(amg->precision == FEM_F32 || amg->precision == FEM_F64, "AMG precision must be FEM_F32 or FEM_F64");
   FEM_CHECK_ARG(amg->degree >= 1 && amg->degree <= 8, "AMG Chebyshev degree must be in [1, 8]");
+  FEM_CHECK_ARG(amg->level0_form == FEM_FORM_STANDARD ||
+                    (amg->level0_form == FEM_FORM_DIFFERENCE && amg->precision == FEM_F32),
+                "AMG level 0: the difference form is an FP32 storage form");
   const fem_amg_level_t& L0 = amg->levels[0];
   FEM_CHECK_ARG(L0.pat && L0.sell_vals && L0.dinv, "AMG level 0 needs the SELL pattern, values and dinv");
   for (int l = 0; l < amg->num_levels; ++l) {
@@ -1534,6 +1579,21 @@ int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, doub
     hipLaunchKernelGGL(k_sell_to_bcsr<1>, dim3(g), dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
   else
     FEM_CHECK_ARG(false, "fem_amg_sell_to_bcsr: vec must be 1 or 3");
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_amg_sell_to_f32_diff(const fem_pattern_t* pat, const double* sell_vals, float* out, void* stream) {
+  FEM_CHECK_ARG(pat && sell_vals && out, "fem_amg_sell_to_f32_diff: NULL argument");
+  hipStream_t s = as_stream(stream);
+  if (pat->num_slices == 0) return FEM_OK;
+  const unsigned g = (unsigned)ceil_div(pat->num_slices * kSlice, 256);
+  if (pat->vec == 3)
+    hipLaunchKernelGGL(k_sell_to_diff32<3>, dim3(g), dim3(256), 0, s, *pat, sell_vals, out);
+  else if (pat->vec == 1)
+    hipLaunchKernelGGL(k_sell_to_diff32<1>, dim3(g), dim3(256), 0, s, *pat, sell_vals, out);
+  else
+    FEM_CHECK_ARG(false, "fem_amg_sell_to_f32_diff: vec must be 1 or 3");
   FEM_LAUNCHED();
   return FEM_OK;
 }
@@ -1829,11 +1889,14 @@ int64_t fem_amg_solve_ws_bytes(const fem_pattern_t* pat, const fem_amg_t* amg) {
   return fem_amg_ws_bytes(amg) + (int64_t)(4 * align256(n * 8) + align256(np * 8) + align256(sizeof(AmgScalars)));
 }
 
-int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_amg_t* amg, const double* b, double* x,
-                      const fem_amg_options_t* opt, void* ws, void* host_status, fem_pcg_result_t* result,
-                      void* stream) {
+int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const float* vals32, const fem_amg_t* amg,
+                      const double* b, double* x, const fem_amg_options_t* opt, void* ws, void* host_status,
+                      fem_pcg_result_t* result, void* stream) {
   if (int rc = check_amg(amg)) return rc;
   FEM_CHECK_ARG(pat && vals && b && x && opt && ws && host_status && result, "fem_amg_pcg_solve: NULL argument");
+  const bool outer32 = vals32 != nullptr;
+  FEM_CHECK_ARG(!outer32 || (opt->reliable_delta > 0.0 && opt->reliable_delta < 1.0),
+                "fem_amg_pcg_solve: reliable_delta must be in (0, 1) with an FP32 outer operator");
   FEM_CHECK_ARG(amg->levels[0].pat->num_rows == pat->num_rows && amg->levels[0].pat->vec == pat->vec,
                 "fem_amg_pcg_solve: the hierarchy was built for another pattern");
   FEM_CHECK_ARG(opt->maxiter >= 0 && opt->check_every >= 1, "fem_amg_pcg_solve: bad options");
@@ -1856,14 +1919,19 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_am
   const int32_t* done = &sc->done;
   const unsigned g = grid_for(n, 256);
   *result = fem_pcg_result_t{};
-  // |b|^2, r = b - A x, |r|^2
+  // r = b - A x with the FP64 matrix, sc->rr = |r|^2
+  auto true_residual = [&]() -> int {
+    ApplyArgs ar{x, r, b, nullptr, nullptr, 0.0, 0.0, nullptr};
+    if (int rc = sell_apply<AP_RESID, double>(*pat, vals, ar, nullptr, s)) return rc;
+    hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, r, r, part, (const int32_t*)nullptr);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, &sc->rr);
+    FEM_LAUNCHED();
+    return FEM_OK;
+  };
   hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, b, b, part, (const int32_t*)nullptr);
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, &sc->bb);
   FEM_LAUNCHED();
-  ApplyArgs ar{x, r, b, nullptr, nullptr, 0.0, 0.0, nullptr};
-  if (int rc = sell_apply<AP_RESID, double>(*pat, vals, ar, nullptr, s)) return rc;
-  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, r, r, part, (const int32_t*)nullptr);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, &sc->rr);
+  if (int rc = true_residual()) return rc;
   hipLaunchKernelGGL(k_cg_start, dim3(1), dim3(1), 0, s, sc, opt->rtol, opt->atol, opt->maxiter);
   FEM_LAUNCHED();
   AmgScalars h{};
@@ -1881,6 +1949,12 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_am
   double cheb_ms = 0.0, cycle_ms = 0.0;
   int samples = 0;
   int it = 0;
+  // FP32 outer operator: the recursive residual is replaced by the FP64 true
+  // residual when it has fallen by reliable_delta since the last replacement,
+  // and when it meets the tolerance (convergence is decided on the true one)
+  double rr_repl = h.rr;
+  const double delta2 = opt->reliable_delta * opt->reliable_delta;
+  int replacements = 0;
   while (!h.done) {
     hipEvent_t* e = opt->timing ? ev : nullptr;
     if (int rc = (amg->precision == FEM_F32 ? vcycle_impl<float>(*amg, r, z, cyc, s, e)
@@ -1891,7 +1965,9 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_am
     hipLaunchKernelGGL(k_cg_p, dim3(g), dim3(256), 0, s, n, z, p, sc);
     FEM_LAUNCHED();
     ApplyArgs apq{p, q, nullptr, nullptr, nullptr, 0.0, 0.0, part};
-    if (int rc = sell_apply<AP_PQ, double>(*pat, vals, apq, done, s)) return rc;
+    if (int rc = outer32 ? sell_apply<AP_PQ, float, true>(*pat, vals32, apq, done, s)
+                         : sell_apply<AP_PQ, double>(*pat, vals, apq, done, s))
+      return rc;
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, part, (int)pat->num_slices, sc);
     hipLaunchKernelGGL(k_cg_xr, dim3(kDotBlocks), dim3(256), 0, s, n, p, q, x, r, part, sc);
     hipLaunchKernelGGL(k_cg_rr, dim3(1), dim3(256), 0, s, part, kDotBlocks, sc);
@@ -1906,6 +1982,14 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_am
         cheb_ms += a;
         cycle_ms += b;
         ++samples;
+      }
+      if (outer32 && !h.nan && (h.done ? h.iter < opt->maxiter : h.rr < delta2 * rr_repl)) {
+        if (int rc = true_residual()) return rc;
+        hipLaunchKernelGGL(k_cg_replaced, dim3(1), dim3(1), 0, s, sc);
+        FEM_LAUNCHED();
+        if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &h)) return rc;
+        rr_repl = h.rr;
+        ++replacements;
       }
       if (it >= opt->maxiter && !h.done) break;
     }
@@ -1924,7 +2008,7 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const fem_am
   AmgScalars fin{};
   if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &fin)) return rc;
   result->iterations = fin.iter;
-  result->replacements = 0;
+  result->replacements = replacements;
   result->b_norm = sqrt(fin.bb);
   result->res_norm = sqrt(fin.rz);
   result->status = fin.nan ? FEM_ERR_NAN : (fin.rr > fin.stop2 ? FEM_ERR_NOT_CONVERGED : FEM_OK);

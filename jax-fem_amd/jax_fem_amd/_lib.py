@@ -69,13 +69,13 @@ class FemAmgLevel(C.Structure):
 
 
 class FemAmg(C.Structure):
-    _fields_ = [('num_levels', _i32), ('precision', _i32), ('degree', _i32), ('pad', _i32),
+    _fields_ = [('num_levels', _i32), ('precision', _i32), ('degree', _i32), ('level0_form', _i32),
                 ('levels', C.POINTER(FemAmgLevel)), ('coarse_n', _i64), ('coarse_inv', _vp)]
 
 
 class FemAmgOptions(C.Structure):
     _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32), ('timing', _i32),
-                ('pad', _i32)]
+                ('pad', _i32), ('reliable_delta', _f64)]
 
 
 _P = C.POINTER
@@ -118,6 +118,7 @@ _SIGNATURES = {
     'fem_amg_tentative': (C.c_int, [_i64, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fem_amg_bc_mask': (C.c_int, [_i64, _i64, _vp, _vp, _vp]),
     'fem_amg_sell_to_bcsr': (C.c_int, [_P(FemPattern), _vp, _vp, _vp]),
+    'fem_amg_sell_to_f32_diff': (C.c_int, [_P(FemPattern), _vp, _vp, _vp]),
     'fem_amg_dinv': (C.c_int, [_P(FemBcsr), _i32, _vp, _vp]),
     'fem_amg_lmax_ws_bytes': (_i64, [_i64]),
     'fem_amg_lmax': (C.c_int, [_P(FemBcsr), _P(FemPattern), _vp, _vp, _i32, _P(_f64), _vp, _vp]),
@@ -134,7 +135,7 @@ _SIGNATURES = {
     'fem_amg_ws_bytes': (_i64, [_P(FemAmg)]),
     'fem_amg_vcycle': (C.c_int, [_P(FemAmg), _vp, _vp, _vp, _vp]),
     'fem_amg_solve_ws_bytes': (_i64, [_P(FemPattern), _P(FemAmg)]),
-    'fem_amg_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _P(FemAmg), _vp, _vp, _P(FemAmgOptions), _vp, _vp,
+    'fem_amg_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemAmg), _vp, _vp, _P(FemAmgOptions), _vp, _vp,
                                     _P(FemPcgResult), _vp]),
 }
 

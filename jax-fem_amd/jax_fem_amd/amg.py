@@ -68,6 +68,8 @@ class DeviceAMG:
         self.L = dev.L
         self._mask = None
         self._f32 = {}
+        self.vals32 = None
+        self._diff_fresh = False
         self.levels = []
         self.amg = None
 
@@ -156,6 +158,18 @@ class DeviceAMG:
         check(self.L.fem_amg_lmax(C.byref(s), pat, ptr(sell_vals), ptr(dinv), POWER_STEPS, C.byref(lam), ptr(ws),
                                   stream_ptr()), 'fem_amg_lmax')
         return float(lam.value)
+
+    def _sell_diff32(self, vals64):
+        """FP32 difference-form copy of the FP64 SELL matrix (V-cycle level 0 and outer CG)."""
+        dev = self.dev
+        n = dev.sell_blocks * dev.V * dev.V
+        out = self._f32.get('sell')
+        if out is None or out.numel() != n:
+            out = torch.empty(n, dtype=torch.float32, device=vals64.device)
+            self._f32['sell'] = out
+        check(self.L.fem_amg_sell_to_f32_diff(dev.pref, ptr(vals64), ptr(out), stream_ptr()),
+              'fem_amg_sell_to_f32_diff')
+        return out
 
     def _to_f32(self, t, key):
         out = self._f32.get(key)
@@ -262,9 +276,11 @@ class DeviceAMG:
             e = arr[l]
             if l == 0:
                 e.pat = C.pointer(dev.pat)
-                sv = self._to_f32(vals64[:dev.sell_blocks * dev.V * dev.V], 'sell') if f32 else vals64
+                sv = self._sell_diff32(vals64) if f32 else vals64
                 keep.append(sv)
                 e.sell_vals = sv.data_ptr()
+                self.vals32 = sv if f32 else None
+                self._diff_fresh = f32
             for name in ('A', 'P', 'R'):
                 M = lv[name]
                 v = self._to_f32(M.vals, (l, name)) if (f32 and not (name == 'A' and l == 0)) else M.vals
@@ -276,6 +292,7 @@ class DeviceAMG:
         amg.num_levels = len(self.levels)
         amg.precision = _lib.FEM_F32 if f32 else _lib.FEM_F64
         amg.degree = degree
+        amg.level0_form = _lib.FEM_FORM_DIFFERENCE if f32 else _lib.FEM_FORM_STANDARD
         amg.levels = arr
         amg.coarse_n = self.coarse.rows * self.coarse.br
         amg.coarse_inv = self.coarse_inv.data_ptr()
@@ -302,14 +319,20 @@ class DeviceAMG:
         o = _lib.FemAmgOptions()
         o.rtol, o.atol, o.maxiter, o.check_every = cfg['tol'], cfg['atol'], cfg['maxiter'], cfg['check_every']
         o.timing = 1 if cfg.get('timing', False) else 0
+        o.reliable_delta = cfg.get('reliable_delta', 0.01)
         nb = int(self.L.fem_amg_solve_ws_bytes(dev.pref, C.byref(self.amg)))
         key = ('amg_ws', nb)
         if dev.buffers.get('amg_ws_key') != key:
             dev.buffers['amg_ws'] = self._empty(nb, torch.uint8)
             dev.buffers['amg_ws_key'] = key
+        if self.vals32 is not None and not self._diff_fresh:
+            self._sell_diff32(vals64)   # a reused hierarchy: level 0 and the outer operator follow the matrix
+        self._diff_fresh = False
         x = x0.clone()
         res = _lib.FemPcgResult()
-        check(self.L.fem_amg_pcg_solve(dev.pref, ptr(vals64), C.byref(self.amg), ptr(b), ptr(x), C.byref(o),
+        outer32 = self.vals32 if cfg.get('outer', 'fp64') == 'fp32' else None
+        check(self.L.fem_amg_pcg_solve(dev.pref, ptr(vals64), ptr(outer32), C.byref(self.amg), ptr(b), ptr(x),
+                                       C.byref(o),
                                        ptr(dev.buffers['amg_ws']), C.c_void_p(dev.host_status.data_ptr()),
                                        C.byref(res), stream_ptr()), 'fem_amg_pcg_solve')
         return x, res

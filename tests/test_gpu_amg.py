@@ -177,3 +177,27 @@ def test_amg_hex27_matches_jacobi_pcg(gpu):
     assert its < its_j / 2, (its, its_j)
     y = solver(p, {'amgx_solver': {'precision': 'mixed', 'tol': 1e-12, 'atol': 0.}})[0]
     assert torch.equal(x, y)
+
+
+def test_amg_mixed_outer_fp32_replacements(gpu):
+    # precision 'mixed': the outer CG runs on the FP32 difference-form matrix and
+    # replaces its residual by the FP64 one; the FP64 path needs no replacement
+    p = _elastic(16)
+    dev, A, _, mask, _, b = _assembled(p)
+    ref = spla.spsolve(A.tocsc(), b.cpu().numpy())
+    out = {}
+    for precision in ('fp64', 'mixed'):
+        x = solver(p, {'amgx_solver': {'precision': precision, 'tol': 1e-12, 'atol': 0.,
+                                       'outer': 'fp32' if precision == 'mixed' else 'fp64'}})[0]
+        st = p._solve_stats['linear'][0]
+        out[precision] = (x.reshape(-1).cpu().numpy(), st['iterations'], st['replacements'], st['res_norm'])
+    assert out['fp64'][2] == 0 and out['mixed'][2] >= 2, out
+    for precision, (x, its, repl, res) in out.items():
+        assert np.linalg.norm(x - ref) <= 1e-9 * np.linalg.norm(ref), precision
+    assert out['mixed'][1] <= out['fp64'][1] + 5
+    # setup reuse: the FP32 copy follows the current matrix (the tangent is re-assembled per solve)
+    opts = {'amgx_solver': {'precision': 'mixed', 'outer': 'fp32', 'tol': 1e-12, 'atol': 0., 'setup_reuse': 3}}
+    x1 = solver(p, opts)[0].reshape(-1).cpu().numpy()
+    x2 = solver(p, opts)[0].reshape(-1).cpu().numpy()
+    assert np.linalg.norm(x2 - ref) <= 1e-9 * np.linalg.norm(ref)
+    assert np.linalg.norm(x1 - ref) <= 1e-9 * np.linalg.norm(ref)

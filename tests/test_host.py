@@ -39,7 +39,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.FemBcsr) == 3 * 8 + 2 * 4 + 3 * 8
     assert ctypes.sizeof(_lib.FemAmgLevel) == 8 + 8 + 3 * ctypes.sizeof(_lib.FemBcsr) + 8 + 8
     assert ctypes.sizeof(_lib.FemAmg) == 4 * 4 + 8 + 8 + 8
-    assert ctypes.sizeof(_lib.FemAmgOptions) == 8 + 8 + 4 * 4
+    assert ctypes.sizeof(_lib.FemAmgOptions) == 8 + 8 + 4 * 4 + 8
 
 
 def test_amgx_solver_options(tmp_path):
@@ -51,6 +51,8 @@ def test_amgx_solver_options(tmp_path):
         linear_config({'amgx_solver': {'smoother': 'JACOBI'}})
     with pytest.raises(ValueError):
         linear_config({'amgx_solver': {'precision': 'fp16'}})
+    with pytest.raises(ValueError):
+        linear_config({'amgx_solver': {'outer': 'fp32'}})     # an FP32 outer operator needs precision 'mixed'
     # an AMGX config file (solver.py:158-187 layout): tolerance and max_iters are taken
     path = tmp_path / 'amgx.json'
     path.write_text('{"config_version": 2, "solver": {"solver": "PCG", "tolerance": 1e-9, "max_iters": 77}}')
