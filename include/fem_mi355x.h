@@ -327,7 +327,9 @@ typedef struct {
 } fem_amg_level_t;
 
 typedef struct {
-  int32_t        num_levels;     /* smoothed levels (>= 1)                    */
+  int32_t        num_levels;     /* smoothed levels (>= 0; with 0 the matrix
+                                    itself is the coarsest level and the cycle
+                                    is its dense inverse, levels may be NULL) */
   int32_t        precision;      /* value storage of A, P, R and the level-0
                                     SELL copy: FEM_F32 or FEM_F64; vectors and
                                     arithmetic are FP64                       */

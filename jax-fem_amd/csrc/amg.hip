@@ -1311,8 +1311,9 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
     dd[l] = take(n);
     res[l] = take(n);
   }
-  rhs[NL] = take(amg.coarse_n);
-  cur[NL] = take(amg.coarse_n);
+  // with no levels the cycle is the dense solve of the whole matrix
+  rhs[NL] = NL == 0 ? const_cast<double*>(r) : take(amg.coarse_n);
+  cur[NL] = NL == 0 ? z : take(amg.coarse_n);
   const int deg = amg.degree;
   // down
   for (int l = 0; l < NL; ++l) {
@@ -1340,9 +1341,11 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
     xb[l] = other;
   }
   // coarsest
+  if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[0], s));
   hipLaunchKernelGGL(k_dense_gemv, dim3((unsigned)ceil_div(amg.coarse_n * 64, 256)), dim3(256), 0, s, amg.coarse_n,
                      amg.coarse_inv, rhs[NL], cur[NL]);
   FEM_LAUNCHED();
+  if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[1], s));
   // up
   for (int l = NL - 1; l >= 0; --l) {
     const fem_amg_level_t& L = amg.levels[l];
@@ -1376,13 +1379,15 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
 }
 
 int check_amg(const fem_amg_t* amg) {
-  FEM_CHECK_ARG(amg && amg->levels && amg->num_levels >= 1 && amg->coarse_inv && amg->coarse_n > 0,
+  FEM_CHECK_ARG(amg && (amg->levels || amg->num_levels == 0) && amg->num_levels >= 0 && amg->coarse_inv &&
+                    amg->coarse_n > 0,
                 "AMG hierarchy descriptor incomplete");
   FEM_CHECK_ARG(amg->precision == FEM_F32 || amg->precision == FEM_F64, "AMG precision must be FEM_F32 or FEM_F64");
   FEM_CHECK_ARG(amg->degree >= 1 && amg->degree <= 8, "AMG Chebyshev degree must be in [1, 8]");
   FEM_CHECK_ARG(amg->level0_form == FEM_FORM_STANDARD ||
                     (amg->level0_form == FEM_FORM_DIFFERENCE && amg->precision == FEM_F32),
                 "AMG level 0: the difference form is an FP32 storage form");
+  if (amg->num_levels == 0) return FEM_OK;   // the dense coarse inverse is the whole preconditioner
   const fem_amg_level_t& L0 = amg->levels[0];
   FEM_CHECK_ARG(L0.pat && L0.sell_vals && L0.dinv, "AMG level 0 needs the SELL pattern, values and dinv");
   for (int l = 0; l < amg->num_levels; ++l) {
@@ -1870,7 +1875,7 @@ int fem_amg_to_f32(int64_t n, const double* in, float* out, void* stream) {
 }
 
 int64_t fem_amg_ws_bytes(const fem_amg_t* amg) {
-  if (!amg || !amg->levels || amg->num_levels < 1 || !amg->levels[0].pat) return 0;
+  if (!amg || amg->num_levels < 0 || (amg->num_levels > 0 && (!amg->levels || !amg->levels[0].pat))) return 0;
   return vcycle_ws(*amg);
 }
 
@@ -1897,7 +1902,8 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const float*
   const bool outer32 = vals32 != nullptr;
   FEM_CHECK_ARG(!outer32 || (opt->reliable_delta > 0.0 && opt->reliable_delta < 1.0),
                 "fem_amg_pcg_solve: reliable_delta must be in (0, 1) with an FP32 outer operator");
-  FEM_CHECK_ARG(amg->levels[0].pat->num_rows == pat->num_rows && amg->levels[0].pat->vec == pat->vec,
+  FEM_CHECK_ARG(amg->num_levels == 0 ? amg->coarse_n == pat->num_rows * pat->vec
+                                     : amg->levels[0].pat->num_rows == pat->num_rows && amg->levels[0].pat->vec == pat->vec,
                 "fem_amg_pcg_solve: the hierarchy was built for another pattern");
   FEM_CHECK_ARG(opt->maxiter >= 0 && opt->check_every >= 1, "fem_amg_pcg_solve: bad options");
   hipStream_t s = as_stream(stream);

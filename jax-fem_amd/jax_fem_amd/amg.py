@@ -224,8 +224,6 @@ class DeviceAMG:
             levels.append(dict(A=A, agg=agg, n_agg=n_agg, P0=P0, P0m=P0m, Bc=Bc, P=P, R=R, R_src=src, W=W,
                                plans=(plan_w0, plan_w, plan_c)))
             A, B, bs = Ac, Bc, ns
-        if not levels:
-            raise _lib.FemError("the matrix is too small for a multigrid hierarchy; use 'jax_solver'")
         n_c = A.rows * A.br
         if n_c > 4096:
             raise _lib.FemError(f"AMG coarsening stalled at {n_c} unknowns (the dense coarse solve takes <= 4096)")
@@ -237,7 +235,7 @@ class DeviceAMG:
         """Hierarchy of the current FP64 matrix (standard-form SELL values `vals64`)."""
         dev, L = self.dev, self.L
         levels, coarse = self._structure()
-        A0 = levels[0]['A']
+        A0 = levels[0]['A'] if levels else coarse   # no levels: the matrix is the coarsest level
         check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), stream_ptr()), 'fem_amg_sell_to_bcsr')
         for l, lv in enumerate(levels):
             A = lv['A']
@@ -272,6 +270,7 @@ class DeviceAMG:
         f32 = precision == _lib.FEM_F32
         keep = []
         arr = (_lib.FemAmgLevel * len(self.levels))()
+        self.vals32, self._diff_fresh = None, False
         for l, lv in enumerate(self.levels):
             e = arr[l]
             if l == 0:
@@ -292,8 +291,8 @@ class DeviceAMG:
         amg.num_levels = len(self.levels)
         amg.precision = _lib.FEM_F32 if f32 else _lib.FEM_F64
         amg.degree = degree
-        amg.level0_form = _lib.FEM_FORM_DIFFERENCE if f32 else _lib.FEM_FORM_STANDARD
-        amg.levels = arr
+        amg.level0_form = _lib.FEM_FORM_DIFFERENCE if (f32 and self.levels) else _lib.FEM_FORM_STANDARD
+        amg.levels = arr if self.levels else None
         amg.coarse_n = self.coarse.rows * self.coarse.br
         amg.coarse_inv = self.coarse_inv.data_ptr()
         self._keep = (keep, arr)

@@ -201,3 +201,22 @@ def test_amg_mixed_outer_fp32_replacements(gpu):
     x2 = solver(p, opts)[0].reshape(-1).cpu().numpy()
     assert np.linalg.norm(x2 - ref) <= 1e-9 * np.linalg.norm(ref)
     assert np.linalg.norm(x1 - ref) <= 1e-9 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_amg_small_problem_dense_coarse_solve(gpu, precision):
+    # 4x2x2 cells: 135 unknowns, below the coarse size, so the hierarchy has no
+    # smoothed level and the preconditioner is the dense inverse of the matrix
+    p = _elastic(4)
+    dev, A, _, mask, _, b = _assembled(p)
+    assert dev.n <= 512
+    amg = dev.amg_setup(_cfg(precision))
+    assert amg.levels == [] and amg.amg.num_levels == 0
+    ref = spla.spsolve(A.tocsc(), b.cpu().numpy())
+    r = b.cpu().numpy().copy()
+    z = amg.vcycle(torch.as_tensor(r, device=dev.dm.device)).cpu().numpy()
+    assert np.linalg.norm(z - ref) <= 1e-10 * np.linalg.norm(ref)
+    sol = solver(p, {'amgx_solver': {'precision': precision, 'tol': 1e-12, 'atol': 0.}})[0]
+    x = sol.reshape(-1).cpu().numpy()
+    assert np.linalg.norm(x - ref) <= 1e-10 * np.linalg.norm(ref)
+    assert p._solve_stats['linear'][0]['iterations'] <= 2

@@ -305,42 +305,42 @@ def test_assembly_deterministic(gpu):
 
 
 # --------------------------------------------------------- end-to-end fixtures
-def _fixture_poisson(precision):
+def _fixture_poisson(precision, method='jax_solver'):
     d = golden('linear_poisson')
     mesh = Mesh(d['points'], d['cells'])
     bc = [[lambda p: isclose(p[0], 0.), lambda p: isclose(p[0], 1.)], [0, 0], [lambda p: 0., lambda p: 1.]]
     p = LinearPoisson(mesh, vec=1, dim=3, dirichlet_bc_info=bc)
-    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    sol = solver(p, {method: {'precision': precision}})[0]
     return sol, d['sol'], 5
 
 
-def _fixture_cube(precision):
+def _fixture_cube(precision, method='jax_solver'):
     d = golden('linear_elasticity_cube')
     mesh = Mesh(d['points'], d['cells'])
     left = lambda p: isclose(p[0], 0.)
     bc = [[left] * 3, [0, 1, 2], [lambda p: 1.] * 3]
     p = LinearElasticityCube(mesh, vec=3, dim=3, dirichlet_bc_info=bc, location_fns=[lambda p: isclose(p[0], 1.)])
-    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    sol = solver(p, {method: {'precision': precision}})[0]
     return sol, d['sol'], 5
 
 
-def _fixture_cylinder(precision):
+def _fixture_cylinder(precision, method='jax_solver'):
     d = golden('linear_elasticity_cylinder')
     mesh = Mesh(d['points'], d['cells'])
     bottom = lambda p: isclose(p[2], 0.)
     bc = [[bottom] * 3, [0, 1, 2], [lambda p: 0.] * 3]
     p = LinearElasticityCylinder(mesh, vec=3, dim=3, dirichlet_bc_info=bc, location_fns=[lambda p: isclose(p[2], 10.)])
-    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    sol = solver(p, {method: {'precision': precision}})[0]
     return sol, d['sol'], 3
 
 
-def _fixture_hyper(precision):
+def _fixture_hyper(precision, method='jax_solver'):
     d = golden('hyperelasticity')
     mesh = Mesh(d['points'], d['cells'])
     top, bottom = (lambda p: isclose(p[2], 10.)), (lambda p: isclose(p[2], 0.))
     bc = [[bottom] * 3 + [top] * 3, [0, 1, 2, 0, 1, 2], [lambda p: 0.] * 5 + [lambda p: 1.]]
     p = HyperElasticity(mesh, vec=3, dim=3, dirichlet_bc_info=bc)
-    sol = solver(p, {'jax_solver': {'precision': precision}})[0]
+    sol = solver(p, {method: {'precision': precision}})[0]
     # traction on the top face (test_hyper_elasticity.py:36-70, 121-127)
     bi = p.fes[0].get_boundary_conditions_inds([top])[0]
     g, ns = p.fes[0].get_face_shape_grads(bi)
@@ -353,8 +353,9 @@ def _fixture_hyper(precision):
     return sol, d['sol'], 5
 
 
+@pytest.mark.parametrize('method', ['jax_solver', 'amgx_solver'])
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
-def test_fenicsx_plasticity_fixture(gpu, precision):
+def test_fenicsx_plasticity_fixture(gpu, precision, method):
     """J2 plasticity, 5 load steps with the state carried by internal variables
     (test_plasticity.py:94-161): solution at decimal 5, average sigma_zz at decimal 3."""
     d = golden('plasticity')
@@ -368,7 +369,7 @@ def test_fenicsx_plasticity_fixture(gpu, precision):
     avg = []
     for disp in d['disps']:
         problem.fe.update_Dirichlet_boundary_conditions(bc(float(disp)))
-        sol = solver(problem, {'jax_solver': {'precision': precision}})[0]
+        sol = solver(problem, {method: {'precision': precision}})[0]
         problem.update_stress_strain(sol)
         avg.append(float(problem.compute_avg_stress()[2, 2]))
     assert isinstance(problem.device().material, M.J2Plastic)
@@ -413,12 +414,13 @@ def test_j2_tangent_and_residual_vs_oracle(gpu):
     assert rel(dev.apply_operator(ut, x).cpu().numpy(), A @ x.cpu().numpy()) < 1e-12
 
 
+@pytest.mark.parametrize('method', ['jax_solver', 'amgx_solver'])
 @pytest.mark.parametrize('precision', ['fp64', 'mixed'])
 @pytest.mark.parametrize('case', ['poisson', 'cube', 'cylinder', 'hyper'])
-def test_fenicsx_fixtures(gpu, case, precision):
+def test_fenicsx_fixtures(gpu, case, precision, method):
     fn = {'poisson': _fixture_poisson, 'cube': _fixture_cube, 'cylinder': _fixture_cylinder,
           'hyper': _fixture_hyper}[case]
-    sol, ref, decimal = fn(precision)
+    sol, ref, decimal = fn(precision, method)
     got = sol.cpu().numpy().reshape(ref.shape).astype(np.float32)  # save_sol writes float32 (utils.py:45)
     np.testing.assert_array_almost_equal(ref, got, decimal=decimal)
 
