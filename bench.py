@@ -417,6 +417,12 @@ def workload_neo_hookean(args):
     class NeoHookeanCube(Problem):
         # tensor map of the reference's hyperelasticity example (P = d psi / dF)
         def get_tensor_map(self):
+            if args.tabulated:
+                # the same law as a tabulated map (P written out: torch's second derivatives
+                # of the energy through linalg.det are NaN at F = I)
+                from jax_fem_amd import materials as M
+                return M.Tabulated(M.NeoHookean(mu=mu, kappa=kappa))
+
             def psi(F):
                 J = torch.linalg.det(F)
                 I1 = torch.trace(F.T @ F)
@@ -474,7 +480,7 @@ def workload_neo_hookean(args):
                                                  res['fp64']['pcg_iterations'] / res['fp64']['newton_iterations'],
                                                  "s per Newton step", u)
     _emit(f"HEX8 Neo-Hookean cube {n}^3, {p.fes[0].num_total_nodes * 3} dofs, 5 load steps, Newton |R| <= 1e-8, "
-          f"{_solver_name(args)}",
+          f"{_solver_name(args)}" + (", tabulated law" if args.tabulated else ""),
           res['mixed']['s_per_newton_step'], "f32+f64", extra)
 
 
@@ -778,6 +784,8 @@ def main():
     ap.add_argument('--only', choices=['fp64', 'mixed', 'amg_fp64', 'amg_mixed'], default=None,
                     help='time one path only (no adjoint, no summary line); for profiling')
     ap.add_argument('--nh-cells', type=int, default=64)
+    ap.add_argument('--tabulated', action='store_true',
+                    help='neo_hookean: evaluate the law through the per-quad table (any tensor map)')
     ap.add_argument('--simp-cells', type=int, nargs=3, default=[160, 80, 40])
     ap.add_argument('--hex27-cells', type=int, default=48)
     ap.add_argument('--fd-h', type=float, default=1e-6)

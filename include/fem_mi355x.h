@@ -73,12 +73,21 @@ enum {                                       /* constitutive laws (tensor_map)*/
   FEM_MAT_NEO_HOOKEAN = 4,    /* P = d psi / dF, psi = mu/2 (J^-2/3 I1 - 3) +
                                  kappa/2 (J - 1)^2, mu = c0 s, kappa = c1 s,
                                  s = per-quad parameter or 1         (vec 3) */
-  FEM_MAT_J2_PLASTIC = 5      /* small-strain J2 perfect plasticity, radial
+  FEM_MAT_J2_PLASTIC = 5,     /* small-strain J2 perfect plasticity, radial
                                  return from the per-quad state (sigma_old,
                                  eps_old): lambda = c0, mu = c1, yield stress
                                  c2 (tests/benchmarks/plasticity/
                                  test_plasticity.py:32-64)          (vec 3) */
+  FEM_MAT_TABULATED = 6       /* any tensor map: P(grad u) and its derivative
+                                 read per quadrature point from `state`,
+                                 evaluated beforehand at the current u (the
+                                 value_and_jacfwd of problem.py:262-266):
+                                 state + cq * FEM_TAB_STRIDE(vec) holds P[V][3],
+                                 D[V][3][V][3] = dP_ij / dG_kl, then
+                                 dP/dtheta[V][3] (read when has_param), cq =
+                                 cell * Q + quad                  (vec 1, 3) */
 };
+#define FEM_TAB_STRIDE(vec) (6 * (vec) + 9 * (vec) * (vec))
 
 /* ---- plain descriptor structs (device pointers) ------------------------ */
 typedef struct {
@@ -105,7 +114,9 @@ typedef struct {
   double  c[6];                  /* constants, meaning per kind above         */
   const double* param;           /* (num_cells, Q) or NULL                    */
   const double* state;           /* J2_PLASTIC: (num_cells, Q, 18) = sigma_old
-                                    (3x3 row-major) then eps_old; else NULL  */
+                                    (3x3 row-major) then eps_old; TABULATED:
+                                    (num_cells, Q, FEM_TAB_STRIDE(vec));
+                                    else NULL                                */
 } fem_material_t;
 
 typedef struct {
@@ -219,6 +230,9 @@ int fem_geometry(const fem_mesh_t* mesh, double* shape_grads, double* JxW,
  * LINEAR_ELASTIC or SIMP law run on MFMA (FP64 MFMA for FEM_F64). */
 int fem_cell_tangent(const fem_mesh_t* mesh, const fem_material_t* mat, int32_t precision,
                      const double* u, void* Ke, void* stream);
+/* Gradients of u at the quadrature points (problem.py:200-203):
+ * G[c][q][i][j] = sum_a u[cells[c][a]][i] * dN_a/dx_j (FP64, (C, Q, vec, 3)). */
+int fem_quad_grad(const fem_mesh_t* mesh, const double* u, double* G, void* stream);
 /* Per-cell internal-force residual re[c][a][vec] (FP64). */
 int fem_cell_residual(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u,
                       double* re, void* stream);

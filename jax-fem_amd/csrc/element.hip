@@ -240,6 +240,63 @@ __global__ __launch_bounds__(256) void k_cell_vector_q8(fem_mesh_t mesh, fem_mat
   }
 }
 
+// ------------------------------------------------------- quadrature gradients
+// G[c][q][i][j] = sum_a u[cells[c][a]][i] dN_a/dx_j (problem.py:200-203): one
+// thread per (cell, quad), Jacobian from coordinates relative to the first node.
+template <class Cfg, int V>
+__global__ __launch_bounds__(256) void k_quad_grad(fem_mesh_t mesh, const double* __restrict__ u,
+                                                   double* __restrict__ G) {
+  constexpr int N = Cfg::N, Q = Cfg::Q;
+  __shared__ double s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= mesh.num_cells * Q) return;
+  const int64_t c = e / Q;
+  const int q = (int)(e - c * Q);
+  const int32_t* cn = mesh.cells + c * N;
+  const double* p0 = mesh.points + (int64_t)cn[0] * 3;
+  double J[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) J[k] = 0.0;
+  for (int a = 1; a < N; ++a) {
+    const double* pa = mesh.points + (int64_t)cn[a] * 3;
+    const double d0 = pa[0] - p0[0], d1 = pa[1] - p0[1], d2 = pa[2] - p0[2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double dn = s_dN[q][a][j];
+      J[0 * 3 + j] += d0 * dn;
+      J[1 * 3 + j] += d1 * dn;
+      J[2 * 3 + j] += d2 * dn;
+    }
+  }
+  double iJ[9];
+  inv3(J, det3(J), iJ);
+  double Gl[V][3];
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Gl[i][j] = 0.0;
+  for (int a = 0; a < N; ++a) {
+    const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
+    double g[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g[j] = e0 * iJ[0 * 3 + j] + e1 * iJ[1 * 3 + j] + e2 * iJ[2 * 3 + j];
+    const int64_t node = cn[a];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const double ua = u[node * V + i];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Gl[i][j] += ua * g[j];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G[(e * V + i) * 3 + j] = Gl[i][j];
+}
+
 // ------------------------------------------------------------------ adjoint
 // g[c][q] = - JxW (dP/dtheta)(grad u) : grad lam   at quad q of cell c.
 template <class Cfg, class MAT>
@@ -531,6 +588,11 @@ static int with_mat(const fem_material_t& mat, int vec, F&& f) {
     case FEM_MAT_J2_PLASTIC:
       if (vec != 3 || !mat.state) break;
       return f(MatJ2{});
+    case FEM_MAT_TABULATED:
+      if (!mat.state) break;
+      if (vec == 1) return f(MatTabulated<1>{});
+      if (vec == 3) return f(MatTabulated<3>{});
+      break;
     default:
       break;
   }
@@ -623,6 +685,22 @@ static int cell_vector(const fem_mesh_t* mesh, const fem_material_t* mat, const 
       FEM_LAUNCHED();
       return FEM_OK;
     });
+  });
+}
+
+int fem_quad_grad(const fem_mesh_t* mesh, const double* u, double* G, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(u && G, "fem_quad_grad: NULL argument");
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    const dim3 grid((unsigned)ceil_div(mesh->num_cells * Cfg::Q, 256));
+    if (mesh->vec == 1)
+      hipLaunchKernelGGL((k_quad_grad<Cfg, 1>), grid, dim3(256), 0, as_stream(stream), *mesh, u, G);
+    else
+      hipLaunchKernelGGL((k_quad_grad<Cfg, 3>), grid, dim3(256), 0, as_stream(stream), *mesh, u, G);
+    FEM_LAUNCHED();
+    return FEM_OK;
   });
 }
 

@@ -332,4 +332,72 @@ struct MatJ2 {
   }
 };
 
+// Tabulated law: P and D = dP/d(grad u) come from a table filled at the
+// current u before the launch (the value_and_jacfwd of an arbitrary tensor map,
+// problem.py:262-266), dP/dtheta likewise for the adjoint.  The state is the
+// quad point's table row; the contractions with grad N stay in the kernels.
+template <int V_>
+struct MatTabulated {
+  static constexpr int V = V_;
+  static constexpr bool kNeedsU = false;
+  static constexpr int kStride = 6 * V + 9 * V * V;
+  template <typename T>
+  struct St {
+    const double* s;  // P[V][3], D[V][3][V][3], dP/dtheta[V][3]
+  };
+  template <typename T>
+  __device__ static St<T> prep(const fem_material_t& m, int64_t cq, const T (&)[V][3]) {
+    return St<T>{m.state + cq * kStride};
+  }
+  template <typename T>
+  __device__ static void stress(const St<T>& st, const T (&)[V][3], T (&P)[V][3]) {
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) P[i][j] = T(st.s[i * 3 + j]);
+  }
+  template <typename T>
+  __device__ static void tangent(const St<T>& st, const T* ga, const T* gb, T w, T (&K)[V][V]) {
+    const double* D = st.s + 3 * V;
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int k = 0; k < V; ++k) {
+        T acc = T(0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          T t = T(0);
+#pragma unroll
+          for (int l = 0; l < 3; ++l) t += T(D[((i * 3 + j) * V + k) * 3 + l]) * gb[l];
+          acc += ga[j] * t;
+        }
+        K[i][k] += w * acc;
+      }
+  }
+  template <typename T>
+  __device__ static void jvp(const St<T>& st, const T (&dG)[V][3], T (&dP)[V][3]) {
+    const double* D = st.s + 3 * V;
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        T t = T(0);
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+#pragma unroll
+          for (int l = 0; l < 3; ++l) t += T(D[((i * 3 + j) * V + k) * 3 + l]) * dG[k][l];
+        dP[i][j] = t;
+      }
+  }
+  template <typename T>
+  __device__ static void dparam(const fem_material_t& m, int64_t, const St<T>& st, const T (&)[V][3],
+                                T (&dP)[V][3]) {
+    const double* d = st.s + 3 * V + 9 * V * V;
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dP[i][j] = m.has_param ? T(d[i * 3 + j]) : T(0);
+  }
+};
+
 }  // namespace fem

@@ -292,10 +292,17 @@ class DeviceProblem:
         else:
             self.param_index = None
             self.param = None
+        self.tab_ivs = None
         if mat.has_state:
             sig = _flatten_iv(ivs[mat.sig_index], (C_, Q, 3, 3), dm.device).reshape(C_, Q, 9)
             eps = _flatten_iv(ivs[mat.eps_index], (C_, Q, 3, 3), dm.device).reshape(C_, Q, 9)
             self.state = torch.cat([sig, eps], dim=2).contiguous()
+        elif mat.tabulated:
+            # per-quad table of P and dP/dG, filled at the current u before each launch
+            if 'tab' not in self.buffers:
+                self.buffers['tab'] = dm.empty(max(C_ * Q * _lib.FEM_TAB_STRIDE(self.V), 1))
+            self.state = self.buffers['tab']
+            self.tab_ivs = [_flatten_iv(iv, (C_, Q), dm.device).detach() for iv in ivs]
         else:
             self.state = None
         self.mat_desc = mat.to_struct(self.param.data_ptr() if self.param is not None else None,
@@ -370,9 +377,34 @@ class DeviceProblem:
         return self.buffers[key]
 
     # -------------------------------------------------------------- compute
+    def quad_grad(self, u):
+        """grad u at every quadrature point, (C*Q, V, 3) FP64 (problem.py:200-203)."""
+        dm = self.dm
+        G = dm.empty(dm.C * dm.Q, self.V, 3)
+        check(self.L.fem_quad_grad(dm.ref, ptr(u), ptr(G), stream_ptr()), 'fem_quad_grad')
+        return G
+
+    def _tabulate(self, u, jac):
+        """Fill the tabulated law's table at u: P, and with `jac` also D = dP/dG.
+        A residual leaves D as the last tangent filled it (the operator of a
+        solve is the tangent at its linearisation point)."""
+        if not self.material.tabulated:
+            return
+        from .materials import evaluate_points
+        dm = self.dm
+        n, V = dm.C * dm.Q, self.V
+        G = self.quad_grad(u)
+        with torch.no_grad():
+            P, D = evaluate_points(self.material, G, self.tab_ivs, jac)
+        tab = self.state[:n * _lib.FEM_TAB_STRIDE(V)].view(n, _lib.FEM_TAB_STRIDE(V))
+        tab[:, :3 * V] = P.reshape(n, 3 * V)
+        if jac:
+            tab[:, 3 * V:3 * V + 9 * V * V] = D.reshape(n, 9 * V * V)
+
     def residual_unconstrained(self, u):
         """R(u) without Dirichlet rows, FP64 (problem.py:439-445)."""
         dm = self.dm
+        self._tabulate(u, False)
         cell = self.cell_buffer()
         check(self.L.fem_cell_residual(dm.ref, self.mref, ptr(u), ptr(cell), stream_ptr()), 'fem_cell_residual')
         res = dm.empty(self.n)
@@ -383,6 +415,7 @@ class DeviceProblem:
     def cell_internal_force(self, u):
         """Per-cell internal force K_e(u) u_e as (C, N, V), FP64 (the laplace-kernel value of problem.py:189-214)."""
         dm = self.dm
+        self._tabulate(u, False)
         cell = self.cell_buffer()
         check(self.L.fem_cell_residual(dm.ref, self.mref, ptr(u), ptr(cell), stream_ptr()), 'fem_cell_residual')
         return cell[:dm.C * dm.N * self.V].reshape(dm.C, dm.N, self.V).clone()
@@ -409,6 +442,7 @@ class DeviceProblem:
         s = stream_ptr()
         # FP32 matrices are stored in difference form (exact on translations), FP64 in standard form
         form = _lib.FEM_FORM_DIFFERENCE if precision == _lib.FEM_F32 else _lib.FEM_FORM_STANDARD
+        self._tabulate(u, True)
         check(self.L.fem_cell_tangent(dm.ref, self.mref, ke_precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
         check(self.L.fem_gather_matrix(self.pref, ke_storage, ptr(Ke), precision, form, ptr(vals), ptr(dinv), s),
               'fem_gather_matrix')
@@ -431,6 +465,7 @@ class DeviceProblem:
         return op
 
     def apply_operator(self, u, x):
+        self._tabulate(u, True)
         op = self.operator(u)
         y = self.dm.empty(self.n)
         check(self.L.fem_operator_apply(C.byref(op), ptr(x), ptr(y), stream_ptr()), 'fem_operator_apply')
@@ -556,6 +591,7 @@ class DeviceProblem:
         dm = self.dm
         dt = torch.float32 if precision in (_lib.FEM_F32, _lib.FEM_BF16X3) else torch.float64
         Ke = dm.empty(dm.C * dm.N * dm.N * self.V * self.V, dtype=dt)
+        self._tabulate(u, True)
         check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), stream_ptr()),
               'fem_cell_tangent')
         N, V = dm.N, self.V

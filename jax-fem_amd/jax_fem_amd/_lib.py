@@ -15,6 +15,11 @@ FEM_OK, FEM_ERR_BAD_ARG, FEM_ERR_NOT_CONVERGED, FEM_ERR_NAN, FEM_ERR_HIP, FEM_ER
 FEM_F32, FEM_F64, FEM_BF16X3 = 0, 1, 2
 FEM_FORM_STANDARD, FEM_FORM_DIFFERENCE = 0, 1
 FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN, FEM_MAT_J2_PLASTIC = 1, 2, 3, 4, 5
+FEM_MAT_TABULATED = 6
+
+
+def FEM_TAB_STRIDE(vec):
+    return 6 * vec + 9 * vec * vec
 
 _vp = C.c_void_p
 _i32, _i64, _f64 = C.c_int32, C.c_int64, C.c_double
@@ -92,6 +97,7 @@ _SIGNATURES = {
     'fem_sell_fill': (C.c_int, [_P(FemPattern), _vp, _vp]),
     'fem_geometry': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp]),
     'fem_cell_tangent': (C.c_int, [_P(FemMesh), _P(FemMaterial), _i32, _vp, _vp, _vp]),
+    'fem_quad_grad': (C.c_int, [_P(FemMesh), _vp, _vp, _vp]),
     'fem_cell_residual': (C.c_int, [_P(FemMesh), _P(FemMaterial), _vp, _vp, _vp]),
     'fem_cell_jvp': (C.c_int, [_P(FemMesh), _P(FemMaterial), _vp, _vp, _vp, _vp]),
     'fem_cell_load': (C.c_int, [_P(FemMesh), _vp, _vp, _vp]),

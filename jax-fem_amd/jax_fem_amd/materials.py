@@ -10,17 +10,22 @@ closed-form tangents (csrc/materials.hpp).  ``get_tensor_map`` may return
 * a plain torch function ``f(u_grad, *internal_vars) -> P`` written like the
   reference's JAX one; ``identify`` evaluates it on sample gradients (float64,
   CPU) and matches it to one of the laws, recovering the constants.  A map
-  that matches none raises ``NotImplementedError``.
+  that matches none becomes a ``Tabulated`` law: the map and its derivative
+  are evaluated at every quadrature point before each kernel launch and the
+  kernels read the table.
 
 Every descriptor is also callable with the reference signature, so
 post-processing code that calls the tensor map (e.g. a traction integral) keeps
 working on torch tensors.
 """
+import logging
 import math
 
 import torch
 
 from . import _lib
+
+logger = logging.getLogger(__name__)
 
 
 class Material:
@@ -33,6 +38,7 @@ class Material:
         raise NotImplementedError
 
     has_state = False
+    tabulated = False
 
     def to_struct(self, param_ptr=None, state_ptr=None):
         m = _lib.FemMaterial()
@@ -205,6 +211,146 @@ class J2Plastic(Material):
                 f"sig_index={self.sig_index}, eps_index={self.eps_index})")
 
 
+class Tabulated(Material):
+    """Any tensor map ``fn(u_grad, *internal_vars) -> P`` (problem.py:283-287).
+
+    Before each residual or tangent launch, ``fn`` is evaluated on the device
+    at every quadrature point by torch (vmapped over points, or called on the
+    batch when it broadcasts), together with its forward-mode derivative
+    D = dP/d(u_grad) (the value_and_jacfwd of problem.py:262-266, one jvp per
+    component of u_grad); the kernels read the table (FEM_MAT_TABULATED) and
+    keep the contractions with the shape gradients, the assembly and the
+    solve.  Parameter gradients are the vjp of ``fn`` with cotangent
+    -JxW grad(lambda) at each point (solver.py:1386-1416).  The linear solvers
+    are CG methods, so D must have major symmetry (D_ijkl = D_klij), as every
+    hyperelastic or potential-derived law has.
+    """
+    kind = _lib.FEM_MAT_TABULATED
+    tabulated = True
+
+    def __init__(self, fn, vec=None):
+        self.fn = fn
+        self.vec = vec
+
+    def constants(self):
+        return ()
+
+    def __call__(self, u_grad, *iv):
+        return self.fn(u_grad, *iv)
+
+    def __repr__(self):
+        return f"Tabulated({getattr(self.fn, '__name__', self.fn)!r}, vec={self.vec})"
+
+
+_CHUNK = 1 << 19      # points per evaluation batch of a tabulated law
+_SYM_TOL = 1e-9      # relative asymmetry of D accepted by the CG solvers
+
+
+def _batched_forms(fn, V):
+    """Ways to evaluate fn on a batch of points: vmapped per point (derivatives
+    by a vmapped jacfwd, or by one batched jvp per direction), or called on the
+    batch (maps written with broadcasting torch ops).  The first whose values and
+    derivatives agree with per-point evaluation is used: nested torch.func
+    transforms are not reliable for every map (vmap of jacfwd of torch.func.grad
+    through linalg.det disagrees with per-point jacfwd)."""
+    def vmapped(g, *iv):
+        return torch.func.vmap(lambda a, *b: fn(a, *b).reshape(V, 3))(g, *iv)
+
+    def point_with_aux(a, *b):
+        p = fn(a, *b).reshape(V, 3)
+        return p, p
+
+    def vmapped_jac(g, *iv):
+        # value and jacfwd in one vmapped pass (the 3V directions batched together)
+        D, P = torch.func.vmap(torch.func.jacfwd(point_with_aux, has_aux=True))(g, *iv)
+        return P, D
+    vmapped.jac = vmapped_jac
+
+    def vmapped_jvps(g, *iv):   # the same values, derivatives by one jvp per direction
+        return vmapped(g, *iv)
+
+    def broadcast(g, *iv):
+        return fn(g, *iv).reshape(g.shape[0], V, 3)
+    return (vmapped, vmapped_jvps, broadcast)
+
+
+def evaluate_points(mat, G, ivs, jac=False):
+    """P (n, V, 3) and, with ``jac``, D (n, V, 3, V, 3) = dP/dG of a tabulated
+    law at n points G (n, V, 3) with internal variables ivs [(n, ...)].
+
+    The first call checks the batched evaluation and its derivative against
+    per-point calls of the map (torch.func.jacfwd) on sample points and keeps
+    the batched form that agrees."""
+    fn, V = mat.fn, mat.vec
+    n = G.shape[0]
+    f = getattr(mat, '_form', None)
+    if f is None and n:
+        idx = torch.unique(torch.linspace(0, n - 1, min(n, 7), device=G.device).long())
+        gs, ivs_s = G[idx], [t[idx] for t in ivs]
+        ref = torch.stack([fn(gs[i], *[t[i] for t in ivs_s]).reshape(V, 3) for i in range(len(idx))])
+        dref = torch.stack([torch.func.jacfwd(lambda a: fn(a, *[t[i] for t in ivs_s]).reshape(V, 3))(gs[i])
+                            for i in range(len(idx))])
+        scale = max(float(ref.abs().max()), 1e-300)
+        dscale = max(float(dref.abs().max()), 1e-300)
+        why = []
+        for form in _batched_forms(fn, V):
+            try:
+                P = form(gs, *ivs_s)
+                D = _directional(form, gs, ivs_s, V)[1]
+            except Exception as e:  # the form does not apply to this map
+                why.append(f"{form.__name__}: {type(e).__name__}: {e}")
+                continue
+            ep, ed = float((P - ref).abs().max()) / scale, float((D - dref).abs().max()) / dscale
+            if ep <= 1e-12 and (ed <= 1e-10 or not torch.isfinite(dref).all()):
+                f = form
+                break
+            why.append(f"{form.__name__}: relative differences {ep:.2e} (P), {ed:.2e} (dP/dG)")
+        if f is None:
+            raise NotImplementedError("the tensor map evaluates neither under torch.func.vmap nor on a batch of "
+                                      "points with the same values and derivatives as per point (" +
+                                      "; ".join(why) + ")")
+        mat._form = f
+    P = torch.empty((n, V, 3), dtype=torch.float64, device=G.device)
+    D = torch.empty((n, V, 3, V, 3), dtype=torch.float64, device=G.device) if jac else None
+    for s in range(0, n, _CHUNK):
+        e = min(n, s + _CHUNK)
+        g, iv = G[s:e], [t[s:e] for t in ivs]
+        if jac:
+            P[s:e], D[s:e] = _directional(f, g, iv, V)
+            d = D[s:e].reshape(e - s, 3 * V, 3 * V)
+            bad = int((~torch.isfinite(d)).any(dim=(1, 2)).sum())
+            if bad:
+                raise NotImplementedError(
+                    f"the derivative of the tensor map is not finite at {bad} quadrature points (torch's second "
+                    "derivatives of linalg.det are NaN at matrices with repeated singular values, e.g. at F = I "
+                    "when P is torch.func.grad of an energy; write P explicitly)")
+            asym = float((d - d.transpose(1, 2)).abs().max())
+            if asym > _SYM_TOL * max(float(d.abs().max()), 1e-300):
+                raise NotImplementedError("the tangent of the tensor map is not symmetric (dP_ij/dG_kl != "
+                                          "dP_kl/dG_ij); the device linear solvers are CG methods")
+        else:
+            P[s:e] = f(g, *iv)
+    return P, D
+
+
+def _directional(form, g, iv, V):
+    """P and D: one vmapped jacfwd pass for the per-point form, one forward-mode
+    jvp per component of G for the broadcasting form."""
+    if hasattr(form, 'jac'):
+        return form.jac(g, *iv)
+    m = g.shape[0]
+    D = torch.empty((m, V, 3, V, 3), dtype=torch.float64, device=g.device)
+    P = None
+    for k in range(V):
+        for l in range(3):
+            t = torch.zeros_like(g)
+            t[:, k, l] = 1.0
+            p, dp = torch.func.jvp(lambda x: form(x, *iv), (g,), (t,))
+            D[:, :, :, k, l] = dp
+            P = p if P is None else P
+    return P, D
+
+
 def _scalar(x):
     x = torch.as_tensor(x)
     return x.reshape(x.shape[:-1]) if x.dim() and x.shape[-1] == 1 else x
@@ -234,10 +380,24 @@ def identify(tensor_map, vec, iv_shapes=()):
     ``u_grad`` (problem.py:208).  At most one internal variable with a single
     value per quad point can drive the device laws.
     """
+    if isinstance(tensor_map, Tabulated):
+        if tensor_map.vec not in (None, vec):
+            raise ValueError(f"{tensor_map!r} is for vec={tensor_map.vec}, problem has vec={vec}")
+        return Tabulated(tensor_map.fn, vec)
     if isinstance(tensor_map, Material):
         if tensor_map.vec != vec:
             raise ValueError(f"{tensor_map!r} is for vec={tensor_map.vec}, problem has vec={vec}")
         return tensor_map
+    try:
+        return _identify_closed_form(tensor_map, vec, iv_shapes)
+    except NotImplementedError as e:
+        if vec not in (1, 3):
+            raise
+        logger.info("tensor map outside the closed-form laws (%s): tabulated law", e)
+        return Tabulated(tensor_map, vec)
+
+
+def _identify_closed_form(tensor_map, vec, iv_shapes):
     gen = torch.Generator().manual_seed(1234)
 
     def rnd(scale=1.0):

@@ -264,6 +264,18 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
             cots.append(g.reshape(ivs[k].shape).to(ivs[k].dtype))
     from ._device import _eval_quad_map, _flatten_iv
     dm = dev.dm
+    if dev.material.tabulated and any(torch.is_tensor(t) and t.requires_grad for t in ivs):
+        # lambda^T R = sum_q JxW P(grad u_q, theta_q) : grad lambda_q, so -lambda^T dR/dtheta is
+        # the vjp of the tensor map with cotangent -JxW grad lambda at each point
+        gu, gl = dev.quad_grad(u), dev.quad_grad(lam)
+        if 'jxw' not in dev.buffers:
+            dev.buffers['jxw'] = dm.geometry(want_jxw=True)[1].reshape(-1)
+        with torch.enable_grad():
+            civ = [_flatten_iv(t, (dm.C, dm.Q), dm.device) for t in ivs]
+            P = dev.material._form(gu, *civ)
+        if P.requires_grad:
+            outs.append(P)
+            cots.append(-dev.buffers['jxw'][:, None, None] * gl)
     if hasattr(problem, 'get_mass_map') and any(torch.is_tensor(t) and t.requires_grad for t in ivs):
         lq = dev.load_adjoint(lam).reshape(-1, dev.V)
         xq = dev.buffers['xq']

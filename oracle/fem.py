@@ -91,8 +91,13 @@ class Material:
 
     kind: 'poisson' (k), 'elastic' (lmbda, mu), 'simp' (lmbda1, mu1, Emin, E0, p; per-quad
     theta), 'neo_hookean' (mu, kappa; optional per-quad scale), 'j2' (lmbda, mu, sig0; per-quad
-    state (..., 18) = sigma_old then eps_old, 3x3 row-major, passed as theta).
+    state (..., 18) = sigma_old then eps_old, 3x3 row-major, passed as theta), 'user' (fn: any
+    map P = fn(G, theta) written with numpy operations that carry complex numbers; its
+    derivatives are complex-step derivatives, Squire & Trapp, SIAM Review 40 (1998) 110-112:
+    df/dx = Im f(x + i h) / h with h = 1e-30, exact to rounding for analytic maps, in place of
+    the forward-mode AD of problem.py:262-266).
     """
+    CSTEP = 1e-30
 
     def __init__(self, kind, **c):
         self.kind, self.c = kind, c
@@ -127,6 +132,8 @@ class Material:
         return sig - beta[..., None, None] * sd, sd, beta, gam
 
     def stress(self, G, theta=None):
+        if self.kind == 'user':
+            return np.real(self.c['fn'](G, theta))
         if self.kind == 'j2':
             return self._j2(G, theta)[0]
         if self.kind == 'poisson':
@@ -148,6 +155,16 @@ class Material:
 
     def tangent(self, G, theta=None):
         """A (...,V,3,V,3)."""
+        if self.kind == 'user':
+            G = np.asarray(G, dtype=float)
+            V = G.shape[-2]
+            A = np.empty(G.shape + (V, 3))
+            for k in range(V):
+                for l in range(3):
+                    Gc = G.astype(complex)
+                    Gc[..., k, l] += 1j * self.CSTEP
+                    A[..., k, l] = np.imag(self.c['fn'](Gc, theta)) / self.CSTEP
+            return A
         d = np.eye(3)
         if self.kind == 'j2':
             _, sd, beta, gam = self._j2(G, theta)
@@ -183,6 +200,9 @@ class Material:
             ka * ((2. * J * J - J) * HH - (J * J - J) * HHt)
 
     def dstress_dtheta(self, G, theta):
+        if self.kind == 'user':
+            th = np.asarray(theta, dtype=float).astype(complex) + 1j * self.CSTEP
+            return np.imag(self.c['fn'](np.asarray(G, dtype=float).astype(complex), th)) / self.CSTEP
         if self.kind == 'simp':
             c = self.c
             dE = self.dE(theta)[..., None, None]

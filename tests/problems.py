@@ -172,3 +172,69 @@ class Plasticity(Problem):
         JxW = self.fe.JxW
         sig = (self.sigmas_old.reshape(-1, 3, 3) * JxW.reshape(-1)[:, None, None]).sum(0)
         return sig / JxW.sum()
+
+
+# ------------------------------------------------- maps outside the closed-form laws
+# Each map is written twice: per point with torch (as a user writes the reference's
+# jax.numpy map) and batched with numpy operations that carry complex numbers (the
+# oracle's complex-step derivatives).
+SVK_LAME = (40., 25.)   # lambda, mu of the Saint Venant-Kirchhoff law
+
+
+def svk_torch(u_grad):
+    """Saint Venant-Kirchhoff: P = F S, S = lambda tr(E) I + 2 mu E, E = (F^T F - I) / 2."""
+    lm, mu = SVK_LAME
+    eye = torch.eye(3, dtype=u_grad.dtype, device=u_grad.device)
+    F = u_grad + eye
+    E = 0.5 * (F.T @ F - eye)
+    return F @ (lm * torch.trace(E) * eye + 2. * mu * E)
+
+
+def svk_np(G, theta=None):
+    lm, mu = SVK_LAME
+    F = G + np.eye(3)
+    E = 0.5 * (np.swapaxes(F, -1, -2) @ F - np.eye(3))
+    tr = np.trace(E, axis1=-2, axis2=-1)[..., None, None]
+    return F @ (lm * tr * np.eye(3) + 2. * mu * E)
+
+
+def scaled_svk_torch(u_grad, theta):
+    """SVK with the moduli scaled by 0.5 + theta^2 (theta: one value per quadrature point)."""
+    return (0.5 + theta ** 2) * svk_torch(u_grad)
+
+
+def scaled_svk_np(G, theta):
+    return (0.5 + np.asarray(theta)[..., None, None] ** 2) * svk_np(G)
+
+
+DIFFUSION_BETA = 4.0
+
+
+def diffusion_torch(u_grad):
+    """Nonlinear diffusion: P = (1 + beta |grad u|^2) grad u (the flux of psi = |g|^2/2 + beta |g|^4/4)."""
+    return (1. + DIFFUSION_BETA * (u_grad * u_grad).sum()) * u_grad
+
+
+def diffusion_np(G, theta=None):
+    return (1. + DIFFUSION_BETA * np.sum(G * G, axis=(-2, -1)))[..., None, None] * G
+
+
+FIBER_K = 30.
+
+
+def fiber_torch(u_grad, a):
+    """SVK matrix with a fibre along the per-point unit vector a: psi_f = k/2 (|F a|^2 - 1)^2."""
+    F = u_grad + torch.eye(3, dtype=u_grad.dtype, device=u_grad.device)
+    Fa = F @ a
+    return svk_torch(u_grad) + 2. * FIBER_K * ((Fa * Fa).sum() - 1.) * torch.outer(Fa, a)
+
+
+class UserMapProblem(Problem):
+    """Problem whose tensor map is `self.tensor_map` (set before construction)."""
+    tensor_map = None
+
+    def get_tensor_map(self):
+        return type(self).tensor_map
+
+    def set_params(self, params):
+        self.internal_vars = list(params) if isinstance(params, (list, tuple)) else [params]
