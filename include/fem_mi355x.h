@@ -151,6 +151,8 @@ typedef struct {
   int64_t        n_bc;
   const int32_t* bc_dofs;        /* (n_bc) unique Dirichlet dofs              */
   double*        cell_scratch;   /* (num_cells * N * vec) doubles             */
+  const double*  mass_M;         /* NULL, or dm/du of a u-dependent mass term
+                                    (C, Q, vec, vec), added as fem_cell_mass_jvp */
 } fem_operator_t;
 
 typedef struct {
@@ -230,9 +232,20 @@ int fem_geometry(const fem_mesh_t* mesh, double* shape_grads, double* JxW,
  * LINEAR_ELASTIC or SIMP law run on MFMA (FP64 MFMA for FEM_F64). */
 int fem_cell_tangent(const fem_mesh_t* mesh, const fem_material_t* mat, int32_t precision,
                      const double* u, void* Ke, void* stream);
-/* Gradients of u at the quadrature points (problem.py:200-203):
- * G[c][q][i][j] = sum_a u[cells[c][a]][i] * dN_a/dx_j (FP64, (C, Q, vec, 3)). */
-int fem_quad_grad(const fem_mesh_t* mesh, const double* u, double* G, void* stream);
+/* u and its gradient at the quadrature points (problem.py:200-203, 228):
+ * G[c][q][i][j] = sum_a u[cells[c][a]][i] * dN_a/dx_j (FP64, (C, Q, vec, 3)),
+ * U[c][q][i] = sum_a u[cells[c][a]][i] * N_a(q) (FP64, (C, Q, vec)).  Either
+ * output may be NULL. */
+int fem_quad_grad(const fem_mesh_t* mesh, const double* u, double* G, double* U, void* stream);
+/* Tangent of a mass term whose map depends on u (the jacfwd of the mass
+ * kernel, problem.py:216-236, 262-266): adds to the per-cell tangents
+ * Ke[c][a][b][i][k] += sum_q N_a(q) N_b(q) M[c][q][i][k] JxW(c, q), where
+ * M = dm/du (FP64, (C, Q, vec, vec)) was evaluated at u beforehand; Ke is in
+ * `precision` (FEM_F32 or FEM_F64), laid out as fem_cell_tangent writes it. */
+int fem_cell_mass_tangent(const fem_mesh_t* mesh, const double* M, int32_t precision, void* Ke, void* stream);
+/* Its product with a vector: re[c][a][i] += sum_q N_a(q) JxW(c, q)
+ * sum_k M[c][q][i][k] v_q[k], v_q = sum_b N_b(q) v[cells[c][b]] (FP64). */
+int fem_cell_mass_jvp(const fem_mesh_t* mesh, const double* M, const double* v, double* re, void* stream);
 /* Per-cell internal-force residual re[c][a][vec] (FP64). */
 int fem_cell_residual(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u,
                       double* re, void* stream);

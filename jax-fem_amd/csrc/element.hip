@@ -19,6 +19,7 @@
 #include "tangent_mfma.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace fem {
 
@@ -245,7 +246,7 @@ __global__ __launch_bounds__(256) void k_cell_vector_q8(fem_mesh_t mesh, fem_mat
 // thread per (cell, quad), Jacobian from coordinates relative to the first node.
 template <class Cfg, int V>
 __global__ __launch_bounds__(256) void k_quad_grad(fem_mesh_t mesh, const double* __restrict__ u,
-                                                   double* __restrict__ G) {
+                                                   double* __restrict__ G, double* __restrict__ U) {
   constexpr int N = Cfg::N, Q = Cfg::Q;
   __shared__ double s_dN[Q][N][3];
   __shared__ double s_w[Q];
@@ -273,13 +274,16 @@ __global__ __launch_bounds__(256) void k_quad_grad(fem_mesh_t mesh, const double
   }
   double iJ[9];
   inv3(J, det3(J), iJ);
-  double Gl[V][3];
+  double Gl[V][3], Ul[V];
 #pragma unroll
-  for (int i = 0; i < V; ++i)
+  for (int i = 0; i < V; ++i) {
+    Ul[i] = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) Gl[i][j] = 0.0;
+  }
   for (int a = 0; a < N; ++a) {
     const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
+    const double na = mesh.shape_vals[q * N + a];
     double g[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) g[j] = e0 * iJ[0 * 3 + j] + e1 * iJ[1 * 3 + j] + e2 * iJ[2 * 3 + j];
@@ -287,14 +291,95 @@ __global__ __launch_bounds__(256) void k_quad_grad(fem_mesh_t mesh, const double
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const double ua = u[node * V + i];
+      Ul[i] += ua * na;
 #pragma unroll
       for (int j = 0; j < 3; ++j) Gl[i][j] += ua * g[j];
     }
   }
 #pragma unroll
-  for (int i = 0; i < V; ++i)
+  for (int i = 0; i < V; ++i) {
+    if (U) U[e * V + i] = Ul[i];
+    if (G) {
 #pragma unroll
-    for (int j = 0; j < 3; ++j) G[(e * V + i) * 3 + j] = Gl[i][j];
+      for (int j = 0; j < 3; ++j) G[(e * V + i) * 3 + j] = Gl[i][j];
+    }
+  }
+}
+
+// -------------------------------------------------- u-dependent mass term
+// MODE 0 (tangent): Ke[c][a][b][i][k] += sum_q N_a N_b M[c][q][i][k] JxW   (Ke in TK)
+// MODE 1 (jvp):     re[c][a][i] += sum_q N_a JxW sum_k M[c][q][i][k] v_q[k]
+template <class Cfg, int V, int MODE, typename TK>
+__global__ __launch_bounds__(256) void k_cell_mass(fem_mesh_t mesh, const double* __restrict__ M,
+                                                   const double* __restrict__ v, TK* __restrict__ Ke,
+                                                   double* __restrict__ re) {
+  using T = double;
+  constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW;
+  __shared__ T s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ CellLds<N, Q, T> s_cell[CPW];
+  __shared__ int32_t s_node[CPW][N];
+  __shared__ double s_mv[CPW][Q][V];   // MODE 1: JxW M v_q
+  const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
+  const int64_t c = (int64_t)blockIdx.x * CPW + w;
+  const bool active = c < mesh.num_cells;
+  load_tables(mesh, s_dN, s_w);
+  __syncthreads();
+  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  CellLds<N, Q, T>& L = s_cell[w];
+  if (MODE == 1) {
+    if (active) {
+      for (int q = t; q < Q; q += TPC) {
+        double vq[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) vq[k] = 0.0;
+        for (int b = 0; b < N; ++b) {
+          const double nb = mesh.shape_vals[q * N + b];
+#pragma unroll
+          for (int k = 0; k < V; ++k) vq[k] += nb * v[(int64_t)s_node[w][b] * V + k];
+        }
+        const double* Mq = M + ((size_t)c * Q + q) * V * V;
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+          double r = 0.0;
+#pragma unroll
+          for (int k = 0; k < V; ++k) r += Mq[i * V + k] * vq[k];
+          s_mv[w][q][i] = r * L.jxw[q];
+        }
+      }
+    }
+    __syncthreads();
+    if (!active) return;
+    for (int it = t; it < N * V; it += TPC) {
+      const int a = it / V, i = it % V;
+      double r = 0.0;
+      for (int q = 0; q < Q; ++q) r += mesh.shape_vals[q * N + a] * s_mv[w][q][i];
+      re[((size_t)c * N + a) * V + i] += r;
+    }
+  } else {
+    if (!active) return;
+    for (int pr = t; pr < N * N; pr += TPC) {
+      const int a = pr / N, b = pr % N;
+      double K[V][V];
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int k = 0; k < V; ++k) K[i][k] = 0.0;
+      for (int q = 0; q < Q; ++q) {
+        const double s = mesh.shape_vals[q * N + a] * mesh.shape_vals[q * N + b] * L.jxw[q];
+        const double* Mq = M + ((size_t)c * Q + q) * V * V;
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+          for (int k = 0; k < V; ++k) K[i][k] += s * Mq[i * V + k];
+      }
+      TK* out = Ke + ((size_t)c * N * N + pr) * (V * V);
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int k = 0; k < V; ++k) out[i * V + k] = TK(double(out[i * V + k]) + K[i][k]);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ adjoint
@@ -688,20 +773,57 @@ static int cell_vector(const fem_mesh_t* mesh, const fem_material_t* mat, const 
   });
 }
 
-int fem_quad_grad(const fem_mesh_t* mesh, const double* u, double* G, void* stream) {
+int fem_quad_grad(const fem_mesh_t* mesh, const double* u, double* G, double* U, void* stream) {
   if (int rc = check_mesh(mesh)) return rc;
-  FEM_CHECK_ARG(u && G, "fem_quad_grad: NULL argument");
+  FEM_CHECK_ARG(u && (G || U), "fem_quad_grad: NULL argument");
   if (mesh->num_cells == 0) return FEM_OK;
   return with_cfg(*mesh, [&](auto cfg) -> int {
     using Cfg = decltype(cfg);
     const dim3 grid((unsigned)ceil_div(mesh->num_cells * Cfg::Q, 256));
     if (mesh->vec == 1)
-      hipLaunchKernelGGL((k_quad_grad<Cfg, 1>), grid, dim3(256), 0, as_stream(stream), *mesh, u, G);
+      hipLaunchKernelGGL((k_quad_grad<Cfg, 1>), grid, dim3(256), 0, as_stream(stream), *mesh, u, G, U);
     else
-      hipLaunchKernelGGL((k_quad_grad<Cfg, 3>), grid, dim3(256), 0, as_stream(stream), *mesh, u, G);
+      hipLaunchKernelGGL((k_quad_grad<Cfg, 3>), grid, dim3(256), 0, as_stream(stream), *mesh, u, G, U);
     FEM_LAUNCHED();
     return FEM_OK;
   });
+}
+
+static int cell_mass(const fem_mesh_t* mesh, const double* M, int mode, int32_t precision, void* Ke,
+                     const double* v, double* re, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(M && (mode == 0 ? Ke != nullptr : (v && re)), "mass term kernel: NULL argument");
+  FEM_CHECK_ARG(precision == FEM_F32 || precision == FEM_F64, "fem_cell_mass_tangent: bad precision");
+  if (mesh->num_cells == 0) return FEM_OK;
+  return with_cfg(*mesh, [&](auto cfg) -> int {
+    using Cfg = decltype(cfg);
+    const dim3 grid(grid_cells<Cfg>(*mesh));
+    hipStream_t s = as_stream(stream);
+    auto go = [&](auto vc) {
+      constexpr int V = decltype(vc)::value;
+      if (mode == 1)
+        hipLaunchKernelGGL((k_cell_mass<Cfg, V, 1, double>), grid, dim3(256), 0, s, *mesh, M, v, (double*)nullptr,
+                           re);
+      else if (precision == FEM_F32)
+        hipLaunchKernelGGL((k_cell_mass<Cfg, V, 0, float>), grid, dim3(256), 0, s, *mesh, M, v, (float*)Ke, re);
+      else
+        hipLaunchKernelGGL((k_cell_mass<Cfg, V, 0, double>), grid, dim3(256), 0, s, *mesh, M, v, (double*)Ke, re);
+    };
+    if (mesh->vec == 1)
+      go(std::integral_constant<int, 1>{});
+    else
+      go(std::integral_constant<int, 3>{});
+    FEM_LAUNCHED();
+    return FEM_OK;
+  });
+}
+
+int fem_cell_mass_tangent(const fem_mesh_t* mesh, const double* M, int32_t precision, void* Ke, void* stream) {
+  return cell_mass(mesh, M, 0, precision, Ke, nullptr, nullptr, stream);
+}
+
+int fem_cell_mass_jvp(const fem_mesh_t* mesh, const double* M, const double* v, double* re, void* stream) {
+  return cell_mass(mesh, M, 1, FEM_F64, nullptr, v, re, stream);
 }
 
 int fem_cell_residual(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u, double* re,

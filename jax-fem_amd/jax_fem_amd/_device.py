@@ -145,6 +145,13 @@ def _eval_quad_map(fn, V, u, x, ivs):
     return torch.stack([point(i) for i in range(P)]) if P else torch.zeros((0, V), dtype=torch.float64, device=dev)
 
 
+def _as_vector(r):
+    """A point map's result as one tensor (maps may return lists of scalars)."""
+    if isinstance(r, (list, tuple)):
+        return torch.stack([torch.as_tensor(e, dtype=torch.float64) for e in r])
+    return r
+
+
 def _flatten_iv(iv, lead, device):
     """(lead..., rest) internal variable -> (prod(lead), rest) float64 device tensor."""
     t = torch.as_tensor(iv)
@@ -181,6 +188,7 @@ class DeviceProblem:
         self.material = None
         self.param = None
         self.load = None
+        self.mass_dyn = None
         self.host_status = torch.empty(128, dtype=torch.uint8).pin_memory()
         self._amg = None
         self._amg_uses = 0
@@ -314,19 +322,29 @@ class DeviceProblem:
         pb, dm = self.problem, self.dm
         V, dev = self.V, dm.device
         total = None
+        self.mass_dyn = None
         if hasattr(pb, 'get_mass_map'):
             if 'xq' not in self.buffers:
                 self.buffers['xq'] = dm.geometry(want_qpts=True)[2].reshape(-1, 3)
             xq = self.buffers['xq']
-            P = xq.shape[0]
             civ = [_flatten_iv(iv, (dm.C, dm.Q), dev) for iv in ivs]
-            m = self._eval_u_independent(pb.get_mass_map(), xq, civ, 'mass_map')
-            cell = self.cell_buffer()
-            check(self.L.fem_cell_load(dm.ref, ptr(m), ptr(cell), stream_ptr()), 'fem_cell_load')
-            total = dm.empty(self.n)
-            check(self.L.fem_gather_nodal(dm.Nn, V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell), None,
-                                          ptr(total), stream_ptr()), 'fem_gather_nodal')
-            del P
+            fn = pb.get_mass_map()
+            m = self._eval_u_independent(fn, xq, civ, 'mass_map', allow_u=True)
+            if m is None:
+                # m depends on u: evaluated at u_q with its derivative before each launch
+                key = ('mass_pm', id(fn))
+                if self._mat_cache.get('mass_key') != key:
+                    from .materials import PointMap
+                    self._mat_cache['mass_key'] = key
+                    self._mat_cache['mass_pm'] = PointMap(lambda u, x, *iv: _as_vector(fn(u, x, *iv)), (V,), (V,),
+                                                          'mass map')
+                self.mass_dyn = dict(pm=self._mat_cache['mass_pm'], rest=[xq] + [c.detach() for c in civ])
+            else:
+                cell = self.cell_buffer()
+                check(self.L.fem_cell_load(dm.ref, ptr(m), ptr(cell), stream_ptr()), 'fem_cell_load')
+                total = dm.empty(self.n)
+                check(self.L.fem_gather_nodal(dm.Nn, V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell), None,
+                                              ptr(total), stream_ptr()), 'fem_gather_nodal')
         if self.surfaces:
             if not hasattr(pb, 'get_surface_maps'):
                 raise NotImplementedError("surface integrals without get_surface_maps are outside the device path")
@@ -347,17 +365,21 @@ class DeviceProblem:
                                               ptr(total), ptr(total), stream_ptr()), 'fem_gather_nodal')
         return total
 
-    def _eval_u_independent(self, fn, xq, civ, what):
+    def _eval_u_independent(self, fn, xq, civ, what, allow_u=False):
+        """fn at u = 0 on every point; None (allow_u) or NotImplementedError when it depends on u."""
         P = xq.shape[0]
-        u0 = torch.zeros((P, self.V), dtype=torch.float64, device=xq.device)
-        vals = _eval_quad_map(fn, self.V, u0, xq, civ)
         if P:
             idx = torch.as_tensor(onp.unique(onp.linspace(0, P - 1, num=min(P, 5)).astype(int)), device=xq.device)
+            sub = [c[idx] for c in civ]
+            z = torch.zeros((len(idx), self.V), dtype=torch.float64, device=xq.device)
             u1 = torch.full((len(idx), self.V), 0.37, dtype=torch.float64, device=xq.device)
-            alt = _eval_quad_map(fn, self.V, u1, xq[idx], [c[idx] for c in civ])
-            if not torch.allclose(alt, vals[idx], rtol=1e-12, atol=0):
+            if not torch.allclose(_eval_quad_map(fn, self.V, u1, xq[idx], sub),
+                                  _eval_quad_map(fn, self.V, z, xq[idx], sub), rtol=1e-12, atol=0):
+                if allow_u:
+                    return None
                 raise NotImplementedError(f"a {what} that depends on u is outside the device path")
-        return vals.contiguous()
+        u0 = torch.zeros((P, self.V), dtype=torch.float64, device=xq.device)
+        return _eval_quad_map(fn, self.V, u0, xq, civ).contiguous()
 
     # ------------------------------------------------------------ buffers
     def cell_buffer(self):
@@ -381,8 +403,46 @@ class DeviceProblem:
         """grad u at every quadrature point, (C*Q, V, 3) FP64 (problem.py:200-203)."""
         dm = self.dm
         G = dm.empty(dm.C * dm.Q, self.V, 3)
-        check(self.L.fem_quad_grad(dm.ref, ptr(u), ptr(G), stream_ptr()), 'fem_quad_grad')
+        check(self.L.fem_quad_grad(dm.ref, ptr(u), ptr(G), None, stream_ptr()), "fem_quad_grad")
         return G
+
+    def quad_vals(self, u):
+        """u at every quadrature point, (C*Q, V) FP64."""
+        dm = self.dm
+        U = dm.empty(dm.C * dm.Q, self.V)
+        check(self.L.fem_quad_grad(dm.ref, ptr(u), None, ptr(U), stream_ptr()), 'fem_quad_grad')
+        return U
+
+    def _mass_eval(self, u, jac):
+        """m(u_q, x_q, *iv) and, with `jac`, M = dm/du at every quadrature point."""
+        md = self.mass_dyn
+        with torch.no_grad():
+            m, M = md['pm'](self.quad_vals(u), md['rest'], jac)
+        if jac:
+            self.buffers['mass_M'] = M.contiguous()
+        return m.contiguous()
+
+    def _base_load(self, u):
+        """Constant loads plus, for a mass map that depends on u, its nodal vector at u."""
+        if self.mass_dyn is None:
+            return self.load
+        dm = self.dm
+        m = self._mass_eval(u, False)
+        cm = self.buffers.get('mass_cell')
+        if cm is None:
+            cm = self.buffers['mass_cell'] = dm.empty(dm.C * dm.N * self.V)
+        check(self.L.fem_cell_load(dm.ref, ptr(m), ptr(cm), stream_ptr()), 'fem_cell_load')
+        out = dm.empty(self.n)
+        check(self.L.fem_gather_nodal(dm.Nn, self.V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cm), ptr(self.load),
+                                      ptr(out), stream_ptr()), 'fem_gather_nodal')
+        return out
+
+    def _mass_tangent(self, u, Ke, ke_storage):
+        if self.mass_dyn is None:
+            return
+        self._mass_eval(u, True)
+        check(self.L.fem_cell_mass_tangent(self.dm.ref, ptr(self.buffers['mass_M']), ke_storage, ptr(Ke),
+                                           stream_ptr()), 'fem_cell_mass_tangent')
 
     def _tabulate(self, u, jac):
         """Fill the tabulated law's table at u: P, and with `jac` also D = dP/dG.
@@ -407,9 +467,10 @@ class DeviceProblem:
         self._tabulate(u, False)
         cell = self.cell_buffer()
         check(self.L.fem_cell_residual(dm.ref, self.mref, ptr(u), ptr(cell), stream_ptr()), 'fem_cell_residual')
+        base = self._base_load(u)
         res = dm.empty(self.n)
         check(self.L.fem_gather_nodal(dm.Nn, self.V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell),
-                                      ptr(self.load), ptr(res), stream_ptr()), 'fem_gather_nodal')
+                                      ptr(base), ptr(res), stream_ptr()), 'fem_gather_nodal')
         return res
 
     def cell_internal_force(self, u):
@@ -444,6 +505,7 @@ class DeviceProblem:
         form = _lib.FEM_FORM_DIFFERENCE if precision == _lib.FEM_F32 else _lib.FEM_FORM_STANDARD
         self._tabulate(u, True)
         check(self.L.fem_cell_tangent(dm.ref, self.mref, ke_precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
+        self._mass_tangent(u, Ke, ke_storage)
         check(self.L.fem_gather_matrix(self.pref, ke_storage, ptr(Ke), precision, form, ptr(vals), ptr(dinv), s),
               'fem_gather_matrix')
         check(self.L.fem_matrix_bc_rows(self.pref, precision, ptr(vals), self.n_bc, ptr(self.bc_dofs), ptr(dinv), s),
@@ -461,11 +523,15 @@ class DeviceProblem:
         op.n_bc = self.n_bc
         op.bc_dofs = self.bc_dofs.data_ptr() if self.n_bc else None
         op.cell_scratch = self.cell_buffer().data_ptr()
+        # the mass term's dm/du is the one of the last tangent (the operator's linearisation point)
+        op.mass_M = self.buffers['mass_M'].data_ptr() if self.mass_dyn is not None else None
         self._op_keep = (op, u)
         return op
 
     def apply_operator(self, u, x):
         self._tabulate(u, True)
+        if self.mass_dyn is not None:
+            self._mass_eval(u, True)
         op = self.operator(u)
         y = self.dm.empty(self.n)
         check(self.L.fem_operator_apply(C.byref(op), ptr(x), ptr(y), stream_ptr()), 'fem_operator_apply')
@@ -594,5 +660,6 @@ class DeviceProblem:
         self._tabulate(u, True)
         check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), stream_ptr()),
               'fem_cell_tangent')
+        self._mass_tangent(u, Ke, _lib.FEM_F32 if dt == torch.float32 else _lib.FEM_F64)
         N, V = dm.N, self.V
         return Ke.reshape(dm.C, N, N, V, V).permute(0, 1, 3, 2, 4).reshape(dm.C, N * V, N * V)

@@ -246,109 +246,125 @@ _CHUNK = 1 << 19      # points per evaluation batch of a tabulated law
 _SYM_TOL = 1e-9      # relative asymmetry of D accepted by the CG solvers
 
 
-def _batched_forms(fn, V):
-    """Ways to evaluate fn on a batch of points: vmapped per point (derivatives
-    by a vmapped jacfwd, or by one batched jvp per direction), or called on the
-    batch (maps written with broadcasting torch ops).  The first whose values and
-    derivatives agree with per-point evaluation is used: nested torch.func
-    transforms are not reliable for every map (vmap of jacfwd of torch.func.grad
-    through linalg.det disagrees with per-point jacfwd)."""
-    def vmapped(g, *iv):
-        return torch.func.vmap(lambda a, *b: fn(a, *b).reshape(V, 3))(g, *iv)
+class PointMap:
+    """A user function f(x, *rest) -> y evaluated at a batch of points, with its
+    forward-mode derivative dy/dx (the value_and_jacfwd of problem.py:262-266).
 
-    def point_with_aux(a, *b):
-        p = fn(a, *b).reshape(V, 3)
-        return p, p
+    x has the per-point shape ``in_shape``, y ``out_shape``; ``rest`` are
+    per-point arguments (internal variables, coordinates) batched along dim 0.
+    Forms tried, the first whose values and derivatives agree with per-point
+    evaluation (torch.func.jacfwd) on sample points being kept: vmapped per
+    point with a vmapped jacfwd, vmapped per point with one batched jvp per
+    component of x, and a plain call on the batch (maps written with
+    broadcasting torch ops).  Nested torch.func transforms are not reliable
+    for every map (vmap of jacfwd through linalg.det/inv disagrees with
+    per-point jacfwd), hence the check.
+    """
 
-    def vmapped_jac(g, *iv):
-        # value and jacfwd in one vmapped pass (the 3V directions batched together)
-        D, P = torch.func.vmap(torch.func.jacfwd(point_with_aux, has_aux=True))(g, *iv)
-        return P, D
-    vmapped.jac = vmapped_jac
+    def __init__(self, fn, in_shape, out_shape, what):
+        self.fn, self.ins, self.outs, self.what = fn, tuple(in_shape), tuple(out_shape), what
+        self.ni, self.no = math.prod(self.ins), math.prod(self.outs)
+        self.form = None
 
-    def vmapped_jvps(g, *iv):   # the same values, derivatives by one jvp per direction
-        return vmapped(g, *iv)
+    def _forms(self):
+        fn, outs = self.fn, self.outs
 
-    def broadcast(g, *iv):
-        return fn(g, *iv).reshape(g.shape[0], V, 3)
-    return (vmapped, vmapped_jvps, broadcast)
+        def vmapped(x, *r):
+            return torch.func.vmap(lambda a, *b: fn(a, *b).reshape(outs))(x, *r)
 
+        def point_with_aux(a, *b):
+            y = fn(a, *b).reshape(outs)
+            return y, y
 
-def evaluate_points(mat, G, ivs, jac=False):
-    """P (n, V, 3) and, with ``jac``, D (n, V, 3, V, 3) = dP/dG of a tabulated
-    law at n points G (n, V, 3) with internal variables ivs [(n, ...)].
+        def vmapped_jac(x, *r):
+            D, Y = torch.func.vmap(torch.func.jacfwd(point_with_aux, has_aux=True))(x, *r)
+            return Y, D
+        vmapped.jac = vmapped_jac
 
-    The first call checks the batched evaluation and its derivative against
-    per-point calls of the map (torch.func.jacfwd) on sample points and keeps
-    the batched form that agrees."""
-    fn, V = mat.fn, mat.vec
-    n = G.shape[0]
-    f = getattr(mat, '_form', None)
-    if f is None and n:
-        idx = torch.unique(torch.linspace(0, n - 1, min(n, 7), device=G.device).long())
-        gs, ivs_s = G[idx], [t[idx] for t in ivs]
-        ref = torch.stack([fn(gs[i], *[t[i] for t in ivs_s]).reshape(V, 3) for i in range(len(idx))])
-        dref = torch.stack([torch.func.jacfwd(lambda a: fn(a, *[t[i] for t in ivs_s]).reshape(V, 3))(gs[i])
+        def vmapped_jvps(x, *r):
+            return vmapped(x, *r)
+
+        def broadcast(x, *r):
+            return fn(x, *r).reshape((x.shape[0],) + outs)
+        return (vmapped, vmapped_jvps, broadcast)
+
+    def _derivative(self, form, x, r):
+        if hasattr(form, 'jac'):
+            return form.jac(x, *r)
+        m = x.shape[0]
+        D = torch.empty((m, self.no, self.ni), dtype=torch.float64, device=x.device)
+        Y = None
+        for k in range(self.ni):
+            t = torch.zeros_like(x).reshape(m, self.ni)
+            t[:, k] = 1.0
+            y, dy = torch.func.jvp(lambda z: form(z, *r), (x,), (t.reshape(x.shape),))
+            D[:, :, k] = dy.reshape(m, self.no)
+            Y = y if Y is None else Y
+        return Y, D.reshape((m,) + self.outs + self.ins)
+
+    def _choose(self, X, rest):
+        fn, n = self.fn, X.shape[0]
+        idx = torch.unique(torch.linspace(0, n - 1, min(n, 7), device=X.device).long())
+        xs, rs = X[idx], [t[idx] for t in rest]
+        ref = torch.stack([fn(xs[i], *[t[i] for t in rs]).reshape(self.outs) for i in range(len(idx))])
+        dref = torch.stack([torch.func.jacfwd(lambda a: fn(a, *[t[i] for t in rs]).reshape(self.outs))(xs[i])
                             for i in range(len(idx))])
         scale = max(float(ref.abs().max()), 1e-300)
         dscale = max(float(dref.abs().max()), 1e-300)
         why = []
-        for form in _batched_forms(fn, V):
+        for form in self._forms():
             try:
-                P = form(gs, *ivs_s)
-                D = _directional(form, gs, ivs_s, V)[1]
+                Y = form(xs, *rs)
+                D = self._derivative(form, xs, rs)[1]
             except Exception as e:  # the form does not apply to this map
                 why.append(f"{form.__name__}: {type(e).__name__}: {e}")
                 continue
-            ep, ed = float((P - ref).abs().max()) / scale, float((D - dref).abs().max()) / dscale
-            if ep <= 1e-12 and (ed <= 1e-10 or not torch.isfinite(dref).all()):
-                f = form
-                break
-            why.append(f"{form.__name__}: relative differences {ep:.2e} (P), {ed:.2e} (dP/dG)")
-        if f is None:
-            raise NotImplementedError("the tensor map evaluates neither under torch.func.vmap nor on a batch of "
-                                      "points with the same values and derivatives as per point (" +
-                                      "; ".join(why) + ")")
-        mat._form = f
-    P = torch.empty((n, V, 3), dtype=torch.float64, device=G.device)
-    D = torch.empty((n, V, 3, V, 3), dtype=torch.float64, device=G.device) if jac else None
-    for s in range(0, n, _CHUNK):
-        e = min(n, s + _CHUNK)
-        g, iv = G[s:e], [t[s:e] for t in ivs]
-        if jac:
-            P[s:e], D[s:e] = _directional(f, g, iv, V)
-            d = D[s:e].reshape(e - s, 3 * V, 3 * V)
+            ey, ed = float((Y - ref).abs().max()) / scale, float((D - dref).abs().max()) / dscale
+            if ey <= 1e-12 and (ed <= 1e-10 or not torch.isfinite(dref).all()):
+                self.form = form
+                return
+            why.append(f"{form.__name__}: relative differences {ey:.2e} (value), {ed:.2e} (derivative)")
+        raise NotImplementedError(f"the {self.what} evaluates neither under torch.func.vmap nor on a batch of "
+                                  "points with the same values and derivatives as per point (" +
+                                  "; ".join(why) + ")")
+
+    def __call__(self, X, rest, jac=False, symmetric=True):
+        """Y (n, *out) and, with ``jac``, D (n, *out, *in).  ``symmetric``: D, as an
+        (out x in) matrix per point, must be symmetric (the CG solvers)."""
+        n = X.shape[0]
+        if self.form is None and n:
+            self._choose(X, rest)
+        Y = torch.empty((n,) + self.outs, dtype=torch.float64, device=X.device)
+        D = torch.empty((n,) + self.outs + self.ins, dtype=torch.float64, device=X.device) if jac else None
+        for s in range(0, n, _CHUNK):
+            e = min(n, s + _CHUNK)
+            x, r = X[s:e], [t[s:e] for t in rest]
+            if not jac:
+                Y[s:e] = self.form(x, *r)
+                continue
+            Y[s:e], D[s:e] = self._derivative(self.form, x, r)
+            d = D[s:e].reshape(e - s, self.no, self.ni)
             bad = int((~torch.isfinite(d)).any(dim=(1, 2)).sum())
             if bad:
                 raise NotImplementedError(
-                    f"the derivative of the tensor map is not finite at {bad} quadrature points (torch's second "
+                    f"the derivative of the {self.what} is not finite at {bad} quadrature points (torch's second "
                     "derivatives of linalg.det are NaN at matrices with repeated singular values, e.g. at F = I "
-                    "when P is torch.func.grad of an energy; write P explicitly)")
-            asym = float((d - d.transpose(1, 2)).abs().max())
-            if asym > _SYM_TOL * max(float(d.abs().max()), 1e-300):
-                raise NotImplementedError("the tangent of the tensor map is not symmetric (dP_ij/dG_kl != "
-                                          "dP_kl/dG_ij); the device linear solvers are CG methods")
-        else:
-            P[s:e] = f(g, *iv)
-    return P, D
+                    "when P is torch.func.grad of an energy; write the map explicitly)")
+            if symmetric:
+                asym = float((d - d.transpose(1, 2)).abs().max())
+                if asym > _SYM_TOL * max(float(d.abs().max()), 1e-300):
+                    raise NotImplementedError(f"the derivative of the {self.what} is not symmetric; the device "
+                                              "linear solvers are CG methods")
+        return Y, D
 
 
-def _directional(form, g, iv, V):
-    """P and D: one vmapped jacfwd pass for the per-point form, one forward-mode
-    jvp per component of G for the broadcasting form."""
-    if hasattr(form, 'jac'):
-        return form.jac(g, *iv)
-    m = g.shape[0]
-    D = torch.empty((m, V, 3, V, 3), dtype=torch.float64, device=g.device)
-    P = None
-    for k in range(V):
-        for l in range(3):
-            t = torch.zeros_like(g)
-            t[:, k, l] = 1.0
-            p, dp = torch.func.jvp(lambda x: form(x, *iv), (g,), (t,))
-            D[:, :, :, k, l] = dp
-            P = p if P is None else P
-    return P, D
+def evaluate_points(mat, G, ivs, jac=False):
+    """P (n, V, 3) and, with ``jac``, D (n, V, 3, V, 3) = dP/dG of a tabulated
+    law at n points G (n, V, 3) with internal variables ivs [(n, ...)]."""
+    pm = getattr(mat, '_pm', None)
+    if pm is None:
+        pm = mat._pm = PointMap(mat.fn, (mat.vec, 3), (mat.vec, 3), 'tensor map')
+    return pm(G, ivs, jac)
 
 
 def _scalar(x):

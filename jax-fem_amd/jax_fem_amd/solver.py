@@ -272,7 +272,7 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
             dev.buffers['jxw'] = dm.geometry(want_jxw=True)[1].reshape(-1)
         with torch.enable_grad():
             civ = [_flatten_iv(t, (dm.C, dm.Q), dm.device) for t in ivs]
-            P = dev.material._form(gu, *civ)
+            P = dev.material._pm.form(gu, *civ)
         if P.requires_grad:
             outs.append(P)
             cots.append(-dev.buffers['jxw'][:, None, None] * gl)
@@ -281,7 +281,8 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
         xq = dev.buffers['xq']
         with torch.enable_grad():
             civ = [_flatten_iv(t, (dm.C, dm.Q), dm.device) for t in ivs]
-            m = _eval_quad_map(problem.get_mass_map(), dev.V, torch.zeros_like(lq), xq, civ)
+            uq = dev.quad_vals(u) if dev.mass_dyn is not None else torch.zeros_like(lq)
+            m = _eval_quad_map(problem.get_mass_map(), dev.V, uq, xq, civ)
         if m.requires_grad:
             outs.append(m)
             cots.append(-lq)

@@ -249,6 +249,27 @@ def cell_load(m, shape_vals, JxW):
     return np.einsum('cqv,qa,cq->cav', m, shape_vals, JxW)
 
 
+def cell_mass_tangent(M, shape_vals, JxW):
+    """Tangent (C, N*V, N*V) of a u-dependent mass term: the jacfwd of the mass kernel
+    (problem.py:216-236, 262-266), sum_q N_a N_b dm_i/du_k JxW for M = dm/du (C,Q,V,V)."""
+    C_, Q, V, _ = M.shape
+    N = shape_vals.shape[1]
+    K = np.einsum('cqik,qa,qb,cq->caibk', M, shape_vals, shape_vals, JxW)
+    return K.reshape(C_, N * V, N * V)
+
+
+def mass_map_derivative(fn, uq, xq, ivs, h=1e-30):
+    """dm/du (C,Q,V,V) of a numpy mass map m(u, x, *ivs) by complex steps
+    (Squire & Trapp 1998), the oracle's counterpart of forward-mode AD."""
+    V = uq.shape[-1]
+    M = np.empty(uq.shape + (V,))
+    for k in range(V):
+        uc = uq.astype(complex)
+        uc[..., k] += 1j * h
+        M[..., k] = np.imag(fn(uc, xq, *ivs)) / h
+    return M
+
+
 def face_load(t, fvals, nanson, bi):
     """Surface-map term (S,N,V) (problem.py:238-259)."""
     return np.einsum('sqv,sqa,sq->sav', t, fvals[bi[:, 1]], nanson)
@@ -372,12 +393,22 @@ class Problem:
         self.bc_dofs, self.bc_vals = dirichlet(self.points, vec, bc_info)
         self.mat, self.theta = mat, theta
         self.load = np.zeros((len(self.points), vec))
+        self.mass_fn, self.mass_ivs = None, ()
 
     def quad_points(self):
         return quad_points(self.points, self.cells, self.sv)
 
     def add_mass(self, m):
         self.load += scatter(cell_load(m, self.sv, self.JxW), self.cells, len(self.points))
+
+    def set_mass_map(self, fn, ivs=()):
+        """A mass map m(u_q, x_q, *ivs) that depends on u (numpy, batched over (C,Q)), kept in
+        the residual and differentiated by complex steps in the tangent."""
+        self.mass_fn, self.mass_ivs = fn, tuple(ivs)
+
+    def _mass_at(self, u):
+        uq = np.einsum('qa,cav->cqv', self.sv, u[self.cells])
+        return uq, self.quad_points()
 
     def add_surface(self, bi, t):
         _, ns = face_shape_grads(self.points, self.cells, bi, self.fg, self.fw, self.fn)
@@ -387,6 +418,10 @@ class Problem:
         u = dofs.reshape(-1, self.vec)
         r = scatter(cell_residual(self.mat, u, self.cells, self.G, self.JxW, self.theta), self.cells,
                     len(self.points)) + self.load
+        if self.mass_fn is not None:
+            uq, xq = self._mass_at(u)
+            m = np.real(self.mass_fn(uq, xq, *self.mass_ivs))
+            r = r + scatter(cell_load(m, self.sv, self.JxW), self.cells, len(self.points))
         r = r.reshape(-1)
         if bc:  # apply_bc_vec (solver.py:290-304)
             r[self.bc_dofs] = dofs[self.bc_dofs] - self.bc_vals
@@ -394,6 +429,9 @@ class Problem:
 
     def tangent(self, dofs, bc=True):
         Ke = cell_tangent(self.mat, dofs.reshape(-1, self.vec), self.cells, self.G, self.JxW, self.theta)
+        if self.mass_fn is not None:
+            uq, xq = self._mass_at(dofs.reshape(-1, self.vec))
+            Ke = Ke + cell_mass_tangent(mass_map_derivative(self.mass_fn, uq, xq, self.mass_ivs), self.sv, self.JxW)
         A = csr_from_cells(Ke, self.cells, self.vec, self.n)
         return zero_rows(A, self.bc_dofs) if bc else A
 

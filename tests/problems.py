@@ -238,3 +238,40 @@ class UserMapProblem(Problem):
 
     def set_params(self, params):
         self.internal_vars = list(params) if isinstance(params, (list, tuple)) else [params]
+
+
+# ------------------------------------------------------- mass maps that depend on u
+REACTION_F = 10.
+
+
+def reaction_torch(u, x):
+    """Cubic reaction with a source: m = u^3 - f x_0 (-lap u + u^3 = f x_0)."""
+    return u ** 3 - REACTION_F * x[0:1]
+
+
+def reaction_np(u, x):
+    return u ** 3 - REACTION_F * x[..., 0:1]
+
+
+SPRING_K = 5e3
+
+
+def spring_torch(u, x):
+    """Nonlinear elastic foundation: m = k |u|^2 u."""
+    return SPRING_K * (u * u).sum() * u
+
+
+def spring_np(u, x):
+    return SPRING_K * np.sum(u * u, axis=-1, keepdims=True) * u
+
+
+HEAT_DT = 0.05
+
+
+def heat_torch(u, x, u_old):
+    """Backward-Euler heat step: m = (u - u_old) / dt (u_old: per-point internal variable)."""
+    return (u - u_old) / HEAT_DT
+
+
+def heat_np(u, x, u_old):
+    return (u - u_old) / HEAT_DT

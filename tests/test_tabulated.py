@@ -132,3 +132,29 @@ def test_non_finite_derivative_is_refused():
     assert torch.isfinite(D).all()
     with pytest.raises(NotImplementedError, match="not finite"):
         M.evaluate_points(mat, torch.zeros(4, 3, 3, dtype=torch.float64), [], jac=True)
+
+
+def test_oracle_mass_tangent_is_the_residual_derivative():
+    """The oracle's tangent with a u-dependent mass term (complex-step dm/du) against
+    central differences of its residual."""
+    from problems import spring_np, reaction_np
+    for vec, fn, mat in ((3, spring_np, OF.Material('elastic', lmbda=1e3, mu=8e2)),
+                         (1, reaction_np, OF.Material('poisson', k=1.))):
+        pts, cells = OF.box_mesh(3, 2, 2, 1., 1., 1.)
+        o = OF.Problem(pts, cells, vec, mat=mat)
+        o.set_mass_map(fn)
+        rng = np.random.default_rng(7)
+        u, v = 0.3 * rng.standard_normal(o.n), rng.standard_normal(o.n)
+        h = 1e-6
+        fd = (o.residual(u + h * v, bc=False) - o.residual(u - h * v, bc=False)) / (2 * h)
+        assert _rel(o.tangent(u, bc=False) @ v, fd) < 1e-7
+
+
+def test_mass_point_map_matches_complex_step():
+    from problems import spring_torch, spring_np
+    rng = np.random.default_rng(8)
+    U, X = rng.standard_normal((200, 3)), rng.uniform(0, 1, (200, 3))
+    pm = M.PointMap(spring_torch, (3,), (3,), 'mass map')
+    m, D = pm(torch.as_tensor(U), [torch.as_tensor(X)], jac=True)
+    assert _rel(m.numpy(), spring_np(U, X)) < 1e-14
+    assert _rel(D.numpy(), OF.mass_map_derivative(spring_np, U, X, ())) < 1e-13
