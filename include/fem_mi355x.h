@@ -153,6 +153,8 @@ typedef struct {
   double*        cell_scratch;   /* (num_cells * N * vec) doubles             */
   const double*  mass_M;         /* NULL, or dm/du of a u-dependent mass term
                                     (C, Q, vec, vec), added as fem_cell_mass_jvp */
+  const double*  cell_K;         /* NULL, or per-cell matrices (C, N, N, vec,
+                                    vec) added as fem_cell_matvec            */
 } fem_operator_t;
 
 typedef struct {
@@ -246,6 +248,11 @@ int fem_cell_mass_tangent(const fem_mesh_t* mesh, const double* M, int32_t preci
 /* Its product with a vector: re[c][a][i] += sum_q N_a(q) JxW(c, q)
  * sum_k M[c][q][i][k] v_q[k], v_q = sum_b N_b(q) v[cells[c][b]] (FP64). */
 int fem_cell_mass_jvp(const fem_mesh_t* mesh, const double* M, const double* v, double* re, void* stream);
+/* Product of per-cell matrices with a nodal vector (the matrix-free form of
+ * a per-cell weak form's jacfwd, problem.py:289-301):
+ * re[c][a][i] += sum_b sum_k K[c][a][b][i][k] v[cells[c][b]][k] (FP64, K laid
+ * out as fem_cell_tangent writes Ke). */
+int fem_cell_matvec(const fem_mesh_t* mesh, const double* K, const double* v, double* re, void* stream);
 /* Per-cell internal-force residual re[c][a][vec] (FP64). */
 int fem_cell_residual(const fem_mesh_t* mesh, const fem_material_t* mat, const double* u,
                       double* re, void* stream);

@@ -382,6 +382,24 @@ __global__ __launch_bounds__(256) void k_cell_mass(fem_mesh_t mesh, const double
   }
 }
 
+// re[c][a][i] += sum_b sum_k K[c][a][b][i][k] v[node_b][k]: one thread per (c, a, i)
+__global__ __launch_bounds__(256) void k_cell_matvec(fem_mesh_t mesh, const double* __restrict__ K,
+                                                     const double* __restrict__ v, double* __restrict__ re) {
+  const int N = mesh.nodes_per_cell, V = mesh.vec;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= mesh.num_cells * N * V) return;
+  const int64_t c = e / (N * V);
+  const int a = (int)((e / V) % N), i = (int)(e % V);
+  const int32_t* cn = mesh.cells + c * N;
+  const double* Ka = K + (((size_t)c * N + a) * N) * V * V + (size_t)i * V;
+  double r = 0.0;
+  for (int b = 0; b < N; ++b) {
+    const int64_t nb = cn[b];
+    for (int k = 0; k < V; ++k) r += Ka[(size_t)b * V * V + k] * v[nb * V + k];
+  }
+  re[e] += r;
+}
+
 // ------------------------------------------------------------------ adjoint
 // g[c][q] = - JxW (dP/dtheta)(grad u) : grad lam   at quad q of cell c.
 template <class Cfg, class MAT>
@@ -816,6 +834,19 @@ static int cell_mass(const fem_mesh_t* mesh, const double* M, int mode, int32_t 
     FEM_LAUNCHED();
     return FEM_OK;
   });
+}
+
+int fem_cell_matvec(const fem_mesh_t* mesh, const double* K, const double* v, double* re, void* stream) {
+  if (int rc = check_mesh(mesh)) return rc;
+  FEM_CHECK_ARG(K && v && re, "fem_cell_matvec: NULL argument");
+  const int N = mesh->nodes_per_cell, V = mesh->vec;
+  FEM_CHECK_ARG(N == 8 || N == 27, "fem_cell_matvec: %d nodes per cell", N);
+  const int64_t rows = mesh->num_cells * N * V;
+  if (rows == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_cell_matvec, dim3((unsigned)ceil_div(rows, 256)), dim3(256), 0, as_stream(stream), *mesh, K,
+                     v, re);
+  FEM_LAUNCHED();
+  return FEM_OK;
 }
 
 int fem_cell_mass_tangent(const fem_mesh_t* mesh, const double* M, int32_t precision, void* Ke, void* stream) {

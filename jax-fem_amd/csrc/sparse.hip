@@ -280,6 +280,7 @@ int fem_operator_apply_impl(const fem_operator_t* op, const double* x, double* y
   int rc = fem_cell_jvp(op->mesh, op->mat, op->u, x, op->cell_scratch, s);
   if (rc) return rc;
   if (op->mass_M && (rc = fem_cell_mass_jvp(op->mesh, op->mass_M, x, op->cell_scratch, s))) return rc;
+  if (op->cell_K && (rc = fem_cell_matvec(op->mesh, op->cell_K, x, op->cell_scratch, s))) return rc;
   rc = fem_gather_nodal(op->mesh->num_nodes, op->mesh->vec, op->inc_ptr, op->inc_idx, op->cell_scratch, nullptr, y, s);
   if (rc) return rc;
   if (op->n_bc > 0) {

@@ -145,6 +145,17 @@ def _eval_quad_map(fn, V, u, x, ivs):
     return torch.stack([point(i) for i in range(P)]) if P else torch.zeros((0, V), dtype=torch.float64, device=dev)
 
 
+def _surface_map_kernel(smap, V):
+    """The surface kernel of problem.py:238-259 for a surface map that depends on u:
+    sum_q m(u_q, x_q, *iv_q) N_a(q) nanson_q, as a per-face function."""
+    def kernel(cell_sol_flat, x, face_shape_vals, face_shape_grads, nanson, *ivs):
+        N = face_shape_vals.shape[1]
+        u = face_shape_vals @ cell_sol_flat.reshape(N, V)                    # (FQ, V)
+        m = torch.func.vmap(lambda a, b, *c: _as_vector(smap(a, b, *c)).reshape(V))(u, x, *ivs)
+        return ((face_shape_vals * nanson[0][:, None])[:, :, None] * m[:, None, :]).sum(0).reshape(-1)
+    return kernel
+
+
 def _as_vector(r):
     """A point map's result as one tensor (maps may return lists of scalars)."""
     if isinstance(r, (list, tuple)):
@@ -189,6 +200,8 @@ class DeviceProblem:
         self.param = None
         self.load = None
         self.mass_dyn = None
+        self.cell_terms = []
+        self.dynamic_surfaces = set()
         self.host_status = torch.empty(128, dtype=torch.uint8).pin_memory()
         self._amg = None
         self._amg_uses = 0
@@ -284,11 +297,16 @@ class DeviceProblem:
             shp = tuple(torch.as_tensor(iv).shape)
             if shp[:2] != (C_, Q):
                 raise ValueError(f"internal variable of shape {shp} must be (num_cells, num_quads, ...) = ({C_}, {Q}, ...)")
-        if not hasattr(pb, 'get_tensor_map'):
-            raise NotImplementedError("problems without get_tensor_map are outside the device path")
+        if not hasattr(pb, 'get_tensor_map') and not hasattr(pb, 'get_universal_kernel'):
+            raise NotImplementedError("problems without get_tensor_map or get_universal_kernel are outside the "
+                                      "device path")
         key = iv_shapes
         if key not in self._mat_cache:
-            self._mat_cache[key] = identify(pb.get_tensor_map(), self.V, iv_shapes)
+            if hasattr(pb, 'get_tensor_map'):
+                self._mat_cache[key] = identify(pb.get_tensor_map(), self.V, iv_shapes)
+            else:   # no laplace term: a law with zero constants keeps the kernels' buffers defined
+                from .materials import LinearElastic, Poisson
+                self._mat_cache[key] = Poisson(0.) if self.V == 1 else LinearElastic(lmbda=0., mu=0.)
         mat = self._mat_cache[key]
         self.material = mat
         if mat.has_param:
@@ -316,13 +334,24 @@ class DeviceProblem:
         self.mat_desc = mat.to_struct(self.param.data_ptr() if self.param is not None else None,
                                       self.state.data_ptr() if self.state is not None else None)
         self.mref = C.byref(self.mat_desc)
+        self.cell_terms = []     # per-cell weak-form terms evaluated by torch (universal kernels)
+        if hasattr(pb, 'get_universal_kernel'):
+            self.cell_terms.append(self._cell_term(pb.get_universal_kernel(), ivs))
         self.load = self._constant_loads(ivs)
+        self.dynamic_surfaces = {i for i, _, _ in self._pending_faces}
+        for i, fn, siv in self._pending_faces:
+            self.cell_terms.append(self._face_term(i, fn, siv))
+        if hasattr(pb, 'get_universal_kernels_surface'):
+            for i, fn in enumerate(pb.get_universal_kernels_surface()):
+                siv = pb.internal_vars_surfaces[i] if i < len(pb.internal_vars_surfaces) else ()
+                self.cell_terms.append(self._face_term(i, fn, siv))
 
     def _constant_loads(self, ivs):
         pb, dm = self.problem, self.dm
         V, dev = self.V, dm.device
         total = None
         self.mass_dyn = None
+        self._pending_faces = []
         if hasattr(pb, 'get_mass_map'):
             if 'xq' not in self.buffers:
                 self.buffers['xq'] = dm.geometry(want_qpts=True)[2].reshape(-1, 3)
@@ -345,9 +374,10 @@ class DeviceProblem:
                 total = dm.empty(self.n)
                 check(self.L.fem_gather_nodal(dm.Nn, V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell), None,
                                               ptr(total), stream_ptr()), 'fem_gather_nodal')
-        if self.surfaces:
-            if not hasattr(pb, 'get_surface_maps'):
-                raise NotImplementedError("surface integrals without get_surface_maps are outside the device path")
+        if self.surfaces and not hasattr(pb, 'get_surface_maps'):
+            if not hasattr(pb, 'get_universal_kernels_surface'):
+                raise AssertionError("Missing definitions for surface integral")   # problem.py:338-344
+        elif self.surfaces:
             smaps = pb.get_surface_maps()
             for i, srf in enumerate(self.surfaces):
                 S = srf['faces'].shape[0]
@@ -355,7 +385,11 @@ class DeviceProblem:
                     continue
                 siv = pb.internal_vars_surfaces[i] if i < len(pb.internal_vars_surfaces) else ()
                 civ = [_flatten_iv(iv, (S, dm.FQ), dev) for iv in siv]
-                t = self._eval_u_independent(smaps[i], srf['xq'].reshape(-1, 3), civ, 'surface_map')
+                t = self._eval_u_independent(smaps[i], srf['xq'].reshape(-1, 3), civ, 'surface_map', allow_u=True)
+                if t is None:
+                    # depends on u: the surface kernel of problem.py:238-259 as a per-face term
+                    self._pending_faces.append((i, _surface_map_kernel(smaps[i], V), siv))
+                    continue
                 rf = dm.empty(S * dm.N * V)
                 check(self.L.fem_face_load(dm.ref, S, ptr(srf['faces']), ptr(t), ptr(rf), stream_ptr()),
                       'fem_face_load')
@@ -406,6 +440,58 @@ class DeviceProblem:
         check(self.L.fem_quad_grad(dm.ref, ptr(u), ptr(G), None, stream_ptr()), "fem_quad_grad")
         return G
 
+    def _cell_term(self, fn, ivs):
+        """get_universal_kernel (problem.py:275-283): fn(cell_sol_flat, x, cell_shape_grads,
+        cell_JxW, cell_v_grads_JxW, *cell_internal_vars) -> (N*V,) per cell."""
+        dm = self.dm
+        if 'uk_geo' not in self.buffers:
+            g, j, x = dm.geometry(want_grads=True, want_jxw=True, want_qpts=True)
+            vg = g[:, :, :, None, :] * j[:, :, None, None, None]
+            self.buffers['uk_geo'] = [x, g, j[:, None, :].contiguous(), vg]
+        rest = self.buffers['uk_geo'] + [_flatten_iv(iv, (dm.C,), dm.device).detach() for iv in ivs]
+        return dict(pm=self._term_map(('cell', id(fn)), fn), rest=rest, ids=None, fn=fn, ivs=ivs, lead=(dm.C,))
+
+    def _face_term(self, i, fn, siv):
+        """get_universal_kernels_surface (problem.py:305-327): fn(cell_sol_flat, x,
+        face_shape_vals, face_shape_grads, face_nanson_scale, *ivs) -> (N*V,) per face."""
+        dm = self.dm
+        srf = self.surfaces[i]
+        key = ('uk_face', i)
+        if key not in self.buffers:
+            faces = srf['faces']
+            g, ns, x = dm.face_geometry(faces, want_grads=True, want_nanson=True, want_qpts=True)
+            fv = dm.face_shape_vals[faces[:, 1].long()].contiguous()
+            self.buffers[key] = [x, fv, g, ns[:, None, :].contiguous()]
+        S = srf['faces'].shape[0]
+        rest = self.buffers[key] + [_flatten_iv(iv, (S,), dm.device).detach() for iv in siv]
+        return dict(pm=self._term_map(('face', i, id(fn)), fn), rest=rest, ids=srf['faces'][:, 0].long(), fn=fn,
+                    ivs=siv, lead=(S,))
+
+    def _term_map(self, key, fn):
+        from .materials import PointMap
+        cache = self._mat_cache.setdefault('terms', {})
+        if key not in cache:
+            nv = self.dm.N * self.V
+            cache[key] = PointMap(lambda cs, *r: _as_vector(fn(cs, *r)), (nv,), (nv,), 'universal kernel')
+        return cache[key]
+
+    def _cell_terms_eval(self, u, jac):
+        """Sum of the per-cell terms: r (C, N, V) and, with `jac`, K (C, N, N, V, V) FP64."""
+        dm = self.dm
+        N, V = dm.N, self.V
+        cs = u.reshape(-1, V)[dm.cells.long()].reshape(dm.C, N * V)
+        r = torch.zeros((dm.C, N, V), dtype=torch.float64, device=dm.device)
+        K = torch.zeros((dm.C, N, N, V, V), dtype=torch.float64, device=dm.device) if jac else None
+        with torch.no_grad():
+            for t in self.cell_terms:
+                x = cs if t['ids'] is None else cs[t['ids']]
+                y, d = t['pm'](x, t['rest'], jac)
+                ids = torch.arange(dm.C, device=dm.device) if t['ids'] is None else t['ids']
+                r.index_add_(0, ids, y.reshape(-1, N, V))
+                if jac:
+                    K.index_add_(0, ids, d.reshape(-1, N, V, N, V).permute(0, 1, 3, 2, 4))
+        return r, K
+
     def quad_vals(self, u):
         """u at every quadrature point, (C*Q, V) FP64."""
         dm = self.dm
@@ -437,6 +523,13 @@ class DeviceProblem:
                                       ptr(out), stream_ptr()), 'fem_gather_nodal')
         return out
 
+    def _cell_terms_tangent(self, u, Ke):
+        if not self.cell_terms:
+            return
+        K = self._cell_terms_eval(u, True)[1].reshape(-1)
+        self.buffers['cell_K'] = K
+        Ke[:K.numel()] += K.to(Ke.dtype)
+
     def _mass_tangent(self, u, Ke, ke_storage):
         if self.mass_dyn is None:
             return
@@ -467,6 +560,8 @@ class DeviceProblem:
         self._tabulate(u, False)
         cell = self.cell_buffer()
         check(self.L.fem_cell_residual(dm.ref, self.mref, ptr(u), ptr(cell), stream_ptr()), 'fem_cell_residual')
+        if self.cell_terms:
+            cell[:dm.C * dm.N * self.V] += self._cell_terms_eval(u, False)[0].reshape(-1)
         base = self._base_load(u)
         res = dm.empty(self.n)
         check(self.L.fem_gather_nodal(dm.Nn, self.V, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(cell),
@@ -506,6 +601,7 @@ class DeviceProblem:
         self._tabulate(u, True)
         check(self.L.fem_cell_tangent(dm.ref, self.mref, ke_precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
         self._mass_tangent(u, Ke, ke_storage)
+        self._cell_terms_tangent(u, Ke)
         check(self.L.fem_gather_matrix(self.pref, ke_storage, ptr(Ke), precision, form, ptr(vals), ptr(dinv), s),
               'fem_gather_matrix')
         check(self.L.fem_matrix_bc_rows(self.pref, precision, ptr(vals), self.n_bc, ptr(self.bc_dofs), ptr(dinv), s),
@@ -525,6 +621,7 @@ class DeviceProblem:
         op.cell_scratch = self.cell_buffer().data_ptr()
         # the mass term's dm/du is the one of the last tangent (the operator's linearisation point)
         op.mass_M = self.buffers['mass_M'].data_ptr() if self.mass_dyn is not None else None
+        op.cell_K = self.buffers['cell_K'].data_ptr() if self.cell_terms else None
         self._op_keep = (op, u)
         return op
 
@@ -532,6 +629,8 @@ class DeviceProblem:
         self._tabulate(u, True)
         if self.mass_dyn is not None:
             self._mass_eval(u, True)
+        if self.cell_terms:
+            self.buffers['cell_K'] = self._cell_terms_eval(u, True)[1].reshape(-1)
         op = self.operator(u)
         y = self.dm.empty(self.n)
         check(self.L.fem_operator_apply(C.byref(op), ptr(x), ptr(y), stream_ptr()), 'fem_operator_apply')
@@ -661,5 +760,6 @@ class DeviceProblem:
         check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), stream_ptr()),
               'fem_cell_tangent')
         self._mass_tangent(u, Ke, _lib.FEM_F32 if dt == torch.float32 else _lib.FEM_F64)
+        self._cell_terms_tangent(u, Ke)
         N, V = dm.N, self.V
         return Ke.reshape(dm.C, N, N, V, V).permute(0, 1, 3, 2, 4).reshape(dm.C, N * V, N * V)

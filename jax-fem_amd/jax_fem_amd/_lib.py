@@ -48,7 +48,7 @@ class FemPattern(C.Structure):
 class FemOperator(C.Structure):
     _fields_ = [('mesh', C.POINTER(FemMesh)), ('mat', C.POINTER(FemMaterial)), ('u', _vp),
                 ('inc_ptr', _vp), ('inc_idx', _vp), ('n_bc', _i64), ('bc_dofs', _vp),
-                ('cell_scratch', _vp), ('mass_M', _vp)]
+                ('cell_scratch', _vp), ('mass_M', _vp), ('cell_K', _vp)]
 
 
 class FemPcgOptions(C.Structure):
@@ -100,6 +100,7 @@ _SIGNATURES = {
     'fem_quad_grad': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp]),
     'fem_cell_mass_tangent': (C.c_int, [_P(FemMesh), _vp, _i32, _vp, _vp]),
     'fem_cell_mass_jvp': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp]),
+    'fem_cell_matvec': (C.c_int, [_P(FemMesh), _vp, _vp, _vp, _vp]),
     'fem_cell_residual': (C.c_int, [_P(FemMesh), _P(FemMaterial), _vp, _vp, _vp]),
     'fem_cell_jvp': (C.c_int, [_P(FemMesh), _P(FemMaterial), _vp, _vp, _vp, _vp]),
     'fem_cell_load': (C.c_int, [_P(FemMesh), _vp, _vp, _vp]),

@@ -73,8 +73,6 @@ class Problem:
             assert num_surfaces == len(self.get_universal_kernels_surface())
         else:
             assert num_surfaces == 0, "Missing definitions for surface integral"
-        if hasattr(self, 'get_universal_kernel') or hasattr(self, 'get_universal_kernels_surface'):
-            raise NotImplementedError("universal kernels are outside the device path")
 
     # ------------------------------------------------------------- hooks
     def custom_init(self, *args):
@@ -95,6 +93,10 @@ class Problem:
 
     def unflatten_fn_sol_list(self, dofs):
         return [dofs.reshape(self.fes[0].num_total_nodes, self.fes[0].vec)]
+
+    def unflatten_fn_dof(self, cell_sol_flat):
+        """(num_nodes*vec,) cell dofs -> [(num_nodes, vec)] (problem.py:114)."""
+        return [cell_sol_flat.reshape(self.fes[0].num_nodes, self.fes[0].vec)]
 
     # --------------------------------------------------- geometry fields
     @property

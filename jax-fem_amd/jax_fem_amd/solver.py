@@ -276,6 +276,21 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
         if P.requires_grad:
             outs.append(P)
             cots.append(-dev.buffers['jxw'][:, None, None] * gl)
+    if dev.cell_terms:
+        # per-cell / per-face weak-form terms: vjp of each with cotangent -lambda on its cell's dofs
+        V = dev.V
+        cl = dm.cells.long()
+        u_c = u.reshape(-1, V)[cl].reshape(dm.C, -1)
+        lam_c = lam.reshape(-1, V)[cl].reshape(dm.C, -1)
+        for t in dev.cell_terms:
+            if not any(torch.is_tensor(x) and x.requires_grad for x in t['ivs']):
+                continue
+            with torch.enable_grad():
+                rest = t['rest'][:4] + [_flatten_iv(x, t['lead'], dm.device) for x in t['ivs']]
+                ids = t['ids']
+                y = t['pm'].form(u_c if ids is None else u_c[ids], *rest)
+            outs.append(y)
+            cots.append(-(lam_c if ids is None else lam_c[ids]))
     if hasattr(problem, 'get_mass_map') and any(torch.is_tensor(t) and t.requires_grad for t in ivs):
         lq = dev.load_adjoint(lam).reshape(-1, dev.V)
         xq = dev.buffers['xq']
@@ -289,6 +304,8 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
     for i, siv in enumerate(sivs):
         if not any(torch.is_tensor(t) and t.requires_grad for t in siv):
             continue
+        if not hasattr(problem, 'get_surface_maps') or i in dev.dynamic_surfaces:
+            continue   # universal or u-dependent surface terms: their vjp is in the per-cell terms above
         srf = dev.surfaces[i]
         S = srf['faces'].shape[0]
         lq = dev.face_load_adjoint(i, lam).reshape(-1, dev.V)

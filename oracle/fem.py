@@ -394,6 +394,7 @@ class Problem:
         self.mat, self.theta = mat, theta
         self.load = np.zeros((len(self.points), vec))
         self.mass_fn, self.mass_ivs = None, ()
+        self.kernels = []    # per-cell / per-face weak-form terms: (cell ids, fn, args)
 
     def quad_points(self):
         return quad_points(self.points, self.cells, self.sv)
@@ -405,6 +406,40 @@ class Problem:
         """A mass map m(u_q, x_q, *ivs) that depends on u (numpy, batched over (C,Q)), kept in
         the residual and differentiated by complex steps in the tangent."""
         self.mass_fn, self.mass_ivs = fn, tuple(ivs)
+
+    def add_cell_kernel(self, fn, ivs=()):
+        """get_universal_kernel (problem.py:275-283): fn(cell_sol (C, N*V), x (C,Q,3),
+        shape_grads (C,Q,N,3), JxW (C,1,Q), v_grads_JxW (C,Q,N,1,3), *ivs) -> (C, N*V),
+        numpy, batched over cells."""
+        x = self.quad_points()
+        vg = self.G[:, :, :, None, :] * self.JxW[:, :, None, None, None]
+        self.kernels.append((np.arange(len(self.cells)), fn, (x, self.G, self.JxW[:, None, :], vg) + tuple(ivs)))
+
+    def add_face_kernel(self, bi, fn, ivs=()):
+        """get_universal_kernels_surface (problem.py:305-327): fn(cell_sol (S, N*V), x (S,FQ,3),
+        face_shape_vals (S,FQ,N), face_shape_grads (S,FQ,N,3), nanson (S,1,FQ), *ivs) -> (S, N*V)."""
+        g, ns = face_shape_grads(self.points, self.cells, bi, self.fg, self.fw, self.fn)
+        x = face_quad_points(self.points, self.cells, bi, self.fv)
+        self.kernels.append((bi[:, 0], fn, (x, self.fv[bi[:, 1]], g, ns[:, None, :]) + tuple(ivs)))
+
+    def _kernel_terms(self, u, jac, h=1e-30):
+        """Per-cell residual (C,N,V) and, with jac, tangent (C, N*V, N*V) of the kernels, the
+        tangent by complex steps (Squire & Trapp 1998)."""
+        C_, N = self.cells.shape
+        V = self.vec
+        r = np.zeros((C_, N * V))
+        K = np.zeros((C_, N * V, N * V)) if jac else None
+        for ids, fn, args in self.kernels:
+            cs = u[self.cells[ids]].reshape(len(ids), N * V)
+            np.add.at(r, ids, np.real(fn(cs, *args)))
+            if jac:
+                Kt = np.empty((len(ids), N * V, N * V))
+                for k in range(N * V):
+                    cc = cs.astype(complex)
+                    cc[:, k] += 1j * h
+                    Kt[:, :, k] = np.imag(fn(cc, *args)) / h
+                np.add.at(K, ids, Kt)
+        return r.reshape(C_, N, V), K
 
     def _mass_at(self, u):
         uq = np.einsum('qa,cav->cqv', self.sv, u[self.cells])
@@ -422,6 +457,8 @@ class Problem:
             uq, xq = self._mass_at(u)
             m = np.real(self.mass_fn(uq, xq, *self.mass_ivs))
             r = r + scatter(cell_load(m, self.sv, self.JxW), self.cells, len(self.points))
+        if self.kernels:
+            r = r + scatter(self._kernel_terms(u, False)[0], self.cells, len(self.points))
         r = r.reshape(-1)
         if bc:  # apply_bc_vec (solver.py:290-304)
             r[self.bc_dofs] = dofs[self.bc_dofs] - self.bc_vals
@@ -432,6 +469,8 @@ class Problem:
         if self.mass_fn is not None:
             uq, xq = self._mass_at(dofs.reshape(-1, self.vec))
             Ke = Ke + cell_mass_tangent(mass_map_derivative(self.mass_fn, uq, xq, self.mass_ivs), self.sv, self.JxW)
+        if self.kernels:
+            Ke = Ke + self._kernel_terms(dofs.reshape(-1, self.vec), True)[1]
         A = csr_from_cells(Ke, self.cells, self.vec, self.n)
         return zero_rows(A, self.bc_dofs) if bc else A
 

@@ -275,3 +275,58 @@ def heat_torch(u, x, u_old):
 
 def heat_np(u, x, u_old):
     return (u - u_old) / HEAT_DT
+
+
+# ------------------------------------------ per-cell weak forms (universal kernels)
+# The reference's get_universal_kernel receives one cell: (cell_sol_flat (N*V,), x (Q,3),
+# cell_shape_grads (Q,N,3), cell_JxW (1,Q), cell_v_grads_JxW (Q,N,1,3), *ivs (Q,...)).
+# The numpy twins take the same arguments with a leading cell axis.
+def energy_kernel_torch(sv, f=10.):
+    """-lap u + u^3 = f x_0 as a weak form: sum_q (grad u . grad v + (u^3 - f x_0) v) JxW."""
+    def kernel(cell_sol_flat, x, cell_shape_grads, cell_JxW, cell_v_grads_JxW, *ivs):
+        N = cell_shape_grads.shape[1]
+        u = cell_sol_flat.reshape(N, 1)
+        u_grads = (u[None, :, :, None] * cell_shape_grads[:, :, None, :]).sum(1)      # (Q, 1, 3)
+        val = (u_grads[:, None, :, :] * cell_v_grads_JxW).sum((0, -1))                  # (N, 1)
+        uq = sv @ u                                                                    # (Q, 1)
+        th = ivs[0] if ivs else 1.
+        src = th * uq ** 3 - f * x[:, 0:1]
+        val = val + (src[:, None, :] * sv[:, :, None] * cell_JxW[0][:, None, None]).sum(0)
+        return val.reshape(-1)
+    return kernel
+
+
+def energy_kernel_np(sv, f=10.):
+    def kernel(cs, x, G, JxW, vg, *ivs):
+        C_, N = cs.shape[0], G.shape[2]
+        u = cs.reshape(C_, N, 1)
+        ug = np.einsum('cav,cqaj->cqvj', u, G)
+        val = np.einsum('cqvj,cqaxj->cav', ug, vg)
+        uq = np.einsum('qa,cav->cqv', sv, u)
+        th = ivs[0] if ivs else 1.
+        src = th * uq ** 3 - f * x[..., 0:1]
+        val = val + np.einsum('cqv,qa,cq->cav', src, sv, JxW[:, 0, :])
+        return val.reshape(C_, -1)
+    return kernel
+
+
+ROBIN_H, ROBIN_UINF = 3., 1.
+
+
+def robin_kernel_torch(cell_sol_flat, x, face_shape_vals, face_shape_grads, nanson, *ivs):
+    """Robin condition as a surface weak form: sum_q h (u - u_inf) v nanson."""
+    N = face_shape_vals.shape[1]
+    uq = face_shape_vals @ cell_sol_flat.reshape(N, 1)                                  # (FQ, 1)
+    return ((ROBIN_H * (uq - ROBIN_UINF))[:, None, :] * face_shape_vals[:, :, None]
+            * nanson[0][:, None, None]).sum(0).reshape(-1)
+
+
+def robin_kernel_np(cs, x, fv, fg, ns, *ivs):
+    S, N = cs.shape[0], fv.shape[2]
+    uq = np.einsum('sqa,sa->sq', fv, cs.reshape(S, N))
+    return np.einsum('sq,sqa,sq->sa', ROBIN_H * (uq - ROBIN_UINF), fv, ns[:, 0, :])
+
+
+def robin_map_torch(u, x):
+    """The same Robin condition as a surface map that depends on u."""
+    return ROBIN_H * (u - ROBIN_UINF)

@@ -40,7 +40,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.FemAmgLevel) == 8 + 8 + 3 * ctypes.sizeof(_lib.FemBcsr) + 8 + 8
     assert ctypes.sizeof(_lib.FemAmg) == 4 * 4 + 8 + 8 + 8
     assert ctypes.sizeof(_lib.FemAmgOptions) == 8 + 8 + 4 * 4 + 8
-    assert ctypes.sizeof(_lib.FemOperator) == 5 * 8 + 8 + 8 + 8 + 8
+    assert ctypes.sizeof(_lib.FemOperator) == 5 * 8 + 8 + 8 + 8 + 8 + 8
 
 
 def test_amgx_solver_options(tmp_path):

@@ -158,3 +158,30 @@ def test_mass_point_map_matches_complex_step():
     m, D = pm(torch.as_tensor(U), [torch.as_tensor(X)], jac=True)
     assert _rel(m.numpy(), spring_np(U, X)) < 1e-14
     assert _rel(D.numpy(), OF.mass_map_derivative(spring_np, U, X, ())) < 1e-13
+
+
+def test_oracle_kernel_terms_match_closed_forms():
+    """The oracle's universal-kernel terms (complex-step tangents) on a weak form that the
+    closed-form oracle also writes: Poisson + cubic reaction + source, and a Robin face term."""
+    from problems import energy_kernel_np, reaction_np, robin_kernel_np, ROBIN_H, ROBIN_UINF
+    pts, cells = OF.box_mesh(3, 2, 2, 1., 1., 1.)
+    a = OF.Problem(pts, cells, 1, mat=OF.Material('poisson', k=1.))
+    a.set_mass_map(reaction_np)
+    b = OF.Problem(pts, cells, 1, mat=OF.Material('poisson', k=0.))
+    b.add_cell_kernel(energy_kernel_np(b.sv))
+    u = 0.4 * np.random.default_rng(9).standard_normal(a.n)
+    assert _rel(b.residual(u, bc=False), a.residual(u, bc=False)) < 1e-13
+    assert _rel(b.tangent(u, bc=False).toarray(), a.tangent(u, bc=False).toarray()) < 1e-13
+    # Robin face term = a u-independent surface load h u_inf plus h * (face mass matrix)
+    bi = OF.boundary_inds(pts, cells, a.face_inds, lambda p: p[0] > 1 - 1e-9)
+    c = OF.Problem(pts, cells, 1, mat=OF.Material('poisson', k=0.))
+    c.add_face_kernel(bi, robin_kernel_np)
+    d = OF.Problem(pts, cells, 1, mat=OF.Material('poisson', k=0.))
+    d.add_surface(bi, np.full((len(bi), d.fw.shape[1], 1), -ROBIN_H * ROBIN_UINF))
+    r0 = c.residual(np.zeros(c.n), bc=False)
+    assert _rel(r0, d.residual(np.zeros(d.n), bc=False)) < 1e-14
+    Kc = c.tangent(u, bc=False)
+    assert _rel(c.residual(u, bc=False) - r0, Kc @ u) < 1e-13       # the face term is affine in u
+    ones = np.ones(c.n)
+    area = 1.0                                                      # the x = 1 face of the unit box
+    assert abs(ones @ (Kc @ ones) - ROBIN_H * area) < 1e-12
