@@ -329,6 +329,18 @@ int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv,
                   const fem_pcg_options_t* opt, void* ws, void* host_status,
                   fem_pcg_result_t* result /* host */, void* stream);
 
+/* Jacobi-preconditioned BiCGSTAB, the method of jax_solve (solver.py:63-92),
+ * for tangents that are not symmetric.  FP64 standard-form matrix only
+ * (opt->precision = FEM_F64); b, x FP64 (x holds x0); dinv from
+ * fem_gather_matrix or fem_jacobi (required when opt->precond).  Uses
+ * rtol, atol, maxiter, check_every and precond of `opt`.  `host_status` is
+ * pinned host memory of at least 128 bytes.  Synchronises.  Returns FEM_OK
+ * or the solve status (result->status). */
+int64_t fem_bicgstab_ws_bytes(const fem_pattern_t* pat);
+int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const double* dinv, const double* b, double* x,
+                       const fem_pcg_options_t* opt, void* ws, void* host_status,
+                       fem_pcg_result_t* result /* host */, void* stream);
+
 /* ---- smoothed-aggregation AMG (the amgx_solver option, solver.py:141-259,
  *      261-284) ------------------------------------------------------------
  * The reference hands the assembled PETSc matrix to AMGX.  Here the hierarchy

@@ -640,6 +640,8 @@ class DeviceProblem:
         """Jacobi-PCG (or AMG-CG for cfg['method'] == 'amg') on the current tangent; returns (x, result struct)."""
         if cfg.get('method') == 'amg':
             return self.amg_solve(b, x0, cfg)
+        if cfg.get('method') == 'bicgstab':
+            return self.bicgstab(b, x0, cfg)
         precision, Ke, vals, dinv = self.current
         o = _lib.FemPcgOptions()
         o.rtol, o.atol, o.maxiter = cfg['tol'], cfg['atol'], cfg['maxiter']
@@ -658,6 +660,33 @@ class DeviceProblem:
                                    ptr(b), ptr(x), C.byref(o), ptr(ws), C.c_void_p(self.host_status.data_ptr()),
                                    C.byref(res), stream_ptr()), 'fem_pcg_solve')
         return x, res
+
+    def bicgstab(self, b, x0, cfg):
+        """FP64 Jacobi-BiCGSTAB on the current FP64 tangent (jax_solve's method, solver.py:63-92)."""
+        precision, _, vals, dinv = self.current
+        o = _lib.FemPcgOptions()
+        o.rtol, o.atol, o.maxiter = cfg['tol'], cfg['atol'], cfg['maxiter']
+        o.check_every, o.precision, o.precond = cfg['check_every'], precision, 1 if cfg['precond'] else 0
+        if 'bicgstab_ws' not in self.buffers:
+            self.buffers['bicgstab_ws'] = self.dm.empty(int(self.L.fem_bicgstab_ws_bytes(self.pref)),
+                                                        dtype=torch.uint8)
+        x = x0.clone()
+        res = _lib.FemPcgResult()
+        rc = self.L.fem_bicgstab_solve(self.pref, ptr(vals), ptr(dinv), ptr(b), ptr(x), C.byref(o),
+                                       ptr(self.buffers['bicgstab_ws']), C.c_void_p(self.host_status.data_ptr()),
+                                       C.byref(res), stream_ptr())
+        if rc not in (_lib.FEM_OK, _lib.FEM_ERR_NOT_CONVERGED, _lib.FEM_ERR_NAN):
+            check(rc, 'fem_bicgstab_solve')
+        return x, res
+
+    @property
+    def nonsymmetric(self):
+        """True when a user map or weak form evaluated so far has a non-symmetric derivative."""
+        pms = [getattr(self.material, '_pm', None)]
+        if self.mass_dyn is not None:
+            pms.append(self.mass_dyn['pm'])
+        pms += [t['pm'] for t in self.cell_terms]
+        return any(pm is not None and pm.nonsymmetric for pm in pms)
 
     def drop_amg_hierarchy(self):
         """Make the next AMG solve rebuild its numeric hierarchy (setup_reuse restarts)."""

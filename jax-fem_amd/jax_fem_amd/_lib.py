@@ -117,6 +117,9 @@ _SIGNATURES = {
     'fem_spmv': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp]),
     'fem_export_csr': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     'fem_operator_apply': (C.c_int, [_P(FemOperator), _vp, _vp, _vp]),
+    'fem_bicgstab_ws_bytes': (_i64, [_P(FemPattern)]),
+    'fem_bicgstab_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _P(FemPcgOptions), _vp, _vp,
+                                     _P(FemPcgResult), _vp]),
     'fem_pcg_ws_bytes': (_i64, [_P(FemPattern), _i32]),
     'fem_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemOperator), _vp, _vp, _P(FemPcgOptions),
                                 _vp, _vp, _P(FemPcgResult), _vp]),

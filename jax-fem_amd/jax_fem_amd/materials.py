@@ -221,9 +221,8 @@ class Tabulated(Material):
     component of u_grad); the kernels read the table (FEM_MAT_TABULATED) and
     keep the contractions with the shape gradients, the assembly and the
     solve.  Parameter gradients are the vjp of ``fn`` with cotangent
-    -JxW grad(lambda) at each point (solver.py:1386-1416).  The linear solvers
-    are CG methods, so D must have major symmetry (D_ijkl = D_klij), as every
-    hyperelastic or potential-derived law has.
+    -JxW grad(lambda) at each point (solver.py:1386-1416).  When D lacks major
+    symmetry (D_ijkl != D_klij) the Newton solves use BiCGSTAB instead of CG.
     """
     kind = _lib.FEM_MAT_TABULATED
     tabulated = True
@@ -265,6 +264,7 @@ class PointMap:
         self.fn, self.ins, self.outs, self.what = fn, tuple(in_shape), tuple(out_shape), what
         self.ni, self.no = math.prod(self.ins), math.prod(self.outs)
         self.form = None
+        self.nonsymmetric = False
 
     def _forms(self):
         fn, outs = self.fn, self.outs
@@ -328,9 +328,11 @@ class PointMap:
                                   "points with the same values and derivatives as per point (" +
                                   "; ".join(why) + ")")
 
-    def __call__(self, X, rest, jac=False, symmetric=True):
-        """Y (n, *out) and, with ``jac``, D (n, *out, *in).  ``symmetric``: D, as an
-        (out x in) matrix per point, must be symmetric (the CG solvers)."""
+    def __call__(self, X, rest, jac=False, symmetric='flag'):
+        """Y (n, *out) and, with ``jac``, D (n, *out, *in).  D, as an (out x in)
+        matrix per point, is checked for symmetry: ``symmetric='require'`` raises
+        when it is not, 'flag' records it in ``self.nonsymmetric`` (the solve then
+        uses BiCGSTAB instead of CG)."""
         n = X.shape[0]
         if self.form is None and n:
             self._choose(X, rest)
@@ -350,21 +352,22 @@ class PointMap:
                     f"the derivative of the {self.what} is not finite at {bad} quadrature points (torch's second "
                     "derivatives of linalg.det are NaN at matrices with repeated singular values, e.g. at F = I "
                     "when P is torch.func.grad of an energy; write the map explicitly)")
-            if symmetric:
+            if self.no == self.ni:
                 asym = float((d - d.transpose(1, 2)).abs().max())
                 if asym > _SYM_TOL * max(float(d.abs().max()), 1e-300):
-                    raise NotImplementedError(f"the derivative of the {self.what} is not symmetric; the device "
-                                              "linear solvers are CG methods")
+                    if symmetric == 'require':
+                        raise NotImplementedError(f"the derivative of the {self.what} is not symmetric")
+                    self.nonsymmetric = True
         return Y, D
 
 
-def evaluate_points(mat, G, ivs, jac=False):
+def evaluate_points(mat, G, ivs, jac=False, symmetric='flag'):
     """P (n, V, 3) and, with ``jac``, D (n, V, 3, V, 3) = dP/dG of a tabulated
     law at n points G (n, V, 3) with internal variables ivs [(n, ...)]."""
     pm = getattr(mat, '_pm', None)
     if pm is None:
         pm = mat._pm = PointMap(mat.fn, (mat.vec, 3), (mat.vec, 3), 'tensor map')
-    return pm(G, ivs, jac)
+    return pm(G, ivs, jac, symmetric)
 
 
 def _scalar(x):

@@ -29,7 +29,8 @@ _NEWTON_OPTION_KEYS = frozenset({'tol', 'rel_tol', 'line_search_flag', 'initial_
 # precision 'fp64' (plain FP64 PCG) or 'mixed' (FP32 matrix and PCG, FP64
 # refinement residual and iterate).
 JAX_SOLVER_DEFAULTS = dict(precond=True, precision='fp64', tol=1e-10, atol=1e-10, maxiter=10000,
-                           check_every=16, reliable_delta=0.01, xcd_remap=True, assembly=None, timing=False)
+                           check_every=16, reliable_delta=0.01, xcd_remap=True, assembly=None, timing=False,
+                           krylov='auto')
 _ASSEMBLY = {None: None, 'fp32': _lib.FEM_F32, 'fp64': _lib.FEM_F64, 'bf16x3': _lib.FEM_BF16X3}
 _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32}
 
@@ -86,7 +87,9 @@ def linear_config(linear_options):
     if cfg['assembly'] not in _ASSEMBLY:
         raise ValueError(f"assembly must be 'fp32', 'fp64' or 'bf16x3', got {cfg['assembly']!r}")
     cfg['assembly_code'] = _ASSEMBLY[cfg['assembly']]
-    cfg['method'] = 'pcg'
+    if cfg['krylov'] not in ('auto', 'cg', 'bicgstab'):
+        raise ValueError(f"krylov must be 'auto', 'cg' or 'bicgstab', got {cfg['krylov']!r}")
+    cfg['method'] = 'bicgstab' if cfg['krylov'] == 'bicgstab' else 'pcg'
     return cfg
 
 
@@ -117,11 +120,25 @@ def _amg_config(user):
 
 
 def linear_solve(dev, u, b, x0, cfg):
-    """Solve the current tangent system; mirrors jax_solve's post-check (solver.py:86-90)."""
+    """Solve the current tangent system; mirrors jax_solve's post-check (solver.py:86-90).
+
+    A tangent that is not symmetric (a user map or weak form whose derivative is
+    not) is solved by FP64 Jacobi-BiCGSTAB, jax_solve's own method; CG and
+    AMG-CG need symmetry."""
+    if cfg.get('method') != 'bicgstab' and dev.nonsymmetric:
+        if cfg.get('krylov', 'auto') == 'cg':
+            raise ValueError("the tangent is not symmetric; 'krylov': 'cg' does not apply")
+        logger.info("the tangent is not symmetric: FP64 Jacobi-BiCGSTAB")
+        cfg = dict(cfg, method='bicgstab', precision='fp64', precision_code=_lib.FEM_F64, precond=True,
+                   check_every=cfg.get('check_every', 16), xcd_remap=cfg.get('xcd_remap', True),
+                   reliable_delta=cfg.get('reliable_delta', 0.01))
+    if cfg.get('method') == 'bicgstab' and dev.current[0] != _lib.FEM_F64:
+        dev.tangent(u, _lib.FEM_F64)          # BiCGSTAB runs on the FP64 matrix
     x, info = dev.solve(u, b, x0, cfg)
     err = info.res_norm
     logger.debug("%s (%s): %d iterations, %d FP64 residual replacements, |b - A x| = %.3g",
-                 'AMG-CG' if cfg.get('method') == 'amg' else 'PCG', cfg['precision'], info.iterations,
+                 {'amg': 'AMG-CG', 'bicgstab': 'BiCGSTAB'}.get(cfg.get('method'), 'PCG'), cfg['precision'],
+                 info.iterations,
                  info.replacements, err)
     if info.status == _lib.FEM_ERR_NOT_CONVERGED:
         logger.warning("the linear solve reached maxiter=%d, |b - A x| = %.3g", cfg['maxiter'], err)
@@ -252,6 +269,9 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
         device=u.device, dtype=torch.float64).reshape(-1).clone()
     if dev.n_bc:
         v[dev.bc_dofs_long] = 0.
+    if dev.nonsymmetric:
+        raise NotImplementedError("the adjoint of a tangent that is not symmetric (A^T lambda = v) is outside the "
+                                  "device path")
     lam, info = linear_solve(dev, u, v, torch.zeros_like(v), lin)
     problem._adjoint_stats = dict(iterations=info.iterations, replacements=info.replacements,
                                   res_norm=info.res_norm, spmv_ms=info.spmv_ms, update_ms=info.update_ms)

@@ -330,3 +330,45 @@ def robin_kernel_np(cs, x, fv, fg, ns, *ivs):
 def robin_map_torch(u, x):
     """The same Robin condition as a surface map that depends on u."""
     return ROBIN_H * (u - ROBIN_UINF)
+
+
+# ------------------------------------------------------- tangents that are not symmetric
+ADVECTION_B = (4., -2., 1.)
+
+
+def advection_kernel_torch(sv):
+    """Convection-diffusion as a weak form: sum_q (grad u . grad v + (b . grad u) v - v) JxW."""
+    b = torch.tensor(ADVECTION_B, dtype=torch.float64, device=sv.device)
+
+    def kernel(cell_sol_flat, x, cell_shape_grads, cell_JxW, cell_v_grads_JxW, *ivs):
+        N = cell_shape_grads.shape[1]
+        u = cell_sol_flat.reshape(N, 1)
+        ug = (u[None, :, :, None] * cell_shape_grads[:, :, None, :]).sum(1)          # (Q, 1, 3)
+        val = (ug[:, None, :, :] * cell_v_grads_JxW).sum((0, -1))                     # (N, 1)
+        adv = (ug[:, 0, :] * b).sum(-1, keepdim=True) - 1.                             # (Q, 1)
+        return (val + (adv[:, None, :] * sv[:, :, None] * cell_JxW[0][:, None, None]).sum(0)).reshape(-1)
+    return kernel
+
+
+def advection_kernel_np(sv):
+    b = np.array(ADVECTION_B)
+
+    def kernel(cs, x, G, JxW, vg, *ivs):
+        C_, N = cs.shape[0], G.shape[2]
+        u = cs.reshape(C_, N, 1)
+        ug = np.einsum('cav,cqaj->cqvj', u, G)
+        val = np.einsum('cqvj,cqaxj->cav', ug, vg)
+        adv = np.einsum('cqj,j->cq', ug[:, :, 0, :], b)[..., None] - 1.
+        return (val + np.einsum('cqv,qa,cq->cav', adv, sv, JxW[:, 0, :])).reshape(C_, -1)
+    return kernel
+
+
+ANISO_K = ((2., 0.8, 0.), (-0.3, 1., 0.2), (0.1, -0.4, 1.5))   # conductivity without symmetry
+
+
+def aniso_torch(u_grad):
+    return u_grad @ torch.tensor(ANISO_K, dtype=u_grad.dtype, device=u_grad.device).T
+
+
+def aniso_np(G, theta=None):
+    return G @ np.array(ANISO_K).T

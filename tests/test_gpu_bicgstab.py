@@ -1,0 +1,93 @@
+"""Tangents that are not symmetric (an advection weak form, a conductivity without
+symmetry) are solved by FP64 Jacobi-BiCGSTAB, jax_solve's method (solver.py:63-92),
+and checked against the oracle's direct solve; BiCGSTAB on a symmetric problem
+agrees with PCG."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import fem as OF
+from problems import (LinearPoisson, UserMapProblem, advection_kernel_np, advection_kernel_torch, aniso_np,
+                      aniso_torch, box, isclose)
+from test_gpu_parity import oracle_problem, rel
+
+from jax_fem_amd.problem import Problem
+from jax_fem_amd.solver import ad_wrapper, solver
+
+pytestmark = pytest.mark.gpu
+
+LEFT, RIGHT = (lambda x: isclose(x[0], 0.)), (lambda x: isclose(x[0], 1.))
+
+
+def _bc_oracle(p, o):
+    dev = p.device()
+    o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
+    if dev.load is not None:
+        o.load = dev.load.cpu().numpy().reshape(-1, p.fes[0].vec)
+    return o
+
+
+class Advection(Problem):
+    def get_universal_kernel(self):
+        return advection_kernel_torch(torch.as_tensor(self.fes[0].shape_vals, device='cuda'))
+
+
+@pytest.mark.parametrize('opts', [{'jax_solver': {}}, {'jax_solver': {'precision': 'mixed'}}],
+                         ids=['fp64', 'mixed-requested'])
+def test_advection_diffusion_bicgstab_vs_oracle(gpu, opts):
+    p = Advection(box(10, 6, 6), vec=1, dim=3, dirichlet_bc_info=[[LEFT, RIGHT], [0, 0], [lambda x: 0., lambda x: 1.]])
+    sol = solver(p, opts)[0].reshape(-1).cpu().numpy()
+    assert p.device().nonsymmetric
+    o = _bc_oracle(p, oracle_problem(p, OF.Material('poisson', k=0.)))
+    o.add_cell_kernel(advection_kernel_np(o.sv))
+    A = o.tangent(np.zeros(o.n))
+    assert abs(A - A.T).max() > 1e-3 * abs(A).max()      # the system is not symmetric
+    ref = o.newton(linear='direct')
+    assert np.linalg.norm(sol - ref) <= 1e-8 * np.linalg.norm(ref)
+
+
+def test_nonsymmetric_conductivity_bicgstab_vs_oracle(gpu):
+    class P(UserMapProblem):
+        tensor_map = staticmethod(aniso_torch)
+
+        def get_mass_map(self):
+            return lambda u, x: torch.stack([-5. * x[1]])
+    p = P(box(8, 6, 5), vec=1, dim=3, dirichlet_bc_info=[[LEFT], [0], [lambda x: 0.]])
+    sol = solver(p, {'jax_solver': {'tol': 1e-12, 'atol': 0.}})[0].reshape(-1).cpu().numpy()
+    assert p.device().material.tabulated and p.device().nonsymmetric
+    o = _bc_oracle(p, oracle_problem(p, OF.Material('user', fn=aniso_np)))
+    ref = o.newton(linear='direct')
+    assert np.linalg.norm(sol - ref) <= 1e-9 * np.linalg.norm(ref)
+    with pytest.raises(ValueError, match="not symmetric"):
+        solver(p, {'jax_solver': {'krylov': 'cg'}})
+
+
+def test_bicgstab_on_a_symmetric_problem_matches_pcg(gpu):
+    class Src(LinearPoisson):
+        def get_mass_map(self):
+            return lambda u, x: torch.stack([-(1. + x[0] * x[1])])
+    p = Src(box(10, 8, 6), vec=1, dim=3, dirichlet_bc_info=[[LEFT], [0], [lambda x: 0.]])
+    a = solver(p, {'jax_solver': {'tol': 1e-12, 'atol': 0.}})[0].clone()
+    its_cg = p._solve_stats['linear'][0]['iterations']
+    b = solver(p, {'jax_solver': {'tol': 1e-12, 'atol': 0., 'krylov': 'bicgstab'}})[0]
+    its_bi = p._solve_stats['linear'][0]['iterations']
+    assert torch.linalg.norm(a - b) <= 1e-10 * torch.linalg.norm(a)
+    assert 0 < its_bi <= its_cg
+    c = solver(p, {'jax_solver': {'tol': 1e-12, 'atol': 0., 'krylov': 'bicgstab'}})[0]
+    assert torch.equal(b, c)                             # deterministic reductions
+
+
+def test_adjoint_of_nonsymmetric_tangent_is_refused(gpu):
+    class P(Advection):
+        def set_params(self, th):
+            self.internal_vars = [th]
+
+        def get_universal_kernel(self):
+            k = advection_kernel_torch(torch.as_tensor(self.fes[0].shape_vals, device='cuda'))
+            return lambda cs, x, g, j, vg, th: k(cs, x, g, j, vg) * (1. + 0. * th.sum())
+    p = P(box(4, 3, 3), vec=1, dim=3, dirichlet_bc_info=[[LEFT, RIGHT], [0, 0], [lambda x: 0., lambda x: 1.]])
+    th = torch.ones((p.num_cells, p.fes[0].num_quads, 1), dtype=torch.float64, device='cuda', requires_grad=True)
+    fwd = ad_wrapper(p)
+    u = fwd(th)[0]
+    with pytest.raises(NotImplementedError, match="not symmetric"):
+        u.sum().backward()

@@ -102,11 +102,17 @@ def test_point_evaluation_broadcasting_map():
     assert _rel(P.numpy(), o.stress(G.numpy())) < 1e-14 and _rel(D.numpy(), o.tangent(G.numpy())) < 1e-13
 
 
-def test_nonsymmetric_tangent_is_refused():
+def test_nonsymmetric_tangent_is_flagged():
     A = torch.as_tensor(np.random.default_rng(6).standard_normal((3, 3, 3, 3)))
     mat = M.Tabulated(lambda g: torch.einsum('ijkl,kl->ij', A, g), 3)
     with pytest.raises(NotImplementedError, match="not symmetric"):
-        M.evaluate_points(mat, torch.as_tensor(_grads(8, 3, 0.1)), [], jac=True)
+        M.evaluate_points(mat, torch.as_tensor(_grads(8, 3, 0.1)), [], jac=True, symmetric='require')
+    mat = M.Tabulated(lambda g: torch.einsum('ijkl,kl->ij', A, g), 3)
+    M.evaluate_points(mat, torch.as_tensor(_grads(8, 3, 0.1)), [], jac=True)
+    assert mat._pm.nonsymmetric
+    sym = M.Tabulated(M.LinearElastic(E=1., nu=0.3), 3)
+    M.evaluate_points(sym, torch.as_tensor(_grads(8, 3, 0.1)), [], jac=True)
+    assert not sym._pm.nonsymmetric
 
 
 def test_identify_falls_back_to_tabulated():
