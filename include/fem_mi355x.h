@@ -329,17 +329,38 @@ int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv,
                   const fem_pcg_options_t* opt, void* ws, void* host_status,
                   fem_pcg_result_t* result /* host */, void* stream);
 
+/* Linear multipoint constraints u = P u_r (the P_mat of solver.py:404-406,
+ * 545-551, 1290-1339): P (rows x cols) and its transpose as scalar CSR. */
+typedef struct {
+  int64_t rows;                  /* N: dofs of the mesh                       */
+  int64_t cols;                  /* M: reduced dofs                           */
+  const int32_t* p_ptr;          /* P: (N+1), (nnz), (nnz)                    */
+  const int32_t* p_idx;
+  const double*  p_val;
+  const int32_t* pt_ptr;         /* P^T: (M+1), (nnz), (nnz)                  */
+  const int32_t* pt_idx;
+  const double*  pt_val;
+} fem_constraint_t;
+
+/* y = A x for a scalar CSR matrix (FP64): the products with P and P^T. */
+int fem_csr_spmv(int64_t rows, const int32_t* ptr, const int32_t* idx, const double* val, const double* x,
+                 double* y, void* stream);
+
 /* Jacobi-preconditioned BiCGSTAB, the method of jax_solve (solver.py:63-92),
- * for tangents that are not symmetric.  FP64 standard-form matrix only
- * (opt->precision = FEM_F64); b, x FP64 (x holds x0); dinv from
- * fem_gather_matrix or fem_jacobi (required when opt->precond).  Uses
- * rtol, atol, maxiter, check_every and precond of `opt`.  `host_status` is
- * pinned host memory of at least 128 bytes.  Synchronises.  Returns FEM_OK
- * or the solve status (result->status). */
-int64_t fem_bicgstab_ws_bytes(const fem_pattern_t* pat);
-int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const double* dinv, const double* b, double* x,
-                       const fem_pcg_options_t* opt, void* ws, void* host_status,
-                       fem_pcg_result_t* result /* host */, void* stream);
+ * for tangents that are not symmetric and for constrained systems.  FP64
+ * standard-form matrix only (opt->precision = FEM_F64); dinv from
+ * fem_gather_matrix or fem_jacobi (required when opt->precond).  With P ==
+ * NULL, b and x (x holds x0) are FP64 vectors of all dofs.  With a
+ * constraint P, the system is the reduced P^T A P x_r = b_r of get_A
+ * (solver.py:545-551), b and x have P->cols entries, and the Jacobi
+ * preconditioner is sum_k P_kj^2 A_kk.  Uses rtol, atol, maxiter,
+ * check_every and precond of `opt`.  `host_status` is pinned host memory of
+ * at least 128 bytes.  Synchronises.  Returns FEM_OK or the solve status
+ * (result->status). */
+int64_t fem_bicgstab_ws_bytes(const fem_pattern_t* pat, const fem_constraint_t* P);
+int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const double* dinv,
+                       const fem_constraint_t* P, const double* b, double* x, const fem_pcg_options_t* opt,
+                       void* ws, void* host_status, fem_pcg_result_t* result /* host */, void* stream);
 
 /* ---- smoothed-aggregation AMG (the amgx_solver option, solver.py:141-259,
  *      261-284) ------------------------------------------------------------

@@ -163,6 +163,29 @@ __global__ __launch_bounds__(256) void k_bi_resid(int64_t n, const double* __res
   }
 }
 
+// y = A x, scalar CSR, one thread per row
+__global__ __launch_bounds__(256) void k_csr_spmv(int64_t rows, const int32_t* __restrict__ ptr,
+                                                  const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                                  const double* __restrict__ x, double* __restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
+    double acc = 0.0;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) acc += val[k] * x[idx[k]];
+    y[i] = acc;
+  }
+}
+
+// reduced Jacobi: dr[j] = 1 / sum_k PT_jk^2 A_kk with A_kk = 1 / dinv[k]
+__global__ __launch_bounds__(256) void k_reduced_dinv(int64_t m, const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ idx,
+                                                      const double* __restrict__ val,
+                                                      const double* __restrict__ dinv, double* __restrict__ dr) {
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m; j += (int64_t)gridDim.x * blockDim.x) {
+    double d = 0.0;
+    for (int32_t k = ptr[j]; k < ptr[j + 1]; ++k) d += val[k] * val[k] / dinv[idx[k]];
+    dr[j] = d != 0.0 ? 1.0 / d : 1.0;
+  }
+}
+
 int read_scalars(const BiScalars* dev, void* host_status, hipStream_t s, BiScalars* out) {
   FEM_HIP(hipMemcpyAsync(host_status, dev, sizeof(BiScalars), hipMemcpyDeviceToHost, s));
   FEM_HIP(hipStreamSynchronize(s));
@@ -177,21 +200,38 @@ using namespace fem;
 
 extern "C" {
 
-int64_t fem_bicgstab_ws_bytes(const fem_pattern_t* pat) {
-  if (!pat) return 0;
-  const int64_t n = std::max<int64_t>(pat->num_rows * pat->vec, 1);
-  return (int64_t)(8 * align256(n * 8) + align256(2 * kBiBlocks * 8) + align256(sizeof(BiScalars)));
+int fem_csr_spmv(int64_t rows, const int32_t* ptr, const int32_t* idx, const double* val, const double* x, double* y,
+                 void* stream) {
+  FEM_CHECK_ARG(rows >= 0 && (rows == 0 || (ptr && x && y)), "fem_csr_spmv: bad arguments");
+  if (rows == 0) return FEM_OK;
+  hipLaunchKernelGGL(k_csr_spmv, dim3(grid_for(rows)), dim3(256), 0, as_stream(stream), rows, ptr, idx, val, x, y);
+  FEM_LAUNCHED();
+  return FEM_OK;
 }
 
-int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const double* dinv, const double* b, double* x,
-                       const fem_pcg_options_t* opt, void* ws, void* host_status, fem_pcg_result_t* result,
-                       void* stream) {
+int64_t fem_bicgstab_ws_bytes(const fem_pattern_t* pat, const fem_constraint_t* P) {
+  if (!pat) return 0;
+  const int64_t nf = std::max<int64_t>(pat->num_rows * pat->vec, 1);
+  const int64_t n = P ? std::max<int64_t>(P->cols, 1) : nf;
+  size_t b = 8 * align256(n * 8) + align256(2 * kBiBlocks * 8) + align256(sizeof(BiScalars));
+  if (P) b += 2 * align256(nf * 8) + align256(n * 8);
+  return (int64_t)b;
+}
+
+int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const double* dinv, const fem_constraint_t* P,
+                       const double* b, double* x, const fem_pcg_options_t* opt, void* ws, void* host_status,
+                       fem_pcg_result_t* result, void* stream) {
   FEM_CHECK_ARG(pat && vals && b && x && opt && ws && host_status && result, "fem_bicgstab_solve: NULL argument");
   FEM_CHECK_ARG(opt->precision == FEM_F64, "fem_bicgstab_solve: the matrix must be FP64 (standard form)");
   FEM_CHECK_ARG(!opt->precond || dinv, "fem_bicgstab_solve: Jacobi preconditioning needs dinv");
   FEM_CHECK_ARG(opt->maxiter >= 0 && opt->check_every >= 1, "fem_bicgstab_solve: bad options");
+  const int64_t nfull = pat->num_rows * pat->vec;
+  FEM_CHECK_ARG(!P || (P->rows == nfull && P->cols >= 0 && P->p_ptr && P->p_idx && P->p_val && P->pt_ptr &&
+                       P->pt_idx && P->pt_val),
+                "fem_bicgstab_solve: constraint does not match the pattern (%lld rows, %lld dofs)",
+                P ? (long long)P->rows : 0LL, (long long)nfull);
   hipStream_t s = as_stream(stream);
-  const int64_t n = pat->num_rows * pat->vec;
+  const int64_t n = P ? P->cols : nfull;
   char* w = static_cast<char*>(ws);
   auto take = [&](size_t bytes) {
     char* p = w;
@@ -208,11 +248,26 @@ int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const doubl
   double* t = (double*)take(n * 8);
   double* part = (double*)take(2 * kBiBlocks * 8);
   BiScalars* sc = (BiScalars*)take(sizeof(BiScalars));
+  double* full0 = P ? (double*)take(nfull * 8) : nullptr;
+  double* full1 = P ? (double*)take(nfull * 8) : nullptr;
+  double* dred = P ? (double*)take(n * 8) : nullptr;
   const double* Dinv = opt->precond ? dinv : nullptr;
   const unsigned g = grid_for(n);
   *result = fem_pcg_result_t{};
+  if (P && Dinv) {
+    hipLaunchKernelGGL(k_reduced_dinv, dim3(g), dim3(256), 0, s, n, P->pt_ptr, P->pt_idx, P->pt_val, Dinv, dred);
+    FEM_LAUNCHED();
+    Dinv = dred;
+  }
+  // y = A x, or P^T A P x for a constrained system
+  auto apply = [&](const double* in, double* out) -> int {
+    if (!P) return fem_spmv(pat, FEM_F64, FEM_FORM_STANDARD, vals, in, out, stream);
+    if (int rc = fem_csr_spmv(nfull, P->p_ptr, P->p_idx, P->p_val, in, full0, stream)) return rc;
+    if (int rc = fem_spmv(pat, FEM_F64, FEM_FORM_STANDARD, vals, full0, full1, stream)) return rc;
+    return fem_csr_spmv(n, P->pt_ptr, P->pt_idx, P->pt_val, full1, out, stream);
+  };
   // r = rhat = b - A x; p = v = 0
-  if (int rc = fem_spmv(pat, FEM_F64, FEM_FORM_STANDARD, vals, x, t, stream)) return rc;
+  if (int rc = apply(x, t)) return rc;
   hipLaunchKernelGGL(k_bi_resid, dim3(g), dim3(256), 0, s, n, b, t, r, rhat);
   hipLaunchKernelGGL(k_bi_zero2, dim3(g), dim3(256), 0, s, n, p, v);
   hipLaunchKernelGGL(k_bi_dot, dim3(kBiBlocks), dim3(256), 0, s, n, b, b, r, r, part, (const BiScalars*)nullptr);
@@ -225,13 +280,13 @@ int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const doubl
   while (!h.done) {
     hipLaunchKernelGGL(k_bi_p, dim3(g), dim3(256), 0, s, n, r, v, Dinv, p, ph, (const BiScalars*)sc);
     FEM_LAUNCHED();
-    if (int rc = fem_spmv(pat, FEM_F64, FEM_FORM_STANDARD, vals, ph, v, stream)) return rc;
+    if (int rc = apply(ph, v)) return rc;
     hipLaunchKernelGGL(k_bi_dot, dim3(kBiBlocks), dim3(256), 0, s, n, rhat, v, (const double*)nullptr,
                        (const double*)nullptr, part, (const BiScalars*)sc);
     hipLaunchKernelGGL(k_bi_scalar<1>, dim3(1), dim3(256), 0, s, part, kBiBlocks, sc, 0.0, 0.0, 0);
     hipLaunchKernelGGL(k_bi_s, dim3(g), dim3(256), 0, s, n, r, v, Dinv, sv, sh, (const BiScalars*)sc);
     FEM_LAUNCHED();
-    if (int rc = fem_spmv(pat, FEM_F64, FEM_FORM_STANDARD, vals, sh, t, stream)) return rc;
+    if (int rc = apply(sh, t)) return rc;
     hipLaunchKernelGGL(k_bi_dot, dim3(kBiBlocks), dim3(256), 0, s, n, t, sv, t, t, part, (const BiScalars*)sc);
     hipLaunchKernelGGL(k_bi_scalar<2>, dim3(1), dim3(256), 0, s, part, kBiBlocks, sc, 0.0, 0.0, 0);
     hipLaunchKernelGGL(k_bi_xr, dim3(g), dim3(256), 0, s, n, ph, sh, sv, t, x, r, (const BiScalars*)sc);
@@ -245,7 +300,7 @@ int fem_bicgstab_solve(const fem_pattern_t* pat, const double* vals, const doubl
     }
   }
   // true residual |b - A x|
-  if (int rc = fem_spmv(pat, FEM_F64, FEM_FORM_STANDARD, vals, x, t, stream)) return rc;
+  if (int rc = apply(x, t)) return rc;
   hipLaunchKernelGGL(k_bi_resid, dim3(g), dim3(256), 0, s, n, b, t, r, (double*)nullptr);
   hipLaunchKernelGGL(k_bi_dot, dim3(kBiBlocks), dim3(256), 0, s, n, r, r, (const double*)nullptr,
                      (const double*)nullptr, part, (const BiScalars*)nullptr);

@@ -202,6 +202,7 @@ class DeviceProblem:
         self.mass_dyn = None
         self.cell_terms = []
         self.dynamic_surfaces = set()
+        self._constraint = None
         self.host_status = torch.empty(128, dtype=torch.uint8).pin_memory()
         self._amg = None
         self._amg_uses = 0
@@ -641,7 +642,7 @@ class DeviceProblem:
         if cfg.get('method') == 'amg':
             return self.amg_solve(b, x0, cfg)
         if cfg.get('method') == 'bicgstab':
-            return self.bicgstab(b, x0, cfg)
+            return self.bicgstab(b, x0, cfg, constrained=cfg.get('constrained', False))
         precision, Ke, vals, dinv = self.current
         o = _lib.FemPcgOptions()
         o.rtol, o.atol, o.maxiter = cfg['tol'], cfg['atol'], cfg['maxiter']
@@ -661,18 +662,57 @@ class DeviceProblem:
                                    C.byref(res), stream_ptr()), 'fem_pcg_solve')
         return x, res
 
-    def bicgstab(self, b, x0, cfg):
-        """FP64 Jacobi-BiCGSTAB on the current FP64 tangent (jax_solve's method, solver.py:63-92)."""
+    def constraint(self):
+        """Device CSR of the problem's P_mat and its transpose (solver.py:545-551), or None."""
+        P = getattr(self.problem, 'P_mat', None)
+        if P is None:
+            return None
+        key = id(P)
+        if self._constraint is None or self._constraint[0] != key:
+            import scipy.sparse as sp
+            Pc = sp.csr_matrix(P)
+            if Pc.shape[0] != self.n:
+                raise ValueError(f"P_mat has {Pc.shape[0]} rows, the problem {self.n} dofs")
+            Pc.sort_indices()
+            Pt = Pc.T.tocsr()
+            Pt.sort_indices()
+            d = self.dm.device
+            arrs = [torch.as_tensor(a, device=d) for a in (Pc.indptr.astype(onp.int32), Pc.indices.astype(onp.int32),
+                                                            Pc.data.astype(onp.float64), Pt.indptr.astype(onp.int32),
+                                                            Pt.indices.astype(onp.int32), Pt.data.astype(onp.float64))]
+            c = _lib.FemConstraint()
+            c.rows, c.cols = Pc.shape
+            c.p_ptr, c.p_idx, c.p_val, c.pt_ptr, c.pt_idx, c.pt_val = [a.data_ptr() for a in arrs]
+            self._constraint = (key, c, arrs)
+        return self._constraint[1]
+
+    def p_apply(self, x_r, transpose=False):
+        """P x_r (full dofs), or P^T x with `transpose` (reduced dofs)."""
+        c = self.constraint()
+        arrs = self._constraint[2]
+        rows = c.cols if transpose else c.rows
+        ptr_, idx, val = (arrs[3], arrs[4], arrs[5]) if transpose else (arrs[0], arrs[1], arrs[2])
+        y = self.dm.empty(int(rows))
+        check(self.L.fem_csr_spmv(int(rows), ptr(ptr_), ptr(idx), ptr(val), ptr(x_r.contiguous()), ptr(y),
+                                  stream_ptr()), 'fem_csr_spmv')
+        return y
+
+    def bicgstab(self, b, x0, cfg, constrained=False):
+        """FP64 Jacobi-BiCGSTAB on the current FP64 tangent (jax_solve's method, solver.py:63-92);
+        with `constrained`, on the reduced P^T A P of get_A (solver.py:545-551)."""
         precision, _, vals, dinv = self.current
         o = _lib.FemPcgOptions()
         o.rtol, o.atol, o.maxiter = cfg['tol'], cfg['atol'], cfg['maxiter']
         o.check_every, o.precision, o.precond = cfg['check_every'], precision, 1 if cfg['precond'] else 0
-        if 'bicgstab_ws' not in self.buffers:
-            self.buffers['bicgstab_ws'] = self.dm.empty(int(self.L.fem_bicgstab_ws_bytes(self.pref)),
-                                                        dtype=torch.uint8)
+        cons = self.constraint() if constrained else None
+        cref = C.byref(cons) if cons is not None else None
+        nb = int(self.L.fem_bicgstab_ws_bytes(self.pref, cref))
+        if self.buffers.get('bicgstab_ws_n') != nb:
+            self.buffers['bicgstab_ws'] = self.dm.empty(nb, dtype=torch.uint8)
+            self.buffers['bicgstab_ws_n'] = nb
         x = x0.clone()
         res = _lib.FemPcgResult()
-        rc = self.L.fem_bicgstab_solve(self.pref, ptr(vals), ptr(dinv), ptr(b), ptr(x), C.byref(o),
+        rc = self.L.fem_bicgstab_solve(self.pref, ptr(vals), ptr(dinv), cref, ptr(b), ptr(x), C.byref(o),
                                        ptr(self.buffers['bicgstab_ws']), C.c_void_p(self.host_status.data_ptr()),
                                        C.byref(res), stream_ptr())
         if rc not in (_lib.FEM_OK, _lib.FEM_ERR_NOT_CONVERGED, _lib.FEM_ERR_NAN):

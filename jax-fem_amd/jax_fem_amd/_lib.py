@@ -51,6 +51,11 @@ class FemOperator(C.Structure):
                 ('cell_scratch', _vp), ('mass_M', _vp), ('cell_K', _vp)]
 
 
+class FemConstraint(C.Structure):
+    _fields_ = [('rows', _i64), ('cols', _i64), ('p_ptr', _vp), ('p_idx', _vp), ('p_val', _vp),
+                ('pt_ptr', _vp), ('pt_idx', _vp), ('pt_val', _vp)]
+
+
 class FemPcgOptions(C.Structure):
     _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32),
                 ('precision', _i32), ('precond', _i32), ('reliable_delta', _f64),
@@ -117,9 +122,10 @@ _SIGNATURES = {
     'fem_spmv': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp]),
     'fem_export_csr': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     'fem_operator_apply': (C.c_int, [_P(FemOperator), _vp, _vp, _vp]),
-    'fem_bicgstab_ws_bytes': (_i64, [_P(FemPattern)]),
-    'fem_bicgstab_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _P(FemPcgOptions), _vp, _vp,
-                                     _P(FemPcgResult), _vp]),
+    'fem_csr_spmv': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_bicgstab_ws_bytes': (_i64, [_P(FemPattern), _P(FemConstraint)]),
+    'fem_bicgstab_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemConstraint), _vp, _vp, _P(FemPcgOptions),
+                                     _vp, _vp, _P(FemPcgResult), _vp]),
     'fem_pcg_ws_bytes': (_i64, [_P(FemPattern), _i32]),
     'fem_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemOperator), _vp, _vp, _P(FemPcgOptions),
                                 _vp, _vp, _P(FemPcgResult), _vp]),

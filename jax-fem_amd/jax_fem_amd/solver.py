@@ -155,10 +155,10 @@ def _bc_increment(dev, dofs):
     return x0
 
 
-def _line_search(dev, dofs, inc):
+def _line_search(dev, dofs, inc, full=lambda v: v, reduce=lambda v: v):
     """Step halving on |R| (solver.py:424-463)."""
     def res_norm(alpha):
-        return dev.norm(dev.residual(dofs + alpha * inc))
+        return dev.norm(reduce(dev.residual(full(dofs + alpha * inc))))
     alpha = 1.
     rn = res_norm(alpha)
     for _ in range(3):
@@ -185,14 +185,28 @@ def solver(problem, solver_options={}):
     dev = problem.device()
     dev.prepare()
     timing = {'local_assembly': 0., 'global_matrix': 0., 'linear': 0.}
+    constrained = dev.constraint() is not None
+    if constrained:
+        # linear multipoint constraints u = P u_r (solver.py:1290-1339): Newton on the reduced dofs,
+        # the reduced system P^T A P solved by FP64 Jacobi-BiCGSTAB as get_A + jax_solve do
+        if lin.get('method') == 'amg':
+            raise NotImplementedError("amgx_solver with P_mat constraints is outside the device path")
+        lin = dict(lin, method='bicgstab', constrained=True, precision='fp64', precision_code=_lib.FEM_F64)
+        full = dev.p_apply
+        reduce = lambda r: dev.p_apply(r, transpose=True)
+    else:
+        full = reduce = lambda v: v
     if 'initial_guess' in cfg:
         dofs = problem._flat(cfg['initial_guess']).clone()
+        if constrained and dofs.numel() != dev.constraint().cols:
+            raise ValueError("with P_mat, initial_guess holds the reduced dofs")
     else:
-        dofs = torch.zeros(dev.n, dtype=torch.float64, device=dev.dm.device)
+        dofs = torch.zeros(dev.constraint().cols if constrained else dev.n, dtype=torch.float64,
+                           device=dev.dm.device)
     rel_tol = cfg.get('rel_tol', 1e-8)
     tol = cfg.get('tol', 1e-6)
 
-    res = dev.residual(dofs)
+    res = reduce(dev.residual(full(dofs)))
     res_val = dev.norm(res)
     res_val0 = res_val
     rel = res_val / res_val0 if res_val0 != 0 else float('nan')
@@ -202,19 +216,23 @@ def solver(problem, solver_options={}):
     while rel > rel_tol and res_val > tol:
         n_iters += 1
         t0 = time.perf_counter()
-        dev.tangent(dofs, lin['precision_code'], lin['assembly_code'])
+        u_full = full(dofs)
+        dev.tangent(u_full, lin['precision_code'], lin['assembly_code'])
         torch.cuda.synchronize()
         timing['global_matrix'] += time.perf_counter() - t0
         b = -res
-        x0 = _bc_increment(dev, dofs)
+        x0 = reduce(_bc_increment(dev, u_full))
         t0 = time.perf_counter()
-        inc, info = linear_solve(dev, dofs, b, x0, lin)
+        inc, info = linear_solve(dev, u_full, b, x0, lin)
         timing['linear'] += time.perf_counter() - t0
         stats.append(dict(iterations=info.iterations, replacements=info.replacements, res_norm=info.res_norm,
                           spmv_ms=info.spmv_ms, update_ms=info.update_ms))
-        dofs = _line_search(dev, dofs, inc) if cfg.get('line_search_flag', False) else dofs + inc
+        if cfg.get('line_search_flag', False):
+            dofs = _line_search(dev, dofs, inc, full, reduce)
+        else:
+            dofs = dofs + inc
         t0 = time.perf_counter()
-        res = dev.residual(dofs)
+        res = reduce(dev.residual(full(dofs)))
         res_val = dev.norm(res)
         timing['local_assembly'] += time.perf_counter() - t0
         rel = res_val / res_val0
@@ -222,6 +240,7 @@ def solver(problem, solver_options={}):
                     n_iters, res_val, rel, info.iterations)
     assert math.isfinite(res_val), "res_val contains NaN, stop the program!"
     assert math.isfinite(dev.dot(dofs, dofs)), "dofs contains NaN, stop the program!"
+    dofs = full(dofs)
     problem._last_u = dofs
     problem._solve_stats = dict(newton_iterations=n_iters, linear=stats, timing=timing,
                                 wall=time.perf_counter() - wall0, res_norm=res_val)
@@ -272,7 +291,14 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
     if dev.nonsymmetric:
         raise NotImplementedError("the adjoint of a tangent that is not symmetric (A^T lambda = v) is outside the "
                                   "device path")
-    lam, info = linear_solve(dev, u, v, torch.zeros_like(v), lin)
+    if dev.constraint() is not None:
+        # solver.py:1402-1412: v_r = P^T v, A_r lambda_r = v_r, lambda = P lambda_r
+        lin_c = dict(lin, method='bicgstab', constrained=True, precision='fp64', precision_code=_lib.FEM_F64)
+        v_r = dev.p_apply(v, transpose=True)
+        lam_r, info = linear_solve(dev, u, v_r, torch.zeros_like(v_r), lin_c)
+        lam = dev.p_apply(lam_r)
+    else:
+        lam, info = linear_solve(dev, u, v, torch.zeros_like(v), lin)
     problem._adjoint_stats = dict(iterations=info.iterations, replacements=info.replacements,
                                   res_norm=info.res_norm, spmv_ms=info.spmv_ms, update_ms=info.update_ms)
     outs, cots = [], []

@@ -495,6 +495,22 @@ class Problem:
         self.newton_its = its
         return dofs
 
+    def newton_constrained(self, P, tol=1e-6, rel_tol=1e-8):
+        """Newton on reduced dofs u = P u_r with the reduced system P^T A P (solver.py:1283-1339,
+        get_A 545-551, x0 404-406), direct solves."""
+        P = sp.csr_matrix(P)
+        d = np.zeros(P.shape[1])
+        res = P.T @ self.residual(P @ d)
+        r0 = rv = np.linalg.norm(res)
+        rel = rv / r0 if r0 else np.nan
+        while rel > rel_tol and rv > tol:
+            A = (P.T @ self.tangent(P @ d) @ P).tocsc()
+            d = d + spla.spsolve(A, -res)
+            res = P.T @ self.residual(P @ d)
+            rv = np.linalg.norm(res)
+            rel = rv / r0
+        return P @ d
+
     @staticmethod
     def linear_solve(A, b, x0, linear, tol=1e-10):
         if linear == 'direct':

@@ -372,3 +372,26 @@ def aniso_torch(u_grad):
 
 def aniso_np(G, theta=None):
     return G @ np.array(ANISO_K).T
+
+
+def periodic_p_mat(points, vec, pairs, tol=1e-5):
+    """Constraint matrix P (n x m) of periodic conditions, u = P u_r: each (side_a, side_b,
+    mapping, component) ties dof `component` of the side-b node at mapping(x_a) to the
+    side-a node at x_a (the construction of applications/periodic_bc/example.py:53-116)."""
+    import scipy.sparse as sp
+    n = len(points) * vec
+    target = np.arange(n)                 # dof -> the dof it copies
+    for side_a, side_b, mapping, comp in pairs:
+        ia = np.flatnonzero([side_a(x) for x in points])
+        ib = np.flatnonzero([side_b(x) for x in points])
+        for a in ia:
+            d = np.linalg.norm(points[ib] - mapping(points[a])[None, :], axis=1)
+            b = ib[np.argmin(d)]
+            assert d.min() < tol
+            target[b * vec + comp] = a * vec + comp
+    keep = np.flatnonzero(target == np.arange(n))
+    red = -np.ones(n, dtype=np.int64)
+    red[keep] = np.arange(len(keep))
+    cols = red[target]
+    assert (cols >= 0).all()
+    return sp.csr_matrix((np.ones(n), (np.arange(n), cols)), shape=(n, len(keep)))
