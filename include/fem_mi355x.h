@@ -454,11 +454,13 @@ int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, doub
  * diagonal entry (an empty coarse row) is first set to 1 in A. */
 int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream);
 /* Largest-eigenvalue estimate of D^-1 A by `steps` power steps from a fixed
- * start vector.  A is the standard-form FP64 SELL matrix (pat, sell_vals) if
- * pat != NULL, else the FP64 block CSR matrix A.  ws >=
- * fem_amg_lmax_ws_bytes(unknowns).  Synchronises the stream. */
+ * start vector.  A is the SELL matrix (pat, sell_vals) if pat != NULL --
+ * FP64 standard form, or FP32 difference form when sell_f32 != 0 -- else
+ * the FP64 block CSR matrix A.  ws >= fem_amg_lmax_ws_bytes(unknowns).
+ * Synchronises the stream. */
 int64_t fem_amg_lmax_ws_bytes(int64_t n);
-int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const double* sell_vals, const double* dinv,
+int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell_vals, int32_t sell_f32,
+                 const double* dinv,
                  int32_t steps, double* lmax, void* ws, void* stream);
 /* C = X Y for block CSR X (br x k) and Y (k x bc), FP64.  The structure is
  * computed once and kept as a plan, so a new matrix costs one numeric pass:

@@ -1639,8 +1639,8 @@ int64_t fem_amg_lmax_ws_bytes(int64_t n) {
   return (int64_t)(2 * align256(n * 8) + align256(kDotBlocks * 8) + 256);
 }
 
-int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const double* sell_vals, const double* dinv,
-                 int32_t steps, double* lmax, void* ws, void* stream) {
+int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell_vals, int32_t sell_f32,
+                 const double* dinv, int32_t steps, double* lmax, void* ws, void* stream) {
   FEM_CHECK_ARG(dinv && lmax && ws && steps >= 1, "fem_amg_lmax: bad arguments");
   FEM_CHECK_ARG(pat ? (sell_vals != nullptr && pat->num_rows > 0) : (A && A->br == A->bc && A->rows > 0),
                 "fem_amg_lmax: give a SELL matrix (pat, sell_vals) or a square block CSR matrix");
@@ -1663,8 +1663,11 @@ int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const double* se
   FEM_LAUNCHED();
   for (int k = 0; k < steps; ++k) {
     ApplyArgs a{x, y, nullptr, dinv, nullptr, 0.0, 0.0, nullptr};
-    if (int rc = pat ? sell_apply<AP_DSTORE, double>(*pat, sell_vals, a, nullptr, s)
-                     : bcsr_apply<AP_DSTORE, double>(*A, a, s))
+    if (int rc = !pat ? bcsr_apply<AP_DSTORE, double>(*A, a, s)
+                 : sell_f32 ? sell_apply<AP_DSTORE, float, true>(*pat, static_cast<const float*>(sell_vals), a,
+                                                                  nullptr, s)
+                            : sell_apply<AP_DSTORE, double>(*pat, static_cast<const double*>(sell_vals), a, nullptr,
+                                                            s))
       return rc;
     hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, y, y, part, (const int32_t*)nullptr);
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, nrm);

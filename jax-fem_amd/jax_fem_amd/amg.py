@@ -150,13 +150,15 @@ class DeviceAMG:
         return d
 
     def _lmax(self, A, dinv, sell_vals=None):
-        """Power estimate of lambda_max(D^-1 A); level 0 runs on the SELL matrix."""
+        """Power estimate of lambda_max(D^-1 A); level 0 runs on the SELL matrix (the FP32
+        difference-form copy for an FP32 hierarchy, which halves the bytes of the 15 products)."""
         s = A.struct()
         ws = self._empty(self.L.fem_amg_lmax_ws_bytes(A.rows * A.br), torch.uint8)
         lam = C.c_double(0.0)
         pat = self.dev.pref if sell_vals is not None else None
-        check(self.L.fem_amg_lmax(C.byref(s), pat, ptr(sell_vals), ptr(dinv), POWER_STEPS, C.byref(lam), ptr(ws),
-                                  stream_ptr()), 'fem_amg_lmax')
+        f32 = 1 if (sell_vals is not None and sell_vals.dtype == torch.float32) else 0
+        check(self.L.fem_amg_lmax(C.byref(s), pat, ptr(sell_vals), f32, ptr(dinv), POWER_STEPS, C.byref(lam),
+                                  ptr(ws), stream_ptr()), 'fem_amg_lmax')
         return float(lam.value)
 
     def _sell_diff32(self, vals64):
@@ -237,10 +239,11 @@ class DeviceAMG:
         levels, coarse = self._structure()
         A0 = levels[0]['A'] if levels else coarse   # no levels: the matrix is the coarsest level
         check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), stream_ptr()), 'fem_amg_sell_to_bcsr')
+        sv32 = self._sell_diff32(vals64) if (precision == _lib.FEM_F32 and levels) else None
         for l, lv in enumerate(levels):
             A = lv['A']
             lv['dinv'] = self._dinv(A, fix=l > 0)
-            lv['lam'] = self._lmax(A, lv['dinv'], vals64 if l == 0 else None)
+            lv['lam'] = self._lmax(A, lv['dinv'], (vals64 if sv32 is None else sv32) if l == 0 else None)
             omega = 4.0 / (3.0 * lv['lam'])
             plan_w0, plan_w, plan_c = lv['plans']
             P = lv['P']
@@ -262,10 +265,10 @@ class DeviceAMG:
         check(L.fem_amg_coarse_inverse(C.byref(cs), ptr(self.coarse_inv), ptr(self._gj_ws), stream_ptr()),
               'fem_amg_coarse_inverse')
         self.levels, self.coarse = levels, coarse
-        self._descriptor(vals64, precision, degree)
+        self._descriptor(vals64, precision, degree, sv32)
         return self
 
-    def _descriptor(self, vals64, precision, degree):
+    def _descriptor(self, vals64, precision, degree, sv32=None):
         dev = self.dev
         f32 = precision == _lib.FEM_F32
         keep = []
@@ -275,7 +278,7 @@ class DeviceAMG:
             e = arr[l]
             if l == 0:
                 e.pat = C.pointer(dev.pat)
-                sv = self._sell_diff32(vals64) if f32 else vals64
+                sv = (sv32 if sv32 is not None else self._sell_diff32(vals64)) if f32 else vals64
                 keep.append(sv)
                 e.sell_vals = sv.data_ptr()
                 self.vals32 = sv if f32 else None
