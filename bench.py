@@ -13,6 +13,8 @@ problem ``solver`` performs exactly one.  Four linear-solve paths are timed:
   amg_mixed  smoothed-aggregation AMG-CG ('amgx_solver', precision 'mixed'):
              FP64 outer CG on the FP64 matrix, V-cycle matrices stored in FP32
              (FP64 arithmetic); hierarchy setup inside the step.  ``value``.
+  amg_mixed_l0fp16  the same with the level-0 smoother matrix stored as FP16
+             D^-1 A (option level0='fp16'; reported, not the value).
   amg_fp64   the same with an FP64 hierarchy.
   mixed      Jacobi-PCG ('jax_solver', 'mixed'): FP32 matrix and PCG, FP64
              iterate and FP64 residual replacement.
@@ -135,6 +137,7 @@ def run(args):
     # FP32-storage ("mixed") and the FP64 variant; the AMG mixed path is the headline value.
     opts = {
         'amg_mixed': {'amgx_solver': {'precision': 'mixed'}},
+        'amg_mixed_l0fp16': {'amgx_solver': {'precision': 'mixed', 'level0': 'fp16'}},
         'amg_fp64': {'amgx_solver': {'precision': 'fp64'}},
         'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}},
         'fp64': {'jax_solver': {'precision': 'fp64'}},
@@ -246,6 +249,13 @@ def run(args):
                      "traffic": amg_traffic.get('k_sell_apply_cheb_f32'),
                      "traffic_source": os.path.relpath(AMG_TRAFFIC_PROFILE, ROOT) if amg_traffic else None,
                      "algorithmic_bytes": cb, "avg_launch_ms": head['spmv_ms']},
+        "level0_fp16": {"kernel": "k_sell_apply<3,CHEB,_Float16,true> (level-0 Chebyshev step on FP16 D^-1 A)",
+                        "s_per_newton_step": res['amg_mixed_l0fp16']['s_per_newton_step'],
+                        "iterations": res['amg_mixed_l0fp16']['iterations'],
+                        "avg_launch_ms": res['amg_mixed_l0fp16']['spmv_ms'],
+                        "algorithmic_bytes": cheb_bytes(dev, 2),
+                        "achieved_GBs": (cheb_bytes(dev, 2) / (res['amg_mixed_l0fp16']['spmv_ms'] * 1e-3) / 1e9
+                                         if res['amg_mixed_l0fp16']['spmv_ms'] else None)},
         "pcg_spmv": {"kernel": "k_pcg_a<3,true,float> (Jacobi-PCG SpMV q = Ap, FP32 storage)",
                      "achieved_GBs": pcg_achieved, "frac": pcg_achieved / PEAK_HBM_GBS if pcg_achieved else None,
                      "avg_launch_ms": res['mixed']['spmv_ms'], "algorithmic_bytes": spmv_bytes,
@@ -331,7 +341,8 @@ def _paths(args):
     if args.solver == 'amg':
         r = args.setup_reuse
         return {'fp64': {'amgx_solver': {'precision': 'fp64', 'setup_reuse': r}},
-                'mixed': {'amgx_solver': {'precision': 'mixed', 'setup_reuse': r, 'outer': args.amg_outer}}}
+                'mixed': {'amgx_solver': {'precision': 'mixed', 'setup_reuse': r, 'outer': args.amg_outer,
+                                          'level0': args.amg_level0}}}
     return {'fp64': {'jax_solver': {'precision': 'fp64'}},
             'mixed': {'jax_solver': {'precision': 'mixed', 'assembly': args.mixed_assembly}}}
 
@@ -773,6 +784,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--cells', type=int, nargs=3, default=[128, 64, 64])
     ap.add_argument('--mixed-assembly', choices=['fp32', 'fp64', 'bf16x3'], default='fp64')
+    ap.add_argument('--amg-level0', choices=['fp32', 'fp16'], default='fp32',
+                    help="AMG mixed path: storage of the level-0 smoother matrix")
     ap.add_argument('--amg-outer', choices=['fp64', 'fp32'], default='fp64',
                     help="with --solver amg: operator of the mixed path's outer CG (fp32: difference form + "
                          "FP64 residual replacement)")
@@ -781,7 +794,7 @@ def main():
     ap.add_argument('--solver', choices=['pcg', 'amg'], default='pcg',
                     help="pcg: Jacobi-PCG ('jax_solver'); amg: smoothed-aggregation AMG-CG ('amgx_solver')")
     ap.add_argument('--no-cpu-baseline', dest='cpu_baseline', action='store_false')
-    ap.add_argument('--only', choices=['fp64', 'mixed', 'amg_fp64', 'amg_mixed'], default=None,
+    ap.add_argument('--only', choices=['fp64', 'mixed', 'amg_fp64', 'amg_mixed', 'amg_mixed_l0fp16'], default=None,
                     help='time one path only (no adjoint, no summary line); for profiling')
     ap.add_argument('--nh-cells', type=int, default=64)
     ap.add_argument('--tabulated', action='store_true',

@@ -60,9 +60,12 @@ enum { FEM_F32 = 0, FEM_F64 = 1,            /* arithmetic precision          */
 enum { FEM_HEX8 = 8, FEM_HEX27 = 27 };       /* = nodes per cell              */
 enum {                                       /* storage form of the matrix    */
   FEM_FORM_STANDARD = 0,      /* block (i, j) holds A_ij                      */
-  FEM_FORM_DIFFERENCE = 1     /* diagonal slot holds R_i = sum_j A_ij and the
+  FEM_FORM_DIFFERENCE = 1,    /* diagonal slot holds R_i = sum_j A_ij and the
                                  product is y_i = sum_{j!=i} A_ij (x_j - x_i)
                                  + R_i x_i (exact on translations)            */
+  FEM_FORM_SCALED_DIFF16 = 2  /* AMG level 0 only: FP16 values of D^-1 A (rows
+                                 scaled by the point-Jacobi inverse diagonal)
+                                 in difference form                          */
 };
 enum {                                       /* constitutive laws (tensor_map)*/
   FEM_MAT_POISSON = 1,        /* P = c0 * grad u                      (vec 1) */
@@ -402,7 +405,9 @@ typedef struct {
                                     arithmetic are FP64                       */
   int32_t        degree;         /* Chebyshev degree of pre/post smoothing    */
   int32_t        level0_form;    /* form of the level-0 SELL values: FEM_FORM_
-                                    STANDARD, or FEM_FORM_DIFFERENCE (FP32)  */
+                                    STANDARD, FEM_FORM_DIFFERENCE (FP32), or
+                                    FEM_FORM_SCALED_DIFF16 (FP16 D^-1 A, with
+                                    the level's dinv; FP32 hierarchy)        */
   const fem_amg_level_t* levels; /* host array [num_levels]                   */
   int64_t        coarse_n;       /* unknowns of the coarsest level            */
   const double*  coarse_inv;     /* (coarse_n x coarse_n) dense inverse       */
@@ -455,9 +460,9 @@ int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, doub
 int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream);
 /* Largest-eigenvalue estimate of D^-1 A by `steps` power steps from a fixed
  * start vector.  A is the SELL matrix (pat, sell_vals) if pat != NULL --
- * FP64 standard form, or FP32 difference form when sell_f32 != 0 -- else
- * the FP64 block CSR matrix A.  ws >= fem_amg_lmax_ws_bytes(unknowns).
- * Synchronises the stream. */
+ * sell_f32 = 0: FP64 standard form, 1: FP32 difference form, 2: FP16 scaled
+ * difference form (FEM_FORM_SCALED_DIFF16) -- else the FP64 block CSR matrix
+ * A.  ws >= fem_amg_lmax_ws_bytes(unknowns).  Synchronises the stream. */
 int64_t fem_amg_lmax_ws_bytes(int64_t n);
 int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell_vals, int32_t sell_f32,
                  const double* dinv,
@@ -498,6 +503,11 @@ int fem_bcsr_transpose_values(const fem_bcsr_t* X, fem_bcsr_t* T, const int32_t*
 /* Dense inverse of a FP64 block CSR matrix (n = rows*br <= 4096) by
  * Gauss-Jordan without pivoting (SPD); ws >= 8*n*n bytes. */
 int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* stream);
+/* FP16 scaled difference-form copy of a standard-form FP64 SELL matrix:
+ * value (a, b) of block (i, j) becomes dinv[i*vec + a] * A_ij[a][b], the
+ * diagonal slot the scaled row-sum block (FEM_FORM_SCALED_DIFF16). */
+int fem_amg_sell_to_f16_scaled(const fem_pattern_t* pat, const double* sell_vals, const double* dinv, void* out,
+                               void* stream);
 /* out[i] = (float)in[i]. */
 int fem_amg_to_f32(int64_t n, const double* in, float* out, void* stream);
 /* z = M r, one V-cycle; ws >= fem_amg_ws_bytes(amg). */

@@ -289,8 +289,12 @@ __global__ void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv,
 // gets dinv = 0, i.e. no smoothing on it.
 // FP32 difference-form copy of a standard-form FP64 SELL matrix: off-diagonal
 // blocks rounded, the diagonal slot holds the FP64 row sum rounded once.
-template <int V>
-__global__ void k_sell_to_diff32(fem_pattern_t pat, const double* __restrict__ sv, float* __restrict__ out) {
+// Difference-form copy of a standard-form SELL matrix in TO: off-diagonal blocks
+// rounded from FP64, the diagonal slot the FP64 row sum rounded once; with
+// dinv, each block row a is scaled by dinv[row*V + a] first.
+template <int V, typename TO>
+__device__ inline void k_sell_to_diff_body(const fem_pattern_t& pat, const double* __restrict__ sv,
+                                           const double* __restrict__ dinv, TO* __restrict__ out) {
   constexpr int VV = V * V;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= pat.num_slices * kSlice) return;
@@ -301,20 +305,35 @@ __global__ void k_sell_to_diff32(fem_pattern_t pat, const double* __restrict__ s
   const int64_t row = pat.sell_row[t];
   const int ds = row >= 0 ? pat.diag_slot[row] : -1;
   const double* vp = sv + (size_t)base * VV + lane;
-  float* op = out + (size_t)base * VV + lane;
+  TO* op = out + (size_t)base * VV + lane;
+  double sc[V];
+#pragma unroll
+  for (int a = 0; a < V; ++a) sc[a] = (dinv && row >= 0) ? dinv[row * V + a] : 1.0;
   double rs[VV];
 #pragma unroll
   for (int v = 0; v < VV; ++v) rs[v] = 0.0;
   for (int k = 0; k < L; ++k)
 #pragma unroll
     for (int v = 0; v < VV; ++v) {
-      const double x = vp[((size_t)k * VV + v) * kSlice];
+      const double x = vp[((size_t)k * VV + v) * kSlice] * sc[v / V];
       rs[v] += x;
-      if (k != ds) op[((size_t)k * VV + v) * kSlice] = (float)x;
+      if (k != ds) op[((size_t)k * VV + v) * kSlice] = (TO)x;
     }
   if (ds >= 0)
 #pragma unroll
-    for (int v = 0; v < VV; ++v) op[((size_t)ds * VV + v) * kSlice] = (float)rs[v];
+    for (int v = 0; v < VV; ++v) op[((size_t)ds * VV + v) * kSlice] = (TO)rs[v];
+}
+
+template <int V>
+__global__ void k_sell_to_diff32(fem_pattern_t pat, const double* __restrict__ sv, float* __restrict__ out) {
+  k_sell_to_diff_body<V, float>(pat, sv, nullptr, out);
+}
+
+// FP16 copy of D^-1 A in difference form (rows scaled by dinv before the rounding)
+template <int V>
+__global__ void k_sell_to_f16_scaled(fem_pattern_t pat, const double* __restrict__ sv, const double* __restrict__ dinv,
+                                     _Float16* __restrict__ out) {
+  k_sell_to_diff_body<V, _Float16>(pat, sv, dinv, out);
 }
 
 template <int BS>
@@ -1019,13 +1038,15 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   // the epilogue's own-row operands are loaded before the matrix stream, so
   // their latency overlaps it (wave 0 finishes the rows)
   constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
+  // FP16 values hold D^-1 A (rows scaled by dinv): the epilogues undo or skip the scaling
+  constexpr bool kScaled = sizeof(T) == 2;
   double e_rhs[V], e_dinv[V], e_d[V], e_x[V];
   if (w == 0 && real) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const int64_t idx = row * V + i;
       e_rhs[i] = kRhs ? a.rhs[idx] : 0.0;
-      e_dinv[i] = (MODE == AP_CHEB || MODE == AP_DSTORE) ? a.dinv[idx] : 0.0;
+      e_dinv[i] = (MODE == AP_CHEB || MODE == AP_DSTORE || (kScaled && MODE == AP_RESID)) ? a.dinv[idx] : 0.0;
       e_d[i] = (MODE == AP_CHEB && a.c1 != 0.0) ? a.d[idx] : 0.0;
       e_x[i] = (MODE == AP_CHEB || MODE == AP_PQ) ? a.x[idx] : 0.0;
     }
@@ -1046,14 +1067,14 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       for (int o = 1; o < W; ++o) t += red[o][lane][i];
       const int64_t idx = row * V + i;
       if (MODE == AP_CHEB) {
-        const double r = e_dinv[i] * (e_rhs[i] - t);
+        const double r = kScaled ? fma(e_dinv[i], e_rhs[i], -t) : e_dinv[i] * (e_rhs[i] - t);
         const double dn = a.c1 != 0.0 ? fma(a.c1, e_d[i], a.c2 * r) : a.c2 * r;
         a.d[idx] = dn;
         a.y[idx] = e_x[i] + dn;
       } else if (MODE == AP_RESID) {
-        a.y[idx] = e_rhs[i] - t;
+        a.y[idx] = kScaled ? e_rhs[i] - t / e_dinv[i] : e_rhs[i] - t;
       } else if (MODE == AP_DSTORE) {
-        a.y[idx] = e_dinv[i] * t;
+        a.y[idx] = kScaled ? t : e_dinv[i] * t;
       } else {
         apply_epilogue<MODE>(a, idx, t);
       }
@@ -1278,6 +1299,12 @@ template <typename T>
 int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgs& a, hipStream_t s) {
   const fem_amg_level_t& L = amg.levels[l];
   if (l == 0) {
+    if (amg.level0_form == FEM_FORM_SCALED_DIFF16) {
+      ApplyArgs b = a;
+      b.dinv = L.dinv;   // the residual undoes the row scaling
+      if (mode == AP_CHEB) return sell_apply<AP_CHEB, _Float16, true>(*L.pat, L.sell_vals, b, nullptr, s);
+      return sell_apply<AP_RESID, _Float16, true>(*L.pat, L.sell_vals, b, nullptr, s);
+    }
     if (amg.level0_form == FEM_FORM_DIFFERENCE) {
       if (mode == AP_CHEB) return sell_apply<AP_CHEB, T, true>(*L.pat, L.sell_vals, a, nullptr, s);
       return sell_apply<AP_RESID, T, true>(*L.pat, L.sell_vals, a, nullptr, s);
@@ -1385,7 +1412,8 @@ int check_amg(const fem_amg_t* amg) {
   FEM_CHECK_ARG(amg->precision == FEM_F32 || amg->precision == FEM_F64, "AMG precision must be FEM_F32 or FEM_F64");
   FEM_CHECK_ARG(amg->degree >= 1 && amg->degree <= 8, "AMG Chebyshev degree must be in [1, 8]");
   FEM_CHECK_ARG(amg->level0_form == FEM_FORM_STANDARD ||
-                    (amg->level0_form == FEM_FORM_DIFFERENCE && amg->precision == FEM_F32),
+                    ((amg->level0_form == FEM_FORM_DIFFERENCE || amg->level0_form == FEM_FORM_SCALED_DIFF16) &&
+                     amg->precision == FEM_F32),
                 "AMG level 0: the difference form is an FP32 storage form");
   if (amg->num_levels == 0) return FEM_OK;   // the dense coarse inverse is the whole preconditioner
   const fem_amg_level_t& L0 = amg->levels[0];
@@ -1603,6 +1631,23 @@ int fem_amg_sell_to_f32_diff(const fem_pattern_t* pat, const double* sell_vals, 
   return FEM_OK;
 }
 
+int fem_amg_sell_to_f16_scaled(const fem_pattern_t* pat, const double* sell_vals, const double* dinv, void* out,
+                               void* stream) {
+  FEM_CHECK_ARG(pat && sell_vals && dinv && out, "fem_amg_sell_to_f16_scaled: NULL argument");
+  hipStream_t s = as_stream(stream);
+  if (pat->num_slices == 0) return FEM_OK;
+  const unsigned g = (unsigned)ceil_div(pat->num_slices * kSlice, 256);
+  _Float16* o = static_cast<_Float16*>(out);
+  if (pat->vec == 3)
+    hipLaunchKernelGGL(k_sell_to_f16_scaled<3>, dim3(g), dim3(256), 0, s, *pat, sell_vals, dinv, o);
+  else if (pat->vec == 1)
+    hipLaunchKernelGGL(k_sell_to_f16_scaled<1>, dim3(g), dim3(256), 0, s, *pat, sell_vals, dinv, o);
+  else
+    FEM_CHECK_ARG(false, "fem_amg_sell_to_f16_scaled: vec must be 1 or 3");
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
 int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream) {
   FEM_CHECK_ARG(A && dinv && A->br == A->bc && (A->rows == 0 || (A->row_ptr && A->col_idx && A->vals)),
                 "fem_amg_dinv: bad arguments");
@@ -1664,6 +1709,7 @@ int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell
   for (int k = 0; k < steps; ++k) {
     ApplyArgs a{x, y, nullptr, dinv, nullptr, 0.0, 0.0, nullptr};
     if (int rc = !pat ? bcsr_apply<AP_DSTORE, double>(*A, a, s)
+                 : sell_f32 == 2 ? sell_apply<AP_DSTORE, _Float16, true>(*pat, sell_vals, a, nullptr, s)
                  : sell_f32 ? sell_apply<AP_DSTORE, float, true>(*pat, static_cast<const float*>(sell_vals), a,
                                                                   nullptr, s)
                             : sell_apply<AP_DSTORE, double>(*pat, static_cast<const double*>(sell_vals), a, nullptr,

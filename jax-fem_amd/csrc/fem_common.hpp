@@ -117,10 +117,11 @@ __device__ inline void sell_slice_partial(const fem_pattern_t& pat, const T* __r
 }
 
 // Slice split chosen by a sweep of (W, U) on the 1.6M-dof HEX8 elasticity
-// matrix: FP32 4 waves x unroll 2, FP64 2 waves x unroll 2.
+// matrix: FP32 4 waves x unroll 2, FP64 2 waves x unroll 2 (FP16 storage
+// takes the FP32 split).
 template <typename T>
 struct SliceCfg {
-  static constexpr int W = sizeof(T) == 4 ? 4 : 2;
+  static constexpr int W = sizeof(T) <= 4 ? 4 : 2;
   static constexpr int U = 2;
   static constexpr int NT = 64 * W;
 };

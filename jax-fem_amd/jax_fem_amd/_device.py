@@ -740,7 +740,7 @@ class DeviceProblem:
         from .amg import DeviceAMG
         if self._amg is None:
             self._amg = DeviceAMG(self)
-        return self._amg.setup(vals, cfg['amg_precision'], cfg['degree'])
+        return self._amg.setup(vals, cfg['amg_precision'], cfg['degree'], cfg.get('level0', 'fp32'))
 
     def amg_solve(self, b, x0, cfg):
         """AMG-CG on the current FP64 matrix.  With cfg['setup_reuse'] = k > 0 the
@@ -751,6 +751,7 @@ class DeviceProblem:
         amg = self._amg
         reuse = (cfg.get('setup_reuse', 0) > 0 and amg is not None and amg.amg is not None
                  and amg.precision == cfg['amg_precision'] and self._amg_uses < cfg['setup_reuse']
+                 and amg.level0_f16 == (cfg.get('level0', 'fp32') == 'fp16')
                  and amg.structure_current())
         if reuse:
             self._amg_uses += 1

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
 
 FEM_OK, FEM_ERR_BAD_ARG, FEM_ERR_NOT_CONVERGED, FEM_ERR_NAN, FEM_ERR_HIP, FEM_ERR_CAPACITY = 0, -1, -2, -3, -4, -5
 FEM_F32, FEM_F64, FEM_BF16X3 = 0, 1, 2
-FEM_FORM_STANDARD, FEM_FORM_DIFFERENCE = 0, 1
+FEM_FORM_STANDARD, FEM_FORM_DIFFERENCE, FEM_FORM_SCALED_DIFF16 = 0, 1, 2
 FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN, FEM_MAT_J2_PLASTIC = 1, 2, 3, 4, 5
 FEM_MAT_TABULATED = 6
 
@@ -150,6 +150,7 @@ _SIGNATURES = {
     'fem_bcsr_transpose_values': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp]),
     'fem_amg_coarse_inverse': (C.c_int, [_P(FemBcsr), _vp, _vp, _vp]),
     'fem_amg_to_f32': (C.c_int, [_i64, _vp, _vp, _vp]),
+    'fem_amg_sell_to_f16_scaled': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp]),
     'fem_amg_ws_bytes': (_i64, [_P(FemAmg)]),
     'fem_amg_vcycle': (C.c_int, [_P(FemAmg), _vp, _vp, _vp, _vp]),
     'fem_amg_solve_ws_bytes': (_i64, [_P(FemPattern), _P(FemAmg)]),

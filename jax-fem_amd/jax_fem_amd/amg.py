@@ -72,6 +72,7 @@ class DeviceAMG:
         self._diff_fresh = False
         self.levels = []
         self.amg = None
+        self.level0_f16 = False
 
     # -------------------------------------------------------------- helpers
     def _empty(self, n, dtype=torch.float64):
@@ -156,7 +157,7 @@ class DeviceAMG:
         ws = self._empty(self.L.fem_amg_lmax_ws_bytes(A.rows * A.br), torch.uint8)
         lam = C.c_double(0.0)
         pat = self.dev.pref if sell_vals is not None else None
-        f32 = 1 if (sell_vals is not None and sell_vals.dtype == torch.float32) else 0
+        f32 = 0 if sell_vals is None else {torch.float32: 1, torch.float16: 2}.get(sell_vals.dtype, 0)
         check(self.L.fem_amg_lmax(C.byref(s), pat, ptr(sell_vals), f32, ptr(dinv), POWER_STEPS, C.byref(lam),
                                   ptr(ws), stream_ptr()), 'fem_amg_lmax')
         return float(lam.value)
@@ -171,6 +172,18 @@ class DeviceAMG:
             self._f32['sell'] = out
         check(self.L.fem_amg_sell_to_f32_diff(dev.pref, ptr(vals64), ptr(out), stream_ptr()),
               'fem_amg_sell_to_f32_diff')
+        return out
+
+    def _sell_f16(self, vals64, dinv):
+        """FP16 copy of D^-1 A in difference form (the level-0 smoother matrix of level0='fp16')."""
+        dev = self.dev
+        n = dev.sell_blocks * dev.V * dev.V
+        out = self._f32.get('sell16')
+        if out is None or out.numel() != n:
+            out = torch.empty(n, dtype=torch.float16, device=vals64.device)
+            self._f32['sell16'] = out
+        check(self.L.fem_amg_sell_to_f16_scaled(dev.pref, ptr(vals64), ptr(dinv), ptr(out), stream_ptr()),
+              'fem_amg_sell_to_f16_scaled')
         return out
 
     def _to_f32(self, t, key):
@@ -233,16 +246,21 @@ class DeviceAMG:
         self._struct_src = (dev.pat_tensors['row_ptr'], dev.bc_dofs)
         return self._struct
 
-    def setup(self, vals64, precision=_lib.FEM_F64, degree=DEGREE):
-        """Hierarchy of the current FP64 matrix (standard-form SELL values `vals64`)."""
+    def setup(self, vals64, precision=_lib.FEM_F64, degree=DEGREE, level0='fp32'):
+        """Hierarchy of the current FP64 matrix (standard-form SELL values `vals64`).
+        level0 ('fp32' or 'fp16', FP32 hierarchies only): storage of the level-0
+        smoother matrix, FP32 difference form or FP16 D^-1 A in difference form."""
         dev, L = self.dev, self.L
         levels, coarse = self._structure()
         A0 = levels[0]['A'] if levels else coarse   # no levels: the matrix is the coarsest level
         check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), stream_ptr()), 'fem_amg_sell_to_bcsr')
-        sv32 = self._sell_diff32(vals64) if (precision == _lib.FEM_F32 and levels) else None
+        f16 = level0 == 'fp16' and precision == _lib.FEM_F32 and bool(levels)
+        sv32 = self._sell_diff32(vals64) if (precision == _lib.FEM_F32 and levels and not f16) else None
         for l, lv in enumerate(levels):
             A = lv['A']
             lv['dinv'] = self._dinv(A, fix=l > 0)
+            if l == 0 and f16:
+                sv32 = self._sell_f16(vals64, lv['dinv'])
             lv['lam'] = self._lmax(A, lv['dinv'], (vals64 if sv32 is None else sv32) if l == 0 else None)
             omega = 4.0 / (3.0 * lv['lam'])
             plan_w0, plan_w, plan_c = lv['plans']
@@ -294,7 +312,9 @@ class DeviceAMG:
         amg.num_levels = len(self.levels)
         amg.precision = _lib.FEM_F32 if f32 else _lib.FEM_F64
         amg.degree = degree
-        amg.level0_form = _lib.FEM_FORM_DIFFERENCE if (f32 and self.levels) else _lib.FEM_FORM_STANDARD
+        self.level0_f16 = sv32 is not None and sv32.dtype == torch.float16
+        amg.level0_form = (_lib.FEM_FORM_SCALED_DIFF16 if self.level0_f16 else _lib.FEM_FORM_DIFFERENCE) \
+            if (f32 and self.levels) else _lib.FEM_FORM_STANDARD
         amg.levels = arr if self.levels else None
         amg.coarse_n = self.coarse.rows * self.coarse.br
         amg.coarse_inv = self.coarse_inv.data_ptr()
@@ -328,11 +348,15 @@ class DeviceAMG:
             dev.buffers['amg_ws'] = self._empty(nb, torch.uint8)
             dev.buffers['amg_ws_key'] = key
         if self.vals32 is not None and not self._diff_fresh:
-            self._sell_diff32(vals64)   # a reused hierarchy: level 0 and the outer operator follow the matrix
+            # a reused hierarchy: level 0 and the outer operator follow the matrix
+            if self.level0_f16:
+                self._sell_f16(vals64, self.levels[0]['dinv'])
+            else:
+                self._sell_diff32(vals64)
         self._diff_fresh = False
         x = x0.clone()
         res = _lib.FemPcgResult()
-        outer32 = self.vals32 if cfg.get('outer', 'fp64') == 'fp32' else None
+        outer32 = self.vals32 if (cfg.get('outer', 'fp64') == 'fp32' and not self.level0_f16) else None
         check(self.L.fem_amg_pcg_solve(dev.pref, ptr(vals64), ptr(outer32), C.byref(self.amg), ptr(b), ptr(x),
                                        C.byref(o),
                                        ptr(dev.buffers['amg_ws']), C.c_void_p(dev.host_status.data_ptr()),

@@ -41,7 +41,7 @@ _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32
 # (Newton steps).  From an AMGX config file only solver.tolerance and
 # solver.max_iters are taken (as atol and maxiter, AMGX's ABSOLUTE convergence).
 AMG_SOLVER_DEFAULTS = dict(cfg_path=None, precision='fp64', tol=1e-10, atol=1e-10, maxiter=1000, check_every=1,
-                           degree=2, timing=False, setup_reuse=0, outer='fp64', reliable_delta=0.01)
+                           degree=2, timing=False, setup_reuse=0, outer='fp64', reliable_delta=0.01, level0='fp32')
 
 
 def _resolve_solver_options(solver_options):
@@ -111,6 +111,9 @@ def _amg_config(user):
         raise ValueError(f"amgx_solver precision must be 'fp64' or 'mixed', got {cfg['precision']!r}")
     if cfg['outer'] not in ('fp64', 'fp32') or (cfg['outer'] == 'fp32' and cfg['precision'] != 'mixed'):
         raise ValueError("amgx_solver outer must be 'fp64', or 'fp32' with precision 'mixed'")
+    if cfg['level0'] not in ('fp32', 'fp16') or (cfg['level0'] == 'fp16' and
+                                                (cfg['precision'] != 'mixed' or cfg['outer'] != 'fp64')):
+        raise ValueError("amgx_solver level0 must be 'fp32', or 'fp16' with precision 'mixed' and outer 'fp64'")
     cfg['method'] = 'amg'
     cfg['amg_precision'] = _lib.FEM_F32 if cfg['precision'] == 'mixed' else _lib.FEM_F64
     cfg['precision_code'] = _lib.FEM_F64      # the outer CG operator is the FP64 matrix

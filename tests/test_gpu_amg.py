@@ -220,3 +220,28 @@ def test_amg_small_problem_dense_coarse_solve(gpu, precision):
     x = sol.reshape(-1).cpu().numpy()
     assert np.linalg.norm(x - ref) <= 1e-10 * np.linalg.norm(ref)
     assert p._solve_stats['linear'][0]['iterations'] <= 2
+
+
+def test_amg_level0_fp16_scaled_smoother(gpu):
+    # level0='fp16': the level-0 smoother reads FP16 D^-1 A in difference form; the cycle
+    # differs from the FP32 one by the FP16 rounding, the solve reaches the same solution
+    p = _elastic(16)
+    dev, A, (gp, gi), mask, pts, b = _assembled(p)
+    amg32 = dev.amg_setup(_cfg('mixed'))
+    r = torch.as_tensor(np.random.default_rng(12).standard_normal(dev.n), device=dev.dm.device)
+    r[torch.as_tensor(mask, device=r.device)] = 0.
+    z32 = amg32.vcycle(r).clone()
+    lam32 = amg32.levels[0]['lam']
+    amg16 = dev.amg_setup(_cfg('mixed', level0='fp16'))
+    assert amg16.level0_f16 and amg16.amg.level0_form == _lib.FEM_FORM_SCALED_DIFF16
+    assert abs(amg16.levels[0]['lam'] - lam32) <= 2e-3 * lam32
+    z16 = amg16.vcycle(r)
+    assert torch.linalg.norm(z16 - z32) <= 5e-3 * torch.linalg.norm(z32)
+    assert float(z16[torch.as_tensor(mask, device=r.device)].abs().max()) == 0.0
+    ref = spla.spsolve(A.tocsc(), b.cpu().numpy())
+    its = {}
+    for level0 in ('fp32', 'fp16'):
+        x = solver(p, {'amgx_solver': {'precision': 'mixed', 'level0': level0, 'tol': 1e-12, 'atol': 0.}})[0]
+        its[level0] = p._solve_stats['linear'][0]['iterations']
+        assert np.linalg.norm(x.reshape(-1).cpu().numpy() - ref) <= 1e-9 * np.linalg.norm(ref)
+    assert its['fp16'] <= its['fp32'] + 3, its

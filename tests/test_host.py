@@ -54,6 +54,11 @@ def test_amgx_solver_options(tmp_path):
         linear_config({'amgx_solver': {'precision': 'fp16'}})
     with pytest.raises(ValueError):
         linear_config({'amgx_solver': {'outer': 'fp32'}})     # an FP32 outer operator needs precision 'mixed'
+    assert linear_config({'amgx_solver': {'precision': 'mixed', 'level0': 'fp16'}})['level0'] == 'fp16'
+    with pytest.raises(ValueError):
+        linear_config({'amgx_solver': {'level0': 'fp16'}})    # FP16 level 0 needs the FP32 hierarchy
+    with pytest.raises(ValueError):
+        linear_config({'amgx_solver': {'precision': 'mixed', 'level0': 'fp16', 'outer': 'fp32'}})
     # an AMGX config file (solver.py:158-187 layout): tolerance and max_iters are taken
     path = tmp_path / 'amgx.json'
     path.write_text('{"config_version": 2, "solver": {"solver": "PCG", "tolerance": 1e-9, "max_iters": 77}}')
