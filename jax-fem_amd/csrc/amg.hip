@@ -289,6 +289,11 @@ __global__ void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv,
 // gets dinv = 0, i.e. no smoothing on it.
 // FP32 difference-form copy of a standard-form FP64 SELL matrix: off-diagonal
 // blocks rounded, the diagonal slot holds the FP64 row sum rounded once.
+// FP16 values of D^-1 A are stored times 2^8: entries down to 2.4e-7 stay normal
+// FP16 numbers (smaller couplings would lose bits as subnormals), entries up to
+// 255 stay finite; the kernels multiply the products back by 2^-8 (exact).
+constexpr double kF16Scale = 256.0;
+
 // Difference-form copy of a standard-form SELL matrix in TO: off-diagonal blocks
 // rounded from FP64, the diagonal slot the FP64 row sum rounded once; with
 // dinv, each block row a is scaled by dinv[row*V + a] first.
@@ -308,7 +313,7 @@ __device__ inline void k_sell_to_diff_body(const fem_pattern_t& pat, const doubl
   TO* op = out + (size_t)base * VV + lane;
   double sc[V];
 #pragma unroll
-  for (int a = 0; a < V; ++a) sc[a] = (dinv && row >= 0) ? dinv[row * V + a] : 1.0;
+  for (int a = 0; a < V; ++a) sc[a] = (dinv && row >= 0) ? kF16Scale * dinv[row * V + a] : 1.0;
   double rs[VV];
 #pragma unroll
   for (int v = 0; v < VV; ++v) rs[v] = 0.0;
@@ -1065,6 +1070,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       double t = acc[i];
 #pragma unroll
       for (int o = 1; o < W; ++o) t += red[o][lane][i];
+      if (kScaled) t *= 1.0 / kF16Scale;
       const int64_t idx = row * V + i;
       if (MODE == AP_CHEB) {
         const double r = kScaled ? fma(e_dinv[i], e_rhs[i], -t) : e_dinv[i] * (e_rhs[i] - t);
