@@ -88,6 +88,7 @@ def build_problem(nx, ny, nz):
 
 TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_pcg_a_hbm_traffic_cantilever_128x64x64.md')
 AMG_TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_sell_apply_cheb_hbm_traffic_cantilever_128x64x64.md')
+F16_TRAFFIC_PROFILE = os.path.join(ROOT, 'profiles', 'k_sell_apply_cheb_f16_cantilever_128x64x64.md')
 
 
 def measured_traffic(cells, path=TRAFFIC_PROFILE):
@@ -254,6 +255,7 @@ def run(args):
                         "iterations": res['amg_mixed_l0fp16']['iterations'],
                         "avg_launch_ms": res['amg_mixed_l0fp16']['spmv_ms'],
                         "algorithmic_bytes": cheb_bytes(dev, 2),
+                        "traffic": measured_traffic(args.cells, F16_TRAFFIC_PROFILE).get('k_sell_apply_cheb_f16'),
                         "achieved_GBs": (cheb_bytes(dev, 2) / (res['amg_mixed_l0fp16']['spmv_ms'] * 1e-3) / 1e9
                                          if res['amg_mixed_l0fp16']['spmv_ms'] else None)},
         "pcg_spmv": {"kernel": "k_pcg_a<3,true,float> (Jacobi-PCG SpMV q = Ap, FP32 storage)",
