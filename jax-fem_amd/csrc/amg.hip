@@ -1018,6 +1018,33 @@ __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t*
   }
 }
 
+// Long block rows (R and the coarse operators, hundreds of (block, column) pairs):
+// one 256-thread workgroup per block row, partial sums met in LDS.
+template <int BR, int BC, int MODE, typename T>
+__global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int32_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci, const T* __restrict__ v,
+                                                        ApplyArgs a) {
+  __shared__ double smem[4];
+  const int64_t row = blockIdx.x;
+  double acc[BR];
+#pragma unroll
+  for (int i = 0; i < BR; ++i) acc[i] = 0.0;
+  const int b = rp[row];
+  const int nf = (rp[row + 1] - b) * BC;
+  for (int f = threadIdx.x; f < nf; f += 256) {
+    const int k = b + f / BC, j = f % BC;
+    const double xv = a.x[(int64_t)ci[k] * BC + j];
+    const T* vb = v + (size_t)k * BR * BC + j;
+#pragma unroll
+    for (int i = 0; i < BR; ++i) acc[i] = fma(double(vb[i * BC]), xv, acc[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < BR; ++i) {
+    const double t = block_sum_256(acc[i], smem);
+    if (threadIdx.x == 0) apply_epilogue<MODE>(a, row * BR + i, t);
+  }
+}
+
 // Standard-form SELL-64 operator (level 0), FP64 vectors, one block per slice.
 // DIFF: the matrix is in difference form (the diagonal slot holds the row
 // sum); the own-row entries of the input then enter every product.
@@ -1236,21 +1263,30 @@ __global__ __launch_bounds__(256) void k_cg_rr(const double* __restrict__ part, 
 }
 
 // ------------------------------------------------------------ dispatch
+// (block, column) pairs per row above which a whole workgroup takes one row: on the
+// 1.6M-dof cantilever the level-1 restriction (about 1,800 pairs per row) drops from
+// 28.8 to 11.3 us, while the level-1 operator rows (270 pairs) are faster with one wave
+// (19.4 against 30.8 us)
+constexpr double kRowThreshold = 400.0;
+
 template <int MODE, typename T>
 int bcsr_apply(const fem_bcsr_t& M, const ApplyArgs& a, hipStream_t s) {
   if (M.rows == 0) return FEM_OK;
-  // lanes per block row from the mean number of (block, column) pairs in a row
+  // lanes per block row from the mean number of (block, column) pairs in a row: a
+  // sweep of 1..64 lanes on the level-0 prolongation of the 1.6M-dof cantilever (26
+  // pairs per row) gave 277 / 122 / 80 / 70 / 87 / 121 / 210 us, the long rows of R
+  // and the coarse operators (200-600 pairs) take a whole wave
   const double per_row = M.rows ? double(M.nnzb) * M.bc / double(M.rows) : 0.0;
-  const int G = per_row > 32.0 ? 64 : (per_row > 6.0 ? 16 : 8);
-  const unsigned grid = (unsigned)ceil_div(M.rows * G, 256);
+  const int G = per_row > kRowThreshold ? 256 : (per_row > 48.0 ? 64 : 8);
+  const unsigned grid = (unsigned)(G == 256 ? M.rows : ceil_div(M.rows * G, 256));
   const T* v = static_cast<const T*>(M.vals);
 #define FEM_AMG_APPLY(BR, BC)                                                                                  \
   if (M.br == BR && M.bc == BC) {                                                                              \
-    if (G == 64)                                                                                               \
-      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 64, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
+    if (G == 256)                                                                                              \
+      hipLaunchKernelGGL((k_bcsr_apply_row<BR, BC, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
                          M.col_idx, v, a);                                                                     \
-    else if (G == 16)                                                                                          \
-      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 16, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
+    else if (G == 64)                                                                                          \
+      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 64, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
                          M.col_idx, v, a);                                                                     \
     else                                                                                                       \
       hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 8, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr,  \
