@@ -1264,7 +1264,7 @@ __global__ __launch_bounds__(256) void k_cg_rr(const double* __restrict__ part, 
 
 // ------------------------------------------------------------ dispatch
 // (block, column) pairs per row above which a whole workgroup takes one row: on the
-// 1.6M-dof cantilever the level-1 restriction (about 1,800 pairs per row) drops from
+// 1.6M-dof cantilever the level-1 restriction (about 2,200 pairs per row) drops from
 // 28.8 to 11.3 us, while the level-1 operator rows (270 pairs) are faster with one wave
 // (19.4 against 30.8 us)
 constexpr double kRowThreshold = 400.0;
