@@ -14,18 +14,36 @@
 namespace fem {
 
 // ------------------------------------------------------- nodal gather
-__global__ void k_gather_nodal(int64_t num_nodes, int V, const int32_t* __restrict__ inc_ptr,
-                               const int32_t* __restrict__ inc_idx, const double* __restrict__ ent,
-                               const double* base, double* out) {
+// V is a template parameter so the per-node sums stay in registers; the
+// incident entries are visited two at a time to keep two gathers in flight.
+template <int V>
+__global__ __launch_bounds__(256) void k_gather_nodal(int64_t num_nodes, const int32_t* __restrict__ inc_ptr,
+                                                      const int32_t* __restrict__ inc_idx,
+                                                      const double* __restrict__ ent, const double* base,
+                                                      double* out) {
   for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < num_nodes;
        n += (int64_t)gridDim.x * blockDim.x) {
     const int b = inc_ptr[n], e = inc_ptr[n + 1];
-    double acc[3] = {0., 0., 0.};
-    for (int t = b; t < e; ++t) {
-      const int64_t src = (int64_t)inc_idx[t] * V;
-      for (int i = 0; i < V; ++i) acc[i] += ent[src + i];
+    double acc[V], acc2[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) acc[i] = acc2[i] = 0.0;
+    int t = b;
+    for (; t + 1 < e; t += 2) {
+      const double* p0 = ent + (int64_t)inc_idx[t] * V;
+      const double* p1 = ent + (int64_t)inc_idx[t + 1] * V;
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        acc[i] += p0[i];
+        acc2[i] += p1[i];
+      }
     }
-    for (int i = 0; i < V; ++i) out[n * V + i] = (base ? base[n * V + i] : 0.0) + acc[i];
+    if (t < e) {
+      const double* p0 = ent + (int64_t)inc_idx[t] * V;
+#pragma unroll
+      for (int i = 0; i < V; ++i) acc[i] += p0[i];
+    }
+#pragma unroll
+    for (int i = 0; i < V; ++i) out[n * V + i] = (base ? base[n * V + i] : 0.0) + (acc[i] + acc2[i]);
   }
 }
 
@@ -296,8 +314,14 @@ int fem_gather_nodal(int64_t num_nodes, int32_t vec, const int32_t* inc_ptr, con
                      const double* ent, const double* base, double* out, void* stream) {
   FEM_CHECK_ARG(num_nodes > 0 && vec >= 1 && vec <= 3 && inc_ptr && inc_idx && ent && out,
                 "fem_gather_nodal: bad arguments");
-  hipLaunchKernelGGL(k_gather_nodal, dim3(grid_for(num_nodes, 256)), dim3(256), 0, as_stream(stream), num_nodes, vec,
-                     inc_ptr, inc_idx, ent, base, out);
+  const dim3 grid(grid_for(num_nodes, 256));
+  hipStream_t s = as_stream(stream);
+  if (vec == 3)
+    hipLaunchKernelGGL(k_gather_nodal<3>, grid, dim3(256), 0, s, num_nodes, inc_ptr, inc_idx, ent, base, out);
+  else if (vec == 1)
+    hipLaunchKernelGGL(k_gather_nodal<1>, grid, dim3(256), 0, s, num_nodes, inc_ptr, inc_idx, ent, base, out);
+  else
+    hipLaunchKernelGGL(k_gather_nodal<2>, grid, dim3(256), 0, s, num_nodes, inc_ptr, inc_idx, ent, base, out);
   FEM_LAUNCHED();
   return FEM_OK;
 }
