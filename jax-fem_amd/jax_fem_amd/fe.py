@@ -205,9 +205,26 @@ class FiniteElement:
         return boundary_inds_list
 
     # --------------------------------------------------- post-processing
+    def _quad_kernel(self, sol, want_grad):
+        """u or grad u at the quadrature points by fem_quad_grad, for a solution that
+        does not carry autograd history; None when torch must do it (differentiable)."""
+        import torch
+        from ._lib import check, ptr, stream_ptr
+        dm = self.device_mesh()
+        if (torch.is_tensor(sol) and sol.requires_grad) or self.vec not in (1, 3):
+            return None
+        u = torch.as_tensor(sol, dtype=torch.float64, device=dm.device).reshape(-1).contiguous()
+        out = dm.empty(dm.C * dm.Q, self.vec, 3) if want_grad else dm.empty(dm.C * dm.Q, self.vec)
+        check(dm.L.fem_quad_grad(dm.ref, ptr(u), ptr(out) if want_grad else None, None if want_grad else ptr(out),
+                                 stream_ptr()), 'fem_quad_grad')
+        return out.reshape((dm.C, dm.Q, self.vec, 3) if want_grad else (dm.C, dm.Q, self.vec))
+
     def convert_from_dof_to_quad(self, sol):
         """(C,Q,vec) solution at quadrature points (fe.py:324-341)."""
         import torch
+        out = self._quad_kernel(sol, False)
+        if out is not None:
+            return out
         dm = self.device_mesh()
         sol = torch.as_tensor(sol, dtype=torch.float64, device=dm.device)
         sv = dm.shape_vals
@@ -225,6 +242,9 @@ class FiniteElement:
     def sol_to_grad(self, sol):
         """(C,Q,vec,dim) solution gradient at quadrature points (fe.py:366-382)."""
         import torch
+        out = self._quad_kernel(sol, True)
+        if out is not None:
+            return out
         dm = self.device_mesh()
         sol = torch.as_tensor(sol, dtype=torch.float64, device=dm.device)
         return torch.einsum('cav,cqad->cqvd', sol[dm.cells.long()], self.shape_grads)

@@ -204,3 +204,26 @@ def test_fenicsx_hyperelasticity_fixture_tabulated(gpu):
         HyperElasticity.get_tensor_map = orig
     got = sol.cpu().numpy().reshape(ref.shape).astype(np.float32)
     np.testing.assert_array_almost_equal(ref, got, decimal=decimal)
+
+
+@pytest.mark.parametrize('ele', ['HEX8', 'HEX27'])
+def test_quad_point_utilities_kernel_vs_torch(gpu, ele):
+    """sol_to_grad / convert_from_dof_to_quad (fe.py:324-382): the fem_quad_grad kernel (plain
+    tensors) against the differentiable torch contraction (tensors with autograd history)."""
+    if ele == 'HEX8':
+        mesh, order = distorted_box((3, 4, 3), seed=11), None
+    else:
+        m = box_mesh_hex27(2, 2, 1, 1., 1., 1.)
+        mesh, order = Mesh(m.points, m.cells_dict['hexahedron27']), 4
+
+    class P(UserMapProblem):
+        tensor_map = staticmethod(svk_torch)
+    p = P(mesh, vec=3, dim=3, ele_type=ele, quadrature_order=order)
+    fe = p.fes[0]
+    u = torch.as_tensor(np.random.default_rng(12).standard_normal((fe.num_total_nodes, 3)), device='cuda')
+    g_k, q_k = fe.sol_to_grad(u), fe.convert_from_dof_to_quad(u)
+    ur = u.clone().requires_grad_(True)
+    g_t, q_t = fe.sol_to_grad(ur), fe.convert_from_dof_to_quad(ur)
+    assert g_t.requires_grad and not g_k.requires_grad
+    assert rel(g_k.cpu().numpy(), g_t.detach().cpu().numpy()) < 1e-13
+    assert rel(q_k.cpu().numpy(), q_t.detach().cpu().numpy()) < 1e-14
