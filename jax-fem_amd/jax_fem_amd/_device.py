@@ -362,13 +362,12 @@ class DeviceProblem:
             m = self._eval_u_independent(fn, xq, civ, 'mass_map', allow_u=True)
             if m is None:
                 # m depends on u: evaluated at u_q with its derivative before each launch
-                key = ('mass_pm', id(fn))
-                if self._mat_cache.get('mass_key') != key:
+                hit = self._mat_cache.get('mass_pm')
+                if hit is None or hit[0] is not fn:
                     from .materials import PointMap
-                    self._mat_cache['mass_key'] = key
-                    self._mat_cache['mass_pm'] = PointMap(lambda u, x, *iv: _as_vector(fn(u, x, *iv)), (V,), (V,),
-                                                          'mass map')
-                self.mass_dyn = dict(pm=self._mat_cache['mass_pm'], rest=[xq] + [c.detach() for c in civ])
+                    hit = (fn, PointMap(lambda u, x, *iv: _as_vector(fn(u, x, *iv)), (V,), (V,), 'mass map'))
+                    self._mat_cache['mass_pm'] = hit
+                self.mass_dyn = dict(pm=hit[1], rest=[xq] + [c.detach() for c in civ])
             else:
                 cell = self.cell_buffer()
                 check(self.L.fem_cell_load(dm.ref, ptr(m), ptr(cell), stream_ptr()), 'fem_cell_load')
@@ -450,7 +449,7 @@ class DeviceProblem:
             vg = g[:, :, :, None, :] * j[:, :, None, None, None]
             self.buffers['uk_geo'] = [x, g, j[:, None, :].contiguous(), vg]
         rest = self.buffers['uk_geo'] + [_flatten_iv(iv, (dm.C,), dm.device).detach() for iv in ivs]
-        return dict(pm=self._term_map(('cell', id(fn)), fn), rest=rest, ids=None, fn=fn, ivs=ivs, lead=(dm.C,))
+        return dict(pm=self._term_map(('cell',), fn), rest=rest, ids=None, fn=fn, ivs=ivs, lead=(dm.C,))
 
     def _face_term(self, i, fn, siv):
         """get_universal_kernels_surface (problem.py:305-327): fn(cell_sol_flat, x,
@@ -465,16 +464,19 @@ class DeviceProblem:
             self.buffers[key] = [x, fv, g, ns[:, None, :].contiguous()]
         S = srf['faces'].shape[0]
         rest = self.buffers[key] + [_flatten_iv(iv, (S,), dm.device).detach() for iv in siv]
-        return dict(pm=self._term_map(('face', i, id(fn)), fn), rest=rest, ids=srf['faces'][:, 0].long(), fn=fn,
+        return dict(pm=self._term_map(('face', i), fn), rest=rest, ids=srf['faces'][:, 0].long(), fn=fn,
                     ivs=siv, lead=(S,))
 
-    def _term_map(self, key, fn):
+    def _term_map(self, slot, fn):
+        """The PointMap of one weak-form term (one per slot, kept while the user function is the same object)."""
         from .materials import PointMap
         cache = self._mat_cache.setdefault('terms', {})
-        if key not in cache:
+        hit = cache.get(slot)
+        if hit is None or hit[0] is not fn:
             nv = self.dm.N * self.V
-            cache[key] = PointMap(lambda cs, *r: _as_vector(fn(cs, *r)), (nv,), (nv,), 'universal kernel')
-        return cache[key]
+            hit = (fn, PointMap(lambda cs, *r: _as_vector(fn(cs, *r)), (nv,), (nv,), 'universal kernel'))
+            cache[slot] = hit
+        return hit[1]
 
     def _cell_terms_eval(self, u, jac):
         """Sum of the per-cell terms: r (C, N, V) and, with `jac`, K (C, N, N, V, V) FP64."""
