@@ -91,3 +91,11 @@ def test_adjoint_of_nonsymmetric_tangent_is_refused(gpu):
     u = fwd(th)[0]
     with pytest.raises(NotImplementedError, match="not symmetric"):
         u.sum().backward()
+
+
+def test_bicgstab_not_converged_raises(gpu):
+    """The reference's post-check (solver.py:86-90) on the BiCGSTAB path: maxiter 1 leaves
+    |b - A x| above 0.1 and the solve fails loudly."""
+    p = Advection(box(10, 6, 6), vec=1, dim=3, dirichlet_bc_info=[[LEFT, RIGHT], [0, 0], [lambda x: 0., lambda x: 1.]])
+    with pytest.raises(AssertionError, match="failed to converge"):
+        solver(p, {'jax_solver': {'maxiter': 1}})
