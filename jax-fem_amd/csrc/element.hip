@@ -62,20 +62,33 @@ __global__ __launch_bounds__(256) void k_cell_tangent(fem_mesh_t mesh, fem_mater
     }
   }
   __syncthreads();
-  if (!active) return;
-  for (int pr = t; pr < N * N; pr += TPC) {
-    const int a = pr / N, b = pr % N;
-    T K[V][V];
+  // HEX8: a cell's blocks are staged in LDS and stored lane-contiguously (each store
+  // instruction writes 64 consecutive values of the cell's Ke instead of one value
+  // of 64 blocks 72 bytes apart)
+  constexpr bool kStage = N == 8;
+  __shared__ T s_K[kStage ? CPW : 1][kStage ? N * N * V * V : 1];
+  if (active) {
+    for (int pr = t; pr < N * N; pr += TPC) {
+      const int a = pr / N, b = pr % N;
+      T K[V][V];
 #pragma unroll
-    for (int i = 0; i < V; ++i)
+      for (int i = 0; i < V; ++i)
 #pragma unroll
-      for (int k = 0; k < V; ++k) K[i][k] = T(0);
-    for (int q = 0; q < Q; ++q) MAT::tangent(s_st[w][q], L.g[q][a], L.g[q][b], L.jxw[q], K);
-    T* out = Ke + ((size_t)c * N * N + pr) * (V * V);
+        for (int k = 0; k < V; ++k) K[i][k] = T(0);
+      for (int q = 0; q < Q; ++q) MAT::tangent(s_st[w][q], L.g[q][a], L.g[q][b], L.jxw[q], K);
+      T* out = kStage ? &s_K[w][pr * V * V] : Ke + ((size_t)c * N * N + pr) * (V * V);
 #pragma unroll
-    for (int i = 0; i < V; ++i)
+      for (int i = 0; i < V; ++i)
 #pragma unroll
-      for (int k = 0; k < V; ++k) out[i * V + k] = K[i][k];
+        for (int k = 0; k < V; ++k) out[i * V + k] = K[i][k];
+    }
+  }
+  if (kStage) {
+    __syncthreads();
+    if (active) {
+      T* out = Ke + (size_t)c * N * N * V * V;
+      for (int e = t; e < N * N * V * V; e += TPC) out[e] = s_K[w][e];
+    }
   }
 }
 
