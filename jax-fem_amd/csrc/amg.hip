@@ -806,6 +806,7 @@ __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows
                                                         const double* __restrict__ p0, const int32_t* __restrict__ agg,
                                                         const double* __restrict__ dinv, double omega,
                                                         double* __restrict__ pv) {
+  const int32_t plan_len = plan_ptr[nb];
   for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
     int64_t lo = 0, hi = rows;  // row i: crp[i] <= b < crp[i+1]
     while (hi - lo > 1) {
@@ -821,12 +822,12 @@ __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows
     for (int r = 0; r < BS; ++r)
 #pragma unroll
       for (int j = 0; j < NS; ++j) acc[r][j] = 0.0;
-#pragma unroll 4
+    // the thread's (A block, P0 block) pairs are contiguous: read through a window
+    ListWindow<4> pw(plan, 2 * (int64_t)plan_len);
     for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
-      const int2 ef = reinterpret_cast<const int2*>(plan)[q];
       double xa[BS][BS], ya[BS][NS];
-      load_block<BS, BS>(av + (size_t)ef.x * BS * BS, xa);
-      load_block<BS, NS>(p0 + (size_t)ef.y * BS * NS, ya);
+      load_block<BS, BS>(av + (size_t)pw[2 * (int64_t)q] * BS * BS, xa);
+      load_block<BS, NS>(p0 + (size_t)pw[2 * (int64_t)q + 1] * BS * NS, ya);
 #pragma unroll
       for (int r = 0; r < BS; ++r)
 #pragma unroll

@@ -157,4 +157,51 @@ __device__ inline void inv3(const T* A, T det, T* B) {
   B[8] = (A[0] * A[4] - A[1] * A[3]) * id;
 }
 
+// ------------------------------------------------------- list windows
+// A lane walking its own int32 list in order: entries come from W-entry windows
+// fetched by 16-byte loads aligned to the array's address, one wide load per W
+// entries.  Lanes of a wave walk different lists, so between two narrow reads of
+// one lane its cache line is usually gone; a window keeps the next W entries in
+// registers.  Windows that would run past [0, len) are read entry by entry.
+template <int W>
+struct ListWindow {
+  static_assert(W % 4 == 0, "windows are whole 16-byte loads");
+  const int32_t* __restrict__ a;
+  int64_t len;
+  int off;         // entries between the preceding 16-byte boundary and a[0]
+  int64_t base;    // window start, in entries from that boundary
+  int32_t v[W];
+
+  __device__ ListWindow(const int32_t* p, int64_t n)
+      : a(p), len(n), off((int)(((uintptr_t)p & 15) >> 2)), base(-1) {}
+
+  __device__ int32_t operator[](int64_t i) {
+    const int64_t s = i + off;
+    const int64_t b = s & ~(int64_t)(W - 1);
+    if (b != base) {
+      base = b;
+      const int64_t first = b - off;
+      if (first >= 0 && first + W <= len) {
+        const int4* q = reinterpret_cast<const int4*>(a + first);
+#pragma unroll
+        for (int j = 0; j < W / 4; ++j) {
+          const int4 x = q[j];
+          v[4 * j] = x.x;
+          v[4 * j + 1] = x.y;
+          v[4 * j + 2] = x.z;
+          v[4 * j + 3] = x.w;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < W; ++j) v[j] = (first + j >= 0 && first + j < len) ? a[first + j] : 0;
+      }
+    }
+    const int o = (int)(s - b);
+    int32_t r = v[0];
+#pragma unroll
+    for (int j = 1; j < W; ++j) r = o == j ? v[j] : r;
+    return r;
+  }
+};
+
 }  // namespace fem

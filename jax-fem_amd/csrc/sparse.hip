@@ -52,7 +52,8 @@ __global__ __launch_bounds__(256) void k_gather_nodal(int64_t num_nodes, const i
 // block (row, col_k) and store them (zeros in padding slots).  DIFF stores
 // the row-sum block R_row = sum_k A(row, k) (accumulated in TS) in the
 // diagonal slot instead of the diagonal block (difference form of the SpMV).
-// dinv (may be NULL) receives 1 / A(row*V+c, row*V+c).
+// dinv (may be NULL) receives 1 / A(row*V+c, row*V+c).  The slot bounds and
+// the cell-block ids are read through windows (a lane's lists are contiguous).
 template <int V, bool DIFF, typename TS, typename T>
 __global__ __launch_bounds__(256) void k_gather_matrix(fem_pattern_t pat, const TS* __restrict__ Ke,
                                                        T* __restrict__ vals, T* __restrict__ dinv) {
@@ -75,14 +76,18 @@ __global__ __launch_bounds__(256) void k_gather_matrix(fem_pattern_t pat, const 
   for (int v = 0; v < VV; ++v) rsum[v] = TS(0);
 #pragma unroll
   for (int c = 0; c < V; ++c) diag[c] = TS(0);
+  ListWindow<4> cptr(pat.contrib_ptr, pat.nnzb + 1);
+  ListWindow<8> cidx(pat.contrib_idx, (int64_t)pat.num_cells * pat.nodes_per_cell * pat.nodes_per_cell);
+  int c1 = len > 0 ? cptr[rb] : 0;
   for (int k = 0; k < L; ++k) {
     TS acc[VV];
 #pragma unroll
     for (int v = 0; v < VV; ++v) acc[v] = TS(0);
     if (k < len) {
-      const int c0 = pat.contrib_ptr[rb + k], c1 = pat.contrib_ptr[rb + k + 1];
+      const int c0 = c1;
+      c1 = cptr[rb + k + 1];
       for (int ci = c0; ci < c1; ++ci) {
-        const TS* src = Ke + (size_t)pat.contrib_idx[ci] * VV;
+        const TS* src = Ke + (size_t)cidx[ci] * VV;
 #pragma unroll
         for (int v = 0; v < VV; ++v) acc[v] += src[v];
       }
@@ -331,6 +336,8 @@ int fem_gather_matrix(const fem_pattern_t* pat, int32_t ke_precision, const void
   if (int rc = check_pat(pat)) return rc;
   FEM_CHECK_ARG(Ke && vals && pat->contrib_ptr && pat->contrib_idx && pat->diag_slot,
                 "fem_gather_matrix: NULL argument");
+  FEM_CHECK_ARG(pat->nnzb > 0 && pat->num_cells > 0 && pat->nodes_per_cell > 0,
+                "fem_gather_matrix: pattern nnzb, num_cells and nodes_per_cell must be set");
   FEM_CHECK_ARG(ke_precision == precision || (ke_precision == FEM_F64 && precision == FEM_F32),
                 "fem_gather_matrix: Ke precision must equal the matrix precision or be FP64");
   FEM_CHECK_ARG(form == FEM_FORM_STANDARD || form == FEM_FORM_DIFFERENCE, "fem_gather_matrix: bad form");
