@@ -304,6 +304,43 @@ def test_assembly_deterministic(gpu):
     assert np.array_equal(outs[0], outs[1])
 
 
+@pytest.mark.parametrize('shift', [1, 2, 3])
+def test_gather_with_unaligned_index_lists(gpu, shift):
+    """fem_gather_matrix reads the slot bounds and the cell-block ids through
+    16-byte windows aligned to each list's address.  Lists that start 4, 8 or 12
+    bytes past a 16-byte boundary (a C caller's arrays need not be aligned) give
+    the same matrix, bit for bit, in both storage forms."""
+    import ctypes as C
+    p, o, u = make_material_problem('neo_hookean')
+    dev = p.device()
+    dev.prepare()
+    dev.tangent(torch.as_tensor(u, device='cuda'), _lib.FEM_F64)
+    Ke = dev.current[1]
+    L = _lib.lib()
+    pat = _lib.FemPattern.from_buffer_copy(dev.pat)
+    keep = []
+    for name in ('contrib_ptr', 'contrib_idx'):
+        a = dev.pat_tensors[name]
+        buf = torch.full((a.numel() + 8,), -7, dtype=torch.int32, device='cuda')
+        view = buf[shift:shift + a.numel()]
+        view.copy_(a)
+        assert view.data_ptr() % 16 == 4 * shift
+        setattr(pat, name, view.data_ptr())
+        keep.append(buf)
+    for precision, form, dt in ((_lib.FEM_F64, _lib.FEM_FORM_STANDARD, torch.float64),
+                                (_lib.FEM_F32, _lib.FEM_FORM_DIFFERENCE, torch.float32)):
+        outs = []
+        for pref in (dev.pref, C.byref(pat)):
+            vals = torch.full((dev.sell_blocks * p.fes[0].vec ** 2,), 3., dtype=dt, device='cuda')
+            dinv = torch.zeros(dev.n, dtype=dt, device='cuda')
+            _lib.check(L.fem_gather_matrix(pref, _lib.FEM_F64, _lib.ptr(Ke), precision, form, _lib.ptr(vals),
+                                           _lib.ptr(dinv), _lib.stream_ptr()), 'fem_gather_matrix')
+            torch.cuda.synchronize()
+            outs.append((vals, dinv))
+        assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1]), (precision, form)
+        assert torch.isfinite(outs[0][0]).all() and bool((outs[0][1] != 0).any())
+
+
 # --------------------------------------------------------- end-to-end fixtures
 def _fixture_poisson(precision, method='jax_solver'):
     d = golden('linear_poisson')
