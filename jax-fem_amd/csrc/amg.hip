@@ -1168,9 +1168,17 @@ __global__ __launch_bounds__(256) void k_dot_part(int64_t n, const double* __res
 }
 
 __device__ inline double sum_parts(const double* part, int np, double* red) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) s += part[i];
-  return block_sum_256(s, red);
+  // four independent sums keep four loads in flight per thread (a fixed order)
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int i = threadIdx.x;
+  for (; i + 768 < np; i += 1024) {
+    s0 += part[i];
+    s1 += part[i + 256];
+    s2 += part[i + 512];
+    s3 += part[i + 768];
+  }
+  for (; i < np; i += 256) s0 += part[i];
+  return block_sum_256((s0 + s1) + (s2 + s3), red);
 }
 
 __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ part, int np, double* __restrict__ out) {
