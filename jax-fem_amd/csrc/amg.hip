@@ -271,17 +271,36 @@ __global__ void k_bc_mask_set(int64_t n_bc, const int32_t* __restrict__ dofs, ui
 }
 
 // ------------------------------------------------------ matrix helpers
+// One block per slice: KC slots of the slice's 64 rows are read coalesced into
+// LDS, then each row's KC*V*V entries (contiguous in the block CSR) are stored
+// as one contiguous run, so the stores fill whole cache lines.
 template <int V>
-__global__ void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv, double* __restrict__ out) {
-  constexpr int VV = V * V;
-  for (int64_t row = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; row < pat.num_rows;
-       row += (int64_t)gridDim.x * blockDim.x) {
-    const int rb = pat.row_ptr[row], len = pat.row_ptr[row + 1] - rb;
-    const int64_t pos = pat.sell_pos[row];
-    const double* vp = sv + (size_t)pat.slice_ptr[pos >> 6] * VV + (pos & 63);
-    for (int k = 0; k < len; ++k)
-#pragma unroll
-      for (int v = 0; v < VV; ++v) out[(size_t)(rb + k) * VV + v] = vp[((size_t)k * VV + v) * kSlice];
+__global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv,
+                                                      double* __restrict__ out) {
+  constexpr int VV = V * V, KC = 4;
+  __shared__ double tile[kSlice][KC * VV + 1];
+  __shared__ int s_dst[kSlice], s_len[kSlice];
+  const int64_t s = blockIdx.x;
+  const int base = pat.slice_ptr[s];
+  const int L = (pat.slice_ptr[s + 1] - base) >> 6;
+  if (threadIdx.x < kSlice) {
+    const int row = pat.sell_row[s * kSlice + threadIdx.x];
+    s_dst[threadIdx.x] = row >= 0 ? pat.row_ptr[row] : 0;
+    s_len[threadIdx.x] = row >= 0 ? pat.row_ptr[row + 1] - pat.row_ptr[row] : 0;
+  }
+  const double* src = sv + (size_t)base * VV;
+  for (int k0 = 0; k0 < L; k0 += KC) {
+    const int nc = min(KC, L - k0) * VV;  // entries per row in this chunk
+    __syncthreads();                      // row data ready, previous tile consumed
+    for (int f = threadIdx.x; f < nc * kSlice; f += 256) {
+      const int lane = f & (kSlice - 1), c = f >> 6;  // c = slot-in-chunk * VV + entry
+      tile[lane][c] = src[((size_t)k0 * VV + c) * kSlice + lane];
+    }
+    __syncthreads();
+    for (int f = threadIdx.x; f < nc * kSlice; f += 256) {
+      const int lane = f / nc, c = f - lane * nc;
+      if (k0 + c / VV < s_len[lane]) out[((size_t)s_dst[lane] + k0) * VV + c] = tile[lane][c];
+    }
   }
 }
 
@@ -1660,11 +1679,12 @@ int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, doub
   FEM_CHECK_ARG(pat && sell_vals && bcsr_vals, "fem_amg_sell_to_bcsr: NULL argument");
   hipStream_t s = as_stream(stream);
   if (pat->num_rows == 0) return FEM_OK;
-  const unsigned g = grid_for(pat->num_rows, 256);
+  FEM_CHECK_ARG(pat->num_slices == ceil_div(pat->num_rows, kSlice), "fem_amg_sell_to_bcsr: num_slices mismatch");
+  const dim3 g((unsigned)pat->num_slices);
   if (pat->vec == 3)
-    hipLaunchKernelGGL(k_sell_to_bcsr<3>, dim3(g), dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
+    hipLaunchKernelGGL(k_sell_to_bcsr<3>, g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
   else if (pat->vec == 1)
-    hipLaunchKernelGGL(k_sell_to_bcsr<1>, dim3(g), dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
+    hipLaunchKernelGGL(k_sell_to_bcsr<1>, g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
   else
     FEM_CHECK_ARG(false, "fem_amg_sell_to_bcsr: vec must be 1 or 3");
   FEM_LAUNCHED();
