@@ -319,6 +319,44 @@ int fem_export_csr(const fem_pattern_t* pat, int32_t precision, int32_t form, co
 /* y = dR/du(u) x (FP64, matrix-free, BC rows = identity). */
 int fem_operator_apply(const fem_operator_t* op, const double* x, double* y, void* stream);
 
+/* ---- single-pass HEX8 assembly (problem.py:447-460 and solver.py:522-526 in
+ *      one kernel, optionally with the residual scatter of problem.py:426-437)
+ * The cell tangents are formed and summed straight into the SELL values: a
+ * workgroup owns one 64-row slice and walks the cells incident to its rows in
+ * chunks; no per-cell tangent buffer exists.  The plan (built once per pattern)
+ * lists each slice's incident cells and re-expresses every contribution as a
+ * position in that list. */
+typedef struct {
+  const int32_t* slice_cell_ptr; /* (num_slices + 1)                           */
+  const int32_t* slice_cells;    /* unique incident cells of each slice, ascending */
+  const int32_t* contrib_loc;    /* (num_cells*N*N), aligned with contrib_idx:
+                                    position * N*N + a*N + b                   */
+  int32_t        max_slots;      /* largest slot count of a slice (<= 32)     */
+  int32_t        pad;
+} fem_assembly_plan_t;
+
+/* Plan, pass 1: unique incident cells per slice (counts, to be scanned into
+ * slice_cell_ptr).  err_flag > 0 after the call: a slice's rows have more than
+ * 4096 incidences (use the two-pass path). */
+int fem_assembly_plan_counts(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
+                             int32_t* slice_cell_counts, int32_t* err_flag, void* stream);
+/* Plan, pass 2: slice_cells and contrib_loc (err_flag as above, or 1 when a
+ * contribution's cell is missing from its slice list). */
+int fem_assembly_plan_fill(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
+                           const int32_t* slice_cell_ptr, int32_t* slice_cells, int32_t* contrib_loc,
+                           int32_t* err_flag, void* stream);
+/* Assemble the tangent of HEX8 cells with 8 quadrature points at u into the
+ * SELL values: FP64 standard form (precision FEM_F64, form FEM_FORM_STANDARD)
+ * or FP32 difference form (FEM_F32, FEM_FORM_DIFFERENCE); the cell tangents
+ * are FP64 and each matrix value is their FP64 sum in cell order, rounded
+ * once.  dinv (may be NULL): Jacobi inverse diagonal in `precision`.  res (may
+ * be NULL): R(u) without Dirichlet rows, = base (may be NULL) + the scattered
+ * internal force, FP64.  Dirichlet rows are applied afterwards by
+ * fem_matrix_bc_rows. */
+int fem_assemble(const fem_mesh_t* mesh, const fem_material_t* mat, const fem_pattern_t* pat,
+                 const fem_assembly_plan_t* plan, const double* u, int32_t precision, int32_t form, void* vals,
+                 void* dinv, const double* base, double* res, void* stream);
+
 /* ---- Krylov solve (solver.py:63-92, 261-284) -------------------------- */
 /* Workspace bytes for fem_pcg_solve. */
 int64_t fem_pcg_ws_bytes(const fem_pattern_t* pat, int32_t precision);

@@ -1,0 +1,527 @@
+// Single-pass HEX8 assembly: per-cell tangents (and optionally the residual)
+// computed and summed straight into the SELL-64 matrix, with no per-cell
+// tangent buffer in between.
+//
+// Replaces, in one kernel, what jax-fem does in two host-visible stages:
+//   problem.py:447-460  compute_newton_vars: vmapped jacfwd per cell -> problem.V
+//   solver.py:522-526   PETSc setValuesCOO: sum the COO duplicates into AIJ
+//   problem.py:426-437  scatter of the cell residuals (optional, same pass)
+//
+// Work split.  A workgroup owns one SELL slice (64 block rows, lane l = row
+// sell_row[64 s + l]), so every matrix block it writes is its own and no
+// atomics are needed.  The slice's incident cells (the plan's slice cell list,
+// ascending) are processed in chunks of CH cells:
+//   phase A  one thread per (cell, quadrature point): Jacobian, inverse, JxW,
+//            the 8 physical shape gradients and the material state (and the
+//            weighted stress for the residual), staged in LDS;
+//   phase B  one thread per (row lane, slot group): wave w owns the slots
+//            k = w, w + 4, ... of its lane's row and walks each slot's
+//            contribution list (cell, a, b) -- the same cell-ordered list the
+//            two-pass gather sums -- forming K_ab of the cell from the staged
+//            data and adding it to register accumulators.
+// Contributions are summed in cell order, as PETSc and the two-pass gather
+// do, so the result does not depend on scheduling (bit-reproducible).  After
+// the last chunk every lane stores its slots: one coalesced 64-lane store per
+// matrix value.  The difference form's row sum and the Jacobi inverse
+// diagonal are formed from the accumulators in a fixed order.
+//
+// A cell's Jacobian is recomputed once per slice that touches it (about 4.5
+// times for a structured HEX8 box with 64-row slices); each tangent block is
+// computed exactly once.
+#include "element.hpp"
+
+#include <algorithm>
+#include <type_traits>
+
+namespace fem {
+
+constexpr int kAsmThreads = 512;   // 8 waves per slice
+constexpr int kAsmWaves = kAsmThreads / 64;
+constexpr int kSliceIncCap = 4096;  // incident (cell, node) entries of one slice's rows
+
+// ------------------------------------------------------------------ plan
+// Unique incident cells of each slice's rows, ascending.  FILL = false writes
+// the count per slice; FILL = true writes the cells at out_ptr[s].
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_slice_cells(int64_t num_slices, int32_t N,
+                                                     const int32_t* __restrict__ sell_row,
+                                                     const int32_t* __restrict__ inc_ptr,
+                                                     const int32_t* __restrict__ inc_idx,
+                                                     const int32_t* __restrict__ out_ptr, int32_t* __restrict__ out,
+                                                     int32_t* __restrict__ err) {
+  __shared__ int32_t buf[kSliceIncCap];
+  __shared__ int32_t s_off[65];
+  __shared__ int32_t s_scan[256];
+  const int64_t s = blockIdx.x;
+  const int t = threadIdx.x;
+  if (s >= num_slices) return;
+  int row = -1, b = 0, e = 0;
+  if (t < 64) {
+    row = sell_row[s * 64 + t];
+    if (row >= 0) {
+      b = inc_ptr[row];
+      e = inc_ptr[row + 1];
+    }
+    s_off[t + 1] = e - b;
+  }
+  if (t == 0) s_off[0] = 0;
+  __syncthreads();
+  if (t == 0)
+    for (int i = 1; i <= 64; ++i) s_off[i] += s_off[i - 1];
+  __syncthreads();
+  const int total = s_off[64];
+  if (total > kSliceIncCap) {
+    if (t == 0) {
+      atomicMax(err, total);
+      if (!FILL) out[s] = 0;
+    }
+    return;
+  }
+  if (t < 64)
+    for (int j = b; j < e; ++j) buf[s_off[t] + (j - b)] = inc_idx[j] / N;
+  int P = 1;
+  while (P < total) P <<= 1;
+  for (int i = total + t; i < P; i += 256) buf[i] = INT32_MAX;
+  __syncthreads();
+  // bitonic sort of buf[0, P)
+  for (int k = 2; k <= P; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = t; i < P; i += 256) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int32_t x = buf[i], y = buf[l];
+          const bool up = (i & k) == 0;
+          if ((x > y) == up) {
+            buf[i] = y;
+            buf[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // unique entries: each thread takes a contiguous run of the sorted list
+  const int per = (total + 255) / 256;
+  const int i0 = min(total, t * per), i1 = min(total, i0 + per);
+  int cnt = 0;
+  for (int i = i0; i < i1; ++i) cnt += (i == 0 || buf[i] != buf[i - 1]) ? 1 : 0;
+  s_scan[t] = cnt;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int a = t >= off ? s_scan[t - off] : 0;
+    __syncthreads();
+    s_scan[t] += a;
+    __syncthreads();
+  }
+  if (!FILL) {
+    if (t == 255) out[s] = s_scan[255];
+    return;
+  }
+  int pos = out_ptr[s] + s_scan[t] - cnt;
+  for (int i = i0; i < i1; ++i)
+    if (i == 0 || buf[i] != buf[i - 1]) out[pos++] = buf[i];
+}
+
+// contrib_loc[t] = (position of the cell in its slice's cell list) * N*N + a*N + b
+// for every contribution t of every row (one thread per slice lane).
+__global__ __launch_bounds__(256) void k_contrib_local(fem_pattern_t pat, const int32_t* __restrict__ sc_ptr,
+                                                       const int32_t* __restrict__ sc, int32_t* __restrict__ loc,
+                                                       int32_t* __restrict__ err) {
+  const int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (g >= pat.num_slices * kSlice) return;
+  const int row = pat.sell_row[g];
+  if (row < 0) return;
+  const int64_t s = g >> 6;
+  const int NN = pat.nodes_per_cell * pat.nodes_per_cell;
+  const int32_t* list = sc + sc_ptr[s];
+  const int n = sc_ptr[s + 1] - sc_ptr[s];
+  const int t0 = pat.contrib_ptr[pat.row_ptr[row]], t1 = pat.contrib_ptr[pat.row_ptr[row + 1]];
+  for (int t = t0; t < t1; ++t) {
+    const int32_t e = pat.contrib_idx[t];
+    const int32_t c = e / NN;
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (list[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= n || list[lo] != c) {
+      atomicMax(err, 1);
+      return;
+    }
+    loc[t] = lo * NN + (e - c * NN);
+  }
+}
+
+// ------------------------------------------------------------- assembly
+template <class MAT>
+struct AsmCfg {
+  using St = typename MAT::template St<double>;
+  static constexpr int SW = (int)(sizeof(St) / sizeof(double));
+  // cells per chunk: 32 (one phase-A item per thread of the first half) for light
+  // material states, 16 for the large ones (Neo-Hookean, J2)
+  static constexpr int CH = SW <= 4 ? 32 : 16;
+  static constexpr int GS = 8 * 8 * 3 + 1;  // odd stride (in doubles) between cells: no LDS bank conflicts
+};
+
+template <class MAT>
+struct alignas(8) StRow {
+  typename MAT::template St<double> s[8];
+  double pad;  // odd stride between cells
+};
+
+// Isotropic linear laws (elasticity, SIMP): K_ab = A + B^T + tr(B) I with
+// A = sum_q lambda_q w_q g_a (x) g_b and B = sum_q mu_q w_q g_a (x) g_b (the
+// closed form of MatElastic::tangent summed over the quadrature points).
+template <class MAT>
+struct IsoLinear : std::false_type {};
+template <bool S>
+struct IsoLinear<MatElastic<S>> : std::true_type {};
+
+constexpr int kAsmMaxK = 4;  // slots per thread: rows of at most 8 * 4 = 32 blocks
+
+template <class MAT, typename TO, bool DIFF, bool RES>
+__global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
+    fem_mesh_t mesh, fem_material_t mat, fem_pattern_t pat, const int32_t* __restrict__ sc_ptr,
+    const int32_t* __restrict__ sc, const int32_t* __restrict__ cloc, const double* __restrict__ u,
+    TO* __restrict__ vals, TO* __restrict__ dinv, const double* __restrict__ base, double* __restrict__ res) {
+  constexpr int N = 8, Q = 8, V = MAT::V, VV = V * V, MAXK = kAsmMaxK;
+  using Cfg = AsmCfg<MAT>;
+  constexpr int CH = Cfg::CH, GS = Cfg::GS;
+  __shared__ double s_dN[Q][N][3];
+  __shared__ double s_w[Q];
+  __shared__ double s_g[CH][GS];
+  __shared__ double s_jxw[CH][Q + 1];
+  __shared__ StRow<MAT> s_st[CH];
+  __shared__ double s_P[RES ? CH : 1][RES ? Q * V * 3 + 1 : 1];
+  // the difference form's row sums are staged in s_g after the last chunk, half the waves at a time
+  constexpr int kHalf = kAsmWaves / 2;
+  static_assert(kHalf * kSlice * VV <= CH * GS, "row-sum staging does not fit in s_g");
+  double(*s_red)[kSlice][VV] = reinterpret_cast<double(*)[kSlice][VV]>(&s_g[0][0]);
+
+  const int64_t s = blockIdx.x;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  load_tables(mesh, s_dN, s_w);
+  const int sbase = pat.slice_ptr[s];
+  const int L = (pat.slice_ptr[s + 1] - sbase) >> 6;
+  const int row = pat.sell_row[s * kSlice + lane];
+  const bool real = row >= 0;
+  const int rb = real ? pat.row_ptr[row] : 0;
+  const int len = real ? pat.row_ptr[row + 1] - rb : 0;
+  const int ds = real ? pat.diag_slot[row] : -1;
+  const int c0 = sc_ptr[s];
+  const int ncell = sc_ptr[s + 1] - c0;
+
+  double acc[MAXK][VV];
+  int cur[MAXK], cend[MAXK];
+#pragma unroll
+  for (int m = 0; m < MAXK; ++m) {
+    const int k = w + kAsmWaves * m;
+    cur[m] = cend[m] = 0;
+    if (k < len) {
+      cur[m] = pat.contrib_ptr[rb + k];
+      cend[m] = pat.contrib_ptr[rb + k + 1];
+    }
+#pragma unroll
+    for (int v = 0; v < VV; ++v) acc[m][v] = 0.0;
+  }
+  double r[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) r[i] = 0.0;
+  __syncthreads();
+
+  for (int ch0 = 0; ch0 < ncell; ch0 += CH) {
+    // ---- phase A: per (cell, quadrature point) geometry and material state
+    if (t < CH * Q) {
+      const int cc = t / Q, q = t % Q;
+      if (ch0 + cc < ncell) {
+        const int64_t c = sc[c0 + ch0 + cc];
+        const int32_t* cn = mesh.cells + c * N;
+        const double* p0 = mesh.points + (int64_t)cn[0] * 3;
+        const double x00 = p0[0], x01 = p0[1], x02 = p0[2];
+        double J[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) J[k] = 0.0;
+        // rolled loops keep phase A's registers low: the slot accumulators stay live across it
+#pragma unroll 1
+        for (int a = 1; a < N; ++a) {
+          const double* pa = mesh.points + (int64_t)cn[a] * 3;
+          const double d0 = pa[0] - x00, d1 = pa[1] - x01, d2 = pa[2] - x02;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double dn = s_dN[q][a][j];
+            J[0 * 3 + j] += d0 * dn;
+            J[1 * 3 + j] += d1 * dn;
+            J[2 * 3 + j] += d2 * dn;
+          }
+        }
+        const double det = det3(J);
+        double iJ[9];
+        inv3(J, det, iJ);
+        const double jxw = det * s_w[q];
+        double G[V][3];
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) G[i][j] = 0.0;
+#pragma unroll 1
+        for (int a = 0; a < N; ++a) {
+          const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
+          double g[3];
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            g[j] = e0 * iJ[0 * 3 + j] + e1 * iJ[1 * 3 + j] + e2 * iJ[2 * 3 + j];
+            s_g[cc][(q * N + a) * 3 + j] = g[j];
+          }
+          if (MAT::kNeedsU || RES) {
+            const int64_t na = cn[a];
+#pragma unroll
+            for (int i = 0; i < V; ++i) {
+              const double ua = u[na * V + i];
+#pragma unroll
+              for (int j = 0; j < 3; ++j) G[i][j] += ua * g[j];
+            }
+          }
+        }
+        s_jxw[cc][q] = jxw;
+        const auto st = MAT::template prep<double>(mat, c * Q + q, G);
+        s_st[cc].s[q] = st;
+        if (RES) {
+          double P[V][3];
+          MAT::stress(st, G, P);
+#pragma unroll
+          for (int i = 0; i < V; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) s_P[cc][(q * V + i) * 3 + j] = P[i][j] * jxw;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- phase B: this thread's slots, contributions of this chunk's cells
+    const int lim = (ch0 + CH) * N * N;  // contributions of cells before the chunk's end
+#pragma unroll
+    for (int m = 0; m < MAXK; ++m) {
+      while (cur[m] < cend[m]) {
+        const int e = cloc[cur[m]];
+        if (e >= lim) break;
+        ++cur[m];
+        const int cc = e / (N * N) - ch0;
+        const int a = (e >> 3) & 7, b = e & 7;
+        double K[V][V];
+        if constexpr (IsoLinear<MAT>::value) {
+          double A[3][3], B[3][3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) A[i][k] = B[i][k] = 0.0;
+#pragma unroll 2
+          for (int q = 0; q < Q; ++q) {
+            const double* ga = &s_g[cc][(q * N + a) * 3];
+            const double* gb = &s_g[cc][(q * N + b) * 3];
+            const double jw = s_jxw[cc][q];
+            const double wl = jw * s_st[cc].s[q].lam, wm = jw * s_st[cc].s[q].mu;
+            const double gb0 = gb[0], gb1 = gb[1], gb2 = gb[2];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              const double la = wl * ga[i], ma = wm * ga[i];
+              A[i][0] += la * gb0;
+              A[i][1] += la * gb1;
+              A[i][2] += la * gb2;
+              B[i][0] += ma * gb0;
+              B[i][1] += ma * gb1;
+              B[i][2] += ma * gb2;
+            }
+          }
+          const double trB = B[0][0] + B[1][1] + B[2][2];
+#pragma unroll
+          for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) K[i][k] = A[i][k] + B[k][i] + (i == k ? trB : 0.0);
+        } else {
+#pragma unroll
+          for (int i = 0; i < V; ++i)
+#pragma unroll
+            for (int k = 0; k < V; ++k) K[i][k] = 0.0;
+#pragma unroll 2
+          for (int q = 0; q < Q; ++q)
+            MAT::tangent(s_st[cc].s[q], &s_g[cc][(q * N + a) * 3], &s_g[cc][(q * N + b) * 3], s_jxw[cc][q], K);
+        }
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+          for (int k = 0; k < V; ++k) acc[m][i * V + k] += K[i][k];
+        if (RES && w + kAsmWaves * m == ds) {
+          // the diagonal slot lists exactly the row's incidences (c, a, a), in cell order
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            double ri = 0.0;
+#pragma unroll 2
+            for (int q = 0; q < Q; ++q)
+#pragma unroll
+              for (int j = 0; j < 3; ++j) ri += s_P[cc][(q * V + i) * 3 + j] * s_g[cc][(q * N + a) * 3 + j];
+            r[i] += ri;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- difference form: row sums of all slots, combined in a fixed order
+  double rsum[DIFF ? VV : 1];
+  if (DIFF) {
+#pragma unroll
+    for (int v = 0; v < VV; ++v) {
+      double p = 0.0;
+#pragma unroll
+      for (int m = 0; m < MAXK; ++m) p += acc[m][v];
+      rsum[v] = p;
+    }
+    // round 1: waves kHalf.. stage, waves 0..kHalf-1 add (pairs w, w + kHalf)
+    if (w >= kHalf)
+#pragma unroll
+      for (int v = 0; v < VV; ++v) s_red[w - kHalf][lane][v] = rsum[v];
+    __syncthreads();
+    if (w < kHalf)
+#pragma unroll
+      for (int v = 0; v < VV; ++v) rsum[v] += s_red[w][lane][v];
+    __syncthreads();
+    // round 2: the pair sums, read by the thread that owns the diagonal slot
+    if (w < kHalf)
+#pragma unroll
+      for (int v = 0; v < VV; ++v) s_red[w][lane][v] = rsum[v];
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < VV; ++v) {
+      double sum = s_red[0][lane][v];
+#pragma unroll
+      for (int o = 1; o < kHalf; ++o) sum += s_red[o][lane][v];
+      rsum[v] = sum;
+    }
+  }
+
+  // ---- stores: one coalesced 64-lane store per value of each owned slot
+  TO* vp = vals + (size_t)sbase * VV + lane;
+#pragma unroll
+  for (int m = 0; m < MAXK; ++m) {
+    const int k = w + kAsmWaves * m;
+    if (k >= L) continue;
+    if (k == ds && dinv) {
+#pragma unroll
+      for (int c = 0; c < V; ++c) {
+        const double d = acc[m][c * V + c];
+        dinv[(int64_t)row * V + c] = d != 0.0 ? TO(1.0 / d) : TO(0);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < VV; ++v) vp[((size_t)k * VV + v) * kSlice] = TO((DIFF && k == ds) ? rsum[v] : acc[m][v]);
+  }
+  if (RES && real && w == (ds >= 0 ? ds % kAsmWaves : 0)) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) res[(int64_t)row * V + i] = (base ? base[(int64_t)row * V + i] : 0.0) + r[i];
+  }
+}
+
+// ------------------------------------------------------------- dispatch
+template <class F>
+static int with_mat8(const fem_material_t& mat, int vec, F&& f) {
+  switch (mat.kind) {
+    case FEM_MAT_POISSON:
+      if (vec == 1) return f(MatPoisson{});
+      break;
+    case FEM_MAT_LINEAR_ELASTIC:
+      if (vec == 3) return f(MatElastic<false>{});
+      break;
+    case FEM_MAT_SIMP:
+      if (vec == 3 && mat.has_param) return f(MatElastic<true>{});
+      break;
+    case FEM_MAT_NEO_HOOKEAN:
+      if (vec == 3) return f(MatNeoHookean{});
+      break;
+    case FEM_MAT_J2_PLASTIC:
+      if (vec == 3 && mat.state) return f(MatJ2{});
+      break;
+    default:
+      break;
+  }
+  set_error("fem_assemble: material kind %d (vec=%d) is assembled by the two-pass path "
+            "(fem_cell_tangent + fem_gather_matrix)", mat.kind, vec);
+  return FEM_ERR_BAD_ARG;
+}
+
+}  // namespace fem
+
+using namespace fem;
+
+extern "C" {
+
+int fem_assembly_plan_counts(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
+                             int32_t* slice_cell_counts, int32_t* err_flag, void* stream) {
+  FEM_CHECK_ARG(pat && inc_ptr && inc_idx && slice_cell_counts && err_flag, "fem_assembly_plan_counts: NULL argument");
+  if (pat->num_slices == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  FEM_HIP(hipMemsetAsync(err_flag, 0, sizeof(int32_t), st));
+  hipLaunchKernelGGL(k_slice_cells<false>, dim3((unsigned)pat->num_slices), dim3(256), 0, st, pat->num_slices,
+                     pat->nodes_per_cell, pat->sell_row, inc_ptr, inc_idx, (const int32_t*)nullptr,
+                     slice_cell_counts, err_flag);
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_assembly_plan_fill(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
+                           const int32_t* slice_cell_ptr, int32_t* slice_cells, int32_t* contrib_loc,
+                           int32_t* err_flag, void* stream) {
+  FEM_CHECK_ARG(pat && inc_ptr && inc_idx && slice_cell_ptr && slice_cells && contrib_loc && err_flag,
+                "fem_assembly_plan_fill: NULL argument");
+  if (pat->num_slices == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_slice_cells<true>, dim3((unsigned)pat->num_slices), dim3(256), 0, st, pat->num_slices,
+                     pat->nodes_per_cell, pat->sell_row, inc_ptr, inc_idx, slice_cell_ptr, slice_cells, err_flag);
+  FEM_LAUNCHED();
+  const int64_t lanes = pat->num_slices * kSlice;
+  hipLaunchKernelGGL(k_contrib_local, dim3((unsigned)ceil_div(lanes, 256)), dim3(256), 0, st, *pat, slice_cell_ptr,
+                     slice_cells, contrib_loc, err_flag);
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_assemble(const fem_mesh_t* mesh, const fem_material_t* mat, const fem_pattern_t* pat,
+                 const fem_assembly_plan_t* plan, const double* u, int32_t precision, int32_t form, void* vals,
+                 void* dinv, const double* base, double* res, void* stream) {
+  FEM_CHECK_ARG(mesh && mat && pat && plan && u && vals, "fem_assemble: NULL argument");
+  FEM_CHECK_ARG(mesh->nodes_per_cell == 8 && mesh->num_quads == 8,
+                "fem_assemble: single-pass assembly covers HEX8 cells with 8 quadrature points");
+  FEM_CHECK_ARG(pat->nodes_per_cell == 8 && pat->vec == mesh->vec && pat->num_cells == mesh->num_cells,
+                "fem_assemble: pattern and mesh disagree");
+  FEM_CHECK_ARG(plan->slice_cell_ptr && plan->slice_cells && plan->contrib_loc, "fem_assemble: plan not built");
+  FEM_CHECK_ARG(plan->max_slots >= 0 && plan->max_slots <= kAsmWaves * kAsmMaxK,
+                "fem_assemble: a slice has %d blocks per row (at most %d)", plan->max_slots, kAsmWaves * kAsmMaxK);
+  FEM_CHECK_ARG((precision == FEM_F64 && form == FEM_FORM_STANDARD) ||
+                    (precision == FEM_F32 && form == FEM_FORM_DIFFERENCE),
+                "fem_assemble: FP64 standard-form or FP32 difference-form storage");
+  FEM_CHECK_ARG(!mat->has_param || mat->param, "fem_assemble: material param missing");
+  if (pat->num_slices == 0) return FEM_OK;
+  hipStream_t st = as_stream(stream);
+  const dim3 grid((unsigned)pat->num_slices), block(kAsmThreads);
+  return with_mat8(*mat, mesh->vec, [&](auto m) -> int {
+    using MAT = decltype(m);
+    auto go = [&](auto res_c) {
+      constexpr bool R = decltype(res_c)::value;
+      if (precision == FEM_F64)
+        hipLaunchKernelGGL((k_assemble_hex8<MAT, double, false, R>), grid, block, 0, st, *mesh, *mat, *pat,
+                           plan->slice_cell_ptr, plan->slice_cells, plan->contrib_loc, u, (double*)vals,
+                           (double*)dinv, base, res);
+      else
+        hipLaunchKernelGGL((k_assemble_hex8<MAT, float, true, R>), grid, block, 0, st, *mesh, *mat, *pat,
+                           plan->slice_cell_ptr, plan->slice_cells, plan->contrib_loc, u, (float*)vals,
+                           (float*)dinv, base, res);
+    };
+    if (res)
+      go(std::true_type{});
+    else
+      go(std::false_type{});
+    FEM_LAUNCHED();
+    return FEM_OK;
+  });
+}
+
+}  // extern "C"

@@ -156,6 +156,11 @@ def _surface_map_kernel(smap, V):
     return kernel
 
 
+def dm_size(dev):
+    """Entries of the per-cell tangent buffer: C * (N * V)^2."""
+    return dev.dm.C * (dev.dm.N * dev.V) ** 2
+
+
 def _as_vector(r):
     """A point map's result as one tensor (maps may return lists of scalars)."""
     if isinstance(r, (list, tuple)):
@@ -206,6 +211,8 @@ class DeviceProblem:
         self.host_status = torch.empty(128, dtype=torch.uint8).pin_memory()
         self._amg = None
         self._amg_uses = 0
+        self._asm_plan = None
+        self.last_residual = None
 
     # ------------------------------------------------------------ pattern
     def _build_pattern(self):
@@ -310,6 +317,15 @@ class DeviceProblem:
                 self._mat_cache[key] = Poisson(0.) if self.V == 1 else LinearElastic(lmbda=0., mu=0.)
         mat = self._mat_cache[key]
         self.material = mat
+        # a closed-form law recognised from a user callable is re-checked at the real
+        # quadrature states of every residual and tangent (_check_law)
+        self._law_check = None
+        if hasattr(pb, 'get_tensor_map') and not mat.tabulated:
+            from .materials import Material
+            fn = pb.get_tensor_map()
+            if not isinstance(fn, Material):
+                self._law_check = dict(fn=fn, key=key, ivs=[_flatten_iv(iv, (C_, Q), dm.device).detach()
+                                                            for iv in ivs])
         if mat.has_param:
             k = getattr(mat, 'param_index', 0)
             if k >= len(ivs):
@@ -400,20 +416,26 @@ class DeviceProblem:
         return total
 
     def _eval_u_independent(self, fn, xq, civ, what, allow_u=False):
-        """fn at u = 0 on every point; None (allow_u) or NotImplementedError when it depends on u."""
+        """fn at u = 0 on every point; None (allow_u) or NotImplementedError when it depends on u.
+
+        The map is evaluated at every point twice more, at two independent random u with
+        distinct components per point (seeded), so a dependence on any combination of the
+        components or one confined to part of the domain is seen: m(u) == m(0) at all
+        points for both draws is required for the map to be a constant load."""
         P = xq.shape[0]
-        if P:
-            idx = torch.as_tensor(onp.unique(onp.linspace(0, P - 1, num=min(P, 5)).astype(int)), device=xq.device)
-            sub = [c[idx] for c in civ]
-            z = torch.zeros((len(idx), self.V), dtype=torch.float64, device=xq.device)
-            u1 = torch.full((len(idx), self.V), 0.37, dtype=torch.float64, device=xq.device)
-            if not torch.allclose(_eval_quad_map(fn, self.V, u1, xq[idx], sub),
-                                  _eval_quad_map(fn, self.V, z, xq[idx], sub), rtol=1e-12, atol=0):
-                if allow_u:
-                    return None
-                raise NotImplementedError(f"a {what} that depends on u is outside the device path")
         u0 = torch.zeros((P, self.V), dtype=torch.float64, device=xq.device)
-        return _eval_quad_map(fn, self.V, u0, xq, civ).contiguous()
+        m0 = _eval_quad_map(fn, self.V, u0, xq, civ).contiguous()
+        if P:
+            gen = torch.Generator(device=xq.device).manual_seed(2024)
+            scale = max(float(m0.abs().max()), 1.0)
+            for _ in range(2):
+                ur = torch.randn((P, self.V), dtype=torch.float64, device=xq.device, generator=gen)
+                mr = _eval_quad_map(fn, self.V, ur, xq, civ)
+                if float((mr - m0).abs().max()) > 1e-13 * scale:
+                    if allow_u:
+                        return None
+                    raise NotImplementedError(f"a {what} that depends on u is outside the device path")
+        return m0
 
     # ------------------------------------------------------------ buffers
     def cell_buffer(self):
@@ -422,15 +444,64 @@ class DeviceProblem:
         return self.buffers['cell']
 
     def matrix_buffers(self, precision):
+        """(SELL values, inverse diagonal) of the matrix in `precision`."""
         key = ('mat', precision)
         if key not in self.buffers:
             dt = torch.float32 if precision == _lib.FEM_F32 else torch.float64
-            dm = self.dm
-            Ke = dm.empty(max(dm.C * dm.N * dm.N * self.V * self.V, 1), dtype=dt)
-            vals = dm.empty(max(self.sell_blocks * self.V * self.V, 1), dtype=dt)
-            dinv = dm.empty(self.n, dtype=dt)
-            self.buffers[key] = (Ke, vals, dinv)
+            vals = self.dm.empty(max(self.sell_blocks * self.V * self.V, 1), dtype=dt)
+            dinv = self.dm.empty(self.n, dtype=dt)
+            self.buffers[key] = (vals, dinv)
         return self.buffers[key]
+
+    def ke_buffer(self, precision):
+        """Per-cell tangent blocks (C, N, N, V, V) of the two-pass assembly."""
+        key = ('ke', precision)
+        if key not in self.buffers:
+            dt = torch.float32 if precision == _lib.FEM_F32 else torch.float64
+            dm = self.dm
+            self.buffers[key] = dm.empty(max(dm.C * dm.N * dm.N * self.V * self.V, 1), dtype=dt)
+        return self.buffers[key]
+
+    _FUSED_KINDS = (_lib.FEM_MAT_POISSON, _lib.FEM_MAT_LINEAR_ELASTIC, _lib.FEM_MAT_SIMP, _lib.FEM_MAT_NEO_HOOKEAN,
+                    _lib.FEM_MAT_J2_PLASTIC)
+
+    def assembly_plan(self):
+        """The single-pass assembly's plan (slice cell lists, slice-local contribution ids), built
+        once per pattern; None where it does not apply (not HEX8 with 8 quadrature points, rows of
+        more than 32 blocks, slices with more than 4096 incidences)."""
+        if self._asm_plan is None:
+            self._asm_plan = False
+            dm, L, s = self.dm, self.L, stream_ptr()
+            sp = self.pat_tensors['slice_ptr']
+            ns = self.pat.num_slices
+            if dm.N == 8 and dm.Q == 8 and dm.C > 0 and ns > 0:
+                max_slots = int(((sp[1:] - sp[:-1]) // 64).max().item())
+                if max_slots <= 32:
+                    counts = dm.empty(ns, dtype=torch.int32)
+                    err = dm.empty(1, dtype=torch.int32)
+                    check(L.fem_assembly_plan_counts(self.pref, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(counts),
+                                                     ptr(err), s), 'fem_assembly_plan_counts')
+                    if int(err.item()) == 0:
+                        sc_ptr = dm.scan(counts)
+                        sc = dm.empty(max(int(sc_ptr[-1].item()), 1), dtype=torch.int32)
+                        loc = dm.empty(max(dm.C * dm.N * dm.N, 1), dtype=torch.int32)
+                        check(L.fem_assembly_plan_fill(self.pref, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(sc_ptr),
+                                                       ptr(sc), ptr(loc), ptr(err), s), 'fem_assembly_plan_fill')
+                        if int(err.item()) == 0:
+                            plan = _lib.FemAssemblyPlan()
+                            plan.slice_cell_ptr, plan.slice_cells, plan.contrib_loc = (
+                                sc_ptr.data_ptr(), sc.data_ptr(), loc.data_ptr())
+                            plan.max_slots = max_slots
+                            self._asm_plan = (plan, (sc_ptr, sc, loc))
+                        else:
+                            logger.info("single-pass assembly plan refused (code %d): two-pass assembly",
+                                        int(err.item()))
+        return self._asm_plan[0] if self._asm_plan else None
+
+    def fused_applies(self, ke_precision):
+        """True when the single-pass kernel assembles this problem's tangent."""
+        return (ke_precision == _lib.FEM_F64 and self.material.kind in self._FUSED_KINDS
+                and self.mass_dyn is None and not self.cell_terms and self.assembly_plan() is not None)
 
     # -------------------------------------------------------------- compute
     def quad_grad(self, u):
@@ -465,7 +536,7 @@ class DeviceProblem:
         S = srf['faces'].shape[0]
         rest = self.buffers[key] + [_flatten_iv(iv, (S,), dm.device).detach() for iv in siv]
         return dict(pm=self._term_map(('face', i), fn), rest=rest, ids=srf['faces'][:, 0].long(), fn=fn,
-                    ivs=siv, lead=(S,))
+                    ivs=siv, lead=(S,), surface=i)
 
     def _term_map(self, slot, fn):
         """The PointMap of one weak-form term (one per slot, kept while the user function is the same object)."""
@@ -526,12 +597,35 @@ class DeviceProblem:
                                       ptr(out), stream_ptr()), 'fem_gather_nodal')
         return out
 
-    def _cell_terms_tangent(self, u, Ke):
+    def _cell_terms_tangent(self, u, Ke, faces=True):
         if not self.cell_terms:
+            return
+        if not faces:   # the cell kernels only (the cell part of the reference's V)
+            terms, self.cell_terms = self.cell_terms, [t for t in self.cell_terms if 'surface' not in t]
+            try:
+                if self.cell_terms:
+                    Ke[:dm_size(self)] += self._cell_terms_eval(u, True)[1].reshape(-1).to(Ke.dtype)
+            finally:
+                self.cell_terms = terms
             return
         K = self._cell_terms_eval(u, True)[1].reshape(-1)
         self.buffers['cell_K'] = K
         Ke[:K.numel()] += K.to(Ke.dtype)
+
+    def face_tangents(self, u):
+        """Per-face tangents [(S, N*V, N*V)] of each surface in the reference's jac layout
+        (compute_face with jac, problem.py:398-424, 457-458): the jacfwd of the surface kernel,
+        zero for a surface map that does not depend on u."""
+        dm = self.dm
+        N, V = dm.N, self.V
+        cs = u.reshape(-1, V)[dm.cells.long()].reshape(dm.C, N * V)
+        out = [torch.zeros((srf['faces'].shape[0], N * V, N * V), dtype=torch.float64, device=dm.device)
+               for srf in self.surfaces]
+        with torch.no_grad():
+            for t in self.cell_terms:
+                if 'surface' in t and t['ids'].numel():
+                    out[t['surface']] += t['pm'](cs[t['ids']], t['rest'], True)[1].reshape(-1, N * V, N * V)
+        return out
 
     def _mass_tangent(self, u, Ke, ke_storage):
         if self.mass_dyn is None:
@@ -557,9 +651,44 @@ class DeviceProblem:
         if jac:
             tab[:, 3 * V:3 * V + 9 * V * V] = D.reshape(n, 9 * V * V)
 
+    def _check_law(self, u):
+        """Compare the closed-form law recognised from the user's tensor map with the map itself
+        at real quadrature states: the 48 points of largest |grad u| and 48 spread over the mesh.
+        On a difference the problem switches to the tabulated law (the map itself) for good."""
+        lc = self._law_check
+        if lc is None or lc.get('u') is u:
+            return
+        lc['u'] = u
+        G = self.quad_grad(u)
+        n = G.shape[0]
+        if n == 0:
+            return
+        k = min(n, 48)
+        mag = G.reshape(n, -1).abs().amax(1)
+        idx = torch.unique(torch.cat([torch.topk(mag, k).indices,
+                                      torch.linspace(0, n - 1, k, device=G.device).long()]))
+        Gs, ivs = G[idx], [t[idx] for t in lc['ivs']]
+        from .materials import PointMap, Tabulated
+        pm = self._mat_cache.get(('law_pm', lc['key']))
+        if pm is None:
+            pm = self._mat_cache[('law_pm', lc['key'])] = PointMap(lc['fn'], (self.V, 3), (self.V, 3), 'tensor map')
+        with torch.no_grad():
+            Pu = pm(Gs, ivs)[0]
+            mat = self.material
+            Pd = vmap(lambda g, *r: mat(g, *r))(Gs, *ivs).reshape(Pu.shape)
+        scale = max(float(Pu.abs().max()), float(Pd.abs().max()), 1e-300)
+        diff = float((Pu - Pd).abs().max())
+        if diff <= 1e-9 * scale:
+            return
+        logger.warning("the tensor map differs from %r at the quadrature states of this solve (relative "
+                       "difference %.3g): switching to the tabulated law", self.material, diff / scale)
+        self._mat_cache[lc['key']] = Tabulated(lc['fn'], self.V)
+        self.prepare()
+
     def residual_unconstrained(self, u):
         """R(u) without Dirichlet rows, FP64 (problem.py:439-445)."""
         dm = self.dm
+        self._check_law(u)
         self._tabulate(u, False)
         cell = self.cell_buffer()
         check(self.L.fem_cell_residual(dm.ref, self.mref, ptr(u), ptr(cell), stream_ptr()), 'fem_cell_residual')
@@ -581,36 +710,59 @@ class DeviceProblem:
 
     def residual(self, u, scale=1.0):
         """R(u) with Dirichlet rows u - g (solver.py:290-304)."""
-        res = self.residual_unconstrained(u)
+        return self.apply_residual_bc(u, self.residual_unconstrained(u), scale)
+
+    def apply_residual_bc(self, u, res, scale=1.0):
+        """res[d] = u[d] - scale g[d] on the Dirichlet dofs, in place (apply_bc_vec, solver.py:290-304)."""
         check(self.L.fem_residual_bc(self.n_bc, ptr(self.bc_dofs), ptr(self.bc_vals), ptr(u), ptr(res),
                                      float(scale), stream_ptr()), 'fem_residual_bc')
         return res
 
-    def tangent(self, u, precision, ke_precision=None):
+    def tangent(self, u, precision, ke_precision=None, residual=False, fused=None):
         """Assemble the row-eliminated tangent in `precision` (problem.py:447-460, solver.py:522-529).
 
-        ke_precision: arithmetic of the per-cell tangents (default: `precision`); FP64 cell
-        tangents are summed in FP64 and rounded once into an FP32 matrix.  FEM_BF16X3 (HEX27
-        elastic laws) forms them from split-BF16 MFMA products and stores them in FP32.
+        ke_precision: arithmetic of the per-cell tangents (default: FP64 for an FP32 matrix, else
+        `precision`); FP64 cell tangents are summed in FP64 and rounded once into an FP32
+        matrix.  FEM_BF16X3 (HEX27 elastic laws) forms them from split-BF16 MFMA products and
+        stores them in FP32.  HEX8 problems with a built-in law assemble in one pass
+        (fem_assemble: no per-cell tangent buffer); the rest through per-cell tangents and a
+        gather.  With `residual`, R(u) without Dirichlet rows is formed too (from the same pass
+        when it is single) and kept in self.last_residual.
         """
         dm = self.dm
-        ke_precision = precision if ke_precision is None else ke_precision
-        _, vals, dinv = self.matrix_buffers(precision)
-        ke_storage = _lib.FEM_F32 if ke_precision == _lib.FEM_BF16X3 else ke_precision
-        Ke = self.matrix_buffers(ke_storage)[0]
+        ke_precision = _lib.FEM_F64 if ke_precision is None and precision == _lib.FEM_F32 else \
+            (precision if ke_precision is None else ke_precision)
+        vals, dinv = self.matrix_buffers(precision)
         s = stream_ptr()
         # FP32 matrices are stored in difference form (exact on translations), FP64 in standard form
         form = _lib.FEM_FORM_DIFFERENCE if precision == _lib.FEM_F32 else _lib.FEM_FORM_STANDARD
+        self._check_law(u)
         self._tabulate(u, True)
-        check(self.L.fem_cell_tangent(dm.ref, self.mref, ke_precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
-        self._mass_tangent(u, Ke, ke_storage)
-        self._cell_terms_tangent(u, Ke)
-        check(self.L.fem_gather_matrix(self.pref, ke_storage, ptr(Ke), precision, form, ptr(vals), ptr(dinv), s),
-              'fem_gather_matrix')
+        if fused is None:
+            fused = self.fused_applies(ke_precision)
+        Ke, res = None, None
+        if fused:
+            plan = self.assembly_plan()
+            base, res = None, None
+            if residual:
+                base, res = self._base_load(u), dm.empty(self.n)
+            check(self.L.fem_assemble(dm.ref, self.mref, self.pref, C.byref(plan), ptr(u), precision, form,
+                                      ptr(vals), ptr(dinv), ptr(base), ptr(res), s), 'fem_assemble')
+        else:
+            ke_storage = _lib.FEM_F32 if ke_precision == _lib.FEM_BF16X3 else ke_precision
+            Ke = self.ke_buffer(ke_storage)
+            check(self.L.fem_cell_tangent(dm.ref, self.mref, ke_precision, ptr(u), ptr(Ke), s), 'fem_cell_tangent')
+            self._mass_tangent(u, Ke, ke_storage)
+            self._cell_terms_tangent(u, Ke)
+            check(self.L.fem_gather_matrix(self.pref, ke_storage, ptr(Ke), precision, form, ptr(vals), ptr(dinv),
+                                           s), 'fem_gather_matrix')
+            if residual:
+                res = self.residual_unconstrained(u)
         check(self.L.fem_matrix_bc_rows(self.pref, precision, ptr(vals), self.n_bc, ptr(self.bc_dofs), ptr(dinv), s),
               'fem_matrix_bc_rows')
         self.current = (precision, Ke, vals, dinv)
         self.form = form
+        self.last_residual = res
         return self.current
 
     def operator(self, u):
@@ -823,8 +975,10 @@ class DeviceProblem:
               'fem_export_csr')
         return indptr.cpu().numpy(), indices[:nnz].cpu().numpy(), data[:nnz].cpu().numpy()
 
-    def cell_tangents(self, u, precision=_lib.FEM_F64):
-        """Per-cell tangents (C, N*V, N*V) in the reference's jac layout (problem.py:262-266, 453)."""
+    def cell_tangents(self, u, precision=_lib.FEM_F64, faces=True):
+        """Per-cell tangents (C, N*V, N*V) in the reference's jac layout (problem.py:262-266, 453).
+        With faces=False the per-face terms of u-dependent surface maps and surface kernels are
+        left out (the reference keeps them as separate face blocks, problem.py:457-458)."""
         dm = self.dm
         dt = torch.float32 if precision in (_lib.FEM_F32, _lib.FEM_BF16X3) else torch.float64
         Ke = dm.empty(dm.C * dm.N * dm.N * self.V * self.V, dtype=dt)
@@ -832,6 +986,6 @@ class DeviceProblem:
         check(self.L.fem_cell_tangent(dm.ref, self.mref, precision, ptr(u), ptr(Ke), stream_ptr()),
               'fem_cell_tangent')
         self._mass_tangent(u, Ke, _lib.FEM_F32 if dt == torch.float32 else _lib.FEM_F64)
-        self._cell_terms_tangent(u, Ke)
+        self._cell_terms_tangent(u, Ke, faces)
         N, V = dm.N, self.V
         return Ke.reshape(dm.C, N, N, V, V).permute(0, 1, 3, 2, 4).reshape(dm.C, N * V, N * V)

@@ -56,6 +56,11 @@ class FemConstraint(C.Structure):
                 ('pt_ptr', _vp), ('pt_idx', _vp), ('pt_val', _vp)]
 
 
+class FemAssemblyPlan(C.Structure):
+    _fields_ = [('slice_cell_ptr', _vp), ('slice_cells', _vp), ('contrib_loc', _vp), ('max_slots', _i32),
+                ('pad', _i32)]
+
+
 class FemPcgOptions(C.Structure):
     _fields_ = [('rtol', _f64), ('atol', _f64), ('maxiter', _i32), ('check_every', _i32),
                 ('precision', _i32), ('precond', _i32), ('reliable_delta', _f64),
@@ -122,6 +127,10 @@ _SIGNATURES = {
     'fem_spmv': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp]),
     'fem_export_csr': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     'fem_operator_apply': (C.c_int, [_P(FemOperator), _vp, _vp, _vp]),
+    'fem_assembly_plan_counts': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _vp]),
+    'fem_assembly_plan_fill': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_assemble': (C.c_int, [_P(FemMesh), _P(FemMaterial), _P(FemPattern), _P(FemAssemblyPlan), _vp, _i32,
+                               _i32, _vp, _vp, _vp, _vp, _vp]),
     'fem_csr_spmv': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fem_bicgstab_ws_bytes': (_i64, [_P(FemPattern), _P(FemConstraint)]),
     'fem_bicgstab_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemConstraint), _vp, _vp, _P(FemPcgOptions),

@@ -154,9 +154,10 @@ class Problem:
         dev.prepare()
         u = self._flat(sol_list)
         from . import _lib
-        dev.tangent(u, _lib.FEM_F64)
+        # residual and FP64 tangent at u, in one pass where the single-pass assembly applies
+        dev.tangent(u, _lib.FEM_F64, residual=True)
         self._last_u = u
-        return self.unflatten_fn_sol_list(dev.residual_unconstrained(u))
+        return self.unflatten_fn_sol_list(dev.last_residual)
 
     # ------------------------------------------- reference COO exports
     @property
@@ -170,16 +171,15 @@ class Problem:
 
     @property
     def V(self):
-        """Tangent values in COO order after newton_update (problem.py:453-458).
-
-        Face blocks are zero: surface maps on the device path do not depend on u.
-        """
+        """Tangent values in COO order after newton_update (problem.py:453-458): the cell
+        blocks (laplace, mass and universal cell kernels), then the face blocks of each
+        surface (zero for a surface map that does not depend on u)."""
         u = getattr(self, '_last_u', None)
         if u is None:
             raise AttributeError("V is available after newton_update")
-        cells = self.device().cell_tangents(u).reshape(-1).cpu().numpy()
-        nv = self.fes[0].num_nodes * self.fes[0].vec
-        faces = [onp.zeros(len(bi) * nv * nv) for bi in self.boundary_inds_list]
+        dev = self.device()
+        cells = dev.cell_tangents(u, faces=False).reshape(-1).cpu().numpy()
+        faces = [f.reshape(-1).cpu().numpy() for f in dev.face_tangents(u)]
         return onp.hstack([cells] + faces)
 
     def _coo_inds(self):

@@ -209,7 +209,19 @@ def solver(problem, solver_options={}):
     rel_tol = cfg.get('rel_tol', 1e-8)
     tol = cfg.get('tol', 1e-6)
 
-    res = reduce(dev.residual(full(dofs)))
+    # the first residual and tangent at the same u: one pass where the single-pass assembly
+    # applies (newton_update, problem.py:477-491)
+    t0 = time.perf_counter()
+    have_tangent = not constrained and dev.fused_applies(
+        lin['assembly_code'] if lin['assembly_code'] is not None else _lib.FEM_F64)
+    if have_tangent:
+        u_full = full(dofs)
+        dev.tangent(u_full, lin['precision_code'], lin['assembly_code'], residual=True)
+        res = dev.apply_residual_bc(u_full, dev.last_residual)
+        torch.cuda.synchronize()
+        timing['global_matrix'] += time.perf_counter() - t0
+    else:
+        res = reduce(dev.residual(full(dofs)))
     res_val = dev.norm(res)
     res_val0 = res_val
     rel = res_val / res_val0 if res_val0 != 0 else float('nan')
@@ -220,9 +232,11 @@ def solver(problem, solver_options={}):
         n_iters += 1
         t0 = time.perf_counter()
         u_full = full(dofs)
-        dev.tangent(u_full, lin['precision_code'], lin['assembly_code'])
-        torch.cuda.synchronize()
-        timing['global_matrix'] += time.perf_counter() - t0
+        if not have_tangent:
+            dev.tangent(u_full, lin['precision_code'], lin['assembly_code'])
+            torch.cuda.synchronize()
+            timing['global_matrix'] += time.perf_counter() - t0
+        have_tangent = False
         b = -res
         x0 = reduce(_bc_increment(dev, u_full))
         t0 = time.perf_counter()

@@ -395,3 +395,27 @@ def periodic_p_mat(points, vec, pairs, tol=1e-5):
     cols = red[target]
     assert (cols >= 0).all()
     return sp.csr_matrix((np.ones(n), (np.arange(n), cols)), shape=(n, len(keep)))
+
+
+# Linear elasticity below an energy threshold, stiffening above it:
+# psi = 1/2 eps:C:eps + BETA max(q - Q0, 0)^3 with q = eps:eps, so
+# P = sigma + 6 BETA max(q - Q0, 0)^2 eps (a symmetric tangent).  Every state
+# the law probes of jax_fem_amd.materials.identify visit has q < Q0.
+TH_LM, TH_MU = 0.3 / (1.3 * 0.4), 1. / 2.6          # E = 1, nu = 0.3
+TH_Q0, TH_BETA = 100., 2e-5
+
+
+def threshold_torch(u_grad):
+    eps = 0.5 * (u_grad + u_grad.T)
+    sig = TH_LM * torch.trace(eps) * torch.eye(3, dtype=u_grad.dtype, device=u_grad.device) + 2. * TH_MU * eps
+    ex = torch.clamp((eps * eps).sum() - TH_Q0, min=0.)
+    return sig + 6. * TH_BETA * ex ** 2 * eps
+
+
+def threshold_np(G, theta=None):
+    eps = 0.5 * (G + np.swapaxes(G, -1, -2))
+    tr = np.trace(eps, axis1=-2, axis2=-1)[..., None, None]
+    sig = TH_LM * tr * np.eye(3) + 2. * TH_MU * eps
+    q = np.sum(eps * eps, axis=(-2, -1))
+    ex = np.where(np.real(q) > TH_Q0, q - TH_Q0, 0.)[..., None, None]
+    return sig + 6. * TH_BETA * ex ** 2 * eps

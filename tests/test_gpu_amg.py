@@ -245,3 +245,26 @@ def test_amg_level0_fp16_scaled_smoother(gpu):
         its[level0] = p._solve_stats['linear'][0]['iterations']
         assert np.linalg.norm(x.reshape(-1).cpu().numpy() - ref) <= 1e-9 * np.linalg.norm(ref)
     assert its['fp16'] <= its['fp32'] + 3, its
+
+
+def test_sell_to_bcsr_on_a_length_sorted_pattern(gpu):
+    # HEX27 rows have 27 to 125 blocks, so the SELL rows are length-sorted within windows
+    # (sell_row is not the identity): the block-CSR copy must still put each row's blocks
+    # in pattern order, bit for bit
+    from jax_fem_amd._lib import check, ptr, stream_ptr
+    from jax_fem_amd.generate_mesh import Mesh, box_mesh_hex27
+    m = box_mesh_hex27(4, 3, 3, 1., 1., 1.)
+    clamp = [[lambda x: isclose(x[2], 0.)] * 3, [0, 1, 2], [lambda x: 0.] * 3]
+    p = LoadedElastic(Mesh(m.points, m.cells_dict['hexahedron27']), vec=3, dim=3, ele_type='HEX27',
+                      quadrature_order=4, dirichlet_bc_info=clamp, location_fns=[lambda x: isclose(x[2], 1.)])
+    dev, A, (gp, gi), mask, pts, b = _assembled(p)
+    assert dev.sell_sorted
+    sell_row = dev.pat_tensors['sell_row'].cpu().numpy()
+    assert not np.array_equal(sell_row[:dev.dm.Nn], np.arange(dev.dm.Nn))
+    vals = dev.current[2]
+    out = torch.empty(dev.nnzb * 9, dtype=torch.float64, device='cuda')
+    check(dev.L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals), ptr(out), stream_ptr()), 'fem_amg_sell_to_bcsr')
+    B = sp.bsr_matrix((out.cpu().numpy().reshape(-1, 3, 3), gi, gp), shape=(dev.n, dev.n)).tocsr()
+    B.sort_indices()
+    assert np.array_equal(B.indptr, A.indptr) and np.array_equal(B.indices, A.indices)
+    assert np.array_equal(B.data, A.data)

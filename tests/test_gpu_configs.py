@@ -9,6 +9,7 @@ import pytest
 import torch
 
 from oracle import fem as OF
+from oracle_setup import oracle_problem
 from problems import box, isclose
 
 from jax_fem_amd.generate_mesh import Mesh, box_mesh_hex27
@@ -210,9 +211,9 @@ def test_hex27_efield_adjoint_vs_oracle(gpu, precision):
     J.backward()
     g = e.grad.cpu().numpy()
     Q = p.fes[0].num_quads
-    o = _oracle(p, OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., NU))), Emin=0., E0=1., p=1.),
-                theta=np.repeat(E.cpu().numpy()[:, None], Q, axis=1))
-    o.load = p.device().load.cpu().numpy().reshape(-1, 3)
+    # the oracle integrates the body force itself (oracle_setup), nothing is copied from the device
+    o = oracle_problem(p, OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., NU))), Emin=0., E0=1.,
+                                      p=1.), theta=np.repeat(E.cpu().numpy()[:, None], Q, axis=1))
     uo = o.newton(linear='direct')
     assert _rel(u.detach().cpu().numpy(), uo) < 1e-9
     gq, lam = o.adjoint_theta(uo, -o.load.reshape(-1))

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import torch
 
+import oracle_setup
 from oracle import fem as OF
 from problems import box, isclose, periodic_p_mat
 from test_gpu_parity import oracle_problem
@@ -48,9 +49,7 @@ def test_periodic_poisson_vs_oracle(gpu):
     p.set_params(theta)
     sol = solver(p, {'jax_solver': {'tol': 1e-12, 'atol': 0.}})[0].reshape(-1).cpu().numpy()
     dev = p.device()
-    o = oracle_problem(p, OF.Material('poisson', k=1.))
-    o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
-    o.load = dev.load.cpu().numpy().reshape(-1, 1)
+    o = oracle_setup.oracle_problem(p, OF.Material('poisson', k=1.))
     ref = o.newton_constrained(p.P_mat)
     assert np.linalg.norm(sol - ref) <= 1e-9 * np.linalg.norm(ref)
     # the solution is periodic: u(0, y, z) = u(1, y, z)

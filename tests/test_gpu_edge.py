@@ -6,6 +6,7 @@ import pytest
 import torch
 
 from oracle import fem as OF
+from oracle_setup import oracle_problem
 from problems import LoadedElastic, box, isclose
 
 from jax_fem_amd import materials as M
@@ -59,10 +60,8 @@ def test_single_cell(gpu, ele):
     assert dev.dm.C == 1 and dev.pat.num_slices == 1
     sol = solver(p, {'jax_solver': {'precision': 'fp64', 'tol': 1e-13, 'atol': 0.}})[0].cpu().numpy()
     fe = p.fes[0]
-    o = OF.Problem(fe.points, fe.cells, 3, ele_type=ele, quadrature_order=order,
-                   mat=OF.Material('elastic', **dict(zip(('lmbda', 'mu'), OF.lame(70e3, 0.3)))))
-    o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
-    o.load = dev.load.cpu().numpy().reshape(-1, 3)
+    o = oracle_problem(p, OF.Material('elastic', **dict(zip(('lmbda', 'mu'), OF.lame(70e3, 0.3)))))
+    assert np.array_equal(o.bc_dofs, dev.bc_dofs.cpu().numpy())
     ref = o.newton(linear='direct').reshape(-1, 3)
     assert np.linalg.norm(sol - ref) / np.linalg.norm(ref) < 1e-10
 
@@ -97,9 +96,6 @@ def test_partial_last_slice(gpu, n):
     for precision in ('fp64', 'mixed'):
         sol = solver(p, {'jax_solver': {'precision': precision, 'tol': 1e-12, 'atol': 0.}})[0].cpu().numpy()
         fe = p.fes[0]
-        o = OF.Problem(fe.points, fe.cells, 3, mat=OF.Material('elastic', **dict(zip(('lmbda', 'mu'),
-                                                                                  OF.lame(70e3, 0.3)))))
-        o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
-        o.load = dev.load.cpu().numpy().reshape(-1, 3)
+        o = oracle_problem(p, OF.Material('elastic', **dict(zip(('lmbda', 'mu'), OF.lame(70e3, 0.3)))))
         ref = o.newton(linear='direct').reshape(-1, 3)
         assert np.linalg.norm(sol - ref) / np.linalg.norm(ref) < 1e-8, precision

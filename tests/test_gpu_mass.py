@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 import torch
 
+import oracle_setup
 from oracle import fem as OF
 from problems import (HEAT_DT, LinearPoisson, UserMapProblem, box, heat_np, heat_torch, isclose, reaction_np,
                       reaction_torch, spring_np, spring_torch)
@@ -20,12 +21,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _oracle(p, mat, fn, ivs=()):
-    dev = p.device()
-    dev.refresh_bc()
-    o = oracle_problem(p, mat)
-    o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
-    if dev.load is not None:
-        o.load = dev.load.cpu().numpy().reshape(-1, p.fes[0].vec)
+    # Dirichlet data and surface loads from the problem's own definition; the mass map
+    # depends on u and stays in the oracle's residual and tangent
+    o = oracle_setup.oracle_problem(p, mat, mass=False)
     o.set_mass_map(fn, ivs)
     return o
 

@@ -9,6 +9,7 @@ import pytest
 import scipy.sparse as sp
 import torch
 
+import oracle_setup
 from conftest import golden
 from oracle import basis as OB
 from oracle import fem as OF
@@ -314,7 +315,7 @@ def test_gather_with_unaligned_index_lists(gpu, shift):
     p, o, u = make_material_problem('neo_hookean')
     dev = p.device()
     dev.prepare()
-    dev.tangent(torch.as_tensor(u, device='cuda'), _lib.FEM_F64)
+    dev.tangent(torch.as_tensor(u, device='cuda'), _lib.FEM_F64, fused=False)   # the two-pass path keeps Ke
     Ke = dev.current[1]
     L = _lib.lib()
     pat = _lib.FemPattern.from_buffer_copy(dev.pat)
@@ -521,11 +522,9 @@ def test_adjoint_simp_vs_oracle_and_fd(gpu, precision):
     J.backward()
     g = th.grad.cpu().numpy()
     # oracle: same problem, adjoint by sparse direct solves
-    o = oracle_problem(p, OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., 0.3))), Emin=1e-3, E0=1., p=3.),
-                       theta=np.repeat(theta.cpu().numpy()[:, None], 8, axis=1))
-    dev = p.device()
-    o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
-    o.load = dev.load.cpu().numpy().reshape(-1, 3)
+    o = oracle_setup.oracle_problem(p, OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., 0.3))),
+                                                   Emin=1e-3, E0=1., p=3.),
+                                    theta=np.repeat(theta.cpu().numpy()[:, None], 8, axis=1))
     u = o.newton(linear='direct')
     gq, lam = o.adjoint_theta(u, -o.load.reshape(-1))
     go = gq.sum(1)

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 import torch
 
+import oracle_setup
 from oracle import fem as OF
 from problems import (UserMapProblem, box, diffusion_np, diffusion_torch, fiber_torch, isclose, scaled_svk_np,
                       scaled_svk_torch, svk_np, svk_torch)
@@ -92,10 +93,7 @@ def test_svk_newton_vs_oracle(gpu, opts):
     p = _svk_problem()
     sol = solver(p, opts)[0].reshape(-1).cpu().numpy()
     assert p._solve_stats['newton_iterations'] >= 3   # a nonlinear solve
-    o = oracle_problem(p, OF.Material('user', fn=svk_np))
-    dev = p.device()
-    o.bc_dofs, o.bc_vals = dev.bc_dofs.cpu().numpy().astype(np.int64), dev.bc_vals.cpu().numpy()
-    o.load = dev.load.cpu().numpy().reshape(-1, 3)
+    o = oracle_setup.oracle_problem(p, OF.Material('user', fn=svk_np))
     ref = o.newton(linear='direct')
     assert np.linalg.norm(sol - ref) <= 1e-8 * np.linalg.norm(ref)
 

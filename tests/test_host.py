@@ -230,3 +230,23 @@ def test_identify_reference_j2_return_map():
         np.testing.assert_allclose(mat(G, so, eo).numpy(), f(G, so, eo).numpy(), rtol=1e-12, atol=1e-9)
         st = np.concatenate([so.numpy().reshape(9), eo.numpy().reshape(9)])
         np.testing.assert_allclose(o.stress(G.numpy(), st), f(G, so, eo).numpy(), rtol=1e-12, atol=1e-9)
+
+
+def test_mass_map_u_dependence_probe():
+    # a map that depends on u only through component differences, one confined to part of
+    # the domain, and a constant: only the last is a constant load
+    from jax_fem_amd._device import DeviceProblem
+
+    class _Fake:
+        V = 3
+    xq = torch.as_tensor(np.random.default_rng(0).uniform(0., 1., (400, 3)))
+    diff = lambda u, x: 10. * torch.stack([u[0] - u[1], u[1] - u[2], u[2] - u[0]]) - 1.
+    local = lambda u, x: torch.where(x[0] > 0.97, u, torch.zeros_like(u)) + 1.
+    const = lambda u, x: torch.stack([x[0], 2. + 0. * x[1], x[2] ** 2])
+    probe = lambda fn: DeviceProblem._eval_u_independent(_Fake(), fn, xq, [], 'mass_map', allow_u=True)
+    assert probe(diff) is None
+    assert int((xq[:, 0] > 0.97).sum()) > 0 and probe(local) is None
+    m = probe(const)
+    assert m is not None and torch.equal(m, torch.stack([xq[:, 0], 2. + 0. * xq[:, 1], xq[:, 2] ** 2], 1))
+    with pytest.raises(NotImplementedError):
+        DeviceProblem._eval_u_independent(_Fake(), diff, xq, [], 'surface_map', allow_u=False)
