@@ -68,6 +68,9 @@ enum {                                       /* storage form of the matrix    */
                                  in difference form                          */
 };
 enum {                                       /* constitutive laws (tensor_map)*/
+  FEM_MAT_NONE = 0,           /* no laplace term (a problem whose weak form is all
+                                 universal kernels): the cell kernels write zeros
+                                                                  (vec 1 to 4) */
   FEM_MAT_POISSON = 1,        /* P = c0 * grad u                      (vec 1) */
   FEM_MAT_LINEAR_ELASTIC = 2, /* sigma = c0 tr(eps) I + 2 c1 eps      (vec 3) */
   FEM_MAT_SIMP = 3,           /* as LINEAR_ELASTIC with lambda = c0 E(t),
@@ -96,7 +99,8 @@ enum {                                       /* constitutive laws (tensor_map)*/
 typedef struct {
   int32_t nodes_per_cell;        /* N: 8 (HEX8) or 27 (HEX27)                 */
   int32_t num_quads;             /* Q                                         */
-  int32_t vec;                   /* 1 or 3                                    */
+  int32_t vec;                   /* 1 or 3; 1 to 4 (the sum over the variables of
+                                    a several-variable problem) with FEM_MAT_NONE */
   int32_t num_face_quads;        /* FQ                                        */
   int64_t num_cells;
   int64_t num_nodes;

@@ -24,10 +24,13 @@
 namespace fem {
 
 // ------------------------------------------------------------------ tangent
+// Quadrature points are staged Cfg::Q at a time; each thread keeps the blocks
+// of its PPT (a, b) pairs in registers across the chunks.
 template <class Cfg, class MAT, typename T>
 __global__ __launch_bounds__(256) void k_cell_tangent(fem_mesh_t mesh, fem_material_t mat,
                                                       const double* __restrict__ u, T* __restrict__ Ke) {
   constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW, V = MAT::V;
+  constexpr int PPT = (N * N + TPC - 1) / TPC;  // (a, b) pairs per thread
   using St = typename MAT::template St<T>;
   __shared__ T s_dN[Q][N][3];
   __shared__ double s_w[Q];
@@ -38,49 +41,68 @@ __global__ __launch_bounds__(256) void k_cell_tangent(fem_mesh_t mesh, fem_mater
   const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
   const int64_t c = (int64_t)blockIdx.x * CPW + w;
   const bool active = c < mesh.num_cells;
-  load_tables(mesh, s_dN, s_w);
-  __syncthreads();
-  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  const int Qt = mesh.num_quads;
   CellLds<N, Q, T>& L = s_cell[w];
-  if (MAT::kNeedsU && active && t < N) {
+  T K[PPT][V][V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) s_u[w][t][i] = T(u[(int64_t)s_node[w][t] * V + i]);
-  }
-  __syncthreads();
-  if (active) {
-    for (int q = t; q < Q; q += TPC) {
-      T G[V][3];
-      if (MAT::kNeedsU) {
-        quad_grad<N, Q, V, T>(L, s_u[w], q, G);
-      } else {
+  for (int p = 0; p < PPT; ++p)
 #pragma unroll
-        for (int i = 0; i < V; ++i)
+    for (int i = 0; i < V; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) G[i][j] = T(0);
+      for (int k = 0; k < V; ++k) K[p][i][k] = T(0);
+  for (int q0 = 0; q0 < Qt; q0 += Q) {
+    const int qn = min(Q, Qt - q0);
+    __syncthreads();  // the previous chunk's tables and states are no longer read
+    load_tables_chunk(mesh, q0, qn, s_dN, s_w);
+    __syncthreads();
+    stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, L, s_node[w], qn);
+    if (MAT::kNeedsU && active && t < N) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) s_u[w][t][i] = T(u[(int64_t)s_node[w][t] * V + i]);
+    }
+    __syncthreads();
+    if (active) {
+      for (int q = t; q < qn; q += TPC) {
+        T G[V][3];
+        if (MAT::kNeedsU) {
+          quad_grad<N, Q, V, T>(L, s_u[w], q, G);
+        } else {
+#pragma unroll
+          for (int i = 0; i < V; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) G[i][j] = T(0);
+        }
+        s_st[w][q] = MAT::template prep<T>(mat, c * Qt + q0 + q, G);
       }
-      s_st[w][q] = MAT::template prep<T>(mat, c * Q + q, G);
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int p = 0; p < PPT; ++p) {
+        const int pr = t + p * TPC;
+        if (pr < N * N) {
+          const int a = pr / N, b = pr % N;
+          for (int q = 0; q < qn; ++q) MAT::tangent(s_st[w][q], L.g[q][a], L.g[q][b], L.jxw[q], K[p]);
+        }
+      }
     }
   }
-  __syncthreads();
   // HEX8: a cell's blocks are staged in LDS and stored lane-contiguously (each store
   // instruction writes 64 consecutive values of the cell's Ke instead of one value
   // of 64 blocks 72 bytes apart)
   constexpr bool kStage = N == 8;
   __shared__ T s_K[kStage ? CPW : 1][kStage ? N * N * V * V : 1];
   if (active) {
-    for (int pr = t; pr < N * N; pr += TPC) {
-      const int a = pr / N, b = pr % N;
-      T K[V][V];
 #pragma unroll
-      for (int i = 0; i < V; ++i)
+    for (int p = 0; p < PPT; ++p) {
+      const int pr = t + p * TPC;
+      if (pr < N * N) {
+        T* out = kStage ? &s_K[w][pr * V * V] : Ke + ((size_t)c * N * N + pr) * (V * V);
 #pragma unroll
-        for (int k = 0; k < V; ++k) K[i][k] = T(0);
-      for (int q = 0; q < Q; ++q) MAT::tangent(s_st[w][q], L.g[q][a], L.g[q][b], L.jxw[q], K);
-      T* out = kStage ? &s_K[w][pr * V * V] : Ke + ((size_t)c * N * N + pr) * (V * V);
+        for (int i = 0; i < V; ++i)
 #pragma unroll
-      for (int i = 0; i < V; ++i)
-#pragma unroll
-        for (int k = 0; k < V; ++k) out[i * V + k] = K[i][k];
+          for (int k = 0; k < V; ++k) out[i * V + k] = K[p][i][k];
+      }
     }
   }
   if (kStage) {
@@ -100,6 +122,7 @@ __global__ __launch_bounds__(256) void k_cell_vector(fem_mesh_t mesh, fem_materi
                                                      const double* __restrict__ u, const double* __restrict__ v,
                                                      double* __restrict__ re) {
   constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW, V = MAT::V;
+  constexpr int RPT = (N * V + TPC - 1) / TPC;  // (a, i) rows per thread
   __shared__ T s_dN[Q][N][3];
   __shared__ double s_w[Q];
   __shared__ CellLds<N, Q, T> s_cell[CPW];
@@ -110,46 +133,63 @@ __global__ __launch_bounds__(256) void k_cell_vector(fem_mesh_t mesh, fem_materi
   const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
   const int64_t c = (int64_t)blockIdx.x * CPW + w;
   const bool active = c < mesh.num_cells;
-  load_tables(mesh, s_dN, s_w);
-  __syncthreads();
-  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  const int Qt = mesh.num_quads;
   CellLds<N, Q, T>& L = s_cell[w];
-  if (active && t < N) {
-    const int64_t n = s_node[w][t];
+  T r[RPT];
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      s_u[w][t][i] = T(u[n * V + i]);
-      if (MODE == 1) s_v[w][t][i] = T(v[n * V + i]);
-    }
-  }
-  __syncthreads();
-  if (active) {
-    for (int q = t; q < Q; q += TPC) {
-      T G[V][3], P[V][3];
-      quad_grad<N, Q, V, T>(L, s_u[w], q, G);
-      const auto st = MAT::template prep<T>(mat, c * Q + q, G);
-      if (MODE == 0) {
-        MAT::stress(st, G, P);
-      } else {
-        T dG[V][3];
-        quad_grad<N, Q, V, T>(L, s_v[w], q, dG);
-        MAT::jvp(st, dG, P);
+  for (int p = 0; p < RPT; ++p) r[p] = T(0);
+  for (int q0 = 0; q0 < Qt; q0 += Q) {
+    const int qn = min(Q, Qt - q0);
+    __syncthreads();
+    load_tables_chunk(mesh, q0, qn, s_dN, s_w);
+    __syncthreads();
+    stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, L, s_node[w], qn);
+    if (active && t < N) {
+      const int64_t n = s_node[w][t];
+#pragma unroll
+      for (int i = 0; i < V; ++i) {
+        s_u[w][t][i] = T(u[n * V + i]);
+        if (MODE == 1) s_v[w][t][i] = T(v[n * V + i]);
       }
-      const T jw = L.jxw[q];
+    }
+    __syncthreads();
+    if (active) {
+      for (int q = t; q < qn; q += TPC) {
+        T G[V][3], P[V][3];
+        quad_grad<N, Q, V, T>(L, s_u[w], q, G);
+        const auto st = MAT::template prep<T>(mat, c * Qt + q0 + q, G);
+        if (MODE == 0) {
+          MAT::stress(st, G, P);
+        } else {
+          T dG[V][3];
+          quad_grad<N, Q, V, T>(L, s_v[w], q, dG);
+          MAT::jvp(st, dG, P);
+        }
+        const T jw = L.jxw[q];
 #pragma unroll
-      for (int i = 0; i < V; ++i)
+        for (int i = 0; i < V; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) s_P[w][q][i][j] = P[i][j] * jw;
+          for (int j = 0; j < 3; ++j) s_P[w][q][i][j] = P[i][j] * jw;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int p = 0; p < RPT; ++p) {
+        const int it = t + p * TPC;
+        if (it < N * V) {
+          const int a = it / V, i = it % V;
+          for (int q = 0; q < qn; ++q)
+            r[p] += s_P[w][q][i][0] * L.g[q][a][0] + s_P[w][q][i][1] * L.g[q][a][1] + s_P[w][q][i][2] * L.g[q][a][2];
+        }
+      }
     }
   }
-  __syncthreads();
   if (!active) return;
-  for (int it = t; it < N * V; it += TPC) {
-    const int a = it / V, i = it % V;
-    T r = 0;
-    for (int q = 0; q < Q; ++q)
-      r += s_P[w][q][i][0] * L.g[q][a][0] + s_P[w][q][i][1] * L.g[q][a][1] + s_P[w][q][i][2] * L.g[q][a][2];
-    re[((size_t)c * N + a) * V + i] = double(r);
+#pragma unroll
+  for (int p = 0; p < RPT; ++p) {
+    const int it = t + p * TPC;
+    if (it < N * V) re[(size_t)c * N * V + it] = double(r[p]);
   }
 }
 
@@ -261,14 +301,17 @@ template <class Cfg, int V>
 __global__ __launch_bounds__(256) void k_quad_grad(fem_mesh_t mesh, const double* __restrict__ u,
                                                    double* __restrict__ G, double* __restrict__ U) {
   constexpr int N = Cfg::N, Q = Cfg::Q;
+  const int Qt = mesh.num_quads;
+  // reference gradients from LDS when all points fit, else straight from the (cached) table
   __shared__ double s_dN[Q][N][3];
   __shared__ double s_w[Q];
-  load_tables(mesh, s_dN, s_w);
+  if (Qt <= Q) load_tables_chunk(mesh, 0, Qt, s_dN, s_w);
   __syncthreads();
+  const double(*dN)[N][3] = Qt <= Q ? s_dN : reinterpret_cast<const double(*)[N][3]>(mesh.shape_grads_ref);
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= mesh.num_cells * Q) return;
-  const int64_t c = e / Q;
-  const int q = (int)(e - c * Q);
+  if (e >= mesh.num_cells * Qt) return;
+  const int64_t c = e / Qt;
+  const int q = (int)(e - c * Qt);
   const int32_t* cn = mesh.cells + c * N;
   const double* p0 = mesh.points + (int64_t)cn[0] * 3;
   double J[9];
@@ -279,7 +322,7 @@ __global__ __launch_bounds__(256) void k_quad_grad(fem_mesh_t mesh, const double
     const double d0 = pa[0] - p0[0], d1 = pa[1] - p0[1], d2 = pa[2] - p0[2];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const double dn = s_dN[q][a][j];
+      const double dn = dN[q][a][j];
       J[0 * 3 + j] += d0 * dn;
       J[1 * 3 + j] += d1 * dn;
       J[2 * 3 + j] += d2 * dn;
@@ -295,7 +338,7 @@ __global__ __launch_bounds__(256) void k_quad_grad(fem_mesh_t mesh, const double
     for (int j = 0; j < 3; ++j) Gl[i][j] = 0.0;
   }
   for (int a = 0; a < N; ++a) {
-    const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
+    const double e0 = dN[q][a][0], e1 = dN[q][a][1], e2 = dN[q][a][2];
     const double na = mesh.shape_vals[q * N + a];
     double g[3];
 #pragma unroll
@@ -328,6 +371,7 @@ __global__ __launch_bounds__(256) void k_cell_mass(fem_mesh_t mesh, const double
                                                    double* __restrict__ re) {
   using T = double;
   constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW;
+  constexpr int PPT = (N * N + TPC - 1) / TPC, RPT = (N * V + TPC - 1) / TPC;
   __shared__ T s_dN[Q][N][3];
   __shared__ double s_w[Q];
   __shared__ CellLds<N, Q, T> s_cell[CPW];
@@ -336,61 +380,93 @@ __global__ __launch_bounds__(256) void k_cell_mass(fem_mesh_t mesh, const double
   const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
   const int64_t c = (int64_t)blockIdx.x * CPW + w;
   const bool active = c < mesh.num_cells;
-  load_tables(mesh, s_dN, s_w);
-  __syncthreads();
-  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  const int Qt = mesh.num_quads;
   CellLds<N, Q, T>& L = s_cell[w];
-  if (MODE == 1) {
-    if (active) {
-      for (int q = t; q < Q; q += TPC) {
-        double vq[V];
+  double K[MODE == 0 ? PPT : 1][V][V];
+  double r[MODE == 1 ? RPT : 1];
 #pragma unroll
-        for (int k = 0; k < V; ++k) vq[k] = 0.0;
-        for (int b = 0; b < N; ++b) {
-          const double nb = mesh.shape_vals[q * N + b];
+  for (int p = 0; p < (MODE == 0 ? PPT : 1); ++p)
 #pragma unroll
-          for (int k = 0; k < V; ++k) vq[k] += nb * v[(int64_t)s_node[w][b] * V + k];
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int k = 0; k < V; ++k) K[p][i][k] = 0.0;
+#pragma unroll
+  for (int p = 0; p < (MODE == 1 ? RPT : 1); ++p) r[p] = 0.0;
+  for (int q0 = 0; q0 < Qt; q0 += Q) {
+    const int qn = min(Q, Qt - q0);
+    __syncthreads();
+    load_tables_chunk(mesh, q0, qn, s_dN, s_w);
+    __syncthreads();
+    stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, L, s_node[w], qn);
+    const double* sv = mesh.shape_vals + (size_t)q0 * N;
+    if (MODE == 1) {
+      if (active) {
+        for (int q = t; q < qn; q += TPC) {
+          double vq[V];
+#pragma unroll
+          for (int k = 0; k < V; ++k) vq[k] = 0.0;
+          for (int b = 0; b < N; ++b) {
+            const double nb = sv[q * N + b];
+#pragma unroll
+            for (int k = 0; k < V; ++k) vq[k] += nb * v[(int64_t)s_node[w][b] * V + k];
+          }
+          const double* Mq = M + ((size_t)c * Qt + q0 + q) * V * V;
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            double rr = 0.0;
+#pragma unroll
+            for (int k = 0; k < V; ++k) rr += Mq[i * V + k] * vq[k];
+            s_mv[w][q][i] = rr * L.jxw[q];
+          }
         }
-        const double* Mq = M + ((size_t)c * Q + q) * V * V;
+      }
+      __syncthreads();
+      if (active) {
 #pragma unroll
-        for (int i = 0; i < V; ++i) {
-          double r = 0.0;
+        for (int p = 0; p < RPT; ++p) {
+          const int it = t + p * TPC;
+          if (it < N * V) {
+            const int a = it / V, i = it % V;
+            for (int q = 0; q < qn; ++q) r[p] += sv[q * N + a] * s_mv[w][q][i];
+          }
+        }
+      }
+    } else if (active) {
 #pragma unroll
-          for (int k = 0; k < V; ++k) r += Mq[i * V + k] * vq[k];
-          s_mv[w][q][i] = r * L.jxw[q];
+      for (int p = 0; p < PPT; ++p) {
+        const int pr = t + p * TPC;
+        if (pr < N * N) {
+          const int a = pr / N, b = pr % N;
+          for (int q = 0; q < qn; ++q) {
+            const double s = sv[q * N + a] * sv[q * N + b] * L.jxw[q];
+            const double* Mq = M + ((size_t)c * Qt + q0 + q) * V * V;
+#pragma unroll
+            for (int i = 0; i < V; ++i)
+#pragma unroll
+              for (int k = 0; k < V; ++k) K[p][i][k] += s * Mq[i * V + k];
+          }
         }
       }
     }
-    __syncthreads();
-    if (!active) return;
-    for (int it = t; it < N * V; it += TPC) {
-      const int a = it / V, i = it % V;
-      double r = 0.0;
-      for (int q = 0; q < Q; ++q) r += mesh.shape_vals[q * N + a] * s_mv[w][q][i];
-      re[((size_t)c * N + a) * V + i] += r;
+  }
+  if (!active) return;
+  if (MODE == 1) {
+#pragma unroll
+    for (int p = 0; p < RPT; ++p) {
+      const int it = t + p * TPC;
+      if (it < N * V) re[(size_t)c * N * V + it] += r[p];
     }
   } else {
-    if (!active) return;
-    for (int pr = t; pr < N * N; pr += TPC) {
-      const int a = pr / N, b = pr % N;
-      double K[V][V];
 #pragma unroll
-      for (int i = 0; i < V; ++i)
-#pragma unroll
-        for (int k = 0; k < V; ++k) K[i][k] = 0.0;
-      for (int q = 0; q < Q; ++q) {
-        const double s = mesh.shape_vals[q * N + a] * mesh.shape_vals[q * N + b] * L.jxw[q];
-        const double* Mq = M + ((size_t)c * Q + q) * V * V;
+    for (int p = 0; p < PPT; ++p) {
+      const int pr = t + p * TPC;
+      if (pr < N * N) {
+        TK* out = Ke + ((size_t)c * N * N + pr) * (V * V);
 #pragma unroll
         for (int i = 0; i < V; ++i)
 #pragma unroll
-          for (int k = 0; k < V; ++k) K[i][k] += s * Mq[i * V + k];
+          for (int k = 0; k < V; ++k) out[i * V + k] = TK(double(out[i * V + k]) + K[p][i][k]);
       }
-      TK* out = Ke + ((size_t)c * N * N + pr) * (V * V);
-#pragma unroll
-      for (int i = 0; i < V; ++i)
-#pragma unroll
-        for (int k = 0; k < V; ++k) out[i * V + k] = TK(double(out[i * V + k]) + K[i][k]);
     }
   }
 }
@@ -430,32 +506,39 @@ __global__ __launch_bounds__(256) void k_cell_adjoint(fem_mesh_t mesh, fem_mater
   const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
   const int64_t c = (int64_t)blockIdx.x * CPW + w;
   const bool active = c < mesh.num_cells;
-  load_tables(mesh, s_dN, s_w);
-  __syncthreads();
-  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
+  const int Qt = mesh.num_quads;
   CellLds<N, Q, T>& L = s_cell[w];
-  if (active && t < N) {
-    const int64_t n = s_node[w][t];
+  for (int q0 = 0; q0 < Qt; q0 += Q) {
+    const int qn = min(Q, Qt - q0);
+    __syncthreads();
+    load_tables_chunk(mesh, q0, qn, s_dN, s_w);
+    __syncthreads();
+    stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, L, s_node[w], qn);
+    if (active && t < N) {
+      const int64_t n = s_node[w][t];
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      s_u[w][t][i] = u[n * V + i];
-      s_l[w][t][i] = lam[n * V + i];
+      for (int i = 0; i < V; ++i) {
+        s_u[w][t][i] = u[n * V + i];
+        s_l[w][t][i] = lam[n * V + i];
+      }
     }
-  }
-  __syncthreads();
-  if (!active) return;
-  for (int q = t; q < Q; q += TPC) {
-    T G[V][3], GL[V][3], dP[V][3];
-    quad_grad<N, Q, V, T>(L, s_u[w], q, G);
-    quad_grad<N, Q, V, T>(L, s_l[w], q, GL);
-    const auto st = MAT::template prep<T>(mat, c * Q + q, G);
-    MAT::dparam(mat, c * Q + q, st, G, dP);
-    T s = 0;
+    __syncthreads();
+    if (active) {
+      for (int q = t; q < qn; q += TPC) {
+        T G[V][3], GL[V][3], dP[V][3];
+        quad_grad<N, Q, V, T>(L, s_u[w], q, G);
+        quad_grad<N, Q, V, T>(L, s_l[w], q, GL);
+        const int64_t cq = c * Qt + q0 + q;
+        const auto st = MAT::template prep<T>(mat, cq, G);
+        MAT::dparam(mat, cq, st, G, dP);
+        T s = 0;
 #pragma unroll
-    for (int i = 0; i < V; ++i)
+        for (int i = 0; i < V; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j) s += dP[i][j] * GL[i][j];
-    gout[c * Q + q] = -s * L.jxw[q];
+          for (int j = 0; j < 3; ++j) s += dP[i][j] * GL[i][j];
+        gout[cq] = -s * L.jxw[q];
+      }
+    }
   }
 }
 
@@ -539,22 +622,28 @@ __global__ __launch_bounds__(256) void k_geometry(fem_mesh_t mesh, double* __res
   const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
   const int64_t c = (int64_t)blockIdx.x * CPW + w;
   const bool active = c < mesh.num_cells;
-  load_tables(mesh, s_dN, s_w);
-  __syncthreads();
-  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
-  if (!active) return;
+  const int Qt = mesh.num_quads;
   CellLds<N, Q, T>& L = s_cell[w];
-  if (shape_grads)
-    for (int it = t; it < Q * N * 3; it += TPC) shape_grads[(size_t)c * Q * N * 3 + it] = (&L.g[0][0][0])[it];
-  if (JxW)
-    for (int q = t; q < Q; q += TPC) JxW[c * Q + q] = L.jxw[q];
-  if (qpts)
-    for (int it = t; it < Q * 3; it += TPC) {
-      const int q = it / 3, d = it % 3;
-      double x = 0;
-      for (int a = 0; a < N; ++a) x += mesh.shape_vals[q * N + a] * L.xd[a][d];
-      qpts[(size_t)c * Q * 3 + it] = x;
-    }
+  for (int q0 = 0; q0 < Qt; q0 += Q) {
+    const int qn = min(Q, Qt - q0);
+    __syncthreads();
+    load_tables_chunk(mesh, q0, qn, s_dN, s_w);
+    __syncthreads();
+    stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, L, s_node[w], qn);
+    if (!active) continue;
+    const size_t cq0 = (size_t)c * Qt + q0;
+    if (shape_grads)
+      for (int it = t; it < qn * N * 3; it += TPC) shape_grads[cq0 * N * 3 + it] = (&L.g[0][0][0])[it];
+    if (JxW)
+      for (int q = t; q < qn; q += TPC) JxW[cq0 + q] = L.jxw[q];
+    if (qpts)
+      for (int it = t; it < qn * 3; it += TPC) {
+        const int q = it / 3, d = it % 3;
+        double x = 0;
+        for (int a = 0; a < N; ++a) x += mesh.shape_vals[(size_t)(q0 + q) * N + a] * L.xd[a][d];
+        qpts[cq0 * 3 + it] = x;
+      }
+  }
 }
 
 // re[c][a][i] = sum_q m[c][q][i] N_a(q) JxW(c, q); the vec count is runtime (1..3).
@@ -563,6 +652,7 @@ __global__ __launch_bounds__(256) void k_cell_load(fem_mesh_t mesh, int V, const
                                                    double* __restrict__ re) {
   using T = double;
   constexpr int N = Cfg::N, Q = Cfg::Q, TPC = Cfg::TPC, CPW = Cfg::CPW;
+  constexpr int RPT = (N * 3 + TPC - 1) / TPC;  // (a, i) rows per thread for vec <= 3
   __shared__ T s_dN[Q][N][3];
   __shared__ double s_w[Q];
   __shared__ CellLds<N, Q, T> s_cell[CPW];
@@ -570,16 +660,33 @@ __global__ __launch_bounds__(256) void k_cell_load(fem_mesh_t mesh, int V, const
   const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
   const int64_t c = (int64_t)blockIdx.x * CPW + w;
   const bool active = c < mesh.num_cells;
-  load_tables(mesh, s_dN, s_w);
-  __syncthreads();
-  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
-  if (!active) return;
+  const int Qt = mesh.num_quads;
   CellLds<N, Q, T>& L = s_cell[w];
-  for (int it = t; it < N * V; it += TPC) {
-    const int a = it / V, i = it % V;
-    double r = 0;
-    for (int q = 0; q < Q; ++q) r += m[((size_t)c * Q + q) * V + i] * mesh.shape_vals[q * N + a] * L.jxw[q];
-    re[((size_t)c * N + a) * V + i] = r;
+  double r[RPT];
+#pragma unroll
+  for (int p = 0; p < RPT; ++p) r[p] = 0.0;
+  for (int q0 = 0; q0 < Qt; q0 += Q) {
+    const int qn = min(Q, Qt - q0);
+    __syncthreads();
+    load_tables_chunk(mesh, q0, qn, s_dN, s_w);
+    __syncthreads();
+    stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, L, s_node[w], qn);
+    if (!active) continue;
+#pragma unroll
+    for (int p = 0; p < RPT; ++p) {
+      const int it = t + p * TPC;
+      if (it < N * V) {
+        const int a = it / V, i = it % V;
+        for (int q = 0; q < qn; ++q)
+          r[p] += m[((size_t)c * Qt + q0 + q) * V + i] * mesh.shape_vals[(size_t)(q0 + q) * N + a] * L.jxw[q];
+      }
+    }
+  }
+  if (!active) return;
+#pragma unroll
+  for (int p = 0; p < RPT; ++p) {
+    const int it = t + p * TPC;
+    if (it < N * V) re[(size_t)c * N * V + it] = r[p];
   }
 }
 
@@ -596,16 +703,21 @@ __global__ __launch_bounds__(256) void k_cell_load_adjoint(fem_mesh_t mesh, int 
   const int w = threadIdx.x / TPC, t = threadIdx.x % TPC;
   const int64_t c = (int64_t)blockIdx.x * CPW + w;
   const bool active = c < mesh.num_cells;
-  load_tables(mesh, s_dN, s_w);
-  __syncthreads();
-  stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, s_cell[w], s_node[w]);
-  if (!active) return;
+  const int Qt = mesh.num_quads;
   CellLds<N, Q, T>& L = s_cell[w];
-  for (int it = t; it < Q * V; it += TPC) {
-    const int q = it / V, i = it % V;
-    double r = 0;
-    for (int a = 0; a < N; ++a) r += lam[(int64_t)s_node[w][a] * V + i] * mesh.shape_vals[q * N + a];
-    lq[((size_t)c * Q + q) * V + i] = r * L.jxw[q];
+  for (int q0 = 0; q0 < Qt; q0 += Q) {
+    const int qn = min(Q, Qt - q0);
+    __syncthreads();
+    load_tables_chunk(mesh, q0, qn, s_dN, s_w);
+    __syncthreads();
+    stage_geometry<N, Q, TPC, T>(mesh, c, active, t, s_dN, s_w, L, s_node[w], qn);
+    if (!active) continue;
+    for (int it = t; it < qn * V; it += TPC) {
+      const int q = it / V, i = it % V;
+      double r = 0;
+      for (int a = 0; a < N; ++a) r += lam[(int64_t)s_node[w][a] * V + i] * mesh.shape_vals[(size_t)(q0 + q) * N + a];
+      lq[((size_t)c * Qt + q0 + q) * V + i] = r * L.jxw[q];
+    }
   }
 }
 
@@ -677,12 +789,15 @@ static int grid_cells(const fem_mesh_t& m) {
 // Calls f(Cfg{}) for the supported (nodes per cell, quads) pairs.
 template <class F>
 static int with_cfg(const fem_mesh_t& m, F&& f) {
+  // quadrature points are staged 8 (HEX8, 8 points) or 27 at a time, so any order runs
   const int N = m.nodes_per_cell, Q = m.num_quads;
-  if (N == 8 && Q == 8) return f(ElemCfg<8, 8>{});
-  if (N == 8 && Q == 27) return f(ElemCfg<8, 27>{});
-  if (N == 27 && Q == 27) return f(ElemCfg<27, 27>{});
+  if (Q >= 1 && Q <= 4096) {
+    if (N == 8 && Q == 8) return f(ElemCfg<8, 8>{});
+    if (N == 8) return f(ElemCfg<8, 27>{});
+    if (N == 27) return f(ElemCfg<27, 27>{});
+  }
   set_error("unsupported element/quadrature combination: %d nodes, %d quadrature points "
-            "(supported: HEX8 with 8 or 27 points, HEX27 with 27 points)", N, Q);
+            "(supported: HEX8 and HEX27 with 1 to 4096 points)", N, Q);
   return FEM_ERR_BAD_ARG;
 }
 
@@ -721,7 +836,29 @@ static int check_mesh(const fem_mesh_t* m) {
   FEM_CHECK_ARG(m && m->points && m->cells && m->shape_grads_ref && m->quad_weights && m->shape_vals,
                 "mesh descriptor has NULL fields");
   FEM_CHECK_ARG(m->num_cells >= 0 && m->num_nodes > 0, "mesh sizes invalid");
-  FEM_CHECK_ARG(m->vec == 1 || m->vec == 3, "vec must be 1 or 3 (got %d)", m->vec);
+  FEM_CHECK_ARG(m->vec >= 1 && m->vec <= 4, "vec must be 1 to 4 (got %d)", m->vec);
+  return FEM_OK;
+}
+
+__global__ void k_zero(int64_t n, double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = 0.0;
+}
+
+__global__ void k_zero_f(int64_t n, float* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = 0.f;
+}
+
+// FEM_MAT_NONE: the law contributes nothing; the output holds n zeros (FP64, or FP32 with f32).
+static int zero_output(void* out, int64_t n, bool f32, void* stream) {
+  if (n <= 0) return FEM_OK;
+  const unsigned grid = (unsigned)std::min<int64_t>(ceil_div(n, 256), 4096);
+  if (f32)
+    hipLaunchKernelGGL(k_zero_f, dim3(grid), dim3(256), 0, as_stream(stream), n, (float*)out);
+  else
+    hipLaunchKernelGGL(k_zero, dim3(grid), dim3(256), 0, as_stream(stream), n, (double*)out);
+  FEM_LAUNCHED();
   return FEM_OK;
 }
 
@@ -753,6 +890,9 @@ int fem_cell_tangent(const fem_mesh_t* mesh, const fem_material_t* mat, int32_t 
   FEM_CHECK_ARG(precision != FEM_BF16X3 || hex27_tangent_mfma_applies(*mesh, *mat),
                 "fem_cell_tangent: FEM_BF16X3 needs HEX27 cells, 27 quadrature points and an elastic law");
   if (mesh->num_cells == 0) return FEM_OK;
+  if (mat->kind == FEM_MAT_NONE)
+    return zero_output(Ke, mesh->num_cells * mesh->nodes_per_cell * mesh->nodes_per_cell * mesh->vec * mesh->vec,
+                       precision != FEM_F64, stream);
   if (precision != FEM_F32 && hex27_tangent_mfma_applies(*mesh, *mat))
     return hex27_tangent_mfma(*mesh, *mat, precision, Ke, as_stream(stream));
   return with_cfg(*mesh, [&](auto cfg) -> int {
@@ -777,6 +917,8 @@ static int cell_vector(const fem_mesh_t* mesh, const fem_material_t* mat, const 
   FEM_CHECK_ARG(mat && u && re && (mode == 0 || v), "cell vector kernel: NULL argument");
   FEM_CHECK_ARG(!mat->has_param || mat->param, "cell vector kernel: material param missing");
   if (mesh->num_cells == 0) return FEM_OK;
+  if (mat->kind == FEM_MAT_NONE)
+    return zero_output(re, mesh->num_cells * mesh->nodes_per_cell * mesh->vec, false, stream);
   if (mesh->nodes_per_cell == 8 && mesh->num_quads == 8)
     return with_mat(*mat, mesh->vec, [&](auto m) -> int {
       using MAT = decltype(m);
@@ -807,10 +949,11 @@ static int cell_vector(const fem_mesh_t* mesh, const fem_material_t* mat, const 
 int fem_quad_grad(const fem_mesh_t* mesh, const double* u, double* G, double* U, void* stream) {
   if (int rc = check_mesh(mesh)) return rc;
   FEM_CHECK_ARG(u && (G || U), "fem_quad_grad: NULL argument");
+  FEM_CHECK_ARG(mesh->vec == 1 || mesh->vec == 3, "fem_quad_grad: vec must be 1 or 3 (got %d)", mesh->vec);
   if (mesh->num_cells == 0) return FEM_OK;
   return with_cfg(*mesh, [&](auto cfg) -> int {
     using Cfg = decltype(cfg);
-    const dim3 grid((unsigned)ceil_div(mesh->num_cells * Cfg::Q, 256));
+    const dim3 grid((unsigned)ceil_div(mesh->num_cells * mesh->num_quads, 256));
     if (mesh->vec == 1)
       hipLaunchKernelGGL((k_quad_grad<Cfg, 1>), grid, dim3(256), 0, as_stream(stream), *mesh, u, G, U);
     else
@@ -825,6 +968,7 @@ static int cell_mass(const fem_mesh_t* mesh, const double* M, int mode, int32_t 
   if (int rc = check_mesh(mesh)) return rc;
   FEM_CHECK_ARG(M && (mode == 0 ? Ke != nullptr : (v && re)), "mass term kernel: NULL argument");
   FEM_CHECK_ARG(precision == FEM_F32 || precision == FEM_F64, "fem_cell_mass_tangent: bad precision");
+  FEM_CHECK_ARG(mesh->vec == 1 || mesh->vec == 3, "mass term kernel: vec must be 1 or 3 (got %d)", mesh->vec);
   if (mesh->num_cells == 0) return FEM_OK;
   return with_cfg(*mesh, [&](auto cfg) -> int {
     using Cfg = decltype(cfg);

@@ -6,6 +6,9 @@
 
 namespace fem {
 
+// Q is the number of quadrature points staged at a time: kernels walk the
+// mesh's num_quads points in chunks of Q (one chunk when they are equal), so
+// any quadrature order runs with the LDS footprint of Q points.
 template <int N_, int Q_>
 struct ElemCfg {
   static constexpr int N = N_;
@@ -23,12 +26,13 @@ struct CellLds {
   T g[Q][N][3];
 };
 
-// Stage node coordinates, Jacobians and physical gradients of cell c.
+// Stage node coordinates, Jacobians and physical gradients of cell c for the
+// first qn (default Q) staged quadrature points.
 // Must be called by all threads of the workgroup (contains barriers).
 template <int N, int Q, int TPC, typename T>
 __device__ inline void stage_geometry(const fem_mesh_t& mesh, int64_t c, bool active, int t,
                                       const T (&dN)[Q][N][3], const double (&w)[Q], CellLds<N, Q, T>& L,
-                                      int32_t (&node)[N]) {
+                                      int32_t (&node)[N], int qn = Q) {
   if (active && t < N) {
     const int32_t n = mesh.cells[c * N + t];
     node[t] = n;
@@ -38,7 +42,7 @@ __device__ inline void stage_geometry(const fem_mesh_t& mesh, int64_t c, bool ac
   }
   __syncthreads();
   if (active) {
-    for (int q = t; q < Q; q += TPC) {
+    for (int q = t; q < qn; q += TPC) {
       T J[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) J[k] = T(0);
@@ -61,7 +65,7 @@ __device__ inline void stage_geometry(const fem_mesh_t& mesh, int64_t c, bool ac
   }
   __syncthreads();
   if (active) {
-    for (int it = t; it < Q * N; it += TPC) {
+    for (int it = t; it < qn * N; it += TPC) {
       const int q = it / N, a = it % N;
       const T* iJ = L.invJ[q];
       const T d0 = dN[q][a][0], d1 = dN[q][a][1], d2 = dN[q][a][2];
@@ -77,6 +81,15 @@ __device__ inline void load_tables(const fem_mesh_t& mesh, T (&dN)[Q][N][3], dou
   T* flat = &dN[0][0][0];
   for (int i = threadIdx.x; i < Q * N * 3; i += blockDim.x) flat[i] = T(mesh.shape_grads_ref[i]);
   for (int i = threadIdx.x; i < Q; i += blockDim.x) w[i] = mesh.quad_weights[i];
+}
+
+// Reference gradients and weights of quadrature points q0 .. q0 + qn - 1 (qn <= Q).
+template <int N, int Q, typename T>
+__device__ inline void load_tables_chunk(const fem_mesh_t& mesh, int q0, int qn, T (&dN)[Q][N][3], double (&w)[Q]) {
+  T* flat = &dN[0][0][0];
+  const double* src = mesh.shape_grads_ref + (size_t)q0 * N * 3;
+  for (int i = threadIdx.x; i < qn * N * 3; i += blockDim.x) flat[i] = T(src[i]);
+  for (int i = threadIdx.x; i < qn; i += blockDim.x) w[i] = mesh.quad_weights[q0 + i];
 }
 
 // grad f at quad q from nodal values f[N][V] (LDS)

@@ -529,6 +529,14 @@ int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv, 
     return pcg_run<3, float>(pat, (const float*)vals, (const float*)dinv, op, b, x, opt, ws, hsc, result, s);
   if (V == 3 && opt->precision == FEM_F64)
     return pcg_run<3, double>(pat, (const double*)vals, (const double*)dinv, op, b, x, opt, ws, hsc, result, s);
+  if (V == 2 && opt->precision == FEM_F32)
+    return pcg_run<2, float>(pat, (const float*)vals, (const float*)dinv, op, b, x, opt, ws, hsc, result, s);
+  if (V == 2 && opt->precision == FEM_F64)
+    return pcg_run<2, double>(pat, (const double*)vals, (const double*)dinv, op, b, x, opt, ws, hsc, result, s);
+  if (V == 4 && opt->precision == FEM_F32)
+    return pcg_run<4, float>(pat, (const float*)vals, (const float*)dinv, op, b, x, opt, ws, hsc, result, s);
+  if (V == 4 && opt->precision == FEM_F64)
+    return pcg_run<4, double>(pat, (const double*)vals, (const double*)dinv, op, b, x, opt, ws, hsc, result, s);
   set_error("fem_pcg_solve: unsupported vec %d", V);
   return FEM_ERR_BAD_ARG;
 }

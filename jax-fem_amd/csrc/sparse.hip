@@ -279,6 +279,11 @@ static int with_vt(int vec, int precision, F&& f) {
   if (vec == 1 && precision == FEM_F64) return f(std::integral_constant<int, 1>{}, double{});
   if (vec == 3 && precision == FEM_F32) return f(std::integral_constant<int, 3>{}, float{});
   if (vec == 3 && precision == FEM_F64) return f(std::integral_constant<int, 3>{}, double{});
+  // block sizes of several variables on one mesh (sum of their vec)
+  if (vec == 2 && precision == FEM_F32) return f(std::integral_constant<int, 2>{}, float{});
+  if (vec == 2 && precision == FEM_F64) return f(std::integral_constant<int, 2>{}, double{});
+  if (vec == 4 && precision == FEM_F32) return f(std::integral_constant<int, 4>{}, float{});
+  if (vec == 4 && precision == FEM_F64) return f(std::integral_constant<int, 4>{}, double{});
   set_error("unsupported vec=%d / precision=%d", vec, precision);
   return FEM_ERR_BAD_ARG;
 }
@@ -317,7 +322,7 @@ extern "C" {
 
 int fem_gather_nodal(int64_t num_nodes, int32_t vec, const int32_t* inc_ptr, const int32_t* inc_idx,
                      const double* ent, const double* base, double* out, void* stream) {
-  FEM_CHECK_ARG(num_nodes > 0 && vec >= 1 && vec <= 3 && inc_ptr && inc_idx && ent && out,
+  FEM_CHECK_ARG(num_nodes > 0 && vec >= 1 && vec <= 4 && inc_ptr && inc_idx && ent && out,
                 "fem_gather_nodal: bad arguments");
   const dim3 grid(grid_for(num_nodes, 256));
   hipStream_t s = as_stream(stream);
@@ -325,8 +330,10 @@ int fem_gather_nodal(int64_t num_nodes, int32_t vec, const int32_t* inc_ptr, con
     hipLaunchKernelGGL(k_gather_nodal<3>, grid, dim3(256), 0, s, num_nodes, inc_ptr, inc_idx, ent, base, out);
   else if (vec == 1)
     hipLaunchKernelGGL(k_gather_nodal<1>, grid, dim3(256), 0, s, num_nodes, inc_ptr, inc_idx, ent, base, out);
-  else
+  else if (vec == 2)
     hipLaunchKernelGGL(k_gather_nodal<2>, grid, dim3(256), 0, s, num_nodes, inc_ptr, inc_idx, ent, base, out);
+  else
+    hipLaunchKernelGGL(k_gather_nodal<4>, grid, dim3(256), 0, s, num_nodes, inc_ptr, inc_idx, ent, base, out);
   FEM_LAUNCHED();
   return FEM_OK;
 }

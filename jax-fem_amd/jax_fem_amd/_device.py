@@ -180,7 +180,7 @@ class DeviceProblem:
     """Pattern, boundary data, loads and solver buffers of one Problem."""
 
     def __init__(self, problem):
-        fe = problem.fes[0]
+        fe = problem._fe_device     # fes[0], or for several variables one element with vec = sum(vec)
         self.problem = problem
         self.fe = fe
         self.dm = fe.device_mesh()
@@ -188,6 +188,14 @@ class DeviceProblem:
         self.L = dm.L
         self.V = fe.vec
         self.n = dm.Nn * self.V
+        # several variables: variable i holds components voff[i] .. voff[i] + vec_i - 1 of each
+        # node block; cperm[k] = position in the device's per-cell vector of entry k of the
+        # reference's per-cell vector (variables concatenated, problem.py:86-92)
+        self.vecs = [f.vec for f in problem.fes]
+        self.voff = onp.cumsum([0] + self.vecs[:-1])
+        self.cperm = onp.concatenate([(onp.arange(dm.N)[:, None] * self.V + o + onp.arange(v)[None, :]).reshape(-1)
+                                      for v, o in zip(self.vecs, self.voff)])
+        self.cperm_inv = onp.argsort(self.cperm)
         self.inc_ptr, self.inc_idx = dm.incidence(dm.cells)
         self._build_pattern()
         self.surfaces = []
@@ -270,15 +278,17 @@ class DeviceProblem:
 
     # --------------------------------------------------------- preparation
     def refresh_bc(self):
-        fe = self.fe
-        sources = tuple((id(a), id(b), id(c)) for a, b, c in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list))
+        fes = self.problem.fes
+        sources = tuple((id(a), id(b), id(c)) for fe in fes
+                        for a, b, c in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list))
         if sources == self._bc_sources:
             return
         dofs, vals = [], []
-        for node_inds, vec_inds, v in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list):
-            node_inds = onp.asarray(node_inds).reshape(-1)
-            dofs.append(node_inds * self.V + onp.asarray(vec_inds).reshape(-1))
-            vals.append(onp.broadcast_to(onp.asarray(v, dtype=onp.float64).reshape(-1), node_inds.shape))
+        for fe, off in zip(fes, self.voff):
+            for node_inds, vec_inds, v in zip(fe.node_inds_list, fe.vec_inds_list, fe.vals_list):
+                node_inds = onp.asarray(node_inds).reshape(-1)
+                dofs.append(node_inds * self.V + off + onp.asarray(vec_inds).reshape(-1))
+                vals.append(onp.broadcast_to(onp.asarray(v, dtype=onp.float64).reshape(-1), node_inds.shape))
         if dofs:
             d = onp.concatenate(dofs)
             v = onp.concatenate(vals)
@@ -312,9 +322,9 @@ class DeviceProblem:
         if key not in self._mat_cache:
             if hasattr(pb, 'get_tensor_map'):
                 self._mat_cache[key] = identify(pb.get_tensor_map(), self.V, iv_shapes)
-            else:   # no laplace term: a law with zero constants keeps the kernels' buffers defined
-                from .materials import LinearElastic, Poisson
-                self._mat_cache[key] = Poisson(0.) if self.V == 1 else LinearElastic(lmbda=0., mu=0.)
+            else:   # no laplace term: the cell kernels contribute zeros
+                from .materials import NoLaw
+                self._mat_cache[key] = NoLaw(self.V)
         mat = self._mat_cache[key]
         self.material = mat
         # a closed-form law recognised from a user callable is re-checked at the real
@@ -513,12 +523,16 @@ class DeviceProblem:
 
     def _cell_term(self, fn, ivs):
         """get_universal_kernel (problem.py:275-283): fn(cell_sol_flat, x, cell_shape_grads,
-        cell_JxW, cell_v_grads_JxW, *cell_internal_vars) -> (N*V,) per cell."""
+        cell_JxW, cell_v_grads_JxW, *cell_internal_vars) -> (N*V,) per cell.  With several
+        variables the shape gradients repeat per variable, (Q, N*nvars, 3), and cell_JxW is
+        (nvars, Q), as problem.py:130-182 concatenates them."""
         dm = self.dm
         if 'uk_geo' not in self.buffers:
+            nv = len(self.vecs)
             g, j, x = dm.geometry(want_grads=True, want_jxw=True, want_qpts=True)
+            g = torch.cat([g] * nv, dim=2) if nv > 1 else g
             vg = g[:, :, :, None, :] * j[:, :, None, None, None]
-            self.buffers['uk_geo'] = [x, g, j[:, None, :].contiguous(), vg]
+            self.buffers['uk_geo'] = [x, g, j[:, None, :].expand(-1, nv, -1).contiguous(), vg]
         rest = self.buffers['uk_geo'] + [_flatten_iv(iv, (dm.C,), dm.device).detach() for iv in ivs]
         return dict(pm=self._term_map(('cell',), fn), rest=rest, ids=None, fn=fn, ivs=ivs, lead=(dm.C,))
 
@@ -529,10 +543,13 @@ class DeviceProblem:
         srf = self.surfaces[i]
         key = ('uk_face', i)
         if key not in self.buffers:
+            nv = len(self.vecs)
             faces = srf['faces']
             g, ns, x = dm.face_geometry(faces, want_grads=True, want_nanson=True, want_qpts=True)
             fv = dm.face_shape_vals[faces[:, 1].long()].contiguous()
-            self.buffers[key] = [x, fv, g, ns[:, None, :].contiguous()]
+            if nv > 1:
+                fv, g = torch.cat([fv] * nv, dim=2), torch.cat([g] * nv, dim=2)
+            self.buffers[key] = [x, fv, g, ns[:, None, :].expand(-1, nv, -1).contiguous()]
         S = srf['faces'].shape[0]
         rest = self.buffers[key] + [_flatten_iv(iv, (S,), dm.device).detach() for iv in siv]
         return dict(pm=self._term_map(('face', i), fn), rest=rest, ids=srf['faces'][:, 0].long(), fn=fn,
@@ -549,17 +566,29 @@ class DeviceProblem:
             cache[slot] = hit
         return hit[1]
 
+    def cell_dofs_reference(self, u):
+        """(C, N*V) cell dof vectors of the device vector u in the reference's per-cell order."""
+        dm = self.dm
+        cs = u.reshape(-1, self.V)[dm.cells.long()].reshape(dm.C, dm.N * self.V)
+        return cs if len(self.vecs) == 1 else cs[:, torch.as_tensor(self.cperm, device=cs.device)]
+
     def _cell_terms_eval(self, u, jac):
-        """Sum of the per-cell terms: r (C, N, V) and, with `jac`, K (C, N, N, V, V) FP64."""
+        """Sum of the per-cell terms: r (C, N, V) and, with `jac`, K (C, N, N, V, V) FP64.
+        The user's kernels see the reference's per-cell order; their values are moved to
+        the device's node-block order here."""
         dm = self.dm
         N, V = dm.N, self.V
-        cs = u.reshape(-1, V)[dm.cells.long()].reshape(dm.C, N * V)
+        cs = self.cell_dofs_reference(u)
+        inv = torch.as_tensor(self.cperm_inv, device=dm.device) if len(self.vecs) > 1 else None
         r = torch.zeros((dm.C, N, V), dtype=torch.float64, device=dm.device)
         K = torch.zeros((dm.C, N, N, V, V), dtype=torch.float64, device=dm.device) if jac else None
         with torch.no_grad():
             for t in self.cell_terms:
                 x = cs if t['ids'] is None else cs[t['ids']]
                 y, d = t['pm'](x, t['rest'], jac)
+                if inv is not None:
+                    y = y[:, inv]
+                    d = d[:, inv][:, :, inv] if jac else d
                 ids = torch.arange(dm.C, device=dm.device) if t['ids'] is None else t['ids']
                 r.index_add_(0, ids, y.reshape(-1, N, V))
                 if jac:
@@ -618,7 +647,7 @@ class DeviceProblem:
         zero for a surface map that does not depend on u."""
         dm = self.dm
         N, V = dm.N, self.V
-        cs = u.reshape(-1, V)[dm.cells.long()].reshape(dm.C, N * V)
+        cs = self.cell_dofs_reference(u)
         out = [torch.zeros((srf['faces'].shape[0], N * V, N * V), dtype=torch.float64, device=dm.device)
                for srf in self.surfaces]
         with torch.no_grad():
@@ -667,19 +696,26 @@ class DeviceProblem:
         mag = G.reshape(n, -1).abs().amax(1)
         idx = torch.unique(torch.cat([torch.topk(mag, k).indices,
                                       torch.linspace(0, n - 1, k, device=G.device).long()]))
-        Gs, ivs = G[idx], [t[idx] for t in lc['ivs']]
+        # the map is evaluated where identify() probed it: on the CPU, in float64
+        Gs, ivs = G[idx].cpu(), [t[idx].cpu() for t in lc['ivs']]
         from .materials import PointMap, Tabulated
         pm = self._mat_cache.get(('law_pm', lc['key']))
         if pm is None:
             pm = self._mat_cache[('law_pm', lc['key'])] = PointMap(lc['fn'], (self.V, 3), (self.V, 3), 'tensor map')
+        mat = self.material
+        law = lambda g, *r: mat(g, *r)
         with torch.no_grad():
             Pu = pm(Gs, ivs)[0]
-            mat = self.material
-            Pd = vmap(lambda g, *r: mat(g, *r))(Gs, *ivs).reshape(Pu.shape)
-        scale = max(float(Pu.abs().max()), float(Pd.abs().max()), 1e-300)
+            Pd = vmap(law)(Gs, *ivs).reshape(Pu.shape)
+            # stiffness scale of the law (|P| per unit |grad u| near the reference state), so
+            # that states at or near grad u = 0 compare against rounding of that size
+            gen = torch.Generator().manual_seed(7)
+            G1 = 1e-6 * (torch.rand((1, self.V, 3), generator=gen, dtype=torch.float64) - 0.5)
+            k = float(vmap(law)(G1, *[t[:1] for t in ivs]).abs().max()) / float(G1.abs().max())
+        scale = max(float(Pu.abs().max()), float(Pd.abs().max()), k * max(float(Gs.abs().max()), 1e-3), 1e-300)
         diff = float((Pu - Pd).abs().max())
-        if diff <= 1e-9 * scale:
-            return
+        if diff <= 1e-9 * scale or not (diff == diff and scale < float('inf')):
+            return   # agreement, or non-finite states that the solve itself reports
         logger.warning("the tensor map differs from %r at the quadrature states of this solve (relative "
                        "difference %.3g): switching to the tabulated law", self.material, diff / scale)
         self._mat_cache[lc['key']] = Tabulated(lc['fn'], self.V)
@@ -891,6 +927,9 @@ class DeviceProblem:
         precision, _, vals, _ = self.current
         if precision != _lib.FEM_F64:
             raise _lib.FemError("the AMG hierarchy is built from the FP64 matrix")
+        if len(self.vecs) > 1 or self.V not in (1, 3):
+            raise NotImplementedError("the AMG hierarchy takes one scalar or one vector variable (vec 1 or 3); "
+                                      "solve several variables with 'jax_solver'")
         from .amg import DeviceAMG
         if self._amg is None:
             self._amg = DeviceAMG(self)
@@ -988,4 +1027,8 @@ class DeviceProblem:
         self._mass_tangent(u, Ke, _lib.FEM_F32 if dt == torch.float32 else _lib.FEM_F64)
         self._cell_terms_tangent(u, Ke, faces)
         N, V = dm.N, self.V
-        return Ke.reshape(dm.C, N, N, V, V).permute(0, 1, 3, 2, 4).reshape(dm.C, N * V, N * V)
+        K = Ke.reshape(dm.C, N, N, V, V).permute(0, 1, 3, 2, 4).reshape(dm.C, N * V, N * V)
+        if len(self.vecs) > 1:    # the reference's per-cell order (variables concatenated)
+            p = torch.as_tensor(self.cperm, device=K.device)
+            K = K[:, p][:, :, p]
+        return K

@@ -16,6 +16,7 @@ FEM_F32, FEM_F64, FEM_BF16X3 = 0, 1, 2
 FEM_FORM_STANDARD, FEM_FORM_DIFFERENCE, FEM_FORM_SCALED_DIFF16 = 0, 1, 2
 FEM_MAT_POISSON, FEM_MAT_LINEAR_ELASTIC, FEM_MAT_SIMP, FEM_MAT_NEO_HOOKEAN, FEM_MAT_J2_PLASTIC = 1, 2, 3, 4, 5
 FEM_MAT_TABULATED = 6
+FEM_MAT_NONE = 0
 
 
 def FEM_TAB_STRIDE(vec):

@@ -52,6 +52,24 @@ class Material:
         return m
 
 
+class NoLaw(Material):
+    """No laplace term: a problem whose weak form is all universal kernels
+    (problem.py:275-301); the device cell kernels contribute zeros."""
+    kind = _lib.FEM_MAT_NONE
+
+    def __init__(self, vec):
+        self.vec = int(vec)
+
+    def constants(self):
+        return ()
+
+    def __call__(self, u_grad, *iv):
+        return torch.zeros_like(u_grad)
+
+    def __repr__(self):
+        return f"NoLaw(vec={self.vec})"
+
+
 class Poisson(Material):
     """P = k grad u (tests/benchmarks/linear_poisson/test_linear_poisson.py:15-17).
 

@@ -33,34 +33,77 @@ class Problem:
     additional_info: tuple = ()
 
     def __post_init__(self):
-        if isinstance(self.mesh, list):
-            if len(self.mesh) != 1:
-                raise NotImplementedError("multi-variable problems are outside the device path")
-            unwrap = lambda x: x[0] if isinstance(x, list) else x
-            self.vec, self.ele_type = unwrap(self.vec), unwrap(self.ele_type)
-            self.quadrature_rule, self.quadrature_order = unwrap(self.quadrature_rule), unwrap(self.quadrature_order)
-            self.dirichlet_bc_info = unwrap(self.dirichlet_bc_info) if (
-                isinstance(self.dirichlet_bc_info, list) and len(self.dirichlet_bc_info) == 1) else self.dirichlet_bc_info
-            self.mesh = self.mesh[0]
-        self.mesh = [self.mesh]
-        self.vec = [self.vec]
-        self.ele_type = [self.ele_type]
-        self.quadrature_rule = [self.quadrature_rule]
-        self.quadrature_order = [self.quadrature_order]
-        self.dirichlet_bc_info = [self.dirichlet_bc_info]
-        self.num_vars = 1
-        self.fes = [FiniteElement(mesh=self.mesh[0], vec=self.vec[0], dim=self.dim, ele_type=self.ele_type[0],
-                                  quadrature_rule=self.quadrature_rule[0],
-                                  quadrature_order=self.quadrature_order[0],
-                                  dirichlet_bc_info=self.dirichlet_bc_info[0])]
+        if isinstance(self.mesh, list) and len(self.mesh) > 1:
+            self._init_variables()
+        else:
+            if isinstance(self.mesh, list):
+                unwrap = lambda x: x[0] if isinstance(x, list) else x
+                self.vec, self.ele_type = unwrap(self.vec), unwrap(self.ele_type)
+                self.quadrature_rule, self.quadrature_order = unwrap(self.quadrature_rule), unwrap(self.quadrature_order)
+                self.dirichlet_bc_info = unwrap(self.dirichlet_bc_info) if (
+                    isinstance(self.dirichlet_bc_info, list) and len(self.dirichlet_bc_info) == 1) \
+                    else self.dirichlet_bc_info
+                self.mesh = self.mesh[0]
+            self.mesh = [self.mesh]
+            self.vec = [self.vec]
+            self.ele_type = [self.ele_type]
+            self.quadrature_rule = [self.quadrature_rule]
+            self.quadrature_order = [self.quadrature_order]
+            self.dirichlet_bc_info = [self.dirichlet_bc_info]
+            self.num_vars = 1
+            self.fes = [FiniteElement(mesh=self.mesh[0], vec=self.vec[0], dim=self.dim, ele_type=self.ele_type[0],
+                                      quadrature_rule=self.quadrature_rule[0],
+                                      quadrature_order=self.quadrature_order[0],
+                                      dirichlet_bc_info=self.dirichlet_bc_info[0])]
+            self._fe_device = self.fes[0]
+            self.offset = [0]
+            self.num_total_dofs_all_vars = self.fes[0].num_total_dofs
+        self._finish_init()
+
+    def _init_variables(self):
+        """Several variables (problem.py:58-119): lists of meshes, vecs, element types, BC infos.
+
+        The device path takes variables that share one mesh and one element: their dofs
+        form one node block of size sum(vec) inside the device (node-major, the variables'
+        components side by side), while every array the user sees keeps the reference's
+        layout: variable i at offset[i] + vec_i * node + comp (problem.py:82-90).
+        """
+        n = len(self.mesh)
+        per_var = lambda x: list(x) if isinstance(x, list) else [x] * n
+        self.vec, self.ele_type = per_var(self.vec), per_var(self.ele_type)
+        self.quadrature_rule, self.quadrature_order = per_var(self.quadrature_rule), per_var(self.quadrature_order)
+        self.dirichlet_bc_info = per_var(self.dirichlet_bc_info)
+        self.num_vars = n
+        self.fes = [FiniteElement(mesh=self.mesh[i], vec=self.vec[i], dim=self.dim, ele_type=self.ele_type[i],
+                                  quadrature_rule=self.quadrature_rule[i], quadrature_order=self.quadrature_order[i],
+                                  dirichlet_bc_info=self.dirichlet_bc_info[i]) for i in range(n)]
+        f0 = self.fes[0]
+        for fe in self.fes[1:]:
+            if (fe.ele_type != f0.ele_type or fe.num_quads != f0.num_quads or
+                    not onp.array_equal(fe.cells, f0.cells) or not onp.array_equal(fe.points, f0.points)):
+                raise NotImplementedError("variables on different meshes or elements (e.g. Taylor-Hood pairs) are "
+                                          "outside the device path: the variables must share one mesh and element")
+        if sum(self.vec) > 4:
+            raise NotImplementedError(f"the variables' components add up to {sum(self.vec)}; the device path "
+                                      "takes at most 4 per node")
+        for hook in ('get_tensor_map', 'get_mass_map', 'get_surface_maps'):
+            if hasattr(self, hook):
+                raise NotImplementedError(f"several variables on the device path take their weak form from "
+                                          f"get_universal_kernel(s_surface), not {hook}")
+        self._fe_device = FiniteElement(mesh=self.mesh[0], vec=sum(self.vec), dim=self.dim,
+                                        ele_type=self.ele_type[0], quadrature_rule=self.quadrature_rule[0],
+                                        quadrature_order=self.quadrature_order[0], dirichlet_bc_info=None)
+        Nn = f0.num_total_nodes
+        self.offset = list(onp.cumsum([0] + [Nn * v for v in self.vec[:-1]]))
+        self.num_total_dofs_all_vars = Nn * sum(self.vec)
+
+    def _finish_init(self):
         fe = self.fes[0]
-        self.cells_list = [fe.cells]
+        self.cells_list = [f.cells for f in self.fes]
         self.num_cells = fe.num_cells
         self.boundary_inds_list = fe.get_boundary_conditions_inds(self.location_fns)
-        self.offset = [0]
-        self.num_total_dofs_all_vars = fe.num_total_dofs
-        self.num_nodes_cumsum = onp.cumsum([0, fe.num_nodes])
-        self.cells_list_face_list = [[fe.cells[bi[:, 0]]] for bi in self.boundary_inds_list]
+        self.num_nodes_cumsum = onp.cumsum([0] + [f.num_nodes for f in self.fes])
+        self.cells_list_face_list = [[f.cells[bi[:, 0]] for f in self.fes] for bi in self.boundary_inds_list]
         self.internal_vars = ()
         self.internal_vars_surfaces = [() for _ in range(len(self.boundary_inds_list))]
         self._dev = None
@@ -92,11 +135,20 @@ class Problem:
         return self._dev
 
     def unflatten_fn_sol_list(self, dofs):
-        return [dofs.reshape(self.fes[0].num_total_nodes, self.fes[0].vec)]
+        """Device dof vector -> [(num_nodes, vec_i)] per variable (problem.py:116-117)."""
+        if self.num_vars == 1:
+            return [dofs.reshape(self.fes[0].num_total_nodes, self.fes[0].vec)]
+        blk = dofs.reshape(self.fes[0].num_total_nodes, sum(self.vec))
+        offs = onp.cumsum([0] + self.vec)
+        return [blk[:, offs[i]:offs[i + 1]].contiguous() for i in range(self.num_vars)]
 
     def unflatten_fn_dof(self, cell_sol_flat):
-        """(num_nodes*vec,) cell dofs -> [(num_nodes, vec)] (problem.py:114)."""
-        return [cell_sol_flat.reshape(self.fes[0].num_nodes, self.fes[0].vec)]
+        """(num_nodes*vec + ...,) cell dofs -> [(num_nodes, vec_i)] (problem.py:114)."""
+        out, k = [], 0
+        for fe in self.fes:
+            out.append(cell_sol_flat[k:k + fe.num_nodes * fe.vec].reshape(fe.num_nodes, fe.vec))
+            k += fe.num_nodes * fe.vec
+        return out
 
     # --------------------------------------------------- geometry fields
     @property
@@ -136,10 +188,15 @@ class Problem:
 
     # ------------------------------------------------ residual / Newton
     def _flat(self, sol_list):
+        """[(num_nodes, vec_i)] -> the device dof vector (node blocks of sum(vec) values)."""
         import torch
         dev = self.device()
-        s = sol_list[0] if isinstance(sol_list, (list, tuple)) else sol_list
-        return torch.as_tensor(s).detach().to(device=dev.dm.device, dtype=torch.float64).reshape(-1).contiguous()
+        as_dev = lambda s: torch.as_tensor(s).detach().to(device=dev.dm.device, dtype=torch.float64)
+        if self.num_vars == 1:
+            s = sol_list[0] if isinstance(sol_list, (list, tuple)) else sol_list
+            return as_dev(s).reshape(-1).contiguous()
+        Nn = self.fes[0].num_total_nodes
+        return torch.cat([as_dev(s).reshape(Nn, v) for s, v in zip(sol_list, self.vec)], dim=1).reshape(-1)
 
     def compute_residual(self, sol_list):
         """Residual list [(num_nodes, vec)] without Dirichlet rows (problem.py:462-475)."""
@@ -187,5 +244,6 @@ class Problem:
         return [inds] + [inds[bi[:, 0]] for bi in self.boundary_inds_list]
 
     def _cell_dof_inds(self):
-        fe = self.fes[0]
-        return (fe.vec * fe.cells[:, :, None] + onp.arange(fe.vec)[None, None, :]).reshape(fe.num_cells, -1)
+        """(C, N*vec + ...) reference dof numbers of each cell, variables concatenated (problem.py:86-92)."""
+        return onp.hstack([(fe.vec * fe.cells[:, :, None] + onp.arange(fe.vec)[None, None, :] + o).reshape(
+            fe.num_cells, -1) for fe, o in zip(self.fes, self.offset)])

@@ -343,8 +343,8 @@ def implicit_vjp(problem, sol_list, params, v_list, adjoint_solver_options):
         # per-cell / per-face weak-form terms: vjp of each with cotangent -lambda on its cell's dofs
         V = dev.V
         cl = dm.cells.long()
-        u_c = u.reshape(-1, V)[cl].reshape(dm.C, -1)
-        lam_c = lam.reshape(-1, V)[cl].reshape(dm.C, -1)
+        u_c = dev.cell_dofs_reference(u)
+        lam_c = dev.cell_dofs_reference(lam)
         for t in dev.cell_terms:
             if not any(torch.is_tensor(x) and x.requires_grad for x in t['ivs']):
                 continue

@@ -21,9 +21,21 @@ docstring.  Pins (tests/test_oracle.py, CPU):
               cylinder, Neo-Hookean cylinder + top traction); PCG and BiCGSTAB
               vs a direct solve.
   adjoint     central finite differences of the forward solve; compliance
-              self-adjointness (lambda = u); sign of dC/dtheta.
-Parity unpinned: the order of basix's HEX27 quadrature points (only matters for
-per-quad parameter layouts), the JAX BiCGSTAB iterate sequence, and jax-fem's
-ad_wrapper output at the benchmark configurations (no reference test covers
-ad_wrapper; the adjoint is pinned by finite differences and closed forms only).
+              self-adjointness (lambda = u); sign of dC/dtheta; the reference's
+              own printed ad_wrapper gradient drho[0, 0] = 0.002266954599447443
+              (docs/source/learn/compute_gradients/example.ipynb cell 31),
+              reproduced to 2e-12 at cell 0, quadrature point 0
+              (tests/test_oracle_gradient.py).
+  variables   MultiProblem (several variables on one mesh, problem.py:58-119):
+              free thermal expansion and the prescribed-expansion temperature in
+              closed form, and the single-variable oracle when uncoupled
+              (tests/test_oracle_multivar.py).
+  cpu_fem.c   (the CPU baseline, oracle/cpu.py) against the numpy oracle: CSR
+              pattern bit for bit, tangent values, BiCGSTAB / PCG iterates
+              (tests/test_oracle_cpu.py).
+Parity unpinned: the JAX BiCGSTAB iterate sequence, and jax-fem's ad_wrapper
+output at the benchmark configurations other than the notebook's (the adjoint
+there is pinned by finite differences and closed forms).  The quadrature-point
+order is pinned at HEX8 by the notebook gradient; for HEX27 it only matters for
+per-quad parameter layouts.
 """

@@ -533,3 +533,102 @@ class Problem:
         Gl = np.einsum('cav,cqaj->cqvj', L[self.cells], self.G)
         dP = self.mat.dstress_dtheta(Gu, self.theta)
         return -np.einsum('cqvj,cqvj,cq->cq', dP, Gl, self.JxW), lam
+
+
+# ------------------------------------------------------ several variables
+class MultiProblem:
+    """Several variables on one mesh (problem.py:58-119), restated.
+
+    Variable i has vecs[i] components; its dofs sit at offset[i] + vec_i * node + comp
+    with offset[i] = sum_{j<i} num_nodes * vec_j (problem.py:82-90, 109-119).  A cell's
+    dof vector is the concatenation over the variables of their (N, vec_i) blocks
+    (find_ind, problem.py:86-92), and the COO pairs repeat it over rows and columns
+    (problem.py:95-97).  The weak form comes from universal kernels (problem.py:275-301):
+    fn(cell_sol (C, N*sum(vec)), x (C,Q,3), shape_grads (C,Q,N*nvars,3), JxW (C,nvars,Q),
+    v_grads_JxW (C,Q,N*nvars,1,3), *ivs) -> (C, N*sum(vec)), numpy, batched over cells; the
+    per-cell matrices are complex-step derivatives (Squire & Trapp 1998).
+    """
+
+    def __init__(self, points, cells, vecs, ele_type='HEX8', quadrature_order=None, bc_infos=None):
+        self.points = np.asarray(points, float)
+        self.cells = np.asarray(cells, np.int64)
+        self.vecs = list(vecs)
+        self.nv = len(self.vecs)
+        self.sv, self.sg, self.w = B.shape_vals_and_grads(ele_type, quadrature_order)
+        self.G, self.JxW = shape_grads(self.points, self.cells, self.sg, self.w)
+        Nn = len(self.points)
+        self.offset = np.cumsum([0] + [Nn * v for v in self.vecs[:-1]])
+        self.n = Nn * sum(self.vecs)
+        dofs, vals = [], []
+        for i, info in enumerate(bc_infos or [None] * self.nv):
+            d, v = dirichlet(self.points, self.vecs[i], info)
+            dofs.append(d + self.offset[i])
+            vals.append(v)
+        self.bc_dofs = np.concatenate(dofs).astype(np.int64)
+        self.bc_vals = np.concatenate(vals)
+        self.kernels = []
+
+    def cell_inds(self):
+        """(C, N*sum(vec)) global dofs of each cell, variables concatenated (problem.py:86-92)."""
+        return np.hstack([(v * self.cells[:, :, None] + np.arange(v)[None, None, :] + o).reshape(len(self.cells), -1)
+                          for v, o in zip(self.vecs, self.offset)])
+
+    def coo_pattern(self):
+        inds = self.cell_inds()
+        return (np.repeat(inds[:, :, None], inds.shape[1], axis=2).reshape(-1),
+                np.repeat(inds[:, None, :], inds.shape[1], axis=1).reshape(-1))
+
+    def add_cell_kernel(self, fn, ivs=()):
+        x = quad_points(self.points, self.cells, self.sv)
+        g = np.concatenate([self.G] * self.nv, axis=2)
+        jxw = np.repeat(self.JxW[:, None, :], self.nv, axis=1)
+        vg = g[:, :, :, None, :] * self.JxW[:, :, None, None, None]
+        self.kernels.append((fn, (x, g, jxw, vg) + tuple(ivs)))
+
+    def _terms(self, dofs, jac, h=1e-30):
+        inds = self.cell_inds()
+        cs = dofs[inds]
+        r = np.zeros_like(cs)
+        K = np.zeros((len(cs), cs.shape[1], cs.shape[1])) if jac else None
+        for fn, args in self.kernels:
+            r += np.real(fn(cs, *args))
+            if jac:
+                for k in range(cs.shape[1]):
+                    cc = cs.astype(complex)
+                    cc[:, k] += 1j * h
+                    K[:, :, k] += np.imag(fn(cc, *args)) / h
+        return inds, r, K
+
+    def residual(self, dofs, bc=True):
+        inds, r, _ = self._terms(dofs, False)
+        out = np.zeros(self.n)
+        np.add.at(out, inds.reshape(-1), r.reshape(-1))
+        if bc:  # apply_bc_vec (solver.py:290-304)
+            out[self.bc_dofs] = dofs[self.bc_dofs] - self.bc_vals
+        return out
+
+    def tangent(self, dofs, bc=True):
+        inds, _, K = self._terms(dofs, True)
+        I, J = self.coo_pattern()
+        A = sp.coo_matrix((K.reshape(-1), (I, J)), shape=(self.n, self.n)).tocsr()
+        A.sum_duplicates()
+        A.sort_indices()
+        return zero_rows(A, self.bc_dofs) if bc else A
+
+    def newton(self, tol=1e-6, rel_tol=1e-8):
+        """Newton loop of solver.py:1295-1333 (x0 of solver.py:402-409), direct solves."""
+        dofs = np.zeros(self.n)
+        res = self.residual(dofs)
+        r0 = rv = np.linalg.norm(res)
+        rel = rv / r0 if r0 else np.nan
+        while rel > rel_tol and rv > tol:
+            dofs = dofs + spla.spsolve(self.tangent(dofs).tocsc(), -res)
+            res = self.residual(dofs)
+            rv = np.linalg.norm(res)
+            rel = rv / r0
+        return dofs
+
+    def sol_list(self, dofs):
+        """[(num_nodes, vec_i)] (unflatten_fn_sol_list, problem.py:116-117)."""
+        Nn = len(self.points)
+        return [dofs[o:o + Nn * v].reshape(Nn, v) for v, o in zip(self.vecs, self.offset)]

@@ -419,3 +419,56 @@ def threshold_np(G, theta=None):
     q = np.sum(eps * eps, axis=(-2, -1))
     ex = np.where(np.real(q) > TH_Q0, q - TH_Q0, 0.)[..., None, None]
     return sig + 6. * TH_BETA * ex ** 2 * eps
+
+
+# Coupled thermoelasticity, one backward-Euler step (the universal kernel of
+# applications/thermal_mechanical_full/example.py:41-102 in 3D, T_old = 0 and
+# eps_old = 0): variables u (vec 3) and the temperature change T (vec 1);
+#   u rows: sigma(u, T) : grad v,  sigma = lmbda tr(eps) I + 2 mu eps - kappa T I
+#   T rows: C/dt T Q + kappa T0 tr(eps)/dt Q + k grad T . grad Q
+TM_E, TM_NU = 70e3, 0.3
+TM_LM, TM_MU = TM_E * TM_NU / ((1 + TM_NU) * (1 - 2 * TM_NU)), TM_E / (2. * (1. + TM_NU))
+TM_ALPHA, TM_T0, TM_DT = 2.31e-5, 293., 0.1
+TM_CAP, TM_COND = 910e-6 * 2700., 237e-6
+
+
+def thermo_np(sv, kappa):
+    """Batched over cells (oracle MultiProblem kernel); sv (Q, N) shape values."""
+    def fn(cs, x, g, jxw, vg):
+        C_, N = cs.shape[0], sv.shape[1]
+        u, T = cs[:, :3 * N].reshape(C_, N, 3), cs[:, 3 * N:]
+        gu, gT, vgu, vgT = g[:, :, :N], g[:, :, N:], vg[:, :, :N, 0], vg[:, :, N:, 0]
+        jT = jxw[:, 1]
+        Gu = np.einsum('cav,cqaj->cqvj', u, gu)
+        eps = 0.5 * (Gu + np.swapaxes(Gu, -1, -2))
+        tr = np.trace(eps, axis1=-2, axis2=-1)
+        Tq = np.einsum('qa,ca->cq', sv, T)
+        sig = (TM_LM * tr - kappa * Tq)[..., None, None] * np.eye(3) + 2. * TM_MU * eps
+        val_u = np.einsum('cqij,cqaj->cai', sig, vgu).reshape(C_, -1)
+        val_T = (TM_CAP / TM_DT * np.einsum('cq,qa,cq->ca', Tq, sv, jT)
+                 + kappa * TM_T0 / TM_DT * np.einsum('cq,qa,cq->ca', tr, sv, jT)
+                 + TM_COND * np.einsum('cqj,cqaj->ca', np.einsum('ca,cqaj->cqj', T, gT), vgT))
+        return np.concatenate([val_u, val_T], axis=1)
+    return fn
+
+
+def thermo_torch(sv, kappa):
+    """One cell (device universal kernel); sv (Q, N) torch shape values."""
+    def fn(cell_sol_flat, x, g, jxw, vg):
+        N = sv.shape[1]
+        u, T = cell_sol_flat[:3 * N].reshape(N, 3), cell_sol_flat[3 * N:]
+        gu, gT, vgu, vgT = g[:, :N], g[:, N:], vg[:, :N, 0], vg[:, N:, 0]
+        jT = jxw[1]
+        Gu = (u[None, :, :, None] * gu[:, :, None, :]).sum(1)
+        eps = 0.5 * (Gu + Gu.transpose(1, 2))
+        tr = eps.diagonal(dim1=1, dim2=2).sum(-1)
+        Tq = (sv * T[None, :]).sum(1)
+        eye = torch.eye(3, dtype=cell_sol_flat.dtype, device=cell_sol_flat.device)
+        sig = (TM_LM * tr - kappa * Tq)[:, None, None] * eye + 2. * TM_MU * eps
+        val_u = (sig[:, None, :, :] * vgu[:, :, None, :]).sum((0, 3)).reshape(-1)
+        gTq = (T[None, :, None] * gT).sum(1)
+        val_T = (TM_CAP / TM_DT * (Tq[:, None] * sv * jT[:, None]).sum(0)
+                 + kappa * TM_T0 / TM_DT * (tr[:, None] * sv * jT[:, None]).sum(0)
+                 + TM_COND * (gTq[:, None, :] * vgT).sum((0, 2)))
+        return torch.cat([val_u, val_T])
+    return fn

@@ -106,3 +106,78 @@ def test_hex27_solution_with_oracle_loads(gpu):
     o = oracle_problem(p, OF.Material('elastic', **dict(zip(('lmbda', 'mu'), OF.lame(1e3, 0.3)))))
     ref = o.newton(linear='direct').reshape(-1, 3)
     assert np.linalg.norm(sol - ref) / np.linalg.norm(ref) < 1e-10
+
+
+# ------------------------------------- HEX27 at the reference's default quadrature
+class NH27(Problem):
+    def get_tensor_map(self):
+        return M.NeoHookean(E=10., nu=0.3)
+
+    def get_mass_map(self):
+        return lambda u, x: torch.stack([0.1 * x[0], -0.2 + 0. * x[1], 0.3 * x[2] ** 2])
+
+    def get_surface_maps(self):
+        return [lambda u, x: torch.stack([0. * x[0], x[1], -1. + 0. * x[2]])]
+
+
+def test_hex27_default_quadrature_order_10(gpu):
+    # basis.py:64: HEX27 defaults to quadrature order 10 (6 x 6 x 6 = 216 points, 36 per face);
+    # the cell kernels stage 27 points at a time
+    mesh = curved_hex27((2, 2, 2))
+    p = NH27(mesh, vec=3, dim=3, ele_type='HEX27',
+             dirichlet_bc_info=[[FACES['z0']] * 3, [0, 1, 2], [lambda x: 0.] * 3], location_fns=[FACES['x1']])
+    fe = p.fes[0]
+    assert fe.num_quads == 216 and fe.num_face_quads == 36
+    sv, sg, w = OB.shape_vals_and_grads('HEX27')
+    assert len(w) == 216
+    g, jxw = fe.get_shape_grads()
+    G, J = OF.shape_grads(mesh.points, mesh.cells, sg, w)
+    assert rel(g.cpu().numpy(), G) < 1e-12 and rel(jxw.cpu().numpy(), J) < 1e-12
+    assert abs(float(jxw.sum()) - 1.0) < 1e-12
+    mat = OF.Material('neo_hookean', mu=10. / 2.6, kappa=10. / (3 * 0.4))
+    o = oracle_problem(p, mat)
+    dev = p.device()
+    dev.prepare()
+    assert rel(dev.load.cpu().numpy().reshape(-1, 3), o.load) < 1e-13
+    u = 0.01 * np.random.default_rng(7).standard_normal(o.n)
+    ut = torch.as_tensor(u, device='cuda')
+    K = dev.cell_tangents(ut).cpu().numpy()
+    assert rel(K, OF.cell_tangent(o.mat, u.reshape(-1, 3), o.cells, o.G, o.JxW)) < 1e-12
+    r = p.compute_residual([ut.reshape(-1, 3)])[0].cpu().numpy()
+    assert rel(r, o.residual(u, bc=False).reshape(-1, 3)) < 1e-12
+    sol = solver(p, {'jax_solver': {'precision': 'fp64', 'tol': 1e-12, 'atol': 0.}})[0].cpu().numpy()
+    ref = o.newton(linear='direct').reshape(-1, 3)
+    assert np.linalg.norm(sol - ref) / np.linalg.norm(ref) < 1e-9
+
+
+class SIMP27(Problem):
+    def get_tensor_map(self):
+        return M.SIMPElastic(E0=1., Emin=1e-3, nu=0.3, penal=3.)
+
+    def get_mass_map(self):
+        return lambda u, x, th: torch.tensor([0., 0., 1.], dtype=torch.float64)
+
+    def set_params(self, theta):
+        self.internal_vars = [theta]
+
+
+def test_hex27_order_10_adjoint_vs_oracle(gpu):
+    # a per-quadrature-point SIMP density on 216 points per cell: the adjoint kernel's
+    # (cell, point) indexing over several staged chunks
+    mesh = curved_hex27((2, 2, 2))
+    p = SIMP27(mesh, vec=3, dim=3, ele_type='HEX27', quadrature_order=10,
+               dirichlet_bc_info=[[FACES['z0']] * 3, [0, 1, 2], [lambda x: 0.] * 3])
+    C, Q = p.num_cells, p.fes[0].num_quads
+    theta = torch.as_tensor(np.random.default_rng(3).uniform(0.4, 1.0, (C, Q)), device='cuda')
+    opts = {'jax_solver': {'precision': 'fp64', 'tol': 1e-12, 'atol': 0.}}
+    fwd = ad_wrapper(p, opts, opts)
+    th = theta.clone().requires_grad_(True)
+    u = fwd(th)[0]
+    o = oracle_problem(p, OF.Material('simp', **dict(zip(('lmbda1', 'mu1'), OF.lame(1., 0.3))), Emin=1e-3, E0=1.,
+                                      p=3.), theta=theta.cpu().numpy())
+    uo = o.newton(linear='direct')
+    assert np.linalg.norm(u.detach().cpu().numpy().reshape(-1) - uo) <= 1e-9 * np.linalg.norm(uo)
+    J = torch.dot(-p.device().load, u.reshape(-1))
+    J.backward()
+    gq, _ = o.adjoint_theta(uo, -o.load.reshape(-1))
+    assert rel(th.grad.cpu().numpy(), gq) < 1e-8
