@@ -815,6 +815,118 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_
   }
 }
 
+// Products with fewer output blocks than this split each block's plan over a wave.
+constexpr int64_t kSplitBelow = 200000;
+
+// C = X Y by its plan, YC lanes per output block: lane c of a group forms column
+// c of the block (the group reads the X block together; lane c reads column c
+// of the Y block), so a product needs XR + XR*XC + XC registers instead of a
+// whole block pair and many more products are in flight per CU.  With S > 1
+// (few output blocks: the coarse levels), the S groups of a wave split one
+// block's plan (group g takes products g, g + S, ...) and group 0 sums their
+// columns in group order.  Both orders are fixed: bit-reproducible.
+template <int XR, int XC, int YC, int S>
+__global__ __launch_bounds__(256) void k_spgemm_numeric_g(int64_t nb, const int32_t* __restrict__ plan_ptr,
+                                                          const int32_t* __restrict__ plan,
+                                                          const double* __restrict__ xv,
+                                                          const double* __restrict__ yv, double* __restrict__ cv) {
+  constexpr int G = YC, GPW = 64 / G;  // lanes per group, groups per wave (the last 64 - GPW*G lanes idle)
+  static_assert(S == 1 || S == GPW, "a block is split over one group or over the whole wave");
+  constexpr int BPW = GPW / S;  // output blocks per wave
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int grp = lane / G, c = lane - grp * G;
+  const int64_t b = wave * BPW + grp / S;
+  const int sg = grp % S;
+  double acc[XR];
+#pragma unroll
+  for (int i = 0; i < XR; ++i) acc[i] = 0.0;
+  if (grp < GPW && b < nb) {
+    const int q1 = plan_ptr[b + 1];
+#pragma unroll 2
+    for (int q = plan_ptr[b] + sg; q < q1; q += S) {
+      const int2 ef = reinterpret_cast<const int2*>(plan)[q];
+      double xa[XR][XC], yc[XC];
+      load_block<XR, XC>(xv + (size_t)ef.x * XR * XC, xa);
+      const double* y = yv + (size_t)ef.y * XC * YC + c;
+#pragma unroll
+      for (int m = 0; m < XC; ++m) yc[m] = y[m * YC];
+#pragma unroll
+      for (int i = 0; i < XR; ++i)
+#pragma unroll
+        for (int m = 0; m < XC; ++m) acc[i] = fma(xa[i][m], yc[m], acc[i]);
+    }
+  }
+  if (S > 1) {
+    // lanes c of groups 1 .. S-1 hand their columns to lane c of group 0, in group order
+#pragma unroll
+    for (int i = 0; i < XR; ++i) {
+      double t = acc[i];
+      for (int g = 1; g < S; ++g) t += __shfl(acc[i], c + g * G, 64);
+      acc[i] = t;
+    }
+  }
+  if (grp < GPW && sg == 0 && b < nb) {
+    double* out = cv + (size_t)b * XR * YC + c;
+#pragma unroll
+    for (int i = 0; i < XR; ++i) out[i * YC] = acc[i];
+  }
+}
+
+// P = P0 - omega D^-1 (A P0) along the plan of A P0, NS lanes per block of P
+// (lane j forms column j), as in k_spgemm_numeric_g.
+template <int BS, int NS>
+__global__ __launch_bounds__(256) void k_smooth_numeric_g(int64_t nb, int64_t rows, const int32_t* __restrict__ crp,
+                                                          const int32_t* __restrict__ cci,
+                                                          const int32_t* __restrict__ plan_ptr,
+                                                          const int32_t* __restrict__ plan,
+                                                          const double* __restrict__ av,
+                                                          const double* __restrict__ p0,
+                                                          const int32_t* __restrict__ agg,
+                                                          const double* __restrict__ dinv, double omega,
+                                                          double* __restrict__ pv) {
+  constexpr int G = NS, GPW = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int grp = lane / G, j = lane - grp * G;
+  const int64_t b = wave * GPW + grp;
+  if (grp >= GPW || b >= nb) return;
+  int64_t lo = 0, hi = rows;  // row i: crp[i] <= b < crp[i+1]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (crp[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int64_t i = lo;
+  double acc[BS];
+#pragma unroll
+  for (int r = 0; r < BS; ++r) acc[r] = 0.0;
+  const int q1 = plan_ptr[b + 1];
+#pragma unroll 2
+  for (int q = plan_ptr[b]; q < q1; ++q) {
+    const int2 ef = reinterpret_cast<const int2*>(plan)[q];
+    double xa[BS][BS], yc[BS];
+    load_block<BS, BS>(av + (size_t)ef.x * BS * BS, xa);
+    const double* y = p0 + (size_t)ef.y * BS * NS + j;
+#pragma unroll
+    for (int m = 0; m < BS; ++m) yc[m] = y[m * NS];
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+      for (int m = 0; m < BS; ++m) acc[r] = fma(xa[r][m], yc[m], acc[r]);
+  }
+  const bool own = cci[b] == agg[i];
+  double* out = pv + (size_t)b * BS * NS + j;
+#pragma unroll
+  for (int r = 0; r < BS; ++r) {
+    double v = -omega * dinv[i * BS + r] * acc[r];
+    if (own) v += p0[(i * BS + r) * NS + j];
+    out[r * NS] = v;
+  }
+}
+
 // P = P0 - omega D^-1 (A P0) along the plan of A P0 (Y = P0 as block CSR, one
 // block per row at column agg): one thread per block of P.
 template <int BS, int NS>
@@ -977,40 +1089,45 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __r
 // ---------------------------------------------------------- operator apply
 enum { AP_STORE = 0, AP_ADD = 1, AP_RESID = 2, AP_CHEB = 3, AP_DSTORE = 4, AP_PQ = 5 };
 
-struct ApplyArgs {
-  const double* x;     // gathered input
-  double* y;           // output (CHEB: the new iterate)
+// TX: storage of the input vector and of the Chebyshev direction, TY: of the
+// output.  Arithmetic is FP64 whatever the storage; the FP32 hierarchy keeps
+// its level-0 vectors in FP32 (rhs and dinv stay FP64).
+template <typename TX = double, typename TY = double>
+struct ApplyArgsT {
+  const TX* x;         // gathered input
+  TY* y;               // output (CHEB: the new iterate)
   const double* rhs;   // RESID, CHEB
   const double* dinv;  // CHEB, DSTORE
-  double* d;           // CHEB: search direction (in/out)
+  TX* d;               // CHEB: search direction (in/out)
   double c1, c2;       // CHEB: d = c1 d + c2 D^-1 (rhs - A x), y = x + d
   double* part;        // PQ: per-block partials of <x, A x>
 };
+using ApplyArgs = ApplyArgsT<double, double>;
 
-template <int MODE>
-__device__ inline void apply_epilogue(const ApplyArgs& a, int64_t idx, double ax) {
+template <int MODE, typename TX, typename TY>
+__device__ inline void apply_epilogue(const ApplyArgsT<TX, TY>& a, int64_t idx, double ax) {
   if (MODE == AP_STORE || MODE == AP_PQ) {
-    a.y[idx] = ax;
+    a.y[idx] = TY(ax);
   } else if (MODE == AP_ADD) {
-    a.y[idx] += ax;
+    a.y[idx] = TY(double(a.y[idx]) + ax);
   } else if (MODE == AP_RESID) {
-    a.y[idx] = a.rhs[idx] - ax;
+    a.y[idx] = TY(a.rhs[idx] - ax);
   } else if (MODE == AP_DSTORE) {
-    a.y[idx] = a.dinv[idx] * ax;
+    a.y[idx] = TY(a.dinv[idx] * ax);
   } else {
     const double r = a.dinv[idx] * (a.rhs[idx] - ax);
-    const double dn = a.c1 != 0.0 ? fma(a.c1, a.d[idx], a.c2 * r) : a.c2 * r;
-    a.d[idx] = dn;
-    a.y[idx] = a.x[idx] + dn;
+    const double dn = a.c1 != 0.0 ? fma(a.c1, double(a.d[idx]), a.c2 * r) : a.c2 * r;
+    a.d[idx] = TX(dn);
+    a.y[idx] = TY(double(a.x[idx]) + dn);
   }
 }
 
 // Block CSR operator, G lanes per block row: the lanes stride over the row's
 // (block, column) pairs and accumulate every block row entry.
-template <int BR, int BC, int G, int MODE, typename T>
+template <int BR, int BC, int G, int MODE, typename T, typename TX = double, typename TY = double>
 __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t* __restrict__ rp,
                                                     const int32_t* __restrict__ ci, const T* __restrict__ v,
-                                                    ApplyArgs a) {
+                                                    ApplyArgsT<TX, TY> a) {
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t row = gid / G;
   const int sub = (int)(gid % G);
@@ -1022,7 +1139,7 @@ __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t*
     const int nf = (rp[row + 1] - b) * BC;
     for (int f = sub; f < nf; f += G) {
       const int k = b + f / BC, j = f % BC;
-      const double xv = a.x[(int64_t)ci[k] * BC + j];
+      const double xv = double(a.x[(int64_t)ci[k] * BC + j]);
       const T* vb = v + (size_t)k * BR * BC + j;
 #pragma unroll
       for (int i = 0; i < BR; ++i) acc[i] = fma(double(vb[i * BC]), xv, acc[i]);
@@ -1040,10 +1157,10 @@ __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t*
 
 // Long block rows (R and the coarse operators, hundreds of (block, column) pairs):
 // one 256-thread workgroup per block row, partial sums met in LDS.
-template <int BR, int BC, int MODE, typename T>
+template <int BR, int BC, int MODE, typename T, typename TX = double, typename TY = double>
 __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int32_t* __restrict__ rp,
                                                         const int32_t* __restrict__ ci, const T* __restrict__ v,
-                                                        ApplyArgs a) {
+                                                        ApplyArgsT<TX, TY> a) {
   __shared__ double smem[4];
   const int64_t row = blockIdx.x;
   double acc[BR];
@@ -1053,7 +1170,7 @@ __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int3
   const int nf = (rp[row + 1] - b) * BC;
   for (int f = threadIdx.x; f < nf; f += 256) {
     const int k = b + f / BC, j = f % BC;
-    const double xv = a.x[(int64_t)ci[k] * BC + j];
+    const double xv = double(a.x[(int64_t)ci[k] * BC + j]);
     const T* vb = v + (size_t)k * BR * BC + j;
 #pragma unroll
     for (int i = 0; i < BR; ++i) acc[i] = fma(double(vb[i * BC]), xv, acc[i]);
@@ -1068,9 +1185,10 @@ __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int3
 // Standard-form SELL-64 operator (level 0), FP64 vectors, one block per slice.
 // DIFF: the matrix is in difference form (the diagonal slot holds the row
 // sum); the own-row entries of the input then enter every product.
-template <int V, int MODE, typename T, bool DIFF = false>
+template <int V, int MODE, typename T, bool DIFF = false, typename TX = double, typename TY = double>
 __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pat, const T* __restrict__ vals,
-                                                                ApplyArgs a, const int32_t* __restrict__ done) {
+                                                                ApplyArgsT<TX, TY> a,
+                                                                const int32_t* __restrict__ done) {
   // FP32 residual and power-iteration products unroll the slots by 4 (124 -> 112 us on the
   // 1.6M-dof cantilever); the Chebyshev step, with its larger epilogue, keeps 2, and so does
   // FP16 storage (its residual: 74 us unrolled by 2, 78 us by 4)
@@ -1084,13 +1202,13 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   const int64_t s = xcd_logical_block(blockIdx.x, gridDim.x);
   const int64_t row = pat.sell_row[s * kSlice + lane];
   const bool real = row >= 0;
-  auto xat = [&](int64_t c, double (&xc)[V]) {
+  auto xat = [&](int64_t c, TX (&xc)[V]) {
 #pragma unroll
     for (int j = 0; j < V; ++j) xc[j] = a.x[c * V + j];
   };
   double xr[V], acc[V];
 #pragma unroll
-  for (int j = 0; j < V; ++j) xr[j] = (DIFF && real) ? a.x[row * V + j] : 0.0;
+  for (int j = 0; j < V; ++j) xr[j] = (DIFF && real) ? double(a.x[row * V + j]) : 0.0;
   // the epilogue's own-row operands are loaded before the matrix stream, so
   // their latency overlaps it (wave 0 finishes the rows)
   constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
@@ -1103,11 +1221,11 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       const int64_t idx = row * V + i;
       e_rhs[i] = kRhs ? a.rhs[idx] : 0.0;
       e_dinv[i] = (MODE == AP_CHEB || MODE == AP_DSTORE || (kScaled && MODE == AP_RESID)) ? a.dinv[idx] : 0.0;
-      e_d[i] = (MODE == AP_CHEB && a.c1 != 0.0) ? a.d[idx] : 0.0;
-      e_x[i] = (MODE == AP_CHEB || MODE == AP_PQ) ? a.x[idx] : 0.0;
+      e_d[i] = (MODE == AP_CHEB && a.c1 != 0.0) ? double(a.d[idx]) : 0.0;
+      e_x[i] = (MODE == AP_CHEB || MODE == AP_PQ) ? double(a.x[idx]) : 0.0;
     }
   }
-  sell_slice_partial<V, DIFF, W, U, T, decltype(xat), double>(pat, vals, s, w, lane, row, xat, xr, acc);
+  sell_slice_partial<V, DIFF, W, U, T, decltype(xat), TX>(pat, vals, s, w, lane, row, xat, xr, acc);
   if (w > 0) {
 #pragma unroll
     for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
@@ -1126,12 +1244,12 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       if (MODE == AP_CHEB) {
         const double r = kScaled ? fma(e_dinv[i], e_rhs[i], -t) : e_dinv[i] * (e_rhs[i] - t);
         const double dn = a.c1 != 0.0 ? fma(a.c1, e_d[i], a.c2 * r) : a.c2 * r;
-        a.d[idx] = dn;
-        a.y[idx] = e_x[i] + dn;
+        a.d[idx] = TX(dn);
+        a.y[idx] = TY(e_x[i] + dn);
       } else if (MODE == AP_RESID) {
-        a.y[idx] = kScaled ? e_rhs[i] - t / e_dinv[i] : e_rhs[i] - t;
+        a.y[idx] = TY(kScaled ? e_rhs[i] - t / e_dinv[i] : e_rhs[i] - t);
       } else if (MODE == AP_DSTORE) {
-        a.y[idx] = kScaled ? t : e_dinv[i] * t;
+        a.y[idx] = TY(kScaled ? t : e_dinv[i] * t);
       } else {
         apply_epilogue<MODE>(a, idx, t);
       }
@@ -1145,10 +1263,11 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
 }
 
 // first Chebyshev step from a zero iterate: d = c2 D^-1 rhs, x = d
+template <typename TX>
 __global__ void k_cheb_first(int64_t n, const double* __restrict__ rhs, const double* __restrict__ dinv, double c2,
-                             double* __restrict__ d, double* __restrict__ x) {
+                             TX* __restrict__ d, TX* __restrict__ x) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const double v = c2 * dinv[i] * rhs[i];
+    const TX v = TX(c2 * dinv[i] * rhs[i]);
     d[i] = v;
     x[i] = v;
   }
@@ -1301,8 +1420,8 @@ __global__ __launch_bounds__(256) void k_cg_rr(const double* __restrict__ part, 
 // (19.4 against 30.8 us)
 constexpr double kRowThreshold = 400.0;
 
-template <int MODE, typename T>
-int bcsr_apply(const fem_bcsr_t& M, const ApplyArgs& a, hipStream_t s) {
+template <int MODE, typename T, typename TX = double, typename TY = double>
+int bcsr_apply(const fem_bcsr_t& M, const ApplyArgsT<TX, TY>& a, hipStream_t s) {
   if (M.rows == 0) return FEM_OK;
   // lanes per block row from the mean number of (block, column) pairs in a row: a
   // sweep of 1..64 lanes on the level-0 prolongation of the 1.6M-dof cantilever (26
@@ -1315,14 +1434,14 @@ int bcsr_apply(const fem_bcsr_t& M, const ApplyArgs& a, hipStream_t s) {
 #define FEM_AMG_APPLY(BR, BC)                                                                                  \
   if (M.br == BR && M.bc == BC) {                                                                              \
     if (G == 256)                                                                                              \
-      hipLaunchKernelGGL((k_bcsr_apply_row<BR, BC, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
-                         M.col_idx, v, a);                                                                     \
+      hipLaunchKernelGGL((k_bcsr_apply_row<BR, BC, MODE, T, TX, TY>), dim3(grid), dim3(256), 0, s, M.rows,      \
+                         M.row_ptr, M.col_idx, v, a);                                                          \
     else if (G == 64)                                                                                          \
-      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 64, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr, \
-                         M.col_idx, v, a);                                                                     \
+      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 64, MODE, T, TX, TY>), dim3(grid), dim3(256), 0, s, M.rows,      \
+                         M.row_ptr, M.col_idx, v, a);                                                          \
     else                                                                                                       \
-      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 8, MODE, T>), dim3(grid), dim3(256), 0, s, M.rows, M.row_ptr,  \
-                         M.col_idx, v, a);                                                                     \
+      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 8, MODE, T, TX, TY>), dim3(grid), dim3(256), 0, s, M.rows,       \
+                         M.row_ptr, M.col_idx, v, a);                                                          \
     FEM_LAUNCHED();                                                                                            \
     return FEM_OK;                                                                                             \
   }
@@ -1336,15 +1455,18 @@ int bcsr_apply(const fem_bcsr_t& M, const ApplyArgs& a, hipStream_t s) {
   return FEM_ERR_BAD_ARG;
 }
 
-template <int MODE, typename T, bool DIFF = false>
-int sell_apply(const fem_pattern_t& pat, const void* vals, const ApplyArgs& a, const int32_t* done, hipStream_t s) {
+template <int MODE, typename T, bool DIFF = false, typename TX = double, typename TY = double>
+int sell_apply(const fem_pattern_t& pat, const void* vals, const ApplyArgsT<TX, TY>& a, const int32_t* done,
+               hipStream_t s) {
   if (pat.num_slices == 0) return FEM_OK;
   const unsigned grid = (unsigned)pat.num_slices;
   const T* v = static_cast<const T*>(vals);
   if (pat.vec == 3)
-    hipLaunchKernelGGL((k_sell_apply<3, MODE, T, DIFF>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
+    hipLaunchKernelGGL((k_sell_apply<3, MODE, T, DIFF, TX, TY>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a,
+                       done);
   else if (pat.vec == 1)
-    hipLaunchKernelGGL((k_sell_apply<1, MODE, T, DIFF>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a, done);
+    hipLaunchKernelGGL((k_sell_apply<1, MODE, T, DIFF, TX, TY>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a,
+                       done);
   else
     FEM_CHECK_ARG(false, "AMG: vec must be 1 or 3");
   FEM_LAUNCHED();
@@ -1369,12 +1491,12 @@ struct Cheb {
   }
 };
 
-template <typename T>
-int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgs& a, hipStream_t s) {
+template <typename T, typename TX = double, typename TY = double>
+int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgsT<TX, TY>& a, hipStream_t s) {
   const fem_amg_level_t& L = amg.levels[l];
   if (l == 0) {
     if (amg.level0_form == FEM_FORM_SCALED_DIFF16) {
-      ApplyArgs b = a;
+      ApplyArgsT<TX, TY> b = a;
       b.dinv = L.dinv;   // the residual undoes the row scaling
       if (mode == AP_CHEB) return sell_apply<AP_CHEB, _Float16, true>(*L.pat, L.sell_vals, b, nullptr, s);
       return sell_apply<AP_RESID, _Float16, true>(*L.pat, L.sell_vals, b, nullptr, s);
@@ -1390,8 +1512,75 @@ int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgs& a, hipSt
   return bcsr_apply<AP_RESID, T>(L.A, a, s);
 }
 
+// Pre-smoothing (Chebyshev from a zero iterate), residual and restriction on
+// level l, vectors stored as TX.  On return *xo is the smoothed iterate and
+// *oo the other iterate buffer.
+template <typename T, typename TX>
+int level_down(const fem_amg_t& amg, int l, int64_t n, const double* rhs, double* rhs_next, TX* x, TX* other,
+               TX* dd, TX* res, TX** xo, TX** oo, hipStream_t s) {
+  const fem_amg_level_t& L = amg.levels[l];
+  const Cheb cb(L.lmax);
+  hipLaunchKernelGGL(k_cheb_first<TX>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, rhs, L.dinv, 1.0 / cb.theta, dd,
+                     x);
+  FEM_LAUNCHED();
+  double rho = 1.0 / cb.sigma;
+  for (int k = 1; k < amg.degree; ++k) {
+    const double rn = 1.0 / (2.0 * cb.sigma - rho);
+    ApplyArgsT<TX, TX> a{x, other, rhs, L.dinv, dd, rn * rho, 2.0 * rn / cb.delta, nullptr};
+    if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
+    rho = rn;
+    std::swap(x, other);
+  }
+  ApplyArgsT<TX, TX> ar{x, res, rhs, nullptr, nullptr, 0.0, 0.0, nullptr};
+  if (int rc = level_apply<T>(amg, l, AP_RESID, ar, s)) return rc;
+  ApplyArgsT<TX, double> arr{res, rhs_next, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
+  if (int rc = bcsr_apply<AP_STORE, T>(L.R, arr, s)) return rc;
+  *xo = x;
+  *oo = other;
+  return FEM_OK;
+}
+
+// Prolongation of the coarse correction xc into x and post-smoothing on level
+// l.  z (level 0 only, FP64): the output of the last Chebyshev step.  ev: see
+// vcycle_impl.
+template <typename T, typename TX>
+int level_up(const fem_amg_t& amg, int l, const double* rhs, const double* xc, TX* x, TX* other, TX* dd, double* z,
+             TX** xo, hipStream_t s, hipEvent_t* ev) {
+  const fem_amg_level_t& L = amg.levels[l];
+  const Cheb cb(L.lmax);
+  ApplyArgsT<double, TX> ap{xc, x, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
+  if (int rc = bcsr_apply<AP_ADD, T>(L.P, ap, s)) return rc;
+  double rho = 1.0 / cb.sigma;
+  for (int k = 0; k < amg.degree; ++k) {
+    double c1 = 0.0, c2 = 1.0 / cb.theta;
+    if (k > 0) {
+      const double rn = 1.0 / (2.0 * cb.sigma - rho);
+      c1 = rn * rho;
+      c2 = 2.0 * rn / cb.delta;
+      rho = rn;
+    }
+    const bool timed = ev && l == 0 && k == 0;
+    if (timed) FEM_HIP(hipEventRecord(ev[0], s));
+    if (z && k == amg.degree - 1) {
+      ApplyArgsT<TX, double> a{x, z, rhs, L.dinv, dd, c1, c2, nullptr};
+      if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
+    } else {
+      ApplyArgsT<TX, TX> a{x, other, rhs, L.dinv, dd, c1, c2, nullptr};
+      if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
+      std::swap(x, other);
+    }
+    if (timed) FEM_HIP(hipEventRecord(ev[1], s));
+  }
+  *xo = x;
+  return FEM_OK;
+}
+
 // ev (may be NULL): events recorded around the whole cycle (ev[2], ev[3]) and
 // around its first level-0 post-smoothing Chebyshev kernel (ev[0], ev[1]).
+// Level-0 vectors are stored in the hierarchy's precision T (FP32 for the
+// FP32 hierarchy: the iterates, the Chebyshev direction and the residual;
+// the right-hand side, D^-1 and the output z stay FP64); coarser levels are
+// FP64 throughout.
 template <typename T>
 int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipStream_t s,
                 hipEvent_t* ev = nullptr) {
@@ -1415,31 +1604,23 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
   // with no levels the cycle is the dense solve of the whole matrix
   rhs[NL] = NL == 0 ? const_cast<double*>(r) : take(amg.coarse_n);
   cur[NL] = NL == 0 ? z : take(amg.coarse_n);
-  const int deg = amg.degree;
+  T* x0 = nullptr;   // level 0's iterate and spare buffer, in T
+  T* o0 = nullptr;
   // down
   for (int l = 0; l < NL; ++l) {
-    const fem_amg_level_t& L = amg.levels[l];
     const int64_t n = level_n(amg, l);
-    const Cheb cb(L.lmax);
-    double* x = xa[l];
-    double* other = xb[l];
-    hipLaunchKernelGGL(k_cheb_first, dim3(grid_for(n, 256)), dim3(256), 0, s, n, rhs[l], L.dinv, 1.0 / cb.theta,
-                       dd[l], x);
-    FEM_LAUNCHED();
-    double rho = 1.0 / cb.sigma;
-    for (int k = 1; k < deg; ++k) {
-      const double rn = 1.0 / (2.0 * cb.sigma - rho);
-      ApplyArgs a{x, other, rhs[l], L.dinv, dd[l], rn * rho, 2.0 * rn / cb.delta, nullptr};
-      if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
-      rho = rn;
-      std::swap(x, other);
+    if (l == 0) {
+      if (int rc = level_down<T, T>(amg, 0, n, rhs[0], rhs[1], reinterpret_cast<T*>(xa[0]),
+                                    reinterpret_cast<T*>(xb[0]), reinterpret_cast<T*>(dd[0]),
+                                    reinterpret_cast<T*>(res[0]), &x0, &o0, s))
+        return rc;
+    } else {
+      double *x = nullptr, *o = nullptr;
+      if (int rc = level_down<T, double>(amg, l, n, rhs[l], rhs[l + 1], xa[l], xb[l], dd[l], res[l], &x, &o, s))
+        return rc;
+      cur[l] = x;
+      xb[l] = o;
     }
-    ApplyArgs ar{x, res[l], rhs[l], nullptr, nullptr, 0.0, 0.0, nullptr};
-    if (int rc = level_apply<T>(amg, l, AP_RESID, ar, s)) return rc;
-    ApplyArgs arr{res[l], rhs[l + 1], nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
-    if (int rc = bcsr_apply<AP_STORE, T>(L.R, arr, s)) return rc;
-    cur[l] = x;
-    xb[l] = other;
   }
   // coarsest
   if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[0], s));
@@ -1449,31 +1630,16 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
   if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[1], s));
   // up
   for (int l = NL - 1; l >= 0; --l) {
-    const fem_amg_level_t& L = amg.levels[l];
-    const Cheb cb(L.lmax);
-    double* x = cur[l];
-    double* other = xb[l];
-    ApplyArgs ap{cur[l + 1], x, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
-    if (int rc = bcsr_apply<AP_ADD, T>(L.P, ap, s)) return rc;
-    double rho = 1.0 / cb.sigma;
-    for (int k = 0; k < deg; ++k) {
-      double c1 = 0.0, c2 = 1.0 / cb.theta;
-      if (k > 0) {
-        const double rn = 1.0 / (2.0 * cb.sigma - rho);
-        c1 = rn * rho;
-        c2 = 2.0 * rn / cb.delta;
-        rho = rn;
-      }
-      double* out = (l == 0 && k == deg - 1) ? z : other;
-      ApplyArgs a{x, out, rhs[l], L.dinv, dd[l], c1, c2, nullptr};
-      const bool timed = ev && l == 0 && k == 0;
-      if (timed) FEM_HIP(hipEventRecord(ev[0], s));
-      if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
-      if (timed) FEM_HIP(hipEventRecord(ev[1], s));
-      other = x;
-      x = out;
+    if (l == 0) {
+      T* x = nullptr;
+      if (int rc = level_up<T, T>(amg, 0, rhs[0], cur[1], x0, o0, reinterpret_cast<T*>(dd[0]), z, &x, s, ev))
+        return rc;
+    } else {
+      double* x = nullptr;
+      if (int rc = level_up<T, double>(amg, l, rhs[l], cur[l + 1], cur[l], xb[l], dd[l], nullptr, &x, s, ev))
+        return rc;
+      cur[l] = x;
     }
-    cur[l] = x;
   }
   if (ev) FEM_HIP(hipEventRecord(ev[3], s));
   return FEM_OK;
@@ -1865,11 +2031,20 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
   const double* xv = static_cast<const double*>(X->vals);
   const double* yv = static_cast<const double*>(Y->vals);
   double* cv = static_cast<double*>(const_cast<void*>(C->vals));
-  const unsigned g = grid_for(C->nnzb, 256);
+  // YC lanes per output block; products with few output blocks (the coarse levels) split
+  // each block's plan over a whole wave
+  const bool split = C->nnzb < kSplitBelow;
 #define FEM_AMG_NUM(XR, XC, YC)                                                                               \
   if (X->br == XR && X->bc == XC && Y->bc == YC) {                                                            \
-    hipLaunchKernelGGL((k_spgemm_numeric<XR, XC, YC>), dim3(g), dim3(256), 0, s, C->nnzb, plan_ptr, plan, xv, yv, \
-                       cv);                                                                                   \
+    constexpr int GPW = 64 / (YC);                                                                            \
+    const int64_t waves = split ? C->nnzb : ceil_div(C->nnzb, GPW);                                           \
+    const dim3 grid((unsigned)ceil_div(waves, 4));                                                            \
+    if (split)                                                                                                \
+      hipLaunchKernelGGL((k_spgemm_numeric_g<XR, XC, YC, GPW>), grid, dim3(256), 0, s, C->nnzb, plan_ptr, plan, \
+                         xv, yv, cv);                                                                         \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_spgemm_numeric_g<XR, XC, YC, 1>), grid, dim3(256), 0, s, C->nnzb, plan_ptr, plan,  \
+                         xv, yv, cv);                                                                         \
     FEM_LAUNCHED();                                                                                           \
     return FEM_OK;                                                                                            \
   }
@@ -1892,13 +2067,13 @@ int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bc
   hipStream_t s = as_stream(stream);
   if (P->nnzb == 0) return FEM_OK;
   FEM_CHECK_ARG(plan && A->vals && P0->vals && P->vals && P->col_idx, "fem_amg_smooth_prolongator: arrays missing");
-  const unsigned g = grid_for(P->nnzb, 256);
   const double* av = static_cast<const double*>(A->vals);
   const double* p0 = static_cast<const double*>(P0->vals);
   double* pv = static_cast<double*>(const_cast<void*>(P->vals));
 #define FEM_AMG_SMOOTH(BS, NS)                                                                                  \
   if (P->br == BS && P->bc == NS) {                                                                             \
-    hipLaunchKernelGGL((k_smooth_numeric<BS, NS>), dim3(g), dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,         \
+    const dim3 grid((unsigned)ceil_div(ceil_div(P->nnzb, 64 / (NS)), 4));                                       \
+    hipLaunchKernelGGL((k_smooth_numeric_g<BS, NS>), grid, dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,        \
                        P->col_idx, plan_ptr, plan, av, p0, agg, dinv, omega, pv);                               \
     FEM_LAUNCHED();                                                                                             \
     return FEM_OK;                                                                                              \

@@ -696,8 +696,16 @@ class DeviceProblem:
         mag = G.reshape(n, -1).abs().amax(1)
         idx = torch.unique(torch.cat([torch.topk(mag, k).indices,
                                       torch.linspace(0, n - 1, k, device=G.device).long()]))
-        # the map is evaluated where identify() probed it: on the CPU, in float64
+        # the map is evaluated where identify() probed it: on the CPU, in float64; states with
+        # non-finite entries are left to the solve, which reports them
         Gs, ivs = G[idx].cpu(), [t[idx].cpu() for t in lc['ivs']]
+        fin = torch.isfinite(Gs.reshape(len(idx), -1)).all(1)
+        for t in ivs:
+            fin &= torch.isfinite(t.reshape(len(idx), -1)).all(1)
+        if not bool(fin.all()):
+            Gs, ivs = Gs[fin], [t[fin] for t in ivs]
+            if Gs.shape[0] == 0:
+                return
         from .materials import PointMap, Tabulated
         pm = self._mat_cache.get(('law_pm', lc['key']))
         if pm is None:

@@ -188,6 +188,9 @@ def solver(problem, solver_options={}):
     dev = problem.device()
     dev.prepare()
     timing = {'local_assembly': 0., 'global_matrix': 0., 'linear': 0.}
+    if lin.get('method') == 'amg' and len(dev.vecs) > 1:
+        raise NotImplementedError("the AMG hierarchy takes one scalar or one vector variable (vec 1 or 3); "
+                                  "solve several variables with 'jax_solver'")
     constrained = dev.constraint() is not None
     if constrained:
         # linear multipoint constraints u = P u_r (solver.py:1290-1339): Newton on the reduced dofs,
