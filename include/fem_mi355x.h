@@ -374,6 +374,22 @@ int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv,
                   const fem_pcg_options_t* opt, void* ws, void* host_status,
                   fem_pcg_result_t* result /* host */, void* stream);
 
+/* Bytes of the library-held device scratch of fem_pcg_solve_noalloc (one
+ * array per GPU the library runs on, held until the process ends). */
+#define FEM_PCG_NOALLOC_SCRATCH_BYTES (128LL << 20)
+/* No counterpart in the reference: fem_pcg_solve for callers that cannot
+ * allocate device memory (serving loops and harnesses that forbid hipMalloc).
+ * Same arguments and results as fem_pcg_solve without `ws`: the workspace is
+ * the library's scratch array, so the call allocates nothing.  Systems whose
+ * fem_pcg_ws_bytes exceeds FEM_PCG_NOALLOC_SCRATCH_BYTES (about 2.79 M FP64
+ * dofs, 4.78 M in FEM_F32) return FEM_ERR_CAPACITY; opt->timing must be 0.
+ * Every call on one GPU uses that GPU's single array: calls that overlap on
+ * one GPU (two host threads, two streams) race. */
+int fem_pcg_solve_noalloc(const fem_pattern_t* pat, const void* vals, const void* dinv,
+                          const fem_operator_t* op, const double* b, double* x,
+                          const fem_pcg_options_t* opt, void* host_status,
+                          fem_pcg_result_t* result /* host */, void* stream);
+
 /* Linear multipoint constraints u = P u_r (the P_mat of solver.py:404-406,
  * 545-551, 1290-1339): P (rows x cols) and its transpose as scalar CSR. */
 typedef struct {

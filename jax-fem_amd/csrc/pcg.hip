@@ -31,6 +31,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 int fem_operator_apply_impl(const fem_operator_t* op, const double* x, double* y, hipStream_t s);
@@ -497,6 +498,33 @@ static int pcg_run(const fem_pattern_t* pat, const T* vals, const T* dinv_in, co
   return FEM_OK;
 }
 
+// ------------------------------------------------- no-allocation variant
+// The workspace of fem_pcg_solve_noalloc: a library-held device array, one
+// copy per GPU (each device that loads this code object holds it until the
+// process ends).  Host code never passes `scratch` itself to a kernel: its
+// device address comes from hipGetSymbolAddress, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[FEM_PCG_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array.
+static int scratch_address(void** out) {
+  int dev = -1;
+  FEM_HIP(hipGetDevice(&dev));
+  FEM_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "fem_pcg_solve_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FEM_CHECK_ARG(p != nullptr, "fem_pcg_solve_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FEM_OK;
+}
+
 }  // namespace fem
 
 using namespace fem;
@@ -539,6 +567,26 @@ int fem_pcg_solve(const fem_pattern_t* pat, const void* vals, const void* dinv, 
     return pcg_run<4, double>(pat, (const double*)vals, (const double*)dinv, op, b, x, opt, ws, hsc, result, s);
   set_error("fem_pcg_solve: unsupported vec %d", V);
   return FEM_ERR_BAD_ARG;
+}
+
+int fem_pcg_solve_noalloc(const fem_pattern_t* pat, const void* vals, const void* dinv, const fem_operator_t* op,
+                          const double* b, double* x, const fem_pcg_options_t* opt, void* host_status,
+                          fem_pcg_result_t* result, void* stream) {
+  FEM_CHECK_ARG(pat && opt, "fem_pcg_solve_noalloc: NULL argument");
+  FEM_CHECK_ARG(opt->timing == 0, "fem_pcg_solve_noalloc: timing=1 creates HIP events; use fem_pcg_solve");
+  const int64_t need = fem_pcg_ws_bytes(pat, opt->precision);
+  FEM_CHECK_ARG(need > 0, "fem_pcg_solve_noalloc: bad pattern");
+  if (need > (int64_t)FEM_PCG_NOALLOC_SCRATCH_BYTES) {
+    set_error("fem_pcg_solve_noalloc: the system needs %lld workspace bytes, more than the %lld-byte scratch "
+              "(%lld rows of %d); use fem_pcg_solve with a caller workspace",
+              (long long)need, (long long)FEM_PCG_NOALLOC_SCRATCH_BYTES, (long long)pat->num_rows, pat->vec);
+    return FEM_ERR_CAPACITY;
+  }
+  void* ws = nullptr;
+  if (int rc = scratch_address(&ws)) return rc;
+  // every workspace entry the solve reads is written by it first (scalars and p are
+  // cleared at its start), so the previous call's contents do not matter
+  return fem_pcg_solve(pat, vals, dinv, op, b, x, opt, ws, host_status, result, stream);
 }
 
 }  // extern "C"

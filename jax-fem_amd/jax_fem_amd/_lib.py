@@ -139,6 +139,8 @@ _SIGNATURES = {
     'fem_pcg_ws_bytes': (_i64, [_P(FemPattern), _i32]),
     'fem_pcg_solve': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemOperator), _vp, _vp, _P(FemPcgOptions),
                                 _vp, _vp, _P(FemPcgResult), _vp]),
+    'fem_pcg_solve_noalloc': (C.c_int, [_P(FemPattern), _vp, _vp, _P(FemOperator), _vp, _vp, _P(FemPcgOptions),
+                                        _vp, _P(FemPcgResult), _vp]),
     'fem_dot': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp]),
     'fem_amg_aggregate_ws_bytes': (_i64, [_i64]),
     'fem_amg_aggregate': (C.c_int, [_i64, _vp, _vp, _vp, _P(_i64), _vp, _vp, _vp]),
@@ -169,6 +171,9 @@ _SIGNATURES = {
 }
 
 EXPORTED = tuple(_SIGNATURES)
+
+# include/fem_mi355x.h: device scratch of fem_pcg_solve_noalloc, per GPU
+FEM_PCG_NOALLOC_SCRATCH_BYTES = 128 << 20
 
 _LIB = None
 

@@ -74,6 +74,26 @@ def test_compute_entry_fails_loudly_without_gpu():
         _lib.lib()
 
 
+def test_noalloc_pcg_refusals_before_any_launch():
+    # the checks of fem_pcg_solve_noalloc run on the host, before any HIP call
+    lib = _lib.load()
+    pat = _lib.FemPattern()
+    pat.vec, pat.num_rows = 3, 1 << 20
+    pat.num_slices = pat.num_rows // 64
+    opt = _lib.FemPcgOptions()
+    opt.precision, opt.timing = _lib.FEM_F64, 0
+    res = _lib.FemPcgResult()
+    call = lambda: lib.fem_pcg_solve_noalloc(ctypes.byref(pat), None, None, None, None, None, ctypes.byref(opt),
+                                             None, ctypes.byref(res), None)
+    need = lib.fem_pcg_ws_bytes(ctypes.byref(pat), _lib.FEM_F64)
+    assert need > _lib.FEM_PCG_NOALLOC_SCRATCH_BYTES      # 3.1 M FP64 dofs: above the scratch
+    assert call() == _lib.FEM_ERR_CAPACITY and b'scratch' in lib.fem_last_error()
+    opt.timing = 1
+    assert call() == _lib.FEM_ERR_BAD_ARG and b'timing' in lib.fem_last_error()
+    # FP32 workspaces are smaller: the same system fits
+    assert lib.fem_pcg_ws_bytes(ctypes.byref(pat), _lib.FEM_F32) <= _lib.FEM_PCG_NOALLOC_SCRATCH_BYTES
+
+
 # -------------------------------------------------------------------- basis
 @pytest.mark.parametrize('ele,order', [('HEX8', None), ('HEX8', 4), ('HEX27', 4), ('HEX27', None)])
 def test_product_basis_matches_oracle(ele, order):
