@@ -102,11 +102,13 @@ def measured_traffic(cells, path=TRAFFIC_PROFILE):
     return {}
 
 
-def cheb_bytes(dev, elem_bytes, vec=3):
-    """Algorithmic bytes of one level-0 Chebyshev step of the V-cycle (k_sell_apply, CHEB): the
-    matrix blocks (values + one column index each, padding not counted) once, and the FP64
-    vectors x (gathered, counted once), rhs, D^-1, d (read and written) and the new x."""
-    return dev.nnzb * (vec * vec * elem_bytes + 4) + dev.n * 8 * 6
+def cheb_bytes(dev, elem_bytes, vec=3, vec_bytes=4):
+    """Algorithmic bytes of the timed level-0 Chebyshev step of the V-cycle (k_sell_apply, CHEB:
+    the first post-smoothing step, whose direction starts from zero, so d is written, not read):
+    the matrix blocks (values + one column index each, padding not counted) once; x (gathered,
+    counted once), d and the new x in the FP32 hierarchy's level-0 vector precision; the FP64
+    rhs and D^-1."""
+    return dev.nnzb * (vec * vec * elem_bytes + 4) + dev.n * (3 * vec_bytes + 16)
 
 
 def assembly_bytes(dev):

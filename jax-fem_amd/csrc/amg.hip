@@ -818,13 +818,13 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_
 // Products with fewer output blocks than this split each block's plan over a wave.
 constexpr int64_t kSplitBelow = 200000;
 
-// C = X Y by its plan, YC lanes per output block: lane c of a group forms column
-// c of the block (the group reads the X block together; lane c reads column c
-// of the Y block), so a product needs XR + XR*XC + XC registers instead of a
-// whole block pair and many more products are in flight per CU.  With S > 1
-// (few output blocks: the coarse levels), the S groups of a wave split one
-// block's plan (group g takes products g, g + S, ...) and group 0 sums their
-// columns in group order.  Both orders are fixed: bit-reproducible.
+// C = X Y by its plan for products with few output blocks (the coarse levels,
+// where one thread per block leaves the GPU nearly idle on long serial sums):
+// YC lanes form one column group (lane c forms column c of the block: the group
+// reads the X block together, lane c reads column c of the Y block), and with
+// S > 1 the S groups of a wave split one block's plan (group g takes products
+// g, g + S, ...); group 0 sums their columns in group order.  The order is
+// fixed: bit-reproducible.
 template <int XR, int XC, int YC, int S>
 __global__ __launch_bounds__(256) void k_spgemm_numeric_g(int64_t nb, const int32_t* __restrict__ plan_ptr,
                                                           const int32_t* __restrict__ plan,
@@ -870,60 +870,6 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric_g(int64_t nb, const int3
     double* out = cv + (size_t)b * XR * YC + c;
 #pragma unroll
     for (int i = 0; i < XR; ++i) out[i * YC] = acc[i];
-  }
-}
-
-// P = P0 - omega D^-1 (A P0) along the plan of A P0, NS lanes per block of P
-// (lane j forms column j), as in k_spgemm_numeric_g.
-template <int BS, int NS>
-__global__ __launch_bounds__(256) void k_smooth_numeric_g(int64_t nb, int64_t rows, const int32_t* __restrict__ crp,
-                                                          const int32_t* __restrict__ cci,
-                                                          const int32_t* __restrict__ plan_ptr,
-                                                          const int32_t* __restrict__ plan,
-                                                          const double* __restrict__ av,
-                                                          const double* __restrict__ p0,
-                                                          const int32_t* __restrict__ agg,
-                                                          const double* __restrict__ dinv, double omega,
-                                                          double* __restrict__ pv) {
-  constexpr int G = NS, GPW = 64 / G;
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int grp = lane / G, j = lane - grp * G;
-  const int64_t b = wave * GPW + grp;
-  if (grp >= GPW || b >= nb) return;
-  int64_t lo = 0, hi = rows;  // row i: crp[i] <= b < crp[i+1]
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (crp[mid] <= b)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const int64_t i = lo;
-  double acc[BS];
-#pragma unroll
-  for (int r = 0; r < BS; ++r) acc[r] = 0.0;
-  const int q1 = plan_ptr[b + 1];
-#pragma unroll 2
-  for (int q = plan_ptr[b]; q < q1; ++q) {
-    const int2 ef = reinterpret_cast<const int2*>(plan)[q];
-    double xa[BS][BS], yc[BS];
-    load_block<BS, BS>(av + (size_t)ef.x * BS * BS, xa);
-    const double* y = p0 + (size_t)ef.y * BS * NS + j;
-#pragma unroll
-    for (int m = 0; m < BS; ++m) yc[m] = y[m * NS];
-#pragma unroll
-    for (int r = 0; r < BS; ++r)
-#pragma unroll
-      for (int m = 0; m < BS; ++m) acc[r] = fma(xa[r][m], yc[m], acc[r]);
-  }
-  const bool own = cci[b] == agg[i];
-  double* out = pv + (size_t)b * BS * NS + j;
-#pragma unroll
-  for (int r = 0; r < BS; ++r) {
-    double v = -omega * dinv[i * BS + r] * acc[r];
-    if (own) v += p0[(i * BS + r) * NS + j];
-    out[r * NS] = v;
   }
 }
 
@@ -1941,13 +1887,19 @@ int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell
   w += align256(kDotBlocks * 8);
   double* nrm = reinterpret_cast<double*>(w);
   const unsigned g = grid_for(n, 256);
+  auto norm2 = [&](const double* v, double* out) {
+    hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, v, v, part, (const int32_t*)nullptr);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, out);
+  };
   hipLaunchKernelGGL(k_start_vec, dim3(g), dim3(256), 0, s, n, x);
   FEM_LAUNCHED();
-  hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, x, x, part, (const int32_t*)nullptr);
-  hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, nrm);
-  hipLaunchKernelGGL(k_scale_by_norm, dim3(g), dim3(256), 0, s, n, x, nrm, x);
-  FEM_LAUNCHED();
+  // y_k = (D^-1 A)^k x_0, rescaled to unit norm every kRescale steps (the iterate
+  // grows like lmax^k); lmax = |y_s| / |y_{s-1}|, the norm of the last step's
+  // product of a unit vector
+  constexpr int kRescale = 8;
   for (int k = 0; k < steps; ++k) {
+    const bool last = k == steps - 1;
+    if (last) norm2(x, nrm + 1);
     ApplyArgs a{x, y, nullptr, dinv, nullptr, 0.0, 0.0, nullptr};
     if (int rc = !pat ? bcsr_apply<AP_DSTORE, double>(*A, a, s)
                  : sell_f32 == 2 ? sell_apply<AP_DSTORE, _Float16, true>(*pat, sell_vals, a, nullptr, s)
@@ -1956,15 +1908,19 @@ int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell
                             : sell_apply<AP_DSTORE, double>(*pat, static_cast<const double*>(sell_vals), a, nullptr,
                                                             s))
       return rc;
-    hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, y, y, part, (const int32_t*)nullptr);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(256), 0, s, part, kDotBlocks, nrm);
-    hipLaunchKernelGGL(k_scale_by_norm, dim3(g), dim3(256), 0, s, n, y, nrm, x);
+    if (last) {
+      norm2(y, nrm);
+    } else if (k % kRescale == kRescale - 1) {
+      norm2(y, nrm);
+      hipLaunchKernelGGL(k_scale_by_norm, dim3(g), dim3(256), 0, s, n, y, nrm, y);
+    }
     FEM_LAUNCHED();
+    std::swap(x, y);
   }
-  double v = 0.0;  // |y|^2 of the last step
-  FEM_HIP(hipMemcpyAsync(&v, nrm, sizeof(double), hipMemcpyDeviceToHost, s));
+  double v[2] = {0.0, 0.0};  // |y_s|^2, |y_{s-1}|^2
+  FEM_HIP(hipMemcpyAsync(v, nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   FEM_HIP(hipStreamSynchronize(s));
-  *lmax = sqrt(v);
+  *lmax = sqrt(v[0] / v[1]);
   return FEM_OK;
 }
 
@@ -2031,8 +1987,10 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
   const double* xv = static_cast<const double*>(X->vals);
   const double* yv = static_cast<const double*>(Y->vals);
   double* cv = static_cast<double*>(const_cast<void*>(C->vals));
-  // YC lanes per output block; products with few output blocks (the coarse levels) split
-  // each block's plan over a whole wave
+  // one thread per output block; products with few output blocks (the coarse levels) split
+  // each block's plan over a whole wave (YC lanes per column group).  On level 0 of the
+  // 1.6M-dof cantilever the grouped form with one block per group is slower (A P: 3.50 ->
+  // 4.00 ms; it issues about four times the load instructions per product)
   const bool split = C->nnzb < kSplitBelow;
 #define FEM_AMG_NUM(XR, XC, YC)                                                                               \
   if (X->br == XR && X->bc == XC && Y->bc == YC) {                                                            \
@@ -2043,8 +2001,8 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
       hipLaunchKernelGGL((k_spgemm_numeric_g<XR, XC, YC, GPW>), grid, dim3(256), 0, s, C->nnzb, plan_ptr, plan, \
                          xv, yv, cv);                                                                         \
     else                                                                                                      \
-      hipLaunchKernelGGL((k_spgemm_numeric_g<XR, XC, YC, 1>), grid, dim3(256), 0, s, C->nnzb, plan_ptr, plan,  \
-                         xv, yv, cv);                                                                         \
+      hipLaunchKernelGGL((k_spgemm_numeric<XR, XC, YC>), dim3(grid_for(C->nnzb, 256)), dim3(256), 0, s,       \
+                         C->nnzb, plan_ptr, plan, xv, yv, cv);                                                \
     FEM_LAUNCHED();                                                                                           \
     return FEM_OK;                                                                                            \
   }
@@ -2070,10 +2028,10 @@ int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bc
   const double* av = static_cast<const double*>(A->vals);
   const double* p0 = static_cast<const double*>(P0->vals);
   double* pv = static_cast<double*>(const_cast<void*>(P->vals));
+  const unsigned g = grid_for(P->nnzb, 256);
 #define FEM_AMG_SMOOTH(BS, NS)                                                                                  \
   if (P->br == BS && P->bc == NS) {                                                                             \
-    const dim3 grid((unsigned)ceil_div(ceil_div(P->nnzb, 64 / (NS)), 4));                                       \
-    hipLaunchKernelGGL((k_smooth_numeric_g<BS, NS>), grid, dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,        \
+    hipLaunchKernelGGL((k_smooth_numeric<BS, NS>), dim3(g), dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,         \
                        P->col_idx, plan_ptr, plan, av, p0, agg, dinv, omega, pv);                               \
     FEM_LAUNCHED();                                                                                             \
     return FEM_OK;                                                                                              \

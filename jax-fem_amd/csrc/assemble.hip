@@ -178,6 +178,7 @@ template <bool S>
 struct IsoLinear<MatElastic<S>> : std::true_type {};
 
 constexpr int kAsmMaxK = 4;  // slots per thread: rows of at most 8 * 4 = 32 blocks
+constexpr int kResWave0 = kAsmWaves - 3;  // waves forming the residual's components
 
 template <class MAT, typename TO, bool DIFF, bool RES>
 __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
@@ -192,7 +193,15 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
   __shared__ double s_g[CH][GS];
   __shared__ double s_jxw[CH][Q + 1];
   __shared__ StRow<MAT> s_st[CH];
-  __shared__ double s_P[RES ? CH : 1][RES ? Q * V * 3 + 1 : 1];
+  // phase A1 stages the chunk's node coordinates and u in s_buf; with the residual, the
+  // weighted stresses of phase A2 reuse it
+  constexpr int kXU = CH * N * (3 + V);
+  constexpr int kP = RES ? CH * (Q * V * 3 + 1) : 0;
+  __shared__ double s_buf[kXU > kP ? kXU : kP];
+  __shared__ int64_t s_c[CH];
+  double(*s_x)[N][3] = reinterpret_cast<double(*)[N][3]>(s_buf);
+  double(*s_u)[N][V] = reinterpret_cast<double(*)[N][V]>(s_buf + CH * N * 3);
+  double(*s_P)[Q * V * 3 + 1] = reinterpret_cast<double(*)[Q * V * 3 + 1]>(s_buf);
   // the difference form's row sums are staged in s_g after the last chunk, half the waves at a time
   constexpr int kHalf = kAsmWaves / 2;
   static_assert(kHalf * kSlice * VV <= CH * GS, "row-sum staging does not fit in s_g");
@@ -212,7 +221,9 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
   const int ncell = sc_ptr[s + 1] - c0;
 
   double acc[MAXK][VV];
-  int cur[MAXK], cend[MAXK];
+  // cur: next contribution of each owned slot; nxt: its slice-local id, loaded one
+  // contribution ahead so the load overlaps the block being formed
+  int cur[MAXK], cend[MAXK], nxt[MAXK];
 #pragma unroll
   for (int m = 0; m < MAXK; ++m) {
     const int k = w + kAsmWaves * m;
@@ -221,89 +232,119 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
       cur[m] = pat.contrib_ptr[rb + k];
       cend[m] = pat.contrib_ptr[rb + k + 1];
     }
+    nxt[m] = cur[m] < cend[m] ? cloc[cur[m]] : INT32_MAX;
 #pragma unroll
     for (int v = 0; v < VV; ++v) acc[m][v] = 0.0;
   }
-  double r[V];
-#pragma unroll
-  for (int i = 0; i < V; ++i) r[i] = 0.0;
+  // residual: component ri of the row is formed by wave kResWave0 + ri (the last waves,
+  // which own one slot fewer on 27-block rows) from the diagonal slot's list, which
+  // holds exactly the row's incidences (c, a, a) in cell order
+  const int ri = w - kResWave0;
+  const bool resw = RES && real && ri >= 0 && ri < V;
+  int rcur = 0, rend = 0, rnxt = INT32_MAX;
+  if (resw) {
+    rcur = pat.contrib_ptr[rb + ds];
+    rend = pat.contrib_ptr[rb + ds + 1];
+    rnxt = rcur < rend ? cloc[rcur] : INT32_MAX;
+  }
+  double r = 0.0;
   __syncthreads();
 
   for (int ch0 = 0; ch0 < ncell; ch0 += CH) {
-    // ---- phase A: per (cell, quadrature point) geometry and material state
-    if (t < CH * Q) {
-      const int cc = t / Q, q = t % Q;
+    // ---- phase A1: the chunk's node coordinates (and u) staged in LDS, one (cell, node) per thread
+    if (t < CH * N) {
+      const int cc = t / N, a = t % N;
       if (ch0 + cc < ncell) {
         const int64_t c = sc[c0 + ch0 + cc];
-        const int32_t* cn = mesh.cells + c * N;
-        const double* p0 = mesh.points + (int64_t)cn[0] * 3;
-        const double x00 = p0[0], x01 = p0[1], x02 = p0[2];
-        double J[9];
+        if (a == 0) s_c[cc] = c;
+        const int64_t na = mesh.cells[c * N + a];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) J[k] = 0.0;
-        // rolled loops keep phase A's registers low: the slot accumulators stay live across it
+        for (int j = 0; j < 3; ++j) s_x[cc][a][j] = mesh.points[na * 3 + j];
+        if (MAT::kNeedsU || RES)
+#pragma unroll
+          for (int i = 0; i < V; ++i) s_u[cc][a][i] = u[na * V + i];
+      }
+    }
+    __syncthreads();
+    // ---- phase A2: per (cell, quadrature point) geometry and material state
+    double Pw[RES ? V : 1][3];
+    const int acc_cc = t / Q, acc_q = t % Q;
+    const bool a2 = t < CH * Q && ch0 + acc_cc < ncell;
+    if (a2) {
+      const int cc = acc_cc, q = acc_q;
+      const int64_t c = s_c[cc];
+      const double x00 = s_x[cc][0][0], x01 = s_x[cc][0][1], x02 = s_x[cc][0][2];
+      double J[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) J[k] = 0.0;
 #pragma unroll 1
-        for (int a = 1; a < N; ++a) {
-          const double* pa = mesh.points + (int64_t)cn[a] * 3;
-          const double d0 = pa[0] - x00, d1 = pa[1] - x01, d2 = pa[2] - x02;
+      for (int a = 1; a < N; ++a) {
+        const double d0 = s_x[cc][a][0] - x00, d1 = s_x[cc][a][1] - x01, d2 = s_x[cc][a][2] - x02;
 #pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const double dn = s_dN[q][a][j];
-            J[0 * 3 + j] += d0 * dn;
-            J[1 * 3 + j] += d1 * dn;
-            J[2 * 3 + j] += d2 * dn;
+        for (int j = 0; j < 3; ++j) {
+          const double dn = s_dN[q][a][j];
+          J[0 * 3 + j] += d0 * dn;
+          J[1 * 3 + j] += d1 * dn;
+          J[2 * 3 + j] += d2 * dn;
+        }
+      }
+      const double det = det3(J);
+      double iJ[9];
+      inv3(J, det, iJ);
+      const double jxw = det * s_w[q];
+      double G[V][3];
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) G[i][j] = 0.0;
+#pragma unroll 1
+      for (int a = 0; a < N; ++a) {
+        const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
+        double g[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          g[j] = e0 * iJ[0 * 3 + j] + e1 * iJ[1 * 3 + j] + e2 * iJ[2 * 3 + j];
+          s_g[cc][(q * N + a) * 3 + j] = g[j];
+        }
+        if (MAT::kNeedsU || RES) {
+#pragma unroll
+          for (int i = 0; i < V; ++i) {
+            const double ua = s_u[cc][a][i];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) G[i][j] += ua * g[j];
           }
         }
-        const double det = det3(J);
-        double iJ[9];
-        inv3(J, det, iJ);
-        const double jxw = det * s_w[q];
-        double G[V][3];
+      }
+      s_jxw[cc][q] = jxw;
+      const auto st = MAT::template prep<double>(mat, c * Q + q, G);
+      s_st[cc].s[q] = st;
+      if (RES) {
+        double P[V][3];
+        MAT::stress(st, G, P);
 #pragma unroll
         for (int i = 0; i < V; ++i)
 #pragma unroll
-          for (int j = 0; j < 3; ++j) G[i][j] = 0.0;
-#pragma unroll 1
-        for (int a = 0; a < N; ++a) {
-          const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
-          double g[3];
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            g[j] = e0 * iJ[0 * 3 + j] + e1 * iJ[1 * 3 + j] + e2 * iJ[2 * 3 + j];
-            s_g[cc][(q * N + a) * 3 + j] = g[j];
-          }
-          if (MAT::kNeedsU || RES) {
-            const int64_t na = cn[a];
-#pragma unroll
-            for (int i = 0; i < V; ++i) {
-              const double ua = u[na * V + i];
-#pragma unroll
-              for (int j = 0; j < 3; ++j) G[i][j] += ua * g[j];
-            }
-          }
-        }
-        s_jxw[cc][q] = jxw;
-        const auto st = MAT::template prep<double>(mat, c * Q + q, G);
-        s_st[cc].s[q] = st;
-        if (RES) {
-          double P[V][3];
-          MAT::stress(st, G, P);
-#pragma unroll
-          for (int i = 0; i < V; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) s_P[cc][(q * V + i) * 3 + j] = P[i][j] * jxw;
-        }
+          for (int j = 0; j < 3; ++j) Pw[i][j] = P[i][j] * jxw;
       }
+    }
+    if (RES) {
+      // the weighted stresses overwrite the coordinates' storage once every thread has read it
+      __syncthreads();
+      if (a2)
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) s_P[acc_cc][(acc_q * V + i) * 3 + j] = Pw[i][j];
     }
     __syncthreads();
     // ---- phase B: this thread's slots, contributions of this chunk's cells
     const int lim = (ch0 + CH) * N * N;  // contributions of cells before the chunk's end
 #pragma unroll
     for (int m = 0; m < MAXK; ++m) {
-      while (cur[m] < cend[m]) {
-        const int e = cloc[cur[m]];
-        if (e >= lim) break;
+      int e = nxt[m];
+      while (e < lim) {
         ++cur[m];
+        const int en = cur[m] < cend[m] ? cloc[cur[m]] : INT32_MAX;
         const int cc = e / (N * N) - ch0;
         const int a = (e >> 3) & 7, b = e & 7;
         double K[V][V];
@@ -349,18 +390,23 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
         for (int i = 0; i < V; ++i)
 #pragma unroll
           for (int k = 0; k < V; ++k) acc[m][i * V + k] += K[i][k];
-        if (RES && w + kAsmWaves * m == ds) {
-          // the diagonal slot lists exactly the row's incidences (c, a, a), in cell order
-#pragma unroll
-          for (int i = 0; i < V; ++i) {
-            double ri = 0.0;
+        e = en;
+      }
+      nxt[m] = e;
+    }
+    if (RES && resw) {
+      while (rnxt < lim) {
+        ++rcur;
+        const int en = rcur < rend ? cloc[rcur] : INT32_MAX;
+        const int cc = rnxt / (N * N) - ch0;
+        const int a = (rnxt >> 3) & 7;
+        double t = 0.0;
 #pragma unroll 2
-            for (int q = 0; q < Q; ++q)
+        for (int q = 0; q < Q; ++q)
 #pragma unroll
-              for (int j = 0; j < 3; ++j) ri += s_P[cc][(q * V + i) * 3 + j] * s_g[cc][(q * N + a) * 3 + j];
-            r[i] += ri;
-          }
-        }
+          for (int j = 0; j < 3; ++j) t += s_P[cc][(q * V + ri) * 3 + j] * s_g[cc][(q * N + a) * 3 + j];
+        r += t;
+        rnxt = en;
       }
     }
     __syncthreads();
@@ -415,9 +461,9 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
 #pragma unroll
     for (int v = 0; v < VV; ++v) vp[((size_t)k * VV + v) * kSlice] = TO((DIFF && k == ds) ? rsum[v] : acc[m][v]);
   }
-  if (RES && real && w == (ds >= 0 ? ds % kAsmWaves : 0)) {
-#pragma unroll
-    for (int i = 0; i < V; ++i) res[(int64_t)row * V + i] = (base ? base[(int64_t)row * V + i] : 0.0) + r[i];
+  if (RES && resw) {
+    const int64_t idx = (int64_t)row * V + ri;
+    res[idx] = (base ? base[idx] : 0.0) + r;
   }
 }
 
