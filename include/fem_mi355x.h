@@ -332,9 +332,12 @@ int fem_operator_apply(const fem_operator_t* op, const double* x, double* y, voi
  * position in that list. */
 typedef struct {
   const int32_t* slice_cell_ptr; /* (num_slices + 1)                           */
-  const int32_t* slice_cells;    /* unique incident cells of each slice, ascending */
+  const int32_t* slice_cells;    /* unique incident cells of each slice, in chunk
+                                    order: cells whose lowest slice lane is even
+                                    first, each group by ascending id         */
   const int32_t* contrib_loc;    /* (num_cells*N*N), aligned with contrib_idx:
-                                    position * N*N + a*N + b                   */
+                                    position * N*N + a*N + b, ascending within
+                                    each slot                                  */
   int32_t        max_slots;      /* largest slot count of a slice (<= 32)     */
   int32_t        pad;
 } fem_assembly_plan_t;
@@ -352,8 +355,8 @@ int fem_assembly_plan_fill(const fem_pattern_t* pat, const int32_t* inc_ptr, con
 /* Assemble the tangent of HEX8 cells with 8 quadrature points at u into the
  * SELL values: FP64 standard form (precision FEM_F64, form FEM_FORM_STANDARD)
  * or FP32 difference form (FEM_F32, FEM_FORM_DIFFERENCE); the cell tangents
- * are FP64 and each matrix value is their FP64 sum in cell order, rounded
- * once.  dinv (may be NULL): Jacobi inverse diagonal in `precision`.  res (may
+ * are FP64 and each matrix value is their FP64 sum in the plan's chunk
+ * order, rounded once.  dinv (may be NULL): Jacobi inverse diagonal in `precision`.  res (may
  * be NULL): R(u) without Dirichlet rows, = base (may be NULL) + the scattered
  * internal force, FP64.  Dirichlet rows are applied afterwards by
  * fem_matrix_bc_rows. */

@@ -152,6 +152,86 @@ __global__ __launch_bounds__(256) void k_contrib_local(fem_pattern_t pat, const 
   }
 }
 
+// Chunk order of each slice's cell list.  A cell is ranked by the parity of the
+// lowest slice lane among its nodes, then by id: on a structured HEX8 box in
+// natural row order, the cells of one parity touch every row of a z-line slice
+// once, so the assembly's fixed-size cell chunks keep every row lane busy
+// (in id order a chunk covers half the rows).  Rewrites the slice's cell list
+// in that order and re-expresses contrib_loc in it, each slot's contributions
+// sorted by position (the order the assembly sums them in).  One 64-thread
+// workgroup per slice; integer LDS minima only, so the result is deterministic.
+__global__ __launch_bounds__(64) void k_slice_reorder(fem_pattern_t pat, const int32_t* __restrict__ inc_ptr,
+                                                      const int32_t* __restrict__ inc_idx,
+                                                      const int32_t* __restrict__ sc_ptr, int32_t* __restrict__ sc,
+                                                      int32_t* __restrict__ loc, const int32_t* __restrict__ err) {
+  __shared__ int32_t s_cell[kSliceIncCap];
+  __shared__ int32_t s_rank[kSliceIncCap];
+  __shared__ int32_t s_cnt[64];
+  if (*err) return;  // the plan is refused (a slice over capacity or a cell missing from its list)
+  const int64_t s = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int c0 = sc_ptr[s], n = sc_ptr[s + 1] - c0;
+  const int N = pat.nodes_per_cell, NN = N * N;
+  for (int i = lane; i < n; i += 64) {
+    s_cell[i] = sc[c0 + i];
+    s_rank[i] = 64;  // lowest lane among the cell's nodes, 64 = none yet
+  }
+  __syncthreads();
+  const int row = pat.sell_row[s * kSlice + lane];
+  auto find = [&](int32_t c) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_cell[mid] < c) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
+  if (row >= 0)
+    for (int j = inc_ptr[row]; j < inc_ptr[row + 1]; ++j) {
+      const int p = find(inc_idx[j] / N);
+      if (p < n) atomicMin(&s_rank[p], lane);
+    }
+  __syncthreads();
+  // parity-0 cells first, each group in id order: rank = group offset + count of the
+  // group's cells before it (per-lane runs, then a scan over the 64 lanes)
+  const int per = (n + 63) / 64;
+  const int i0 = min(n, lane * per), i1 = min(n, i0 + per);
+  int odd = 0;
+  for (int i = i0; i < i1; ++i) odd += s_rank[i] & 1;
+  s_cnt[lane] = odd;
+  __syncthreads();
+  if (lane == 0)
+    for (int l = 1; l < 64; ++l) s_cnt[l] += s_cnt[l - 1];
+  __syncthreads();
+  const int n_odd = s_cnt[63], n_even = n - n_odd;
+  int o = lane ? s_cnt[lane - 1] : 0;  // odd cells before i0
+  int e = i0 - o;                      // even cells before i0
+  for (int i = i0; i < i1; ++i) {
+    const bool is_odd = s_rank[i] & 1;
+    s_rank[i] = is_odd ? n_even + o++ : e++;
+  }
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) sc[c0 + s_rank[i]] = s_cell[i];
+  if (row < 0) return;
+  const int rb = pat.row_ptr[row], len = pat.row_ptr[row + 1] - rb;
+  for (int k = 0; k < len; ++k) {
+    const int t0 = pat.contrib_ptr[rb + k], t1 = pat.contrib_ptr[rb + k + 1];
+    for (int t = t0; t < t1; ++t) {
+      const int v = loc[t];  // in [0, n * NN): k_contrib_local found every contribution's cell
+      loc[t] = s_rank[v / NN] * NN + v % NN;
+    }
+    for (int t = t0 + 1; t < t1; ++t) {  // insertion sort of the slot's few entries
+      const int v = loc[t];
+      int u = t - 1;
+      while (u >= t0 && loc[u] > v) {
+        loc[u + 1] = loc[u];
+        --u;
+      }
+      loc[u + 1] = v;
+    }
+  }
+}
+
 // ------------------------------------------------------------- assembly
 template <class MAT>
 struct AsmCfg {
@@ -209,6 +289,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
 
   const int64_t s = blockIdx.x;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const double iso_c1 = IsoLinear<MAT>::value ? mat.c[1] : 0.0;
   load_tables(mesh, s_dN, s_w);
   const int sbase = pat.slice_ptr[s];
   const int L = (pat.slice_ptr[s + 1] - sbase) >> 6;
@@ -315,8 +396,13 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
           }
         }
       }
-      s_jxw[cc][q] = jxw;
       const auto st = MAT::template prep<double>(mat, c * Q + q, G);
+      // elastic laws: lambda and mu are c0 E and c1 E at every point, so the tangent needs the
+      // single weight JxW E per point (E = mu / c1 when c1 > 0)
+      double wq = jxw;
+      if constexpr (IsoLinear<MAT>::value)
+        if (iso_c1 > 0.0) wq = jxw * (double(st.mu) / iso_c1);
+      s_jxw[cc][q] = wq;
       s_st[cc].s[q] = st;
       if (RES) {
         double P[V][3];
@@ -348,35 +434,32 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
         const int cc = e / (N * N) - ch0;
         const int a = (e >> 3) & 7, b = e & 7;
         double K[V][V];
-        if constexpr (IsoLinear<MAT>::value) {
-          double A[3][3], B[3][3];
+        if (IsoLinear<MAT>::value && iso_c1 > 0.0) {
+          // K = c0 M + c1 M^T + c1 tr(M) I with M = sum_q JxW E g_a (x) g_b
+          double M[3][3];
 #pragma unroll
           for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) A[i][k] = B[i][k] = 0.0;
-#pragma unroll 2
+            for (int k = 0; k < 3; ++k) M[i][k] = 0.0;
+#pragma unroll 4
           for (int q = 0; q < Q; ++q) {
             const double* ga = &s_g[cc][(q * N + a) * 3];
             const double* gb = &s_g[cc][(q * N + b) * 3];
-            const double jw = s_jxw[cc][q];
-            const double wl = jw * s_st[cc].s[q].lam, wm = jw * s_st[cc].s[q].mu;
+            const double wq = s_jxw[cc][q];
             const double gb0 = gb[0], gb1 = gb[1], gb2 = gb[2];
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-              const double la = wl * ga[i], ma = wm * ga[i];
-              A[i][0] += la * gb0;
-              A[i][1] += la * gb1;
-              A[i][2] += la * gb2;
-              B[i][0] += ma * gb0;
-              B[i][1] += ma * gb1;
-              B[i][2] += ma * gb2;
+              const double wa = wq * ga[i];
+              M[i][0] += wa * gb0;
+              M[i][1] += wa * gb1;
+              M[i][2] += wa * gb2;
             }
           }
-          const double trB = B[0][0] + B[1][1] + B[2][2];
+          const double c0 = mat.c[0], tr = iso_c1 * (M[0][0] + M[1][1] + M[2][2]);
 #pragma unroll
           for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) K[i][k] = A[i][k] + B[k][i] + (i == k ? trB : 0.0);
+            for (int k = 0; k < 3; ++k) K[i][k] = c0 * M[i][k] + iso_c1 * M[k][i] + (i == k ? tr : 0.0);
         } else {
 #pragma unroll
           for (int i = 0; i < V; ++i)
@@ -526,6 +609,9 @@ int fem_assembly_plan_fill(const fem_pattern_t* pat, const int32_t* inc_ptr, con
   const int64_t lanes = pat->num_slices * kSlice;
   hipLaunchKernelGGL(k_contrib_local, dim3((unsigned)ceil_div(lanes, 256)), dim3(256), 0, st, *pat, slice_cell_ptr,
                      slice_cells, contrib_loc, err_flag);
+  FEM_LAUNCHED();
+  hipLaunchKernelGGL(k_slice_reorder, dim3((unsigned)pat->num_slices), dim3(64), 0, st, *pat, inc_ptr, inc_idx,
+                     slice_cell_ptr, slice_cells, contrib_loc, err_flag);
   FEM_LAUNCHED();
   return FEM_OK;
 }
