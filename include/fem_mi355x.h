@@ -335,23 +335,29 @@ typedef struct {
   const int32_t* slice_cells;    /* unique incident cells of each slice, in chunk
                                     order: cells whose lowest slice lane is even
                                     first, each group by ascending id         */
+  const int32_t* slice_pair_ptr; /* (num_slices + 1)                           */
+  const int32_t* slice_pairs;    /* the slice's (cell, local node a, row lane)
+                                    pairs, sorted keys (rank * 8 + a) * 64 +
+                                    lane, rank = position in slice_cells      */
   const int32_t* contrib_loc;    /* (num_cells*N*N), aligned with contrib_idx:
-                                    position * N*N + a*N + b, ascending within
-                                    each slot                                  */
+                                    pair * N + b (pair = index in the slice's
+                                    pair list), ascending within each slot    */
   int32_t        max_slots;      /* largest slot count of a slice (<= 32)     */
-  int32_t        pad;
+  int32_t        max_pairs;      /* largest pair count of a slice (<= 1024)   */
 } fem_assembly_plan_t;
 
-/* Plan, pass 1: unique incident cells per slice (counts, to be scanned into
- * slice_cell_ptr).  err_flag > 0 after the call: a slice's rows have more than
- * 4096 incidences (use the two-pass path). */
+/* Plan, pass 1: unique incident cells and (cell, row node) pairs per slice
+ * (counts, to be scanned into slice_cell_ptr and slice_pair_ptr).  err_flag > 0
+ * after the call: a slice's rows have more than 4096 incidences (use the
+ * two-pass path). */
 int fem_assembly_plan_counts(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
-                             int32_t* slice_cell_counts, int32_t* err_flag, void* stream);
-/* Plan, pass 2: slice_cells and contrib_loc (err_flag as above, or 1 when a
- * contribution's cell is missing from its slice list). */
+                             int32_t* slice_cell_counts, int32_t* slice_pair_counts, int32_t* err_flag,
+                             void* stream);
+/* Plan, pass 2: slice_cells, slice_pairs and contrib_loc (err_flag as above, or
+ * 1 when a contribution's cell is missing from its slice list). */
 int fem_assembly_plan_fill(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
-                           const int32_t* slice_cell_ptr, int32_t* slice_cells, int32_t* contrib_loc,
-                           int32_t* err_flag, void* stream);
+                           const int32_t* slice_cell_ptr, const int32_t* slice_pair_ptr, int32_t* slice_cells,
+                           int32_t* slice_pairs, int32_t* contrib_loc, int32_t* err_flag, void* stream);
 /* Assemble the tangent of HEX8 cells with 8 quadrature points at u into the
  * SELL values: FP64 standard form (precision FEM_F64, form FEM_FORM_STANDARD)
  * or FP32 difference form (FEM_F32, FEM_FORM_DIFFERENCE); the cell tangents
@@ -442,6 +448,10 @@ typedef struct {
   int64_t        cols;           /* block columns                             */
   int64_t        nnzb;           /* stored blocks                             */
   int32_t        br, bc;         /* block shape (1, 3 or 6)                   */
+  int32_t        f32;            /* 0: vals are FP64, 1: FP32 (the setup of an
+                                    FP32 hierarchy; in a fem_amg_t the levels'
+                                    storage follows fem_amg_t.precision)     */
+  int32_t        pad;
   const int32_t* row_ptr;        /* (rows+1)                                  */
   const int32_t* col_idx;        /* (nnzb), ascending within a row            */
   const void*    vals;           /* (nnzb*br*bc), row-major blocks            */
@@ -513,9 +523,10 @@ int fem_amg_bc_mask(int64_t n, int64_t n_bc, const int32_t* bc_dofs, uint8_t* ma
 /* FP32 difference-form copy (off-diagonal blocks rounded, diagonal slot = the
  * FP64 row sum rounded once) of a standard-form FP64 SELL matrix. */
 int fem_amg_sell_to_f32_diff(const fem_pattern_t* pat, const double* sell_vals, float* out, void* stream);
-/* Block CSR copy (FP64) of a standard-form FP64 SELL matrix; the pattern's
- * row_ptr/col_idx are the CSR structure. */
-int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, double* bcsr_vals, void* stream);
+/* Block CSR copy (FEM_F64 or FEM_F32 values) of a standard-form FP64 SELL
+ * matrix; the pattern's row_ptr/col_idx are the CSR structure. */
+int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, void* bcsr_vals,
+                         int32_t precision, void* stream);
 /* dinv = 1 / diag(A) (FP64 A); on levels >= 1 (fix_zero = 1) a zero
  * diagonal entry (an empty coarse row) is first set to 1 in A. */
 int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream);

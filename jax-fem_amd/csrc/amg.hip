@@ -274,9 +274,9 @@ __global__ void k_bc_mask_set(int64_t n_bc, const int32_t* __restrict__ dofs, ui
 // One block per slice: KC slots of the slice's 64 rows are read coalesced into
 // LDS, then each row's KC*V*V entries (contiguous in the block CSR) are stored
 // as one contiguous run, so the stores fill whole cache lines.
-template <int V>
+template <int V, typename TO>
 __global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv,
-                                                      double* __restrict__ out) {
+                                                      TO* __restrict__ out) {
   constexpr int VV = V * V, KC = 4;
   __shared__ double tile[kSlice][KC * VV + 1];
   __shared__ int s_dst[kSlice], s_len[kSlice];
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const d
     __syncthreads();
     for (int f = threadIdx.x; f < nc * kSlice; f += 256) {
       const int lane = f / nc, c = f - lane * nc;
-      if (k0 + c / VV < s_len[lane]) out[((size_t)s_dst[lane] + k0) * VV + c] = tile[lane][c];
+      if (k0 + c / VV < s_len[lane]) out[((size_t)s_dst[lane] + k0) * VV + c] = TO(tile[lane][c]);
     }
   }
 }
@@ -360,9 +360,9 @@ __global__ void k_sell_to_f16_scaled(fem_pattern_t pat, const double* __restrict
   k_sell_to_diff_body<V, _Float16>(pat, sv, dinv, out);
 }
 
-template <int BS>
+template <int BS, typename T>
 __global__ void k_bcsr_dinv(int64_t rows, const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                            double* __restrict__ v, int fix, double* __restrict__ dinv) {
+                            T* __restrict__ v, int fix, double* __restrict__ dinv) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x) {
     int k = -1;
     for (int e = rp[i]; e < rp[i + 1]; ++e)
@@ -377,9 +377,9 @@ __global__ void k_bcsr_dinv(int64_t rows, const int32_t* __restrict__ rp, const 
     }
 #pragma unroll
     for (int c = 0; c < BS; ++c) {
-      double& d = v[((size_t)k * BS + c) * BS + c];
-      if (fix && d == 0.0) d = 1.0;
-      dinv[i * BS + c] = d != 0.0 ? 1.0 / d : 0.0;
+      T& d = v[((size_t)k * BS + c) * BS + c];
+      if (fix && d == T(0)) d = T(1);
+      dinv[i * BS + c] = d != T(0) ? 1.0 / double(d) : 0.0;
     }
   }
 }
@@ -402,12 +402,12 @@ __global__ void k_t_fill(int64_t rows, const int32_t* __restrict__ rp, const int
     }
 }
 
-template <int BR, int BC>
-__global__ void k_t_values(int64_t nnzb, const int32_t* __restrict__ src, const double* __restrict__ xv,
-                           double* __restrict__ tv) {
+template <int BR, int BC, typename T>
+__global__ void k_t_values(int64_t nnzb, const int32_t* __restrict__ src, const T* __restrict__ xv,
+                           T* __restrict__ tv) {
   for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < nnzb; q += (int64_t)gridDim.x * blockDim.x) {
-    const double* s = xv + (size_t)src[q] * BR * BC;
-    double* d = tv + (size_t)q * BR * BC;
+    const T* s = xv + (size_t)src[q] * BR * BC;
+    T* d = tv + (size_t)q * BR * BC;
 #pragma unroll
     for (int i = 0; i < BR; ++i)
 #pragma unroll
@@ -758,15 +758,36 @@ __global__ __launch_bounds__(NT) void k_spgemm_row_big(BView X, BView Y, int32_t
   }
 }
 
-// Load an R x C FP64 block; blocks whose size is a multiple of 16 bytes (all
-// but 3x3 and 1x1, given 256-byte aligned bases) use 16-byte loads.
-template <int R, int C>
-__device__ inline void load_block(const double* __restrict__ p, double (&a)[R][C]) {
-  if constexpr ((R * C) % 2 == 0 && (R * C * 8) % 16 == 0 && R * C > 9) {
+// Load an R x C block (FP64 or FP32 storage) into FP64 registers.  Blocks whose
+// size is a multiple of 16 bytes (given 256-byte aligned bases: FP64 blocks but
+// 3x3 and 1x1, FP32 6x6 blocks) use 16-byte loads; FP32 blocks of an even
+// count (3x6, 6x3) 8-byte loads.
+template <int R, int C, typename T = double>
+__device__ inline void load_block(const T* __restrict__ p, double (&a)[R][C]) {
+  constexpr int n = R * C;
+  if constexpr (sizeof(T) == 8 && n % 2 == 0 && n > 9) {
     const double2* p2 = reinterpret_cast<const double2*>(p);
 #pragma unroll
-    for (int k = 0; k < R * C / 2; ++k) {
+    for (int k = 0; k < n / 2; ++k) {
       const double2 v = p2[k];
+      a[(2 * k) / C][(2 * k) % C] = v.x;
+      a[(2 * k + 1) / C][(2 * k + 1) % C] = v.y;
+    }
+  } else if constexpr (sizeof(T) == 4 && n % 4 == 0 && (n * 4) % 16 == 0 && n > 9) {
+    const float4* p4 = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int k = 0; k < n / 4; ++k) {
+      const float4 v = p4[k];
+      a[(4 * k) / C][(4 * k) % C] = v.x;
+      a[(4 * k + 1) / C][(4 * k + 1) % C] = v.y;
+      a[(4 * k + 2) / C][(4 * k + 2) % C] = v.z;
+      a[(4 * k + 3) / C][(4 * k + 3) % C] = v.w;
+    }
+  } else if constexpr (sizeof(T) == 4 && n % 2 == 0 && n > 9) {
+    const float2* p2 = reinterpret_cast<const float2*>(p);
+#pragma unroll
+    for (int k = 0; k < n / 2; ++k) {
+      const float2 v = p2[k];
       a[(2 * k) / C][(2 * k) % C] = v.x;
       a[(2 * k + 1) / C][(2 * k + 1) % C] = v.y;
     }
@@ -774,15 +795,15 @@ __device__ inline void load_block(const double* __restrict__ p, double (&a)[R][C
 #pragma unroll
     for (int i = 0; i < R; ++i)
 #pragma unroll
-      for (int j = 0; j < C; ++j) a[i][j] = p[i * C + j];
+      for (int j = 0; j < C; ++j) a[i][j] = double(p[i * C + j]);
   }
 }
 
 // C = X Y by its plan: one thread per output block, products in plan order.
-template <int XR, int XC, int YC>
+template <int XR, int XC, int YC, typename T>
 __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_t* __restrict__ plan_ptr,
-                                                        const int32_t* __restrict__ plan, const double* __restrict__ xv,
-                                                        const double* __restrict__ yv, double* __restrict__ cv) {
+                                                        const int32_t* __restrict__ plan, const T* __restrict__ xv,
+                                                        const T* __restrict__ yv, T* __restrict__ cv) {
   for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
     double acc[XR][YC];
 #pragma unroll
@@ -795,11 +816,11 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_
 #pragma unroll kUnroll
     for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
       const int2 ef = reinterpret_cast<const int2*>(plan)[q];
-      const double* x = xv + (size_t)ef.x * XR * XC;
-      const double* y = yv + (size_t)ef.y * XC * YC;
+      const T* x = xv + (size_t)ef.x * XR * XC;
+      const T* y = yv + (size_t)ef.y * XC * YC;
       double xa[XR][XC], ya[XC][YC];
-      load_block<XR, XC>(x, xa);
-      load_block<XC, YC>(y, ya);
+      load_block<XR, XC, T>(x, xa);
+      load_block<XC, YC, T>(y, ya);
 #pragma unroll
       for (int i = 0; i < XR; ++i)
 #pragma unroll
@@ -807,11 +828,11 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_
 #pragma unroll
           for (int m = 0; m < XC; ++m) acc[i][j] = fma(xa[i][m], ya[m][j], acc[i][j]);
     }
-    double* c = cv + (size_t)b * XR * YC;
+    T* c = cv + (size_t)b * XR * YC;
 #pragma unroll
     for (int i = 0; i < XR; ++i)
 #pragma unroll
-      for (int j = 0; j < YC; ++j) c[i * YC + j] = acc[i][j];
+      for (int j = 0; j < YC; ++j) c[i * YC + j] = T(acc[i][j]);
   }
 }
 
@@ -825,11 +846,10 @@ constexpr int64_t kSplitBelow = 200000;
 // S > 1 the S groups of a wave split one block's plan (group g takes products
 // g, g + S, ...); group 0 sums their columns in group order.  The order is
 // fixed: bit-reproducible.
-template <int XR, int XC, int YC, int S>
+template <int XR, int XC, int YC, int S, typename T>
 __global__ __launch_bounds__(256) void k_spgemm_numeric_g(int64_t nb, const int32_t* __restrict__ plan_ptr,
-                                                          const int32_t* __restrict__ plan,
-                                                          const double* __restrict__ xv,
-                                                          const double* __restrict__ yv, double* __restrict__ cv) {
+                                                          const int32_t* __restrict__ plan, const T* __restrict__ xv,
+                                                          const T* __restrict__ yv, T* __restrict__ cv) {
   constexpr int G = YC, GPW = 64 / G;  // lanes per group, groups per wave (the last 64 - GPW*G lanes idle)
   static_assert(S == 1 || S == GPW, "a block is split over one group or over the whole wave");
   constexpr int BPW = GPW / S;  // output blocks per wave
@@ -847,10 +867,10 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric_g(int64_t nb, const int3
     for (int q = plan_ptr[b] + sg; q < q1; q += S) {
       const int2 ef = reinterpret_cast<const int2*>(plan)[q];
       double xa[XR][XC], yc[XC];
-      load_block<XR, XC>(xv + (size_t)ef.x * XR * XC, xa);
-      const double* y = yv + (size_t)ef.y * XC * YC + c;
+      load_block<XR, XC, T>(xv + (size_t)ef.x * XR * XC, xa);
+      const T* y = yv + (size_t)ef.y * XC * YC + c;
 #pragma unroll
-      for (int m = 0; m < XC; ++m) yc[m] = y[m * YC];
+      for (int m = 0; m < XC; ++m) yc[m] = double(y[m * YC]);
 #pragma unroll
       for (int i = 0; i < XR; ++i)
 #pragma unroll
@@ -867,22 +887,22 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric_g(int64_t nb, const int3
     }
   }
   if (grp < GPW && sg == 0 && b < nb) {
-    double* out = cv + (size_t)b * XR * YC + c;
+    T* out = cv + (size_t)b * XR * YC + c;
 #pragma unroll
-    for (int i = 0; i < XR; ++i) out[i * YC] = acc[i];
+    for (int i = 0; i < XR; ++i) out[i * YC] = T(acc[i]);
   }
 }
 
 // P = P0 - omega D^-1 (A P0) along the plan of A P0 (Y = P0 as block CSR, one
 // block per row at column agg): one thread per block of P.
-template <int BS, int NS>
+template <int BS, int NS, typename T>
 __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows, const int32_t* __restrict__ crp,
                                                         const int32_t* __restrict__ cci,
                                                         const int32_t* __restrict__ plan_ptr,
-                                                        const int32_t* __restrict__ plan, const double* __restrict__ av,
+                                                        const int32_t* __restrict__ plan, const T* __restrict__ av,
                                                         const double* __restrict__ p0, const int32_t* __restrict__ agg,
                                                         const double* __restrict__ dinv, double omega,
-                                                        double* __restrict__ pv) {
+                                                        T* __restrict__ pv) {
   const int32_t plan_len = plan_ptr[nb];
   for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
     int64_t lo = 0, hi = rows;  // row i: crp[i] <= b < crp[i+1]
@@ -903,7 +923,7 @@ __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows
     ListWindow<4> pw(plan, 2 * (int64_t)plan_len);
     for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
       double xa[BS][BS], ya[BS][NS];
-      load_block<BS, BS>(av + (size_t)pw[2 * (int64_t)q] * BS * BS, xa);
+      load_block<BS, BS, T>(av + (size_t)pw[2 * (int64_t)q] * BS * BS, xa);
       load_block<BS, NS>(p0 + (size_t)pw[2 * (int64_t)q + 1] * BS * NS, ya);
 #pragma unroll
       for (int r = 0; r < BS; ++r)
@@ -913,7 +933,7 @@ __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows
           for (int j = 0; j < NS; ++j) acc[r][j] = fma(xa[r][m], ya[m][j], acc[r][j]);
     }
     const bool own = cci[b] == agg[i];
-    double* out = pv + (size_t)b * BS * NS;
+    T* out = pv + (size_t)b * BS * NS;
 #pragma unroll
     for (int r = 0; r < BS; ++r) {
       const double sc = -omega * dinv[i * BS + r];
@@ -921,7 +941,7 @@ __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows
       for (int j = 0; j < NS; ++j) {
         double v = sc * acc[r][j];
         if (own) v += p0[(i * BS + r) * NS + j];
-        out[r * NS + j] = v;
+        out[r * NS + j] = T(v);
       }
     }
   }
@@ -984,12 +1004,14 @@ __global__ void k_dense_zero(int64_t nn, double* __restrict__ M) {
     M[i] = 0.0;
 }
 
+template <typename T>
 __global__ void k_dense_scatter(int64_t rows, int br, int bc, int64_t n, const int32_t* __restrict__ rp,
-                                const int32_t* __restrict__ ci, const double* __restrict__ v, double* __restrict__ M) {
+                                const int32_t* __restrict__ ci, const T* __restrict__ v, double* __restrict__ M) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x)
     for (int k = rp[i]; k < rp[i + 1]; ++k)
       for (int a = 0; a < br; ++a)
-        for (int b = 0; b < bc; ++b) M[(i * br + a) * n + (int64_t)ci[k] * bc + b] = v[((size_t)k * br + a) * bc + b];
+        for (int b = 0; b < bc; ++b)
+          M[(i * br + a) * n + (int64_t)ci[k] * bc + b] = double(v[((size_t)k * br + a) * bc + b]);
 }
 
 // one Gauss-Jordan pivot step src -> dst (in-place inversion formula)
@@ -1022,7 +1044,9 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
 
 // y = M x, one wave per row
 __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __restrict__ M,
-                                                    const double* __restrict__ x, double* __restrict__ y) {
+                                                    const double* __restrict__ x, double* __restrict__ y,
+                                                    const int32_t* __restrict__ done) {
+  if (done && *done) return;
   const int lane = threadIdx.x & 63;
   const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   if (i >= n) return;
@@ -1047,6 +1071,8 @@ struct ApplyArgsT {
   TX* d;               // CHEB: search direction (in/out)
   double c1, c2;       // CHEB: d = c1 d + c2 D^-1 (rhs - A x), y = x + d
   double* part;        // PQ: per-block partials of <x, A x>
+  const int32_t* done; // NULL, or a flag: the kernel returns at once when it is set
+                       // (V-cycles queued after the outer CG converged)
 };
 using ApplyArgs = ApplyArgsT<double, double>;
 
@@ -1074,6 +1100,7 @@ template <int BR, int BC, int G, int MODE, typename T, typename TX = double, typ
 __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t* __restrict__ rp,
                                                     const int32_t* __restrict__ ci, const T* __restrict__ v,
                                                     ApplyArgsT<TX, TY> a) {
+  if (a.done && *a.done) return;
   const int64_t gid = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t row = gid / G;
   const int sub = (int)(gid % G);
@@ -1101,6 +1128,40 @@ __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t*
   }
 }
 
+// Square coarse operators (6x6 blocks, tens of blocks per row): one wave per
+// block row, each lane a whole block (read with 16-byte loads from the row's
+// contiguous block stream) times its 6-entry slice of x; the lanes' partial
+// rows meet in a wave reduction.
+template <int B, int MODE, typename T, typename TX = double, typename TY = double>
+__global__ __launch_bounds__(256) void k_bcsr_apply_blk(int64_t rows, const int32_t* __restrict__ rp,
+                                                        const int32_t* __restrict__ ci, const T* __restrict__ v,
+                                                        ApplyArgsT<TX, TY> a) {
+  if (a.done && *a.done) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= rows) return;
+  double acc[B];
+#pragma unroll
+  for (int i = 0; i < B; ++i) acc[i] = 0.0;
+  const int k1 = rp[row + 1];
+  for (int k = rp[row] + lane; k < k1; k += 64) {
+    double blk[B][B], xc[B];
+    load_block<B, B, T>(v + (size_t)k * B * B, blk);
+    const TX* xp = a.x + (int64_t)ci[k] * B;
+#pragma unroll
+    for (int j = 0; j < B; ++j) xc[j] = double(xp[j]);
+#pragma unroll
+    for (int i = 0; i < B; ++i)
+#pragma unroll
+      for (int j = 0; j < B; ++j) acc[i] = fma(blk[i][j], xc[j], acc[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < B; ++i) acc[i] = wave_sum(acc[i]);
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < B; ++i) apply_epilogue<MODE>(a, row * B + i, acc[i]);
+}
+
 // Long block rows (R and the coarse operators, hundreds of (block, column) pairs):
 // one 256-thread workgroup per block row, partial sums met in LDS.
 template <int BR, int BC, int MODE, typename T, typename TX = double, typename TY = double>
@@ -1108,6 +1169,7 @@ __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int3
                                                         const int32_t* __restrict__ ci, const T* __restrict__ v,
                                                         ApplyArgsT<TX, TY> a) {
   __shared__ double smem[4];
+  if (a.done && *a.done) return;
   const int64_t row = blockIdx.x;
   double acc[BR];
 #pragma unroll
@@ -1141,7 +1203,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   constexpr int W = SliceCfg<T>::W;
   constexpr int U = (sizeof(T) == 4 && MODE != AP_CHEB) ? 4 : SliceCfg<T>::U;
   __shared__ double red[W][kSlice][V];
-  if (done && *done) return;
+  if ((done && *done) || (a.done && *a.done)) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   // XCD-contiguous slice order: neighbouring slices (which gather the same
   // vector entries) run on one XCD and share its L2
@@ -1156,7 +1218,9 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
 #pragma unroll
   for (int j = 0; j < V; ++j) xr[j] = (DIFF && real) ? double(a.x[row * V + j]) : 0.0;
   // the epilogue's own-row operands are loaded before the matrix stream, so
-  // their latency overlaps it (wave 0 finishes the rows)
+  // their latency overlaps it (wave 0 finishes the rows; spreading the rows'
+  // components over the waves measured no faster: 111.5 -> 113.2 us for the
+  // 1.6M-dof cantilever's FP32 Chebyshev step)
   constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
   // FP16 values hold D^-1 A (rows scaled by dinv): the epilogues undo or skip the scaling
   constexpr bool kScaled = sizeof(T) == 2;
@@ -1211,7 +1275,8 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
 // first Chebyshev step from a zero iterate: d = c2 D^-1 rhs, x = d
 template <typename TX>
 __global__ void k_cheb_first(int64_t n, const double* __restrict__ rhs, const double* __restrict__ dinv, double c2,
-                             TX* __restrict__ d, TX* __restrict__ x) {
+                             TX* __restrict__ d, TX* __restrict__ x, const int32_t* __restrict__ done) {
+  if (done && *done) return;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const TX v = TX(c2 * dinv[i] * rhs[i]);
     d[i] = v;
@@ -1377,6 +1442,12 @@ int bcsr_apply(const fem_bcsr_t& M, const ApplyArgsT<TX, TY>& a, hipStream_t s) 
   const int G = per_row > kRowThreshold ? 256 : (per_row > 48.0 ? 64 : 8);
   const unsigned grid = (unsigned)(G == 256 ? M.rows : ceil_div(M.rows * G, 256));
   const T* v = static_cast<const T*>(M.vals);
+  if (M.br == 6 && M.bc == 6 && G == 64) {   // square coarse operators: a lane per block
+    hipLaunchKernelGGL((k_bcsr_apply_blk<6, MODE, T, TX, TY>), dim3((unsigned)ceil_div(M.rows, 4)), dim3(256), 0, s,
+                       M.rows, M.row_ptr, M.col_idx, v, a);
+    FEM_LAUNCHED();
+    return FEM_OK;
+  }
 #define FEM_AMG_APPLY(BR, BC)                                                                                  \
   if (M.br == BR && M.bc == BC) {                                                                              \
     if (G == 256)                                                                                              \
@@ -1444,15 +1515,15 @@ int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgsT<TX, TY>&
     if (amg.level0_form == FEM_FORM_SCALED_DIFF16) {
       ApplyArgsT<TX, TY> b = a;
       b.dinv = L.dinv;   // the residual undoes the row scaling
-      if (mode == AP_CHEB) return sell_apply<AP_CHEB, _Float16, true>(*L.pat, L.sell_vals, b, nullptr, s);
-      return sell_apply<AP_RESID, _Float16, true>(*L.pat, L.sell_vals, b, nullptr, s);
+      if (mode == AP_CHEB) return sell_apply<AP_CHEB, _Float16, true>(*L.pat, L.sell_vals, b, a.done, s);
+      return sell_apply<AP_RESID, _Float16, true>(*L.pat, L.sell_vals, b, a.done, s);
     }
     if (amg.level0_form == FEM_FORM_DIFFERENCE) {
-      if (mode == AP_CHEB) return sell_apply<AP_CHEB, T, true>(*L.pat, L.sell_vals, a, nullptr, s);
-      return sell_apply<AP_RESID, T, true>(*L.pat, L.sell_vals, a, nullptr, s);
+      if (mode == AP_CHEB) return sell_apply<AP_CHEB, T, true>(*L.pat, L.sell_vals, a, a.done, s);
+      return sell_apply<AP_RESID, T, true>(*L.pat, L.sell_vals, a, a.done, s);
     }
-    if (mode == AP_CHEB) return sell_apply<AP_CHEB, T>(*L.pat, L.sell_vals, a, nullptr, s);
-    return sell_apply<AP_RESID, T>(*L.pat, L.sell_vals, a, nullptr, s);
+    if (mode == AP_CHEB) return sell_apply<AP_CHEB, T>(*L.pat, L.sell_vals, a, a.done, s);
+    return sell_apply<AP_RESID, T>(*L.pat, L.sell_vals, a, a.done, s);
   }
   if (mode == AP_CHEB) return bcsr_apply<AP_CHEB, T>(L.A, a, s);
   return bcsr_apply<AP_RESID, T>(L.A, a, s);
@@ -1463,23 +1534,23 @@ int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgsT<TX, TY>&
 // *oo the other iterate buffer.
 template <typename T, typename TX>
 int level_down(const fem_amg_t& amg, int l, int64_t n, const double* rhs, double* rhs_next, TX* x, TX* other,
-               TX* dd, TX* res, TX** xo, TX** oo, hipStream_t s) {
+               TX* dd, TX* res, TX** xo, TX** oo, hipStream_t s, const int32_t* done) {
   const fem_amg_level_t& L = amg.levels[l];
   const Cheb cb(L.lmax);
   hipLaunchKernelGGL(k_cheb_first<TX>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, rhs, L.dinv, 1.0 / cb.theta, dd,
-                     x);
+                     x, done);
   FEM_LAUNCHED();
   double rho = 1.0 / cb.sigma;
   for (int k = 1; k < amg.degree; ++k) {
     const double rn = 1.0 / (2.0 * cb.sigma - rho);
-    ApplyArgsT<TX, TX> a{x, other, rhs, L.dinv, dd, rn * rho, 2.0 * rn / cb.delta, nullptr};
+    ApplyArgsT<TX, TX> a{x, other, rhs, L.dinv, dd, rn * rho, 2.0 * rn / cb.delta, nullptr, done};
     if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
     rho = rn;
     std::swap(x, other);
   }
-  ApplyArgsT<TX, TX> ar{x, res, rhs, nullptr, nullptr, 0.0, 0.0, nullptr};
+  ApplyArgsT<TX, TX> ar{x, res, rhs, nullptr, nullptr, 0.0, 0.0, nullptr, done};
   if (int rc = level_apply<T>(amg, l, AP_RESID, ar, s)) return rc;
-  ApplyArgsT<TX, double> arr{res, rhs_next, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
+  ApplyArgsT<TX, double> arr{res, rhs_next, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr, done};
   if (int rc = bcsr_apply<AP_STORE, T>(L.R, arr, s)) return rc;
   *xo = x;
   *oo = other;
@@ -1491,10 +1562,10 @@ int level_down(const fem_amg_t& amg, int l, int64_t n, const double* rhs, double
 // vcycle_impl.
 template <typename T, typename TX>
 int level_up(const fem_amg_t& amg, int l, const double* rhs, const double* xc, TX* x, TX* other, TX* dd, double* z,
-             TX** xo, hipStream_t s, hipEvent_t* ev) {
+             TX** xo, hipStream_t s, hipEvent_t* ev, const int32_t* done) {
   const fem_amg_level_t& L = amg.levels[l];
   const Cheb cb(L.lmax);
-  ApplyArgsT<double, TX> ap{xc, x, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr};
+  ApplyArgsT<double, TX> ap{xc, x, nullptr, nullptr, nullptr, 0.0, 0.0, nullptr, done};
   if (int rc = bcsr_apply<AP_ADD, T>(L.P, ap, s)) return rc;
   double rho = 1.0 / cb.sigma;
   for (int k = 0; k < amg.degree; ++k) {
@@ -1508,10 +1579,10 @@ int level_up(const fem_amg_t& amg, int l, const double* rhs, const double* xc, T
     const bool timed = ev && l == 0 && k == 0;
     if (timed) FEM_HIP(hipEventRecord(ev[0], s));
     if (z && k == amg.degree - 1) {
-      ApplyArgsT<TX, double> a{x, z, rhs, L.dinv, dd, c1, c2, nullptr};
+      ApplyArgsT<TX, double> a{x, z, rhs, L.dinv, dd, c1, c2, nullptr, done};
       if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
     } else {
-      ApplyArgsT<TX, TX> a{x, other, rhs, L.dinv, dd, c1, c2, nullptr};
+      ApplyArgsT<TX, TX> a{x, other, rhs, L.dinv, dd, c1, c2, nullptr, done};
       if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
       std::swap(x, other);
     }
@@ -1529,7 +1600,7 @@ int level_up(const fem_amg_t& amg, int l, const double* rhs, const double* xc, T
 // FP64 throughout.
 template <typename T>
 int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipStream_t s,
-                hipEvent_t* ev = nullptr) {
+                hipEvent_t* ev = nullptr, const int32_t* done = nullptr) {
   if (ev) FEM_HIP(hipEventRecord(ev[2], s));
   const int NL = amg.num_levels;
   std::vector<double*> rhs(NL + 1), xa(NL), xb(NL), dd(NL), res(NL), cur(NL + 1);
@@ -1558,11 +1629,12 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
     if (l == 0) {
       if (int rc = level_down<T, T>(amg, 0, n, rhs[0], rhs[1], reinterpret_cast<T*>(xa[0]),
                                     reinterpret_cast<T*>(xb[0]), reinterpret_cast<T*>(dd[0]),
-                                    reinterpret_cast<T*>(res[0]), &x0, &o0, s))
+                                    reinterpret_cast<T*>(res[0]), &x0, &o0, s, done))
         return rc;
     } else {
       double *x = nullptr, *o = nullptr;
-      if (int rc = level_down<T, double>(amg, l, n, rhs[l], rhs[l + 1], xa[l], xb[l], dd[l], res[l], &x, &o, s))
+      if (int rc = level_down<T, double>(amg, l, n, rhs[l], rhs[l + 1], xa[l], xb[l], dd[l], res[l], &x, &o, s,
+                                         done))
         return rc;
       cur[l] = x;
       xb[l] = o;
@@ -1571,18 +1643,18 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
   // coarsest
   if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[0], s));
   hipLaunchKernelGGL(k_dense_gemv, dim3((unsigned)ceil_div(amg.coarse_n * 64, 256)), dim3(256), 0, s, amg.coarse_n,
-                     amg.coarse_inv, rhs[NL], cur[NL]);
+                     amg.coarse_inv, rhs[NL], cur[NL], done);
   FEM_LAUNCHED();
   if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[1], s));
   // up
   for (int l = NL - 1; l >= 0; --l) {
     if (l == 0) {
       T* x = nullptr;
-      if (int rc = level_up<T, T>(amg, 0, rhs[0], cur[1], x0, o0, reinterpret_cast<T*>(dd[0]), z, &x, s, ev))
+      if (int rc = level_up<T, T>(amg, 0, rhs[0], cur[1], x0, o0, reinterpret_cast<T*>(dd[0]), z, &x, s, ev, done))
         return rc;
     } else {
       double* x = nullptr;
-      if (int rc = level_up<T, double>(amg, l, rhs[l], cur[l + 1], cur[l], xb[l], dd[l], nullptr, &x, s, ev))
+      if (int rc = level_up<T, double>(amg, l, rhs[l], cur[l + 1], cur[l], xb[l], dd[l], nullptr, &x, s, ev, done))
         return rc;
       cur[l] = x;
     }
@@ -1625,6 +1697,14 @@ int64_t vcycle_ws(const fem_amg_t& amg) {
 }
 
 BView view(const fem_bcsr_t& M) { return BView{M.rows, M.row_ptr, M.col_idx}; }
+
+// numeric SpGEMM and prolongator smoothing on FP64 or FP32 block values
+template <typename T>
+int spgemm_numeric_t(const fem_bcsr_t& X, const fem_bcsr_t& Y, const fem_bcsr_t& C, const int32_t* plan_ptr,
+                     const int32_t* plan, hipStream_t s);
+template <typename T>
+int smooth_t(const fem_bcsr_t& A, const fem_bcsr_t& P0, const fem_bcsr_t& P, const int32_t* plan_ptr,
+             const int32_t* plan, const int32_t* agg, const double* dinv, double omega, hipStream_t s);
 
 template <bool FILL>
 int spgemm_dispatch(int64_t max_pairs, const fem_bcsr_t& X, const fem_bcsr_t& Y, int32_t* counts, const PlanOut& o,
@@ -1787,18 +1867,27 @@ int fem_amg_bc_mask(int64_t n, int64_t n_bc, const int32_t* bc_dofs, uint8_t* ma
   return FEM_OK;
 }
 
-int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, double* bcsr_vals, void* stream) {
+int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, void* bcsr_vals, int32_t precision,
+                         void* stream) {
   FEM_CHECK_ARG(pat && sell_vals && bcsr_vals, "fem_amg_sell_to_bcsr: NULL argument");
+  FEM_CHECK_ARG(precision == FEM_F32 || precision == FEM_F64, "fem_amg_sell_to_bcsr: bad precision");
   hipStream_t s = as_stream(stream);
   if (pat->num_rows == 0) return FEM_OK;
   FEM_CHECK_ARG(pat->num_slices == ceil_div(pat->num_rows, kSlice), "fem_amg_sell_to_bcsr: num_slices mismatch");
+  FEM_CHECK_ARG(pat->vec == 1 || pat->vec == 3, "fem_amg_sell_to_bcsr: vec must be 1 or 3");
   const dim3 g((unsigned)pat->num_slices);
-  if (pat->vec == 3)
-    hipLaunchKernelGGL(k_sell_to_bcsr<3>, g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
-  else if (pat->vec == 1)
-    hipLaunchKernelGGL(k_sell_to_bcsr<1>, g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals);
+  auto go = [&](auto t) {
+    using TO = decltype(t);
+    TO* o = static_cast<TO*>(bcsr_vals);
+    if (pat->vec == 3)
+      hipLaunchKernelGGL((k_sell_to_bcsr<3, TO>), g, dim3(256), 0, s, *pat, sell_vals, o);
+    else
+      hipLaunchKernelGGL((k_sell_to_bcsr<1, TO>), g, dim3(256), 0, s, *pat, sell_vals, o);
+  };
+  if (precision == FEM_F32)
+    go(float{});
   else
-    FEM_CHECK_ARG(false, "fem_amg_sell_to_bcsr: vec must be 1 or 3");
+    go(double{});
   FEM_LAUNCHED();
   return FEM_OK;
 }
@@ -1841,23 +1930,24 @@ int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stre
   hipStream_t s = as_stream(stream);
   if (A->rows == 0) return FEM_OK;
   const unsigned g = grid_for(A->rows, 256);
-  double* v = const_cast<double*>(static_cast<const double*>(A->vals));
-  switch (A->br) {
-    case 1:
-      hipLaunchKernelGGL(k_bcsr_dinv<1>, dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v, fix_zero,
-                         dinv);
-      break;
-    case 3:
-      hipLaunchKernelGGL(k_bcsr_dinv<3>, dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v, fix_zero,
-                         dinv);
-      break;
-    case 6:
-      hipLaunchKernelGGL(k_bcsr_dinv<6>, dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v, fix_zero,
-                         dinv);
-      break;
-    default:
-      FEM_CHECK_ARG(false, "fem_amg_dinv: unsupported block size %d", A->br);
-  }
+  FEM_CHECK_ARG(A->br == 1 || A->br == 3 || A->br == 6, "fem_amg_dinv: unsupported block size %d", A->br);
+  auto go = [&](auto t) {
+    using T = decltype(t);
+    T* v = const_cast<T*>(static_cast<const T*>(A->vals));
+    if (A->br == 1)
+      hipLaunchKernelGGL((k_bcsr_dinv<1, T>), dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v,
+                         fix_zero, dinv);
+    else if (A->br == 3)
+      hipLaunchKernelGGL((k_bcsr_dinv<3, T>), dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v,
+                         fix_zero, dinv);
+    else
+      hipLaunchKernelGGL((k_bcsr_dinv<6, T>), dim3(g), dim3(256), 0, s, A->rows, A->row_ptr, A->col_idx, v,
+                         fix_zero, dinv);
+  };
+  if (A->f32)
+    go(float{});
+  else
+    go(double{});
   FEM_LAUNCHED();
   return FEM_OK;
 }
@@ -1901,7 +1991,7 @@ int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell
     const bool last = k == steps - 1;
     if (last) norm2(x, nrm + 1);
     ApplyArgs a{x, y, nullptr, dinv, nullptr, 0.0, 0.0, nullptr};
-    if (int rc = !pat ? bcsr_apply<AP_DSTORE, double>(*A, a, s)
+    if (int rc = !pat ? (A->f32 ? bcsr_apply<AP_DSTORE, float>(*A, a, s) : bcsr_apply<AP_DSTORE, double>(*A, a, s))
                  : sell_f32 == 2 ? sell_apply<AP_DSTORE, _Float16, true>(*pat, sell_vals, a, nullptr, s)
                  : sell_f32 ? sell_apply<AP_DSTORE, float, true>(*pat, static_cast<const float*>(sell_vals), a,
                                                                   nullptr, s)
@@ -1984,9 +2074,37 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
   hipStream_t s = as_stream(stream);
   if (C->nnzb == 0) return FEM_OK;
   FEM_CHECK_ARG(plan && X->vals && Y->vals && C->vals, "fem_bcsr_spgemm_numeric: values missing");
-  const double* xv = static_cast<const double*>(X->vals);
-  const double* yv = static_cast<const double*>(Y->vals);
-  double* cv = static_cast<double*>(const_cast<void*>(C->vals));
+  FEM_CHECK_ARG(X->f32 == Y->f32 && X->f32 == C->f32, "fem_bcsr_spgemm_numeric: operands of different precisions");
+  if (X->f32) return spgemm_numeric_t<float>(*X, *Y, *C, plan_ptr, plan, s);
+  return spgemm_numeric_t<double>(*X, *Y, *C, plan_ptr, plan, s);
+}
+
+int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P, const int32_t* plan_ptr,
+                               const int32_t* plan, const int32_t* agg, const double* dinv, double omega,
+                               void* stream) {
+  FEM_CHECK_ARG(A && P0 && P && plan_ptr && agg && dinv, "fem_amg_smooth_prolongator: NULL argument");
+  FEM_CHECK_ARG(A->br == A->bc && P0->br == A->bc && P->rows == A->rows && P->br == P0->br && P->bc == P0->bc &&
+                    P0->nnzb == P0->rows,
+                "fem_amg_smooth_prolongator: shapes do not match");
+  FEM_CHECK_ARG(!P0->f32 && A->f32 == P->f32, "fem_amg_smooth_prolongator: P0 is FP64, A and P of one precision");
+  hipStream_t s = as_stream(stream);
+  if (P->nnzb == 0) return FEM_OK;
+  FEM_CHECK_ARG(plan && A->vals && P0->vals && P->vals && P->col_idx, "fem_amg_smooth_prolongator: arrays missing");
+  if (A->f32) return smooth_t<float>(*A, *P0, *P, plan_ptr, plan, agg, dinv, omega, s);
+  return smooth_t<double>(*A, *P0, *P, plan_ptr, plan, agg, dinv, omega, s);
+}
+
+}  // extern "C"
+
+namespace fem {
+namespace {
+template <typename T>
+int spgemm_numeric_t(const fem_bcsr_t& Xd, const fem_bcsr_t& Yd, const fem_bcsr_t& Cd, const int32_t* plan_ptr,
+                     const int32_t* plan, hipStream_t s) {
+  const fem_bcsr_t *X = &Xd, *Y = &Yd, *C = &Cd;
+  const T* xv = static_cast<const T*>(X->vals);
+  const T* yv = static_cast<const T*>(Y->vals);
+  T* cv = static_cast<T*>(const_cast<void*>(C->vals));
   // one thread per output block; products with few output blocks (the coarse levels) split
   // each block's plan over a whole wave (YC lanes per column group).  On level 0 of the
   // 1.6M-dof cantilever the grouped form with one block per group is slower (A P: 3.50 ->
@@ -1998,10 +2116,10 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
     const int64_t waves = split ? C->nnzb : ceil_div(C->nnzb, GPW);                                           \
     const dim3 grid((unsigned)ceil_div(waves, 4));                                                            \
     if (split)                                                                                                \
-      hipLaunchKernelGGL((k_spgemm_numeric_g<XR, XC, YC, GPW>), grid, dim3(256), 0, s, C->nnzb, plan_ptr, plan, \
-                         xv, yv, cv);                                                                         \
+      hipLaunchKernelGGL((k_spgemm_numeric_g<XR, XC, YC, GPW, T>), grid, dim3(256), 0, s, C->nnzb, plan_ptr,   \
+                         plan, xv, yv, cv);                                                                   \
     else                                                                                                      \
-      hipLaunchKernelGGL((k_spgemm_numeric<XR, XC, YC>), dim3(grid_for(C->nnzb, 256)), dim3(256), 0, s,       \
+      hipLaunchKernelGGL((k_spgemm_numeric<XR, XC, YC, T>), dim3(grid_for(C->nnzb, 256)), dim3(256), 0, s,    \
                          C->nnzb, plan_ptr, plan, xv, yv, cv);                                                \
     FEM_LAUNCHED();                                                                                           \
     return FEM_OK;                                                                                            \
@@ -2015,23 +2133,17 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
   return FEM_ERR_BAD_ARG;
 }
 
-int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P, const int32_t* plan_ptr,
-                               const int32_t* plan, const int32_t* agg, const double* dinv, double omega,
-                               void* stream) {
-  FEM_CHECK_ARG(A && P0 && P && plan_ptr && agg && dinv, "fem_amg_smooth_prolongator: NULL argument");
-  FEM_CHECK_ARG(A->br == A->bc && P0->br == A->bc && P->rows == A->rows && P->br == P0->br && P->bc == P0->bc &&
-                    P0->nnzb == P0->rows,
-                "fem_amg_smooth_prolongator: shapes do not match");
-  hipStream_t s = as_stream(stream);
-  if (P->nnzb == 0) return FEM_OK;
-  FEM_CHECK_ARG(plan && A->vals && P0->vals && P->vals && P->col_idx, "fem_amg_smooth_prolongator: arrays missing");
-  const double* av = static_cast<const double*>(A->vals);
+template <typename T>
+int smooth_t(const fem_bcsr_t& Ad, const fem_bcsr_t& P0d, const fem_bcsr_t& Pd, const int32_t* plan_ptr,
+             const int32_t* plan, const int32_t* agg, const double* dinv, double omega, hipStream_t s) {
+  const fem_bcsr_t *A = &Ad, *P0 = &P0d, *P = &Pd;
+  const T* av = static_cast<const T*>(A->vals);
   const double* p0 = static_cast<const double*>(P0->vals);
-  double* pv = static_cast<double*>(const_cast<void*>(P->vals));
+  T* pv = static_cast<T*>(const_cast<void*>(P->vals));
   const unsigned g = grid_for(P->nnzb, 256);
 #define FEM_AMG_SMOOTH(BS, NS)                                                                                  \
   if (P->br == BS && P->bc == NS) {                                                                             \
-    hipLaunchKernelGGL((k_smooth_numeric<BS, NS>), dim3(g), dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,         \
+    hipLaunchKernelGGL((k_smooth_numeric<BS, NS, T>), dim3(g), dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,      \
                        P->col_idx, plan_ptr, plan, av, p0, agg, dinv, omega, pv);                               \
     FEM_LAUNCHED();                                                                                             \
     return FEM_OK;                                                                                              \
@@ -2043,6 +2155,10 @@ int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bc
   set_error("fem_amg_smooth_prolongator: unsupported block shape %d x %d", P->br, P->bc);
   return FEM_ERR_BAD_ARG;
 }
+}  // namespace
+}  // namespace fem
+
+extern "C" {
 
 int fem_bcsr_transpose_structure(const fem_bcsr_t* X, fem_bcsr_t* T, int32_t* src, void* ws, void* stream) {
   FEM_CHECK_ARG(X && T && ws && T->row_ptr, "fem_bcsr_transpose_structure: NULL argument");
@@ -2079,12 +2195,16 @@ int fem_bcsr_transpose_values(const fem_bcsr_t* X, fem_bcsr_t* T, const int32_t*
   FEM_CHECK_ARG(T->nnzb == X->nnzb && T->br == X->bc && T->bc == X->br, "fem_bcsr_transpose_values: shapes differ");
   hipStream_t s = as_stream(stream);
   if (X->nnzb == 0) return FEM_OK;
-  const double* xv = static_cast<const double*>(X->vals);
-  double* tv = static_cast<double*>(const_cast<void*>(T->vals));
+  FEM_CHECK_ARG(X->f32 == T->f32, "fem_bcsr_transpose_values: X and X^T of different precisions");
   const unsigned g = grid_for(X->nnzb, 256);
 #define FEM_AMG_T(BR, BC)                                                                                   \
   if (X->br == BR && X->bc == BC) {                                                                         \
-    hipLaunchKernelGGL((k_t_values<BR, BC>), dim3(g), dim3(256), 0, s, X->nnzb, src, xv, tv);               \
+    if (X->f32)                                                                                             \
+      hipLaunchKernelGGL((k_t_values<BR, BC, float>), dim3(g), dim3(256), 0, s, X->nnzb, src,                \
+                         static_cast<const float*>(X->vals), static_cast<float*>(const_cast<void*>(T->vals)));    \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_t_values<BR, BC, double>), dim3(g), dim3(256), 0, s, X->nnzb, src,               \
+                         static_cast<const double*>(X->vals), static_cast<double*>(const_cast<void*>(T->vals))); \
     FEM_LAUNCHED();                                                                                         \
     return FEM_OK;                                                                                          \
   }
@@ -2108,8 +2228,12 @@ int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* str
   double* b = static_cast<double*>(ws);
   const unsigned g = grid_for(n * n, 256);
   hipLaunchKernelGGL(k_dense_zero, dim3(g), dim3(256), 0, s, n * n, a);
-  hipLaunchKernelGGL(k_dense_scatter, dim3(grid_for(A->rows, 256)), dim3(256), 0, s, A->rows, A->br, A->bc, n,
-                     A->row_ptr, A->col_idx, static_cast<const double*>(A->vals), a);
+  if (A->f32)
+    hipLaunchKernelGGL(k_dense_scatter<float>, dim3(grid_for(A->rows, 256)), dim3(256), 0, s, A->rows, A->br, A->bc,
+                       n, A->row_ptr, A->col_idx, static_cast<const float*>(A->vals), a);
+  else
+    hipLaunchKernelGGL(k_dense_scatter<double>, dim3(grid_for(A->rows, 256)), dim3(256), 0, s, A->rows, A->br,
+                       A->bc, n, A->row_ptr, A->col_idx, static_cast<const double*>(A->vals), a);
   FEM_LAUNCHED();
   for (int64_t k = 0; k < n; ++k) {
     hipLaunchKernelGGL(k_gj_step, dim3(g), dim3(256), 0, s, n, k, a, b);
@@ -2220,8 +2344,8 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const float*
   int replacements = 0;
   while (!h.done) {
     hipEvent_t* e = opt->timing ? ev : nullptr;
-    if (int rc = (amg->precision == FEM_F32 ? vcycle_impl<float>(*amg, r, z, cyc, s, e)
-                                            : vcycle_impl<double>(*amg, r, z, cyc, s, e)))
+    if (int rc = (amg->precision == FEM_F32 ? vcycle_impl<float>(*amg, r, z, cyc, s, e, done)
+                                            : vcycle_impl<double>(*amg, r, z, cyc, s, e, done)))
       return rc;
     hipLaunchKernelGGL(k_dot_part, dim3(kDotBlocks), dim3(256), 0, s, n, r, z, part, done);
     hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, part, kDotBlocks, sc);

@@ -9,21 +9,17 @@
 //
 // Work split.  A workgroup owns one SELL slice (64 block rows, lane l = row
 // sell_row[64 s + l]), so every matrix block it writes is its own and no
-// atomics are needed.  The slice's incident cells (the plan's slice cell list,
-// ascending) are processed in chunks of CH cells:
-//   phase A  one thread per (cell, quadrature point): Jacobian, inverse, JxW,
-//            the 8 physical shape gradients and the material state (and the
-//            weighted stress for the residual), staged in LDS;
-//   phase B  one thread per (row lane, slot group): wave w owns the slots
-//            k = w, w + 4, ... of its lane's row and walks each slot's
-//            contribution list (cell, a, b) -- the same cell-ordered list the
-//            two-pass gather sums -- forming K_ab of the cell from the staged
-//            data and adding it to register accumulators.
-// Contributions are summed in cell order, as PETSc and the two-pass gather
-// do, so the result does not depend on scheduling (bit-reproducible).  After
-// the last chunk every lane stores its slots: one coalesced 64-lane store per
-// matrix value.  The difference form's row sum and the Jacobi inverse
-// diagonal are formed from the accumulators in a fixed order.
+// atomics are needed.  The plan lists the slice's (cell, local node a, row
+// lane) pairs in a fixed order (cells in chunk order, see k_slice_reorder) and
+// re-expresses each slot's contributions as (pair, column node b) entries.
+// The pairs are walked in chunks of at most 16 cells / 32 pairs; per chunk the
+// cells' geometry is staged in LDS, one thread per (pair, b) forms the 3x3
+// block K_ab (and the pair's residual contribution), and the thread owning a
+// slot (wave w owns the slots w, w + 8, ... of its lane's row) adds its
+// blocks in pair order.  The sums do not depend on scheduling
+// (bit-reproducible).  After the last chunk every lane stores its slots: one
+// coalesced 64-lane store per matrix value.  The difference form's row sum and
+// the Jacobi inverse diagonal are formed from the accumulators in a fixed order.
 //
 // A cell's Jacobian is recomputed once per slice that touches it (about 4.5
 // times for a structured HEX8 box with 64-row slices); each tangent block is
@@ -48,7 +44,7 @@ __global__ __launch_bounds__(256) void k_slice_cells(int64_t num_slices, int32_t
                                                      const int32_t* __restrict__ inc_ptr,
                                                      const int32_t* __restrict__ inc_idx,
                                                      const int32_t* __restrict__ out_ptr, int32_t* __restrict__ out,
-                                                     int32_t* __restrict__ err) {
+                                                     int32_t* __restrict__ pair_out, int32_t* __restrict__ err) {
   __shared__ int32_t buf[kSliceIncCap];
   __shared__ int32_t s_off[65];
   __shared__ int32_t s_scan[256];
@@ -70,6 +66,7 @@ __global__ __launch_bounds__(256) void k_slice_cells(int64_t num_slices, int32_t
     for (int i = 1; i <= 64; ++i) s_off[i] += s_off[i - 1];
   __syncthreads();
   const int total = s_off[64];
+  if (!FILL && t == 0) pair_out[s] = total <= kSliceIncCap ? total : 0;  // the slice's (cell, row node) pairs
   if (total > kSliceIncCap) {
     if (t == 0) {
       atomicMax(err, total);
@@ -163,9 +160,11 @@ __global__ __launch_bounds__(256) void k_contrib_local(fem_pattern_t pat, const 
 __global__ __launch_bounds__(64) void k_slice_reorder(fem_pattern_t pat, const int32_t* __restrict__ inc_ptr,
                                                       const int32_t* __restrict__ inc_idx,
                                                       const int32_t* __restrict__ sc_ptr, int32_t* __restrict__ sc,
+                                                      const int32_t* __restrict__ sp_ptr, int32_t* __restrict__ sp,
                                                       int32_t* __restrict__ loc, const int32_t* __restrict__ err) {
   __shared__ int32_t s_cell[kSliceIncCap];
   __shared__ int32_t s_rank[kSliceIncCap];
+  __shared__ int32_t s_key[kSliceIncCap];
   __shared__ int32_t s_cnt[64];
   if (*err) return;  // the plan is refused (a slice over capacity or a cell missing from its list)
   const int64_t s = blockIdx.x;
@@ -212,13 +211,60 @@ __global__ __launch_bounds__(64) void k_slice_reorder(fem_pattern_t pat, const i
   }
   __syncthreads();
   for (int i = lane; i < n; i += 64) sc[c0 + s_rank[i]] = s_cell[i];
+  // the slice's (cell, local node a, row lane) pairs, keyed (rank * 8 + a) * 64 + lane and
+  // sorted: the order the assembly forms and sums the blocks in
+  const int p0 = sp_ptr[s], np = sp_ptr[s + 1] - p0;
+  const int ib = row >= 0 ? inc_ptr[row] : 0, ie = row >= 0 ? inc_ptr[row + 1] : 0;
+  __syncthreads();
+  s_cnt[lane] = ie - ib;
+  __syncthreads();
+  if (lane == 0)
+    for (int l = 1; l < 64; ++l) s_cnt[l] += s_cnt[l - 1];
+  __syncthreads();
+  {
+    int pos = lane ? s_cnt[lane - 1] : 0;
+    for (int j = ib; j < ie; ++j) {
+      const int32_t e = inc_idx[j];
+      const int q = find(e / N);
+      s_key[pos++] = ((q < n ? s_rank[q] : 0) * 8 + e % N) * 64 + lane;
+    }
+  }
+  int P2 = 1;
+  while (P2 < np) P2 <<= 1;
+  for (int i = np + lane; i < P2; i += 64) s_key[i] = INT32_MAX;
+  __syncthreads();
+  for (int k = 2; k <= P2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < P2; i += 64) {
+        const int l = i ^ j;
+        if (l > i) {
+          const int32_t x = s_key[i], y = s_key[l];
+          if ((x > y) == ((i & k) == 0)) {
+            s_key[i] = y;
+            s_key[l] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = lane; i < np; i += 64) sp[p0 + i] = s_key[i];
   if (row < 0) return;
+  auto pair_of = [&](int32_t key) {
+    int lo = 0, hi = np;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (s_key[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  };
   const int rb = pat.row_ptr[row], len = pat.row_ptr[row + 1] - rb;
   for (int k = 0; k < len; ++k) {
     const int t0 = pat.contrib_ptr[rb + k], t1 = pat.contrib_ptr[rb + k + 1];
     for (int t = t0; t < t1; ++t) {
       const int v = loc[t];  // in [0, n * NN): k_contrib_local found every contribution's cell
-      loc[t] = s_rank[v / NN] * NN + v % NN;
+      const int a = (v % NN) / N, b = v % N;
+      loc[t] = pair_of((s_rank[v / NN] * 8 + a) * 64 + lane) * N + b;
     }
     for (int t = t0 + 1; t < t1; ++t) {  // insertion sort of the slot's few entries
       const int v = loc[t];
@@ -234,62 +280,86 @@ __global__ __launch_bounds__(64) void k_slice_reorder(fem_pattern_t pat, const i
 
 // ------------------------------------------------------------- assembly
 template <class MAT>
-struct AsmCfg {
-  using St = typename MAT::template St<double>;
-  static constexpr int SW = (int)(sizeof(St) / sizeof(double));
-  // cells per chunk: 32 (one phase-A item per thread of the first half) for light
-  // material states, 16 for the large ones (Neo-Hookean, J2)
-  static constexpr int CH = SW <= 4 ? 32 : 16;
-  static constexpr int GS = 8 * 8 * 3 + 1;  // odd stride (in doubles) between cells: no LDS bank conflicts
-};
-
-template <class MAT>
 struct alignas(8) StRow {
   typename MAT::template St<double> s[8];
   double pad;  // odd stride between cells
 };
 
-// Isotropic linear laws (elasticity, SIMP): K_ab = A + B^T + tr(B) I with
-// A = sum_q lambda_q w_q g_a (x) g_b and B = sum_q mu_q w_q g_a (x) g_b (the
-// closed form of MatElastic::tangent summed over the quadrature points).
+// Isotropic linear laws (elasticity, SIMP): lambda = c0 E and mu = c1 E at every
+// point, so K_ab = c0 M + c1 M^T + c1 tr(M) I with the single weighted sum
+// M = sum_q JxW E g_a (x) g_b (the closed form of MatElastic::tangent).
 template <class MAT>
-struct IsoLinear : std::false_type {};
+struct IsoLinear : std::false_type {
+  __device__ static double E(const fem_material_t&, int64_t) { return 1.0; }
+};
 template <bool S>
-struct IsoLinear<MatElastic<S>> : std::true_type {};
+struct IsoLinear<MatElastic<S>> : std::true_type {
+  // the modulus factor E of lambda = c0 E, mu = c1 E at point cq
+  __device__ static double E(const fem_material_t& m, int64_t cq) {
+    return S ? MatElastic<S>::modulus(m, m.param[cq]) : 1.0;
+  }
+};
 
-constexpr int kAsmMaxK = 4;  // slots per thread: rows of at most 8 * 4 = 32 blocks
-constexpr int kResWave0 = kAsmWaves - 3;  // waves forming the residual's components
+constexpr int kAsmMaxK = 4;               // slots per thread: rows of at most 8 * 4 = 32 blocks
+constexpr int kResWave0 = kAsmWaves - 3;  // waves summing the residual's components
+constexpr int kAsmGS = 8 * 8 * 3 + 1;     // odd stride (in doubles) between cells' gradients
 
+// Chunk sizes: at most CH cells and PM (cell, row node) pairs per chunk (on a
+// z-line slice every cell holds two of the slice's nodes, so PM = 2 CH).
+template <class MAT>
+struct AsmCfg {
+  using St = typename MAT::template St<double>;
+  static constexpr bool kBig = sizeof(St) > 4 * sizeof(double);  // Neo-Hookean, J2: large point states
+  static constexpr int CH = kBig ? 16 : 32;
+  static constexpr int PM = 2 * CH;
+};
+constexpr int kSlicePairCap = 1024;  // pairs (and cells) of one slice, staged in LDS
+
+// The slice's (cell, local node a, row lane) pairs are walked in chunks; per
+// chunk:
+//   A1  one thread per (cell, node): node coordinates (and u) to LDS;
+//   A2  one thread per (cell, quadrature point): Jacobian, inverse, JxW, the 8
+//       physical gradients, the material state (and the weighted stress);
+//   B1  one thread per (pair, column node b): the 3x3 block K_ab of the pair's
+//       cell (the pair's 8 threads share g_a and read the 8 g_b side by side)
+//       and, for b < V, component b of the pair's residual contribution, all
+//       staged in LDS;
+//   B2  every owner thread (row lane, slot) adds the staged blocks of its
+//       slots, in pair order (which is the plan's fixed cell order).
 template <class MAT, typename TO, bool DIFF, bool RES>
 __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
     fem_mesh_t mesh, fem_material_t mat, fem_pattern_t pat, const int32_t* __restrict__ sc_ptr,
-    const int32_t* __restrict__ sc, const int32_t* __restrict__ cloc, const double* __restrict__ u,
-    TO* __restrict__ vals, TO* __restrict__ dinv, const double* __restrict__ base, double* __restrict__ res) {
-  constexpr int N = 8, Q = 8, V = MAT::V, VV = V * V, MAXK = kAsmMaxK;
-  using Cfg = AsmCfg<MAT>;
-  constexpr int CH = Cfg::CH, GS = Cfg::GS;
+    const int32_t* __restrict__ sc, const int32_t* __restrict__ sp_ptr, const int32_t* __restrict__ sp,
+    const int32_t* __restrict__ cloc, const double* __restrict__ u, TO* __restrict__ vals, TO* __restrict__ dinv,
+    const double* __restrict__ base, double* __restrict__ res) {
+  constexpr int N = 8, Q = 8, V = MAT::V, VV = V * V, MAXK = kAsmMaxK, GS = kAsmGS;
+  constexpr int CH = AsmCfg<MAT>::CH, PM = AsmCfg<MAT>::PM;
+  constexpr bool kIso = IsoLinear<MAT>::value;
   __shared__ double s_dN[Q][N][3];
   __shared__ double s_w[Q];
   __shared__ double s_g[CH][GS];
-  __shared__ double s_jxw[CH][Q + 1];
-  __shared__ StRow<MAT> s_st[CH];
-  // phase A1 stages the chunk's node coordinates and u in s_buf; with the residual, the
-  // weighted stresses of phase A2 reuse it
+  __shared__ double s_wq[CH][Q + 1];                        // JxW (elastic laws: JxW E)
+  __shared__ StRow<MAT> s_st[kIso ? 1 : CH];
+  __shared__ double s_K[PM][N][VV];                         // staged blocks of the chunk's pairs
+  __shared__ double s_R[RES ? PM : 1][V];                   // staged residual contributions
+  __shared__ int64_t s_c[CH];
+  __shared__ int32_t s_key[kSlicePairCap];   // the slice's pair keys
+  __shared__ int32_t s_cid[kSlicePairCap];   // the slice's cells, in rank order
+  // A1 stages coordinates and u in s_buf; with the residual, A2's weighted stresses reuse it
   constexpr int kXU = CH * N * (3 + V);
   constexpr int kP = RES ? CH * (Q * V * 3 + 1) : 0;
   __shared__ double s_buf[kXU > kP ? kXU : kP];
-  __shared__ int64_t s_c[CH];
   double(*s_x)[N][3] = reinterpret_cast<double(*)[N][3]>(s_buf);
   double(*s_u)[N][V] = reinterpret_cast<double(*)[N][V]>(s_buf + CH * N * 3);
   double(*s_P)[Q * V * 3 + 1] = reinterpret_cast<double(*)[Q * V * 3 + 1]>(s_buf);
-  // the difference form's row sums are staged in s_g after the last chunk, half the waves at a time
+  // the difference form's row sums are staged in s_K after the last chunk, half the waves at a time
   constexpr int kHalf = kAsmWaves / 2;
-  static_assert(kHalf * kSlice * VV <= CH * GS, "row-sum staging does not fit in s_g");
-  double(*s_red)[kSlice][VV] = reinterpret_cast<double(*)[kSlice][VV]>(&s_g[0][0]);
+  static_assert(kHalf * kSlice * VV <= PM * N * VV, "row-sum staging does not fit in s_K");
+  double(*s_red)[kSlice][VV] = reinterpret_cast<double(*)[kSlice][VV]>(&s_K[0][0][0]);
 
   const int64_t s = blockIdx.x;
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
-  const double iso_c1 = IsoLinear<MAT>::value ? mat.c[1] : 0.0;
+  const double iso_c1 = kIso ? mat.c[1] : 0.0;  // elastic laws: the slots sum M; K is formed at the end
   load_tables(mesh, s_dN, s_w);
   const int sbase = pat.slice_ptr[s];
   const int L = (pat.slice_ptr[s + 1] - sbase) >> 6;
@@ -298,12 +368,15 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
   const int rb = real ? pat.row_ptr[row] : 0;
   const int len = real ? pat.row_ptr[row + 1] - rb : 0;
   const int ds = real ? pat.diag_slot[row] : -1;
-  const int c0 = sc_ptr[s];
-  const int ncell = sc_ptr[s + 1] - c0;
+  const int c0 = sc_ptr[s], ncs = sc_ptr[s + 1] - c0;
+  const int p_base = sp_ptr[s], np = sp_ptr[s + 1] - p_base;
+  // np, ncs <= kSlicePairCap: fem_assemble refuses plans with larger slices
+  for (int i = t; i < np; i += kAsmThreads) s_key[i] = sp[p_base + i];
+  for (int i = t; i < ncs; i += kAsmThreads) s_cid[i] = sc[c0 + i];
 
   double acc[MAXK][VV];
-  // cur: next contribution of each owned slot; nxt: its slice-local id, loaded one
-  // contribution ahead so the load overlaps the block being formed
+  // cur: next contribution of each owned slot; nxt: its entry (pair * 8 + b), loaded one
+  // contribution ahead
   int cur[MAXK], cend[MAXK], nxt[MAXK];
 #pragma unroll
   for (int m = 0; m < MAXK; ++m) {
@@ -317,9 +390,8 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
 #pragma unroll
     for (int v = 0; v < VV; ++v) acc[m][v] = 0.0;
   }
-  // residual: component ri of the row is formed by wave kResWave0 + ri (the last waves,
-  // which own one slot fewer on 27-block rows) from the diagonal slot's list, which
-  // holds exactly the row's incidences (c, a, a) in cell order
+  // residual: component ri of the row is summed by wave kResWave0 + ri from the diagonal
+  // slot's list, whose entries are exactly the row's pairs
   const int ri = w - kResWave0;
   const bool resw = RES && real && ri >= 0 && ri < V;
   int rcur = 0, rend = 0, rnxt = INT32_MAX;
@@ -331,26 +403,33 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
   double r = 0.0;
   __syncthreads();
 
-  for (int ch0 = 0; ch0 < ncell; ch0 += CH) {
-    // ---- phase A1: the chunk's node coordinates (and u) staged in LDS, one (cell, node) per thread
-    if (t < CH * N) {
+  for (int p0 = 0; p0 < np;) {
+    // ---- the chunk: pairs [p0, p1), cells of ranks [r0, r0 + ncell)
+    const int r0 = s_key[p0] >> 9;
+    int lo = p0 + 1, hi = min(np, p0 + PM);  // first pair of rank >= r0 + CH, at most p0 + PM
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if ((s_key[mid] >> 9) < r0 + CH) lo = mid + 1; else hi = mid;
+    }
+    const int p1 = lo;
+    const int ncell = (s_key[p1 - 1] >> 9) - r0 + 1;
+    // ---- A1: node coordinates (and u)
+    if (t < ncell * N) {
       const int cc = t / N, a = t % N;
-      if (ch0 + cc < ncell) {
-        const int64_t c = sc[c0 + ch0 + cc];
-        if (a == 0) s_c[cc] = c;
-        const int64_t na = mesh.cells[c * N + a];
+      const int64_t c = s_cid[r0 + cc];
+      if (a == 0) s_c[cc] = c;
+      const int64_t na = mesh.cells[c * N + a];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) s_x[cc][a][j] = mesh.points[na * 3 + j];
-        if (MAT::kNeedsU || RES)
+      for (int j = 0; j < 3; ++j) s_x[cc][a][j] = mesh.points[na * 3 + j];
+      if (MAT::kNeedsU || RES)
 #pragma unroll
-          for (int i = 0; i < V; ++i) s_u[cc][a][i] = u[na * V + i];
-      }
+        for (int i = 0; i < V; ++i) s_u[cc][a][i] = u[na * V + i];
     }
     __syncthreads();
-    // ---- phase A2: per (cell, quadrature point) geometry and material state
+    // ---- A2: per (cell, quadrature point) geometry and material state
     double Pw[RES ? V : 1][3];
     const int acc_cc = t / Q, acc_q = t % Q;
-    const bool a2 = t < CH * Q && ch0 + acc_cc < ncell;
+    const bool a2 = t < ncell * Q;
     if (a2) {
       const int cc = acc_cc, q = acc_q;
       const int64_t c = s_c[cc];
@@ -358,7 +437,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
       double J[9];
 #pragma unroll
       for (int k = 0; k < 9; ++k) J[k] = 0.0;
-#pragma unroll 1
+#pragma unroll
       for (int a = 1; a < N; ++a) {
         const double d0 = s_x[cc][a][0] - x00, d1 = s_x[cc][a][1] - x01, d2 = s_x[cc][a][2] - x02;
 #pragma unroll
@@ -378,7 +457,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
       for (int i = 0; i < V; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) G[i][j] = 0.0;
-#pragma unroll 1
+#pragma unroll
       for (int a = 0; a < N; ++a) {
         const double e0 = s_dN[q][a][0], e1 = s_dN[q][a][1], e2 = s_dN[q][a][2];
         double g[3];
@@ -397,13 +476,8 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
         }
       }
       const auto st = MAT::template prep<double>(mat, c * Q + q, G);
-      // elastic laws: lambda and mu are c0 E and c1 E at every point, so the tangent needs the
-      // single weight JxW E per point (E = mu / c1 when c1 > 0)
-      double wq = jxw;
-      if constexpr (IsoLinear<MAT>::value)
-        if (iso_c1 > 0.0) wq = jxw * (double(st.mu) / iso_c1);
-      s_jxw[cc][q] = wq;
-      s_st[cc].s[q] = st;
+      if constexpr (!kIso) s_st[cc].s[q] = st;
+      s_wq[cc][q] = kIso ? jxw * IsoLinear<MAT>::E(mat, c * Q + q) : jxw;  // elastic laws: JxW E
       if (RES) {
         double P[V][3];
         MAT::stress(st, G, P);
@@ -423,56 +497,62 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
           for (int j = 0; j < 3; ++j) s_P[acc_cc][(acc_q * V + i) * 3 + j] = Pw[i][j];
     }
     __syncthreads();
-    // ---- phase B: this thread's slots, contributions of this chunk's cells
-    const int lim = (ch0 + CH) * N * N;  // contributions of cells before the chunk's end
+    // ---- B1: one block per (pair, column node)
+    if (t < (p1 - p0) * N) {
+      const int pp = t / N, b = t % N;
+      const int key = s_key[p0 + pp];
+      const int cc = (key >> 9) - r0, a = (key >> 6) & 7;
+      double K[V][V];
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int k = 0; k < V; ++k) K[i][k] = 0.0;
+      if constexpr (kIso) {
+        // M = sum_q JxW E g_a (x) g_b
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const double* ga = &s_g[cc][(q * N + a) * 3];
+          const double* gb = &s_g[cc][(q * N + b) * 3];
+          const double wq = s_wq[cc][q];
+          const double gb0 = gb[0], gb1 = gb[1], gb2 = gb[2];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const double wa = wq * ga[i];
+            K[i][0] += wa * gb0;
+            K[i][1] += wa * gb1;
+            K[i][2] += wa * gb2;
+          }
+        }
+      } else {
+#pragma unroll 2
+        for (int q = 0; q < Q; ++q)
+          MAT::tangent(s_st[cc].s[q], &s_g[cc][(q * N + a) * 3], &s_g[cc][(q * N + b) * 3], s_wq[cc][q], K);
+      }
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+#pragma unroll
+        for (int k = 0; k < V; ++k) s_K[pp][b][i * V + k] = K[i][k];
+      if (RES && b < V) {
+        double rr = 0.0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) rr += s_P[cc][(q * V + b) * 3 + j] * s_g[cc][(q * N + a) * 3 + j];
+        s_R[RES ? pp : 0][b] = rr;
+      }
+    }
+    __syncthreads();
+    // ---- B2: owners add their slots' blocks of this chunk, in pair order
+    const int lim = p1 * N;
 #pragma unroll
     for (int m = 0; m < MAXK; ++m) {
       int e = nxt[m];
       while (e < lim) {
         ++cur[m];
         const int en = cur[m] < cend[m] ? cloc[cur[m]] : INT32_MAX;
-        const int cc = e / (N * N) - ch0;
-        const int a = (e >> 3) & 7, b = e & 7;
-        double K[V][V];
-        if (IsoLinear<MAT>::value && iso_c1 > 0.0) {
-          // K = c0 M + c1 M^T + c1 tr(M) I with M = sum_q JxW E g_a (x) g_b
-          double M[3][3];
+        const double* kb = &s_K[e / N - p0][e % N][0];
 #pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) M[i][k] = 0.0;
-#pragma unroll 4
-          for (int q = 0; q < Q; ++q) {
-            const double* ga = &s_g[cc][(q * N + a) * 3];
-            const double* gb = &s_g[cc][(q * N + b) * 3];
-            const double wq = s_jxw[cc][q];
-            const double gb0 = gb[0], gb1 = gb[1], gb2 = gb[2];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const double wa = wq * ga[i];
-              M[i][0] += wa * gb0;
-              M[i][1] += wa * gb1;
-              M[i][2] += wa * gb2;
-            }
-          }
-          const double c0 = mat.c[0], tr = iso_c1 * (M[0][0] + M[1][1] + M[2][2]);
-#pragma unroll
-          for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) K[i][k] = c0 * M[i][k] + iso_c1 * M[k][i] + (i == k ? tr : 0.0);
-        } else {
-#pragma unroll
-          for (int i = 0; i < V; ++i)
-#pragma unroll
-            for (int k = 0; k < V; ++k) K[i][k] = 0.0;
-#pragma unroll 2
-          for (int q = 0; q < Q; ++q)
-            MAT::tangent(s_st[cc].s[q], &s_g[cc][(q * N + a) * 3], &s_g[cc][(q * N + b) * 3], s_jxw[cc][q], K);
-        }
-#pragma unroll
-        for (int i = 0; i < V; ++i)
-#pragma unroll
-          for (int k = 0; k < V; ++k) acc[m][i * V + k] += K[i][k];
+        for (int v = 0; v < VV; ++v) acc[m][v] += kb[v];
         e = en;
       }
       nxt[m] = e;
@@ -481,18 +561,27 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
       while (rnxt < lim) {
         ++rcur;
         const int en = rcur < rend ? cloc[rcur] : INT32_MAX;
-        const int cc = rnxt / (N * N) - ch0;
-        const int a = (rnxt >> 3) & 7;
-        double t = 0.0;
-#pragma unroll 2
-        for (int q = 0; q < Q; ++q)
-#pragma unroll
-          for (int j = 0; j < 3; ++j) t += s_P[cc][(q * V + ri) * 3 + j] * s_g[cc][(q * N + a) * 3 + j];
-        r += t;
+        r += s_R[RES ? rnxt / N - p0 : 0][ri];
         rnxt = en;
       }
     }
     __syncthreads();
+    p0 = p1;
+  }
+  if constexpr (kIso) {
+    // the slots summed M: K = c0 M + c1 M^T + c1 tr(M) I
+    const double c0m = mat.c[0];
+#pragma unroll
+    for (int m = 0; m < MAXK; ++m) {
+      double M[9];
+#pragma unroll
+      for (int v = 0; v < 9; ++v) M[v] = acc[m][v];
+      const double tr = iso_c1 * (M[0] + M[4] + M[8]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[m][i * 3 + k] = c0m * M[i * 3 + k] + iso_c1 * M[k * 3 + i] + (i == k ? tr : 0.0);
+    }
   }
 
   // ---- difference form: row sums of all slots, combined in a fixed order
@@ -514,7 +603,7 @@ __global__ __launch_bounds__(kAsmThreads) void k_assemble_hex8(
 #pragma unroll
       for (int v = 0; v < VV; ++v) rsum[v] += s_red[w][lane][v];
     __syncthreads();
-    // round 2: the pair sums, read by the thread that owns the diagonal slot
+    // round 2: the pair sums, read by every thread
     if (w < kHalf)
 #pragma unroll
       for (int v = 0; v < VV; ++v) s_red[w][lane][v] = rsum[v];
@@ -584,34 +673,38 @@ using namespace fem;
 extern "C" {
 
 int fem_assembly_plan_counts(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
-                             int32_t* slice_cell_counts, int32_t* err_flag, void* stream) {
-  FEM_CHECK_ARG(pat && inc_ptr && inc_idx && slice_cell_counts && err_flag, "fem_assembly_plan_counts: NULL argument");
+                             int32_t* slice_cell_counts, int32_t* slice_pair_counts, int32_t* err_flag,
+                             void* stream) {
+  FEM_CHECK_ARG(pat && inc_ptr && inc_idx && slice_cell_counts && slice_pair_counts && err_flag,
+                "fem_assembly_plan_counts: NULL argument");
   if (pat->num_slices == 0) return FEM_OK;
   hipStream_t st = as_stream(stream);
   FEM_HIP(hipMemsetAsync(err_flag, 0, sizeof(int32_t), st));
   hipLaunchKernelGGL(k_slice_cells<false>, dim3((unsigned)pat->num_slices), dim3(256), 0, st, pat->num_slices,
                      pat->nodes_per_cell, pat->sell_row, inc_ptr, inc_idx, (const int32_t*)nullptr,
-                     slice_cell_counts, err_flag);
+                     slice_cell_counts, slice_pair_counts, err_flag);
   FEM_LAUNCHED();
   return FEM_OK;
 }
 
 int fem_assembly_plan_fill(const fem_pattern_t* pat, const int32_t* inc_ptr, const int32_t* inc_idx,
-                           const int32_t* slice_cell_ptr, int32_t* slice_cells, int32_t* contrib_loc,
-                           int32_t* err_flag, void* stream) {
-  FEM_CHECK_ARG(pat && inc_ptr && inc_idx && slice_cell_ptr && slice_cells && contrib_loc && err_flag,
+                           const int32_t* slice_cell_ptr, const int32_t* slice_pair_ptr, int32_t* slice_cells,
+                           int32_t* slice_pairs, int32_t* contrib_loc, int32_t* err_flag, void* stream) {
+  FEM_CHECK_ARG(pat && inc_ptr && inc_idx && slice_cell_ptr && slice_pair_ptr && slice_cells && slice_pairs &&
+                    contrib_loc && err_flag,
                 "fem_assembly_plan_fill: NULL argument");
   if (pat->num_slices == 0) return FEM_OK;
   hipStream_t st = as_stream(stream);
   hipLaunchKernelGGL(k_slice_cells<true>, dim3((unsigned)pat->num_slices), dim3(256), 0, st, pat->num_slices,
-                     pat->nodes_per_cell, pat->sell_row, inc_ptr, inc_idx, slice_cell_ptr, slice_cells, err_flag);
+                     pat->nodes_per_cell, pat->sell_row, inc_ptr, inc_idx, slice_cell_ptr, slice_cells,
+                     (int32_t*)nullptr, err_flag);
   FEM_LAUNCHED();
   const int64_t lanes = pat->num_slices * kSlice;
   hipLaunchKernelGGL(k_contrib_local, dim3((unsigned)ceil_div(lanes, 256)), dim3(256), 0, st, *pat, slice_cell_ptr,
                      slice_cells, contrib_loc, err_flag);
   FEM_LAUNCHED();
   hipLaunchKernelGGL(k_slice_reorder, dim3((unsigned)pat->num_slices), dim3(64), 0, st, *pat, inc_ptr, inc_idx,
-                     slice_cell_ptr, slice_cells, contrib_loc, err_flag);
+                     slice_cell_ptr, slice_cells, slice_pair_ptr, slice_pairs, contrib_loc, err_flag);
   FEM_LAUNCHED();
   return FEM_OK;
 }
@@ -624,9 +717,13 @@ int fem_assemble(const fem_mesh_t* mesh, const fem_material_t* mat, const fem_pa
                 "fem_assemble: single-pass assembly covers HEX8 cells with 8 quadrature points");
   FEM_CHECK_ARG(pat->nodes_per_cell == 8 && pat->vec == mesh->vec && pat->num_cells == mesh->num_cells,
                 "fem_assemble: pattern and mesh disagree");
-  FEM_CHECK_ARG(plan->slice_cell_ptr && plan->slice_cells && plan->contrib_loc, "fem_assemble: plan not built");
+  FEM_CHECK_ARG(plan->slice_cell_ptr && plan->slice_cells && plan->slice_pair_ptr && plan->slice_pairs &&
+                    plan->contrib_loc,
+                "fem_assemble: plan not built");
   FEM_CHECK_ARG(plan->max_slots >= 0 && plan->max_slots <= kAsmWaves * kAsmMaxK,
                 "fem_assemble: a slice has %d blocks per row (at most %d)", plan->max_slots, kAsmWaves * kAsmMaxK);
+  FEM_CHECK_ARG(plan->max_pairs >= 0 && plan->max_pairs <= kSlicePairCap,
+                "fem_assemble: a slice has %d (cell, row node) pairs (at most %d)", plan->max_pairs, kSlicePairCap);
   FEM_CHECK_ARG((precision == FEM_F64 && form == FEM_FORM_STANDARD) ||
                     (precision == FEM_F32 && form == FEM_FORM_DIFFERENCE),
                 "fem_assemble: FP64 standard-form or FP32 difference-form storage");
@@ -640,12 +737,12 @@ int fem_assemble(const fem_mesh_t* mesh, const fem_material_t* mat, const fem_pa
       constexpr bool R = decltype(res_c)::value;
       if (precision == FEM_F64)
         hipLaunchKernelGGL((k_assemble_hex8<MAT, double, false, R>), grid, block, 0, st, *mesh, *mat, *pat,
-                           plan->slice_cell_ptr, plan->slice_cells, plan->contrib_loc, u, (double*)vals,
-                           (double*)dinv, base, res);
+                           plan->slice_cell_ptr, plan->slice_cells, plan->slice_pair_ptr, plan->slice_pairs,
+                           plan->contrib_loc, u, (double*)vals, (double*)dinv, base, res);
       else
         hipLaunchKernelGGL((k_assemble_hex8<MAT, float, true, R>), grid, block, 0, st, *mesh, *mat, *pat,
-                           plan->slice_cell_ptr, plan->slice_cells, plan->contrib_loc, u, (float*)vals,
-                           (float*)dinv, base, res);
+                           plan->slice_cell_ptr, plan->slice_cells, plan->slice_pair_ptr, plan->slice_pairs,
+                           plan->contrib_loc, u, (float*)vals, (float*)dinv, base, res);
     };
     if (res)
       go(std::true_type{});

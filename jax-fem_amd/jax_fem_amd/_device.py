@@ -488,21 +488,27 @@ class DeviceProblem:
                 max_slots = int(((sp[1:] - sp[:-1]) // 64).max().item())
                 if max_slots <= 32:
                     counts = dm.empty(ns, dtype=torch.int32)
+                    pcounts = dm.empty(ns, dtype=torch.int32)
                     err = dm.empty(1, dtype=torch.int32)
                     check(L.fem_assembly_plan_counts(self.pref, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(counts),
-                                                     ptr(err), s), 'fem_assembly_plan_counts')
-                    if int(err.item()) == 0:
+                                                     ptr(pcounts), ptr(err), s), 'fem_assembly_plan_counts')
+                    max_pairs = int(pcounts.max().item()) if ns else 0
+                    if int(err.item()) == 0 and max_pairs <= 1024:
                         sc_ptr = dm.scan(counts)
+                        sp_ptr = dm.scan(pcounts)
                         sc = dm.empty(max(int(sc_ptr[-1].item()), 1), dtype=torch.int32)
+                        sp = dm.empty(max(int(sp_ptr[-1].item()), 1), dtype=torch.int32)
                         loc = dm.empty(max(dm.C * dm.N * dm.N, 1), dtype=torch.int32)
                         check(L.fem_assembly_plan_fill(self.pref, ptr(self.inc_ptr), ptr(self.inc_idx), ptr(sc_ptr),
-                                                       ptr(sc), ptr(loc), ptr(err), s), 'fem_assembly_plan_fill')
+                                                       ptr(sp_ptr), ptr(sc), ptr(sp), ptr(loc), ptr(err), s),
+                              'fem_assembly_plan_fill')
                         if int(err.item()) == 0:
                             plan = _lib.FemAssemblyPlan()
                             plan.slice_cell_ptr, plan.slice_cells, plan.contrib_loc = (
                                 sc_ptr.data_ptr(), sc.data_ptr(), loc.data_ptr())
-                            plan.max_slots = max_slots
-                            self._asm_plan = (plan, (sc_ptr, sc, loc))
+                            plan.slice_pair_ptr, plan.slice_pairs = sp_ptr.data_ptr(), sp.data_ptr()
+                            plan.max_slots, plan.max_pairs = max_slots, max_pairs
+                            self._asm_plan = (plan, (sc_ptr, sc, sp_ptr, sp, loc))
                         else:
                             logger.info("single-pass assembly plan refused (code %d): two-pass assembly",
                                         int(err.item()))

@@ -58,8 +58,8 @@ class FemConstraint(C.Structure):
 
 
 class FemAssemblyPlan(C.Structure):
-    _fields_ = [('slice_cell_ptr', _vp), ('slice_cells', _vp), ('contrib_loc', _vp), ('max_slots', _i32),
-                ('pad', _i32)]
+    _fields_ = [('slice_cell_ptr', _vp), ('slice_cells', _vp), ('slice_pair_ptr', _vp), ('slice_pairs', _vp),
+                ('contrib_loc', _vp), ('max_slots', _i32), ('max_pairs', _i32)]
 
 
 class FemPcgOptions(C.Structure):
@@ -75,8 +75,8 @@ class FemPcgResult(C.Structure):
 
 
 class FemBcsr(C.Structure):
-    _fields_ = [('rows', _i64), ('cols', _i64), ('nnzb', _i64), ('br', _i32), ('bc', _i32),
-                ('row_ptr', _vp), ('col_idx', _vp), ('vals', _vp)]
+    _fields_ = [('rows', _i64), ('cols', _i64), ('nnzb', _i64), ('br', _i32), ('bc', _i32), ('f32', _i32),
+                ('pad', _i32), ('row_ptr', _vp), ('col_idx', _vp), ('vals', _vp)]
 
 
 class FemAmgLevel(C.Structure):
@@ -128,8 +128,8 @@ _SIGNATURES = {
     'fem_spmv': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp]),
     'fem_export_csr': (C.c_int, [_P(FemPattern), _i32, _i32, _vp, _vp, _vp, _vp, _vp]),
     'fem_operator_apply': (C.c_int, [_P(FemOperator), _vp, _vp, _vp]),
-    'fem_assembly_plan_counts': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _vp]),
-    'fem_assembly_plan_fill': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_assembly_plan_counts': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _vp, _vp]),
+    'fem_assembly_plan_fill': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fem_assemble': (C.c_int, [_P(FemMesh), _P(FemMaterial), _P(FemPattern), _P(FemAssemblyPlan), _vp, _i32,
                                _i32, _vp, _vp, _vp, _vp, _vp]),
     'fem_csr_spmv': (C.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -147,7 +147,7 @@ _SIGNATURES = {
     'fem_amg_members': (C.c_int, [_i64, _vp, _i64, _vp, _vp, _vp, _vp]),
     'fem_amg_tentative': (C.c_int, [_i64, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     'fem_amg_bc_mask': (C.c_int, [_i64, _i64, _vp, _vp, _vp]),
-    'fem_amg_sell_to_bcsr': (C.c_int, [_P(FemPattern), _vp, _vp, _vp]),
+    'fem_amg_sell_to_bcsr': (C.c_int, [_P(FemPattern), _vp, _vp, _i32, _vp]),
     'fem_amg_sell_to_f32_diff': (C.c_int, [_P(FemPattern), _vp, _vp, _vp]),
     'fem_amg_dinv': (C.c_int, [_P(FemBcsr), _i32, _vp, _vp]),
     'fem_amg_lmax_ws_bytes': (_i64, [_i64]),

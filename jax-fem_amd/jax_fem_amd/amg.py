@@ -49,6 +49,7 @@ class Bcsr:
         v = self.vals if vals is None else vals
         s = _lib.FemBcsr()
         s.rows, s.cols, s.nnzb, s.br, s.bc = self.rows, self.cols, self.nnzb, self.br, self.bc
+        s.f32 = 1 if v is not None and v.dtype == torch.float32 else 0
         s.row_ptr = self.row_ptr.data_ptr()
         s.col_idx = self.col_idx.data_ptr() if self.nnzb else None
         s.vals = v.data_ptr() if self.nnzb else None
@@ -68,6 +69,7 @@ class DeviceAMG:
         self.L = dev.L
         self._mask = None
         self._f32 = {}
+        self._vbuf = {}
         self.vals32 = None
         self._diff_fresh = False
         self.levels = []
@@ -186,6 +188,18 @@ class DeviceAMG:
               'fem_amg_sell_to_f16_scaled')
         return out
 
+    def _values(self, M, dtype):
+        """Point M.vals at M's value buffer of `dtype`: the structure allocates FP64 values; an FP32
+        hierarchy (FP32 Galerkin products) keeps a second, FP32 buffer, allocated on first use."""
+        bufs = M.__dict__.setdefault('_bufs', {})
+        if M.vals is not None:
+            bufs.setdefault(M.vals.dtype, M.vals)
+        if dtype not in bufs:
+            n = M.nnzb * M.br * M.bc
+            bufs[dtype] = self._empty(n, dtype)[:n]
+        M.vals = bufs[dtype]
+        return M.vals
+
     def _to_f32(self, t, key):
         out = self._f32.get(key)
         if out is None or out.numel() != t.numel():
@@ -252,8 +266,18 @@ class DeviceAMG:
         smoother matrix, FP32 difference form or FP16 D^-1 A in difference form."""
         dev, L = self.dev, self.L
         levels, coarse = self._structure()
+        # an FP32 hierarchy forms its Galerkin products from FP32 values (FP64 arithmetic):
+        # level 0's block-CSR copy, P, R, A P and the coarse operators are FP32
+        dt = torch.float32 if precision == _lib.FEM_F32 else torch.float64
+        for lv in levels:
+            for name in ('A', 'P', 'R', 'W'):
+                self._values(lv[name], dt)
+        # a matrix without levels is its own coarse matrix: its dense inverse is formed from the
+        # FP64 values, so the cycle is the exact solve in either precision
+        self._values(coarse, dt if levels else torch.float64)
         A0 = levels[0]['A'] if levels else coarse   # no levels: the matrix is the coarsest level
-        check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), stream_ptr()), 'fem_amg_sell_to_bcsr')
+        prec0 = _lib.FEM_F32 if A0.vals.dtype == torch.float32 else _lib.FEM_F64
+        check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), prec0, stream_ptr()), 'fem_amg_sell_to_bcsr')
         f16 = level0 == 'fp16' and precision == _lib.FEM_F32 and bool(levels)
         sv32 = self._sell_diff32(vals64) if (precision == _lib.FEM_F32 and levels and not f16) else None
         for l, lv in enumerate(levels):
@@ -303,7 +327,7 @@ class DeviceAMG:
                 self._diff_fresh = f32
             for name in ('A', 'P', 'R'):
                 M = lv[name]
-                v = self._to_f32(M.vals, (l, name)) if (f32 and not (name == 'A' and l == 0)) else M.vals
+                v = M.vals     # in the hierarchy's precision
                 keep.append(v)
                 setattr(e, name, M.struct(v))
             e.dinv = lv['dinv'].data_ptr()

@@ -40,7 +40,7 @@ _PRECISIONS = {'fp64': _lib.FEM_F64, 'mixed': _lib.FEM_F32, 'fp32': _lib.FEM_F32
 # FP32; `setup_reuse` = k lets one hierarchy precondition k further solves
 # (Newton steps).  From an AMGX config file only solver.tolerance and
 # solver.max_iters are taken (as atol and maxiter, AMGX's ABSOLUTE convergence).
-AMG_SOLVER_DEFAULTS = dict(cfg_path=None, precision='fp64', tol=1e-10, atol=1e-10, maxiter=1000, check_every=1,
+AMG_SOLVER_DEFAULTS = dict(cfg_path=None, precision='fp64', tol=1e-10, atol=1e-10, maxiter=1000, check_every=4,
                            degree=2, timing=False, setup_reuse=0, outer='fp64', reliable_delta=0.01, level0='fp32')
 
 

@@ -104,7 +104,10 @@ def test_hierarchy_matches_oracle(gpu, make, vec):
     assert _rel(inv, H.coarse_inv) < 1e-8
 
 
-@pytest.mark.parametrize('precision,tol', [('fp64', 1e-11), ('mixed', 1e-5)])
+# the FP32 hierarchy forms its Galerkin products and Jacobi diagonals from FP32 values: their
+# rounding (1e-7), amplified through the coarse levels' solves, moves the cycle's output by a
+# few 1e-5 against the FP64 oracle
+@pytest.mark.parametrize('precision,tol', [('fp64', 1e-11), ('mixed', 1e-4)])
 def test_vcycle_matches_oracle(gpu, precision, tol):
     p = _elastic(32)
     dev, A, (gp, gi), mask, pts, b = _assembled(p)
@@ -262,9 +265,14 @@ def test_sell_to_bcsr_on_a_length_sorted_pattern(gpu):
     sell_row = dev.pat_tensors['sell_row'].cpu().numpy()
     assert not np.array_equal(sell_row[:dev.dm.Nn], np.arange(dev.dm.Nn))
     vals = dev.current[2]
-    out = torch.empty(dev.nnzb * 9, dtype=torch.float64, device='cuda')
-    check(dev.L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals), ptr(out), stream_ptr()), 'fem_amg_sell_to_bcsr')
-    B = sp.bsr_matrix((out.cpu().numpy().reshape(-1, 3, 3), gi, gp), shape=(dev.n, dev.n)).tocsr()
-    B.sort_indices()
-    assert np.array_equal(B.indptr, A.indptr) and np.array_equal(B.indices, A.indices)
-    assert np.array_equal(B.data, A.data)
+    for precision, dtype in ((_lib.FEM_F64, torch.float64), (_lib.FEM_F32, torch.float32)):
+        out = torch.empty(dev.nnzb * 9, dtype=dtype, device='cuda')
+        check(dev.L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals), ptr(out), precision, stream_ptr()),
+              'fem_amg_sell_to_bcsr')
+        B = sp.bsr_matrix((out.cpu().numpy().astype(np.float64).reshape(-1, 3, 3), gi, gp),
+                          shape=(dev.n, dev.n)).tocsr()
+        B.sort_indices()
+        assert np.array_equal(B.indptr, A.indptr) and np.array_equal(B.indices, A.indices)
+        # FP32: each value rounded once from the FP64 matrix
+        ref = A.data if dtype == torch.float64 else A.data.astype(np.float32).astype(np.float64)
+        assert np.array_equal(B.data, ref)

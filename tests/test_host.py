@@ -36,7 +36,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.FemPattern) == 4 * 8 + 4 * 4 + 9 * 8
     assert ctypes.sizeof(_lib.FemPcgOptions) == 8 + 8 + 4 * 4 + 8 + 8
     assert ctypes.sizeof(_lib.FemPcgResult) == 16 + 16 + 16 + 8
-    assert ctypes.sizeof(_lib.FemBcsr) == 3 * 8 + 2 * 4 + 3 * 8
+    assert ctypes.sizeof(_lib.FemBcsr) == 3 * 8 + 4 * 4 + 3 * 8
     assert ctypes.sizeof(_lib.FemAmgLevel) == 8 + 8 + 3 * ctypes.sizeof(_lib.FemBcsr) + 8 + 8
     assert ctypes.sizeof(_lib.FemAmg) == 4 * 4 + 8 + 8 + 8
     assert ctypes.sizeof(_lib.FemAmgOptions) == 8 + 8 + 4 * 4 + 8
