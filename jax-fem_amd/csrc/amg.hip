@@ -758,6 +758,34 @@ __global__ __launch_bounds__(NT) void k_spgemm_row_big(BView X, BView Y, int32_t
   }
 }
 
+// n consecutive entries in their storage type, with 16-byte loads where n and
+// the entry size allow (p is then 16-byte aligned: blocks of 36 FP32 entries)
+template <int n, typename T>
+__device__ inline void load_raw(const T* __restrict__ p, T (&a)[n]) {
+  if constexpr (sizeof(T) == 4 && n % 4 == 0) {
+    const float4* p4 = reinterpret_cast<const float4*>(p);
+#pragma unroll
+    for (int k = 0; k < n / 4; ++k) {
+      const float4 v = p4[k];
+      a[4 * k] = v.x;
+      a[4 * k + 1] = v.y;
+      a[4 * k + 2] = v.z;
+      a[4 * k + 3] = v.w;
+    }
+  } else if constexpr (sizeof(T) == 8 && n % 2 == 0) {
+    const double2* p2 = reinterpret_cast<const double2*>(p);
+#pragma unroll
+    for (int k = 0; k < n / 2; ++k) {
+      const double2 v = p2[k];
+      a[2 * k] = v.x;
+      a[2 * k + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < n; ++k) a[k] = p[k];
+  }
+}
+
 // Load an R x C block (FP64 or FP32 storage) into FP64 registers.  Blocks whose
 // size is a multiple of 16 bytes (given 256-byte aligned bases: FP64 blocks but
 // 3x3 and 1x1, FP32 6x6 blocks) use 16-byte loads; FP32 blocks of an even
@@ -1133,7 +1161,7 @@ __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t*
 // contiguous block stream) times its 6-entry slice of x; the lanes' partial
 // rows meet in a wave reduction.
 template <int B, int MODE, typename T, typename TX = double, typename TY = double>
-__global__ __launch_bounds__(256) void k_bcsr_apply_blk(int64_t rows, const int32_t* __restrict__ rp,
+__global__ __launch_bounds__(256, 6) void k_bcsr_apply_blk(int64_t rows, const int32_t* __restrict__ rp,
                                                         const int32_t* __restrict__ ci, const T* __restrict__ v,
                                                         ApplyArgsT<TX, TY> a) {
   if (a.done && *a.done) return;
@@ -1144,22 +1172,29 @@ __global__ __launch_bounds__(256) void k_bcsr_apply_blk(int64_t rows, const int3
 #pragma unroll
   for (int i = 0; i < B; ++i) acc[i] = 0.0;
   const int k1 = rp[row + 1];
+#pragma unroll 1
   for (int k = rp[row] + lane; k < k1; k += 64) {
-    double blk[B][B], xc[B];
-    load_block<B, B, T>(v + (size_t)k * B * B, blk);
+    // the block stays in its storage type until each product (FP32 blocks: half the
+    // registers of an FP64 copy, so more waves stay resident)
+    T blk[B * B];
+    load_raw<B * B, T>(v + (size_t)k * B * B, blk);
     const TX* xp = a.x + (int64_t)ci[k] * B;
+    double xc[B];
 #pragma unroll
     for (int j = 0; j < B; ++j) xc[j] = double(xp[j]);
 #pragma unroll
     for (int i = 0; i < B; ++i)
 #pragma unroll
-      for (int j = 0; j < B; ++j) acc[i] = fma(blk[i][j], xc[j], acc[i]);
+      for (int j = 0; j < B; ++j) acc[i] = fma(double(blk[i * B + j]), xc[j], acc[i]);
   }
+  // every lane holds the (bitwise identical) wave sums; lane i finishes entry i of the row
+  double t = 0.0;
 #pragma unroll
-  for (int i = 0; i < B; ++i) acc[i] = wave_sum(acc[i]);
-  if (lane == 0)
-#pragma unroll
-    for (int i = 0; i < B; ++i) apply_epilogue<MODE>(a, row * B + i, acc[i]);
+  for (int i = 0; i < B; ++i) {
+    const double si = wave_sum(acc[i]);
+    t = lane == i ? si : t;
+  }
+  if (lane < B) apply_epilogue<MODE>(a, row * B + lane, t);
 }
 
 // Long block rows (R and the coarse operators, hundreds of (block, column) pairs):
@@ -1197,11 +1232,10 @@ template <int V, int MODE, typename T, bool DIFF = false, typename TX = double, 
 __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pat, const T* __restrict__ vals,
                                                                 ApplyArgsT<TX, TY> a,
                                                                 const int32_t* __restrict__ done) {
-  // FP32 residual and power-iteration products unroll the slots by 4 (124 -> 112 us on the
-  // 1.6M-dof cantilever); the Chebyshev step, with its larger epilogue, keeps 2, and so does
-  // FP16 storage (its residual: 74 us unrolled by 2, 78 us by 4)
+  // FP32 products unroll the slots by 4 (the residual: 124 -> 112 us on the 1.6M-dof
+  // cantilever); FP16 storage keeps 2 (its residual: 74 us unrolled by 2, 78 us by 4)
   constexpr int W = SliceCfg<T>::W;
-  constexpr int U = (sizeof(T) == 4 && MODE != AP_CHEB) ? 4 : SliceCfg<T>::U;
+  constexpr int U = sizeof(T) == 4 ? 4 : SliceCfg<T>::U;
   __shared__ double red[W][kSlice][V];
   if ((done && *done) || (a.done && *a.done)) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1217,13 +1251,17 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   double xr[V], acc[V];
 #pragma unroll
   for (int j = 0; j < V; ++j) xr[j] = (DIFF && real) ? double(a.x[row * V + j]) : 0.0;
-  // the epilogue's own-row operands are loaded before the matrix stream, so
-  // their latency overlaps it (wave 0 finishes the rows; spreading the rows'
-  // components over the waves measured no faster: 111.5 -> 113.2 us for the
-  // 1.6M-dof cantilever's FP32 Chebyshev step)
   constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
   // FP16 values hold D^-1 A (rows scaled by dinv): the epilogues undo or skip the scaling
   constexpr bool kScaled = sizeof(T) == 2;
+  sell_slice_partial<V, DIFF, W, U, T, decltype(xat), TX>(pat, vals, s, w, lane, row, xat, xr, acc);
+  if (w > 0) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
+  }
+  // wave 0 loads the epilogue's own-row operands after the matrix stream: held across
+  // it they cost 40 registers, which halves the waves resident per SIMD (the other
+  // slices' streams hide this latency)
   double e_rhs[V], e_dinv[V], e_d[V], e_x[V];
   if (w == 0 && real) {
 #pragma unroll
@@ -1234,11 +1272,6 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       e_d[i] = (MODE == AP_CHEB && a.c1 != 0.0) ? double(a.d[idx]) : 0.0;
       e_x[i] = (MODE == AP_CHEB || MODE == AP_PQ) ? double(a.x[idx]) : 0.0;
     }
-  }
-  sell_slice_partial<V, DIFF, W, U, T, decltype(xat), TX>(pat, vals, s, w, lane, row, xat, xr, acc);
-  if (w > 0) {
-#pragma unroll
-    for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
   }
   __syncthreads();
   if (w != 0) return;
