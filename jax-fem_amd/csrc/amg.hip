@@ -1104,22 +1104,44 @@ struct ApplyArgsT {
 };
 using ApplyArgs = ApplyArgsT<double, double>;
 
+// The epilogue's own-entry operands (loaded ahead of the product where a kernel can
+// overlap their latency with it) and the epilogue itself.
+struct EpiOperands {
+  double y, rhs, dinv, d, x;
+};
+
 template <int MODE, typename TX, typename TY>
-__device__ inline void apply_epilogue(const ApplyArgsT<TX, TY>& a, int64_t idx, double ax) {
+__device__ inline EpiOperands epilogue_load(const ApplyArgsT<TX, TY>& a, int64_t idx) {
+  EpiOperands e{0.0, 0.0, 0.0, 0.0, 0.0};
+  if (MODE == AP_ADD) e.y = double(a.y[idx]);
+  if (MODE == AP_RESID || MODE == AP_CHEB) e.rhs = a.rhs[idx];
+  if (MODE == AP_CHEB || MODE == AP_DSTORE) e.dinv = a.dinv[idx];
+  if (MODE == AP_CHEB && a.c1 != 0.0) e.d = double(a.d[idx]);
+  if (MODE == AP_CHEB) e.x = double(a.x[idx]);
+  return e;
+}
+
+template <int MODE, typename TX, typename TY>
+__device__ inline void epilogue_store(const ApplyArgsT<TX, TY>& a, int64_t idx, double ax, const EpiOperands& e) {
   if (MODE == AP_STORE || MODE == AP_PQ) {
     a.y[idx] = TY(ax);
   } else if (MODE == AP_ADD) {
-    a.y[idx] = TY(double(a.y[idx]) + ax);
+    a.y[idx] = TY(e.y + ax);
   } else if (MODE == AP_RESID) {
-    a.y[idx] = TY(a.rhs[idx] - ax);
+    a.y[idx] = TY(e.rhs - ax);
   } else if (MODE == AP_DSTORE) {
-    a.y[idx] = TY(a.dinv[idx] * ax);
+    a.y[idx] = TY(e.dinv * ax);
   } else {
-    const double r = a.dinv[idx] * (a.rhs[idx] - ax);
-    const double dn = a.c1 != 0.0 ? fma(a.c1, double(a.d[idx]), a.c2 * r) : a.c2 * r;
+    const double r = e.dinv * (e.rhs - ax);
+    const double dn = a.c1 != 0.0 ? fma(a.c1, e.d, a.c2 * r) : a.c2 * r;
     a.d[idx] = TX(dn);
-    a.y[idx] = TY(double(a.x[idx]) + dn);
+    a.y[idx] = TY(e.x + dn);
   }
+}
+
+template <int MODE, typename TX, typename TY>
+__device__ inline void apply_epilogue(const ApplyArgsT<TX, TY>& a, int64_t idx, double ax) {
+  epilogue_store<MODE>(a, idx, ax, epilogue_load<MODE>(a, idx));
 }
 
 // Block CSR operator, G lanes per block row: the lanes stride over the row's
@@ -1156,45 +1178,60 @@ __global__ __launch_bounds__(256) void k_bcsr_apply(int64_t rows, const int32_t*
   }
 }
 
-// Square coarse operators (6x6 blocks, tens of blocks per row): one wave per
-// block row, each lane a whole block (read with 16-byte loads from the row's
-// contiguous block stream) times its 6-entry slice of x; the lanes' partial
-// rows meet in a wave reduction.
-template <int B, int MODE, typename T, typename TX = double, typename TY = double>
+// Block CSR operator, a lane per block: G lanes per block row (G = 64: one
+// wave per row) stride over the row's blocks, each lane reading whole blocks
+// (16- or 8-byte loads from the row's contiguous block stream) times its
+// BC-entry slice of x.  The group's partial rows meet in a shuffle reduction
+// (every lane of the group then holds the bitwise identical sums) and lane i
+// of the group finishes entry i of the row.  A single epilogue operand (the
+// prolongation's y, the residual's rhs) is loaded before the block stream so
+// that its latency overlaps it; the Chebyshev step's four are loaded after it
+// (held across it they would spill at 6 waves per SIMD).
+template <int BR, int BC, int G, int MODE, typename T, typename TX = double, typename TY = double>
 __global__ __launch_bounds__(256, 6) void k_bcsr_apply_blk(int64_t rows, const int32_t* __restrict__ rp,
-                                                        const int32_t* __restrict__ ci, const T* __restrict__ v,
-                                                        ApplyArgsT<TX, TY> a) {
+                                                           const int32_t* __restrict__ ci, const T* __restrict__ v,
+                                                           ApplyArgsT<TX, TY> a) {
+  static_assert(G >= BR && G <= 64 && (G & (G - 1)) == 0, "G: a power of two, at least the block rows");
   if (a.done && *a.done) return;
-  const int lane = threadIdx.x & 63;
-  const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= rows) return;
-  double acc[B];
+  const int sub = threadIdx.x & (G - 1);
+  const int64_t row = ((int64_t)blockIdx.x * 256 + threadIdx.x) / G;
+  const bool live = row < rows;  // a group is live or dead as a whole: its shuffles stay inside it
+  const bool fin = live && sub < BR;
+  const int64_t idx = row * BR + sub;
+  constexpr bool kEarly = MODE == AP_ADD || MODE == AP_RESID;
+  EpiOperands e{0.0, 0.0, 0.0, 0.0, 0.0};
+  if (kEarly && fin) e = epilogue_load<MODE>(a, idx);
+  double acc[BR];
 #pragma unroll
-  for (int i = 0; i < B; ++i) acc[i] = 0.0;
-  const int k1 = rp[row + 1];
+  for (int i = 0; i < BR; ++i) acc[i] = 0.0;
+  if (live) {
+    const int k1 = rp[row + 1];
 #pragma unroll 1
-  for (int k = rp[row] + lane; k < k1; k += 64) {
-    // the block stays in its storage type until each product (FP32 blocks: half the
-    // registers of an FP64 copy, so more waves stay resident)
-    T blk[B * B];
-    load_raw<B * B, T>(v + (size_t)k * B * B, blk);
-    const TX* xp = a.x + (int64_t)ci[k] * B;
-    double xc[B];
+    for (int k = rp[row] + sub; k < k1; k += G) {
+      // the block stays in its storage type until each product (FP32 blocks: half the
+      // registers of an FP64 copy, so more waves stay resident)
+      T blk[BR * BC];
+      load_raw<BR * BC, T>(v + (size_t)k * BR * BC, blk);
+      const TX* xp = a.x + (int64_t)ci[k] * BC;
+      double xc[BC];
 #pragma unroll
-    for (int j = 0; j < B; ++j) xc[j] = double(xp[j]);
+      for (int j = 0; j < BC; ++j) xc[j] = double(xp[j]);
 #pragma unroll
-    for (int i = 0; i < B; ++i)
+      for (int i = 0; i < BR; ++i)
 #pragma unroll
-      for (int j = 0; j < B; ++j) acc[i] = fma(double(blk[i * B + j]), xc[j], acc[i]);
+        for (int j = 0; j < BC; ++j) acc[i] = fma(double(blk[i * BC + j]), xc[j], acc[i]);
+    }
   }
-  // every lane holds the (bitwise identical) wave sums; lane i finishes entry i of the row
+  if (!kEarly && fin) e = epilogue_load<MODE>(a, idx);
   double t = 0.0;
 #pragma unroll
-  for (int i = 0; i < B; ++i) {
-    const double si = wave_sum(acc[i]);
-    t = lane == i ? si : t;
+  for (int i = 0; i < BR; ++i) {
+    double si = acc[i];
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) si += __shfl_xor(si, off, 64);
+    t = sub == i ? si : t;
   }
-  if (lane < B) apply_epilogue<MODE>(a, row * B + lane, t);
+  if (fin) epilogue_store<MODE>(a, idx, t, e);
 }
 
 // Long block rows (R and the coarse operators, hundreds of (block, column) pairs):
@@ -1203,9 +1240,12 @@ template <int BR, int BC, int MODE, typename T, typename TX = double, typename T
 __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int32_t* __restrict__ rp,
                                                         const int32_t* __restrict__ ci, const T* __restrict__ v,
                                                         ApplyArgsT<TX, TY> a) {
-  __shared__ double smem[4];
+  __shared__ double smem[4][BR];
   if (a.done && *a.done) return;
   const int64_t row = blockIdx.x;
+  const int w = threadIdx.x >> 6;
+  EpiOperands e{0.0, 0.0, 0.0, 0.0, 0.0};
+  if (threadIdx.x < BR) e = epilogue_load<MODE>(a, row * BR + threadIdx.x);
   double acc[BR];
 #pragma unroll
   for (int i = 0; i < BR; ++i) acc[i] = 0.0;
@@ -1218,10 +1258,16 @@ __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int3
 #pragma unroll
     for (int i = 0; i < BR; ++i) acc[i] = fma(double(vb[i * BC]), xv, acc[i]);
   }
+  // the four waves' sums of all block rows meet in LDS behind one barrier
 #pragma unroll
   for (int i = 0; i < BR; ++i) {
-    const double t = block_sum_256(acc[i], smem);
-    if (threadIdx.x == 0) apply_epilogue<MODE>(a, row * BR + i, t);
+    const double si = wave_sum(acc[i]);
+    if ((threadIdx.x & 63) == 0) smem[w][i] = si;
+  }
+  __syncthreads();
+  if (threadIdx.x < BR) {
+    const int i = threadIdx.x;
+    epilogue_store<MODE>(a, row * BR + i, (smem[0][i] + smem[1][i]) + (smem[2][i] + smem[3][i]), e);
   }
 }
 
@@ -1458,51 +1504,65 @@ __global__ __launch_bounds__(256) void k_cg_rr(const double* __restrict__ part, 
 }
 
 // ------------------------------------------------------------ dispatch
-// (block, column) pairs per row above which a whole workgroup takes one row: on the
-// 1.6M-dof cantilever the level-1 restriction (about 2,200 pairs per row) drops from
-// 28.8 to 11.3 us, while the level-1 operator rows (270 pairs) are faster with one wave
-// (19.4 against 30.8 us)
-constexpr double kRowThreshold = 400.0;
+// Launch shapes of the block CSR operators, from the mean blocks per row:
+//   - scalar (1x1) hierarchies: G lanes strided over a row's entries (k_bcsr_apply);
+//   - long rows of few block rows (the restriction to and the operator of a small
+//     coarse level): one 256-thread workgroup per row (k_bcsr_apply_row), since one
+//     wave per row would leave most of the chip idle;
+//   - rows of more than kWaveBlocks blocks (the level-1 operator: 45, the level-1
+//     restriction: 195 on the 1.6M-dof cantilever): one wave per row, a lane per block;
+//   - short rows (the prolongations: 4 to 5 blocks): kSmallG lanes per row (at least
+//     the block's row count), a lane per block.
+constexpr double kWaveBlocks = 12.0;
+constexpr double kRowBlocks = 64.0;
+constexpr int64_t kRowKernelRows = 4096;
+constexpr int kSmallG = 4;
+
+template <int BR, int BC, int MODE, typename T, typename TX, typename TY>
+void bcsr_apply_shape(const fem_bcsr_t& M, const T* v, const ApplyArgsT<TX, TY>& a, hipStream_t s) {
+  const double bpr = double(M.nnzb) / double(M.rows);
+  constexpr int GS = kSmallG >= BR ? kSmallG : 8;
+  if (bpr > kRowBlocks && M.rows < kRowKernelRows)
+    hipLaunchKernelGGL((k_bcsr_apply_row<BR, BC, MODE, T, TX, TY>), dim3((unsigned)M.rows), dim3(256), 0, s, M.rows,
+                       M.row_ptr, M.col_idx, v, a);
+  else if (bpr > kWaveBlocks)
+    hipLaunchKernelGGL((k_bcsr_apply_blk<BR, BC, 64, MODE, T, TX, TY>), dim3((unsigned)ceil_div(M.rows * 64, 256)),
+                       dim3(256), 0, s, M.rows, M.row_ptr, M.col_idx, v, a);
+  else
+    hipLaunchKernelGGL((k_bcsr_apply_blk<BR, BC, GS, MODE, T, TX, TY>), dim3((unsigned)ceil_div(M.rows * GS, 256)),
+                       dim3(256), 0, s, M.rows, M.row_ptr, M.col_idx, v, a);
+}
 
 template <int MODE, typename T, typename TX = double, typename TY = double>
 int bcsr_apply(const fem_bcsr_t& M, const ApplyArgsT<TX, TY>& a, hipStream_t s) {
   if (M.rows == 0) return FEM_OK;
-  // lanes per block row from the mean number of (block, column) pairs in a row: a
-  // sweep of 1..64 lanes on the level-0 prolongation of the 1.6M-dof cantilever (26
-  // pairs per row) gave 277 / 122 / 80 / 70 / 87 / 121 / 210 us, the long rows of R
-  // and the coarse operators (200-600 pairs) take a whole wave
-  const double per_row = M.rows ? double(M.nnzb) * M.bc / double(M.rows) : 0.0;
-  const int G = per_row > kRowThreshold ? 256 : (per_row > 48.0 ? 64 : 8);
-  const unsigned grid = (unsigned)(G == 256 ? M.rows : ceil_div(M.rows * G, 256));
   const T* v = static_cast<const T*>(M.vals);
-  if (M.br == 6 && M.bc == 6 && G == 64) {   // square coarse operators: a lane per block
-    hipLaunchKernelGGL((k_bcsr_apply_blk<6, MODE, T, TX, TY>), dim3((unsigned)ceil_div(M.rows, 4)), dim3(256), 0, s,
-                       M.rows, M.row_ptr, M.col_idx, v, a);
-    FEM_LAUNCHED();
-    return FEM_OK;
+  if (M.br == 1 && M.bc == 1) {
+    // lanes per row from the mean entries per row (a whole workgroup above 400)
+    const double per_row = double(M.nnzb) / double(M.rows);
+    if (per_row > 400.0)
+      hipLaunchKernelGGL((k_bcsr_apply_row<1, 1, MODE, T, TX, TY>), dim3((unsigned)M.rows), dim3(256), 0, s, M.rows,
+                         M.row_ptr, M.col_idx, v, a);
+    else if (per_row > 48.0)
+      hipLaunchKernelGGL((k_bcsr_apply<1, 1, 64, MODE, T, TX, TY>), dim3((unsigned)ceil_div(M.rows * 64, 256)),
+                         dim3(256), 0, s, M.rows, M.row_ptr, M.col_idx, v, a);
+    else
+      hipLaunchKernelGGL((k_bcsr_apply<1, 1, 8, MODE, T, TX, TY>), dim3((unsigned)ceil_div(M.rows * 8, 256)),
+                         dim3(256), 0, s, M.rows, M.row_ptr, M.col_idx, v, a);
+  } else if (M.br == 3 && M.bc == 3) {
+    bcsr_apply_shape<3, 3, MODE, T>(M, v, a, s);
+  } else if (M.br == 6 && M.bc == 6) {
+    bcsr_apply_shape<6, 6, MODE, T>(M, v, a, s);
+  } else if (M.br == 3 && M.bc == 6) {
+    bcsr_apply_shape<3, 6, MODE, T>(M, v, a, s);
+  } else if (M.br == 6 && M.bc == 3) {
+    bcsr_apply_shape<6, 3, MODE, T>(M, v, a, s);
+  } else {
+    set_error("block CSR operator: unsupported block shape %d x %d", M.br, M.bc);
+    return FEM_ERR_BAD_ARG;
   }
-#define FEM_AMG_APPLY(BR, BC)                                                                                  \
-  if (M.br == BR && M.bc == BC) {                                                                              \
-    if (G == 256)                                                                                              \
-      hipLaunchKernelGGL((k_bcsr_apply_row<BR, BC, MODE, T, TX, TY>), dim3(grid), dim3(256), 0, s, M.rows,      \
-                         M.row_ptr, M.col_idx, v, a);                                                          \
-    else if (G == 64)                                                                                          \
-      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 64, MODE, T, TX, TY>), dim3(grid), dim3(256), 0, s, M.rows,      \
-                         M.row_ptr, M.col_idx, v, a);                                                          \
-    else                                                                                                       \
-      hipLaunchKernelGGL((k_bcsr_apply<BR, BC, 8, MODE, T, TX, TY>), dim3(grid), dim3(256), 0, s, M.rows,       \
-                         M.row_ptr, M.col_idx, v, a);                                                          \
-    FEM_LAUNCHED();                                                                                            \
-    return FEM_OK;                                                                                             \
-  }
-  FEM_AMG_APPLY(1, 1)
-  FEM_AMG_APPLY(3, 3)
-  FEM_AMG_APPLY(6, 6)
-  FEM_AMG_APPLY(3, 6)
-  FEM_AMG_APPLY(6, 3)
-#undef FEM_AMG_APPLY
-  set_error("block CSR operator: unsupported block shape %d x %d", M.br, M.bc);
-  return FEM_ERR_BAD_ARG;
+  FEM_LAUNCHED();
+  return FEM_OK;
 }
 
 template <int MODE, typename T, bool DIFF = false, typename TX = double, typename TY = double>
