@@ -527,6 +527,10 @@ int fem_amg_sell_to_f32_diff(const fem_pattern_t* pat, const double* sell_vals, 
  * matrix; the pattern's row_ptr/col_idx are the CSR structure. */
 int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, void* bcsr_vals,
                          int32_t precision, void* stream);
+/* Both FP32 copies above (block CSR and difference-form SELL) in one pass over
+ * the FP64 SELL values: the level-0 inputs of an FP32 hierarchy. */
+int fem_amg_sell_to_f32_copies(const fem_pattern_t* pat, const double* sell_vals, float* bcsr_vals,
+                               float* diff_vals, void* stream);
 /* dinv = 1 / diag(A) (FP64 A); on levels >= 1 (fix_zero = 1) a zero
  * diagonal entry (an empty coarse row) is first set to 1 in A. */
 int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream);

@@ -273,13 +273,18 @@ __global__ void k_bc_mask_set(int64_t n_bc, const int32_t* __restrict__ dofs, ui
 // ------------------------------------------------------ matrix helpers
 // One block per slice: KC slots of the slice's 64 rows are read coalesced into
 // LDS, then each row's KC*V*V entries (contiguous in the block CSR) are stored
-// as one contiguous run, so the stores fill whole cache lines.
-template <int V, typename TO>
+// as one contiguous run, so the stores fill whole cache lines.  DIFF: the same
+// pass also writes the FP32 difference-form SELL copy (dout): off-diagonal
+// entries rounded as they are read, and each row's FP64 row sums (summed over
+// the slots in slot order by the thread that owns the (row, entry) pair)
+// rounded once into the diagonal slot.
+template <int V, typename TO, bool DIFF = false>
 __global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv,
-                                                      TO* __restrict__ out) {
+                                                      TO* __restrict__ out, float* __restrict__ dout) {
   constexpr int VV = V * V, KC = 4;
+  constexpr int NOWN = (kSlice * VV + 255) / 256;  // (row, entry) sums owned per thread
   __shared__ double tile[kSlice][KC * VV + 1];
-  __shared__ int s_dst[kSlice], s_len[kSlice];
+  __shared__ int s_dst[kSlice], s_len[kSlice], s_ds[kSlice];
   const int64_t s = blockIdx.x;
   const int base = pat.slice_ptr[s];
   const int L = (pat.slice_ptr[s + 1] - base) >> 6;
@@ -287,19 +292,47 @@ __global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const d
     const int row = pat.sell_row[s * kSlice + threadIdx.x];
     s_dst[threadIdx.x] = row >= 0 ? pat.row_ptr[row] : 0;
     s_len[threadIdx.x] = row >= 0 ? pat.row_ptr[row + 1] - pat.row_ptr[row] : 0;
+    s_ds[threadIdx.x] = row >= 0 ? pat.diag_slot[row] : -1;
   }
   const double* src = sv + (size_t)base * VV;
+  float* dst = DIFF ? dout + (size_t)base * VV : nullptr;
+  double rs[NOWN];
+#pragma unroll
+  for (int o = 0; o < NOWN; ++o) rs[o] = 0.0;
   for (int k0 = 0; k0 < L; k0 += KC) {
     const int nc = min(KC, L - k0) * VV;  // entries per row in this chunk
     __syncthreads();                      // row data ready, previous tile consumed
     for (int f = threadIdx.x; f < nc * kSlice; f += 256) {
       const int lane = f & (kSlice - 1), c = f >> 6;  // c = slot-in-chunk * VV + entry
-      tile[lane][c] = src[((size_t)k0 * VV + c) * kSlice + lane];
+      const size_t at = ((size_t)k0 * VV + c) * kSlice + lane;
+      const double x = src[at];
+      tile[lane][c] = x;
+      if (DIFF && k0 + c / VV != s_ds[lane]) dst[at] = float(x);
     }
     __syncthreads();
     for (int f = threadIdx.x; f < nc * kSlice; f += 256) {
       const int lane = f / nc, c = f - lane * nc;
       if (k0 + c / VV < s_len[lane]) out[((size_t)s_dst[lane] + k0) * VV + c] = TO(tile[lane][c]);
+    }
+    if (DIFF) {
+#pragma unroll
+      for (int o = 0; o < NOWN; ++o) {
+        const int p = threadIdx.x + 256 * o;
+        if (p < kSlice * VV) {
+          const int lane = p & (kSlice - 1), v = p >> 6;
+          for (int kk = 0; kk < nc / VV; ++kk) rs[o] += tile[lane][kk * VV + v];
+        }
+      }
+    }
+  }
+  if (DIFF) {
+#pragma unroll
+    for (int o = 0; o < NOWN; ++o) {
+      const int p = threadIdx.x + 256 * o;
+      if (p < kSlice * VV) {
+        const int lane = p & (kSlice - 1), v = p >> 6;
+        if (s_ds[lane] >= 0) dst[((size_t)s_ds[lane] * VV + v) * kSlice + lane] = float(rs[o]);
+      }
     }
   }
 }
@@ -1973,14 +2006,30 @@ int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, void
     using TO = decltype(t);
     TO* o = static_cast<TO*>(bcsr_vals);
     if (pat->vec == 3)
-      hipLaunchKernelGGL((k_sell_to_bcsr<3, TO>), g, dim3(256), 0, s, *pat, sell_vals, o);
+      hipLaunchKernelGGL((k_sell_to_bcsr<3, TO>), g, dim3(256), 0, s, *pat, sell_vals, o, (float*)nullptr);
     else
-      hipLaunchKernelGGL((k_sell_to_bcsr<1, TO>), g, dim3(256), 0, s, *pat, sell_vals, o);
+      hipLaunchKernelGGL((k_sell_to_bcsr<1, TO>), g, dim3(256), 0, s, *pat, sell_vals, o, (float*)nullptr);
   };
   if (precision == FEM_F32)
     go(float{});
   else
     go(double{});
+  FEM_LAUNCHED();
+  return FEM_OK;
+}
+
+int fem_amg_sell_to_f32_copies(const fem_pattern_t* pat, const double* sell_vals, float* bcsr_vals, float* diff_vals,
+                               void* stream) {
+  FEM_CHECK_ARG(pat && sell_vals && bcsr_vals && diff_vals, "fem_amg_sell_to_f32_copies: NULL argument");
+  hipStream_t s = as_stream(stream);
+  if (pat->num_rows == 0) return FEM_OK;
+  FEM_CHECK_ARG(pat->num_slices == ceil_div(pat->num_rows, kSlice), "fem_amg_sell_to_f32_copies: num_slices mismatch");
+  FEM_CHECK_ARG(pat->vec == 1 || pat->vec == 3, "fem_amg_sell_to_f32_copies: vec must be 1 or 3");
+  const dim3 g((unsigned)pat->num_slices);
+  if (pat->vec == 3)
+    hipLaunchKernelGGL((k_sell_to_bcsr<3, float, true>), g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals, diff_vals);
+  else
+    hipLaunchKernelGGL((k_sell_to_bcsr<1, float, true>), g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals, diff_vals);
   FEM_LAUNCHED();
   return FEM_OK;
 }

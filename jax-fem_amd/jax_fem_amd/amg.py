@@ -164,15 +164,19 @@ class DeviceAMG:
                                   ptr(ws), stream_ptr()), 'fem_amg_lmax')
         return float(lam.value)
 
-    def _sell_diff32(self, vals64):
-        """FP32 difference-form copy of the FP64 SELL matrix (V-cycle level 0 and outer CG)."""
+    def _sell_diff32_buffer(self, vals64):
         dev = self.dev
         n = dev.sell_blocks * dev.V * dev.V
         out = self._f32.get('sell')
         if out is None or out.numel() != n:
             out = torch.empty(n, dtype=torch.float32, device=vals64.device)
             self._f32['sell'] = out
-        check(self.L.fem_amg_sell_to_f32_diff(dev.pref, ptr(vals64), ptr(out), stream_ptr()),
+        return out
+
+    def _sell_diff32(self, vals64):
+        """FP32 difference-form copy of the FP64 SELL matrix (V-cycle level 0 and outer CG)."""
+        out = self._sell_diff32_buffer(vals64)
+        check(self.L.fem_amg_sell_to_f32_diff(self.dev.pref, ptr(vals64), ptr(out), stream_ptr()),
               'fem_amg_sell_to_f32_diff')
         return out
 
@@ -277,9 +281,17 @@ class DeviceAMG:
         self._values(coarse, dt if levels else torch.float64)
         A0 = levels[0]['A'] if levels else coarse   # no levels: the matrix is the coarsest level
         prec0 = _lib.FEM_F32 if A0.vals.dtype == torch.float32 else _lib.FEM_F64
-        check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), prec0, stream_ptr()), 'fem_amg_sell_to_bcsr')
         f16 = level0 == 'fp16' and precision == _lib.FEM_F32 and bool(levels)
-        sv32 = self._sell_diff32(vals64) if (precision == _lib.FEM_F32 and levels and not f16) else None
+        sv32 = None
+        if precision == _lib.FEM_F32 and levels and not f16:
+            # level 0's FP32 block-CSR copy (the Galerkin products' input) and its FP32
+            # difference-form SELL copy (the V-cycle's level-0 matrix) in one pass
+            sv32 = self._sell_diff32_buffer(vals64)
+            check(L.fem_amg_sell_to_f32_copies(dev.pref, ptr(vals64), ptr(A0.vals), ptr(sv32), stream_ptr()),
+                  'fem_amg_sell_to_f32_copies')
+        else:
+            check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), prec0, stream_ptr()),
+                  'fem_amg_sell_to_bcsr')
         for l, lv in enumerate(levels):
             A = lv['A']
             lv['dinv'] = self._dinv(A, fix=l > 0)

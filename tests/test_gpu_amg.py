@@ -276,3 +276,36 @@ def test_sell_to_bcsr_on_a_length_sorted_pattern(gpu):
         # FP32: each value rounded once from the FP64 matrix
         ref = A.data if dtype == torch.float64 else A.data.astype(np.float32).astype(np.float64)
         assert np.array_equal(B.data, ref)
+
+
+@pytest.mark.parametrize('mesh', ['hex8', 'hex27_sorted', 'poisson'])
+def test_fused_f32_copies_match_the_separate_copies(gpu, mesh):
+    # fem_amg_sell_to_f32_copies writes the FP32 block-CSR copy and the FP32 difference-form
+    # SELL copy in one pass: both bit for bit the outputs of the separate copy kernels
+    # (padding lanes, length-sorted rows and the vec-1 pattern included)
+    from jax_fem_amd._lib import check, ptr, stream_ptr
+    if mesh == 'hex27_sorted':
+        from jax_fem_amd.generate_mesh import Mesh, box_mesh_hex27
+        m = box_mesh_hex27(4, 3, 3, 1., 1., 1.)
+        clamp = [[lambda x: isclose(x[2], 0.)] * 3, [0, 1, 2], [lambda x: 0.] * 3]
+        p = LoadedElastic(Mesh(m.points, m.cells_dict['hexahedron27']), vec=3, dim=3, ele_type='HEX27',
+                          quadrature_order=4, dirichlet_bc_info=clamp, location_fns=[lambda x: isclose(x[2], 1.)])
+    else:
+        p = _elastic(6) if mesh == 'hex8' else _poisson(6)
+    dev = _assembled(p)[0]
+    if mesh == 'hex27_sorted':
+        assert dev.sell_sorted
+    vals = dev.current[2]
+    V = dev.V
+    bcsr = torch.empty(dev.nnzb * V * V, dtype=torch.float32, device='cuda')
+    diff = torch.empty(dev.sell_blocks * V * V, dtype=torch.float32, device='cuda')
+    check(dev.L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals), ptr(bcsr), _lib.FEM_F32, stream_ptr()),
+          'fem_amg_sell_to_bcsr')
+    check(dev.L.fem_amg_sell_to_f32_diff(dev.pref, ptr(vals), ptr(diff), stream_ptr()), 'fem_amg_sell_to_f32_diff')
+    bcsr2 = torch.full_like(bcsr, float('nan'))
+    diff2 = torch.full_like(diff, float('nan'))
+    check(dev.L.fem_amg_sell_to_f32_copies(dev.pref, ptr(vals), ptr(bcsr2), ptr(diff2), stream_ptr()),
+          'fem_amg_sell_to_f32_copies')
+    torch.cuda.synchronize()
+    assert torch.equal(bcsr2.view(torch.int32), bcsr.view(torch.int32))
+    assert torch.equal(diff2.view(torch.int32), diff.view(torch.int32))
