@@ -871,23 +871,56 @@ __global__ __launch_bounds__(256) void k_spgemm_numeric(int64_t nb, const int32_
     for (int i = 0; i < XR; ++i)
 #pragma unroll
       for (int j = 0; j < YC; ++j) acc[i][j] = 0.0;
-    // several products' loads in flight per thread (the 6x3 x 3x6 case is
-    // register-bound and runs best without)
-    constexpr int kUnroll = (XR == 6 && XC == 3) ? 1 : 4;
+    const int q0 = plan_ptr[b], q1 = plan_ptr[b + 1];
+    const int2* pl = reinterpret_cast<const int2*>(plan);
+    if constexpr (sizeof(T) == 4) {
+      // FP32 blocks, software-pipelined: NP products' blocks are issued together
+      // and the next NP plan entries are fetched while they are multiplied (the
+      // compiler's own unrolling keeps about one product in flight); products
+      // are added in plan order.  NP = 1 where the accumulator alone takes 72
+      // registers (6-row outputs)
+      constexpr int NP = XR == 3 ? 2 : 1;
+      int2 e[NP];
+#pragma unroll
+      for (int u = 0; u < NP; ++u) e[u] = q0 + u < q1 ? pl[q0 + u] : make_int2(0, 0);
+      for (int q = q0; q < q1; q += NP) {
+        T xa[NP][XR * XC], ya[NP][XC * YC];
+#pragma unroll
+        for (int u = 0; u < NP; ++u)
+          if (u == 0 || q + u < q1) {
+            load_raw<XR * XC, T>(xv + (size_t)e[u].x * XR * XC, xa[u]);
+            load_raw<XC * YC, T>(yv + (size_t)e[u].y * XC * YC, ya[u]);
+          }
+#pragma unroll
+        for (int u = 0; u < NP; ++u) e[u] = q + NP + u < q1 ? pl[q + NP + u] : make_int2(0, 0);
+#pragma unroll
+        for (int u = 0; u < NP; ++u)
+          if (u == 0 || q + u < q1)
+#pragma unroll
+            for (int i = 0; i < XR; ++i)
+#pragma unroll
+              for (int j = 0; j < YC; ++j)
+#pragma unroll
+                for (int m = 0; m < XC; ++m)
+                  acc[i][j] = fma(double(xa[u][i * XC + m]), double(ya[u][m * YC + j]), acc[i][j]);
+      }
+    } else {
+      // several products' loads in flight per thread (the 6x3 x 3x6 case is
+      // register-bound and runs best without)
+      constexpr int kUnroll = (XR == 6 && XC == 3) ? 1 : 4;
 #pragma unroll kUnroll
-    for (int q = plan_ptr[b]; q < plan_ptr[b + 1]; ++q) {
-      const int2 ef = reinterpret_cast<const int2*>(plan)[q];
-      const T* x = xv + (size_t)ef.x * XR * XC;
-      const T* y = yv + (size_t)ef.y * XC * YC;
-      double xa[XR][XC], ya[XC][YC];
-      load_block<XR, XC, T>(x, xa);
-      load_block<XC, YC, T>(y, ya);
+      for (int q = q0; q < q1; ++q) {
+        const int2 ef = pl[q];
+        double xa[XR][XC], ya[XC][YC];
+        load_block<XR, XC, T>(xv + (size_t)ef.x * XR * XC, xa);
+        load_block<XC, YC, T>(yv + (size_t)ef.y * XC * YC, ya);
 #pragma unroll
-      for (int i = 0; i < XR; ++i)
+        for (int i = 0; i < XR; ++i)
 #pragma unroll
-        for (int j = 0; j < YC; ++j)
+          for (int j = 0; j < YC; ++j)
 #pragma unroll
-          for (int m = 0; m < XC; ++m) acc[i][j] = fma(xa[i][m], ya[m][j], acc[i][j]);
+            for (int m = 0; m < XC; ++m) acc[i][j] = fma(xa[i][m], ya[m][j], acc[i][j]);
+      }
     }
     T* c = cv + (size_t)b * XR * YC;
 #pragma unroll
