@@ -528,9 +528,11 @@ int fem_amg_sell_to_f32_diff(const fem_pattern_t* pat, const double* sell_vals, 
 int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, void* bcsr_vals,
                          int32_t precision, void* stream);
 /* Both FP32 copies above (block CSR and difference-form SELL) in one pass over
- * the FP64 SELL values: the level-0 inputs of an FP32 hierarchy. */
+ * the FP64 SELL values: the level-0 inputs of an FP32 hierarchy.  dinv (n, may
+ * be NULL): the inverse diagonal of the block-CSR copy, as fem_amg_dinv forms
+ * it (1 / the rounded diagonal entries, 0 for a zero entry). */
 int fem_amg_sell_to_f32_copies(const fem_pattern_t* pat, const double* sell_vals, float* bcsr_vals,
-                               float* diff_vals, void* stream);
+                               float* diff_vals, double* dinv, void* stream);
 /* dinv = 1 / diag(A) (FP64 A); on levels >= 1 (fix_zero = 1) a zero
  * diagonal entry (an empty coarse row) is first set to 1 in A. */
 int fem_amg_dinv(const fem_bcsr_t* A, int32_t fix_zero, double* dinv, void* stream);

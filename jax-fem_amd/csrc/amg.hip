@@ -277,28 +277,31 @@ __global__ void k_bc_mask_set(int64_t n_bc, const int32_t* __restrict__ dofs, ui
 // pass also writes the FP32 difference-form SELL copy (dout): off-diagonal
 // entries rounded as they are read, and each row's FP64 row sums (summed over
 // the slots in slot order by the thread that owns the (row, entry) pair)
-// rounded once into the diagonal slot.
+// rounded once into the diagonal slot; with dinv, also 1 / the rounded diagonal
+// entries (0 for a zero entry), as fem_amg_dinv forms them from the copy.
 template <int V, typename TO, bool DIFF = false>
 __global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const double* __restrict__ sv,
-                                                      TO* __restrict__ out, float* __restrict__ dout) {
+                                                      TO* __restrict__ out, float* __restrict__ dout,
+                                                      double* __restrict__ dinv) {
   constexpr int VV = V * V, KC = 4;
   constexpr int NOWN = (kSlice * VV + 255) / 256;  // (row, entry) sums owned per thread
   __shared__ double tile[kSlice][KC * VV + 1];
-  __shared__ int s_dst[kSlice], s_len[kSlice], s_ds[kSlice];
+  __shared__ int s_dst[kSlice], s_len[kSlice], s_ds[kSlice], s_row[kSlice];
   const int64_t s = blockIdx.x;
   const int base = pat.slice_ptr[s];
   const int L = (pat.slice_ptr[s + 1] - base) >> 6;
   if (threadIdx.x < kSlice) {
     const int row = pat.sell_row[s * kSlice + threadIdx.x];
+    s_row[threadIdx.x] = row;
     s_dst[threadIdx.x] = row >= 0 ? pat.row_ptr[row] : 0;
     s_len[threadIdx.x] = row >= 0 ? pat.row_ptr[row + 1] - pat.row_ptr[row] : 0;
     s_ds[threadIdx.x] = row >= 0 ? pat.diag_slot[row] : -1;
   }
   const double* src = sv + (size_t)base * VV;
   float* dst = DIFF ? dout + (size_t)base * VV : nullptr;
-  double rs[NOWN];
+  double rs[NOWN], dg[NOWN];  // row sums; diagonal entries (of the owned pairs on the block diagonal)
 #pragma unroll
-  for (int o = 0; o < NOWN; ++o) rs[o] = 0.0;
+  for (int o = 0; o < NOWN; ++o) rs[o] = dg[o] = 0.0;
   for (int k0 = 0; k0 < L; k0 += KC) {
     const int nc = min(KC, L - k0) * VV;  // entries per row in this chunk
     __syncthreads();                      // row data ready, previous tile consumed
@@ -321,6 +324,8 @@ __global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const d
         if (p < kSlice * VV) {
           const int lane = p & (kSlice - 1), v = p >> 6;
           for (int kk = 0; kk < nc / VV; ++kk) rs[o] += tile[lane][kk * VV + v];
+          const int kd = s_ds[lane] - k0;
+          if (v % (V + 1) == 0 && kd >= 0 && kd < nc / VV) dg[o] = tile[lane][kd * VV + v];
         }
       }
     }
@@ -332,6 +337,10 @@ __global__ __launch_bounds__(256) void k_sell_to_bcsr(fem_pattern_t pat, const d
       if (p < kSlice * VV) {
         const int lane = p & (kSlice - 1), v = p >> 6;
         if (s_ds[lane] >= 0) dst[((size_t)s_ds[lane] * VV + v) * kSlice + lane] = float(rs[o]);
+        if (dinv && v % (V + 1) == 0 && s_row[lane] >= 0) {
+          const double d = double(TO(dg[o]));
+          dinv[(int64_t)s_row[lane] * V + v / (V + 1)] = d != 0.0 ? 1.0 / d : 0.0;
+        }
       }
     }
   }
@@ -1158,6 +1167,39 @@ __global__ void k_gj_step(int64_t n, int64_t k, const double* __restrict__ src, 
       v = src[e] - src[i * n + k] * src[k * n + j] * d;
     dst[e] = v;
   }
+}
+
+// The whole Gauss-Jordan elimination of a small dense matrix (n <= kGjSmall) in one
+// workgroup: the matrix in LDS, one barrier per pivot, each step the arithmetic of
+// k_gj_step (so the inverse is the same bit for bit) without a launch per pivot.
+constexpr int kGjSmall = 64;
+__global__ __launch_bounds__(1024) void k_gj_small(int n, double* __restrict__ a) {
+  __shared__ double m[2][kGjSmall * kGjSmall];
+  const int nn = n * n;
+  for (int e = threadIdx.x; e < nn; e += 1024) m[0][e] = a[e];
+  __syncthreads();
+  int c = 0;
+  for (int k = 0; k < n; ++k) {
+    const double* src = m[c];
+    double* dst = m[c ^ 1];
+    const double d = 1.0 / src[k * n + k];
+    for (int e = threadIdx.x; e < nn; e += 1024) {
+      const int i = e / n, j = e % n;
+      double v;
+      if (i == k && j == k)
+        v = d;
+      else if (i == k)
+        v = src[k * n + j] * d;
+      else if (j == k)
+        v = -src[i * n + k] * d;
+      else
+        v = src[e] - src[i * n + k] * src[k * n + j] * d;
+      dst[e] = v;
+    }
+    __syncthreads();
+    c ^= 1;
+  }
+  for (int e = threadIdx.x; e < nn; e += 1024) a[e] = m[c][e];
 }
 
 __global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -2073,9 +2115,11 @@ int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, void
     using TO = decltype(t);
     TO* o = static_cast<TO*>(bcsr_vals);
     if (pat->vec == 3)
-      hipLaunchKernelGGL((k_sell_to_bcsr<3, TO>), g, dim3(256), 0, s, *pat, sell_vals, o, (float*)nullptr);
+      hipLaunchKernelGGL((k_sell_to_bcsr<3, TO>), g, dim3(256), 0, s, *pat, sell_vals, o, (float*)nullptr,
+                         (double*)nullptr);
     else
-      hipLaunchKernelGGL((k_sell_to_bcsr<1, TO>), g, dim3(256), 0, s, *pat, sell_vals, o, (float*)nullptr);
+      hipLaunchKernelGGL((k_sell_to_bcsr<1, TO>), g, dim3(256), 0, s, *pat, sell_vals, o, (float*)nullptr,
+                         (double*)nullptr);
   };
   if (precision == FEM_F32)
     go(float{});
@@ -2086,7 +2130,7 @@ int fem_amg_sell_to_bcsr(const fem_pattern_t* pat, const double* sell_vals, void
 }
 
 int fem_amg_sell_to_f32_copies(const fem_pattern_t* pat, const double* sell_vals, float* bcsr_vals, float* diff_vals,
-                               void* stream) {
+                               double* dinv, void* stream) {
   FEM_CHECK_ARG(pat && sell_vals && bcsr_vals && diff_vals, "fem_amg_sell_to_f32_copies: NULL argument");
   hipStream_t s = as_stream(stream);
   if (pat->num_rows == 0) return FEM_OK;
@@ -2094,9 +2138,11 @@ int fem_amg_sell_to_f32_copies(const fem_pattern_t* pat, const double* sell_vals
   FEM_CHECK_ARG(pat->vec == 1 || pat->vec == 3, "fem_amg_sell_to_f32_copies: vec must be 1 or 3");
   const dim3 g((unsigned)pat->num_slices);
   if (pat->vec == 3)
-    hipLaunchKernelGGL((k_sell_to_bcsr<3, float, true>), g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals, diff_vals);
+    hipLaunchKernelGGL((k_sell_to_bcsr<3, float, true>), g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals, diff_vals,
+                       dinv);
   else
-    hipLaunchKernelGGL((k_sell_to_bcsr<1, float, true>), g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals, diff_vals);
+    hipLaunchKernelGGL((k_sell_to_bcsr<1, float, true>), g, dim3(256), 0, s, *pat, sell_vals, bcsr_vals, diff_vals,
+                       dinv);
   FEM_LAUNCHED();
   return FEM_OK;
 }
@@ -2444,6 +2490,11 @@ int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* str
     hipLaunchKernelGGL(k_dense_scatter<double>, dim3(grid_for(A->rows, 256)), dim3(256), 0, s, A->rows, A->br,
                        A->bc, n, A->row_ptr, A->col_idx, static_cast<const double*>(A->vals), a);
   FEM_LAUNCHED();
+  if (n <= kGjSmall) {
+    hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, s, (int)n, a);
+    FEM_LAUNCHED();
+    return FEM_OK;
+  }
   for (int64_t k = 0; k < n; ++k) {
     hipLaunchKernelGGL(k_gj_step, dim3(g), dim3(256), 0, s, n, k, a, b);
     std::swap(a, b);

@@ -149,7 +149,7 @@ _SIGNATURES = {
     'fem_amg_bc_mask': (C.c_int, [_i64, _i64, _vp, _vp, _vp]),
     'fem_amg_sell_to_bcsr': (C.c_int, [_P(FemPattern), _vp, _vp, _i32, _vp]),
     'fem_amg_sell_to_f32_diff': (C.c_int, [_P(FemPattern), _vp, _vp, _vp]),
-    'fem_amg_sell_to_f32_copies': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp]),
+    'fem_amg_sell_to_f32_copies': (C.c_int, [_P(FemPattern), _vp, _vp, _vp, _vp, _vp]),
     'fem_amg_dinv': (C.c_int, [_P(FemBcsr), _i32, _vp, _vp]),
     'fem_amg_lmax_ws_bytes': (_i64, [_i64]),
     'fem_amg_lmax': (C.c_int, [_P(FemBcsr), _P(FemPattern), _vp, _i32, _vp, _i32, _P(_f64), _vp, _vp]),

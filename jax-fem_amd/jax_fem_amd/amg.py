@@ -282,19 +282,21 @@ class DeviceAMG:
         A0 = levels[0]['A'] if levels else coarse   # no levels: the matrix is the coarsest level
         prec0 = _lib.FEM_F32 if A0.vals.dtype == torch.float32 else _lib.FEM_F64
         f16 = level0 == 'fp16' and precision == _lib.FEM_F32 and bool(levels)
-        sv32 = None
+        sv32 = dinv0 = None
         if precision == _lib.FEM_F32 and levels and not f16:
-            # level 0's FP32 block-CSR copy (the Galerkin products' input) and its FP32
-            # difference-form SELL copy (the V-cycle's level-0 matrix) in one pass
+            # level 0's FP32 block-CSR copy (the Galerkin products' input), its FP32
+            # difference-form SELL copy (the V-cycle's level-0 matrix) and its inverse
+            # diagonal in one pass
             sv32 = self._sell_diff32_buffer(vals64)
-            check(L.fem_amg_sell_to_f32_copies(dev.pref, ptr(vals64), ptr(A0.vals), ptr(sv32), stream_ptr()),
-                  'fem_amg_sell_to_f32_copies')
+            dinv0 = self._empty(A0.rows * A0.br)
+            check(L.fem_amg_sell_to_f32_copies(dev.pref, ptr(vals64), ptr(A0.vals), ptr(sv32), ptr(dinv0),
+                                               stream_ptr()), 'fem_amg_sell_to_f32_copies')
         else:
             check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), prec0, stream_ptr()),
                   'fem_amg_sell_to_bcsr')
         for l, lv in enumerate(levels):
             A = lv['A']
-            lv['dinv'] = self._dinv(A, fix=l > 0)
+            lv['dinv'] = dinv0 if (l == 0 and dinv0 is not None) else self._dinv(A, fix=l > 0)
             if l == 0 and f16:
                 sv32 = self._sell_f16(vals64, lv['dinv'])
             lv['lam'] = self._lmax(A, lv['dinv'], (vals64 if sv32 is None else sv32) if l == 0 else None)

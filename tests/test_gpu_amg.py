@@ -1,6 +1,8 @@
 """Device smoothed-aggregation AMG (the `amgx_solver` option) against the CPU
 restatement oracle/amg.py on the same assembled matrix, and AMG-CG solves
 against a direct solve of the exported matrix."""
+import ctypes as C
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -304,8 +306,41 @@ def test_fused_f32_copies_match_the_separate_copies(gpu, mesh):
     check(dev.L.fem_amg_sell_to_f32_diff(dev.pref, ptr(vals), ptr(diff), stream_ptr()), 'fem_amg_sell_to_f32_diff')
     bcsr2 = torch.full_like(bcsr, float('nan'))
     diff2 = torch.full_like(diff, float('nan'))
-    check(dev.L.fem_amg_sell_to_f32_copies(dev.pref, ptr(vals), ptr(bcsr2), ptr(diff2), stream_ptr()),
+    dinv2 = torch.full((dev.n,), float('nan'), dtype=torch.float64, device='cuda')
+    check(dev.L.fem_amg_sell_to_f32_copies(dev.pref, ptr(vals), ptr(bcsr2), ptr(diff2), ptr(dinv2), stream_ptr()),
           'fem_amg_sell_to_f32_copies')
+    # the inverse diagonal as fem_amg_dinv forms it from the block-CSR copy
+    d = DeviceAMG(dev)
+    from jax_fem_amd.amg import Bcsr
+    rp, ci = dev.pat_tensors['row_ptr'], dev.pat_tensors['col_idx']
+    M = Bcsr(dev.n // V, dev.n // V, V, V, rp, ci, bcsr)
+    dinv = d._dinv(M, fix=False)
     torch.cuda.synchronize()
     assert torch.equal(bcsr2.view(torch.int32), bcsr.view(torch.int32))
     assert torch.equal(diff2.view(torch.int32), diff.view(torch.int32))
+    assert torch.equal(dinv2.view(torch.int64), dinv.view(torch.int64))
+
+
+@pytest.mark.parametrize('nb', [4, 10, 12])
+def test_coarse_inverse_against_numpy(gpu, nb):
+    # the dense coarse solve's inverse: one workgroup in LDS up to 64 unknowns (nb <= 10 blocks
+    # of 6), a launch per pivot above; both against numpy's inverse of the same SPD matrix
+    from jax_fem_amd._lib import check, ptr, stream_ptr
+    from jax_fem_amd.amg import Bcsr
+    rng = np.random.default_rng(nb)
+    n = 6 * nb
+    G = rng.standard_normal((n, n))
+    Adense = G @ G.T + n * np.eye(n)
+    blocks = Adense.reshape(nb, 6, nb, 6).transpose(0, 2, 1, 3).reshape(-1)
+    dev = torch.device('cuda')
+    rp = torch.arange(0, nb * nb + 1, nb, dtype=torch.int32, device=dev)
+    ci = torch.arange(nb, dtype=torch.int32, device=dev).repeat(nb)
+    M = Bcsr(nb, nb, 6, 6, rp, ci, torch.tensor(blocks, dtype=torch.float64, device=dev))
+    s = M.struct()
+    inv = torch.empty(n * n, dtype=torch.float64, device=dev)
+    ws = torch.empty(n * n, dtype=torch.float64, device=dev)
+    L = _lib.load()
+    check(L.fem_amg_coarse_inverse(C.byref(s), ptr(inv), ptr(ws), stream_ptr()), 'fem_amg_coarse_inverse')
+    got = inv.cpu().numpy().reshape(n, n)
+    ref = np.linalg.inv(Adense)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
