@@ -294,9 +294,9 @@ def run(args):
         "s_per_adjoint_gradient": {"mixed": head['s_per_adjoint_gradient'],
                                    "fp64": res['fp64']['s_per_adjoint_gradient']},
         "roofline": {"bound": "hbm",
-                     "kernel": "k_sell_apply<3,CHEB,float,true> (level-0 Chebyshev step of the V-cycle: SELL-64 3x3-block "
-                               "product on FP32 difference-form storage with FP64 arithmetic, fused D^-1 "
-                               "residual and update)",
+                     "kernel": "k_sell_apply<3,CHEB,float,true,float,float> (level-0 Chebyshev step of the V-cycle: "
+                               "SELL-64 3x3-block product on FP32 difference-form storage and FP32 level-0 "
+                               "vectors with FP64 arithmetic, fused D^-1 residual and update)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
                      "traffic": amg_traffic.get('k_sell_apply_cheb_f32'),
