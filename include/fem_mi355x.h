@@ -545,6 +545,10 @@ int64_t fem_amg_lmax_ws_bytes(int64_t n);
 int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell_vals, int32_t sell_f32,
                  const double* dinv,
                  int32_t steps, double* lmax, void* ws, void* stream);
+/* The same estimate written to device memory (lmax_dev[0]) without
+ * synchronising: a hierarchy's setup reads all levels' estimates back once. */
+int fem_amg_lmax_async(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell_vals, int32_t sell_f32,
+                       const double* dinv, int32_t steps, double* lmax_dev, void* ws, void* stream);
 /* C = X Y for block CSR X (br x k) and Y (k x bc), FP64.  The structure is
  * computed once and kept as a plan, so a new matrix costs one numeric pass:
  *   symbolic  row_counts[r] = blocks of row r of C; row_pairs[r] (may be
@@ -572,6 +576,11 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
 int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P, const int32_t* plan_ptr,
                                const int32_t* plan, const int32_t* agg, const double* dinv, double omega,
                                void* stream);
+/* The same with omega = 4 / (3 lmax_dev[0]) formed on the device from the
+ * level's fem_amg_lmax_async estimate. */
+int fem_amg_smooth_prolongator_dev(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P,
+                                   const int32_t* plan_ptr, const int32_t* plan, const int32_t* agg,
+                                   const double* dinv, const double* lmax_dev, void* stream);
 /* T = X^T (block CSR; T->rows = X->cols, blocks bc x br).  Structure:
  * T->row_ptr, T->col_idx (ascending) and src (nnzb: the X entry of each T
  * entry); ws >= 4*X->cols rounded up to 256, + fem_scan_ws_bytes(X->cols).

@@ -1005,7 +1005,9 @@ __global__ __launch_bounds__(256) void k_smooth_numeric(int64_t nb, int64_t rows
                                                         const int32_t* __restrict__ plan, const T* __restrict__ av,
                                                         const double* __restrict__ p0, const int32_t* __restrict__ agg,
                                                         const double* __restrict__ dinv, double omega,
-                                                        T* __restrict__ pv) {
+                                                        const double* __restrict__ lmax_dev, T* __restrict__ pv) {
+  // lmax_dev: the level's lambda_max estimate on the device; omega = 4 / (3 lambda_max)
+  if (lmax_dev) omega = 4.0 / (3.0 * lmax_dev[0]);
   const int32_t plan_len = plan_ptr[nb];
   for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
     int64_t lo = 0, hi = rows;  // row i: crp[i] <= b < crp[i+1]
@@ -1521,6 +1523,11 @@ __global__ void k_scale_by_norm(int64_t n, const T* __restrict__ y, const double
     x[i] = T(double(y[i]) * inv);
 }
 
+// *out = sqrt(nrm[0] / nrm[1])
+__global__ void k_lmax_ratio(const double* __restrict__ nrm, double* __restrict__ out) {
+  if (threadIdx.x == 0) *out = sqrt(nrm[0] / nrm[1]);
+}
+
 // per-block partials of <x, x> for FP32 x (FP64 sums)
 __global__ __launch_bounds__(256) void k_norm2_part_f32(int64_t n, const float* __restrict__ x,
                                                         double* __restrict__ part) {
@@ -1954,7 +1961,8 @@ int spgemm_numeric_t(const fem_bcsr_t& X, const fem_bcsr_t& Y, const fem_bcsr_t&
                      const int32_t* plan, hipStream_t s);
 template <typename T>
 int smooth_t(const fem_bcsr_t& A, const fem_bcsr_t& P0, const fem_bcsr_t& P, const int32_t* plan_ptr,
-             const int32_t* plan, const int32_t* agg, const double* dinv, double omega, hipStream_t s);
+             const int32_t* plan, const int32_t* agg, const double* dinv, double omega, const double* lmax_dev,
+             hipStream_t s);
 
 template <bool FILL>
 int spgemm_dispatch(int64_t max_pairs, const fem_bcsr_t& X, const fem_bcsr_t& Y, int32_t* counts, const PlanOut& o,
@@ -2228,12 +2236,26 @@ extern "C" {
 
 int64_t fem_amg_lmax_ws_bytes(int64_t n) {
   n = std::max<int64_t>(n, 1);
-  return (int64_t)(2 * align256(n * 8) + align256(kDotBlocks * 8) + 256);
+  return (int64_t)(2 * align256(n * 8) + align256(kDotBlocks * 8) + 256);  // iterates, partials, 3 scalars
 }
 
 int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell_vals, int32_t sell_f32,
                  const double* dinv, int32_t steps, double* lmax, void* ws, void* stream) {
   FEM_CHECK_ARG(dinv && lmax && ws && steps >= 1, "fem_amg_lmax: bad arguments");
+  const int64_t n = pat ? (pat->num_rows > 0 ? pat->num_rows * pat->vec : 0) : (A ? A->rows * A->br : 0);
+  // the estimate is formed on the device in the workspace's last slot and read back
+  double* dev = reinterpret_cast<double*>(static_cast<char*>(ws) + 2 * align256(std::max<int64_t>(n, 1) * 8) +
+                                          align256(kDotBlocks * 8)) + 2;
+  if (int rc = fem_amg_lmax_async(A, pat, sell_vals, sell_f32, dinv, steps, dev, ws, stream)) return rc;
+  hipStream_t s = as_stream(stream);
+  FEM_HIP(hipMemcpyAsync(lmax, dev, sizeof(double), hipMemcpyDeviceToHost, s));
+  FEM_HIP(hipStreamSynchronize(s));
+  return FEM_OK;
+}
+
+int fem_amg_lmax_async(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell_vals, int32_t sell_f32,
+                       const double* dinv, int32_t steps, double* lmax_dev, void* ws, void* stream) {
+  FEM_CHECK_ARG(dinv && lmax_dev && ws && steps >= 1, "fem_amg_lmax: bad arguments");
   FEM_CHECK_ARG(pat ? (sell_vals != nullptr && pat->num_rows > 0) : (A && A->br == A->bc && A->rows > 0),
                 "fem_amg_lmax: give a SELL matrix (pat, sell_vals) or a square block CSR matrix");
   hipStream_t s = as_stream(stream);
@@ -2292,10 +2314,9 @@ int fem_amg_lmax(const fem_bcsr_t* A, const fem_pattern_t* pat, const void* sell
     return FEM_OK;
   };
   if (int rc = (pat && sell_f32 == 1) ? run(float{}) : run(double{})) return rc;
-  double v[2] = {0.0, 0.0};  // |y_s|^2, |y_{s-1}|^2
-  FEM_HIP(hipMemcpyAsync(v, nrm, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-  FEM_HIP(hipStreamSynchronize(s));
-  *lmax = sqrt(v[0] / v[1]);
+  // lmax = sqrt(|y_s|^2 / |y_{s-1}|^2)
+  hipLaunchKernelGGL(k_lmax_ratio, dim3(1), dim3(64), 0, s, nrm, lmax_dev);
+  FEM_LAUNCHED();
   return FEM_OK;
 }
 
@@ -2375,8 +2396,24 @@ int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bc
   hipStream_t s = as_stream(stream);
   if (P->nnzb == 0) return FEM_OK;
   FEM_CHECK_ARG(plan && A->vals && P0->vals && P->vals && P->col_idx, "fem_amg_smooth_prolongator: arrays missing");
-  if (A->f32) return smooth_t<float>(*A, *P0, *P, plan_ptr, plan, agg, dinv, omega, s);
-  return smooth_t<double>(*A, *P0, *P, plan_ptr, plan, agg, dinv, omega, s);
+  if (A->f32) return smooth_t<float>(*A, *P0, *P, plan_ptr, plan, agg, dinv, omega, nullptr, s);
+  return smooth_t<double>(*A, *P0, *P, plan_ptr, plan, agg, dinv, omega, nullptr, s);
+}
+
+int fem_amg_smooth_prolongator_dev(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P,
+                                   const int32_t* plan_ptr, const int32_t* plan, const int32_t* agg,
+                                   const double* dinv, const double* lmax_dev, void* stream) {
+  FEM_CHECK_ARG(A && P0 && P && plan_ptr && agg && dinv && lmax_dev,
+                "fem_amg_smooth_prolongator_dev: NULL argument");
+  FEM_CHECK_ARG(A->br == A->bc && P0->br == A->bc && P->rows == A->rows && P->br == P0->br && P->bc == P0->bc &&
+                    P0->nnzb == P0->rows,
+                "fem_amg_smooth_prolongator_dev: shapes do not match");
+  FEM_CHECK_ARG(!P0->f32 && A->f32 == P->f32, "fem_amg_smooth_prolongator_dev: P0 is FP64, A and P of one precision");
+  hipStream_t s = as_stream(stream);
+  if (P->nnzb == 0) return FEM_OK;
+  FEM_CHECK_ARG(plan && A->vals && P0->vals && P->vals && P->col_idx, "fem_amg_smooth_prolongator_dev: arrays missing");
+  if (A->f32) return smooth_t<float>(*A, *P0, *P, plan_ptr, plan, agg, dinv, 0.0, lmax_dev, s);
+  return smooth_t<double>(*A, *P0, *P, plan_ptr, plan, agg, dinv, 0.0, lmax_dev, s);
 }
 
 }  // extern "C"
@@ -2420,7 +2457,8 @@ int spgemm_numeric_t(const fem_bcsr_t& Xd, const fem_bcsr_t& Yd, const fem_bcsr_
 
 template <typename T>
 int smooth_t(const fem_bcsr_t& Ad, const fem_bcsr_t& P0d, const fem_bcsr_t& Pd, const int32_t* plan_ptr,
-             const int32_t* plan, const int32_t* agg, const double* dinv, double omega, hipStream_t s) {
+             const int32_t* plan, const int32_t* agg, const double* dinv, double omega, const double* lmax_dev,
+             hipStream_t s) {
   const fem_bcsr_t *A = &Ad, *P0 = &P0d, *P = &Pd;
   const T* av = static_cast<const T*>(A->vals);
   const double* p0 = static_cast<const double*>(P0->vals);
@@ -2429,7 +2467,7 @@ int smooth_t(const fem_bcsr_t& Ad, const fem_bcsr_t& P0d, const fem_bcsr_t& Pd, 
 #define FEM_AMG_SMOOTH(BS, NS)                                                                                  \
   if (P->br == BS && P->bc == NS) {                                                                             \
     hipLaunchKernelGGL((k_smooth_numeric<BS, NS, T>), dim3(g), dim3(256), 0, s, P->nnzb, P->rows, P->row_ptr,      \
-                       P->col_idx, plan_ptr, plan, av, p0, agg, dinv, omega, pv);                               \
+                       P->col_idx, plan_ptr, plan, av, p0, agg, dinv, omega, lmax_dev, pv);                     \
     FEM_LAUNCHED();                                                                                             \
     return FEM_OK;                                                                                              \
   }

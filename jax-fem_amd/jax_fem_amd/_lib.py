@@ -153,12 +153,15 @@ _SIGNATURES = {
     'fem_amg_dinv': (C.c_int, [_P(FemBcsr), _i32, _vp, _vp]),
     'fem_amg_lmax_ws_bytes': (_i64, [_i64]),
     'fem_amg_lmax': (C.c_int, [_P(FemBcsr), _P(FemPattern), _vp, _i32, _vp, _i32, _P(_f64), _vp, _vp]),
+    'fem_amg_lmax_async': (C.c_int, [_P(FemBcsr), _P(FemPattern), _vp, _i32, _vp, _i32, _vp, _vp, _vp]),
     'fem_bcsr_spgemm_symbolic': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp, _P(_i64), _vp, _vp, _vp]),
     'fem_bcsr_spgemm_plan': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _i64, _P(FemBcsr), _vp, _vp, _vp, _vp, _vp,
                                        _vp]),
     'fem_bcsr_spgemm_numeric': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp]),
     'fem_amg_smooth_prolongator': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp, _vp, _f64,
                                              _vp]),
+    'fem_amg_smooth_prolongator_dev': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp, _vp, _vp,
+                                                 _vp]),
     'fem_bcsr_transpose_structure': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp, _vp]),
     'fem_bcsr_transpose_values': (C.c_int, [_P(FemBcsr), _P(FemBcsr), _vp, _vp]),
     'fem_amg_coarse_inverse': (C.c_int, [_P(FemBcsr), _vp, _vp, _vp]),

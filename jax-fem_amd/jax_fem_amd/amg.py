@@ -164,6 +164,16 @@ class DeviceAMG:
                                   ptr(ws), stream_ptr()), 'fem_amg_lmax')
         return float(lam.value)
 
+    def _lmax_async(self, A, dinv, out, sell_vals=None):
+        """_lmax's estimate written to the device scalar `out` (a 1-element FP64 tensor) without
+        a host synchronisation."""
+        s = A.struct()
+        ws = self._empty(self.L.fem_amg_lmax_ws_bytes(A.rows * A.br), torch.uint8)
+        pat = self.dev.pref if sell_vals is not None else None
+        f32 = 0 if sell_vals is None else {torch.float32: 1, torch.float16: 2}.get(sell_vals.dtype, 0)
+        check(self.L.fem_amg_lmax_async(C.byref(s), pat, ptr(sell_vals), f32, ptr(dinv), POWER_STEPS, ptr(out),
+                                        ptr(ws), stream_ptr()), 'fem_amg_lmax_async')
+
     def _sell_diff32_buffer(self, vals64):
         dev = self.dev
         n = dev.sell_blocks * dev.V * dev.V
@@ -294,23 +304,29 @@ class DeviceAMG:
         else:
             check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), prec0, stream_ptr()),
                   'fem_amg_sell_to_bcsr')
+        # each level's lambda_max estimate stays on the device (the prolongator smoothing forms
+        # omega = 4 / (3 lambda_max) there); the host reads them all back once, after the loop
+        lam_dev = self._empty(max(len(levels), 1))
         for l, lv in enumerate(levels):
             A = lv['A']
             lv['dinv'] = dinv0 if (l == 0 and dinv0 is not None) else self._dinv(A, fix=l > 0)
             if l == 0 and f16:
                 sv32 = self._sell_f16(vals64, lv['dinv'])
-            lv['lam'] = self._lmax(A, lv['dinv'], (vals64 if sv32 is None else sv32) if l == 0 else None)
-            omega = 4.0 / (3.0 * lv['lam'])
+            lam_l = lam_dev[l:l + 1]
+            self._lmax_async(A, lv['dinv'], lam_l, (vals64 if sv32 is None else sv32) if l == 0 else None)
             plan_w0, plan_w, plan_c = lv['plans']
             P = lv['P']
             as_, p0s, ps = A.struct(), lv['P0m'].struct(), P.struct()
-            check(L.fem_amg_smooth_prolongator(C.byref(as_), C.byref(p0s), C.byref(ps), ptr(plan_w0[0]),
-                                               ptr(plan_w0[1]), ptr(lv['agg']), ptr(lv['dinv']), omega,
-                                               stream_ptr()), 'fem_amg_smooth_prolongator')
+            check(L.fem_amg_smooth_prolongator_dev(C.byref(as_), C.byref(p0s), C.byref(ps), ptr(plan_w0[0]),
+                                                   ptr(plan_w0[1]), ptr(lv['agg']), ptr(lv['dinv']), ptr(lam_l),
+                                                   stream_ptr()), 'fem_amg_smooth_prolongator_dev')
             self._transpose_values(P, lv['R'], lv['R_src'])
             self._spgemm_numeric(A, P, lv['W'], plan_w)
             nxt = levels[l + 1]['A'] if l + 1 < len(levels) else coarse
             self._spgemm_numeric(lv['R'], lv['W'], nxt, plan_c)
+        if levels:
+            for lv, lam in zip(levels, lam_dev.cpu().tolist()):
+                lv['lam'] = lam
         self._dinv(coarse, fix=True)
         n_c = coarse.rows * coarse.br
         if getattr(self, '_coarse_n', None) != n_c:

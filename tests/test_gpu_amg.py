@@ -344,3 +344,29 @@ def test_coarse_inverse_against_numpy(gpu, nb):
     got = inv.cpu().numpy().reshape(n, n)
     ref = np.linalg.inv(Adense)
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_device_lmax_and_omega_match_the_host_path(gpu, precision):
+    # the setup keeps each level's lambda_max estimate on the device and forms omega there
+    # (fem_amg_lmax_async, fem_amg_smooth_prolongator_dev): the same estimate as the
+    # synchronising fem_amg_lmax, and the same smoothed prolongator, bit for bit, as
+    # fem_amg_smooth_prolongator with omega = 4 / (3 lambda_max) passed from the host
+    from jax_fem_amd._lib import check, ptr, stream_ptr
+    p = _elastic(16)
+    dev = _assembled(p)[0]
+    amg = dev.amg_setup(_cfg(precision))
+    for l, lv in enumerate(amg.levels):
+        sell = (amg.vals32 if amg.vals32 is not None else dev.current[2]) if l == 0 else None
+        lam_host = amg._lmax(lv['A'], lv['dinv'], sell)
+        assert lam_host == lv['lam']
+        P = lv['P']
+        ref = torch.empty_like(P.vals)
+        as_, p0s, ps = lv['A'].struct(), lv['P0m'].struct(), P.struct(ref)
+        plan_w0 = lv['plans'][0]
+        check(amg.L.fem_amg_smooth_prolongator(C.byref(as_), C.byref(p0s), C.byref(ps), ptr(plan_w0[0]),
+                                               ptr(plan_w0[1]), ptr(lv['agg']), ptr(lv['dinv']),
+                                               4.0 / (3.0 * lam_host), stream_ptr()), 'fem_amg_smooth_prolongator')
+        torch.cuda.synchronize()
+        bits = torch.int32 if P.vals.dtype == torch.float32 else torch.int64
+        assert torch.equal(ref.view(bits), P.vals.view(bits))
