@@ -1232,8 +1232,8 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __r
 enum { AP_STORE = 0, AP_ADD = 1, AP_RESID = 2, AP_CHEB = 3, AP_DSTORE = 4, AP_PQ = 5 };
 
 // TX: storage of the input vector and of the Chebyshev direction, TY: of the
-// output.  Arithmetic is FP64 whatever the storage; the FP32 hierarchy keeps
-// its level-0 vectors in FP32 (rhs and dinv stay FP64).
+// output.  Arithmetic is FP64 whatever the storage; the V-cycle's vectors are
+// FP64, the level-0 power iteration on the FP32 copy keeps FP32 iterates.
 template <typename TX = double, typename TY = double>
 struct ApplyArgsT {
   const TX* x;         // gathered input
@@ -1423,9 +1423,10 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
                                                                 ApplyArgsT<TX, TY> a,
                                                                 const int32_t* __restrict__ done) {
   // FP32 products unroll the slots by 4 (the residual: 124 -> 112 us on the 1.6M-dof
-  // cantilever); FP16 storage keeps 2 (its residual: 74 us unrolled by 2, 78 us by 4)
+  // cantilever), the Chebyshev step, with its larger epilogue, by 2 (119.5 against 124.6 us);
+  // FP16 storage keeps 2 (its residual: 74 us unrolled by 2, 78 us by 4)
   constexpr int W = SliceCfg<T>::W;
-  constexpr int U = sizeof(T) == 4 ? 4 : SliceCfg<T>::U;
+  constexpr int U = sizeof(T) == 4 ? (MODE == AP_CHEB ? 2 : 4) : SliceCfg<T>::U;
   __shared__ double red[W][kSlice][V];
   if ((done && *done) || (a.done && *a.done)) return;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1444,14 +1445,9 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
   // FP16 values hold D^-1 A (rows scaled by dinv): the epilogues undo or skip the scaling
   constexpr bool kScaled = sizeof(T) == 2;
-  sell_slice_partial<V, DIFF, W, U, T, decltype(xat), TX>(pat, vals, s, w, lane, row, xat, xr, acc);
-  if (w > 0) {
-#pragma unroll
-    for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
-  }
-  // wave 0 loads the epilogue's own-row operands after the matrix stream: held across
-  // it they cost 40 registers, which halves the waves resident per SIMD (the other
-  // slices' streams hide this latency)
+  // wave 0 loads the epilogue's own-row operands before the matrix stream, so their
+  // latency overlaps it (with the FP64 vectors of the cycle, loading them after the
+  // stream, which saves 40 registers, measured slower: Chebyshev step 119.5 -> 123.3 us)
   double e_rhs[V], e_dinv[V], e_d[V], e_x[V];
   if (w == 0 && real) {
 #pragma unroll
@@ -1462,6 +1458,11 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
       e_d[i] = (MODE == AP_CHEB && a.c1 != 0.0) ? double(a.d[idx]) : 0.0;
       e_x[i] = (MODE == AP_CHEB || MODE == AP_PQ) ? double(a.x[idx]) : 0.0;
     }
+  }
+  sell_slice_partial<V, DIFF, W, U, T, decltype(xat), TX>(pat, vals, s, w, lane, row, xat, xr, acc);
+  if (w > 0) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
   }
   __syncthreads();
   if (w != 0) return;
@@ -1851,10 +1852,12 @@ int level_up(const fem_amg_t& amg, int l, const double* rhs, const double* xc, T
 
 // ev (may be NULL): events recorded around the whole cycle (ev[2], ev[3]) and
 // around its first level-0 post-smoothing Chebyshev kernel (ev[0], ev[1]).
-// Level-0 vectors are stored in the hierarchy's precision T (FP32 for the
-// FP32 hierarchy: the iterates, the Chebyshev direction and the residual;
-// the right-hand side, D^-1 and the output z stay FP64); coarser levels are
-// FP64 throughout.
+// Vectors are FP64 on every level, whatever the storage T of the matrices:
+// FP32 level-0 iterates (which move 20 MB less per level-0 product on the
+// 1.6M-dof cantilever) round the differences x_j - x_i the difference form
+// multiplies, and on the high-contrast SIMP cantilever (160x80x40, E ratio
+// 10^3) the adjoint solve then took 84 instead of 76 iterations (106 with the
+// power iteration on FP32 iterates as well).
 template <typename T>
 int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipStream_t s,
                 hipEvent_t* ev = nullptr, const int32_t* done = nullptr) {
@@ -1878,24 +1881,15 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
   // with no levels the cycle is the dense solve of the whole matrix
   rhs[NL] = NL == 0 ? const_cast<double*>(r) : take(amg.coarse_n);
   cur[NL] = NL == 0 ? z : take(amg.coarse_n);
-  T* x0 = nullptr;   // level 0's iterate and spare buffer, in T
-  T* o0 = nullptr;
   // down
   for (int l = 0; l < NL; ++l) {
     const int64_t n = level_n(amg, l);
-    if (l == 0) {
-      if (int rc = level_down<T, T>(amg, 0, n, rhs[0], rhs[1], reinterpret_cast<T*>(xa[0]),
-                                    reinterpret_cast<T*>(xb[0]), reinterpret_cast<T*>(dd[0]),
-                                    reinterpret_cast<T*>(res[0]), &x0, &o0, s, done))
-        return rc;
-    } else {
-      double *x = nullptr, *o = nullptr;
-      if (int rc = level_down<T, double>(amg, l, n, rhs[l], rhs[l + 1], xa[l], xb[l], dd[l], res[l], &x, &o, s,
-                                         done))
-        return rc;
-      cur[l] = x;
-      xb[l] = o;
-    }
+    double *x = nullptr, *o = nullptr;
+    if (int rc = level_down<T, double>(amg, l, n, rhs[l], rhs[l + 1], xa[l], xb[l], dd[l], res[l], &x, &o, s,
+                                       done))
+      return rc;
+    cur[l] = x;
+    xb[l] = o;
   }
   // coarsest
   if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[0], s));
@@ -1903,18 +1897,13 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
                      amg.coarse_inv, rhs[NL], cur[NL], done);
   FEM_LAUNCHED();
   if (ev && NL == 0) FEM_HIP(hipEventRecord(ev[1], s));
-  // up
+  // up (level 0's last Chebyshev step writes the cycle's output z)
   for (int l = NL - 1; l >= 0; --l) {
-    if (l == 0) {
-      T* x = nullptr;
-      if (int rc = level_up<T, T>(amg, 0, rhs[0], cur[1], x0, o0, reinterpret_cast<T*>(dd[0]), z, &x, s, ev, done))
-        return rc;
-    } else {
-      double* x = nullptr;
-      if (int rc = level_up<T, double>(amg, l, rhs[l], cur[l + 1], cur[l], xb[l], dd[l], nullptr, &x, s, ev, done))
-        return rc;
-      cur[l] = x;
-    }
+    double* x = nullptr;
+    if (int rc = level_up<T, double>(amg, l, rhs[l], cur[l + 1], cur[l], xb[l], dd[l], l == 0 ? z : nullptr, &x, s,
+                                     ev, done))
+      return rc;
+    cur[l] = x;
   }
   if (ev) FEM_HIP(hipEventRecord(ev[3], s));
   return FEM_OK;
