@@ -1445,11 +1445,27 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   constexpr bool kRhs = MODE == AP_RESID || MODE == AP_CHEB;
   // FP16 values hold D^-1 A (rows scaled by dinv): the epilogues undo or skip the scaling
   constexpr bool kScaled = sizeof(T) == 2;
-  // wave 0 loads the epilogue's own-row operands before the matrix stream, so their
-  // latency overlaps it (with the FP64 vectors of the cycle, loading them after the
-  // stream, which saves 40 registers, measured slower: Chebyshev step 119.5 -> 123.3 us)
+  // The epilogue's own-row operands are loaded before the matrix stream, so their
+  // latency overlaps it.  With a thread per (row, component) pair (kDist: 4 waves),
+  // thread p finishes component p % V of lane p / V after the partial sums meet in
+  // LDS, holding 4 operands; otherwise wave 0 finishes its lanes' rows (the outer
+  // product's <p, q> partial is then one wave sum).
+  constexpr bool kDist = MODE != AP_PQ && SliceCfg<T>::NT >= kSlice * V;
+  const int pl = threadIdx.x / V, pi = threadIdx.x % V;
+  int64_t prow = -1;
+  double p_rhs = 0.0, p_dinv = 0.0, p_d = 0.0, p_x = 0.0;
+  if (kDist && threadIdx.x < kSlice * V) {
+    prow = pat.sell_row[s * kSlice + pl];
+    if (prow >= 0) {
+      const int64_t idx = prow * V + pi;
+      p_rhs = kRhs ? a.rhs[idx] : 0.0;
+      p_dinv = (MODE == AP_CHEB || MODE == AP_DSTORE || (kScaled && MODE == AP_RESID)) ? a.dinv[idx] : 0.0;
+      p_d = (MODE == AP_CHEB && a.c1 != 0.0) ? double(a.d[idx]) : 0.0;
+      p_x = MODE == AP_CHEB ? double(a.x[idx]) : 0.0;
+    }
+  }
   double e_rhs[V], e_dinv[V], e_d[V], e_x[V];
-  if (w == 0 && real) {
+  if (!kDist && w == 0 && real) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const int64_t idx = row * V + i;
@@ -1460,11 +1476,33 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
     }
   }
   sell_slice_partial<V, DIFF, W, U, T, decltype(xat), TX>(pat, vals, s, w, lane, row, xat, xr, acc);
-  if (w > 0) {
+  if (kDist || w > 0) {
 #pragma unroll
     for (int i = 0; i < V; ++i) red[w][lane][i] = acc[i];
   }
   __syncthreads();
+  if constexpr (kDist) {
+    if (prow >= 0) {
+      double t = red[0][pl][pi];
+#pragma unroll
+      for (int o = 1; o < W; ++o) t += red[o][pl][pi];
+      if (kScaled) t *= 1.0 / kF16Scale;
+      const int64_t idx = prow * V + pi;
+      if (MODE == AP_CHEB) {
+        const double r = kScaled ? fma(p_dinv, p_rhs, -t) : p_dinv * (p_rhs - t);
+        const double dn = a.c1 != 0.0 ? fma(a.c1, p_d, a.c2 * r) : a.c2 * r;
+        a.d[idx] = TX(dn);
+        a.y[idx] = TY(p_x + dn);
+      } else if (MODE == AP_RESID) {
+        a.y[idx] = TY(kScaled ? p_rhs - t / p_dinv : p_rhs - t);
+      } else if (MODE == AP_DSTORE) {
+        a.y[idx] = TY(kScaled ? t : p_dinv * t);
+      } else {
+        apply_epilogue<MODE>(a, idx, t);
+      }
+    }
+    return;
+  }
   if (w != 0) return;
   double pq = 0.0;
   if (real) {
