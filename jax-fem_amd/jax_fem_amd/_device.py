@@ -702,9 +702,17 @@ class DeviceProblem:
         mag = G.reshape(n, -1).abs().amax(1)
         idx = torch.unique(torch.cat([torch.topk(mag, k).indices,
                                       torch.linspace(0, n - 1, k, device=G.device).long()]))
-        # the map is evaluated where identify() probed it: on the CPU, in float64; states with
-        # non-finite entries are left to the solve, which reports them
-        Gs, ivs = G[idx].cpu(), [t[idx].cpu() for t in lc['ivs']]
+        # the map is evaluated where identify() probed it: on the CPU, in float64 (the samples
+        # come over in one copy); states with non-finite entries are left to the solve, which
+        # reports them
+        parts = [G[idx].reshape(len(idx), -1)] + [t[idx].reshape(len(idx), -1).to(G.dtype) for t in lc['ivs']]
+        flat = torch.cat(parts, 1).cpu()
+        o = parts[0].shape[1]
+        Gs, ivs = flat[:, :o].reshape((len(idx),) + tuple(G.shape[1:])), []
+        for t in lc['ivs']:
+            w = parts[len(ivs) + 1].shape[1]
+            ivs.append(flat[:, o:o + w].reshape((len(idx),) + tuple(t.shape[1:])).to(t.dtype))
+            o += w
         fin = torch.isfinite(Gs.reshape(len(idx), -1)).all(1)
         for t in ivs:
             fin &= torch.isfinite(t.reshape(len(idx), -1)).all(1)
@@ -722,10 +730,13 @@ class DeviceProblem:
             Pu = pm(Gs, ivs)[0]
             Pd = vmap(law)(Gs, *ivs).reshape(Pu.shape)
             # stiffness scale of the law (|P| per unit |grad u| near the reference state), so
-            # that states at or near grad u = 0 compare against rounding of that size
-            gen = torch.Generator().manual_seed(7)
-            G1 = 1e-6 * (torch.rand((1, self.V, 3), generator=gen, dtype=torch.float64) - 0.5)
-            k = float(vmap(law)(G1, *[t[:1] for t in ivs]).abs().max()) / float(G1.abs().max())
+            # that states at or near grad u = 0 compare against rounding of that size; formed
+            # once per preparation
+            k = lc.get('stiffness')
+            if k is None:
+                gen = torch.Generator().manual_seed(7)
+                G1 = 1e-6 * (torch.rand((1, self.V, 3), generator=gen, dtype=torch.float64) - 0.5)
+                k = lc['stiffness'] = float(vmap(law)(G1, *[t[:1] for t in ivs]).abs().max()) / float(G1.abs().max())
         scale = max(float(Pu.abs().max()), float(Pd.abs().max()), k * max(float(Gs.abs().max()), 1e-3), 1e-300)
         diff = float((Pu - Pd).abs().max())
         if diff <= 1e-9 * scale or not (diff == diff and scale < float('inf')):
