@@ -26,7 +26,7 @@ The algorithm (not a restatement of any reference file):
   * P = (I - omega D^-1 A) P0, omega = 4 / (3 lambda), lambda from 15 power
     steps on D^-1 A; Ac = P^T A P; zero diagonal entries of Ac set to 1.
   * Coarsening stops at <= 512 unknowns (dense inverse) or 10 levels.
-  * V(2,2) cycle, Chebyshev smoothing on [lambda/30, 1.1 lambda].
+  * V(2,2) cycle, Chebyshev smoothing on [lambda/15, 1.1 lambda].
 """
 import numpy as np
 import scipy.sparse as sp
@@ -34,7 +34,7 @@ import scipy.sparse as sp
 COARSE_MAX = 512
 MAX_LEVELS = 10
 POWER_STEPS = 15
-CHEB_LOW = 1.0 / 30.0
+CHEB_LOW = 1.0 / 15.0
 CHEB_HIGH = 1.1
 RANK_TOL = 1e-10
 
