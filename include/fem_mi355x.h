@@ -576,7 +576,7 @@ int fem_bcsr_spgemm_numeric(const fem_bcsr_t* X, const fem_bcsr_t* Y, fem_bcsr_t
 int fem_amg_smooth_prolongator(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P, const int32_t* plan_ptr,
                                const int32_t* plan, const int32_t* agg, const double* dinv, double omega,
                                void* stream);
-/* The same with omega = 4 / (3 lmax_dev[0]) formed on the device from the
+/* The same with omega = 1.5 / lmax_dev[0] formed on the device from the
  * level's fem_amg_lmax_async estimate. */
 int fem_amg_smooth_prolongator_dev(const fem_bcsr_t* A, const fem_bcsr_t* P0, fem_bcsr_t* P,
                                    const int32_t* plan_ptr, const int32_t* plan, const int32_t* agg,

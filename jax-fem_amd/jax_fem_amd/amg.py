@@ -305,7 +305,7 @@ class DeviceAMG:
             check(L.fem_amg_sell_to_bcsr(dev.pref, ptr(vals64), ptr(A0.vals), prec0, stream_ptr()),
                   'fem_amg_sell_to_bcsr')
         # each level's lambda_max estimate stays on the device (the prolongator smoothing forms
-        # omega = 4 / (3 lambda_max) there); the host reads them all back once, after the loop
+        # omega = 1.5 / lambda_max there); the host reads them all back once, after the loop
         lam_dev = self._empty(max(len(levels), 1))
         for l, lv in enumerate(levels):
             A = lv['A']

@@ -23,7 +23,7 @@ The algorithm (not a restatement of any reference file):
   * Near null space: rigid-body modes (vec 3, coordinates relative to the
     aggregate's first node) or constants (vec 1); rows of Dirichlet dofs zero.
     Tentative P by modified Gram-Schmidt per aggregate.
-  * P = (I - omega D^-1 A) P0, omega = 4 / (3 lambda), lambda from 15 power
+  * P = (I - omega D^-1 A) P0, omega = 1.5 / lambda, lambda from 15 power
     steps on D^-1 A; Ac = P^T A P; zero diagonal entries of Ac set to 1.
   * Coarsening stops at <= 512 unknowns (dense inverse) or 10 levels.
   * V(2,2) cycle, Chebyshev smoothing on [lambda/15, 1.1 lambda].
@@ -247,7 +247,7 @@ class Hierarchy:
             P0, Bc = tentative(aptr, memb, B, bs)
             dinv = safe_dinv(A)
             lam = lams[lvl] if lams is not None else lmax(A, dinv)
-            omega = 4.0 / (3.0 * lam)
+            omega = 1.5 / lam
             P0m = p0_matrix(P0, agg, n_agg)
             P = (P0m - omega * (sp.diags(dinv) @ (A @ P0m))).tocsr()
             Ac = (P.T @ A @ P).tocsr()

@@ -351,7 +351,7 @@ def test_device_lmax_and_omega_match_the_host_path(gpu, precision):
     # the setup keeps each level's lambda_max estimate on the device and forms omega there
     # (fem_amg_lmax_async, fem_amg_smooth_prolongator_dev): the same estimate as the
     # synchronising fem_amg_lmax, and the same smoothed prolongator, bit for bit, as
-    # fem_amg_smooth_prolongator with omega = 4 / (3 lambda_max) passed from the host
+    # fem_amg_smooth_prolongator with omega = 1.5 / lambda_max passed from the host
     from jax_fem_amd._lib import check, ptr, stream_ptr
     p = _elastic(16)
     dev = _assembled(p)[0]
@@ -366,7 +366,7 @@ def test_device_lmax_and_omega_match_the_host_path(gpu, precision):
         plan_w0 = lv['plans'][0]
         check(amg.L.fem_amg_smooth_prolongator(C.byref(as_), C.byref(p0s), C.byref(ps), ptr(plan_w0[0]),
                                                ptr(plan_w0[1]), ptr(lv['agg']), ptr(lv['dinv']),
-                                               4.0 / (3.0 * lam_host), stream_ptr()), 'fem_amg_smooth_prolongator')
+                                               1.5 / lam_host, stream_ptr()), 'fem_amg_smooth_prolongator')
         torch.cuda.synchronize()
         bits = torch.int32 if P.vals.dtype == torch.float32 else torch.int64
         assert torch.equal(ref.view(bits), P.vals.view(bits))
