@@ -474,7 +474,8 @@ typedef struct {
   int32_t        precision;      /* value storage of A, P, R and the level-0
                                     SELL copy: FEM_F32 or FEM_F64; vectors and
                                     arithmetic are FP64                       */
-  int32_t        degree;         /* Chebyshev degree of pre/post smoothing    */
+  int32_t        degree;         /* Chebyshev degree of pre/post smoothing on
+                                    level 0; coarser levels: 2 * degree (<= 8) */
   int32_t        level0_form;    /* form of the level-0 SELL values: FEM_FORM_
                                     STANDARD, FEM_FORM_DIFFERENCE (FP32), or
                                     FEM_FORM_SCALED_DIFF16 (FP16 D^-1 A, with

@@ -26,7 +26,8 @@ The algorithm (not a restatement of any reference file):
   * P = (I - omega D^-1 A) P0, omega = 1.5 / lambda, lambda from 15 power
     steps on D^-1 A; Ac = P^T A P; zero diagonal entries of Ac set to 1.
   * Coarsening stops at <= 512 unknowns (dense inverse) or 10 levels.
-  * V(2,2) cycle, Chebyshev smoothing on [lambda/15, 1.1 lambda].
+  * V(2,2) cycle (degree 4 on the coarser levels), Chebyshev smoothing on
+    [lambda/15, 1.1 lambda].
 """
 import numpy as np
 import scipy.sparse as sp
@@ -289,13 +290,16 @@ class Hierarchy:
         return x
 
     def vcycle(self, r, l=0, pre=2, post=2):
+        """V-cycle; pre/post are level 0's Chebyshev degrees, the coarser levels smooth
+        with twice that degree (at most 8), as the device cycle does."""
         if l == len(self.levels):
             return self.coarse_inv @ r
         L = self.levels[l]
-        x = self.smooth(L, np.zeros_like(r), r, pre, True)
+        dpre, dpost = (pre, post) if l == 0 else (min(2 * pre, 8), min(2 * post, 8))
+        x = self.smooth(L, np.zeros_like(r), r, dpre, True)
         rc = L['P'].T @ (r - L['A'] @ x)
         x = x + L['P'] @ self.vcycle(rc, l + 1, pre, post)
-        return self.smooth(L, x, r, post, False)
+        return self.smooth(L, x, r, dpost, False)
 
 
 def pcg(A, b, M, x0=None, rtol=1e-10, atol=0.0, maxiter=1000):
