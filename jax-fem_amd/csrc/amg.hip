@@ -1146,14 +1146,18 @@ __global__ void k_dense_zero(int64_t nn, double* __restrict__ M) {
     M[i] = 0.0;
 }
 
+// one workgroup per block row (rows <= 4096): its threads stride over the row's block entries
 template <typename T>
-__global__ void k_dense_scatter(int64_t rows, int br, int bc, int64_t n, const int32_t* __restrict__ rp,
-                                const int32_t* __restrict__ ci, const T* __restrict__ v, double* __restrict__ M) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < rows; i += (int64_t)gridDim.x * blockDim.x)
-    for (int k = rp[i]; k < rp[i + 1]; ++k)
-      for (int a = 0; a < br; ++a)
-        for (int b = 0; b < bc; ++b)
-          M[(i * br + a) * n + (int64_t)ci[k] * bc + b] = double(v[((size_t)k * br + a) * bc + b]);
+__global__ __launch_bounds__(256) void k_dense_scatter(int64_t rows, int br, int bc, int64_t n,
+                                                       const int32_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                       const T* __restrict__ v, double* __restrict__ M) {
+  const int64_t i = blockIdx.x;
+  if (i >= rows) return;
+  const int kb = rp[i], per = br * bc, nf = (rp[i + 1] - kb) * per;
+  for (int f = threadIdx.x; f < nf; f += 256) {
+    const int k = kb + f / per, e = f % per, a = e / bc, b = e % bc;
+    M[(i * br + a) * n + (int64_t)ci[k] * bc + b] = double(v[(size_t)k * per + e]);
+  }
 }
 
 // one Gauss-Jordan pivot step src -> dst (in-place inversion formula)
@@ -1174,22 +1178,34 @@ __global__ void k_gj_step(int64_t n, int64_t k, const double* __restrict__ src, 
   }
 }
 
-// The whole Gauss-Jordan elimination of a small dense matrix (n <= kGjSmall) in one
-// workgroup: the matrix in LDS, one barrier per pivot, each step the arithmetic of
+// The whole Gauss-Jordan elimination of a small dense block (n <= kGjSmall: the n x n
+// block at (k0, k0) of a matrix of leading dimension ld, inverted in place) in one
+// workgroup: the block in LDS, one barrier per pivot, each step the arithmetic of
 // k_gj_step (so the inverse is the same bit for bit) without a launch per pivot.
 constexpr int kGjSmall = 64;
-__global__ __launch_bounds__(1024) void k_gj_small(int n, double* __restrict__ a) {
+__global__ __launch_bounds__(1024) void k_gj_small(int n, int64_t ld, int64_t k0, double* __restrict__ a) {
   __shared__ double m[2][kGjSmall * kGjSmall];
+  constexpr int kPer = kGjSmall * kGjSmall / 1024;  // entries per thread: e = threadIdx.x + 1024 u
   const int nn = n * n;
-  for (int e = threadIdx.x; e < nn; e += 1024) m[0][e] = a[e];
+  int ie[kPer], je[kPer];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int e = threadIdx.x + 1024 * u;
+    ie[u] = e / n;
+    je[u] = e % n;
+    if (e < nn) m[0][e] = a[(k0 + ie[u]) * ld + k0 + je[u]];
+  }
   __syncthreads();
   int c = 0;
   for (int k = 0; k < n; ++k) {
     const double* src = m[c];
     double* dst = m[c ^ 1];
     const double d = 1.0 / src[k * n + k];
-    for (int e = threadIdx.x; e < nn; e += 1024) {
-      const int i = e / n, j = e % n;
+#pragma unroll
+    for (int u = 0; u < kPer; ++u) {
+      const int e = threadIdx.x + 1024 * u;
+      if (e >= nn) break;
+      const int i = ie[u], j = je[u];
       double v;
       if (i == k && j == k)
         v = d;
@@ -1204,7 +1220,121 @@ __global__ __launch_bounds__(1024) void k_gj_small(int n, double* __restrict__ a
     __syncthreads();
     c ^= 1;
   }
-  for (int e = threadIdx.x; e < nn; e += 1024) a[e] = m[c][e];
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    const int e = threadIdx.x + 1024 * u;
+    if (e < nn) a[(k0 + ie[u]) * ld + k0 + je[u]] = m[c][e];
+  }
+}
+
+// Blocked Gauss-Jordan (n >= 2 kGjSmall): per pivot block K = [k0, k0 + bk), bk <= kGjSmall,
+// with P = A_KK^-1 (k_gj_small, in place),
+//   A_Kj <- P A_Kj,   A_iK <- -A_iK P,   A_ij <- A_ij - A_iK (P A_Kj)     (i, j outside K):
+// the scalar steps' Schur complements, the pivots taken in the same order, a product of
+// 64-wide panels per block instead of a launch per pivot.
+// k_bgj_panel: colp (n x kGjSmall) <- the old column panel A_iK (i outside K),
+//              rowp (bk x n) <- P A_Kj (j outside K);
+// one workgroup per 64-column tile: P and the tile of A_Kj in LDS, 4 x 4 entries per thread
+// (each workgroup also copies 64 rows of the column panel).
+__global__ __launch_bounds__(256) void k_bgj_panel(int64_t n, int64_t k0, int bk, const double* __restrict__ a,
+                                                   double* __restrict__ colp, double* __restrict__ rowp) {
+  constexpr int T = kGjSmall;
+  __shared__ double sp[T][T + 1];  // P
+  __shared__ double sa[T][T + 1];  // the tile of A_Kj
+  const int64_t tj = blockIdx.x, j0 = tj * T, kb = k0 / T;
+  for (int e = threadIdx.x; e < T * T; e += 256) {
+    const int64_t i = j0 + e / T;
+    const int c = e % T;
+    if (i < n && c < bk && (i < k0 || i >= k0 + bk)) colp[i * T + c] = a[i * n + k0 + c];
+  }
+  if (tj == kb) return;
+  for (int e = threadIdx.x; e < T * T; e += 256) {
+    const int r = e / T, q = e % T;
+    const int64_t j = j0 + q;
+    sp[r][q] = (r < bk && q < bk) ? a[(k0 + r) * n + k0 + q] : 0.0;
+    sa[r][q] = (r < bk && j < n) ? a[(k0 + r) * n + j] : 0.0;
+  }
+  __syncthreads();
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int c = 0; c < bk; ++c) {
+    double x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = sp[tr + 16 * u][c];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[v] = sa[c][tc + 16 * v];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = tr + 16 * u;
+      const int64_t j = j0 + tc + 16 * v;
+      if (r < bk && j < n) rowp[r * n + j] = acc[u][v];
+    }
+}
+
+// k_bgj_update: one workgroup per 64 x 64 tile of A (tiles aligned with the pivot blocks);
+// the tile's column panel and row panel (or P) in LDS, 4 x 4 entries per thread.
+__global__ __launch_bounds__(256) void k_bgj_update(int64_t n, int64_t k0, int bk, double* __restrict__ a,
+                                                    const double* __restrict__ colp,
+                                                    const double* __restrict__ rowp) {
+  constexpr int T = kGjSmall;
+  __shared__ double sc[T][T + 1];  // sc[r][c] = A_iK (old), i = i0 + r
+  __shared__ double sr[T][T + 1];  // sr[c][q] = (P A_Kj)[c][j0 + q], or P[c][q] on the pivot column tile
+  const int64_t kb = k0 / T, ti = blockIdx.y, tj = blockIdx.x, i0 = ti * T, j0 = tj * T;
+  if (ti == kb) {
+    // the pivot rows take P A_Kj; the pivot block keeps P
+    if (tj == kb) return;
+    for (int e = threadIdx.x; e < T * T; e += 256) {
+      const int r = e / T;
+      const int64_t j = j0 + e % T;
+      if (r < bk && j < n) a[(k0 + r) * n + j] = rowp[r * n + j];
+    }
+    return;
+  }
+  const bool pivcol = tj == kb;
+  for (int e = threadIdx.x; e < T * T; e += 256) {
+    const int r = e / T, c = e % T;
+    const int64_t i = i0 + r;
+    sc[r][c] = (i < n && c < bk) ? colp[i * T + c] : 0.0;
+    const int cc = e / T, q = e % T;
+    const int64_t j = j0 + q;
+    sr[cc][q] = (cc < bk && j < n) ? (pivcol ? a[(k0 + cc) * n + j] : rowp[cc * n + j]) : 0.0;
+  }
+  __syncthreads();
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int c = 0; c < bk; ++c) {
+    double x[4], y[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) x[u] = sc[tr + 16 * u][c];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) y[v] = sr[c][tc + 16 * v];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t i = i0 + tr + 16 * u, j = j0 + tc + 16 * v;
+      if (i < n && j < n) a[i * n + j] = pivcol ? -acc[u][v] : a[i * n + j] - acc[u][v];
+    }
 }
 
 __global__ void k_copy(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
@@ -2596,14 +2726,28 @@ int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* str
   const unsigned g = grid_for(n * n, 256);
   hipLaunchKernelGGL(k_dense_zero, dim3(g), dim3(256), 0, s, n * n, a);
   if (A->f32)
-    hipLaunchKernelGGL(k_dense_scatter<float>, dim3(grid_for(A->rows, 256)), dim3(256), 0, s, A->rows, A->br, A->bc,
+    hipLaunchKernelGGL(k_dense_scatter<float>, dim3((unsigned)A->rows), dim3(256), 0, s, A->rows, A->br, A->bc,
                        n, A->row_ptr, A->col_idx, static_cast<const float*>(A->vals), a);
   else
-    hipLaunchKernelGGL(k_dense_scatter<double>, dim3(grid_for(A->rows, 256)), dim3(256), 0, s, A->rows, A->br,
+    hipLaunchKernelGGL(k_dense_scatter<double>, dim3((unsigned)A->rows), dim3(256), 0, s, A->rows, A->br,
                        A->bc, n, A->row_ptr, A->col_idx, static_cast<const double*>(A->vals), a);
   FEM_LAUNCHED();
   if (n <= kGjSmall) {
-    hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, s, (int)n, a);
+    hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, s, (int)n, n, (int64_t)0, a);
+    FEM_LAUNCHED();
+    return FEM_OK;
+  }
+  if (n >= 2 * kGjSmall) {
+    // blocked: the panels (n * kGjSmall + kGjSmall * n doubles) fit in ws (n * n doubles)
+    double* colp = b;
+    double* rowp = b + n * kGjSmall;
+    const unsigned tiles = (unsigned)ceil_div(n, kGjSmall);
+    for (int64_t k0 = 0; k0 < n; k0 += kGjSmall) {
+      const int bk = (int)std::min<int64_t>(kGjSmall, n - k0);
+      hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, s, bk, n, k0, a);
+      hipLaunchKernelGGL(k_bgj_panel, dim3(tiles), dim3(256), 0, s, n, k0, bk, a, colp, rowp);
+      hipLaunchKernelGGL(k_bgj_update, dim3(tiles, tiles), dim3(256), 0, s, n, k0, bk, a, colp, rowp);
+    }
     FEM_LAUNCHED();
     return FEM_OK;
   }

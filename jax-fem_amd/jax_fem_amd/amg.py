@@ -11,7 +11,7 @@ GPU and this module only sequences the calls, sizing the buffers between them
   per level dinv -> MIS-2 aggregates -> tentative P0 (rigid-body modes or
             constants, Dirichlet rows zero) -> lambda_max(D^-1 A) -> A P0 ->
             P = P0 - omega D^-1 A P0 -> R = P^T -> Ac = R (A P)
-  coarsest  dense Gauss-Jordan inverse (<= 512 unknowns)
+  coarsest  dense Gauss-Jordan inverse (<= 512 unknowns, <= 1024 on large problems)
 
 The structure (aggregates, tentative prolongators, patterns, product plans)
 is built once per pattern and Dirichlet set; each new matrix then costs the
@@ -31,7 +31,21 @@ from ._lib import check, ptr, stream_ptr
 
 logger = logging.getLogger('jax_fem_amd')
 
-COARSE_MAX = 512    # unknowns of the coarsest (dense) level
+COARSE_MAX = 1024   # unknowns of the coarsest (dense) level: at most this many,
+COARSE_SMALL = 512  # and at most this many where coarse_limit says so
+
+
+def coarse_limit(nnz0):
+    """Largest dense coarsest level for a level-0 operator of nnz0 entries: 512 unknowns,
+    and up to 1024 while n^3 <= 8 nnz0.  A dense level of n unknowns costs a blocked
+    inverse per setup (about 0.1 ms per 64 unknowns) and saves the smoothed level it
+    replaces (a dozen small launches per V-cycle): that pays where the fine operator, and
+    with it the iteration count, is large (the 1.6M-dof cantilever: 882 unknowns dense,
+    CG 39 -> 37 iterations), not on small problems that converge in a few iterations."""
+    n = COARSE_SMALL
+    while n < COARSE_MAX and (n + 1) ** 3 <= 8 * nnz0:
+        n += 1
+    return n
 MAX_LEVELS = 10
 POWER_STEPS = 15
 DEGREE = 2          # Chebyshev degree of pre- and post-smoothing
@@ -247,7 +261,8 @@ class DeviceAMG:
         A = Bcsr(dev.dm.Nn, dev.dm.Nn, V, V, pt['row_ptr'], pt['col_idx'], self._empty(dev.nnzb * V * V))
         B, bs = None, V
         levels = []
-        while A.rows * A.br > COARSE_MAX and len(levels) < MAX_LEVELS - 1:
+        n_dense = coarse_limit(dev.nnzb * V * V)
+        while A.rows * A.br > n_dense and len(levels) < MAX_LEVELS - 1:
             agg, n_agg = self._aggregate(A.rows, A.row_ptr, A.col_idx)
             if n_agg >= A.rows:
                 break

@@ -25,15 +25,26 @@ The algorithm (not a restatement of any reference file):
     Tentative P by modified Gram-Schmidt per aggregate.
   * P = (I - omega D^-1 A) P0, omega = 1.5 / lambda, lambda from 15 power
     steps on D^-1 A; Ac = P^T A P; zero diagonal entries of Ac set to 1.
-  * Coarsening stops at <= 512 unknowns (dense inverse) or 10 levels.
+  * Coarsening stops at <= coarse_limit(nnz0) unknowns (dense inverse: 512,
+    up to 1024 while n^3 <= 8 nnz0, nnz0 the level-0 operator's entries) or
+    10 levels.
   * V(2,2) cycle (degree 4 on the coarser levels), Chebyshev smoothing on
     [lambda/15, 1.1 lambda].
 """
 import numpy as np
 import scipy.sparse as sp
 
-COARSE_MAX = 512
+COARSE_MAX = 1024
+COARSE_SMALL = 512
 MAX_LEVELS = 10
+
+
+def coarse_limit(nnz0):
+    """Largest dense coarsest level: 512 unknowns, and up to 1024 while n^3 <= 8 nnz0."""
+    n = COARSE_SMALL
+    while n < COARSE_MAX and (n + 1) ** 3 <= 8 * nnz0:
+        n += 1
+    return n
 POWER_STEPS = 15
 CHEB_LOW = 1.0 / 15.0
 CHEB_HIGH = 1.1
@@ -238,7 +249,9 @@ class Hierarchy:
         n_nodes = len(ptr) - 1
         B = None
         lvl = 0
-        while A.shape[0] > COARSE_MAX and len(self.levels) < MAX_LEVELS - 1:
+        # level 0's operator has the FE block pattern: vec^2 entries per node-graph entry
+        n_dense = coarse_limit(len(idx) * vec * vec)
+        while A.shape[0] > n_dense and len(self.levels) < MAX_LEVELS - 1:
             agg, n_agg = mis2_aggregate(ptr, idx)
             if n_agg >= n_nodes:
                 break

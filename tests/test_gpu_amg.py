@@ -321,10 +321,12 @@ def test_fused_f32_copies_match_the_separate_copies(gpu, mesh):
     assert torch.equal(dinv2.view(torch.int64), dinv.view(torch.int64))
 
 
-@pytest.mark.parametrize('nb', [4, 10, 12])
+@pytest.mark.parametrize('nb', [4, 10, 12, 22, 30, 147])
 def test_coarse_inverse_against_numpy(gpu, nb):
     # the dense coarse solve's inverse: one workgroup in LDS up to 64 unknowns (nb <= 10 blocks
-    # of 6), a launch per pivot above; both against numpy's inverse of the same SPD matrix
+    # of 6), a launch per pivot from 65 to 127 (nb = 12), 64-wide pivot blocks from 128 (nb = 22:
+    # 132 unknowns, a last block of 4; nb = 147: 882 unknowns, the 1.6M-dof cantilever's coarse
+    # level); all against numpy's inverse of the same SPD matrix
     from jax_fem_amd._lib import check, ptr, stream_ptr
     from jax_fem_amd.amg import Bcsr
     rng = np.random.default_rng(nb)
