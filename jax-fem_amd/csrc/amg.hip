@@ -1178,52 +1178,65 @@ __global__ void k_gj_step(int64_t n, int64_t k, const double* __restrict__ src, 
   }
 }
 
+// the value of v held by lane l of the wave (l uniform across the wave)
+__device__ inline double readlane_f64(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // The whole Gauss-Jordan elimination of a small dense block (n <= kGjSmall: the n x n
 // block at (k0, k0) of a matrix of leading dimension ld, inverted in place) in one
-// workgroup: the block in LDS, one barrier per pivot, each step the arithmetic of
-// k_gj_step (so the inverse is the same bit for bit) without a launch per pivot.
+// workgroup, each step the arithmetic of k_gj_step (so the inverse is the same bit for
+// bit) without a launch per pivot.  The block stays in registers: wave w of 16 holds rows
+// w, w + 16, w + 32, w + 48, lane j column j (16 waves: 50 us for a 64 x 64 block, 4 waves
+// of 16 rows: 78 us).  Per pivot k only the pivot row goes through LDS (its owner
+// publishes it at the end of step k - 1, double-buffered, one barrier per pivot); entry
+// (i, k) of a wave's own row comes from lane k of that wave.
 constexpr int kGjSmall = 64;
-__global__ __launch_bounds__(1024) void k_gj_small(int n, int64_t ld, int64_t k0, double* __restrict__ a) {
-  __shared__ double m[2][kGjSmall * kGjSmall];
-  constexpr int kPer = kGjSmall * kGjSmall / 1024;  // entries per thread: e = threadIdx.x + 1024 u
-  const int nn = n * n;
-  int ie[kPer], je[kPer];
+constexpr int kGjWaves = 16;
+__global__ __launch_bounds__(64 * kGjWaves) void k_gj_small(int n, int64_t ld, int64_t k0, double* __restrict__ a) {
+  constexpr int kW = kGjWaves, kPer = kGjSmall / kW;
+  __shared__ double rowbuf[2][kGjSmall];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool col = lane < n;
+  double m[kPer];
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int e = threadIdx.x + 1024 * u;
-    ie[u] = e / n;
-    je[u] = e % n;
-    if (e < nn) m[0][e] = a[(k0 + ie[u]) * ld + k0 + je[u]];
+    const int i = w + kW * u;
+    m[u] = (i < n && col) ? a[(k0 + i) * ld + k0 + lane] : 0.0;
   }
+  if (w == 0) rowbuf[0][lane] = m[0];
   __syncthreads();
-  int c = 0;
   for (int k = 0; k < n; ++k) {
-    const double* src = m[c];
-    double* dst = m[c ^ 1];
-    const double d = 1.0 / src[k * n + k];
+    const double rkj = rowbuf[k & 1][lane];
+    const double d = 1.0 / rowbuf[k & 1][k];
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
-      const int e = threadIdx.x + 1024 * u;
-      if (e >= nn) break;
-      const int i = ie[u], j = je[u];
+      const int i = w + kW * u;
+      const double aik = readlane_f64(m[u], k);
       double v;
-      if (i == k && j == k)
+      if (i == k && lane == k)
         v = d;
       else if (i == k)
-        v = src[k * n + j] * d;
-      else if (j == k)
-        v = -src[i * n + k] * d;
+        v = rkj * d;
+      else if (lane == k)
+        v = -aik * d;
       else
-        v = src[e] - src[i * n + k] * src[k * n + j] * d;
-      dst[e] = v;
+        v = m[u] - aik * rkj * d;
+      if (i < n && col) m[u] = v;
     }
+    // the owner of row k + 1 publishes it for the next pivot
+#pragma unroll
+    for (int u = 0; u < kPer; ++u)
+      if (w + kW * u == k + 1) rowbuf[(k + 1) & 1][lane] = m[u];
     __syncthreads();
-    c ^= 1;
   }
 #pragma unroll
   for (int u = 0; u < kPer; ++u) {
-    const int e = threadIdx.x + 1024 * u;
-    if (e < nn) a[(k0 + ie[u]) * ld + k0 + je[u]] = m[c][e];
+    const int i = w + kW * u;
+    if (i < n && col) a[(k0 + i) * ld + k0 + lane] = m[u];
   }
 }
 
@@ -1234,52 +1247,46 @@ __global__ __launch_bounds__(1024) void k_gj_small(int n, int64_t ld, int64_t k0
 // 64-wide panels per block instead of a launch per pivot.
 // k_bgj_panel: colp (n x kGjSmall) <- the old column panel A_iK (i outside K),
 //              rowp (bk x n) <- P A_Kj (j outside K);
-// one workgroup per 64-column tile: P and the tile of A_Kj in LDS, 4 x 4 entries per thread
-// (each workgroup also copies 64 rows of the column panel).
+// one workgroup per 64-column tile and group of 16 rows of P: those rows of P and the tile
+// of A_Kj in LDS, 4 entries per thread (the first row group also copies 64 rows of the
+// column panel).
 __global__ __launch_bounds__(256) void k_bgj_panel(int64_t n, int64_t k0, int bk, const double* __restrict__ a,
                                                    double* __restrict__ colp, double* __restrict__ rowp) {
-  constexpr int T = kGjSmall;
-  __shared__ double sp[T][T + 1];  // P
-  __shared__ double sa[T][T + 1];  // the tile of A_Kj
+  constexpr int T = kGjSmall, RG = 16;
+  __shared__ double sp[RG][T + 1];  // rows r0 .. r0 + 15 of P
+  __shared__ double sa[T][T + 1];   // the tile of A_Kj
   const int64_t tj = blockIdx.x, j0 = tj * T, kb = k0 / T;
-  for (int e = threadIdx.x; e < T * T; e += 256) {
-    const int64_t i = j0 + e / T;
-    const int c = e % T;
-    if (i < n && c < bk && (i < k0 || i >= k0 + bk)) colp[i * T + c] = a[i * n + k0 + c];
+  const int r0 = blockIdx.y * RG;
+  if (blockIdx.y == 0)
+    for (int e = threadIdx.x; e < T * T; e += 256) {
+      const int64_t i = j0 + e / T;
+      const int c = e % T;
+      if (i < n && c < bk && (i < k0 || i >= k0 + bk)) colp[i * T + c] = a[i * n + k0 + c];
+    }
+  if (tj == kb || r0 >= bk) return;
+  for (int e = threadIdx.x; e < RG * T; e += 256) {
+    const int r = e / T, q = e % T;
+    sp[r][q] = (r0 + r < bk && q < bk) ? a[(k0 + r0 + r) * n + k0 + q] : 0.0;
   }
-  if (tj == kb) return;
   for (int e = threadIdx.x; e < T * T; e += 256) {
     const int r = e / T, q = e % T;
     const int64_t j = j0 + q;
-    sp[r][q] = (r < bk && q < bk) ? a[(k0 + r) * n + k0 + q] : 0.0;
     sa[r][q] = (r < bk && j < n) ? a[(k0 + r) * n + j] : 0.0;
   }
   __syncthreads();
   const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-  double acc[4][4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (int c = 0; c < bk; ++c) {
-    double x[4], y[4];
+    const double x = sp[tr][c];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) x[u] = sp[tr + 16 * u][c];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) y[v] = sa[c][tc + 16 * v];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+    for (int v = 0; v < 4; ++v) acc[v] = fma(x, sa[c][tc + 16 * v], acc[v]);
   }
+  const int r = r0 + tr;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int r = tr + 16 * u;
-      const int64_t j = j0 + tc + 16 * v;
-      if (r < bk && j < n) rowp[r * n + j] = acc[u][v];
-    }
+  for (int v = 0; v < 4; ++v) {
+    const int64_t j = j0 + tc + 16 * v;
+    if (r < bk && j < n) rowp[r * n + j] = acc[v];
+  }
 }
 
 // k_bgj_update: one workgroup per 64 x 64 tile of A (tiles aligned with the pivot blocks);
@@ -1302,6 +1309,16 @@ __global__ __launch_bounds__(256) void k_bgj_update(int64_t n, int64_t k0, int b
     return;
   }
   const bool pivcol = tj == kb;
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  // the tile's own entries, loaded ahead of the product (the pivot column tile has none)
+  double old[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t i = i0 + tr + 16 * u, j = j0 + tc + 16 * v;
+      old[u][v] = (!pivcol && i < n && j < n) ? a[i * n + j] : 0.0;
+    }
   for (int e = threadIdx.x; e < T * T; e += 256) {
     const int r = e / T, c = e % T;
     const int64_t i = i0 + r;
@@ -1311,7 +1328,6 @@ __global__ __launch_bounds__(256) void k_bgj_update(int64_t n, int64_t k0, int b
     sr[cc][q] = (cc < bk && j < n) ? (pivcol ? a[(k0 + cc) * n + j] : rowp[cc * n + j]) : 0.0;
   }
   __syncthreads();
-  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
   double acc[4][4];
 #pragma unroll
   for (int u = 0; u < 4; ++u)
@@ -1333,7 +1349,7 @@ __global__ __launch_bounds__(256) void k_bgj_update(int64_t n, int64_t k0, int b
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int64_t i = i0 + tr + 16 * u, j = j0 + tc + 16 * v;
-      if (i < n && j < n) a[i * n + j] = pivcol ? -acc[u][v] : a[i * n + j] - acc[u][v];
+      if (i < n && j < n) a[i * n + j] = pivcol ? -acc[u][v] : old[u][v] - acc[u][v];
     }
 }
 
@@ -2733,7 +2749,7 @@ int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* str
                        A->bc, n, A->row_ptr, A->col_idx, static_cast<const double*>(A->vals), a);
   FEM_LAUNCHED();
   if (n <= kGjSmall) {
-    hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, s, (int)n, n, (int64_t)0, a);
+    hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(64 * kGjWaves), 0, s, (int)n, n, (int64_t)0, a);
     FEM_LAUNCHED();
     return FEM_OK;
   }
@@ -2744,8 +2760,8 @@ int fem_amg_coarse_inverse(const fem_bcsr_t* A, double* inv, void* ws, void* str
     const unsigned tiles = (unsigned)ceil_div(n, kGjSmall);
     for (int64_t k0 = 0; k0 < n; k0 += kGjSmall) {
       const int bk = (int)std::min<int64_t>(kGjSmall, n - k0);
-      hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(1024), 0, s, bk, n, k0, a);
-      hipLaunchKernelGGL(k_bgj_panel, dim3(tiles), dim3(256), 0, s, n, k0, bk, a, colp, rowp);
+      hipLaunchKernelGGL(k_gj_small, dim3(1), dim3(64 * kGjWaves), 0, s, bk, n, k0, a);
+      hipLaunchKernelGGL(k_bgj_panel, dim3(tiles, kGjSmall / 16), dim3(256), 0, s, n, k0, bk, a, colp, rowp);
       hipLaunchKernelGGL(k_bgj_update, dim3(tiles, tiles), dim3(256), 0, s, n, k0, bk, a, colp, rowp);
     }
     FEM_LAUNCHED();
