@@ -103,11 +103,13 @@ def measured_traffic(cells, path=TRAFFIC_PROFILE):
 
 
 def cheb_bytes(dev, elem_bytes, vec=3):
-    """Algorithmic bytes of the timed level-0 Chebyshev step of the V-cycle (k_sell_apply, CHEB:
-    the first post-smoothing step, whose direction starts from zero, so d is written, not read):
-    the matrix blocks (values + one column index each, padding not counted) once; the FP64
-    vectors x (gathered, counted once), rhs and D^-1 read, d and the new x written."""
-    return dev.nnzb * (vec * vec * elem_bytes + 4) + dev.n * 5 * 8
+    """Mean algorithmic bytes of the three timed level-0 Chebyshev steps of a V-cycle
+    (k_sell_apply, CHEB; level-0 degree 2: the pre-smoothing step after k_cheb_first and both
+    post-smoothing steps): the matrix blocks (values + one column index each, padding not
+    counted) once; the FP64 vectors x (gathered, counted once), rhs and D^-1 read, d and the
+    new x written, and d read by the two steps that continue a direction (the first
+    post-smoothing step starts it from zero)."""
+    return dev.nnzb * (vec * vec * elem_bytes + 4) + dev.n * 8 * (5 + 6 + 6) / 3
 
 
 def assembly_bytes(dev):
@@ -293,9 +295,10 @@ def run(args):
         "s_per_adjoint_gradient": {"mixed": head['s_per_adjoint_gradient'],
                                    "fp64": res['fp64']['s_per_adjoint_gradient']},
         "roofline": {"bound": "hbm",
-                     "kernel": "k_sell_apply<3,CHEB,float,true> (level-0 Chebyshev step of the V-cycle: SELL-64 "
-                               "3x3-block product on FP32 difference-form storage with FP64 vectors and "
-                               "arithmetic, fused D^-1 residual and update)",
+                     "kernel": "k_sell_apply<3,CHEB,float,true> (level-0 Chebyshev steps of the V-cycle, the "
+                               "mean of its three launches per cycle: SELL-64 3x3-block product on FP32 "
+                               "difference-form storage with FP64 vectors and arithmetic, fused D^-1 "
+                               "residual and update)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS if achieved else None,
                      "traffic": amg_traffic.get('k_sell_apply_cheb_f32'),

@@ -475,7 +475,7 @@ typedef struct {
                                     SELL copy: FEM_F32 or FEM_F64; vectors and
                                     arithmetic are FP64                       */
   int32_t        degree;         /* Chebyshev degree of pre/post smoothing on
-                                    level 0; coarser levels: 2 * degree (<= 8) */
+                                    level 0; coarser levels: 2 * degree + 1 (<= 9) */
   int32_t        level0_form;    /* form of the level-0 SELL values: FEM_FORM_
                                     STANDARD, FEM_FORM_DIFFERENCE (FP32), or
                                     FEM_FORM_SCALED_DIFF16 (FP16 D^-1 A, with
@@ -491,9 +491,11 @@ typedef struct {
   int32_t maxiter;
   int32_t check_every;           /* iterations between host status reads      */
   int32_t timing;                /* 1: at every status read, time the last
-                                    V-cycle and its first level-0 Chebyshev
-                                    SELL kernel with HIP events: result
-                                    spmv_ms / update_ms = their means         */
+                                    V-cycle and its level-0 Chebyshev SELL
+                                    kernels (the pre-smoothing step and both
+                                    post-smoothing steps at degree >= 2) with
+                                    HIP events: result update_ms = the mean
+                                    cycle, spmv_ms = the mean Chebyshev step */
   int32_t pad;
   double  reliable_delta;        /* FP32 outer operator: replace the residual
                                     when |r| fell by this factor             */

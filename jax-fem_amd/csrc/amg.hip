@@ -1941,13 +1941,13 @@ int64_t level_n(const fem_amg_t& amg, int l) {
 
 size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-// Chebyshev degree of level l: amg.degree on level 0, twice that (at most 8) on the
-// coarser levels, whose products cost a fifth of level 0's or less.  On the 1.6M-dof
-// HEX8 cantilever degree 4 there took the CG from 47 to 39 iterations (57.9 -> 54.1 ms
-// per step; 5: 37, 54.4 ms; 8: 34, 56.3 ms), and from 68 to 53 on the SIMP cantilever,
-// 107 to 83 on the HEX27 cube, 246 to 214 on the Neo-Hookean cube; the 36k-dof Poisson
-// cube, whose steps are launch-bound, went from 4.1 to 4.7 ms.
-inline int level_degree(const fem_amg_t& amg, int l) { return l == 0 ? amg.degree : std::min(2 * amg.degree, 8); }
+// Chebyshev degree of level l: amg.degree on level 0, 2 amg.degree + 1 (at most 9) on
+// the coarser levels, whose products cost a fifth of level 0's or less.  Against the
+// level-0 degree (2) everywhere: on the 1.6M-dof HEX8 cantilever 47 -> 36 CG iterations
+// (coarse degree 3, 4, 5, 6: 39, 37, 36, 35 iterations at 51.1, 50.7, 50.6, 50.9 ms per
+// step, with its 882-unknown dense coarse level), the SIMP cantilever 68 -> 47, the HEX27
+// cube 107 -> 77 (degree 4: 51 and 83).
+inline int level_degree(const fem_amg_t& amg, int l) { return l == 0 ? amg.degree : std::min(2 * amg.degree + 1, 9); }
 
 // Chebyshev on [lo, hi] = [lmax / 15, 1.1 lmax].  A sweep of the lower bound
 // (lmax / 12 ... / 50, upper 1.05 ... 1.3 lmax) over the HEX8 cantilever
@@ -1988,10 +1988,10 @@ int level_apply(const fem_amg_t& amg, int l, int mode, const ApplyArgsT<TX, TY>&
 
 // Pre-smoothing (Chebyshev from a zero iterate), residual and restriction on
 // level l, vectors stored as TX.  On return *xo is the smoothed iterate and
-// *oo the other iterate buffer.
+// *oo the other iterate buffer.  ev: see vcycle_impl.
 template <typename T, typename TX>
 int level_down(const fem_amg_t& amg, int l, int64_t n, const double* rhs, double* rhs_next, TX* x, TX* other,
-               TX* dd, TX* res, TX** xo, TX** oo, hipStream_t s, const int32_t* done) {
+               TX* dd, TX* res, TX** xo, TX** oo, hipStream_t s, hipEvent_t* ev, const int32_t* done) {
   const fem_amg_level_t& L = amg.levels[l];
   const Cheb cb(L.lmax);
   hipLaunchKernelGGL(k_cheb_first<TX>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, rhs, L.dinv, 1.0 / cb.theta, dd,
@@ -2002,7 +2002,10 @@ int level_down(const fem_amg_t& amg, int l, int64_t n, const double* rhs, double
   for (int k = 1; k < deg; ++k) {
     const double rn = 1.0 / (2.0 * cb.sigma - rho);
     ApplyArgsT<TX, TX> a{x, other, rhs, L.dinv, dd, rn * rho, 2.0 * rn / cb.delta, nullptr, done};
+    const bool timed = ev && l == 0 && k == 1;
+    if (timed) FEM_HIP(hipEventRecord(ev[4], s));
     if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
+    if (timed) FEM_HIP(hipEventRecord(ev[5], s));
     rho = rn;
     std::swap(x, other);
   }
@@ -2036,7 +2039,9 @@ int level_up(const fem_amg_t& amg, int l, const double* rhs, const double* xc, T
       rho = rn;
     }
     const bool timed = ev && l == 0 && k == 0;
+    const bool timed_last = ev && l == 0 && k > 0 && k == deg - 1;
     if (timed) FEM_HIP(hipEventRecord(ev[0], s));
+    if (timed_last) FEM_HIP(hipEventRecord(ev[6], s));
     if (z && k == deg - 1) {
       ApplyArgsT<TX, double> a{x, z, rhs, L.dinv, dd, c1, c2, nullptr, done};
       if (int rc = level_apply<T>(amg, l, AP_CHEB, a, s)) return rc;
@@ -2046,13 +2051,17 @@ int level_up(const fem_amg_t& amg, int l, const double* rhs, const double* xc, T
       std::swap(x, other);
     }
     if (timed) FEM_HIP(hipEventRecord(ev[1], s));
+    if (timed_last) FEM_HIP(hipEventRecord(ev[7], s));
   }
   *xo = x;
   return FEM_OK;
 }
 
 // ev (may be NULL): events recorded around the whole cycle (ev[2], ev[3]) and
-// around its first level-0 post-smoothing Chebyshev kernel (ev[0], ev[1]).
+// around its level-0 Chebyshev SELL kernels: the first post-smoothing step
+// (ev[0], ev[1]) and, with a level-0 degree of at least 2, the pre-smoothing
+// step after k_cheb_first (ev[4], ev[5]) and the last post-smoothing step
+// (ev[6], ev[7]).  With no levels ev[0], ev[1] bracket the dense solve.
 // Vectors are FP64 on every level, whatever the storage T of the matrices:
 // FP32 level-0 iterates (which move 20 MB less per level-0 product on the
 // 1.6M-dof cantilever) round the differences x_j - x_i the difference form
@@ -2087,7 +2096,7 @@ int vcycle_impl(const fem_amg_t& amg, const double* r, double* z, char* ws, hipS
     const int64_t n = level_n(amg, l);
     double *x = nullptr, *o = nullptr;
     if (int rc = level_down<T, double>(amg, l, n, rhs[l], rhs[l + 1], xa[l], xb[l], dd[l], res[l], &x, &o, s,
-                                       done))
+                                       ev, done))
       return rc;
     cur[l] = x;
     xb[l] = o;
@@ -2855,16 +2864,19 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const float*
   FEM_LAUNCHED();
   AmgScalars h{};
   if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &h)) return rc;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  constexpr int kEv = 8;
+  hipEvent_t ev[kEv] = {};
+  // the three level-0 Chebyshev SELL launches of a cycle are timed where level 0 has them
+  const bool three = amg->num_levels > 0 && amg->degree >= 2;
   struct EvGuard {
     hipEvent_t* e;
     ~EvGuard() {
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < kEv; ++i)
         if (e[i]) (void)hipEventDestroy(e[i]);
     }
   } guard{ev};
   if (opt->timing)
-    for (int i = 0; i < 4; ++i) FEM_HIP(hipEventCreate(&ev[i]));
+    for (int i = 0; i < kEv; ++i) FEM_HIP(hipEventCreate(&ev[i]));
   double cheb_ms = 0.0, cycle_ms = 0.0;
   int samples = 0;
   int it = 0;
@@ -2894,10 +2906,17 @@ int fem_amg_pcg_solve(const fem_pattern_t* pat, const double* vals, const float*
     ++it;
     if (it % opt->check_every == 0 || it >= opt->maxiter) {
       if (int rc = read_dev(sc, sizeof(AmgScalars), host_status, s, &h)) return rc;
-      if (opt->timing) {
+      // (a cycle queued after convergence exits at once: h.iter < it, no sample)
+      if (opt->timing && h.iter >= it) {
         float a = 0.f, b = 0.f;
         FEM_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
         FEM_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
+        if (three) {
+          float c = 0.f, d = 0.f;
+          FEM_HIP(hipEventElapsedTime(&c, ev[4], ev[5]));
+          FEM_HIP(hipEventElapsedTime(&d, ev[6], ev[7]));
+          a = (a + c + d) / 3.f;
+        }
         cheb_ms += a;
         cycle_ms += b;
         ++samples;

@@ -28,7 +28,7 @@ The algorithm (not a restatement of any reference file):
   * Coarsening stops at <= coarse_limit(nnz0) unknowns (dense inverse: 512,
     up to 1024 while n^3 <= 8 nnz0, nnz0 the level-0 operator's entries) or
     10 levels.
-  * V(2,2) cycle (degree 4 on the coarser levels), Chebyshev smoothing on
+  * V(2,2) cycle (degree 5 on the coarser levels), Chebyshev smoothing on
     [lambda/15, 1.1 lambda].
 """
 import numpy as np
@@ -304,11 +304,11 @@ class Hierarchy:
 
     def vcycle(self, r, l=0, pre=2, post=2):
         """V-cycle; pre/post are level 0's Chebyshev degrees, the coarser levels smooth
-        with twice that degree (at most 8), as the device cycle does."""
+        with 2 degree + 1 (at most 9), as the device cycle does."""
         if l == len(self.levels):
             return self.coarse_inv @ r
         L = self.levels[l]
-        dpre, dpost = (pre, post) if l == 0 else (min(2 * pre, 8), min(2 * post, 8))
+        dpre, dpost = (pre, post) if l == 0 else (min(2 * pre + 1, 9), min(2 * post + 1, 9))
         x = self.smooth(L, np.zeros_like(r), r, dpre, True)
         rc = L['P'].T @ (r - L['A'] @ x)
         x = x + L['P'] @ self.vcycle(rc, l + 1, pre, post)
