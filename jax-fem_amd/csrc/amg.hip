@@ -1191,14 +1191,16 @@ __device__ inline double readlane_f64(double v, int l) {
 // workgroup, each step the arithmetic of k_gj_step (so the inverse is the same bit for
 // bit) without a launch per pivot.  The block stays in registers: wave w of 16 holds rows
 // w, w + 16, w + 32, w + 48, lane j column j (16 waves: 50 us for a 64 x 64 block, 4 waves
-// of 16 rows: 78 us).  Per pivot k only the pivot row goes through LDS (its owner
-// publishes it at the end of step k - 1, double-buffered, one barrier per pivot); entry
-// (i, k) of a wave's own row comes from lane k of that wave.
+// of 16 rows: 78 us).  Per pivot k only the pivot row and its reciprocal pivot go through
+// LDS (the row's owner publishes both at the end of step k - 1, double-buffered, one
+// barrier per pivot); entry (i, k) of a wave's own row comes from lane k of that wave.
+// The loop is issue-bound (four waves per SIMD): one division per pivot instead of one per
+// wave, and selects instead of branches.
 constexpr int kGjSmall = 64;
 constexpr int kGjWaves = 16;
 __global__ __launch_bounds__(64 * kGjWaves) void k_gj_small(int n, int64_t ld, int64_t k0, double* __restrict__ a) {
   constexpr int kW = kGjWaves, kPer = kGjSmall / kW;
-  __shared__ double rowbuf[2][kGjSmall];
+  __shared__ double rowbuf[2][kGjSmall + 1];  // the pivot row, then 1 / its pivot
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool col = lane < n;
   double m[kPer];
@@ -1207,30 +1209,36 @@ __global__ __launch_bounds__(64 * kGjWaves) void k_gj_small(int n, int64_t ld, i
     const int i = w + kW * u;
     m[u] = (i < n && col) ? a[(k0 + i) * ld + k0 + lane] : 0.0;
   }
-  if (w == 0) rowbuf[0][lane] = m[0];
+  if (w == 0) {
+    rowbuf[0][lane] = m[0];
+    const double p0 = readlane_f64(m[0], 0);
+    if (lane == 0) rowbuf[0][kGjSmall] = 1.0 / p0;
+  }
   __syncthreads();
   for (int k = 0; k < n; ++k) {
     const double rkj = rowbuf[k & 1][lane];
-    const double d = 1.0 / rowbuf[k & 1][k];
+    const double d = rowbuf[k & 1][kGjSmall];
+    const double rd = rkj * d;  // the pivot row's new entries
+    const bool pc = lane == k;
 #pragma unroll
     for (int u = 0; u < kPer; ++u) {
       const int i = w + kW * u;
       const double aik = readlane_f64(m[u], k);
-      double v;
-      if (i == k && lane == k)
-        v = d;
-      else if (i == k)
-        v = rkj * d;
-      else if (lane == k)
-        v = -aik * d;
-      else
-        v = m[u] - aik * rkj * d;
+      double v = fma(-(aik * rkj), d, m[u]);
+      v = pc ? -aik * d : v;
+      v = i == k ? (pc ? d : rd) : v;
       if (i < n && col) m[u] = v;
     }
-    // the owner of row k + 1 publishes it for the next pivot
+    // the owner of row k + 1 publishes it and its reciprocal pivot for the next step
+    if (k + 1 < n) {
 #pragma unroll
-    for (int u = 0; u < kPer; ++u)
-      if (w + kW * u == k + 1) rowbuf[(k + 1) & 1][lane] = m[u];
+      for (int u = 0; u < kPer; ++u)
+        if (w + kW * u == k + 1) {
+          rowbuf[(k + 1) & 1][lane] = m[u];
+          const double piv = readlane_f64(m[u], k + 1);
+          if (lane == 0) rowbuf[(k + 1) & 1][kGjSmall] = 1.0 / piv;
+        }
+    }
     __syncthreads();
   }
 #pragma unroll
