@@ -1371,7 +1371,8 @@ __global__ void k_to_f32(int64_t n, const double* __restrict__ a, float* __restr
     b[i] = (float)a[i];
 }
 
-// y = M x, one wave per row
+// y = M x, one wave per row; four independent sums per lane keep four loads in flight
+// (a fixed order: the result does not depend on timing)
 __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __restrict__ M,
                                                     const double* __restrict__ x, double* __restrict__ y,
                                                     const int32_t* __restrict__ done) {
@@ -1379,9 +1380,17 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int64_t n, const double* __r
   const int lane = threadIdx.x & 63;
   const int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
   if (i >= n) return;
-  double s = 0.0;
-  for (int64_t j = lane; j < n; j += 64) s = fma(M[i * n + j], x[j], s);
-  s = wave_sum(s);
+  const double* row = M + i * n;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int64_t j = lane;
+  for (; j + 192 < n; j += 256) {
+    s0 = fma(row[j], x[j], s0);
+    s1 = fma(row[j + 64], x[j + 64], s1);
+    s2 = fma(row[j + 128], x[j + 128], s2);
+    s3 = fma(row[j + 192], x[j + 192], s3);
+  }
+  for (; j < n; j += 64) s0 = fma(row[j], x[j], s0);
+  const double s = wave_sum((s0 + s1) + (s2 + s3));
   if (lane == 0) y[i] = s;
 }
 
@@ -1551,6 +1560,8 @@ __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int3
   for (int i = 0; i < BR; ++i) acc[i] = 0.0;
   const int b = rp[row];
   const int nf = (rp[row + 1] - b) * BC;
+  // unrolled so that several pairs' loads are in flight (the sums keep their order)
+#pragma unroll 4
   for (int f = threadIdx.x; f < nf; f += 256) {
     const int k = b + f / BC, j = f % BC;
     const double xv = double(a.x[(int64_t)ci[k] * BC + j]);
