@@ -270,3 +270,20 @@ def test_mass_map_u_dependence_probe():
     assert m is not None and torch.equal(m, torch.stack([xq[:, 0], 2. + 0. * xq[:, 1], xq[:, 2] ** 2], 1))
     with pytest.raises(NotImplementedError):
         DeviceProblem._eval_u_independent(_Fake(), diff, xq, [], 'surface_map', allow_u=False)
+
+
+def test_dense_coarse_limit_matches_the_oracle_rule():
+    # the dense coarsest level: 512 unknowns, up to 1024 while n^3 <= 8 nnz of level 0; the
+    # device setup and the oracle hierarchy must stop coarsening at the same size
+    from oracle import amg as OA
+    from jax_fem_amd.amg import coarse_limit
+    c1 = 33 ** 3 * 27                 # the 32^3 Poisson cube: about 27 entries per row
+    c2 = 14_340_865 * 9               # the 1.6M-dof cantilever's level-0 entries
+    for nnz0 in (0, 1, c1, 67_000_000, c2, 10 ** 9, 10 ** 12):
+        n = coarse_limit(nnz0)
+        assert n == OA.coarse_limit(nnz0)
+        assert 512 <= n <= 1024
+        assert n == 512 or n ** 3 <= 8 * nnz0
+        assert n == 1024 or (n + 1) ** 3 > 8 * nnz0
+    assert coarse_limit(c1) == 512        # the Poisson cube keeps its 858-unknown level smoothed
+    assert coarse_limit(c2) >= 882        # the cantilever's third level (882 unknowns) is dense
