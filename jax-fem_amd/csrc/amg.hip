@@ -1582,17 +1582,27 @@ __global__ __launch_bounds__(256) void k_bcsr_apply_row(int64_t rows, const int3
   }
 }
 
+// Slice split of the V-cycle's SELL kernels: FP32 storage with FP64 vectors 3 waves
+// per slice (one thread per (row, component) in the epilogue; on the 1.6M-dof HEX8
+// cantilever 3, 4, 6, 8 waves: Chebyshev step 122.5, 123.2, 130.3, 125.6 us, residual
+// 111.9, 113.6, 125.9, 121.7 us), otherwise the split of the Jacobi path's kernels.
+template <typename T>
+struct AmgSliceCfg {
+  static constexpr int W = sizeof(T) == 4 ? 3 : SliceCfg<T>::W;
+  static constexpr int NT = 64 * W;
+};
+
 // Standard-form SELL-64 operator (level 0), FP64 vectors, one block per slice.
 // DIFF: the matrix is in difference form (the diagonal slot holds the row
 // sum); the own-row entries of the input then enter every product.
 template <int V, int MODE, typename T, bool DIFF = false, typename TX = double, typename TY = double>
-__global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pat, const T* __restrict__ vals,
+__global__ __launch_bounds__(AmgSliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pat, const T* __restrict__ vals,
                                                                 ApplyArgsT<TX, TY> a,
                                                                 const int32_t* __restrict__ done) {
   // FP32 products unroll the slots by 4 (the residual: 124 -> 112 us on the 1.6M-dof
   // cantilever), the Chebyshev step, with its larger epilogue, by 2 (119.5 against 124.6 us);
   // FP16 storage keeps 2 (its residual: 74 us unrolled by 2, 78 us by 4)
-  constexpr int W = SliceCfg<T>::W;
+  constexpr int W = AmgSliceCfg<T>::W;
   constexpr int U = sizeof(T) == 4 ? (MODE == AP_CHEB ? 2 : 4) : SliceCfg<T>::U;
   __shared__ double red[W][kSlice][V];
   if ((done && *done) || (a.done && *a.done)) return;
@@ -1617,7 +1627,7 @@ __global__ __launch_bounds__(SliceCfg<T>::NT) void k_sell_apply(fem_pattern_t pa
   // thread p finishes component p % V of lane p / V after the partial sums meet in
   // LDS, holding 4 operands; otherwise wave 0 finishes its lanes' rows (the outer
   // product's <p, q> partial is then one wave sum).
-  constexpr bool kDist = MODE != AP_PQ && SliceCfg<T>::NT >= kSlice * V;
+  constexpr bool kDist = MODE != AP_PQ && AmgSliceCfg<T>::NT >= kSlice * V;
   const int pl = threadIdx.x / V, pi = threadIdx.x % V;
   int64_t prow = -1;
   double p_rhs = 0.0, p_dinv = 0.0, p_d = 0.0, p_x = 0.0;
@@ -1942,10 +1952,10 @@ int sell_apply(const fem_pattern_t& pat, const void* vals, const ApplyArgsT<TX, 
   const unsigned grid = (unsigned)pat.num_slices;
   const T* v = static_cast<const T*>(vals);
   if (pat.vec == 3)
-    hipLaunchKernelGGL((k_sell_apply<3, MODE, T, DIFF, TX, TY>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a,
+    hipLaunchKernelGGL((k_sell_apply<3, MODE, T, DIFF, TX, TY>), dim3(grid), dim3(AmgSliceCfg<T>::NT), 0, s, pat, v, a,
                        done);
   else if (pat.vec == 1)
-    hipLaunchKernelGGL((k_sell_apply<1, MODE, T, DIFF, TX, TY>), dim3(grid), dim3(SliceCfg<T>::NT), 0, s, pat, v, a,
+    hipLaunchKernelGGL((k_sell_apply<1, MODE, T, DIFF, TX, TY>), dim3(grid), dim3(AmgSliceCfg<T>::NT), 0, s, pat, v, a,
                        done);
   else
     FEM_CHECK_ARG(false, "AMG: vec must be 1 or 3");
